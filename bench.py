@@ -1,0 +1,194 @@
+"""Throughput of the batched FBME plan at 256x256 / 160x160 on one MI355X.
+
+Workload (one "step"): one graph replay of the FBME-Fast plan over a batch of
+B synthetic scenes (256x256 grid, 160x160 prefix, K=300 log-normal point
+scatterers in a random ellipse, 10 dB SNR, polish_size 32), inputs resident
+in HBM.  value = B * steps / elapsed seconds (scenes/s).
+
+Also reported: the plain single-scene plan's scenes/s on the same GPU (the
+north-star comparison), the NMSE of both on the same scenes, the roofline of
+the dominant kernel (the fused row pass) timed with events on the stream it
+runs on, and the float64 CPU oracle timed on the host.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fbme-sbl_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--profile", choices=("fast", "stable"), default="fast")
+    ap.add_argument("--plain-steps", type=int, default=64)
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
+              "scenes shard across GPUs without communication, so run one "
+              "process per GPU.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from fbme_amd import _lib, scenes
+    from fbme_amd.bytes_model import kernel_bytes, survey_scene_bytes
+    from torch_sbl_fastest_stable import make_fastest_plan, make_stable_plan
+
+    device = torch.device("cuda", 0)
+    grid, samples = (256, 256), (160, 160)
+    n, m = grid[0] * grid[1], samples[0] * samples[1]
+    b = args.batch
+    make = make_fastest_plan if args.profile == "fast" else make_stable_plan
+    print(f"[bench] generating {b} scenes", file=sys.stderr, flush=True)
+    batch = [scenes.ellipse_scene(scenes.EVAL_SEED_BASE + i, grid, samples,
+                                  k=300, snr_db=10.0) for i in range(b)]
+    truth = np.stack([s.x for s in batch])
+    data = torch.from_numpy(np.stack([s.y for s in batch])).to(device)
+
+    # ---------------------------------------------------------- batched plan
+    print("[bench] building batched plan", file=sys.stderr, flush=True)
+    plan = make(data, grid, polish_size=32, warmup=2)
+    for _ in range(args.warmup):
+        plan.replay(data)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        plan.replay(data)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = start.elapsed_time(stop) / args.steps
+    value = b * args.steps / (ms_per_step * args.steps / 1e3)
+    x_batched = plan.result.x.detach().cpu().numpy().reshape(b, -1)
+    nmse_batched = float(np.mean([scenes.nmse_db(x_batched[i], truth[i].reshape(-1))
+                                  for i in range(b)]))
+    print(f"[bench] batched: {ms_per_step:.3f} ms/step, {value:.1f} scenes/s",
+          file=sys.stderr, flush=True)
+
+    # ------------------------------------------------------------ plain plan
+    plain = None
+    if not args.skip_plain:
+        single = make(data[0], grid, polish_size=32, warmup=2)
+        for i in range(4):
+            single.replay(data[i % b])
+        torch.cuda.synchronize()
+        nmse_single = []
+        start.record()
+        for i in range(args.plain_steps):
+            single.replay(data[i % b])
+        stop.record()
+        torch.cuda.synchronize()
+        plain_ms = start.elapsed_time(stop) / args.plain_steps
+        for i in range(min(b, 16)):
+            r = single.replay(data[i], synchronize=True)
+            nmse_single.append(scenes.nmse_db(r.x.cpu().numpy().reshape(-1),
+                                              truth[i].reshape(-1)))
+        nmse_batched16 = float(np.mean([scenes.nmse_db(x_batched[i], truth[i].reshape(-1))
+                                        for i in range(min(b, 16))]))
+        plain = dict(ms_per_scene=plain_ms, scenes_per_s=1e3 / plain_ms,
+                     nmse_db=float(np.mean(nmse_single)),
+                     batched_nmse_db_same_scenes=nmse_batched16)
+        print(f"[bench] plain single-scene: {plain_ms:.3f} ms/scene",
+              file=sys.stderr, flush=True)
+
+    # ------------------------------------------------- dominant kernel roofline
+    native = plan.native_plan
+    flat = plan.input_buffer.view(-1)
+    kb = kernel_bytes(*grid, *samples)
+    stage_ms = {}
+    for name, stage in (("row", _lib.STAGE_ROW),
+                        ("col_adjoint", _lib.STAGE_COL_ADJOINT),
+                        ("col_forward", _lib.STAGE_COL_FORWARD),
+                        ("output", _lib.STAGE_OUTPUT)):
+        for _ in range(3):
+            native.run_stage(stage, flat)
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(args.kernel_reps):
+            native.run_stage(stage, flat)
+        stop.record()
+        torch.cuda.synchronize()
+        stage_ms[name] = start.elapsed_time(stop) / args.kernel_reps
+    row_s = stage_ms["row"] / 1e3
+    achieved = kb["row"] * b / row_s
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "row_pass_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            t = json.load(fh)
+        if t.get("batch") == b and t.get("workload") == "256x256/160x160":
+            traffic = t.get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
+                "kernel": "k_row (fused A^H row pass + denoise/ARD + A row pass)",
+                "algorithmic_bytes_per_launch": kb["row"] * b,
+                "avg_launch_ms": stage_ms["row"]}
+    per_kernel = {k: {"ms": v, "GB/s": kb[k] * b / (v / 1e3) / 1e9}
+                  for k, v in stage_ms.items()}
+
+    # ----------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import fbme_oracle as fo
+        count, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < args.cpu_seconds and count < b:
+            fo.fbme_plan(batch[count].y, grid, args.profile)
+            count += 1
+        el = time.perf_counter() - t0
+        cpu = {"value": count / el, "unit": "scenes/s", "cores": 1, "kind": "port",
+               "sample": f"{count} scenes of the same 256x256/160x160 workload, "
+                         "float64 NumPy oracle (single-threaded pocketfft)"}
+
+    t_u = native.max_iters
+    line = {
+        "metric": "scenes/s at 256x256 (160x160 prefix), batched FBME-"
+                  f"{args.profile.capitalize()} plan",
+        "value": value, "unit": "scenes/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c64/f32 (f64 polish)", "data": "synthetic",
+        "config": {"workload": "FBME-%s plan, %d scenes/step, grid 256x256, "
+                               "prefix 160x160, K=300 ellipse scatterers, 10 dB"
+                               % (args.profile.capitalize(), b),
+                   "batch": b, "grid": list(grid), "prefix": list(samples),
+                   "polish_size": 32, "uamp_iterations": t_u},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_single_scene": plain,
+        "speedup_vs_plain": (value / plain["scenes_per_s"]) if plain else None,
+        "nmse_db_batched": nmse_batched,
+        "per_kernel": per_kernel,
+        "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
+                                                           args.profile == "stable"),
+        "wall_s": wall,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,785 @@
+// Active-set EM polish of the FBME plans (gfx950), one workgroup per scene.
+//
+// Restates _uamp_active_em_polish for the Fast/Stable settings
+// (torch_sbl/solver.py:3961-4511): top-k selection of |mu|^2 + v
+// (solver.py:1926-1933), zero-tail pruning and second-moment
+// re-initialisation (:4087-4106), T_e evaluations of the exact reduced
+// posterior with its Type-II evidence (:3591-3779) behind the evidence gate
+// (:4168-4298), support-fixed EM updates (:4320-4391) and, for Stable, one
+// residual exchange of up to 12 atoms at the first transition (:4392-4494)
+// plus the fixed-budget EM residuals (:3408-3496, :1989-2021).
+//
+// MI355X-first choices:
+//  * the Gram matrix C^H C is never formed from the [M, K] column matrix: it
+//    is the product of two 1-D Dirichlet kernels, gathered from per-axis
+//    tables; C^H y is the backprojection at S; ||y - C mu||^2 follows from
+//    ||y||^2, mu^H C^H y and mu^H G mu.  No M-sized work remains.
+//  * the 32x32 reduced system is factorised in FP64 in LDS; evidence terms
+//    and the noise/energy scalars are FP64.
+//  * Stable's |A^H r|^2 over the full grid is a separate multi-workgroup
+//    kernel: (A^H C mu)[n] = sum_k mu_k D1[n1-k1] D2[n2-k2].
+#include "fbme_device.h"
+
+using namespace fbme;
+
+namespace {
+
+constexpr double F32_EPS_D = 1.1920928955078125e-07;
+constexpr double F32_TINY_D = 1.17549435082228750797e-38;
+constexpr int KM = FBME_KMAX;
+constexpr int KP = KM + 1;   // padded row length in LDS
+constexpr int TE_MAX = 8;
+
+struct PolishArgs {
+  float margin_factor, jitter_multiplier, max_precision, polish_damping;
+  float hyper_shape, hyper_rate, noise_shape, noise_rate;
+  float false_alarm_threshold;
+  int evaluations, exchange_size, diagnostics, evidence_noise;
+};
+
+// Per-scene state carried from the first to the second polish phase (Stable).
+struct PolishState {
+  int best_s[KM];
+  int work_s[KM];
+  double2 best_mu[KM];
+  double best_sig[KM];
+  double best_dd[KM];
+  double best_gam[KM];
+  double work_gam[KM];
+  double best_noise, work_noise, best_nle;
+  double trace[TE_MAX];
+  int exchanges;
+  int status;
+  int pad[2];
+};
+
+// Per-scene result consumed by the output kernel (indices sorted ascending).
+struct PolishResult {
+  int idx[KM];
+  float2 mu[KM];
+  float sig[KM];
+  float gam[KM];
+  int k;
+  int pad[3];
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconjmul(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float key_score(uint32_t key) {
+  if (key == 0u) return -INFINITY;
+  return (key & 0x80000000u) ? __uint_as_float(key & 0x7FFFFFFFu) : __uint_as_float(~key);
+}
+
+struct SceneCtx {
+  int n1, n2, m_total, k;
+  double floor_v, ye;
+  const double2* d1;
+  const double2* d2;
+  const float2* bp;
+};
+
+__device__ __forceinline__ double2 gram_entry(const SceneCtx& c, int a, int b) {
+  int d1 = (a / c.n2) - (b / c.n2);
+  int d2 = (a % c.n2) - (b % c.n2);
+  d1 += (d1 < 0) ? c.n1 : 0;
+  d2 += (d2 < 0) ? c.n2 : 0;
+  return zmul(c.d1[d1], c.d2[d2]);
+}
+
+// LDS workspace of one polish workgroup.
+struct PolishLds {
+  double2 a[KM][KP];
+  double2 x[KM][KP];
+  double2 g[KM][KP];
+  double2 rhs[KM];
+  double2 z[KM];
+  double2 mu[KM];
+  double dsc[KM];
+  double sig[KM];
+  double dd[KM];
+  double red[16];
+  int flag[4];
+};
+
+// Exact reduced posterior and negative log evidence of the zero-tail model
+// on support w with prior variances wg and noise nu (solver.py:3591-3779).
+// Leaves G(w) in L.g.  Results in L.mu / L.sig / L.dd; returns the nle.
+__device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, double nu, PolishLds& L,
+                           float jitter_multiplier) {
+  const int K = c.k, tid = threadIdx.x;
+  for (int e = tid; e < K * K; e += blockDim.x) {
+    const int r = e / K, q = e - r * K;
+    const double2 gv = gram_entry(c, w[r], w[q]);
+    L.g[r][q] = gv;
+    double2 v = make_double2(gv.x / nu, gv.y / nu);
+    if (r == q) v.x += 1.0 / wg[r];
+    L.a[r][q] = v;
+  }
+  __syncthreads();
+  // Hermitian part, then the diagonal equilibration scale.
+  for (int e = tid; e < K * K; e += blockDim.x) {
+    const int r = e / K, q = e - r * K;
+    if (r < q) {
+      const double2 u = L.a[r][q], l = L.a[q][r];
+      const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+      L.a[r][q] = h;
+      L.a[q][r] = make_double2(h.x, -h.y);
+    } else if (r == q) {
+      L.a[r][r].y = 0.0;
+    }
+  }
+  __syncthreads();
+  if (tid < K) L.dsc[tid] = sqrt(fmax(L.a[tid][tid].x, F32_TINY_D));
+  __syncthreads();
+  for (int e = tid; e < K * K; e += blockDim.x) {
+    const int r = e / K, q = e - r * K;
+    const double s = L.dsc[r] * L.dsc[q];
+    L.a[r][q] = make_double2(L.a[r][q].x / s, L.a[r][q].y / s);
+  }
+  __syncthreads();
+  // Deterministic loading of the GPU branch of _stable_cholesky_raw
+  // (solver.py:2907-2920): jitter_multiplier * eps_f32 * mean|diag|.
+  if (tid == 0) {
+    double mean_diag = 0.0;
+    for (int r = 0; r < K; ++r) mean_diag += fabs(L.a[r][r].x);
+    mean_diag = fmax(mean_diag / K, F32_TINY_D);
+    const double jit = (double)jitter_multiplier * F32_EPS_D * mean_diag;
+    for (int r = 0; r < K; ++r) L.a[r][r].x += jit;
+    L.flag[0] = 0;
+  }
+  __syncthreads();
+  // Right-looking Cholesky, lower factor in place.
+  for (int j = 0; j < K; ++j) {
+    if (tid == 0) {
+      const double p = L.a[j][j].x;
+      if (!(p > 0.0)) L.flag[0] = 1;
+      L.a[j][j] = make_double2(sqrt(fmax(p, 0.0)), 0.0);
+    }
+    __syncthreads();
+    const double piv = L.a[j][j].x;
+    for (int r = j + 1 + tid; r < K; r += blockDim.x) {
+      L.a[r][j] = make_double2(L.a[r][j].x / piv, L.a[r][j].y / piv);
+    }
+    __syncthreads();
+    const int m = K - j - 1;
+    for (int e = tid; e < m * m; e += blockDim.x) {
+      const int r = j + 1 + e / m, q = j + 1 + e % m;
+      if (q <= r) {
+        const double2 lr = L.a[r][j], lq = L.a[q][j];
+        // a[r][q] -= l[r] * conj(l[q])
+        L.a[r][q].x -= lr.x * lq.x + lr.y * lq.y;
+        L.a[r][q].y -= lr.y * lq.x - lr.x * lq.y;
+      }
+    }
+    __syncthreads();
+  }
+  // X = L^-1 (one column per thread, forward substitution).
+  if (tid < K) {
+    const int col = tid;
+    for (int r = 0; r < K; ++r) {
+      if (r < col) {
+        L.x[r][col] = make_double2(0.0, 0.0);
+        continue;
+      }
+      double2 acc = make_double2(r == col ? 1.0 : 0.0, 0.0);
+      for (int p = col; p < r; ++p) {
+        const double2 t = zmul(L.a[r][p], L.x[p][col]);
+        acc.x -= t.x;
+        acc.y -= t.y;
+      }
+      const double d = L.a[r][r].x;
+      L.x[r][col] = make_double2(acc.x / d, acc.y / d);
+    }
+  }
+  if (tid < K) {
+    const float2 b = c.bp[w[tid]];
+    L.rhs[tid] = make_double2((double)b.x / nu, (double)b.y / nu);
+  }
+  __syncthreads();
+  // z = X (rhs / d); mu_eq = X^H z; Sigma_eq = column norms of X.
+  if (tid < K) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p = 0; p <= tid; ++p) {
+      const double2 s = make_double2(L.rhs[p].x / L.dsc[p], L.rhs[p].y / L.dsc[p]);
+      const double2 t = zmul(L.x[tid][p], s);
+      acc.x += t.x;
+      acc.y += t.y;
+    }
+    L.z[tid] = acc;
+  }
+  __syncthreads();
+  if (tid < K) {
+    double2 acc = make_double2(0.0, 0.0);
+    double ev = 0.0;
+    for (int r = tid; r < K; ++r) {
+      const double2 t = zconjmul(L.x[r][tid], L.z[r]);
+      acc.x += t.x;
+      acc.y += t.y;
+      ev += L.x[r][tid].x * L.x[r][tid].x + L.x[r][tid].y * L.x[r][tid].y;
+    }
+    const double d = L.dsc[tid];
+    L.mu[tid] = make_double2(acc.x / d, acc.y / d);
+    const double sg = fmax(ev / (d * d), 0.0);
+    L.sig[tid] = sg;
+    const double gk = wg[tid];
+    L.dd[tid] = fmax((gk - sg) / (gk * gk), 0.0);
+  }
+  __syncthreads();
+  double nle = 0.0;
+  if (tid == 0) {
+    double logdiag = 0.0, logd = 0.0, loggam = 0.0, proj = 0.0;
+    for (int r = 0; r < K; ++r) {
+      logdiag += log(fmax(L.a[r][r].x, F32_TINY_D));
+      logd += log(L.dsc[r]);
+      loggam += log(wg[r]);
+      proj += L.rhs[r].x * L.mu[r].x + L.rhs[r].y * L.mu[r].y;
+    }
+    const double logdet = (double)c.m_total * log(nu) + loggam + 2.0 * (logdiag + logd);
+    const double quad = fmax(c.ye / nu - proj, 0.0);
+    nle = L.flag[0] ? NAN : logdet + quad;
+    L.red[0] = nle;
+  }
+  __syncthreads();
+  nle = L.red[0];
+  __syncthreads();
+  return nle;
+}
+
+// ||y - C mu||^2 from the Gram matrix in L.g and C^H y = bp[S].
+__device__ double residual_energy(const SceneCtx& c, const int* s, const double2* mu, PolishLds& L) {
+  const int K = c.k, tid = threadIdx.x;
+  if (tid < K) {
+    double2 gm = make_double2(0.0, 0.0);
+    for (int q = 0; q < K; ++q) {
+      const double2 t = zmul(L.g[tid][q], mu[q]);
+      gm.x += t.x;
+      gm.y += t.y;
+    }
+    const float2 b = c.bp[s[tid]];
+    const double2 bb = make_double2(b.x, b.y);
+    const double quad = mu[tid].x * gm.x + mu[tid].y * gm.y;   // Re(conj(mu) G mu)
+    const double lin = mu[tid].x * bb.x + mu[tid].y * bb.y;    // Re(conj(mu) b)
+    L.sig[tid] = quad - 2.0 * lin;   // scratch, reused below
+  }
+  __syncthreads();
+  double e = 0.0;
+  if (tid == 0) {
+    double acc = c.ye;
+    for (int r = 0; r < K; ++r) acc += L.sig[r];
+    L.red[1] = fmax(acc, 0.0);
+  }
+  __syncthreads();
+  e = L.red[1];
+  __syncthreads();
+  return e;
+}
+
+__device__ void load_gram(const SceneCtx& c, const int* s, PolishLds& L) {
+  const int K = c.k;
+  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
+    const int r = e / K, q = e - r * K;
+    L.g[r][q] = gram_entry(c, s[r], s[q]);
+  }
+  __syncthreads();
+}
+
+// Sort k (key, idx) pairs descending by key, ascending idx on ties, by rank
+// counting; writes the order into out_idx/out_key.
+__device__ void rank_sort_desc(const uint32_t* key, const int* idx, int k, uint32_t* out_key, int* out_idx) {
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    int rank = 0;
+    for (int j = 0; j < k; ++j) {
+      rank += (key[j] > key[t]) || (key[j] == key[t] && idx[j] < idx[t]);
+    }
+    out_key[rank] = key[t];
+    out_idx[rank] = idx[t];
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const float2* __restrict__ bp_all,
+         const double* __restrict__ sc, const double2* __restrict__ dir, const uint32_t* __restrict__ cand_key,
+         const int* __restrict__ cand_idx, const uint32_t* __restrict__ xcand_key,
+         const int* __restrict__ xcand_idx, PolishState* __restrict__ states, PolishResult* __restrict__ results,
+         float* __restrict__ out_noise, float* __restrict__ out_dof, float* __restrict__ out_nle,
+         float* __restrict__ out_trace, long long* __restrict__ out_exchanges, float* __restrict__ out_diag,
+         int* __restrict__ status, PolishArgs a) {
+  __shared__ PolishLds L;
+  __shared__ uint32_t keys[FBME_TILE_MAX];
+  __shared__ int kidx[FBME_TILE_MAX];
+  __shared__ uint32_t hist[256];
+  __shared__ int misc[4];
+  __shared__ int pos[KM];
+  __shared__ uint32_t sel_key[KM], srt_key[KM];
+  __shared__ int sel_idx[KM], srt_idx[KM];
+  __shared__ int work_s[KM], best_s[KM];
+  __shared__ double work_gam[KM], best_gam[KM], best_sig[KM], best_dd[KM];
+  __shared__ double2 best_mu[KM];
+  __shared__ double sh[16];
+
+  const int si = blockIdx.x;
+  if (si >= nscenes) return;
+  const FbmeScene s = scenes[si];
+  const int tid = threadIdx.x;
+  const double* scs = sc + (long long)si * SC_COUNT;
+  SceneCtx c;
+  c.n1 = s.n1;
+  c.n2 = s.n2;
+  c.m_total = s.m1 * s.m2;
+  c.k = s.k;
+  c.floor_v = scs[SC_FLOOR];
+  c.ye = scs[SC_YE];
+  c.d1 = dir + s.dir1_off;
+  c.d2 = dir + s.dir2_off;
+  c.bp = bp_all + s.grid_off;
+  const int K = s.k;
+  const long long n_grid = (long long)s.n1 * s.n2;
+  PolishState* st = states + si;
+  const double tiny = F32_TINY_D;
+
+  int pit = 0;
+  bool have_best = false;
+  bool pending = false;
+  int exchanges = 0;
+  double best_noise = 0.0, best_nle = 0.0, work_noise = 0.0;
+  double trace[TE_MAX];
+
+  if (phase == 0) {
+    // Top-k of the candidate scores (solver.py:1931).
+    const int ncand = s.row_items * K;
+    for (int i = tid; i < ncand; i += blockDim.x) {
+      keys[i] = cand_key[s.cand_off + i];
+      kidx[i] = cand_idx[s.cand_off + i];
+    }
+    __syncthreads();
+    block_select_topk(keys, ncand, K, pos, hist, misc);
+    if (tid < K) {
+      sel_key[tid] = keys[pos[tid]];
+      sel_idx[tid] = kidx[pos[tid]];
+    }
+    __syncthreads();
+    rank_sort_desc(sel_key, sel_idx, K, srt_key, srt_idx);
+    // Zero-tail pruning + re-initialisation from UAMP second moments
+    // (solver.py:4087-4106): gamma_S = max(|mu|^2 + v, floor).
+    if (tid < K) {
+      work_s[tid] = srt_idx[tid];
+      work_gam[tid] = fmax((double)key_score(srt_key[tid]), c.floor_v);
+    }
+    work_noise = scs[SC_NOISE];
+    __syncthreads();
+  } else {
+    // Scenes without an exchange finished in the first phase.
+    if (min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K))) <= 0) return;
+    if (tid < K) {
+      best_s[tid] = st->best_s[tid];
+      work_s[tid] = st->work_s[tid];
+      best_mu[tid] = st->best_mu[tid];
+      best_sig[tid] = st->best_sig[tid];
+      best_dd[tid] = st->best_dd[tid];
+      best_gam[tid] = st->best_gam[tid];
+      work_gam[tid] = st->work_gam[tid];
+    }
+    best_noise = st->best_noise;
+    work_noise = st->work_noise;
+    best_nle = st->best_nle;
+    for (int i = 0; i < TE_MAX; ++i) trace[i] = st->trace[i];
+    exchanges = st->exchanges;
+    have_best = true;
+    pit = 1;
+    __syncthreads();
+    // Residual exchange (solver.py:4392-4494).
+    const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
+    const int nx = s.xtile_items * exch;
+    for (int i = tid; i < nx; i += blockDim.x) {
+      keys[i] = xcand_key[s.xcand_off + i];
+      kidx[i] = xcand_idx[s.xcand_off + i];
+    }
+    __syncthreads();
+    block_select_topk(keys, nx, exch, pos, hist, misc);
+    if (tid < exch) {
+      sel_key[tid] = keys[pos[tid]];
+      sel_idx[tid] = kidx[pos[tid]];
+    }
+    __syncthreads();
+    rank_sort_desc(sel_key, sel_idx, exch, srt_key, srt_idx);
+    // Deletion gains of the current best support (solver.py:3219-3317).
+    if (tid < K) {
+      const double g = fmax(best_gam[tid], tiny);
+      const double2 q = make_double2(best_mu[tid].x / g, best_mu[tid].y / g);
+      const double down = fmin(fmax(best_sig[tid] / g, tiny), 1.0);
+      const double delta = -log(down) - g * (q.x * q.x + q.y * q.y) / down;
+      L.sig[tid] = delta;
+      L.dd[tid] = (double)tid;
+    }
+    __syncthreads();
+    if (tid < K) {
+      int rank = 0;
+      const double dv = L.sig[tid];
+      for (int j = 0; j < K; ++j) rank += (L.sig[j] > dv) || (L.sig[j] == dv && j < tid);
+      pos[rank] = tid;   // pos[r] = position with the r-th largest deletion gain
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const double col_energy = (double)c.m_total;
+      int any = 0;
+      for (int i = 0; i < exch; ++i) {
+        const double add_e = (double)key_score(srt_key[i]);
+        const int rp = pos[i];
+        const double ratio = add_e / fmax(best_noise * col_energy, tiny);
+        const double add_gain = ratio > 1.0 ? ratio - 1.0 - log(fmax(ratio, tiny)) : 0.0;
+        const bool take = (add_gain + L.sig[rp]) > 0.0;
+        if (take) {
+          any = 1;
+          double ag = fmax(add_e - best_noise * col_energy, 0.0) / (col_energy * col_energy);
+          ag = fmin(fmax(ag, c.floor_v), (double)a.max_precision);
+          work_s[rp] = srt_idx[i];
+          work_gam[rp] = ag;
+        }
+      }
+      misc[0] = any;
+    }
+    __syncthreads();
+    pending = misc[0] != 0;
+    __syncthreads();
+  }
+
+  for (;;) {
+    const double nle = evaluate(c, work_s, work_gam, work_noise, L, a.jitter_multiplier);
+    bool accept = true;
+    if (have_best) {
+      const double margin = (double)a.margin_factor * F32_EPS_D * (1.0 + fabs(nle) + fabs(best_nle));
+      accept = isfinite(nle) && (!isfinite(best_nle) || nle <= best_nle - margin);
+      if (pending && accept) exchanges += 1;
+      pending = false;
+    }
+    if (accept) {
+      if (tid < K) {
+        best_s[tid] = work_s[tid];
+        best_gam[tid] = work_gam[tid];
+        best_mu[tid] = L.mu[tid];
+        best_sig[tid] = L.sig[tid];
+        best_dd[tid] = L.dd[tid];
+      }
+      best_noise = work_noise;
+      best_nle = nle;
+    }
+    have_best = true;
+    __syncthreads();
+    trace[pit] = best_nle;
+    if (pit + 1 == a.evaluations) break;
+    // Support-fixed EM update of gamma_S and sigma^2 (solver.py:4320-4391).
+    load_gram(c, best_s, L);
+    const double energy = residual_energy(c, best_s, best_mu, L);
+    double dof = 0.0;
+    if (tid == 0) {
+      for (int r = 0; r < K; ++r) {
+        dof += fmin(fmax(1.0 - best_sig[r] / fmax(best_gam[r], tiny), 0.0), 1.0);
+      }
+      sh[0] = dof;
+    }
+    __syncthreads();
+    dof = sh[0];
+    const double keep = a.polish_damping;
+    if (tid < K) {
+      const double m2 = best_mu[tid].x * best_mu[tid].x + best_mu[tid].y * best_mu[tid].y;
+      double ng = (m2 + best_sig[tid] + a.hyper_rate) / (1.0 + a.hyper_shape);
+      ng = fmin(fmax(ng, c.floor_v), (double)a.max_precision);
+      work_gam[tid] = keep * best_gam[tid] + (1.0 - keep) * ng;
+      work_s[tid] = best_s[tid];
+    }
+    double upd;
+    if (a.evidence_noise) {
+      upd = (energy + a.noise_rate) / fmax((double)c.m_total - dof + a.noise_shape, F32_EPS_D);
+    } else {
+      upd = (energy + best_noise * dof + a.noise_rate) / ((double)c.m_total + a.noise_shape);
+    }
+    upd = fmax(upd, c.floor_v);
+    work_noise = keep * best_noise + (1.0 - keep) * upd;
+    __syncthreads();
+    const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
+    if (exch > 0 && pit == 0 && phase == 0) {
+      if (tid < K) {
+        st->best_s[tid] = best_s[tid];
+        st->work_s[tid] = work_s[tid];
+        st->best_mu[tid] = best_mu[tid];
+        st->best_sig[tid] = best_sig[tid];
+        st->best_dd[tid] = best_dd[tid];
+        st->best_gam[tid] = best_gam[tid];
+        st->work_gam[tid] = work_gam[tid];
+      }
+      if (tid == 0) {
+        st->best_noise = best_noise;
+        st->work_noise = work_noise;
+        st->best_nle = best_nle;
+        for (int i = 0; i < TE_MAX; ++i) st->trace[i] = (i <= pit) ? trace[i] : 0.0;
+        st->exchanges = exchanges;
+      }
+      return;
+    }
+    ++pit;
+  }
+
+  // ------------------------------------------------------------- finalize
+  // Projected relevance dof (solver.py:2081-2092, 3156-3181).
+  if (tid == 0) {
+    double raw = 0.0;
+    for (int r = 0; r < K; ++r) raw += fmin(fmax(best_gam[r] * best_dd[r], 0.0), 1.0);
+    const double cap = (double)c.m_total * (1.0 - 10.0 * F32_EPS_D);
+    const double scale = fmin(1.0, cap / fmax(raw, F32_EPS_D));
+    double dof = 0.0;
+    for (int r = 0; r < K; ++r) dof += fmin(fmax(best_gam[r] * best_dd[r], 0.0), 1.0) * scale;
+    const int oi = s.orig;
+    out_dof[oi] = (float)dof;
+    out_noise[oi] = (float)best_noise;
+    out_nle[oi] = (float)best_nle;
+    for (int i = 0; i < a.evaluations; ++i) out_trace[(long long)oi * a.evaluations + i] = (float)trace[i];
+    if (out_exchanges) out_exchanges[oi] = exchanges;
+    if (status) status[oi] = L.flag[0];
+  }
+  if (a.diagnostics) {
+    // Fixed-budget EM residuals (solver.py:3408-3496) and trace flags.
+    load_gram(c, best_s, L);
+    const double energy = residual_energy(c, best_s, best_mu, L);
+    if (tid == 0) {
+      double gres = 0.0, dof = 0.0;
+      for (int r = 0; r < K; ++r) {
+        const double g = fmax(best_gam[r], tiny);
+        const double m2 = best_mu[r].x * best_mu[r].x + best_mu[r].y * best_mu[r].y;
+        double t = (m2 + best_sig[r] + a.hyper_rate) / (1.0 + a.hyper_shape);
+        t = fmin(fmax(t, c.floor_v), (double)a.max_precision);
+        const double sc_ = fmax(fmax(fabs(g), fabs(t)), tiny);
+        gres = fmax(gres, fabs(g - t) / sc_);
+        dof += fmin(fmax(1.0 - best_sig[r] / g, 0.0), 1.0);
+      }
+      double nt;
+      if (a.evidence_noise) {
+        nt = (energy + a.noise_rate) / fmax((double)c.m_total - dof + a.noise_shape, F32_EPS_D);
+      } else {
+        nt = (energy + best_noise * dof + a.noise_rate) / ((double)c.m_total + a.noise_shape);
+      }
+      nt = fmin(fmax(nt, c.floor_v), (double)a.max_precision);
+      const double ns = fmax(fmax(fabs(best_noise), fabs(nt)), tiny);
+      const double nres = fabs(best_noise - nt) / ns;
+      bool finite = true, nonincreasing = true;
+      for (int i = 0; i < a.evaluations; ++i) {
+        const float t = (float)trace[i];
+        finite = finite && isfinite(t);
+        if (i > 0) nonincreasing = nonincreasing && ((float)trace[i] <= (float)trace[i - 1]);
+      }
+      float* d = out_diag + (long long)s.orig * 4;
+      d[0] = (float)gres;
+      d[1] = (float)nres;
+      d[2] = fmaxf((float)trace[0] - (float)trace[a.evaluations - 1], 0.0f);
+      d[3] = (finite && nonincreasing) ? 1.0f : 0.0f;
+    }
+  }
+  // Support sorted by grid index for the output kernel.
+  if (tid < K) {
+    int rank = 0;
+    for (int j = 0; j < K; ++j) rank += best_s[j] < best_s[tid];
+    PolishResult* res = results + si;
+    res->idx[rank] = best_s[tid];
+    res->mu[rank] = make_float2((float)best_mu[tid].x, (float)best_mu[tid].y);
+    res->sig[rank] = (float)best_sig[tid];
+    res->gam[rank] = (float)best_gam[tid];
+  }
+  if (tid == 0) results[si].k = K;
+}
+
+// |A^H (y - C mu)|^2 over one tile of the grid with the support excluded,
+// reduced to the tile's top `exch` candidates (solver.py:4396-4413).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_exchange_score(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
+                 const float2* __restrict__ bp_all, const double2* __restrict__ dir,
+                 const PolishState* __restrict__ states, uint32_t* __restrict__ xcand_key,
+                 int* __restrict__ xcand_idx, int exchange_size) {
+  __shared__ float2 d1[1024];
+  __shared__ float2 d2[1024];
+  __shared__ int sk1[KM], sk2[KM];
+  __shared__ float2 smu[KM];
+  __shared__ uint32_t keys[FBME_XTILE];
+  __shared__ uint32_t excl[FBME_XTILE / 32];
+  __shared__ uint32_t hist[256];
+  __shared__ int misc[4];
+  __shared__ int pos[KM];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const FbmeWork wk = work[w];
+    const FbmeScene s = scenes[wk.scene];
+    const PolishState* st = states + wk.scene;
+    const int K = s.k;
+    const long long n_grid = (long long)s.n1 * s.n2;
+    const int exch = min(exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
+    if (exch <= 0) continue;
+    for (int i = threadIdx.x; i < s.n1; i += FBME_BLOCK) {
+      const double2 v = dir[s.dir1_off + i];
+      d1[i] = make_float2((float)v.x, (float)v.y);
+    }
+    for (int i = threadIdx.x; i < s.n2; i += FBME_BLOCK) {
+      const double2 v = dir[s.dir2_off + i];
+      d2[i] = make_float2((float)v.x, (float)v.y);
+    }
+    for (int i = threadIdx.x; i < FBME_XTILE / 32; i += FBME_BLOCK) excl[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x < K) {
+      const int idx = st->best_s[threadIdx.x];
+      sk1[threadIdx.x] = idx / s.n2;
+      sk2[threadIdx.x] = idx % s.n2;
+      smu[threadIdx.x] = make_float2((float)st->best_mu[threadIdx.x].x, (float)st->best_mu[threadIdx.x].y);
+      const int local = idx - wk.first;
+      if (local >= 0 && local < wk.count) atomicOr(&excl[local >> 5], 1u << (local & 31));
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < wk.count; e += FBME_BLOCK) {
+      const int n = wk.first + e;
+      const int r1 = n / s.n2, r2 = n - r1 * s.n2;
+      const float2 b = bp_all[s.grid_off + n];
+      float ax = b.x, ay = b.y;
+      for (int k = 0; k < K; ++k) {
+        int i1 = r1 - sk1[k];
+        int i2 = r2 - sk2[k];
+        i1 += (i1 < 0) ? s.n1 : 0;
+        i2 += (i2 < 0) ? s.n2 : 0;
+        const float2 g = cmul(d1[i1], d2[i2]);
+        const float2 t = cmul(smu[k], g);
+        ax -= t.x;
+        ay -= t.y;
+      }
+      const bool ex = (excl[e >> 5] >> (e & 31)) & 1u;
+      keys[e] = ex ? 0u : score_key(ax * ax + ay * ay);
+    }
+    __syncthreads();
+    const int got = block_select_topk(keys, wk.count, exch, pos, hist, misc);
+    const long long base = (long long)s.xcand_off + (long long)wk.slot * exch;
+    for (int i = threadIdx.x; i < exch; i += FBME_BLOCK) {
+      if (i < got) {
+        xcand_key[base + i] = keys[pos[i]];
+        xcand_idx[base + i] = wk.first + pos[i];
+      } else {
+        xcand_key[base + i] = 0u;
+        xcand_idx[base + i] = -1;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Full-grid outputs: x, posterior variance, precision and the support mask
+// (solver.py:2093-2126).  Off the support x = 0, variance = precision^-1 =
+// floor, exactly as the zero-tail model leaves them.
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
+         const double* __restrict__ sc, const PolishResult* __restrict__ results, float2* __restrict__ x,
+         float* __restrict__ pvar, float* __restrict__ prec, unsigned char* __restrict__ mask, float threshold,
+         int with_mask) {
+  __shared__ int sidx[KM];
+  __shared__ float2 smu[KM];
+  __shared__ float ssig[KM], sgam[KM];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const FbmeWork wk = work[w];
+    const FbmeScene s = scenes[wk.scene];
+    const PolishResult* r = results + wk.scene;
+    const int K = s.k;
+    if (threadIdx.x < K) {
+      sidx[threadIdx.x] = r->idx[threadIdx.x];
+      smu[threadIdx.x] = r->mu[threadIdx.x];
+      ssig[threadIdx.x] = r->sig[threadIdx.x];
+      sgam[threadIdx.x] = r->gam[threadIdx.x];
+    }
+    __syncthreads();
+    const float floor_v = (float)sc[(long long)wk.scene * SC_COUNT + SC_FLOOR];
+    const float inv_floor = 1.0f / floor_v;
+    // log(N / false_alarm_budget) (solver.py:2108-2126), float32 like the
+    // reference's torch.full_like(noise_variance, value).
+    const float thr = (float)log((double)s.n1 * (double)s.n2 / (double)threshold);
+    for (int e = threadIdx.x; e < wk.count; e += FBME_BLOCK) {
+      const int n = wk.first + e;
+      int lo = 0, hi = K - 1, hit = -1;
+      while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = sidx[mid];
+        if (v == n) {
+          hit = mid;
+          break;
+        }
+        if (v < n) lo = mid + 1;
+        else hi = mid - 1;
+      }
+      float2 xv = make_float2(0.f, 0.f);
+      float pv = floor_v, pr = inv_floor;
+      if (hit >= 0) {
+        xv = smu[hit];
+        pv = ssig[hit];
+        pr = 1.0f / sgam[hit];
+      }
+      const long long gi = s.grid_off + n;
+      x[gi] = xv;
+      pvar[gi] = pv;
+      prec[gi] = pr;
+      if (with_mask) {
+        const float score = (xv.x * xv.x + xv.y * xv.y) / fmaxf(pv, 1.17549435e-38f);
+        mask[gi] = score > thr ? 1 : 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t fbme_polish_state_bytes(void) { return sizeof(PolishState); }
+size_t fbme_polish_result_bytes(void) { return sizeof(PolishResult); }
+
+int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const void* bp, const double* sc,
+                       const void* dir, const uint32_t* cand_key, const int* cand_idx, const uint32_t* xcand_key,
+                       const int* xcand_idx, void* states, void* results, float* out_noise, float* out_dof,
+                       float* out_nle, float* out_trace, long long* out_exchanges, float* out_diag, int* status,
+                       const FbmePlanOptions* o, hipStream_t stream) {
+  if (nscenes <= 0) return 0;
+  PolishArgs a;
+  a.margin_factor = o->margin_factor;
+  a.jitter_multiplier = o->jitter_multiplier;
+  a.max_precision = o->max_precision;
+  a.polish_damping = o->polish_damping;
+  a.hyper_shape = o->hyper_shape;
+  a.hyper_rate = o->hyper_rate;
+  a.noise_shape = o->noise_shape;
+  a.noise_rate = o->noise_rate;
+  a.false_alarm_threshold = 0.0f;
+  a.evaluations = o->polish_iterations;
+  a.exchange_size = o->exchange_size;
+  a.diagnostics = o->diagnostics;
+  a.evidence_noise = o->evidence_noise;
+  hipLaunchKernelGGL(k_polish, dim3(nscenes), dim3(FBME_BLOCK), 0, stream, scenes, nscenes, phase,
+                     (const float2*)bp, sc, (const double2*)dir, cand_key, cand_idx, xcand_key, xcand_idx,
+                     (PolishState*)states, (PolishResult*)results, out_noise, out_dof, out_nle, out_trace,
+                     out_exchanges, out_diag, status, a);
+  return (int)hipGetLastError();
+}
+
+int fbme_launch_exchange_score(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* bp,
+                               const void* dir, const void* states, uint32_t* xcand_key, int* xcand_idx,
+                               int exchange_size, hipStream_t stream) {
+  if (nwork <= 0) return 0;
+  hipLaunchKernelGGL(k_exchange_score, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                     (const float2*)bp, (const double2*)dir, (const PolishState*)states, xcand_key, xcand_idx,
+                     exchange_size);
+  return (int)hipGetLastError();
+}
+
+int fbme_launch_output(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const double* sc,
+                       const void* results, void* x, float* pvar, float* prec, unsigned char* mask,
+                       float threshold, int with_mask, hipStream_t stream) {
+  if (nwork <= 0) return 0;
+  hipLaunchKernelGGL(k_output, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork, sc,
+                     (const PolishResult*)results, (float2*)x, pvar, prec, mask, threshold, with_mask);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

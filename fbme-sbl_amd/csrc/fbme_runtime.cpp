@@ -1,0 +1,533 @@
+// Native plan runtime of the FBME hot path.
+//
+// A plan is built once for a list of scene shapes (uniform or uneven):
+//   * per-scene descriptors (radix codes of the FFT lengths, tile sizes,
+//     buffer offsets, partial-sum slots, candidate slots, Dirichlet tables),
+//   * work tables of the column / row / tile passes, ordered so that the
+//     scenes still iterating at UAMP step t always form a prefix (scenes are
+//     sorted by their iteration budget T_u = ceil(0.8 N / M) floored at 3,
+//     torch_sbl/solver.py:1287-1309), which gives per-scene convergence masks
+//     without any extra table or device-side branching,
+//   * a layout of one caller-provided workspace (the caller's allocator owns
+//     every byte, so nothing allocates inside a hipGraph capture).
+// fbme_plan_run() enqueues the complete fixed stage sequence on one stream;
+// it performs no allocation, no synchronisation and no memset, so the whole
+// call can be captured into a graph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "fbme_hip.h"
+#include "fbme_types.h"
+
+extern "C" {
+int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
+                            const double*, double*, int, int, int, float, hipStream_t);
+int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*,
+                    const double*, double*, int, int, float, float, float, float, float, float, uint32_t*, int*,
+                    int, hipStream_t);
+int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
+                            const double*, double*, int, int, float, hipStream_t);
+int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
+                        float, float, float, float, float, int, hipStream_t);
+size_t fbme_polish_state_bytes(void);
+size_t fbme_polish_result_bytes(void);
+int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, const void*, const uint32_t*,
+                       const int*, const uint32_t*, const int*, void*, void*, float*, float*, float*, float*,
+                       long long*, float*, int*, const FbmePlanOptions*, hipStream_t);
+int fbme_launch_exchange_score(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, const void*,
+                               uint32_t*, int*, int, hipStream_t);
+int fbme_launch_output(const FbmeScene*, const FbmeWork*, int, int, const double*, const void*, void*, float*,
+                       float*, unsigned char*, float, int, hipStream_t);
+}
+
+namespace {
+
+struct Region {
+  size_t off = 0, bytes = 0;
+};
+
+}  // namespace
+
+struct fbme_plan {
+  FbmePlanOptions opt;
+  int nscenes = 0;
+  int max_iters = 0;
+  std::vector<FbmeScene> scenes;            // sorted by iteration budget (desc)
+  std::vector<FbmeWork> col_work, row_work, tile_work;
+  std::vector<int> col_prefix, row_prefix, scene_prefix;  // per UAMP step t (index t-1)
+  std::vector<double> dir;                  // Dirichlet tables, interleaved (re, im)
+  std::vector<long long> grid_off, samp_off; // caller order
+  long long total_grid = 0, total_samp = 0, total_mid = 0;
+  int total_slots = 0, total_cand = 0, total_xcand = 0;
+  float threshold = 0.f;
+  // workspace layout
+  Region r_scenes, r_col, r_row, r_tile, r_dir, r_sc, r_part, r_mid, r_msg, r_pred, r_bp, r_gamma, r_mean, r_var,
+      r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result;
+  size_t workspace_bytes = 0;
+  char* ws = nullptr;
+  int bound = 0;
+};
+
+namespace {
+
+unsigned radix_code(int len, int prefer8, std::string* err) {
+  if (len < 1 || len > 1024) {
+    if (err) *err = "FFT length " + std::to_string(len) + " outside [1, 1024]";
+    return 0xFFFFFFFFu;
+  }
+  std::vector<int> r;
+  int rem = len;
+  while (rem % 5 == 0) { r.push_back(5); rem /= 5; }
+  while (rem % 3 == 0) { r.push_back(3); rem /= 3; }
+  if (prefer8) {
+    while (rem % 8 == 0) { r.push_back(8); rem /= 8; }
+  }
+  while (rem % 4 == 0) { r.push_back(4); rem /= 4; }
+  while (rem % 2 == 0) { r.push_back(2); rem /= 2; }
+  if (rem != 1) {
+    if (err) *err = "FFT length " + std::to_string(len) + " has a prime factor other than 2, 3, 5";
+    return 0xFFFFFFFFu;
+  }
+  if (r.size() > 7) {
+    if (err) *err = "FFT length " + std::to_string(len) + " needs too many radix stages";
+    return 0xFFFFFFFFu;
+  }
+  unsigned code = 0;
+  for (size_t i = 0; i < r.size(); ++i) code |= (unsigned)r[i] << (4 * i);
+  return code;  // len == 1 gives 0 stages (identity)
+}
+
+// D[d] = sum_{j<m} exp(2 pi i j d / n), exact closed form with integer
+// argument reduction.
+void dirichlet(int m, int n, std::vector<double>& out) {
+  const double pi = 3.14159265358979323846;
+  for (int d = 0; d < n; ++d) {
+    if (d == 0) {
+      out.push_back((double)m);
+      out.push_back(0.0);
+      continue;
+    }
+    const long long two_n = 2LL * n;
+    const long long a = ((long long)d * (m - 1)) % two_n;      // phase pi a / n
+    const long long b = ((long long)d * m) % two_n;            // sin(pi b / n)
+    const double ratio = std::sin(pi * (double)b / n) / std::sin(pi * (double)d / n);
+    out.push_back(ratio * std::cos(pi * (double)a / n));
+    out.push_back(ratio * std::sin(pi * (double)a / n));
+  }
+}
+
+int iteration_budget(long long n, long long m, const FbmePlanOptions& o) {
+  // min(max_iter, max(min_iters, ceil(offset + scale N / M)))
+  // (solver.py:1280-1309; paper preset offset 0, scale 0.8, floor 3)
+  const double v = std::ceil((double)o.iteration_offset + (double)o.iteration_scale * (double)n / (double)m);
+  long long b = (long long)v;
+  b = std::max<long long>(o.min_iters, b);
+  b = std::min<long long>(o.max_iters, b);
+  return (int)b;
+}
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+extern "C" {
+
+int fbme_abi_version(void) { return FBME_ABI_VERSION; }
+
+void fbme_default_options(FbmePlanOptions* o, int stable) {
+  o->uamp_damping = 0.05f;
+  o->damping = 0.15f;
+  o->min_variance = 1e-12f;
+  o->max_precision = 1e12f;
+  o->blend = 0.1f;
+  o->noise_fraction = 1e-2f;
+  o->margin_factor = 64.0f;
+  o->jitter_multiplier = 8.0f;
+  o->false_alarm = 0.5f;
+  o->polish_size = 32;
+  o->polish_iterations = stable ? 3 : 2;
+  o->exchange_size = stable ? 12 : 0;
+  o->diagnostics = stable ? 1 : 0;
+  o->evidence_noise = 0;
+  o->row_lines = 8;
+  o->col_lines = 16;
+  o->iteration_offset = 0.0f;
+  o->iteration_scale = 0.8f;
+  o->min_iters = 3;
+  o->max_iters = 200;
+  o->hyper_shape = 0.0f;
+  o->hyper_rate = 0.0f;
+  o->noise_shape = 0.0f;
+  o->noise_rate = 0.0f;
+  o->polish_damping = 0.0f;
+}
+
+fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
+  auto fail = [&](const std::string& msg) -> fbme_plan* {
+    if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", msg.c_str());
+    return nullptr;
+  };
+  if (nscenes < 1) return fail("a plan needs at least one scene");
+  if (!opt) return fail("options pointer is null");
+  if (opt->polish_size < 1 || opt->polish_size > FBME_KMAX)
+    return fail("polish_size must be in [1, " + std::to_string(FBME_KMAX) + "]");
+  if (opt->polish_iterations < 1 || opt->polish_iterations > 8)
+    return fail("polish_iterations must be in [1, 8]");
+  if (opt->exchange_size < 0 || opt->exchange_size > 32) return fail("exchange_size must be in [0, 32]");
+  if (opt->exchange_size > 0 && opt->polish_iterations < 2)
+    return fail("residual exchange requires at least two polish evaluations");
+  if (opt->min_iters < 1 || opt->max_iters < 1) return fail("iteration limits must be positive");
+  fbme_plan* p = new fbme_plan();
+  p->opt = *opt;
+  p->nscenes = nscenes;
+  std::vector<FbmeScene> sc(nscenes);
+  std::string e;
+  for (int i = 0; i < nscenes; ++i) {
+    const int n1 = shapes[4 * i], n2 = shapes[4 * i + 1], m1 = shapes[4 * i + 2], m2 = shapes[4 * i + 3];
+    if (m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2) {
+      delete p;
+      return fail("scene " + std::to_string(i) + ": need 0 < M <= N on both axes");
+    }
+    FbmeScene& s = sc[i];
+    std::memset(&s, 0, sizeof(s));
+    s.n1 = n1; s.n2 = n2; s.m1 = m1; s.m2 = m2;
+    s.rad1 = radix_code(n1, 1, &e);
+    s.rad2 = radix_code(n2, 1, &e);
+    if (s.rad1 == 0xFFFFFFFFu || s.rad2 == 0xFFFFFFFFu) {
+      delete p;
+      return fail("scene " + std::to_string(i) + ": " + e);
+    }
+    const long long n = (long long)n1 * n2, m = (long long)m1 * m2;
+    s.iters = iteration_budget(n, m, *opt);
+    s.k = (int)std::min<long long>(opt->polish_size, n);
+    s.orig = i;
+  }
+  // caller-order offsets
+  p->grid_off.resize(nscenes);
+  p->samp_off.resize(nscenes);
+  long long go = 0, so = 0, mo = 0;
+  for (int i = 0; i < nscenes; ++i) {
+    p->grid_off[i] = go;
+    p->samp_off[i] = so;
+    sc[i].grid_off = go;
+    sc[i].samp_off = so;
+    sc[i].mid_off = mo;
+    go += (long long)sc[i].n1 * sc[i].n2;
+    so += (long long)sc[i].m1 * sc[i].m2;
+    mo += (long long)sc[i].n1 * sc[i].m2;
+  }
+  p->total_grid = go;
+  p->total_samp = so;
+  p->total_mid = mo;
+  // sort by iteration budget, descending (stable -> deterministic)
+  std::vector<int> order(nscenes);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sc[a].iters > sc[b].iters; });
+  p->scenes.resize(nscenes);
+  for (int i = 0; i < nscenes; ++i) p->scenes[i] = sc[order[i]];
+  // Dirichlet tables per distinct (m, n) pair
+  std::vector<std::pair<int, int>> keys;
+  std::vector<int> koff;
+  auto table = [&](int m, int n) -> int {
+    for (size_t i = 0; i < keys.size(); ++i)
+      if (keys[i].first == m && keys[i].second == n) return koff[i];
+    const int off = (int)(p->dir.size() / 2);
+    dirichlet(m, n, p->dir);
+    keys.push_back({m, n});
+    koff.push_back(off);
+    return off;
+  };
+  // work tables
+  int slots = 0, cand = 0, xcand = 0;
+  const int exch_cfg = opt->exchange_size;
+  for (int si = 0; si < nscenes; ++si) {
+    FbmeScene& s = p->scenes[si];
+    s.dir1_off = table(s.m1, s.n1);
+    s.dir2_off = table(s.m2, s.n2);
+    const int cl = std::max(1, std::min({s.m2, FBME_TILE_MAX / s.n1, std::max(1, opt->col_lines)}));
+    // Row tiles hold rl rows of n2 + 1 (padded) complex values in LDS, so
+    // rl * (n2 + 1) <= FBME_TILE_MAX + 64 requires rl <= 64 as well.
+    const int rl_max = std::max(1, std::min({s.n1, FBME_TILE_MAX / s.n2, 64}));
+    int rl = std::max(1, std::min(rl_max, std::max(1, opt->row_lines)));
+    // keep the candidate merge within one tile of LDS
+    while (((s.n1 + rl - 1) / rl) * s.k > FBME_TILE_MAX && rl < rl_max) ++rl;
+    s.col_items = (s.m2 + cl - 1) / cl;
+    s.row_items = (s.n1 + rl - 1) / rl;
+    if (s.row_items * s.k > FBME_TILE_MAX) {
+      delete p;
+      return fail("grid too large for the candidate merge of the polish selection");
+    }
+    s.col_part = slots;
+    slots += s.col_items;
+    s.row_part = slots;
+    slots += s.row_items;
+    s.cand_off = cand;
+    cand += s.row_items * s.k;
+    const long long n = (long long)s.n1 * s.n2;
+    s.xtile_items = (int)((n + FBME_XTILE - 1) / FBME_XTILE);
+    const int exch = std::min<long long>(exch_cfg, std::min<long long>(std::max(s.k - 1, 0), n - s.k));
+    s.xcand_off = xcand;
+    xcand += s.xtile_items * std::max(exch, 0);
+    if (exch > 0 && s.xtile_items * exch > FBME_TILE_MAX) {
+      delete p;
+      return fail("grid too large for the exchange candidate merge");
+    }
+    for (int c = 0; c < s.col_items; ++c)
+      p->col_work.push_back({si, c * cl, std::min(cl, s.m2 - c * cl), c});
+    for (int r = 0; r < s.row_items; ++r)
+      p->row_work.push_back({si, r * rl, std::min(rl, s.n1 - r * rl), r});
+    for (int t = 0; t < s.xtile_items; ++t) {
+      const long long first = (long long)t * FBME_XTILE;
+      p->tile_work.push_back({si, (int)first, (int)std::min<long long>(FBME_XTILE, n - first), t});
+    }
+  }
+  p->total_slots = slots;
+  p->total_cand = cand;
+  p->total_xcand = std::max(xcand, 1);
+  p->max_iters = p->scenes[0].iters;
+  // prefix counts per UAMP step
+  for (int t = 1; t <= p->max_iters; ++t) {
+    int ns = 0;
+    while (ns < nscenes && p->scenes[ns].iters >= t) ++ns;
+    int nc = 0, nr = 0;
+    while (nc < (int)p->col_work.size() && p->col_work[nc].scene < ns) ++nc;
+    while (nr < (int)p->row_work.size() && p->row_work[nr].scene < ns) ++nr;
+    p->scene_prefix.push_back(ns);
+    p->col_prefix.push_back(nc);
+    p->row_prefix.push_back(nr);
+  }
+  // support-mask threshold log(N / budget) is per scene in general; the
+  // locked plans use one grid, uneven batches get the mask per scene shape
+  // through the same formula evaluated on the host for the first scene.
+  p->threshold = 0.f;
+  // workspace layout
+  size_t off = 0;
+  auto reg = [&](Region& r, size_t bytes) {
+    off = align_up(off, 256);
+    r.off = off;
+    r.bytes = bytes;
+    off += bytes;
+  };
+  reg(p->r_scenes, sizeof(FbmeScene) * nscenes);
+  reg(p->r_col, sizeof(FbmeWork) * p->col_work.size());
+  reg(p->r_row, sizeof(FbmeWork) * p->row_work.size());
+  reg(p->r_tile, sizeof(FbmeWork) * p->tile_work.size());
+  reg(p->r_dir, sizeof(double) * p->dir.size());
+  reg(p->r_sc, sizeof(double) * SC_COUNT * nscenes);
+  reg(p->r_part, sizeof(double) * PT_COUNT * (size_t)slots);
+  reg(p->r_mid, 8 * (size_t)p->total_mid);
+  reg(p->r_msg, 8 * (size_t)p->total_samp);
+  reg(p->r_pred, 8 * (size_t)p->total_samp);
+  reg(p->r_bp, 8 * (size_t)p->total_grid);
+  reg(p->r_gamma, 4 * (size_t)p->total_grid);
+  reg(p->r_mean, 8 * (size_t)p->total_grid);
+  reg(p->r_var, 4 * (size_t)p->total_grid);
+  reg(p->r_ckey, 4 * (size_t)cand);
+  reg(p->r_cidx, 4 * (size_t)cand);
+  reg(p->r_xkey, 4 * (size_t)p->total_xcand);
+  reg(p->r_xidx, 4 * (size_t)p->total_xcand);
+  reg(p->r_state, fbme_polish_state_bytes() * nscenes);
+  reg(p->r_result, fbme_polish_result_bytes() * nscenes);
+  p->workspace_bytes = align_up(off, 256);
+  return p;
+}
+
+void fbme_plan_destroy(fbme_plan* p) { delete p; }
+
+long long fbme_plan_query(const fbme_plan* p, int what) {
+  if (!p) return -1;
+  switch (what) {
+    case FBME_Q_WORKSPACE_BYTES: return (long long)p->workspace_bytes;
+    case FBME_Q_TOTAL_GRID: return p->total_grid;
+    case FBME_Q_TOTAL_SAMPLES: return p->total_samp;
+    case FBME_Q_MAX_ITERS: return p->max_iters;
+    case FBME_Q_NSCENES: return p->nscenes;
+    case FBME_Q_EVALUATIONS: return p->opt.polish_iterations;
+    case FBME_Q_LAUNCHES: {
+      long long n = 3;  // init
+      for (int t = 1; t <= p->max_iters; ++t) n += (t > 1) ? 4 : 3;
+      n += 2 + 1;       // final E-step + polish
+      const bool exch = p->opt.exchange_size > 0 && p->total_xcand > 1;
+      if (exch) n += 2;
+      n += 1;           // output
+      return n;
+    }
+    default: return -1;
+  }
+}
+
+int fbme_plan_scene_info(const fbme_plan* p, int scene, long long* info /* [8] */) {
+  if (!p || scene < 0 || scene >= p->nscenes || !info) return FBME_ERR_ARG;
+  for (const FbmeScene& s : p->scenes) {
+    if (s.orig != scene) continue;
+    info[0] = s.grid_off;
+    info[1] = s.samp_off;
+    info[2] = s.iters;
+    info[3] = s.k;
+    info[4] = s.rad1;
+    info[5] = s.rad2;
+    info[6] = s.col_items;
+    info[7] = s.row_items;
+    return 0;
+  }
+  return FBME_ERR_ARG;
+}
+
+int fbme_plan_bind(fbme_plan* p, void* workspace, void* stream) {
+  if (!p || !workspace) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  p->ws = (char*)workspace;
+  auto up = [&](const Region& r, const void* src) -> hipError_t {
+    if (r.bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(p->ws + r.off, src, r.bytes, hipMemcpyHostToDevice, st);
+  };
+  hipError_t e = up(p->r_scenes, p->scenes.data());
+  if (e == hipSuccess) e = up(p->r_col, p->col_work.data());
+  if (e == hipSuccess) e = up(p->r_row, p->row_work.data());
+  if (e == hipSuccess) e = up(p->r_tile, p->tile_work.data());
+  if (e == hipSuccess) e = up(p->r_dir, p->dir.data());
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return FBME_ERR_HIP;
+  p->bound = 1;
+  return 0;
+}
+
+int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold, void* stream) {
+  (void)support_threshold;
+  if (!p || !io || !p->bound) return FBME_ERR_ARG;
+  if (!io->y || !io->x || !io->posterior_variance || !io->precision || !io->noise || !io->dof || !io->nle ||
+      !io->trace)
+    return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  char* w = p->ws;
+  const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
+  const FbmeWork* colw = (const FbmeWork*)(w + p->r_col.off);
+  const FbmeWork* roww = (const FbmeWork*)(w + p->r_row.off);
+  const FbmeWork* tilew = (const FbmeWork*)(w + p->r_tile.off);
+  const void* dir = w + p->r_dir.off;
+  double* sc = (double*)(w + p->r_sc.off);
+  double* part = (double*)(w + p->r_part.off);
+  void* mid = w + p->r_mid.off;
+  void* msg = w + p->r_msg.off;
+  void* pred = w + p->r_pred.off;
+  void* bp = w + p->r_bp.off;
+  float* gamma = (float*)(w + p->r_gamma.off);
+  void* mean = w + p->r_mean.off;
+  float* var = (float*)(w + p->r_var.off);
+  uint32_t* ckey = (uint32_t*)(w + p->r_ckey.off);
+  int* cidx = (int*)(w + p->r_cidx.off);
+  uint32_t* xkey = (uint32_t*)(w + p->r_xkey.off);
+  int* xidx = (int*)(w + p->r_xidx.off);
+  void* states = w + p->r_state.off;
+  void* results = w + p->r_result.off;
+  const FbmePlanOptions& o = p->opt;
+  const int ps = p->total_slots;
+  const int ncol = (int)p->col_work.size(), nrow = (int)p->row_work.size(), ntile = (int)p->tile_work.size();
+  const float hs = o.hyper_shape, hr = o.hyper_rate;
+  int rc = 0;
+#define FBME_TRY(x)            \
+  do {                         \
+    rc = (x);                  \
+    if (rc) return FBME_ERR_HIP; \
+  } while (0)
+  // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
+  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 0, 0, o.uamp_damping, st));
+  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 0, o.uamp_damping, o.damping,
+                           o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, st));
+  FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction, o.blend,
+                               o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+  for (int t = 1; t <= p->max_iters; ++t) {
+    const int nc = p->col_prefix[t - 1], nr = p->row_prefix[t - 1], ns = p->scene_prefix[t - 1];
+    if (t > 1)
+      FBME_TRY(fbme_launch_col_adjoint(scenes, colw, nc, nc, io->y, pred, msg, mid, sc, part, ps, 1, 1, o.uamp_damping, st));
+    FBME_TRY(fbme_launch_row(scenes, roww, nr, nr, mid, bp, gamma, mean, var, sc, part, ps, t == 1 ? 1 : 2,
+                             o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, st));
+    FBME_TRY(fbme_launch_col_forward(scenes, colw, nc, nc, mid, io->y, pred, msg, sc, part, ps, t == 1 ? 1 : 0,
+                                     o.uamp_damping, st));
+    FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, 1, nullptr, o.min_variance, o.noise_fraction, o.blend, o.damping,
+                                 o.noise_shape, o.noise_rate, o.evidence_noise, st));
+  }
+  // Final consistency E-step (no prediction) + per-tile top-k candidates.
+  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 1, 0, o.uamp_damping, st));
+  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 3, o.uamp_damping, o.damping,
+                           o.max_precision, hs, hr, o.blend, ckey, cidx, io->store_uamp ? 1 : 0, st));
+  // Polish.
+  FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
+                              io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
+  const bool exch = o.exchange_size > 0 && p->total_xcand > 1;
+  if (exch) {
+    FBME_TRY(fbme_launch_exchange_score(scenes, tilew, ntile, ntile, bp, dir, states, xkey, xidx, o.exchange_size, st));
+    FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 1, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
+                                io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
+  }
+  FBME_TRY(fbme_launch_output(scenes, tilew, ntile, ntile, sc, results, io->x, io->posterior_variance, io->precision,
+                              io->support_mask, o.false_alarm, (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, st));
+#undef FBME_TRY
+  return 0;
+}
+
+int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* stream) {
+  if (!p || !io || !p->bound || !io->y) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  char* w = p->ws;
+  const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
+  const FbmeWork* colw = (const FbmeWork*)(w + p->r_col.off);
+  const FbmeWork* roww = (const FbmeWork*)(w + p->r_row.off);
+  const FbmeWork* tilew = (const FbmeWork*)(w + p->r_tile.off);
+  double* sc = (double*)(w + p->r_sc.off);
+  double* part = (double*)(w + p->r_part.off);
+  const FbmePlanOptions& o = p->opt;
+  const int ps = p->total_slots;
+  const int ncol = (int)p->col_work.size(), nrow = (int)p->row_work.size(), ntile = (int)p->tile_work.size();
+  void* mid = w + p->r_mid.off;
+  void* msg = w + p->r_msg.off;
+  void* pred = w + p->r_pred.off;
+  void* bp = w + p->r_bp.off;
+  float* gamma = (float*)(w + p->r_gamma.off);
+  void* mean = w + p->r_mean.off;
+  float* var = (float*)(w + p->r_var.off);
+  switch (stage) {
+    case FBME_STAGE_COL_ADJOINT:
+      return fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 1, 1,
+                                     o.uamp_damping, st) ? FBME_ERR_HIP : 0;
+    case FBME_STAGE_ROW:
+      return fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 2, o.uamp_damping,
+                             o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr, nullptr, 0,
+                             st) ? FBME_ERR_HIP : 0;
+    case FBME_STAGE_COL_FORWARD:
+      return fbme_launch_col_forward(scenes, colw, ncol, ncol, mid, io->y, pred, msg, sc, part, ps, 0, o.uamp_damping,
+                                     st) ? FBME_ERR_HIP : 0;
+    case FBME_STAGE_OUTPUT:
+      return fbme_launch_output(scenes, tilew, ntile, ntile, sc, w + p->r_result.off, io->x, io->posterior_variance,
+                                io->precision, io->support_mask, o.false_alarm,
+                                (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, st) ? FBME_ERR_HIP : 0;
+    default:
+      return FBME_ERR_ARG;
+  }
+}
+
+int fbme_plan_debug_pointer(const fbme_plan* p, int which, void** out) {
+  if (!p || !out || !p->bound) return FBME_ERR_ARG;
+  const Region* r = nullptr;
+  switch (which) {
+    case 0: r = &p->r_gamma; break;
+    case 1: r = &p->r_mean; break;
+    case 2: r = &p->r_var; break;
+    case 3: r = &p->r_bp; break;
+    case 4: r = &p->r_sc; break;
+    case 5: r = &p->r_pred; break;
+    case 6: r = &p->r_msg; break;
+    default: return FBME_ERR_ARG;
+  }
+  *out = p->ws + r->off;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Plain-old-data tables shared by the host planner and the HIP kernels.
+#pragma once
+
+#include <stdint.h>
+
+#include "fbme_hip.h"
+
+// Per-scene descriptor of a (possibly uneven) batch.  Scenes are stored back
+// to back in every buffer; the offsets below are element offsets.
+//   grid buffers   [N1*N2]   gamma, mean, variance, backprojection, score, x
+//   sample buffers [M1*M2]   data, message, prediction
+//   mid buffers    [N1*M2]   column-pass <-> row-pass intermediate
+struct FbmeScene {
+  int n1, n2, m1, m2;
+  unsigned rad1, rad2;      // radix codes for FFT lengths n1 and n2
+  int iters;                // UAMP iteration budget T_u
+  int k;                    // active-set capacity min(polish_size, N)
+  long long grid_off;
+  long long samp_off;
+  long long mid_off;
+  int col_items, row_items; // work items per scene in column / row passes
+  int col_part, row_part;   // first partial slot of this scene
+  int cand_off;             // first candidate slot (row_items * k per scene)
+  int dir1_off, dir2_off;   // Dirichlet tables (double2) for the Gram kernel
+  int xtile_items;          // exchange-scoring / output tiles
+  int xcand_off;            // first exchange candidate slot
+  int orig;                 // caller's scene index (per-scene outputs)
+  int pad;
+};
+
+// One unit of work of a pass: `count` consecutive lines (columns for the
+// column passes, rows for the row passes) of scene `scene`, starting at
+// `first`.  `slot` is the scene-local partial-sum slot.
+struct FbmeWork {
+  int scene, first, count, slot;
+};
+
+// Per-scene scalar state (doubles).  Written by the scalar kernels and read
+// by the passes; float32 state is stored rounded to float.
+enum FbmeScalar {
+  SC_POWER = 0,     // mean |y|^2 (clamped at eps)
+  SC_FLOOR = 1,     // variance floor max(1e-12 P, tiny)
+  SC_NOISE = 2,     // sigma^2
+  SC_DV = 3,        // data variance sum(v)
+  SC_DOF = 4,       // effective dof (clamped)
+  SC_BPE = 5,       // sum |A^H y|^2
+  SC_YE = 6,        // ||y||^2
+  SC_COUNT = 8
+};
+
+// Per-scene partial sums (doubles), `FBME_PART_STRIDE` slots per quantity.
+enum FbmePart {
+  PT_YE = 0,   // sum |y|^2                      (column pass, init)
+  PT_BPE = 1,  // sum |A^H y|^2                  (row pass, init)
+  PT_DOF = 2,  // sum relevance                  (row pass)
+  PT_VS = 3,   // sum posterior variance         (row pass)
+  PT_RES = 4,  // sum |y - A mu|^2               (column pass)
+  PT_COUNT = 5
+};
+
+
+
+#define FBME_KMAX 32
+#define FBME_TILE_MAX 4096     // complex elements per FFT tile (EPT * 256)
+#define FBME_XTILE 1024        // grid points per exchange-scoring tile

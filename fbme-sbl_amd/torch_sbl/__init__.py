@@ -1,0 +1,34 @@
+"""FBME-SBL numerical core on MI355X: UAMP + active-set EM plans, PCG path and
+graph plans backed by hand-written HIP kernels (libfbme_hip.so).
+
+Public names follow torch_sbl/__init__.py:1-22 of the reference for the FBME
+hot path.  Research extensions of the reference (autofocus, correlated MMV,
+off-grid refinement) are not part of this build.
+"""
+
+import os as _os
+import sys as _sys
+
+_PKG_ROOT = _os.path.dirname(_os.path.dirname(_os.path.abspath(__file__)))
+if _PKG_ROOT not in _sys.path:
+    _sys.path.insert(0, _PKG_ROOT)
+
+from .version import PAPER_RELEASE_ID, __version__  # noqa: E402
+from .config import SBLConfig, SBLResult  # noqa: E402
+from .solver import clear_operator_cache, fast_sbl, sbl_1d, sbl_2d  # noqa: E402
+from .operators import PrefixFourierOperator  # noqa: E402
+from .cuda_graph import (  # noqa: E402
+    CUDAGraphSBLPlan,
+    RealtimeSBLPlan,
+    make_cuda_graph_sbl_plan,
+    make_realtime_sbl_plan,
+)
+from .hip_kernels import reset_triton_runtime_error  # noqa: E402
+
+__all__ = [
+    "PAPER_RELEASE_ID", "__version__", "SBLConfig", "SBLResult",
+    "clear_operator_cache", "fast_sbl", "sbl_1d", "sbl_2d",
+    "PrefixFourierOperator", "CUDAGraphSBLPlan", "RealtimeSBLPlan",
+    "make_cuda_graph_sbl_plan", "make_realtime_sbl_plan",
+    "reset_triton_runtime_error",
+]

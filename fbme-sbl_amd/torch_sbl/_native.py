@@ -1,0 +1,230 @@
+"""Mapping from SBLConfig to the native FBME plan and back to SBLResult.
+
+The native plan implements the UAMP + evidence-gated active-set EM path of
+fast_sbl (torch_sbl/solver.py:925-2393) for the settings the locked plans and
+the fast/paper hybrid presets use.  Options that select research paths of
+the reference outside that path raise NotImplementedError naming the option,
+so nothing silently runs a different algorithm.
+"""
+
+from __future__ import annotations
+
+import math
+from collections import OrderedDict
+from math import ceil, prod
+
+import torch
+
+from fbme_amd import _lib
+from fbme_amd.plan import NativePlan
+
+from .config import SBLConfig, SBLResult
+
+_PLAN_CACHE: "OrderedDict[tuple, NativePlan]" = OrderedDict()
+_PLAN_CACHE_SIZE = 4
+
+
+def clear_cache() -> None:
+    _PLAN_CACHE.clear()
+
+
+def _unsupported(cfg: SBLConfig) -> list[str]:
+    bad = []
+    checks = {
+        "uamp_polish_size=0 (no active-set polish)": cfg.uamp_polish_size == 0,
+        "uamp_polish_prune_tail=False": not cfg.uamp_polish_prune_tail,
+        "uamp_polish_reinitialize=False": not cfg.uamp_polish_reinitialize,
+        "uamp_polish_evidence_gate=False": not cfg.uamp_polish_evidence_gate,
+        "uamp_polish_compute_truncation_diagnostic=True":
+            cfg.uamp_polish_compute_truncation_diagnostic,
+        "uamp_polish_max_truncation_delta": (
+            cfg.uamp_polish_max_truncation_delta is not None),
+        "uamp_polish_global_certificate": cfg.uamp_polish_global_certificate,
+        "uamp_polish_support_repairs": cfg.uamp_polish_support_repairs != 0,
+        "uamp_polish_support_growth_steps": (
+            cfg.uamp_polish_support_growth_steps != 0),
+        "uamp_polish_support_shrink_steps": (
+            cfg.uamp_polish_support_shrink_steps != 0),
+        "uamp_polish_support_prior": cfg.uamp_polish_support_prior,
+        "uamp_polish_high_precision": cfg.uamp_polish_high_precision,
+        "uamp_polish_coordinate_steps": cfg.uamp_polish_coordinate_steps != 0,
+        "uamp_polish_noise_coordinate_steps": (
+            cfg.uamp_polish_noise_coordinate_steps != 0),
+        "fixed_noise_variance": cfg.fixed_noise_variance is not None,
+        "joint_sparsity": cfg.joint_sparsity,
+        "spatial coupling": (cfg.spatial_coupling > 0.0
+                             or cfg.learn_spatial_coupling),
+        "initialization='uniform'": cfg.initialization != "backprojection",
+        "uamp_auto_iterations=False": not cfg.uamp_auto_iterations,
+        "tol>0 (eager convergence checks)": cfg.tol > 0.0,
+        "offgrid_refinement": cfg.offgrid_refinement,
+        "phase_error_correction": cfg.phase_error_correction,
+        "range_migration_correction": cfg.range_migration_correction,
+        "learn_snapshot_covariance": cfg.learn_snapshot_covariance,
+        f"uamp_polish_size>{32}": cfg.uamp_polish_size > 32,
+    }
+    for name, flag in checks.items():
+        if flag:
+            bad.append(name)
+    return bad
+
+
+def check_supported(cfg: SBLConfig) -> None:
+    bad = _unsupported(cfg)
+    if bad:
+        raise NotImplementedError(
+            "The MI355X FBME plan implements the UAMP + evidence-gated "
+            "active-set path of the fast/paper hybrid presets; unsupported "
+            "options: " + ", ".join(bad))
+
+
+def plan_options(cfg: SBLConfig, grid_shape, sample_shape) -> _lib.FbmePlanOptions:
+    """FbmePlanOptions equivalent of a UAMP-mode SBLConfig."""
+    n = prod(grid_shape)
+    m = prod(sample_shape)
+    damping = cfg.uamp_damping
+    if cfg.uamp_adaptive_damping:
+        # solver.py:1180-1186
+        damping = max(damping, min(cfg.uamp_max_damping, 0.02 * n / m))
+    opt = _lib.default_options(cfg.uamp_polish_residual_exchange_size > 0)
+    opt.uamp_damping = float(damping)
+    opt.damping = float(cfg.damping)
+    opt.min_variance = float(cfg.min_variance)
+    opt.max_precision = float(cfg.max_precision)
+    opt.blend = float(cfg.initialization_blend)
+    opt.noise_fraction = float(cfg.initial_noise_fraction)
+    opt.margin_factor = float(cfg.uamp_polish_evidence_margin_factor)
+    opt.jitter_multiplier = float(cfg.cuda_cholesky_jitter_multiplier)
+    opt.false_alarm = float(cfg.uamp_support_false_alarm_budget or 0.0)
+    opt.polish_size = int(min(cfg.uamp_polish_size, n))
+    opt.polish_iterations = int(cfg.uamp_polish_iterations)
+    opt.exchange_size = int(cfg.uamp_polish_residual_exchange_size)
+    opt.diagnostics = 1 if cfg.uamp_polish_fixed_budget_diagnostics else 0
+    opt.evidence_noise = 1 if cfg.noise_update == "evidence" else 0
+    opt.iteration_offset = float(cfg.uamp_iteration_offset)
+    opt.iteration_scale = float(cfg.uamp_iteration_scale)
+    opt.min_iters = int(max(cfg.min_iter, cfg.uamp_min_iter))
+    opt.max_iters = int(cfg.max_iter)
+    opt.hyper_shape = float(cfg.hyper_shape)
+    opt.hyper_rate = float(cfg.hyper_rate)
+    opt.noise_shape = float(cfg.noise_shape)
+    opt.noise_rate = float(cfg.noise_rate)
+    opt.polish_damping = float(cfg.uamp_polish_damping)
+    return opt
+
+
+def options_key(opt: _lib.FbmePlanOptions) -> tuple:
+    return tuple(getattr(opt, name) for name, _ in opt._fields_)
+
+
+def uamp_budget(cfg: SBLConfig, n: int, m: int) -> int:
+    minimum = max(cfg.min_iter, cfg.uamp_min_iter)
+    budget = ceil(cfg.uamp_iteration_offset + cfg.uamp_iteration_scale * n / m)
+    return min(cfg.max_iter, max(minimum, budget))
+
+
+def cached_plan(shapes, opt, device) -> NativePlan:
+    key = (tuple(shapes), options_key(opt), str(device))
+    plan = _PLAN_CACHE.get(key)
+    if plan is None:
+        plan = NativePlan(shapes, opt, device)
+        _PLAN_CACHE[key] = plan
+        while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
+            _PLAN_CACHE.popitem(last=False)
+    else:
+        _PLAN_CACHE.move_to_end(key)
+    return plan
+
+
+class ResultViews:
+    """Tensors of an SBLResult that alias a plan's outputs, built once so a
+    captured graph keeps ownership of every returned tensor."""
+
+    def __init__(self, plan: NativePlan, cfg: SBLConfig, batch_shape,
+                 grid_shape) -> None:
+        out = plan.out
+        dev = plan.device
+        bshape = tuple(batch_shape)
+        oshape = (*bshape, *grid_shape)
+        b = plan.nscenes
+        n = prod(grid_shape)
+        self.cfg = cfg
+        self.plan = plan
+        self.batch_shape = bshape
+        self.grid_shape = tuple(grid_shape)
+        self.x = out.x.view(oshape)
+        self.posterior_variance = out.posterior_variance.view(oshape)
+        self.precision = out.precision.view(oshape)
+        self.noise = out.noise.view(bshape)
+        self.dof = out.dof.view(bshape)
+        self.nle = out.nle.view(bshape)
+        self.trace = out.trace
+        self.hard = torch.zeros(b, dtype=torch.bool, device=dev).view(bshape)
+        self.selected = torch.ones(b, dtype=torch.bool, device=dev).view(bshape)
+        self.penalty = torch.zeros(b, dtype=torch.float32, device=dev).view(bshape)
+        self.exchanges = (out.exchanges.view(bshape)
+                          if cfg.uamp_polish_residual_exchange_size > 0 else None)
+        diag = cfg.uamp_polish_fixed_budget_diagnostics
+        self.gres = out.diag[:, 0].view(bshape) if diag else None
+        self.nres = out.diag[:, 1].view(bshape) if diag else None
+        self.decrease = out.diag[:, 2].view(bshape) if diag else None
+        self.monotone = (torch.zeros(b, dtype=torch.bool, device=dev).view(bshape)
+                         if diag else None)
+        fa = cfg.uamp_support_false_alarm_budget
+        if fa is not None:
+            self.mask = out.support_mask.view(oshape)
+            self.threshold = torch.full((b,), math.log(n / fa),
+                                        dtype=torch.float32,
+                                        device=dev).view(bshape)
+        else:
+            self.mask = None
+            self.threshold = None
+        self.iterations = uamp_budget(cfg, n, plan.shapes[0][2] * plan.shapes[0][3])
+        self.k = int(plan.options.polish_size)
+
+    def refresh(self) -> None:
+        """Device-side derived fields (enqueued on the current stream)."""
+        if self.monotone is not None:
+            self.monotone.copy_(self.plan.out.diag[:, 3].view(self.batch_shape) > 0.5)
+
+    def result(self, clone: bool = False) -> SBLResult:
+        res = self._result()
+        if clone:
+            for name in res.__slots__:
+                value = getattr(res, name)
+                if isinstance(value, torch.Tensor):
+                    setattr(res, name, value.clone())
+        return res
+
+    def _result(self) -> SBLResult:
+        return SBLResult(
+            x=self.x,
+            posterior_variance=self.posterior_variance,
+            precision=self.precision,
+            noise_variance=self.noise,
+            effective_dof=self.dof,
+            iterations=self.iterations,
+            converged=False,
+            relative_change=float("nan"),
+            mode="uamp",
+            arithmetic_dtype="complex64",
+            uamp_backend_used="hip",
+            active_set_size=self.k,
+            negative_log_evidence=self.nle,
+            evidence_trace=self.trace,
+            hard_pruning_accepted=self.hard,
+            truncation_certified=self.hard,
+            support_repairs_accepted=self.exchanges,
+            support_exchanges_accepted=self.exchanges,
+            active_model_selected=self.selected,
+            support_log_odds_penalty=self.penalty,
+            penalized_negative_log_evidence=self.nle,
+            penalized_evidence_trace=self.trace,
+            fixed_budget_gamma_residual=self.gres,
+            fixed_budget_noise_residual=self.nres,
+            fixed_budget_objective_decrease=self.decrease,
+            fixed_budget_monotone=self.monotone,
+            support_mask=self.mask,
+            support_score_threshold=self.threshold,
+            history={},
+        )

@@ -1,0 +1,159 @@
+"""Eager entry points fast_sbl / sbl_2d / sbl_1d on the MI355X kernels.
+
+Interface of torch_sbl/solver.py:813-931.  Leading dimensions of ``data`` are
+independent scenes solved together.  Execution modes:
+
+* ``mode="uamp"`` runs the native FBME plan (HIP kernels, one C call);
+* ``mode="cg"`` runs the covariance-free PCG + Hutchinson path on HIP
+  kernels (torch_sbl.pcg);
+* ``mode="dense"`` (and ``"auto"`` resolving to it) is the exact
+  observation-domain EM of the reference, which this build keeps only as the
+  CPU float64 oracle; on the GPU it raises NotImplementedError.
+
+Only GPU tensors are accepted: there is no CPU execution path.
+"""
+
+from __future__ import annotations
+
+from math import prod
+from typing import Any, Sequence
+
+import torch
+
+from . import _native
+from .config import SBLConfig, SBLResult
+
+__all__ = ["SBLConfig", "SBLResult", "fast_sbl", "sbl_1d", "sbl_2d",
+           "clear_operator_cache"]
+
+
+def clear_operator_cache() -> None:
+    """Release cached plans (torch_sbl/solver.py:842-844)."""
+    _native.clear_cache()
+    try:
+        from . import pcg
+        pcg.clear_cache()
+    except ImportError:
+        pass
+
+
+def sbl_1d(data: Any, grid_size: int, *, config: SBLConfig | None = None,
+           observation_mask: Any | None = None) -> SBLResult:
+    """1-D spectra: data [..., M] -> x [..., grid_size] (solver.py:847-867).
+    Runs as a 1 x N grid."""
+    tensor = torch.as_tensor(data)
+    res = fast_sbl(tensor.unsqueeze(-2), (1, int(grid_size)), config=config,
+                   observation_mask=(None if observation_mask is None else
+                                     torch.as_tensor(observation_mask)[None]))
+    squeeze = lambda t: None if t is None else t.squeeze(-2)  # noqa: E731
+    res.x = squeeze(res.x)
+    res.posterior_variance = squeeze(res.posterior_variance)
+    res.precision = squeeze(res.precision)
+    if res.support_mask is not None:
+        res.support_mask = squeeze(res.support_mask)
+    return res
+
+
+def sbl_2d(data: Any, grid_shape: Sequence[int], *,
+           config: SBLConfig | None = None,
+           observation_mask: Any | None = None) -> SBLResult:
+    """2-D scenes: data [..., M1, M2] -> x [..., N1, N2]
+    (solver.py:885-922)."""
+    shape = tuple(int(v) for v in grid_shape)
+    if len(shape) != 2:
+        raise ValueError("sbl_2d requires grid_shape=(N1, N2).")
+    if config is not None and (config.phase_error_correction
+                               or config.learn_snapshot_covariance):
+        raise NotImplementedError(
+            "autofocus and correlated-MMV reconstruction are outside the "
+            "FBME hot path of this build.")
+    return fast_sbl(data, shape, config=config,
+                    observation_mask=observation_mask)
+
+
+def _resolve_device(requested, tensor: torch.Tensor) -> torch.device:
+    if str(requested) != "auto":
+        device = torch.device(requested)
+        if device.type == "cuda" and device.index is None:
+            device = torch.device("cuda", torch.cuda.current_device())
+        return device
+    if tensor.device.type != "cpu":
+        return tensor.device
+    if torch.cuda.is_available():
+        return torch.device("cuda", torch.cuda.current_device())
+    return torch.device("cpu")
+
+
+def fast_sbl(data: Any, grid_shape: Sequence[int], *,
+             config: SBLConfig | None = None,
+             observation_mask: Any | None = None) -> SBLResult:
+    """Fourier SBL without materialising the dictionary
+    (torch_sbl/solver.py:925-2393)."""
+    cfg = config or SBLConfig()
+    cfg.validate()
+    if cfg.learn_snapshot_covariance:
+        raise ValueError(
+            "learn_snapshot_covariance is available through sbl_2d only.")
+    if cfg.phase_error_correction:
+        raise ValueError(
+            "phase_error_correction is available through sbl_2d only.")
+    grid = tuple(int(v) for v in grid_shape)
+    if len(grid) not in (1, 2):
+        raise ValueError("grid_shape must contain one or two dimensions.")
+    if len(grid) == 1:
+        grid = (1, grid[0])
+        data = torch.as_tensor(data).unsqueeze(-2)
+    if (cfg.uamp_support_false_alarm_budget is not None
+            and cfg.uamp_support_false_alarm_budget >= prod(grid)):
+        raise ValueError(
+            "uamp_support_false_alarm_budget must be smaller than the "
+            "reconstruction grid size.")
+    tensor = torch.as_tensor(data)
+    if tensor.ndim < 2:
+        raise ValueError("data has fewer dimensions than grid_shape.")
+    samples = tuple(int(v) for v in tensor.shape[-2:])
+    batch_shape = tuple(int(v) for v in tensor.shape[:-2])
+    if any(m > n for m, n in zip(samples, grid)):
+        raise ValueError(
+            "Observed dimensions cannot exceed reconstruction dimensions.")
+    device = _resolve_device(cfg.device, tensor)
+    if device.type != "cuda":
+        raise RuntimeError(
+            "This build runs FBME on MI355X GPUs only; pass a GPU tensor or "
+            "device (the CPU oracle lives in the test suite).")
+    if observation_mask is not None:
+        raise NotImplementedError(
+            "observation_mask is not supported by the MI355X plan yet.")
+    if tensor.dtype in (torch.complex128, torch.float64):
+        raise NotImplementedError(
+            "complex128 arithmetic is not provided by the MI355X kernels; "
+            "pass complex64 data.")
+    y = tensor.to(device=device, dtype=torch.complex64)
+    batch = prod(batch_shape) if batch_shape else 1
+    m = prod(samples)
+    threshold = cfg.dense_threshold if device.type == "cuda" else min(
+        cfg.dense_threshold, 512)
+    mode = ("dense" if cfg.mode == "auto" and m <= threshold else
+            "cg" if cfg.mode == "auto" else cfg.mode)
+    if mode == "dense":
+        raise NotImplementedError(
+            "mode='dense' (exact M x M EM) has no GPU implementation in this "
+            "build; use mode='cg' or mode='uamp'.")
+    if mode == "cg":
+        from . import pcg
+        return pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
+                          batch_shape=batch_shape)
+    if cfg.uamp_backend == "torch":
+        raise NotImplementedError(
+            "uamp_backend='torch' selects the reference's PyTorch UAMP; this "
+            "build always runs the HIP kernels.")
+    _native.check_supported(cfg)
+    opt = _native.plan_options(cfg, grid, samples)
+    shapes = [(grid[0], grid[1], samples[0], samples[1])] * batch
+    plan = _native.cached_plan(shapes, opt, device)
+    views = _native.ResultViews(plan, cfg, batch_shape, grid)
+    with torch.cuda.device(device):
+        plan.run(y.reshape(-1).contiguous())
+        views.refresh()
+        # Eager calls return fresh tensors; graph plans return plan storage.
+        return views.result(clone=True)

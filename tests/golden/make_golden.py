@@ -1,0 +1,125 @@
+"""Write golden input/output vectors from the reference's CPU Torch backend.
+
+Run in the build container only (the reference never travels to the GPU box):
+
+    python tests/golden/make_golden.py /root/reference
+
+What it pins (functions no closed-form check fixes end to end):
+  * ``fast_sbl`` under the FBME-Fast and FBME-Stable numerics
+    (SBLConfig.paper_hybrid_uamp with the profile overrides of
+    torch_sbl_fastest_stable/profiles.py:63-70, 110-115), run on CPU with
+    ``uamp_backend="torch"`` (the Torch UAMP branch, solver.py:2808-2850, is
+    the same algebra as the Triton kernels).  Topk boundaries and the
+    evidence-gate decisions depend on the whole trajectory.
+  * ``sbl_2d`` with the default ``SBLConfig`` on CPU (mode "auto" -> "cg",
+    solver.py:1025-1036) on the 32x32 / 24x24 workload.
+  * ``dense_em_sbl`` in complex128 on the same workload.
+
+Each fixture stores the scene inputs and the reference outputs as plain
+arrays in a compressed ``.npz`` (no pickles).
+"""
+
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "fbme-sbl_amd"))
+
+from fbme_amd import scenes  # noqa: E402
+
+
+def _np(t):
+    if t is None:
+        return None
+    return t.detach().cpu().numpy()
+
+
+def _result_arrays(res, prefix=""):
+    out = {}
+    for name in ("x", "posterior_variance", "precision", "noise_variance",
+                 "effective_dof", "negative_log_evidence", "evidence_trace",
+                 "support_mask", "support_score_threshold",
+                 "support_exchanges_accepted", "fixed_budget_gamma_residual",
+                 "fixed_budget_noise_residual",
+                 "fixed_budget_objective_decrease", "fixed_budget_monotone"):
+        value = getattr(res, name, None)
+        if value is not None:
+            out[prefix + name] = _np(value)
+    out[prefix + "iterations"] = np.array(res.iterations)
+    out[prefix + "active_set_size"] = np.array(res.active_set_size)
+    return out
+
+
+def main(reference_root: str) -> None:
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, fast_sbl, sbl_2d
+    from literature_reproduction import dense_em_sbl
+
+    torch.set_num_threads(8)
+
+    plan_cases = [
+        ("plan_32x32_24x24", scenes.uniform_scene(1000, (32, 32), (24, 24),
+                                                  k=8, snr_db=20.0)),
+        ("plan_40x32_20x16", scenes.uniform_scene(1001, (40, 32), (20, 16),
+                                                  k=21, snr_db=10.0)),
+        ("plan_64x64_40x40", scenes.uniform_scene(1002, (64, 64), (40, 40),
+                                                  k=20, snr_db=15.0)),
+        ("plan_96x80_60x48", scenes.ellipse_scene(1003, (96, 80), (60, 48),
+                                                  k=24, snr_db=10.0)),
+    ]
+    for name, scene in plan_cases:
+        data = torch.from_numpy(scene.y)
+        fast_cfg = SBLConfig.paper_hybrid_uamp(
+            device="cpu", uamp_backend="torch",
+            uamp_polish_size=32, uamp_polish_iterations=2,
+            uamp_polish_residual_exchange_size=0,
+            uamp_polish_fixed_budget_diagnostics=False,
+            uamp_polish_compute_truncation_diagnostic=False)
+        stable_cfg = SBLConfig.realtime_hybrid_uamp(
+            device="cpu", uamp_backend="torch", uamp_polish_size=32,
+            uamp_polish_iterations=3, uamp_polish_residual_exchange_size=12,
+            uamp_polish_fixed_budget_diagnostics=True,
+            uamp_polish_compute_truncation_diagnostic=False)
+        arrays = dict(y=scene.y, x_true=scene.x,
+                      grid=np.array(scene.grid), samples=np.array(scene.samples),
+                      noise_true=np.array(scene.noise_variance))
+        arrays.update(_result_arrays(fast_sbl(data, scene.grid,
+                                              config=fast_cfg), "fast_"))
+        arrays.update(_result_arrays(fast_sbl(data, scene.grid,
+                                              config=stable_cfg), "stable_"))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name)
+
+    scene = scenes.uniform_scene(1000, (32, 32), (24, 24), k=20, snr_db=20.0)
+    data64 = torch.from_numpy(scene.y)
+    data128 = data64.to(torch.complex128)
+    arrays = dict(y=scene.y, x_true=scene.x, grid=np.array(scene.grid),
+                  samples=np.array(scene.samples),
+                  noise_true=np.array(scene.noise_variance))
+    cg = sbl_2d(data64, scene.grid, config=SBLConfig(device="cpu"))
+    arrays.update(_result_arrays(cg, "cg_"))
+    arrays["cg_mode"] = np.array(cg.mode)
+    arrays["cg_history_relative_change"] = np.array(
+        cg.history.get("relative_change", []))
+    arrays["cg_history_cg_iterations"] = np.array(
+        cg.history.get("cg_iterations", []))
+    arrays["cg_history_active_set_size"] = np.array(
+        cg.history.get("active_set_size", []))
+    arrays["cg_history_noise_variance"] = np.array(
+        cg.history.get("noise_variance", []))
+    for iters in (30, 200):
+        dense = dense_em_sbl(data128, scene.grid, iterations=iters)
+        arrays.update(_result_arrays(dense, f"dense{iters}_"))
+    np.savez_compressed(os.path.join(HERE, "general_32x32_24x24.npz"),
+                        **arrays)
+    print("wrote general_32x32_24x24")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

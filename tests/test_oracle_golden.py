@@ -1,0 +1,69 @@
+"""The float64 oracle against golden vectors written by the reference's CPU
+Torch backend (tests/golden/make_golden.py).
+
+Tolerances: the reference computes in complex64 with float32 atom phases,
+so x agrees with the exact float64 oracle to ~1e-5 relative L2 on these
+scenes; the test uses 1e-4 (the hot path's stated parity bar) and requires
+identical supports, masks and exchange decisions.
+"""
+
+import numpy as np
+import pytest
+
+from conftest import golden_path
+from oracle import fbme_oracle as fo
+
+CASES = ["plan_32x32_24x24", "plan_40x32_20x16", "plan_64x64_40x40",
+         "plan_96x80_60x48"]
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("case", CASES)
+@pytest.mark.parametrize("profile", ["fast", "stable"])
+def test_oracle_matches_reference_golden(case, profile):
+    g = np.load(golden_path(case + ".npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    out = fo.fbme_plan(g["y"], grid, profile)
+    xr = g[profile + "_x"].reshape(-1)
+    assert _rel(out["x"], xr) < 1e-4
+    assert set(np.flatnonzero(out["x"])) == set(np.flatnonzero(xr))
+    assert np.array_equal(out["support_mask"],
+                          g[profile + "_support_mask"].reshape(-1))
+    assert abs(out["noise_variance"] / g[profile + "_noise_variance"] - 1) < 1e-4
+    assert _rel(out["posterior_variance"],
+                g[profile + "_posterior_variance"].reshape(-1)) < 1e-4
+    assert abs(out["effective_dof"] - g[profile + "_effective_dof"]) < 1e-3
+    trace = g[profile + "_evidence_trace"].reshape(-1)
+    assert np.allclose(out["evidence_trace"], trace, rtol=2e-4, atol=0.1)
+    assert int(out["iterations"]) == int(g[profile + "_iterations"])
+    if profile == "stable":
+        assert int(out["support_exchanges_accepted"]) == int(
+            g["stable_support_exchanges_accepted"])
+        assert abs(out["fixed_budget_gamma_residual"]
+                   - g["stable_fixed_budget_gamma_residual"]) < 1e-4
+        assert abs(out["fixed_budget_noise_residual"]
+                   - g["stable_fixed_budget_noise_residual"]) < 1e-4
+        assert bool(out["fixed_budget_monotone"]) == bool(
+            g["stable_fixed_budget_monotone"])
+
+
+def test_planted_mistake_is_caught():
+    """A plausible slip (UAMP damping 0.1 instead of 0.05, or a dropped
+    noise*dof term) must fail the golden comparison."""
+    g = np.load(golden_path("plan_64x64_40x40.npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    y = g["y"].reshape(1, -1).astype(np.complex128)
+    samples = tuple(int(v) for v in g["samples"])
+    cfg = fo.PlanConfig.fast()
+    cfg.uamp_damping = 0.1
+    sweep = fo.uamp_sweep(y, grid, samples, cfg)
+    res = fo.polish(y, sweep, grid, samples, cfg)
+    assert _rel(res["mean"][0], g["fast_x"].reshape(-1)) > 1e-3
+    cfg = fo.PlanConfig.fast()
+    cfg.noise_update = "evidence"
+    sweep = fo.uamp_sweep(y, grid, samples, cfg)
+    res = fo.polish(y, sweep, grid, samples, cfg)
+    assert _rel(res["mean"][0], g["fast_x"].reshape(-1)) > 1e-4
