@@ -1,0 +1,379 @@
+"""Float64 restatement of the covariance-free PCG + Hutchinson path of
+``sbl_2d`` with mode="cg" (test oracle).
+
+TEST INFRASTRUCTURE ONLY.  Follows torch_sbl/solver.py:
+  probe count rule 1194-1208, Rademacher probes 5979-6000 (the probe stream
+  is drawn from torch's CPU generator, as the reference does on CPU),
+  main loop 1315-1904 (active-set switch 1316-1341 with 3184-3216, CG
+  schedule 1343-1352 with 6086-6118, warm start 1363-1378 with 6121-6138,
+  CG posterior 2991-3036, relevance 3156-3181, MacKay 1669-1677, noise
+  1737-1768, damping 1483-1487 and 1776-1798, convergence 1800-1904),
+  final posterior 2023-2080, PCG 5582-5723, low-rank preconditioner
+  5786-5848, reduced active posterior with diffuse-tail loading 3591-3779.
+The Toeplitz/BTTB matvec of operators.py:299-371 is evaluated as the
+mathematically identical sigma^2 d + A (gamma . A^H d).
+
+Pins: tests/test_oracle_cg.py (PCG against a dense solve, Hutchinson
+unbiasedness against the exact diagonal, preconditioner against the dense
+Woodbury inverse, end-to-end against the reference's golden run).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from math import ceil
+
+import numpy as np
+
+from .fbme_oracle import (F32_EPS, F32_TINY, fourier_columns, prefix_adjoint,
+                          prefix_forward, reduced_posterior, gram_closed_form)
+
+
+@dataclass
+class CGConfig:
+    """SBLConfig defaults relevant to mode='cg' (solver.py:34-176)."""
+
+    max_iter: int = 200
+    min_iter: int = 3
+    tol: float = 1e-4
+    hyper_shape: float = 1e-8
+    hyper_rate: float = 1e-8
+    noise_shape: float = 1e-8
+    noise_rate: float = 1e-8
+    initial_noise_fraction: float = 1e-2
+    initialization_blend: float = 0.1
+    noise_update: str = "em"
+    damping: float = 0.15
+    min_variance: float = 1e-12
+    max_precision: float = 1e12
+    cg_tolerance: float = 1e-5
+    cg_initial_tolerance: float = 1e-2
+    cg_max_iter: int = 150
+    cg_check_interval: int = 8
+    cg_fixed_iterations: int = 0
+    cg_preconditioner_rank: int = 32
+    cg_damping: float = 0.9
+    hutchinson_probes: int = 32
+    max_hutchinson_probes: int = 128
+    auto_probe_scaling: bool = True
+    adaptive_cg: bool = True
+    cg_warm_start: bool = True
+    cg_min_iter: int = 20
+    convergence_patience: int = 2
+    convergence_check_interval: int = 2
+    active_set: bool = True
+    active_set_start: int = 30
+    active_set_check_interval: int = 2
+    active_set_min_size: int = 32
+    active_set_max_size: int = 256
+    active_set_tail_fraction: float = 0.05
+    seed: int = 0
+
+
+def probe_count(n: int, m: int, cfg: CGConfig) -> int:
+    """solver.py:1194-1208"""
+    if not cfg.auto_probe_scaling:
+        return cfg.hutchinson_probes
+    under = n / m
+    rec = 8.0 * under ** 0.5
+    if under > 16.0:
+        rec = max(rec, 2.0 * under)
+    rec = int(rec + 0.999999)
+    return min(cfg.max_hutchinson_probes, max(cfg.hutchinson_probes, rec))
+
+
+def rademacher_probes(batch: int, m: int, count: int, seed: int) -> np.ndarray:
+    """Rademacher signs from torch's CPU generator (solver.py:5979-6000)."""
+    import torch
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    signs = 2 * torch.randint(0, 2, (batch, m, count), generator=g) - 1
+    return signs.numpy().astype(np.float64)
+
+
+def cg_schedule(it: int, cfg: CGConfig) -> float:
+    """Tolerance schedule (solver.py:6086-6118, probes not adaptive)."""
+    if not cfg.adaptive_cg:
+        return cfg.cg_tolerance
+    if it <= 5:
+        return max(cfg.cg_initial_tolerance, cfg.cg_tolerance)
+    if it <= 15:
+        return max(cfg.cg_tolerance,
+                   (cfg.cg_initial_tolerance * cfg.cg_tolerance) ** 0.5)
+    return cfg.cg_tolerance
+
+
+class Operator:
+    """A, A^H and Q = noise I + A Gamma A^H on [B, M, R] blocks."""
+
+    def __init__(self, grid, samples):
+        self.grid = grid
+        self.samples = samples
+        self.n = grid[0] * grid[1]
+        self.m = samples[0] * samples[1]
+
+    def fwd(self, x):      # [B, N, R] -> [B, M, R]
+        b, _, r = x.shape
+        flat = np.moveaxis(x, 2, 1).reshape(b * r, self.n)
+        out = prefix_forward(flat, self.grid, self.samples)
+        return np.moveaxis(out.reshape(b, r, self.m), 1, 2)
+
+    def adj(self, v):      # [B, M, R] -> [B, N, R]
+        b, _, r = v.shape
+        flat = np.moveaxis(v, 2, 1).reshape(b * r, self.m)
+        out = prefix_adjoint(flat, self.grid, self.samples)
+        return np.moveaxis(out.reshape(b, r, self.n), 1, 2)
+
+    def q(self, d, gamma, noise):
+        return noise[:, None, None] * d + self.fwd(gamma[:, :, None] * self.adj(d))
+
+
+def low_rank_preconditioner(gamma, noise, op: Operator, rank, eps, tiny):
+    """Isotropic background plus the strongest atoms (solver.py:5786-5848)."""
+    rank = min(rank, op.m, op.n)
+    order = np.argsort(-gamma, axis=1, kind="stable")[:, :rank]
+    vals = np.take_along_axis(gamma, order, 1)
+    cols = fourier_columns(order, op.grid, op.samples)
+    factors = cols * np.sqrt(vals)[:, None, :]
+    total = gamma.sum(1)
+    omitted = np.maximum(total - vals.sum(1), 0.0)
+    bg = np.maximum(noise + omitted, eps * total)
+    gram = np.einsum("bmk,bml->bkl", np.conj(factors), factors)
+    small = np.eye(rank) + gram / bg[:, None, None]
+    small = 0.5 * (small + np.conj(np.swapaxes(small, 1, 2)))
+    scale = np.maximum(np.mean(np.abs(np.real(np.diagonal(small, axis1=1, axis2=2))), 1), tiny)
+    for mult in (10.0, 1e2, 1e3, 1e4, 1e5):
+        try:
+            chol = np.linalg.cholesky(small + (mult * eps * scale)[:, None, None] * np.eye(rank))
+            break
+        except np.linalg.LinAlgError:
+            chol = None
+    linv = np.linalg.inv(chol)
+    inv = np.conj(np.swapaxes(linv, 1, 2)) @ linv
+    return bg, factors, inv
+
+
+def apply_preconditioner(v, pre):
+    bg, factors, inv = pre
+    base = v / bg[:, None, None]
+    proj = np.conj(np.swapaxes(factors, 1, 2)) @ v
+    corr = inv @ (proj / bg[:, None, None])
+    return base - (factors @ corr) / bg[:, None, None]
+
+
+def pcg(rhs, gamma, noise, op: Operator, *, x0, tol, max_iter, check, rank,
+        fixed=0, eps=F32_EPS, tiny=F32_TINY):
+    """Batched complex PCG with lane masks and true-residual restarts
+    (solver.py:5582-5723)."""
+    if x0 is None:
+        x = np.zeros_like(rhs)
+        r = rhs.copy()
+    else:
+        x = x0.copy()
+        r = rhs - op.q(x, gamma, noise)
+    pre = low_rank_preconditioner(gamma, noise, op, rank, eps, tiny)
+    z = apply_preconditioner(r, pre)
+    d = z.copy()
+    rho = np.real(np.sum(np.conj(r) * z, axis=1))
+    init = np.maximum(np.linalg.norm(rhs, axis=1), eps)
+    best_x = x.copy() if fixed > 0 else None
+    best_rel = np.linalg.norm(r, axis=1) / init if fixed > 0 else None
+    active = np.linalg.norm(r, axis=1) > tol * init
+    d = np.where(active[:, None, :], d, 0.0)
+    limit = fixed if fixed > 0 else max_iter
+    it = 0
+    for it in range(1, limit + 1):
+        qd = op.q(d, gamma, noise)
+        curv = np.real(np.sum(np.conj(d) * qd, axis=1))
+        valid = (active & np.isfinite(curv) & np.isfinite(rho)
+                 & (curv > tiny) & (rho > tiny))
+        alpha = np.where(valid, rho / np.maximum(curv, tiny), 0.0)
+        x = x + alpha[:, None, :] * d
+        r = r - alpha[:, None, :] * qd
+        active = valid
+        if it % check == 0 or it == limit:
+            r = rhs - op.q(x, gamma, noise)
+            rel = np.linalg.norm(r, axis=1) / init
+            if fixed > 0:
+                better = rel < best_rel
+                best_x = np.where(better[:, None, :], x, best_x)
+                best_rel = np.minimum(best_rel, rel)
+            ra = np.isfinite(rel) & (rel > tol)
+            active = (active & ra) if fixed > 0 else ra
+            z = apply_preconditioner(r, pre)
+            rz = np.real(np.sum(np.conj(r) * z, axis=1))
+            active = active & np.isfinite(rz) & (rz > tiny)
+            d = np.where(active[:, None, :], z, 0.0)
+            rho = np.where(active, rz, rho)
+            if fixed == 0 and not active.any():
+                break
+            continue
+        z = apply_preconditioner(r, pre)
+        nrho = np.real(np.sum(np.conj(r) * z, axis=1))
+        vu = active & np.isfinite(nrho) & (nrho > tiny)
+        beta = np.where(vu, nrho / np.maximum(rho, tiny), 0.0)
+        d = np.where(vu[:, None, :], z + beta[:, None, :] * d, 0.0)
+        rho = np.where(vu, nrho, rho)
+        active = vu
+    return (best_x if best_x is not None else x), it
+
+
+def project_relevance(gamma, dd, m, eps):
+    rel = np.clip(gamma * dd, 0.0, 1.0)
+    raw = rel.sum(1)
+    scale = np.minimum(1.0, m * (1.0 - 10.0 * eps) / np.maximum(raw, eps))
+    return rel * scale[:, None]
+
+
+def select_active(gamma, cfg: CGConfig):
+    """solver.py:3184-3216 (batch of one)."""
+    if gamma.shape[0] != 1 or not np.all(np.isfinite(gamma)):
+        return None
+    size_max = min(cfg.active_set_max_size, gamma.shape[1])
+    order = np.argsort(-gamma[0], kind="stable")[:size_max]
+    vals = gamma[0, order]
+    total = gamma[0].sum()
+    target = (1.0 - cfg.active_set_tail_fraction) * total
+    if vals.sum() < target:
+        return None
+    cum = np.cumsum(vals)
+    required = min(int(np.searchsorted(cum, target)), size_max - 1) + 1
+    size = min(size_max, max(cfg.active_set_min_size, required))
+    return order[None, :size]
+
+
+def active_posterior(y, gamma, noise, sup, op: Operator, eps, tiny):
+    """Reduced posterior with diffuse-tail loading nu = noise + omitted
+    variance, CPU Cholesky loading (solver.py:3591-3779, 3897-3924)."""
+    b = y.shape[0]
+    rows = np.arange(b)[:, None]
+    gs = gamma[rows, sup]
+    omitted = np.maximum(gamma.sum(1) - gs.sum(1), 0.0)
+    nu = noise + omitted
+    gram = gram_closed_form(sup, op.grid, op.samples)
+    cdat = prefix_adjoint(y, op.grid, op.samples)[rows, sup]
+    mu, sig, dd, _ = reduced_posterior(gram, cdat, np.sum(np.abs(y) ** 2, 1),
+                                       gs, nu, op.m, eps=eps, tiny=tiny,
+                                       jitter_multiplier=1.0)
+    mean = np.zeros_like(gamma, dtype=np.complex128)
+    mean[rows, sup] = mu
+    var = gamma.copy()
+    var[rows, sup] = sig
+    ddiag = np.zeros_like(gamma)
+    ddiag[rows, sup] = dd
+    return mean, var, ddiag
+
+
+def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
+           tiny=F32_TINY, trace=None) -> dict:
+    """Default ``sbl_2d`` in mode='cg' on [B, M1, M2] (or [M1, M2]) data."""
+    cfg = cfg or CGConfig()
+    data = np.asarray(data)
+    single = data.ndim == 2
+    if single:
+        data = data[None]
+    b = data.shape[0]
+    samples = tuple(data.shape[-2:])
+    grid = tuple(int(v) for v in grid)
+    op = Operator(grid, samples)
+    n, m = op.n, op.m
+    y = data.reshape(b, m).astype(np.complex128)
+    power = np.maximum(np.mean(np.abs(y) ** 2, 1), eps)
+    floor = np.maximum(power * cfg.min_variance, tiny)
+    bp = prefix_adjoint(y, grid, samples)
+    energy = np.abs(bp) ** 2
+    structured = energy / np.maximum(energy.sum(1, keepdims=True), eps) * power[:, None]
+    gamma = np.maximum(cfg.initialization_blend * structured
+                       + (1 - cfg.initialization_blend) * power[:, None] / n,
+                       floor[:, None])
+    noise = np.maximum(cfg.initial_noise_fraction * power, floor)
+    pcount = probe_count(n, m, cfg)
+    probes = rademacher_probes(b, m, pcount, cfg.seed).astype(np.complex128)
+    lprobes = op.adj(probes)
+    prev = np.zeros((b, n), np.complex128)
+    keep = max(cfg.damping, cfg.cg_damping)
+    sup = None
+    sol = None
+    streak = 0
+    converged = False
+    min_iters = max(cfg.min_iter, cfg.cg_min_iter)
+    tol_floor = 3.0 * eps ** 0.5
+    it = 0
+    cg_iters = []
+    for it in range(1, cfg.max_iter + 1):
+        if (cfg.active_set and b == 1 and sup is None
+                and it >= cfg.active_set_start
+                and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
+            sup = select_active(gamma, cfg)
+            if sup is not None:
+                sol = None
+        tol = max(cg_schedule(it, cfg), tol_floor)
+        if sup is not None:
+            mean, var, dd = active_posterior(y, gamma, noise, sup, op, eps, tiny)
+            cg_iters.append(0)
+            sol = None
+        else:
+            rhs = np.concatenate([y[:, :, None], probes], axis=2)
+            sol, nit = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
+                           tol=tol, max_iter=cfg.cg_max_iter,
+                           check=cfg.cg_check_interval,
+                           rank=cfg.cg_preconditioner_rank,
+                           fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
+            cg_iters.append(nit)
+            lifted = op.adj(sol)
+            dd = np.maximum(np.mean(np.real(lifted[:, :, 1:] * np.conj(lprobes)), 2), 0.0)
+            mean = gamma * lifted[:, :, 0]
+            var = np.maximum(gamma - gamma ** 2 * dd, 0.0)
+        rel = project_relevance(gamma, dd, m, eps)
+        dof = rel.sum(1)
+        prec = np.clip((rel + cfg.hyper_shape) / (np.abs(mean) ** 2 + cfg.hyper_rate),
+                       1.0 / cfg.max_precision, cfg.max_precision)
+        new_gamma = 1.0 / prec
+        if sup is not None:
+            mask = np.zeros_like(gamma, bool)
+            mask[np.arange(b)[:, None], sup] = True
+            new_gamma = np.where(mask, new_gamma, gamma)
+        resid = y - prefix_forward(mean, grid, samples)
+        e = np.sum(np.abs(resid) ** 2, 1)
+        if cfg.noise_update == "em":
+            new_noise = (e + noise * dof + cfg.noise_rate) / (m + cfg.noise_shape)
+        else:
+            new_noise = (e + cfg.noise_rate) / np.maximum(m - dof + cfg.noise_shape, eps)
+        new_noise = np.maximum(new_noise, floor)
+        gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, floor[:, None])
+        noise = keep * noise + (1 - keep) * new_noise
+        change = np.max(np.linalg.norm(mean - prev, axis=1)
+                        / np.maximum(np.linalg.norm(mean, axis=1), eps))
+        prev = mean
+        if trace is not None:
+            trace.append(dict(change=change, noise=noise.copy(), active=0 if sup is None else sup.shape[1]))
+        check = (cfg.tol > 0 and it >= min_iters
+                 and ((it - min_iters) % cfg.convergence_check_interval == 0
+                      or it == cfg.max_iter))
+        if check:
+            streak = streak + 1 if change <= cfg.tol else 0
+            if streak >= cfg.convergence_patience:
+                converged = True
+                break
+    final_tol = max(cfg.cg_tolerance, tol_floor)
+    if sup is not None:
+        mean, var, dd = active_posterior(y, gamma, noise, sup, op, eps, tiny)
+    else:
+        rhs = np.concatenate([y[:, :, None], probes], axis=2)
+        sol, _ = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
+                     tol=final_tol, max_iter=cfg.cg_max_iter,
+                     check=cfg.cg_check_interval, rank=cfg.cg_preconditioner_rank,
+                     fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
+        lifted = op.adj(sol)
+        dd = np.maximum(np.mean(np.real(lifted[:, :, 1:] * np.conj(lprobes)), 2), 0.0)
+        mean = gamma * lifted[:, :, 0]
+        var = np.maximum(gamma - gamma ** 2 * dd, 0.0)
+    dof = project_relevance(gamma, dd, m, eps).sum(1)
+    out = dict(x=mean, posterior_variance=var, precision=1.0 / gamma,
+               noise_variance=noise, effective_dof=dof, iterations=it,
+               converged=converged, active_set_size=0 if sup is None else sup.shape[1],
+               cg_iterations=cg_iters, probes=pcount)
+    if single:
+        out = {k: (v[0] if isinstance(v, np.ndarray) and v.shape[:1] == (1,) else v)
+               for k, v in out.items()}
+    return out
