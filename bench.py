@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device copy (counter calibration)")
     return ap.parse_args()
 
 
@@ -150,6 +152,12 @@ def main():
                 "avg_launch_ms": stage_ms["row"]}
     per_kernel = {k: {"ms": v, "GB/s": kb[k] * b / (v / 1e3) / 1e9}
                   for k, v in stage_ms.items()}
+
+    if args.calibrate_copy:
+        src = torch.empty(64 << 20, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     # ----------------------------------------------------------- CPU baseline
     cpu = None

@@ -122,10 +122,22 @@ __device__ __forceinline__ void build_twiddles(float2* tw, int len) {
   }
 }
 
+// LDS index with one pad slot every 16 complex values: stride-R butterfly
+// writes of the first stages then spread over distinct banks.
+__device__ __forceinline__ int lds_idx(int i) { return i + (i >> 4); }
+// LDS complex slots needed for `elems` logical elements.
+#define FBME_LDS_SLOTS(elems) ((elems) + ((elems) >> 4) + 80)
+
+// Copy a precomputed twiddle table (rounded once from double on the host)
+// into LDS.
+__device__ __forceinline__ void load_twiddles(float2* tw, const float2* __restrict__ table, int len) {
+  for (int i = threadIdx.x; i < len; i += blockDim.x) tw[i] = table[i];
+}
+
 // One Stockham stage of radix R over `nlines` lines of length `len` stored at
-// buf[line * stride + j].  Ns is the product of the radices already applied.
-// All inputs of the stage are loaded to registers before the barrier and all
-// outputs written after it, so the stage runs in place.
+// buf[lds_idx(line * stride + j)].  Ns is the product of the radices already
+// applied.  All inputs of the stage are loaded to registers before the
+// barrier and all outputs written after it, so the stage runs in place.
 // EPT bounds the elements per thread: nlines * len <= EPT * blockDim.x.
 template <int R, int EPT, bool INV>
 __device__ __forceinline__ void fft_stage(float2* buf, int len, int nlines, int stride, int ns,
@@ -140,9 +152,9 @@ __device__ __forceinline__ void fft_stage(float2* buf, int len, int nlines, int 
     if (g < total) {
       const int line = g / per_line;
       const int j = g - line * per_line;
-      const float2* src = buf + line * stride + j;
+      const int base = line * stride + j;
 #pragma unroll
-      for (int t = 0; t < R; ++t) v[b][t] = src[t * per_line];
+      for (int t = 0; t < R; ++t) v[b][t] = buf[lds_idx(base + t * per_line)];
     }
   }
   __syncthreads();
@@ -163,9 +175,9 @@ __device__ __forceinline__ void fft_stage(float2* buf, int len, int nlines, int 
         }
       }
       butterfly<R, INV>(v[b]);
-      float2* dst = buf + line * stride + (j - k) * R + k;
+      const int base = line * stride + (j - k) * R + k;
 #pragma unroll
-      for (int t = 0; t < R; ++t) dst[t * ns] = v[b][t];
+      for (int t = 0; t < R; ++t) buf[lds_idx(base + t * ns)] = v[b][t];
     }
   }
   __syncthreads();
@@ -176,7 +188,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int len, int nlines, int 
 // filling `buf` and the twiddle table.
 template <int EPT, bool INV>
 __device__ __forceinline__ void fft_lines(float2* buf, int len, int nlines, int stride, unsigned code,
-                          const float2* tw) {
+                                          const float2* tw) {
   int ns = 1;
   while (code) {
     const int r = code & 15u;
@@ -262,14 +274,36 @@ __device__ int block_select_topk(const uint32_t* keys, int n, int k, int* out_po
       if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      int cum = 0, d = 255;
-      for (; d > 0; --d) {
-        if (cum + (int)hist[d] >= remaining) break;
-        cum += (int)hist[d];
+    if (threadIdx.x < 64) {
+      // Wave-parallel search from the top digit: lane l owns digits
+      // 255-4l .. 252-4l; an inclusive scan of the lane sums finds the lane
+      // where the count of larger-or-equal keys first reaches `remaining`.
+      const int lane = threadIdx.x;
+      const int b0 = 255 - 4 * lane;
+      const int c0 = (int)hist[b0], c1 = (int)hist[b0 - 1], c2 = (int)hist[b0 - 2], c3 = (int)hist[b0 - 3];
+      const int local = c0 + c1 + c2 + c3;
+      int incl = local;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
       }
-      misc[0] = d;
-      misc[1] = remaining - cum;
+      const int excl = incl - local;
+      if (excl < remaining && remaining <= incl) {
+        int cum = excl, d;
+        if (cum + c0 >= remaining) {
+          d = b0;
+        } else if ((cum += c0) + c1 >= remaining) {
+          d = b0 - 1;
+        } else if ((cum += c1) + c2 >= remaining) {
+          d = b0 - 2;
+        } else {
+          cum += c2;
+          d = b0 - 3;
+        }
+        misc[0] = d;
+        misc[1] = remaining - cum;
+      }
     }
     __syncthreads();
     prefix |= ((uint32_t)misc[0]) << shift;

@@ -22,7 +22,7 @@ template <bool INV>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_op_col(const float2* __restrict__ in, float2* __restrict__ out, int n, int rows_in, int rows_out, int cols,
          int tile_cols, unsigned code) {
-  __shared__ float2 buf[OP_TILE + 64];
+  __shared__ float2 buf[FBME_LDS_SLOTS(OP_TILE)];
   __shared__ float2 tw[1024];
   const int b = blockIdx.y;
   const int c0 = blockIdx.x * tile_cols;
@@ -32,14 +32,14 @@ k_op_col(const float2* __restrict__ in, float2* __restrict__ out, int n, int row
   const float2* src = in + (long long)b * rows_in * cols;
   for (int e = threadIdx.x; e < n * cc; e += FBME_BLOCK) {
     const int i = e / cc, c = e - i * cc;
-    buf[c * stride + i] = (i < rows_in) ? src[(long long)i * cols + c0 + c] : make_float2(0.f, 0.f);
+    buf[lds_idx(c * stride + i)] = (i < rows_in) ? src[(long long)i * cols + c0 + c] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lines<16, INV>(buf, n, cc, stride, code, tw);
   float2* dst = out + (long long)b * rows_out * cols;
   for (int e = threadIdx.x; e < rows_out * cc; e += FBME_BLOCK) {
     const int i = e / cc, c = e - i * cc;
-    dst[(long long)i * cols + c0 + c] = buf[c * stride + i];
+    dst[(long long)i * cols + c0 + c] = buf[lds_idx(c * stride + i)];
   }
 }
 
@@ -48,7 +48,7 @@ template <bool INV>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_op_row(const float2* __restrict__ in, float2* __restrict__ out, int n, int len_in, int len_out, int rows,
          int tile_rows, unsigned code) {
-  __shared__ float2 buf[OP_TILE + 64];
+  __shared__ float2 buf[FBME_LDS_SLOTS(OP_TILE)];
   __shared__ float2 tw[1024];
   const int b = blockIdx.y;
   const int r0 = blockIdx.x * tile_rows;
@@ -58,14 +58,14 @@ k_op_row(const float2* __restrict__ in, float2* __restrict__ out, int n, int len
   const float2* src = in + ((long long)b * rows + r0) * len_in;
   for (int e = threadIdx.x; e < rr * n; e += FBME_BLOCK) {
     const int r = e / n, j = e - r * n;
-    buf[r * stride + j] = (j < len_in) ? src[(long long)r * len_in + j] : make_float2(0.f, 0.f);
+    buf[lds_idx(r * stride + j)] = (j < len_in) ? src[(long long)r * len_in + j] : make_float2(0.f, 0.f);
   }
   __syncthreads();
   fft_lines<16, INV>(buf, n, rr, stride, code, tw);
   float2* dst = out + ((long long)b * rows + r0) * len_out;
   for (int e = threadIdx.x; e < rr * len_out; e += FBME_BLOCK) {
     const int r = e / len_out, j = e - r * len_out;
-    dst[(long long)r * len_out + j] = buf[r * stride + j];
+    dst[(long long)r * len_out + j] = buf[lds_idx(r * stride + j)];
   }
 }
 

@@ -301,7 +301,7 @@ __device__ void rank_sort_desc(const uint32_t* key, const int* idx, int k, uint3
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(FBME_BLOCK)
+__global__ void __launch_bounds__(64)
 k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const float2* __restrict__ bp_all,
          const double* __restrict__ sc, const double2* __restrict__ dir, const uint32_t* __restrict__ cand_key,
          const int* __restrict__ cand_idx, const uint32_t* __restrict__ xcand_key,
@@ -309,9 +309,18 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
          float* __restrict__ out_noise, float* __restrict__ out_dof, float* __restrict__ out_nle,
          float* __restrict__ out_trace, long long* __restrict__ out_exchanges, float* __restrict__ out_diag,
          int* __restrict__ status, PolishArgs a) {
-  __shared__ PolishLds L;
-  __shared__ uint32_t keys[FBME_TILE_MAX];
-  __shared__ int kidx[FBME_TILE_MAX];
+  // The candidate keys are consumed before the first evaluation, so they
+  // share LDS with the matrices.
+  __shared__ union {
+    PolishLds L;
+    struct {
+      uint32_t keys[FBME_TILE_MAX];
+      int kidx[FBME_TILE_MAX];
+    } c;
+  } lds;
+  PolishLds& L = lds.L;
+  uint32_t* keys = lds.c.keys;
+  int* kidx = lds.c.kidx;
   __shared__ uint32_t hist[256];
   __shared__ int misc[4];
   __shared__ int pos[KM];
@@ -756,7 +765,9 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
   a.exchange_size = o->exchange_size;
   a.diagnostics = o->diagnostics;
   a.evidence_noise = o->evidence_noise;
-  hipLaunchKernelGGL(k_polish, dim3(nscenes), dim3(FBME_BLOCK), 0, stream, scenes, nscenes, phase,
+  // One wavefront per scene: the K x K work is latency bound, and with a
+  // single wave every barrier is cheap.
+  hipLaunchKernelGGL(k_polish, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
                      (const float2*)bp, sc, (const double2*)dir, cand_key, cand_idx, xcand_key, xcand_idx,
                      (PolishState*)states, (PolishResult*)results, out_noise, out_dof, out_nle, out_trace,
                      out_exchanges, out_diag, status, a);

@@ -27,13 +27,13 @@
 #include "fbme_types.h"
 
 extern "C" {
-int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
-                            const double*, double*, int, int, int, float, hipStream_t);
-int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*,
+int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*, void*,
+                            void*, const double*, double*, int, int, int, float, const void*, hipStream_t);
+int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, int, void*, void*, float*, void*, float*,
                     const double*, double*, int, int, float, float, float, float, float, float, uint32_t*, int*,
-                    int, hipStream_t);
-int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
-                            const double*, double*, int, int, float, hipStream_t);
+                    int, const void*, hipStream_t);
+int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*, void*,
+                            void*, const double*, double*, int, int, float, const void*, hipStream_t);
 int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
                         float, float, float, float, float, int, hipStream_t);
 size_t fbme_polish_state_bytes(void);
@@ -63,12 +63,14 @@ struct fbme_plan {
   std::vector<FbmeWork> col_work, row_work, tile_work;
   std::vector<int> col_prefix, row_prefix, scene_prefix;  // per UAMP step t (index t-1)
   std::vector<double> dir;                  // Dirichlet tables, interleaved (re, im)
+  std::vector<float> twid;                  // twiddle tables, interleaved (re, im)
   std::vector<long long> grid_off, samp_off; // caller order
   long long total_grid = 0, total_samp = 0, total_mid = 0;
   int total_slots = 0, total_cand = 0, total_xcand = 0;
+  int ept_col = 16, ept_row = 16;           // elements per thread of the passes
   float threshold = 0.f;
   // workspace layout
-  Region r_scenes, r_col, r_row, r_tile, r_dir, r_sc, r_part, r_mid, r_msg, r_pred, r_bp, r_gamma, r_mean, r_var,
+  Region r_scenes, r_col, r_row, r_tile, r_dir, r_tw, r_sc, r_part, r_mid, r_msg, r_pred, r_bp, r_gamma, r_mean, r_var,
       r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result;
   size_t workspace_bytes = 0;
   char* ws = nullptr;
@@ -156,8 +158,8 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
   o->exchange_size = stable ? 12 : 0;
   o->diagnostics = stable ? 1 : 0;
   o->evidence_noise = 0;
-  o->row_lines = 8;
-  o->col_lines = 16;
+  o->row_lines = 4;
+  o->col_lines = 8;
   o->iteration_offset = 0.0f;
   o->iteration_scale = 0.8f;
   o->min_iters = 3;
@@ -244,6 +246,23 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
     koff.push_back(off);
     return off;
   };
+  // twiddle tables per distinct FFT length, tw[i] = exp(-2 pi i * i / L)
+  std::vector<int> tw_len, tw_off;
+  auto twiddle = [&](int len) -> int {
+    for (size_t i = 0; i < tw_len.size(); ++i)
+      if (tw_len[i] == len) return tw_off[i];
+    const int off = (int)(p->twid.size() / 2);
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < len; ++i) {
+      // exact integer reduction of the angle to [0, 2 pi)
+      const double a = 2.0 * pi * (double)i / (double)len;
+      p->twid.push_back((float)std::cos(a));
+      p->twid.push_back((float)-std::sin(a));
+    }
+    tw_len.push_back(len);
+    tw_off.push_back(off);
+    return off;
+  };
   // work tables
   int slots = 0, cand = 0, xcand = 0;
   const int exch_cfg = opt->exchange_size;
@@ -251,6 +270,8 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
     FbmeScene& s = p->scenes[si];
     s.dir1_off = table(s.m1, s.n1);
     s.dir2_off = table(s.m2, s.n2);
+    s.tw1_off = twiddle(s.n1);
+    s.tw2_off = twiddle(s.n2);
     const int cl = std::max(1, std::min({s.m2, FBME_TILE_MAX / s.n1, std::max(1, opt->col_lines)}));
     // Row tiles hold rl rows of n2 + 1 (padded) complex values in LDS, so
     // rl * (n2 + 1) <= FBME_TILE_MAX + 64 requires rl <= 64 as well.
@@ -288,6 +309,12 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
       p->tile_work.push_back({si, (int)first, (int)std::min<long long>(FBME_XTILE, n - first), t});
     }
   }
+  auto ept_for = [](int elems) { return elems <= 1024 ? 4 : elems <= 2048 ? 8 : 16; };
+  int max_col = 1, max_row = 1;
+  for (const FbmeWork& w : p->col_work) max_col = std::max(max_col, w.count * p->scenes[w.scene].n1);
+  for (const FbmeWork& w : p->row_work) max_row = std::max(max_row, w.count * p->scenes[w.scene].n2);
+  p->ept_col = ept_for(max_col);
+  p->ept_row = ept_for(max_row);
   p->total_slots = slots;
   p->total_cand = cand;
   p->total_xcand = std::max(xcand, 1);
@@ -320,6 +347,7 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
   reg(p->r_row, sizeof(FbmeWork) * p->row_work.size());
   reg(p->r_tile, sizeof(FbmeWork) * p->tile_work.size());
   reg(p->r_dir, sizeof(double) * p->dir.size());
+  reg(p->r_tw, sizeof(float) * p->twid.size());
   reg(p->r_sc, sizeof(double) * SC_COUNT * nscenes);
   reg(p->r_part, sizeof(double) * PT_COUNT * (size_t)slots);
   reg(p->r_mid, 8 * (size_t)p->total_mid);
@@ -393,6 +421,7 @@ int fbme_plan_bind(fbme_plan* p, void* workspace, void* stream) {
   if (e == hipSuccess) e = up(p->r_row, p->row_work.data());
   if (e == hipSuccess) e = up(p->r_tile, p->tile_work.data());
   if (e == hipSuccess) e = up(p->r_dir, p->dir.data());
+  if (e == hipSuccess) e = up(p->r_tw, p->twid.data());
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   if (e != hipSuccess) return FBME_ERR_HIP;
   p->bound = 1;
@@ -412,6 +441,7 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   const FbmeWork* roww = (const FbmeWork*)(w + p->r_row.off);
   const FbmeWork* tilew = (const FbmeWork*)(w + p->r_tile.off);
   const void* dir = w + p->r_dir.off;
+  const void* tw = w + p->r_tw.off;
   double* sc = (double*)(w + p->r_sc.off);
   double* part = (double*)(w + p->r_part.off);
   void* mid = w + p->r_mid.off;
@@ -438,26 +468,26 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
     if (rc) return FBME_ERR_HIP; \
   } while (0)
   // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
-  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 0, 0, o.uamp_damping, st));
-  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 0, o.uamp_damping, o.damping,
-                           o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, st));
+  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 0, 0, o.uamp_damping, tw, st));
+  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 0, o.uamp_damping, o.damping,
+                           o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, tw, st));
   FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction, o.blend,
                                o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
   for (int t = 1; t <= p->max_iters; ++t) {
     const int nc = p->col_prefix[t - 1], nr = p->row_prefix[t - 1], ns = p->scene_prefix[t - 1];
     if (t > 1)
-      FBME_TRY(fbme_launch_col_adjoint(scenes, colw, nc, nc, io->y, pred, msg, mid, sc, part, ps, 1, 1, o.uamp_damping, st));
-    FBME_TRY(fbme_launch_row(scenes, roww, nr, nr, mid, bp, gamma, mean, var, sc, part, ps, t == 1 ? 1 : 2,
-                             o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, st));
-    FBME_TRY(fbme_launch_col_forward(scenes, colw, nc, nc, mid, io->y, pred, msg, sc, part, ps, t == 1 ? 1 : 0,
-                                     o.uamp_damping, st));
+      FBME_TRY(fbme_launch_col_adjoint(scenes, colw, nc, nc, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 1, o.uamp_damping, tw, st));
+    FBME_TRY(fbme_launch_row(scenes, roww, nr, nr, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, t == 1 ? 1 : 2,
+                             o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, tw, st));
+    FBME_TRY(fbme_launch_col_forward(scenes, colw, nc, nc, p->ept_col, mid, io->y, pred, msg, sc, part, ps, t == 1 ? 1 : 0,
+                                     o.uamp_damping, tw, st));
     FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, 1, nullptr, o.min_variance, o.noise_fraction, o.blend, o.damping,
                                  o.noise_shape, o.noise_rate, o.evidence_noise, st));
   }
   // Final consistency E-step (no prediction) + per-tile top-k candidates.
-  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 1, 0, o.uamp_damping, st));
-  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 3, o.uamp_damping, o.damping,
-                           o.max_precision, hs, hr, o.blend, ckey, cidx, io->store_uamp ? 1 : 0, st));
+  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 0, o.uamp_damping, tw, st));
+  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 3, o.uamp_damping, o.damping,
+                           o.max_precision, hs, hr, o.blend, ckey, cidx, io->store_uamp ? 1 : 0, tw, st));
   // Polish.
   FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
                               io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
@@ -483,6 +513,7 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   const FbmeWork* tilew = (const FbmeWork*)(w + p->r_tile.off);
   double* sc = (double*)(w + p->r_sc.off);
   double* part = (double*)(w + p->r_part.off);
+  const void* tw = w + p->r_tw.off;
   const FbmePlanOptions& o = p->opt;
   const int ps = p->total_slots;
   const int ncol = (int)p->col_work.size(), nrow = (int)p->row_work.size(), ntile = (int)p->tile_work.size();
@@ -495,14 +526,14 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   float* var = (float*)(w + p->r_var.off);
   switch (stage) {
     case FBME_STAGE_COL_ADJOINT:
-      return fbme_launch_col_adjoint(scenes, colw, ncol, ncol, io->y, pred, msg, mid, sc, part, ps, 1, 1,
-                                     o.uamp_damping, st) ? FBME_ERR_HIP : 0;
+      return fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 1,
+                                     o.uamp_damping, tw, st) ? FBME_ERR_HIP : 0;
     case FBME_STAGE_ROW:
-      return fbme_launch_row(scenes, roww, nrow, nrow, mid, bp, gamma, mean, var, sc, part, ps, 2, o.uamp_damping,
-                             o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr, nullptr, 0,
+      return fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 2, o.uamp_damping,
+                             o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr, nullptr, 0, tw,
                              st) ? FBME_ERR_HIP : 0;
     case FBME_STAGE_COL_FORWARD:
-      return fbme_launch_col_forward(scenes, colw, ncol, ncol, mid, io->y, pred, msg, sc, part, ps, 0, o.uamp_damping,
+      return fbme_launch_col_forward(scenes, colw, ncol, ncol, p->ept_col, mid, io->y, pred, msg, sc, part, ps, 0, o.uamp_damping, tw,
                                      st) ? FBME_ERR_HIP : 0;
     case FBME_STAGE_OUTPUT:
       return fbme_launch_output(scenes, tilew, ntile, ntile, sc, w + p->r_result.off, io->x, io->posterior_variance,

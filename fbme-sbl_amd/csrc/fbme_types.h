@@ -25,6 +25,7 @@ struct FbmeScene {
   int xtile_items;          // exchange-scoring / output tiles
   int xcand_off;            // first exchange candidate slot
   int orig;                 // caller's scene index (per-scene outputs)
+  int tw1_off, tw2_off;     // twiddle tables (float2) of lengths n1 and n2
   int pad;
 };
 

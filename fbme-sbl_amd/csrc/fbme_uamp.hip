@@ -41,112 +41,165 @@ __device__ __forceinline__ double part_get(const double* part, int pstride, int 
 }
 
 // ------------------------------------------------------------- column adjoint
-// mode 0: input is the data y (backprojection); partial sum |y|^2.
-// mode 1: input is the UAMP message update of (y, pred, msg).
-template <int EPT>
+// MODE 0: input is the data y (backprojection); partial sum |y|^2.
+// MODE 1: input is the UAMP message update of (y, pred, msg).
+// All global loads of the tile are issued before the first LDS store.
+template <int EPT, int MODE>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
               const float2* __restrict__ y, const float2* __restrict__ pred, float2* __restrict__ msg,
               float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
-              int pstride, int mode, int store_msg, float damping) {
-  __shared__ float2 buf[FBME_TILE_MAX + 64];
+              int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
+  __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
   __shared__ float2 tw[1024];
   __shared__ double red[8];
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const FbmeWork wk = work[w];
     const FbmeScene s = scenes[wk.scene];
     const int len = s.n1, cols = wk.count, stride = len + 1;
-    build_twiddles(tw, len);
+    const int nin = s.m1 * cols;     // nonzero inputs of the tile
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float noise = (float)scs[SC_NOISE];
     const float precision = 1.0f / (dv + noise);
     const float keep = damping, update = 1.0f - damping;
-    double acc = 0.0;
-    for (int e = threadIdx.x; e < len * cols; e += FBME_BLOCK) {
-      const int i = e / cols, c = e - i * cols;
-      float2 v = make_float2(0.f, 0.f);
-      if (i < s.m1) {
+    float2 yv[EPT], pv[EPT], mv[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < nin) {
+        const int i = e / cols, c = e - i * cols;
         const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
-        const float2 yy = y[gi];
-        if (mode == 0) {
-          v = yy;
-          acc += (double)yy.x * yy.x + (double)yy.y * yy.y;
-        } else {
-          const float2 po = pred[gi], mo = msg[gi];
-          const float2 pseudo = make_float2(po.x - dv * mo.x, po.y - dv * mo.y);
-          const float2 cand = make_float2(precision * (yy.x - pseudo.x), precision * (yy.y - pseudo.y));
-          v = make_float2(keep * mo.x + update * cand.x, keep * mo.y + update * cand.y);
-          if (store_msg) msg[gi] = v;
+        yv[q] = y[gi];
+        if (MODE == 1) {
+          pv[q] = pred[gi];
+          mv[q] = msg[gi];
         }
       }
-      buf[c * stride + i] = v;
     }
-    if (mode == 0) {
+    load_twiddles(tw, twtab + s.tw1_off, len);
+    double acc = 0.0;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < nin) {
+        const int i = e / cols, c = e - i * cols;
+        float2 v;
+        if (MODE == 0) {
+          v = yv[q];
+          acc += (double)v.x * v.x + (double)v.y * v.y;
+        } else {
+          const float2 pseudo = make_float2(pv[q].x - dv * mv[q].x, pv[q].y - dv * mv[q].y);
+          const float2 cand = make_float2(precision * (yv[q].x - pseudo.x), precision * (yv[q].y - pseudo.y));
+          v = make_float2(keep * mv[q].x + update * cand.x, keep * mv[q].y + update * cand.y);
+          if (store_msg) msg[s.samp_off + (long long)i * s.m2 + wk.first + c] = v;
+        }
+        buf[lds_idx(c * stride + i)] = v;
+      }
+    }
+    // zero padding rows M1..N1-1
+    for (int e = nin + threadIdx.x; e < len * cols; e += FBME_BLOCK) {
+      const int i = e / cols, c = e - i * cols;
+      buf[lds_idx(c * stride + i)] = make_float2(0.f, 0.f);
+    }
+    if (MODE == 0) {
       acc = block_sum(acc, red);
       if (threadIdx.x == 0) part[(long long)PT_YE * pstride + s.col_part + wk.slot] = acc;
     }
     __syncthreads();
     fft_lines<EPT, true>(buf, len, cols, stride, s.rad1, tw);
-    for (int e = threadIdx.x; e < len * cols; e += FBME_BLOCK) {
-      const int i = e / cols, c = e - i * cols;
-      mid[s.mid_off + (long long)i * s.m2 + wk.first + c] = buf[c * stride + i];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < len * cols) {
+        const int i = e / cols, c = e - i * cols;
+        mid[s.mid_off + (long long)i * s.m2 + wk.first + c] = buf[lds_idx(c * stride + i)];
+      }
     }
     __syncthreads();
   }
 }
 
 // -------------------------------------------------------------------- row pass
-// mode 0: backprojection rows (A^H y), partial sum |A^H y|^2.
-// mode 1: first UAMP step: lifted = bp * (1-d)/(dv+sigma^2), gamma0 built from
+// MODE 0: backprojection rows (A^H y), partial sum |A^H y|^2.
+// MODE 1: first UAMP step: lifted = bp * (1-d)/(dv+sigma^2), gamma0 built from
 //         the backprojection (solver.py:1063-1099), mean_old = 0, v_old = gamma0.
-// mode 2: UAMP step: lifted = row pass of A^H of the column output.
-// mode 3: final E-step: no ARD update, no prediction; writes the polish score
+// MODE 2: UAMP step: lifted = row pass of A^H of the column output.
+// MODE 3: final E-step: no ARD update, no prediction; writes the polish score
 //         |mu|^2 + v as the top-k candidates of this work item.
-template <int EPT>
+template <int EPT, int MODE>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
       float2* __restrict__ mid, float2* __restrict__ bp, float* __restrict__ gamma,
       float2* __restrict__ mean, float* __restrict__ var, const double* __restrict__ sc,
-      double* __restrict__ part, int pstride, int mode, UampArgs a, uint32_t* __restrict__ cand_key,
-      int* __restrict__ cand_idx, int store_final) {
-  __shared__ float2 buf[FBME_TILE_MAX + 64];
+      double* __restrict__ part, int pstride, UampArgs a, uint32_t* __restrict__ cand_key,
+      int* __restrict__ cand_idx, int store_final, const float2* __restrict__ twtab) {
+  constexpr int TILE = EPT * FBME_BLOCK;
+  __shared__ float2 buf[FBME_LDS_SLOTS(TILE)];
   __shared__ float2 tw[1024];
   __shared__ double red[8];
-  __shared__ uint32_t keys[FBME_TILE_MAX];
-  __shared__ uint32_t hist[256];
+  __shared__ uint32_t keys[MODE == 3 ? TILE : 1];
+  __shared__ uint32_t hist[MODE == 3 ? 256 : 1];
   __shared__ int pos[FBME_KMAX];
   __shared__ int misc[4];
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const FbmeWork wk = work[w];
     const FbmeScene s = scenes[wk.scene];
     const int len = s.n2, rows = wk.count, stride = len + 1;
+    const int nel = rows * len;
     const long long n_grid = (long long)s.n1 * s.n2;
     const int n_samp = s.m1 * s.m2;
-    if (mode != 1) build_twiddles(tw, len);
+    const long long gbase = s.grid_off + (long long)wk.first * len;
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float noise = (float)scs[SC_NOISE];
     const float floor_v = (float)scs[SC_FLOOR];
     const float power = (float)scs[SC_POWER];
-    // Row pass of A^H: zero-padded inverse FFT of the M2 leading entries.
-    if (mode != 1) {
-      for (int e = threadIdx.x; e < rows * len; e += FBME_BLOCK) {
-        const int r = e / len, j = e - r * len;
-        float2 v = make_float2(0.f, 0.f);
-        if (j < s.m2) v = mid[s.mid_off + (long long)(wk.first + r) * s.m2 + j];
-        buf[r * stride + j] = v;
+    // Prefetch the per-element state of this tile.
+    float gv[EPT], vv[EPT];
+    float2 mv[EPT];
+    if (MODE >= 1) {
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * FBME_BLOCK;
+        if (e < nel) {
+          if (MODE == 1) {
+            mv[q] = bp[gbase + e];
+          } else {
+            gv[q] = gamma[gbase + e];
+            mv[q] = mean[gbase + e];
+            vv[q] = var[gbase + e];
+          }
+        }
       }
-      __syncthreads();
-      fft_lines<EPT, true>(buf, len, rows, stride, s.rad2, tw);
     }
-    if (mode == 0) {
+    if (MODE != 1) {
+      // Row pass of A^H: zero-padded inverse FFT of the M2 leading entries.
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * FBME_BLOCK;
+        if (e < nel) {
+          const int r = e / len, j = e - r * len;
+          float2 v = make_float2(0.f, 0.f);
+          if (j < s.m2) v = mid[s.mid_off + (long long)(wk.first + r) * s.m2 + j];
+          buf[lds_idx(r * stride + j)] = v;
+        }
+      }
+    }
+    load_twiddles(tw, twtab + s.tw2_off, len);
+    __syncthreads();
+    if (MODE != 1) fft_lines<EPT, true>(buf, len, rows, stride, s.rad2, tw);
+    if (MODE == 0) {
       double acc = 0.0;
-      for (int e = threadIdx.x; e < rows * len; e += FBME_BLOCK) {
-        const int r = e / len, j = e - r * len;
-        const float2 v = buf[r * stride + j];
-        bp[s.grid_off + (long long)(wk.first + r) * len + j] = v;
-        acc += (double)v.x * v.x + (double)v.y * v.y;
+#pragma unroll
+      for (int q = 0; q < EPT; ++q) {
+        const int e = threadIdx.x + q * FBME_BLOCK;
+        if (e < nel) {
+          const int r = e / len, j = e - r * len;
+          const float2 v = buf[lds_idx(r * stride + j)];
+          bp[gbase + e] = v;
+          acc += (double)v.x * v.x + (double)v.y * v.y;
+        }
       }
       acc = block_sum(acc, red);
       if (threadIdx.x == 0) part[(long long)PT_BPE * pstride + s.row_part + wk.slot] = acc;
@@ -155,71 +208,71 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
     }
     const float tau = fmaxf((dv + noise) / (float)n_samp, F32_TINY);
     const float keep = a.damping, update = 1.0f - a.damping;
-    // First step quantities (solver.py:1063-1099, 2775-2782).
     const float first_scale = update / (dv + noise);
     const float bpe_clamped = fmaxf((float)scs[SC_BPE], F32_EPS);
     const float uniform = power / (float)n_grid;
     double dof_acc = 0.0, vs_acc = 0.0;
-    for (int e = threadIdx.x; e < rows * len; e += FBME_BLOCK) {
-      const int r = e / len, j = e - r * len;
-      const long long gi = s.grid_off + (long long)(wk.first + r) * len + j;
-      float2 lifted, mean_old;
-      float prior, v_old;
-      if (mode == 1) {
-        const float2 b = bp[gi];
-        lifted = cscale(b, first_scale);
-        const float energy = b.x * b.x + b.y * b.y;
-        const float structured = energy / bpe_clamped * power;
-        prior = fmaxf(a.blend * structured + (1.0f - a.blend) * uniform, floor_v);
-        mean_old = make_float2(0.f, 0.f);
-        v_old = fmaxf(prior, F32_TINY);
-      } else {
-        lifted = buf[r * stride + j];
-        prior = gamma[gi];
-        mean_old = mean[gi];
-        v_old = fmaxf(var[gi], F32_TINY);
-      }
-      // Denoise (triton_kernels.py:131-158).
-      const float denom = fmaxf(prior + tau, F32_TINY);
-      const float gain = prior / denom;
-      const float2 pm = make_float2(mean_old.x + tau * lifted.x, mean_old.y + tau * lifted.y);
-      const float2 cm = make_float2(gain * pm.x, gain * pm.y);
-      const float cvar = prior * tau / denom;
-      const float2 mu = make_float2(keep * mean_old.x + update * cm.x, keep * mean_old.y + update * cm.y);
-      const float v = fmaxf(keep * v_old + update * cvar, F32_TINY);
-      float rel = 1.0f - v / fmaxf(prior, F32_TINY);
-      rel = fmaxf(0.0f, fminf(1.0f, rel));
-      dof_acc += (double)rel;
-      vs_acc += (double)v;
-      if (mode == 3) {
-        const float score = mu.x * mu.x + mu.y * mu.y + v;
-        keys[e] = score_key(score);
-        if (store_final) {
-          mean[gi] = mu;
-          var[gi] = v;
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < nel) {
+        const int r = e / len, j = e - r * len;
+        float2 lifted, mean_old;
+        float prior, v_old;
+        if (MODE == 1) {
+          const float2 b = mv[q];
+          lifted = cscale(b, first_scale);
+          const float energy = b.x * b.x + b.y * b.y;
+          const float structured = energy / bpe_clamped * power;
+          prior = fmaxf(a.blend * structured + (1.0f - a.blend) * uniform, floor_v);
+          mean_old = make_float2(0.f, 0.f);
+          v_old = fmaxf(prior, F32_TINY);
+        } else {
+          lifted = buf[lds_idx(r * stride + j)];
+          prior = gv[q];
+          mean_old = mv[q];
+          v_old = fmaxf(vv[q], F32_TINY);
         }
-        continue;
+        // Denoise (triton_kernels.py:131-158).
+        const float denom = fmaxf(prior + tau, F32_TINY);
+        const float gain = prior / denom;
+        const float2 pm = make_float2(mean_old.x + tau * lifted.x, mean_old.y + tau * lifted.y);
+        const float2 cm = make_float2(gain * pm.x, gain * pm.y);
+        const float cvar = prior * tau / denom;
+        const float2 mu = make_float2(keep * mean_old.x + update * cm.x, keep * mean_old.y + update * cm.y);
+        const float v = fmaxf(keep * v_old + update * cvar, F32_TINY);
+        float rel = 1.0f - v / fmaxf(prior, F32_TINY);
+        rel = fmaxf(0.0f, fminf(1.0f, rel));
+        dof_acc += (double)rel;
+        vs_acc += (double)v;
+        if (MODE == 3) {
+          keys[e] = score_key(mu.x * mu.x + mu.y * mu.y + v);
+          if (store_final) {
+            mean[gbase + e] = mu;
+            var[gbase + e] = v;
+          }
+        } else {
+          // Fused ARD update (triton_kernels.py:159-180).
+          const float second = mu.x * mu.x + mu.y * mu.y + v;
+          float cp = (second + a.hyper_rate) / (1.0f + a.hyper_shape);
+          cp = fmaxf(floor_v, fminf(a.max_precision, cp));
+          const float nxt = a.prior_damping * prior + (1.0f - a.prior_damping) * cp;
+          gamma[gbase + e] = fmaxf(nxt, floor_v);
+          mean[gbase + e] = mu;
+          var[gbase + e] = v;
+          buf[lds_idx(r * stride + j)] = mu;
+        }
       }
-      // Fused ARD update (triton_kernels.py:159-180).
-      const float second = mu.x * mu.x + mu.y * mu.y + v;
-      float cp = (second + a.hyper_rate) / (1.0f + a.hyper_shape);
-      cp = fmaxf(floor_v, fminf(a.max_precision, cp));
-      const float nxt = a.prior_damping * prior + (1.0f - a.prior_damping) * cp;
-      gamma[gi] = fmaxf(nxt, floor_v);
-      mean[gi] = mu;
-      var[gi] = v;
-      buf[r * stride + j] = mu;
     }
     block_sum2(dof_acc, vs_acc, red);
     if (threadIdx.x == 0) {
       part[(long long)PT_DOF * pstride + s.row_part + wk.slot] = dof_acc;
       part[(long long)PT_VS * pstride + s.row_part + wk.slot] = vs_acc;
     }
-    if (mode == 3) {
+    if (MODE == 3) {
       // Top-k candidates of this tile for the polish selection (solver.py:1926-1933).
       __syncthreads();
-      const int n_el = rows * len;
-      const int got = block_select_topk(keys, n_el, s.k, pos, hist, misc);
+      const int got = block_select_topk(keys, nel, s.k, pos, hist, misc);
       const long long cbase = (long long)s.cand_off + (long long)wk.slot * s.k;
       for (int i = threadIdx.x; i < s.k; i += FBME_BLOCK) {
         if (i < got) {
@@ -233,16 +286,15 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
       __syncthreads();
       continue;
     }
-    if (mode == 1) {
-      // The first step skipped the inverse row FFT, so the twiddles are built here.
-      build_twiddles(tw, len);
-    }
-    __syncthreads();
     // Row pass of A mu keeping the M2 leading outputs.
     fft_lines<EPT, false>(buf, len, rows, stride, s.rad2, tw);
-    for (int e = threadIdx.x; e < rows * s.m2; e += FBME_BLOCK) {
-      const int r = e / s.m2, j = e - r * s.m2;
-      mid[s.mid_off + (long long)(wk.first + r) * s.m2 + j] = buf[r * stride + j];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < rows * s.m2) {
+        const int r = e / s.m2, j = e - r * s.m2;
+        mid[s.mid_off + (long long)(wk.first + r) * s.m2 + j] = buf[lds_idx(r * stride + j)];
+      }
     }
     __syncthreads();
   }
@@ -254,38 +306,51 @@ __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
               const float2* __restrict__ mid, const float2* __restrict__ y, float2* __restrict__ pred,
               float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
-              int pstride, int first_message, float damping) {
-  __shared__ float2 buf[FBME_TILE_MAX + 64];
+              int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
+  __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
   __shared__ float2 tw[1024];
   __shared__ double red[8];
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const FbmeWork wk = work[w];
     const FbmeScene s = scenes[wk.scene];
     const int len = s.n1, cols = wk.count, stride = len + 1;
-    build_twiddles(tw, len);
-    for (int e = threadIdx.x; e < len * cols; e += FBME_BLOCK) {
-      const int i = e / cols, c = e - i * cols;
-      buf[c * stride + i] = mid[s.mid_off + (long long)i * s.m2 + wk.first + c];
+    const int nout = s.m1 * cols;
+    float2 yv[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < len * cols) {
+        const int i = e / cols, c = e - i * cols;
+        buf[lds_idx(c * stride + i)] = mid[s.mid_off + (long long)i * s.m2 + wk.first + c];
+      }
+      if (e < nout) {
+        const int i = e / cols, c = e - i * cols;
+        yv[q] = y[s.samp_off + (long long)i * s.m2 + wk.first + c];
+      }
     }
+    load_twiddles(tw, twtab + s.tw1_off, len);
     __syncthreads();
     fft_lines<EPT, false>(buf, len, cols, stride, s.rad1, tw);
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float precision = 1.0f / (dv + (float)scs[SC_NOISE]);
     double acc = 0.0;
-    for (int e = threadIdx.x; e < s.m1 * cols; e += FBME_BLOCK) {
-      const int i = e / cols, c = e - i * cols;
-      const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
-      const float2 p = buf[c * stride + i];
-      const float2 yy = y[gi];
-      pred[gi] = p;
-      const float rx = yy.x - p.x, ry = yy.y - p.y;
-      acc += (double)rx * rx + (double)ry * ry;
-      if (first_message) {
-        // First message from zero state (triton_kernels.py:66-77 with
-        // prediction = message = 0).
-        msg[gi] = make_float2((1.0f - damping) * (precision * yy.x),
-                              (1.0f - damping) * (precision * yy.y));
+#pragma unroll
+    for (int q = 0; q < EPT; ++q) {
+      const int e = threadIdx.x + q * FBME_BLOCK;
+      if (e < nout) {
+        const int i = e / cols, c = e - i * cols;
+        const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
+        const float2 p = buf[lds_idx(c * stride + i)];
+        pred[gi] = p;
+        const float rx = yv[q].x - p.x, ry = yv[q].y - p.y;
+        acc += (double)rx * rx + (double)ry * ry;
+        if (first_message) {
+          // First message from zero state (triton_kernels.py:66-77 with
+          // prediction = message = 0).
+          msg[gi] = make_float2((1.0f - damping) * (precision * yv[q].x),
+                                (1.0f - damping) * (precision * yv[q].y));
+        }
       }
     }
     acc = block_sum(acc, red);
@@ -370,40 +435,66 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
 }  // namespace
 
 // ----------------------------------------------------------- host launchers
+#define FBME_EPT_DISPATCH(ept, ...)            \
+  switch (ept) {                                \
+    case 4: { constexpr int E = 4; __VA_ARGS__; break; }   \
+    case 8: { constexpr int E = 8; __VA_ARGS__; break; }   \
+    default: { constexpr int E = 16; __VA_ARGS__; break; } \
+  }
+
 extern "C" {
 
-int fbme_launch_col_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid,
+int fbme_launch_col_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
                             const void* y, const void* pred, void* msg, void* mid, const double* sc,
                             double* part, int pstride, int mode, int store_msg, float damping,
-                            hipStream_t stream) {
+                            const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
-  hipLaunchKernelGGL(k_col_adjoint<16>, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                     (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
-                     mode, store_msg, damping);
+  FBME_EPT_DISPATCH(ept, {
+    if (mode == 0)
+      hipLaunchKernelGGL((k_col_adjoint<E, 0>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                         (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                         store_msg, damping, (const float2*)twtab);
+    else
+      hipLaunchKernelGGL((k_col_adjoint<E, 1>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                         (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                         store_msg, damping, (const float2*)twtab);
+  })
   return (int)hipGetLastError();
 }
 
-int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, void* mid, void* bp,
+int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept, void* mid, void* bp,
                     float* gamma, void* mean, float* var, const double* sc, double* part, int pstride,
                     int mode, float damping, float prior_damping, float max_precision, float hyper_shape,
                     float hyper_rate, float blend, uint32_t* cand_key, int* cand_idx, int store_final,
-                    hipStream_t stream) {
+                    const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   UampArgs a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
-  hipLaunchKernelGGL(k_row<16>, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork, (float2*)mid,
-                     (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, mode, a, cand_key, cand_idx,
-                     store_final);
+#define FBME_ROW(M)                                                                                      \
+  hipLaunchKernelGGL((k_row<E, M>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,       \
+                     (float2*)mid, (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, cand_key, \
+                     cand_idx, store_final, (const float2*)twtab)
+  FBME_EPT_DISPATCH(ept, {
+    switch (mode) {
+      case 0: FBME_ROW(0); break;
+      case 1: FBME_ROW(1); break;
+      case 2: FBME_ROW(2); break;
+      default: FBME_ROW(3); break;
+    }
+  })
+#undef FBME_ROW
   return (int)hipGetLastError();
 }
 
-int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid,
+int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
                             const void* mid, const void* y, void* pred, void* msg, const double* sc,
                             double* part, int pstride, int first_message, float damping,
-                            hipStream_t stream) {
+                            const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
-  hipLaunchKernelGGL(k_col_forward<16>, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                     (const float2*)mid, (const float2*)y, (float2*)pred, (float2*)msg, sc, part, pstride,
-                     first_message, damping);
+  FBME_EPT_DISPATCH(ept, {
+    hipLaunchKernelGGL((k_col_forward<E>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                       (const float2*)mid, (const float2*)y, (float2*)pred, (float2*)msg, sc, part, pstride,
+                       first_message, damping, (const float2*)twtab);
+  })
   return (int)hipGetLastError();
 }
 
