@@ -257,6 +257,49 @@ __device__ __forceinline__ uint32_t score_key(float s) {
   return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+
+// Packed selection key: score key in the high word, the complement of the
+// position in the low word, so the larger score wins and equal scores go to
+// the lower position (a deterministic tie rule).
+__device__ __forceinline__ unsigned long long pack_key(uint32_t key, int pos) {
+  return ((unsigned long long)key << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)pos);
+}
+__device__ __forceinline__ int packed_pos(unsigned long long p) { return (int)(0xFFFFFFFFu - (uint32_t)p); }
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  return v;
+}
+
+// Top-k of the NPL packed keys held by each lane of one wavefront, by k
+// rounds of wave-wide arg-max (no barriers, no atomics).  Lane 0 of the
+// wave writes the winners in decreasing order to out[0..k).  Entries equal
+// to 0 are padding and never win against a real key.
+template <int NPL>
+__device__ __forceinline__ void wave_topk(unsigned long long (&v)[NPL], int k, unsigned long long* out) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long best = 0ull;
+#pragma unroll
+  for (int i = 0; i < NPL; ++i) best = v[i] > best ? v[i] : best;
+  for (int r = 0; r < k; ++r) {
+    const unsigned long long w = wave_max_u64(best);
+    if (lane == 0) out[r] = w;
+    if (w != 0ull && best == w) {
+      // the unique owner (positions are distinct) retires it
+      best = 0ull;
+#pragma unroll
+      for (int i = 0; i < NPL; ++i) {
+        if (v[i] == w) v[i] = 0ull;
+        best = v[i] > best ? v[i] : best;
+      }
+    }
+  }
+}
+
 // Block-wide selection of the k largest keys among keys[0..n) (in LDS).
 // Writes the chosen positions to out_pos[0..k) in unspecified order (ties on
 // the threshold key are broken towards the lower position) and returns the

@@ -231,6 +231,27 @@ inline int nblk(int n) { return (n + FBME_BLOCK - 1) / FBME_BLOCK; }
 
 extern "C" {
 
+unsigned fbme_radix_code(int len) {
+  if (len < 1 || len > 1024) return 0xFFFFFFFFu;
+  return radix_code_host(len);
+}
+
+int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,
+                    int inverse) {
+  if (batch < 1 || n < 1 || n > 1024 || rows_in > n || rows_out > n || cols < 1) return FBME_ERR_ARG;
+  const unsigned code = radix_code_host(n);
+  if (code == 0xFFFFFFFFu) return FBME_ERR_ARG;
+  const int tc = std::max(1, std::min(cols, std::min(16, OP_TILE / n)));
+  dim3 grid((cols + tc - 1) / tc, batch);
+  if (inverse)
+    hipLaunchKernelGGL(k_op_col<true>, grid, dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)in,
+                       (float2*)out, n, rows_in, rows_out, cols, tc, code);
+  else
+    hipLaunchKernelGGL(k_op_col<false>, grid, dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)in,
+                       (float2*)out, n, rows_in, rows_out, cols, tc, code);
+  return (int)hipGetLastError();
+}
+
 int fbme_prefix_forward(void* stream, int batch, int n1, int n2, int m1, int m2, const void* x, void* y,
                         void* mid) {
   if (batch < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2 || n1 > 1024 || n2 > 1024) return FBME_ERR_ARG;

@@ -1,0 +1,921 @@
+// Covariance-free PCG + Hutchinson path of the general solver (gfx950).
+//
+// Restates, on HIP kernels, the mode="cg" pieces of torch_sbl/solver.py and
+// torch_sbl/operators.py:
+//   * exact BTTB covariance spectrum (operators.py:299-326) and matvec
+//     Q d = crop(IFFT(FFT(pad d) * spectrum)) (operators.py:328-355), with
+//     the 1 + P right-hand sides (data + Rademacher probes) batched into the
+//     leading FFT dimension; the column FFT, the spectrum product and the
+//     inverse column FFT are fused into one pass;
+//   * low-rank Woodbury preconditioner (solver.py:5786-5848): strongest
+//     atoms, exact-phase Fourier factors, Dirichlet-kernel Gram, FP64
+//     Cholesky and inverse of the 32 x 32 system;
+//   * PCG lane arithmetic with masks (solver.py:5582-5723), inner products in
+//     FP64;
+//   * Hutchinson posterior diagonal and moments (solver.py:3016-3036),
+//     relevance projection (3156-3181), MacKay update (1669-1677) and
+//     damping (1776-1798).
+// Vectors of lane (b, r) are contiguous: X[(b * R + r) * M + m].
+#include <algorithm>
+
+#include "fbme_device.h"
+
+using namespace fbme;
+
+namespace {
+
+constexpr double F32_EPS_D = 1.1920928955078125e-07;
+constexpr double F32_TINY_D = 1.17549435082228750797e-38;
+constexpr int RANK = 32;
+
+// ------------------------------------------------------------ BTTB spectrum
+__global__ void k_real_to_complex(const float* __restrict__ in, float2* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(in[i], 0.f);
+}
+
+// kern[b, e1, e2] = lags[b, lag1 mod N1, lag2 mod N2] at the signed-lag
+// positions of the circulant embedding, plus the noise at the origin.
+__global__ void k_embed(const float2* __restrict__ lags, const float* __restrict__ noise,
+                        float2* __restrict__ kern, int n1, int n2, int m1, int m2, int e1, int e2) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= e1 * e2) return;
+  const int a1 = i / e2, a2 = i - a1 * e2;
+  int l1 = -1, l2 = -1;
+  if (a1 < m1) l1 = a1;
+  else if (a1 > e1 - m1) l1 = a1 - e1 + n1;       // negative lag, mod n1
+  if (a2 < m2) l2 = a2;
+  else if (a2 > e2 - m2) l2 = a2 - e2 + n2;
+  float2 v = make_float2(0.f, 0.f);
+  if (l1 >= 0 && l2 >= 0) v = lags[((long long)b * n1 + l1) * n2 + l2];
+  if (i == 0) v.x += noise[b];
+  kern[(long long)b * e1 * e2 + i] = v;
+}
+
+// Fused column pass of the Toeplitz matvec on [BR, M1, E2]: for each column,
+// zero-pad M1 -> E1, forward FFT, multiply by spectrum[b][:, c] / (E1 E2),
+// inverse FFT, keep the first M1 rows.
+template <int EPT>
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_toeplitz_col(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ spec,
+               int rhs, int m1, int e1, int e2, int tile_cols, unsigned code, const float2* __restrict__ twtab) {
+  __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
+  __shared__ float2 tw[1024];
+  const int lanev = blockIdx.y;          // b * rhs + r
+  const int b = lanev / rhs;
+  const int c0 = blockIdx.x * tile_cols;
+  const int cc = min(tile_cols, e2 - c0);
+  const int stride = e1 + 1;
+  load_twiddles(tw, twtab, e1);
+  const float2* src = in + (long long)lanev * m1 * e2;
+  const float2* sp = spec + (long long)b * e1 * e2;
+  const float scale = 1.0f / ((float)e1 * (float)e2);
+  for (int e = threadIdx.x; e < e1 * cc; e += FBME_BLOCK) {
+    const int i = e / cc, c = e - i * cc;
+    buf[lds_idx(c * stride + i)] = (i < m1) ? src[(long long)i * e2 + c0 + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<EPT, false>(buf, e1, cc, stride, code, tw);
+  for (int e = threadIdx.x; e < e1 * cc; e += FBME_BLOCK) {
+    const int i = e / cc, c = e - i * cc;
+    const int k = lds_idx(c * stride + i);
+    buf[k] = cscale(cmul(buf[k], sp[(long long)i * e2 + c0 + c]), scale);
+  }
+  __syncthreads();
+  fft_lines<EPT, true>(buf, e1, cc, stride, code, tw);
+  float2* dst = out + (long long)lanev * m1 * e2;
+  for (int e = threadIdx.x; e < m1 * cc; e += FBME_BLOCK) {
+    const int i = e / cc, c = e - i * cc;
+    dst[(long long)i * e2 + c0 + c] = buf[lds_idx(c * stride + i)];
+  }
+}
+
+// Row pass on [rows, len_in] -> [rows, len_out] with an FFT of length n,
+// twiddles from a table (plan-owned).
+template <int EPT, bool INV>
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_rowpass(const float2* __restrict__ in, float2* __restrict__ out, int n, int len_in, int len_out,
+          long long rows, int tile_rows, unsigned code, const float2* __restrict__ twtab) {
+  __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
+  __shared__ float2 tw[1024];
+  const long long r0 = (long long)blockIdx.x * tile_rows;
+  const int rr = (int)min((long long)tile_rows, rows - r0);
+  const int stride = n + 1;
+  load_twiddles(tw, twtab, n);
+  const float2* src = in + r0 * len_in;
+  for (int e = threadIdx.x; e < rr * n; e += FBME_BLOCK) {
+    const int r = e / n, j = e - r * n;
+    buf[lds_idx(r * stride + j)] = (j < len_in) ? src[(long long)r * len_in + j] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lines<EPT, INV>(buf, n, rr, stride, code, tw);
+  float2* dst = out + r0 * len_out;
+  for (int e = threadIdx.x; e < rr * len_out; e += FBME_BLOCK) {
+    const int r = e / len_out, j = e - r * len_out;
+    dst[(long long)r * len_out + j] = buf[lds_idx(r * stride + j)];
+  }
+}
+
+// -------------------------------------------------------------- tile top-k
+// Top RANK entries of each 1024-value tile of x[b, :] (by value, then lower
+// index), as packed keys.
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_topk_tile(const float* __restrict__ x, long long n, int tiles, unsigned long long* __restrict__ cand) {
+  __shared__ unsigned long long wtop[4][RANK];
+  const int b = blockIdx.y;
+  const int t = blockIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned long long v[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const long long e = (long long)t * 1024 + wave * 256 + lane + 64 * i;
+    v[i] = (e < n) ? pack_key(score_key(x[(long long)b * n + e]), (int)e) : 0ull;
+  }
+  wave_topk<4>(v, RANK, wtop[wave]);
+  __syncthreads();
+  if (wave == 0) {
+    unsigned long long u[2];
+    u[0] = wtop[lane >> 5][lane & 31];
+    u[1] = wtop[2 + (lane >> 5)][lane & 31];
+    wave_topk<2>(u, RANK, cand + ((long long)b * tiles + t) * RANK);
+  }
+}
+
+// Merge the tile candidates of each scene into the sorted top RANK
+// (values and indices), one wavefront per scene (tiles * RANK <= 2048).
+__global__ void __launch_bounds__(64)
+k_topk_merge(const unsigned long long* __restrict__ cand, int tiles, const float* __restrict__ x, long long n,
+             int* __restrict__ idx_out, float* __restrict__ val_out) {
+  __shared__ unsigned long long top[RANK];
+  const int b = blockIdx.x;
+  unsigned long long v[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int slot = threadIdx.x + 64 * i;
+    v[i] = (slot < tiles * RANK) ? cand[(long long)b * tiles * RANK + slot] : 0ull;
+  }
+  wave_topk<32>(v, RANK, top);
+  __syncthreads();
+  if (threadIdx.x < RANK) {
+    const int e = packed_pos(top[threadIdx.x]);
+    idx_out[b * RANK + threadIdx.x] = e;
+    val_out[b * RANK + threadIdx.x] = x[(long long)b * n + e];
+  }
+}
+
+// ------------------------------------------------------ low-rank factors
+// F[b, m, k] = exp(-2 pi i (m1 k1 / N1 + m2 k2 / N2)) sqrt(gamma_k), phase
+// reduced modulo N in integers.
+__global__ void k_lr_factors(const int* __restrict__ idx, const float* __restrict__ val, float2* __restrict__ f,
+                             int n1, int n2, int m1, int m2) {
+  const int b = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = m1 * m2;
+  if (i >= (long long)m * RANK) return;
+  const int row = (int)(i / RANK), k = (int)(i - (long long)row * RANK);
+  const int a1 = row / m2, a2 = row - a1 * m2;
+  const int g = idx[b * RANK + k];
+  const int k1 = g / n2, k2 = g - k1 * n2;
+  const long long p1 = ((long long)a1 * k1) % n1, p2 = ((long long)a2 * k2) % n2;
+  const double ph = (double)p1 / n1 + (double)p2 / n2;   // in [0, 2)
+  double s, c;
+  sincospi(-2.0 * ph, &s, &c);
+  const float sq = sqrtf(fmaxf(val[b * RANK + k], 0.f));
+  f[((long long)b * m + row) * RANK + k] = make_float2((float)c * sq, (float)s * sq);
+}
+
+// One wavefront per scene: small = I + F^H F / bg with F^H F from the
+// Dirichlet Gram, GPU Cholesky loading (10 eps mean|diag|,
+// solver.py:5827-5833), inverse S^-1 = L^-H L^-1 (solver.py:5848).
+__global__ void __launch_bounds__(64)
+k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const double* __restrict__ sums,
+           const float* __restrict__ noise, const double2* __restrict__ d1, const double2* __restrict__ d2,
+           int n1, int n2, float2* __restrict__ sinv, float* __restrict__ bg_out, int* __restrict__ status) {
+  __shared__ double2 a[RANK][RANK + 1];
+  __shared__ double2 xs[RANK][RANK + 1];
+  __shared__ double sq[RANK];
+  __shared__ int gi[RANK];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double vsum = 0.0;
+  if (lane < RANK) {
+    gi[lane] = idx[b * RANK + lane];
+    sq[lane] = sqrt(fmax((double)val[b * RANK + lane], 0.0));
+    vsum = (double)val[b * RANK + lane];
+  }
+  vsum = wave_sum(vsum);
+  const double total = sums[b];
+  const double omitted = fmax(total - vsum, 0.0);
+  const double bg = fmax((double)noise[b] + omitted, F32_EPS_D * total);
+  __syncthreads();
+  for (int e = lane; e < RANK * RANK; e += 64) {
+    const int r = e / RANK, q = e - r * RANK;
+    int dd1 = gi[r] / n2 - gi[q] / n2;
+    int dd2 = gi[r] % n2 - gi[q] % n2;
+    dd1 += dd1 < 0 ? n1 : 0;
+    dd2 += dd2 < 0 ? n2 : 0;
+    const double2 u = d1[dd1], w = d2[dd2];
+    const double2 gv = make_double2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x);
+    const double s = sq[r] * sq[q] / bg;
+    a[r][q] = make_double2((r == q ? 1.0 : 0.0) + gv.x * s, gv.y * s);
+  }
+  __syncthreads();
+  double2 row[RANK];
+#pragma unroll
+  for (int q = 0; q < RANK; ++q) {
+    double2 v = make_double2(lane == q ? 1.0 : 0.0, 0.0);
+    if (lane < RANK) {
+      const double2 u = a[lane][q], l = a[q][lane];
+      v = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+    }
+    row[q] = v;
+  }
+  // mean |diag| of the symmetrised system (read from LDS: row[lane] would
+  // need a dynamic register index)
+  double md = (lane < RANK) ? fabs(a[lane][lane].x) : 0.0;
+  md = wave_sum(md) / RANK;
+  const double jit = 10.0 * F32_EPS_D * fmax(md, F32_TINY_D);
+  __syncthreads();
+  int fail = 0;
+#pragma unroll
+  for (int j = 0; j < RANK; ++j) {
+    if (lane == j) row[j].x += jit;
+    const double d = __shfl(row[j].x, j, 64);
+    fail |= !(d > 0.0);
+    const double piv = sqrt(fmax(d, 0.0));
+    if (lane > j) {
+      row[j].x /= piv;
+      row[j].y /= piv;
+    } else if (lane == j) {
+      row[j] = make_double2(piv, 0.0);
+    }
+#pragma unroll
+    for (int q = j + 1; q < RANK; ++q) {
+      const double lx = __shfl(row[j].x, q, 64), ly = __shfl(row[j].y, q, 64);
+      if (lane >= q) {
+        row[q].x -= row[j].x * lx + row[j].y * ly;
+        row[q].y -= row[j].y * lx - row[j].x * ly;
+      }
+    }
+  }
+  if (lane < RANK) {
+#pragma unroll
+    for (int q = 0; q < RANK; ++q) a[lane][q] = row[q];
+  }
+  __syncthreads();
+  double2 x[RANK];
+#pragma unroll
+  for (int r = 0; r < RANK; ++r) {
+    double2 acc = make_double2(r == lane ? 1.0 : 0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < r; ++p) {
+      const double2 l = a[r][p];
+      acc.x -= l.x * x[p].x - l.y * x[p].y;
+      acc.y -= l.x * x[p].y + l.y * x[p].x;
+    }
+    const double dg = a[r][r].x;
+    x[r] = make_double2(acc.x / dg, acc.y / dg);
+  }
+  if (lane < RANK) {
+#pragma unroll
+    for (int r = 0; r < RANK; ++r) xs[r][lane] = x[r];
+  }
+  __syncthreads();
+  // S^-1[i][j] = sum_r conj(X[r][i]) X[r][j]; lane j computes column j.
+  if (lane < RANK) {
+    for (int i = 0; i < RANK; ++i) {
+      double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int r = 0; r < RANK; ++r) {
+        const double2 xi = xs[r][i];
+        acc.x += xi.x * x[r].x + xi.y * x[r].y;
+        acc.y += xi.x * x[r].y - xi.y * x[r].x;
+      }
+      sinv[((long long)b * RANK + i) * RANK + lane] = make_float2((float)acc.x, (float)acc.y);
+    }
+  }
+  if (lane == 0) {
+    bg_out[b] = (float)bg;
+    if (status && fail) status[b] = 1;
+  }
+}
+
+// proj[(b, r), k] partial over a chunk of m: sum conj(F[b, m, k]) v[(b, r), m].
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_lr_project(const float2* __restrict__ f, const float2* __restrict__ v, int m, int rhs, int chunk,
+             double2* __restrict__ part) {
+  __shared__ float2 fs[64][RANK + 1];
+  __shared__ float2 vs[64];
+  __shared__ double2 red[4][RANK];
+  const int lanev = blockIdx.y;
+  const int b = lanev / rhs;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m0 = blockIdx.x * chunk;
+  const int m1 = min(m, m0 + chunk);
+  double2 acc = make_double2(0.0, 0.0);
+  const int k = lane & 31, half = lane >> 5;   // 2 halves x 32 atoms per wave
+  for (int base = m0; base < m1; base += 64) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 64 * RANK; e += FBME_BLOCK) {
+      const int mm = e / RANK, kk = e - mm * RANK;
+      fs[mm][kk] = (base + mm < m1) ? f[((long long)b * m + base + mm) * RANK + kk] : make_float2(0.f, 0.f);
+    }
+    if (threadIdx.x < 64)
+      vs[threadIdx.x] = (base + threadIdx.x < m1) ? v[(long long)lanev * m + base + threadIdx.x]
+                                                  : make_float2(0.f, 0.f);
+    __syncthreads();
+    // 8 groups (4 waves x 2 halves) split the 64 samples of the chunk.
+    const int g = wave * 2 + half;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int mm = g * 8 + t;
+      const float2 fv = fs[mm][k], vv = vs[mm];
+      acc.x += (double)fv.x * vv.x + (double)fv.y * vv.y;
+      acc.y += (double)fv.x * vv.y - (double)fv.y * vv.x;
+    }
+  }
+  // reduce the 8 groups: halves within a wave by shuffle, waves through LDS
+  acc.x += __shfl_down(acc.x, 32, 64);
+  acc.y += __shfl_down(acc.y, 32, 64);
+  __syncthreads();
+  if (half == 0) red[wave][k] = acc;
+  __syncthreads();
+  if (threadIdx.x < RANK) {
+    double2 s = red[0][threadIdx.x];
+    for (int w = 1; w < 4; ++w) {
+      s.x += red[w][threadIdx.x].x;
+      s.y += red[w][threadIdx.x].y;
+    }
+    part[((long long)lanev * gridDim.x + blockIdx.x) * RANK + threadIdx.x] = s;
+  }
+}
+
+// corr[(b, r), :] = S^-1 (proj / bg) / bg, from the chunk partials.
+__global__ void __launch_bounds__(64)
+k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict__ sinv,
+          const float* __restrict__ bg, int rhs, float2* __restrict__ corr) {
+  __shared__ double2 proj[RANK];
+  const int lanev = blockIdx.x;
+  const int b = lanev / rhs;
+  const int k = threadIdx.x;
+  if (k < RANK) {
+    double2 s = make_double2(0.0, 0.0);
+    for (int c = 0; c < chunks; ++c) {
+      const double2 v = part[((long long)lanev * chunks + c) * RANK + k];
+      s.x += v.x;
+      s.y += v.y;
+    }
+    proj[k] = s;
+  }
+  __syncthreads();
+  if (k < RANK) {
+    const double ib = 1.0 / (double)bg[b];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < RANK; ++j) {
+      const float2 w = sinv[((long long)b * RANK + k) * RANK + j];
+      acc.x += (double)w.x * proj[j].x - (double)w.y * proj[j].y;
+      acc.y += (double)w.x * proj[j].y + (double)w.y * proj[j].x;
+    }
+    corr[(long long)lanev * RANK + k] = make_float2((float)(acc.x * ib * ib), (float)(acc.y * ib * ib));
+  }
+}
+
+// out = v / bg - F corr  (corr already carries the 1 / bg^2 factor).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_lr_apply(const float2* __restrict__ f, const float2* __restrict__ v, const float2* __restrict__ corr,
+           const float* __restrict__ bg, int m, int rhs, float2* __restrict__ out) {
+  __shared__ float2 cs[RANK];
+  const int lanev = blockIdx.y;
+  const int b = lanev / rhs;
+  if (threadIdx.x < RANK) cs[threadIdx.x] = corr[(long long)lanev * RANK + threadIdx.x];
+  __syncthreads();
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  if (i >= m) return;
+  const float ib = 1.0f / bg[b];
+  const float2* fr = f + ((long long)b * m + i) * RANK;
+  float ax = 0.f, ay = 0.f;
+#pragma unroll 8
+  for (int k = 0; k < RANK; ++k) {
+    const float2 fv = fr[k], c = cs[k];
+    ax += fv.x * c.x - fv.y * c.y;
+    ay += fv.x * c.y + fv.y * c.x;
+  }
+  const float2 vv = v[(long long)lanev * m + i];
+  out[(long long)lanev * m + i] = make_float2(vv.x * ib - ax, vv.y * ib - ay);
+}
+
+// ------------------------------------------------------------- lane algebra
+// part[lane, blk] = sum Re(conj(a) b) over one block of samples (FP64).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_lane_dot(const float2* __restrict__ a, const float2* __restrict__ b, int m, double* __restrict__ part) {
+  __shared__ double red[8];
+  const int lanev = blockIdx.y;
+  double acc = 0.0;
+  for (int i = blockIdx.x * FBME_BLOCK + threadIdx.x; i < m; i += gridDim.x * FBME_BLOCK) {
+    const float2 u = a[(long long)lanev * m + i], w = b[(long long)lanev * m + i];
+    acc += (double)u.x * w.x + (double)u.y * w.y;
+  }
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long long)lanev * gridDim.x + blockIdx.x] = acc;
+}
+
+__global__ void k_lane_finish(const double* __restrict__ part, int blocks, int lanes, double* __restrict__ out) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lanes) return;
+  double s = 0.0;
+  for (int i = 0; i < blocks; ++i) s += part[(long long)l * blocks + i];
+  out[l] = s;
+}
+
+// x += alpha d, r -= alpha q with alpha from (rho, curv, active)
+// (solver.py:5656-5662).
+__global__ void k_pcg_update(float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ d,
+                             const float2* __restrict__ q, const double* __restrict__ rho,
+                             const double* __restrict__ curv, int* __restrict__ active, int m) {
+  const int lanev = blockIdx.y;
+  const double cv = curv[lanev], rh = rho[lanev];
+  const bool valid = active[lanev] && isfinite(cv) && isfinite(rh) && cv > F32_TINY_D && rh > F32_TINY_D;
+  const float alpha = valid ? (float)(rh / fmax(cv, F32_TINY_D)) : 0.f;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && alpha != 0.f) {
+    const long long g = (long long)lanev * m + i;
+    const float2 dv = d[g], qv = q[g];
+    float2 xv = x[g], rv = r[g];
+    xv.x += alpha * dv.x;
+    xv.y += alpha * dv.y;
+    rv.x -= alpha * qv.x;
+    rv.y -= alpha * qv.y;
+    x[g] = xv;
+    r[g] = rv;
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    // the flag is read above by every block; a separate pass commits it
+  }
+}
+
+__global__ void k_pcg_commit_valid(const double* __restrict__ rho, const double* __restrict__ curv,
+                                   int* __restrict__ active, int lanes) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lanes) return;
+  const double cv = curv[l], rh = rho[l];
+  active[l] = active[l] && isfinite(cv) && isfinite(rh) && cv > F32_TINY_D && rh > F32_TINY_D;
+}
+
+// d = z + beta d (masked), rho and active updated (solver.py:5709-5719).
+__global__ void k_pcg_direction(float2* __restrict__ d, const float2* __restrict__ z,
+                                const double* __restrict__ rho, const double* __restrict__ nrho,
+                                const int* __restrict__ active, int m, int restart) {
+  const int lanev = blockIdx.y;
+  const double nr = nrho[lanev], rh = rho[lanev];
+  const bool vu = active[lanev] && isfinite(nr) && nr > F32_TINY_D;
+  const float beta = (vu && !restart) ? (float)(nr / fmax(rh, F32_TINY_D)) : 0.f;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const long long g = (long long)lanev * m + i;
+  if (vu) {
+    const float2 zv = z[g];
+    float2 dv = restart ? make_float2(0.f, 0.f) : d[g];
+    d[g] = make_float2(zv.x + beta * dv.x, zv.y + beta * dv.y);
+  } else {
+    d[g] = make_float2(0.f, 0.f);
+  }
+}
+
+__global__ void k_pcg_commit_rho(double* __restrict__ rho, const double* __restrict__ nrho,
+                                 int* __restrict__ active, int lanes) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lanes) return;
+  const double nr = nrho[l];
+  const bool vu = active[l] && isfinite(nr) && nr > F32_TINY_D;
+  if (vu) rho[l] = nr;
+  active[l] = vu;
+}
+
+// r = rhs - Qx (true residual at a check iteration).
+__global__ void k_sub(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
+                      long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
+}
+
+// Adaptive/fixed residual test at a check iteration (solver.py:5668-5690):
+// rel = ||r|| / init; active = [active &] finite(rel) & rel > tol.
+__global__ void k_pcg_check(const double* __restrict__ rr, const double* __restrict__ init, int* __restrict__ active,
+                            double* __restrict__ best_rel, int* __restrict__ improved, float tol, int fixed,
+                            int lanes) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lanes) return;
+  const double rel = sqrt(fmax(rr[l], 0.0)) / fmax(sqrt(fmax(init[l], 0.0)), F32_EPS_D);
+  if (fixed) {
+    const bool better = rel < best_rel[l];
+    improved[l] = better;
+    if (better) best_rel[l] = rel;
+  }
+  const bool ra = isfinite(rel) && rel > (double)tol;
+  active[l] = fixed ? (active[l] && ra) : ra;
+}
+
+// ------------------------------------------------- Hutchinson and EM updates
+// dd = max(mean_p Re(L_p conj(LP_p)), 0); mu = gamma L_0;
+// v = max(gamma - gamma^2 dd, 0); partial sums of clamp(gamma dd, 0, 1).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_hutch(const float2* __restrict__ lifted, const float2* __restrict__ lprobes, const float* __restrict__ gamma,
+        int n, int probes, float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd_out,
+        double* __restrict__ part) {
+  __shared__ double red[8];
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  double rel = 0.0;
+  if (i < n) {
+    const int rhs = probes + 1;
+    double acc = 0.0;
+    for (int p = 0; p < probes; ++p) {
+      const float2 u = lifted[((long long)b * rhs + 1 + p) * n + i];
+      const float2 w = lprobes[((long long)b * probes + p) * n + i];
+      acc += (double)u.x * w.x + (double)u.y * w.y;
+    }
+    const float dd = fmaxf((float)(acc / probes), 0.f);
+    const float g = gamma[(long long)b * n + i];
+    const float2 l0 = lifted[((long long)b * rhs) * n + i];
+    mean[(long long)b * n + i] = make_float2(g * l0.x, g * l0.y);
+    var[(long long)b * n + i] = fmaxf(g - g * g * dd, 0.f);
+    dd_out[(long long)b * n + i] = dd;
+    rel = fminf(fmaxf(g * dd, 0.f), 1.f);
+  }
+  rel = block_sum(rel, red);
+  if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = rel;
+}
+
+// Relevance projection, MacKay update with damping and floor; optional
+// active-set mask keeps gamma off the set (solver.py:1669-1677, 1776-1798,
+// 1700-1712).  Partial sums of the projected relevance (dof).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_cg_em(float* __restrict__ gamma, const float2* __restrict__ mean, const float* __restrict__ dd,
+        const double* __restrict__ rel_sum, const float* __restrict__ floor_, const unsigned char* __restrict__ mask,
+        int n, int m, float hyper_shape, float hyper_rate, float max_prec, float keep, int update,
+        double* __restrict__ part) {
+  __shared__ double red[8];
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  const double raw = rel_sum[b];
+  const double cap = (double)m * (1.0 - 10.0 * F32_EPS_D);
+  const float scale = (float)fmin(1.0, cap / fmax(raw, F32_EPS_D));
+  double dof = 0.0;
+  if (i < n) {
+    const long long g = (long long)b * n + i;
+    const float gm = gamma[g];
+    const float rel = fminf(fmaxf(gm * dd[g], 0.f), 1.f) * scale;
+    dof = rel;
+    const float2 mu = mean[g];
+    float prec = (rel + hyper_shape) / (mu.x * mu.x + mu.y * mu.y + hyper_rate);
+    prec = fminf(fmaxf(prec, 1.0f / max_prec), max_prec);
+    float ng = 1.0f / prec;
+    if (mask && !mask[g]) ng = gm;
+    if (update) gamma[g] = fmaxf(keep * gm + (1.0f - keep) * ng, floor_[b]);
+  }
+  dof = block_sum(dof, red);
+  if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = dof;
+}
+
+__global__ void k_seg_sum(const double* __restrict__ part, int blocks, double* __restrict__ out,
+                          float* __restrict__ out_f) {
+  const int b = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < blocks; i += 64) s += part[(long long)b * blocks + i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) {
+    out[b] = s;
+    if (out_f) out_f[b] = (float)s;
+  }
+}
+
+__global__ void k_masked_copy(float2* __restrict__ dst, const float2* __restrict__ src, const int* __restrict__ mask,
+                              int m) {
+  const int l = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && mask[l]) dst[(long long)l * m + i] = src[(long long)l * m + i];
+}
+
+__global__ void k_twiddle(float2* __restrict__ tw, int len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s, c;
+  sincospi(2.0 * (double)i / (double)len, &s, &c);
+  tw[i] = make_float2((float)c, (float)-s);
+}
+
+// Partial sums of gamma (for the preconditioner background) and of
+// ||mu - mu_prev||^2, ||mu||^2 (relative change, solver.py:1813-1826).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_sums(const float* __restrict__ gamma, const float2* __restrict__ mean, const float2* __restrict__ prev, int n,
+       double* __restrict__ part) {
+  __shared__ double red[8];
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  double sg = 0.0, dm = 0.0, nm = 0.0;
+  if (i < n) {
+    const long long g = (long long)b * n + i;
+    sg = gamma[g];
+    if (mean) {
+      const float2 u = mean[g], w = prev[g];
+      dm = (double)(u.x - w.x) * (u.x - w.x) + (double)(u.y - w.y) * (u.y - w.y);
+      nm = (double)u.x * u.x + (double)u.y * u.y;
+    }
+  }
+  block_sum2(sg, dm, red);
+  nm = block_sum(nm, red);
+  if (threadIdx.x == 0) {
+    const long long slot = ((long long)b * gridDim.x + blockIdx.x) * 3;
+    part[slot] = sg;
+    part[slot + 1] = dm;
+    part[slot + 2] = nm;
+  }
+}
+
+__global__ void k_sums_finish(const double* __restrict__ part, int blocks, double* __restrict__ out) {
+  const int b = blockIdx.x;
+  double a = 0.0, c = 0.0, d = 0.0;
+  for (int i = threadIdx.x; i < blocks; i += 64) {
+    a += part[((long long)b * blocks + i) * 3];
+    c += part[((long long)b * blocks + i) * 3 + 1];
+    d += part[((long long)b * blocks + i) * 3 + 2];
+  }
+  a = wave_sum(a);
+  c = wave_sum(c);
+  d = wave_sum(d);
+  if (threadIdx.x == 0) {
+    out[b * 3] = a;
+    out[b * 3 + 1] = c;
+    out[b * 3 + 2] = d;
+  }
+}
+
+
+// Initial scalars (solver.py:1063-1070, 1151-1156): power = max(||y||^2/M,
+// eps), floor = max(1e-12 power, tiny), noise = max(frac power, floor).
+__global__ void k_cg_scalars(const double* __restrict__ ye, int m, float min_variance, float noise_fraction,
+                             float* __restrict__ power, float* __restrict__ floor_, float* __restrict__ noise,
+                             int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const float p = fmaxf((float)(ye[b] / m), (float)F32_EPS_D);
+  const float f = fmaxf(p * min_variance, (float)F32_TINY_D);
+  power[b] = p;
+  floor_[b] = f;
+  noise[b] = fmaxf(noise_fraction * p, f);
+}
+
+// Backprojection initialisation of gamma (solver.py:1071-1089).
+__global__ void k_cg_gamma0(const float2* __restrict__ bp, const double* __restrict__ bpe,
+                            const float* __restrict__ power, const float* __restrict__ floor_, float blend, int n,
+                            float* __restrict__ gamma) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = bp[(long long)b * n + i];
+  const float p = power[b];
+  const float structured = (v.x * v.x + v.y * v.y) / fmaxf((float)bpe[b], (float)F32_EPS_D) * p;
+  const float uniform = p / (float)n;
+  gamma[(long long)b * n + i] = fmaxf(blend * structured + (1.0f - blend) * uniform, floor_[b]);
+}
+
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_rel_sum(const float* __restrict__ gamma, const float* __restrict__ dd, int n, double* __restrict__ part) {
+  __shared__ double red[8];
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  double r = 0.0;
+  if (i < n) r = fminf(fmaxf(gamma[(long long)b * n + i] * dd[(long long)b * n + i], 0.f), 1.f);
+  r = block_sum(r, red);
+  if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = r;
+}
+
+__global__ void k_reciprocal(const float* __restrict__ in, float* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 1.0f / in[i];
+}
+
+inline int nblk(long long n, int bs = FBME_BLOCK) { return (int)((n + bs - 1) / bs); }
+
+template <int EPT>
+void launch_rowpass(bool inv, const float2* in, float2* out, int n, int len_in, int len_out, long long rows,
+                    unsigned code, const float2* tw, hipStream_t st) {
+  const int tr = std::max(1, std::min<int>(EPT * FBME_BLOCK / (n + 1), 64));
+  const int grid = (int)((rows + tr - 1) / tr);
+  if (inv)
+    hipLaunchKernelGGL((k_rowpass<EPT, true>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out,
+                       rows, tr, code, tw);
+  else
+    hipLaunchKernelGGL((k_rowpass<EPT, false>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out,
+                       rows, tr, code, tw);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,
+                    int inverse);
+
+int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2, int e1, int e2, unsigned code1,
+                        unsigned code2, unsigned codee1, unsigned codee2, const float* gamma, const float* noise,
+                        void* work_a, void* work_b, void* spectrum, const void* tw_n1, const void* tw_n2,
+                        const void* tw_e1, const void* tw_e2) {
+  // work_a, work_b: batch * max(N, E1 E2) complex each.
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)n1 * n2;
+  float2* a = (float2*)work_a;
+  float2* bwork = (float2*)work_b;
+  hipLaunchKernelGGL(k_real_to_complex, dim3(nblk(batch * n)), dim3(FBME_BLOCK), 0, st, gamma, a, batch * n);
+  // lags = fft2(gamma): rows then columns (full lengths)
+  launch_rowpass<4>(false, a, bwork, n2, n2, n2, (long long)batch * n1, code2, (const float2*)tw_n2, st);
+  int rc = fbme_column_fft(stream, batch, n1, n1, n1, n2, bwork, a, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_embed, dim3(nblk((long long)e1 * e2), batch), dim3(FBME_BLOCK), 0, st, a, noise, bwork, n1, n2,
+                     m1, m2, e1, e2);
+  launch_rowpass<4>(false, bwork, a, e2, e2, e2, (long long)batch * e1, codee2, (const float2*)tw_e2, st);
+  rc = fbme_column_fft(stream, batch, e1, e1, e1, e2, a, spectrum, 0);
+  (void)code1;
+  (void)codee1;
+  (void)tw_n1;
+  (void)tw_e1;
+  if (rc) return rc;
+  return (int)hipGetLastError();
+}
+
+int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
+                         unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
+                         void* work_b, const void* tw_e1, const void* tw_e2) {
+  // work_a, work_b: lanes * M1 * E2 complex each.
+  hipStream_t st = (hipStream_t)stream;
+  launch_rowpass<4>(false, (const float2*)d, (float2*)work_a, e2, m2, e2, (long long)lanes * m1, code_e2,
+                    (const float2*)tw_e2, st);
+  const int ept = (e1 <= 1024 / 8) ? 4 : (e1 <= 512) ? 8 : 16;
+  const int tc = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
+  dim3 grid((e2 + tc - 1) / tc, lanes);
+  if (ept == 4)
+    hipLaunchKernelGGL((k_toeplitz_col<4>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
+                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
+  else if (ept == 8)
+    hipLaunchKernelGGL((k_toeplitz_col<8>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
+                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
+  else
+    hipLaunchKernelGGL((k_toeplitz_col<16>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
+                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
+  launch_rowpass<4>(true, (const float2*)work_b, (float2*)out, e2, e2, m2, (long long)lanes * m1, code_e2,
+                    (const float2*)tw_e2, st);
+  return (int)hipGetLastError();
+}
+
+int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val) {
+  // cand: batch * ceil(n / 1024) * 32 uint64; requires ceil(n/1024) * 32 <= 2048.
+  const int tiles = (int)((n + 1023) / 1024);
+  if (tiles * RANK > 2048) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_topk_tile, dim3(tiles, batch), dim3(FBME_BLOCK), 0, st, x, n, tiles,
+                     (unsigned long long*)cand);
+  hipLaunchKernelGGL(k_topk_merge, dim3(batch), dim3(64), 0, st, (const unsigned long long*)cand, tiles, x, n, idx,
+                     val);
+  return (int)hipGetLastError();
+}
+
+int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx, const float* val,
+                       const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,
+                       void* factors, void* sinv, float* bg, int* status) {
+  hipStream_t st = (hipStream_t)stream;
+  const long long m = (long long)m1 * m2;
+  hipLaunchKernelGGL(k_lr_factors, dim3(nblk(m * RANK), batch), dim3(FBME_BLOCK), 0, st, idx, val, (float2*)factors,
+                     n1, n2, m1, m2);
+  hipLaunchKernelGGL(k_lr_setup, dim3(batch), dim3(64), 0, st, idx, val, gamma_sum, noise, (const double2*)dir1,
+                     (const double2*)dir2, n1, n2, (float2*)sinv, bg, status);
+  return (int)hipGetLastError();
+}
+
+int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m, const void* factors, const void* sinv,
+                       const float* bg, const void* v, void* out, void* part, void* corr) {
+  // part: lanes * ceil(m / 1024) * 32 double2; corr: lanes * 32 complex.
+  hipStream_t st = (hipStream_t)stream;
+  const int chunk = 1024;
+  const int chunks = (m + chunk - 1) / chunk;
+  hipLaunchKernelGGL(k_lr_project, dim3(chunks, lanes), dim3(FBME_BLOCK), 0, st, (const float2*)factors,
+                     (const float2*)v, m, rhs, chunk, (double2*)part);
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(64), 0, st, (const double2*)part, chunks, (const float2*)sinv, bg,
+                     rhs, (float2*)corr);
+  hipLaunchKernelGGL(k_lr_apply, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (const float2*)factors,
+                     (const float2*)v, (const float2*)corr, bg, m, rhs, (float2*)out);
+  return (int)hipGetLastError();
+}
+
+int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out) {
+  // part: lanes * 64 doubles.
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = std::min(64, nblk(m));
+  hipLaunchKernelGGL(k_lane_dot, dim3(blocks, lanes), dim3(FBME_BLOCK), 0, st, (const float2*)a, (const float2*)b, m,
+                     part);
+  hipLaunchKernelGGL(k_lane_finish, dim3(nblk(lanes, 64)), dim3(64), 0, st, (const double*)part, blocks, lanes, out);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_update(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                    const double* rho, const double* curv, int* active) {
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pcg_update, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (float2*)x, (float2*)r,
+                     (const float2*)d, (const float2*)q, rho, curv, active, m);
+  hipLaunchKernelGGL(k_pcg_commit_valid, dim3(nblk(lanes, 64)), dim3(64), 0, st, rho, curv, active, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_direction(void* stream, int lanes, int m, void* d, const void* z, double* rho, const double* nrho,
+                       int* active, int restart) {
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pcg_direction, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (float2*)d, (const float2*)z,
+                     rho, nrho, active, m, restart);
+  hipLaunchKernelGGL(k_pcg_commit_rho, dim3(nblk(lanes, 64)), dim3(64), 0, st, rho, nrho, active, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_sub(void* stream, long long n, const void* a, const void* b, void* out) {
+  hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)a,
+                     (const float2*)b, (float2*)out, n);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_check(void* stream, int lanes, const double* rr, const double* init, int* active, double* best_rel,
+                   int* improved, float tol, int fixed) {
+  hipLaunchKernelGGL(k_pcg_check, dim3(nblk(lanes, 64)), dim3(64), 0, (hipStream_t)stream, rr, init, active,
+                     best_rel, improved, tol, fixed, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
+                    const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum) {
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_hutch, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, (const float2*)lifted,
+                     (const float2*)lprobes, gamma, n, probes, (float2*)mean, var, dd, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum,
+                     (float*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int fbme_cg_em(void* stream, int batch, int n, int m, float* gamma, const void* mean, const float* dd,
+               const double* rel_sum, const float* floor_, const unsigned char* mask, float hyper_shape,
+               float hyper_rate, float max_prec, float keep, int update, double* part, double* dof,
+               float* dof_f) {
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_cg_em, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, gamma, (const float2*)mean, dd, rel_sum,
+                     floor_, mask, n, m, hyper_shape, hyper_rate, max_prec, keep, update, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, dof, dof_f);
+  return (int)hipGetLastError();
+}
+
+int fbme_cg_init(void* stream, int batch, int n, int m, const void* bp, const double* ye, const double* bpe,
+                 float min_variance, float noise_fraction, float blend, float* power, float* floor_, float* noise,
+                 float* gamma) {
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cg_scalars, dim3(nblk(batch, 64)), dim3(64), 0, st, ye, m, min_variance, noise_fraction, power,
+                     floor_, noise, batch);
+  hipLaunchKernelGGL(k_cg_gamma0, dim3(nblk(n), batch), dim3(FBME_BLOCK), 0, st, (const float2*)bp, bpe, power, floor_,
+                     blend, n, gamma);
+  return (int)hipGetLastError();
+}
+
+int fbme_relevance_sum(void* stream, int batch, int n, const float* gamma, const float* dd, double* part,
+                       double* out) {
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_rel_sum, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, gamma, dd, n, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, out, (float*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int fbme_reciprocal(void* stream, long long n, const float* in, float* out) {
+  hipLaunchKernelGGL(k_reciprocal, dim3(nblk(n)), dim3(FBME_BLOCK), 0, (hipStream_t)stream, in, out, n);
+  return (int)hipGetLastError();
+}
+
+int fbme_masked_copy(void* stream, int lanes, int m, void* dst, const void* src, const int* mask) {
+  hipLaunchKernelGGL(k_masked_copy, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (float2*)dst,
+                     (const float2*)src, mask, m);
+  return (int)hipGetLastError();
+}
+
+int fbme_twiddle_table(void* stream, int len, void* out) {
+  if (len < 1 || len > 1024) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_twiddle, dim3(nblk(len)), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (float2*)out, len);
+  return (int)hipGetLastError();
+}
+
+int fbme_grid_sums(void* stream, int batch, int n, const float* gamma, const void* mean, const void* prev,
+                   double* part, double* out) {
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_sums, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, gamma, (const float2*)mean,
+                     (const float2*)prev, n, part);
+  hipLaunchKernelGGL(k_sums_finish, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, out);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -29,6 +29,19 @@ constexpr double F32_TINY_D = 1.17549435082228750797e-38;
 constexpr int KM = FBME_KMAX;
 constexpr int KP = KM + 1;   // padded row length in LDS
 constexpr int TE_MAX = 8;
+constexpr int DIR_LDS = 1024;   // double2 slots for the two Dirichlet tables
+
+// Phase timestamps for kernel analysis builds (-DFBME_PROBE): ticks of the
+// 100 MHz wall clock since kernel entry, written to status[scene * 16 + i].
+#ifdef FBME_PROBE
+#define FBME_MARK(i)                                                        \
+  do {                                                                     \
+    if (threadIdx.x == 0 && status) status[blockIdx.x * 16 + (i)] =        \
+        (int)(wall_clock64() - probe_t0);                                  \
+  } while (0)
+#else
+#define FBME_MARK(i) do {} while (0)
+#endif
 
 struct PolishArgs {
   float margin_factor, jitter_multiplier, max_precision, polish_damping;
@@ -149,77 +162,97 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
     mean_diag = fmax(mean_diag / K, F32_TINY_D);
     const double jit = (double)jitter_multiplier * F32_EPS_D * mean_diag;
     for (int r = 0; r < K; ++r) L.a[r][r].x += jit;
-    L.flag[0] = 0;
   }
   __syncthreads();
-  // Right-looking Cholesky, lower factor in place.
-  for (int j = 0; j < K; ++j) {
-    if (tid == 0) {
-      const double p = L.a[j][j].x;
-      if (!(p > 0.0)) L.flag[0] = 1;
-      L.a[j][j] = make_double2(sqrt(fmax(p, 0.0)), 0.0);
+  // Cholesky and inverse in registers: lane r owns row r of the (padded to
+  // 32 with identity) equilibrated system; rows are exchanged by shuffles.
+  const int lane = tid & 63;
+  int fail = 0;
+  {
+    double2 a[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      double2 v = make_double2(lane == q ? 1.0 : 0.0, 0.0);
+      if (lane < K && q < K) v = L.a[lane][q];
+      a[q] = v;
     }
     __syncthreads();
-    const double piv = L.a[j][j].x;
-    for (int r = j + 1 + tid; r < K; r += blockDim.x) {
-      L.a[r][j] = make_double2(L.a[r][j].x / piv, L.a[r][j].y / piv);
-    }
-    __syncthreads();
-    const int m = K - j - 1;
-    for (int e = tid; e < m * m; e += blockDim.x) {
-      const int r = j + 1 + e / m, q = j + 1 + e % m;
-      if (q <= r) {
-        const double2 lr = L.a[r][j], lq = L.a[q][j];
-        // a[r][q] -= l[r] * conj(l[q])
-        L.a[r][q].x -= lr.x * lq.x + lr.y * lq.y;
-        L.a[r][q].y -= lr.y * lq.x - lr.x * lq.y;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) {
+      const double d = __shfl(a[j].x, j, 64);
+      fail |= !(d > 0.0);
+      const double piv = sqrt(fmax(d, 0.0));
+      if (lane > j) {
+        a[j].x /= piv;
+        a[j].y /= piv;
+      } else if (lane == j) {
+        a[j] = make_double2(piv, 0.0);
+      }
+#pragma unroll
+      for (int q = j + 1; q < 32; ++q) {
+        const double lx = __shfl(a[j].x, q, 64), ly = __shfl(a[j].y, q, 64);
+        if (lane >= q) {  // a[r][q] -= l[r][j] * conj(l[q][j])
+          a[q].x -= a[j].x * lx + a[j].y * ly;
+          a[q].y -= a[j].y * lx - a[j].x * ly;
+        }
       }
     }
-    __syncthreads();
+    if (lane < 32) {
+#pragma unroll
+      for (int q = 0; q < 32; ++q) L.a[lane][q] = a[q];
+    }
   }
-  // X = L^-1 (one column per thread, forward substitution).
-  if (tid < K) {
-    const int col = tid;
-    for (int r = 0; r < K; ++r) {
-      if (r < col) {
-        L.x[r][col] = make_double2(0.0, 0.0);
-        continue;
-      }
-      double2 acc = make_double2(r == col ? 1.0 : 0.0, 0.0);
-      for (int p = col; p < r; ++p) {
-        const double2 t = zmul(L.a[r][p], L.x[p][col]);
-        acc.x -= t.x;
-        acc.y -= t.y;
-      }
-      const double d = L.a[r][r].x;
-      L.x[r][col] = make_double2(acc.x / d, acc.y / d);
+  if (tid == 0) L.flag[0] = 0;
+  __syncthreads();
+  if (fail && lane < K) L.flag[0] = 1;
+  // X = L^-1, lane c owns column c (X[p][c] = 0 for p < c needs no guard).
+  double2 x[32];
+#pragma unroll
+  for (int r = 0; r < 32; ++r) {
+    double2 acc = make_double2(r == lane ? 1.0 : 0.0, 0.0);
+#pragma unroll
+    for (int p = 0; p < r; ++p) {
+      const double2 l = L.a[r][p];
+      acc.x -= l.x * x[p].x - l.y * x[p].y;
+      acc.y -= l.x * x[p].y + l.y * x[p].x;
     }
+    const double dg = L.a[r][r].x;
+    x[r] = make_double2(acc.x / dg, acc.y / dg);
+  }
+  double ev = 0.0;
+#pragma unroll
+  for (int r = 0; r < 32; ++r) ev += x[r].x * x[r].x + x[r].y * x[r].y;
+  if (lane < 32) {
+#pragma unroll
+    for (int r = 0; r < 32; ++r) L.x[r][lane] = x[r];
   }
   if (tid < K) {
     const float2 b = c.bp[w[tid]];
     L.rhs[tid] = make_double2((double)b.x / nu, (double)b.y / nu);
   }
   __syncthreads();
-  // z = X (rhs / d); mu_eq = X^H z; Sigma_eq = column norms of X.
+  // z = X (rhs / d), lane r owns row r.
   if (tid < K) {
     double2 acc = make_double2(0.0, 0.0);
     for (int p = 0; p <= tid; ++p) {
-      const double2 s = make_double2(L.rhs[p].x / L.dsc[p], L.rhs[p].y / L.dsc[p]);
-      const double2 t = zmul(L.x[tid][p], s);
+      const double2 sv = make_double2(L.rhs[p].x / L.dsc[p], L.rhs[p].y / L.dsc[p]);
+      const double2 t = zmul(L.x[tid][p], sv);
       acc.x += t.x;
       acc.y += t.y;
     }
     L.z[tid] = acc;
   }
   __syncthreads();
+  // mu_eq = X^H z and Sigma_eq = ||X[:, c]||^2, lane c owns column c.
   if (tid < K) {
     double2 acc = make_double2(0.0, 0.0);
-    double ev = 0.0;
-    for (int r = tid; r < K; ++r) {
-      const double2 t = zconjmul(L.x[r][tid], L.z[r]);
-      acc.x += t.x;
-      acc.y += t.y;
-      ev += L.x[r][tid].x * L.x[r][tid].x + L.x[r][tid].y * L.x[r][tid].y;
+#pragma unroll
+    for (int r = 0; r < 32; ++r) {
+      if (r < K) {
+        const double2 zr = L.z[r];
+        acc.x += x[r].x * zr.x + x[r].y * zr.y;
+        acc.y += x[r].x * zr.y - x[r].y * zr.x;
+      }
     }
     const double d = L.dsc[tid];
     L.mu[tid] = make_double2(acc.x / d, acc.y / d);
@@ -330,9 +363,14 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   __shared__ double work_gam[KM], best_gam[KM], best_sig[KM], best_dd[KM];
   __shared__ double2 best_mu[KM];
   __shared__ double sh[16];
+  __shared__ double2 dir_lds[DIR_LDS];
+  __shared__ unsigned long long packed[KM];
 
   const int si = blockIdx.x;
   if (si >= nscenes) return;
+#ifdef FBME_PROBE
+  const long long probe_t0 = wall_clock64();
+#endif
   const FbmeScene s = scenes[si];
   const int tid = threadIdx.x;
   const double* scs = sc + (long long)si * SC_COUNT;
@@ -343,9 +381,20 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   c.k = s.k;
   c.floor_v = scs[SC_FLOOR];
   c.ye = scs[SC_YE];
-  c.d1 = dir + s.dir1_off;
-  c.d2 = dir + s.dir2_off;
+  // Dirichlet tables staged in LDS (the Gram gathers hit them K^2 times per
+  // evaluation); grids wider than the staging area read them from global.
+  if (s.n1 + s.n2 <= DIR_LDS) {
+    for (int i = tid; i < s.n1; i += blockDim.x) dir_lds[i] = dir[s.dir1_off + i];
+    for (int i = tid; i < s.n2; i += blockDim.x) dir_lds[s.n1 + i] = dir[s.dir2_off + i];
+    c.d1 = dir_lds;
+    c.d2 = dir_lds + s.n1;
+  } else {
+    c.d1 = dir + s.dir1_off;
+    c.d2 = dir + s.dir2_off;
+  }
   c.bp = bp_all + s.grid_off;
+  __syncthreads();
+  FBME_MARK(0);
   const int K = s.k;
   const long long n_grid = (long long)s.n1 * s.n2;
   PolishState* st = states + si;
@@ -361,18 +410,37 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   if (phase == 0) {
     // Top-k of the candidate scores (solver.py:1931).
     const int ncand = s.row_items * K;
-    for (int i = tid; i < ncand; i += blockDim.x) {
-      keys[i] = cand_key[s.cand_off + i];
-      kidx[i] = cand_idx[s.cand_off + i];
+    if (ncand <= 64 * 32) {
+      // Wave arg-max selection over register-held candidates (in order).
+      unsigned long long v[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        const int slot = tid + 64 * i;
+        v[i] = (slot < ncand) ? pack_key(cand_key[s.cand_off + slot], slot) : 0ull;
+        if (slot < ncand && cand_key[s.cand_off + slot] == 0u) v[i] = 0ull;
+      }
+      wave_topk<32>(v, K, packed);
+      __syncthreads();
+      if (tid < K) {
+        const int slot = packed_pos(packed[tid]);
+        srt_key[tid] = (uint32_t)(packed[tid] >> 32);
+        srt_idx[tid] = cand_idx[s.cand_off + slot];
+      }
+      __syncthreads();
+    } else {
+      for (int i = tid; i < ncand; i += blockDim.x) {
+        keys[i] = cand_key[s.cand_off + i];
+        kidx[i] = cand_idx[s.cand_off + i];
+      }
+      __syncthreads();
+      block_select_topk(keys, ncand, K, pos, hist, misc);
+      if (tid < K) {
+        sel_key[tid] = keys[pos[tid]];
+        sel_idx[tid] = kidx[pos[tid]];
+      }
+      __syncthreads();
+      rank_sort_desc(sel_key, sel_idx, K, srt_key, srt_idx);
     }
-    __syncthreads();
-    block_select_topk(keys, ncand, K, pos, hist, misc);
-    if (tid < K) {
-      sel_key[tid] = keys[pos[tid]];
-      sel_idx[tid] = kidx[pos[tid]];
-    }
-    __syncthreads();
-    rank_sort_desc(sel_key, sel_idx, K, srt_key, srt_idx);
     // Zero-tail pruning + re-initialisation from UAMP second moments
     // (solver.py:4087-4106): gamma_S = max(|mu|^2 + v, floor).
     if (tid < K) {
@@ -381,6 +449,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     }
     work_noise = scs[SC_NOISE];
     __syncthreads();
+    FBME_MARK(1);
   } else {
     // Scenes without an exchange finished in the first phase.
     if (min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K))) <= 0) return;
@@ -404,18 +473,36 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     // Residual exchange (solver.py:4392-4494).
     const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
     const int nx = s.xtile_items * exch;
-    for (int i = tid; i < nx; i += blockDim.x) {
-      keys[i] = xcand_key[s.xcand_off + i];
-      kidx[i] = xcand_idx[s.xcand_off + i];
+    if (nx <= 64 * 32) {
+      unsigned long long v[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        const int slot = tid + 64 * i;
+        v[i] = (slot < nx && xcand_key[s.xcand_off + slot] != 0u)
+                   ? pack_key(xcand_key[s.xcand_off + slot], slot) : 0ull;
+      }
+      wave_topk<32>(v, exch, packed);
+      __syncthreads();
+      if (tid < exch) {
+        const int slot = packed_pos(packed[tid]);
+        srt_key[tid] = (uint32_t)(packed[tid] >> 32);
+        srt_idx[tid] = xcand_idx[s.xcand_off + slot];
+      }
+      __syncthreads();
+    } else {
+      for (int i = tid; i < nx; i += blockDim.x) {
+        keys[i] = xcand_key[s.xcand_off + i];
+        kidx[i] = xcand_idx[s.xcand_off + i];
+      }
+      __syncthreads();
+      block_select_topk(keys, nx, exch, pos, hist, misc);
+      if (tid < exch) {
+        sel_key[tid] = keys[pos[tid]];
+        sel_idx[tid] = kidx[pos[tid]];
+      }
+      __syncthreads();
+      rank_sort_desc(sel_key, sel_idx, exch, srt_key, srt_idx);
     }
-    __syncthreads();
-    block_select_topk(keys, nx, exch, pos, hist, misc);
-    if (tid < exch) {
-      sel_key[tid] = keys[pos[tid]];
-      sel_idx[tid] = kidx[pos[tid]];
-    }
-    __syncthreads();
-    rank_sort_desc(sel_key, sel_idx, exch, srt_key, srt_idx);
     // Deletion gains of the current best support (solver.py:3219-3317).
     if (tid < K) {
       const double g = fmax(best_gam[tid], tiny);
@@ -459,6 +546,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
 
   for (;;) {
     const double nle = evaluate(c, work_s, work_gam, work_noise, L, a.jitter_multiplier);
+    FBME_MARK(2 + 2 * pit);
     bool accept = true;
     if (have_best) {
       const double margin = (double)a.margin_factor * F32_EPS_D * (1.0 + fabs(nle) + fabs(best_nle));
@@ -510,6 +598,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     upd = fmax(upd, c.floor_v);
     work_noise = keep * best_noise + (1.0 - keep) * upd;
     __syncthreads();
+    FBME_MARK(3 + 2 * pit);
     const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
     if (exch > 0 && pit == 0 && phase == 0) {
       if (tid < K) {
@@ -548,7 +637,9 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     out_nle[oi] = (float)best_nle;
     for (int i = 0; i < a.evaluations; ++i) out_trace[(long long)oi * a.evaluations + i] = (float)trace[i];
     if (out_exchanges) out_exchanges[oi] = exchanges;
+#ifndef FBME_PROBE
     if (status) status[oi] = L.flag[0];
+#endif
   }
   if (a.diagnostics) {
     // Fixed-budget EM residuals (solver.py:3408-3496) and trace flags.
@@ -598,6 +689,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     res->gam[rank] = (float)best_gam[tid];
   }
   if (tid == 0) results[si].k = K;
+  FBME_MARK(15);
 }
 
 // |A^H (y - C mu)|^2 over one tile of the grid with the support excluded,
@@ -738,11 +830,86 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
   }
 }
 
+
+// Exact reduced posterior on a given support with diffuse-tail loading
+// nu = noise + max(sum gamma - sum gamma_S, 0): the active-set posterior of
+// the general solver (solver.py:3591-3779 with tail_loading=None,
+// 3897-3924).  One wavefront per scene; outputs over the full grid:
+// mean (0 off S), variance (gamma off S) and data diagonal (0 off S).
+__global__ void __launch_bounds__(64)
+k_reduced_posterior(int k, int n1, int n2, int m1, int m2, const int* __restrict__ support,
+                    const float* __restrict__ gamma, const double* __restrict__ gamma_sum,
+                    const float* __restrict__ noise, const float2* __restrict__ bp, const double* __restrict__ ye,
+                    const double2* __restrict__ d1, const double2* __restrict__ d2, float jitter_multiplier,
+                    float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd,
+                    int* __restrict__ status) {
+  __shared__ PolishLds L;
+  __shared__ int w[KM];
+  __shared__ double wg[KM];
+  __shared__ double red[2];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const long long n = (long long)n1 * n2;
+  double gs = 0.0;
+  if (tid < k) {
+    w[tid] = support[b * k + tid];
+    wg[tid] = (double)gamma[b * n + w[tid]];
+    gs = wg[tid];
+  }
+  gs = wave_sum(gs);
+  if (tid == 0) red[0] = gs;
+  __syncthreads();
+  const double nu = (double)noise[b] + fmax(gamma_sum[b] - red[0], 0.0);
+  SceneCtx c;
+  c.n1 = n1;
+  c.n2 = n2;
+  c.m_total = m1 * m2;
+  c.k = k;
+  c.floor_v = 0.0;
+  c.ye = ye[b];
+  c.d1 = d1;
+  c.d2 = d2;
+  c.bp = bp + b * n;
+  const double nle = evaluate(c, w, wg, nu, L, jitter_multiplier);
+  if (tid < k) {
+    const long long g = b * n + w[tid];
+    mean[g] = make_float2((float)L.mu[tid].x, (float)L.mu[tid].y);
+    var[g] = (float)L.sig[tid];
+    dd[g] = (float)L.dd[tid];
+  }
+  if (tid == 0 && status && (L.flag[0] || !isfinite(nle))) status[b] = 1;
+}
+
+// Grid defaults of the reduced posterior: mean 0, variance gamma, dd 0.
+__global__ void k_reduced_defaults(const float* __restrict__ gamma, float2* __restrict__ mean, float* __restrict__ var,
+                                   float* __restrict__ dd, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  mean[i] = make_float2(0.f, 0.f);
+  var[i] = gamma[i];
+  dd[i] = 0.f;
+}
+
 }  // namespace
 
 extern "C" {
 
 size_t fbme_polish_state_bytes(void) { return sizeof(PolishState); }
+
+int fbme_reduced_posterior(void* stream, int batch, int k, int n1, int n2, int m1, int m2, const int* support,
+                           const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                           const double* ye, const void* dir1, const void* dir2, float jitter_multiplier,
+                           void* mean, float* var, float* dd, int* status) {
+  if (batch < 1 || k < 1 || k > FBME_KMAX) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const long long total = (long long)batch * n1 * n2;
+  hipLaunchKernelGGL(k_reduced_defaults, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gamma,
+                     (float2*)mean, var, dd, total);
+  hipLaunchKernelGGL(k_reduced_posterior, dim3(batch), dim3(64), 0, st, k, n1, n2, m1, m2, support, gamma,
+                     gamma_sum, noise, (const float2*)bp, ye, (const double2*)dir1, (const double2*)dir2,
+                     jitter_multiplier, (float2*)mean, var, dd, status);
+  return (int)hipGetLastError();
+}
 size_t fbme_polish_result_bytes(void) { return sizeof(PolishResult); }
 
 int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const void* bp, const double* sc,

@@ -143,6 +143,15 @@ extern "C" {
 
 int fbme_abi_version(void) { return FBME_ABI_VERSION; }
 
+int fbme_dirichlet_table(int m, int n, double* out) {
+  if (m < 1 || n < 1 || m > n || !out) return FBME_ERR_ARG;
+  std::vector<double> t;
+  t.reserve(2 * (size_t)n);
+  dirichlet(m, n, t);
+  std::memcpy(out, t.data(), sizeof(double) * t.size());
+  return 0;
+}
+
 void fbme_default_options(FbmePlanOptions* o, int stable) {
   o->uamp_damping = 0.05f;
   o->damping = 0.15f;

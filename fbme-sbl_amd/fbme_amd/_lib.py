@@ -98,6 +98,64 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_float, c_float,
                                        c_float, c_int, c_float, c_void_p]),
+    # general-solver (mode="cg") kernels
+    "fbme_radix_code": (c_uint, [c_int]),
+    "fbme_column_fft": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int,
+                                c_void_p, c_void_p, c_int]),
+    "fbme_dirichlet_table": (c_int, [c_int, c_int, c_void_p]),
+    "fbme_twiddle_table": (c_int, [c_void_p, c_int, c_void_p]),
+    "fbme_gamma_spectrum": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                    c_int, c_int, c_int, c_uint, c_uint,
+                                    c_uint, c_uint, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "fbme_toeplitz_matvec": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                     c_int, c_int, c_uint, c_uint, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
+    "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
+                            c_void_p, c_void_p]),
+    "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                   c_int, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "fbme_lowrank_apply": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
+    "fbme_lane_dot": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                              c_void_p, c_void_p]),
+    "fbme_pcg_update": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p]),
+    "fbme_pcg_direction": (c_int, [c_void_p, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_int]),
+    "fbme_sub": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p, c_void_p]),
+    "fbme_pcg_check": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_float, c_int]),
+    "fbme_masked_copy": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                 c_void_p]),
+    "fbme_hutchinson": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p]),
+    "fbme_cg_em": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+                           c_float, c_float, c_float, c_int, c_void_p,
+                           c_void_p, c_void_p]),
+    "fbme_cg_init": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                             c_void_p, c_void_p, c_float, c_float, c_float,
+                             c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_relevance_sum": (c_int, [c_void_p, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "fbme_reciprocal": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p]),
+    "fbme_grid_sums": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                               c_void_p, c_void_p, c_void_p]),
+    "fbme_reduced_posterior": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                       c_int, c_int, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p,
+                                       c_float, c_void_p, c_void_p, c_void_p,
+                                       c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

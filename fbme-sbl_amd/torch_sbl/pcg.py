@@ -1,0 +1,519 @@
+"""General solver, mode="cg": covariance-free PCG + Hutchinson on HIP kernels.
+
+Control flow follows torch_sbl/solver.py:1194-1904 and 2023-2080 (probe
+count and Rademacher probes, adaptive CG tolerance schedule, warm start,
+active-set switch for a single scene, convergence checks); the arithmetic is
+in libfbme_hip.so (fbme_pcg.hip): BTTB spectrum and Toeplitz matvec with the
+1 + P right-hand sides batched into the FFT leading dimension, low-rank
+Woodbury preconditioner, masked PCG lanes with FP64 inner products,
+Hutchinson diagonal, relevance projection, MacKay and noise updates.
+
+Like the reference, this path synchronises with the host at PCG
+checkpoints (early exit), at convergence checks and at the active-set
+switch.  Probes are drawn from torch's CPU generator seeded with
+``config.seed`` (the stream the reference draws on CPU) and copied to the GPU
+once.
+
+Differences from the reference, all documented:
+  * an active-set switch is taken when the 32 strongest variances explain
+    95% of the total (the reference then also selects 32 atoms); cases where
+    the reference would switch to 33-256 atoms keep the PCG posterior;
+  * the low-rank preconditioner requires ceil(N / 1024) <= 64;
+  * 'circulant', 'nystrom' and 'none' preconditioners, Hadamard probes,
+    adaptive probe counts, joint sparsity and spatial coupling raise
+    NotImplementedError.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from math import prod
+
+import numpy as np
+import torch
+
+from fbme_amd import _lib
+
+from .config import SBLConfig, SBLResult
+from .operators import _next_smooth_length
+
+F32_EPS = float(np.finfo(np.float32).eps)
+F32_TINY = float(np.finfo(np.float32).tiny)
+RANK = 32
+
+_TABLES: dict = {}
+
+
+def clear_cache() -> None:
+    _TABLES.clear()
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class _Tables:
+    """Per-shape device tables: twiddles, Dirichlet kernels, radix codes."""
+
+    def __init__(self, grid, samples, device):
+        lib = _lib.load()
+        n1, n2 = grid
+        m1, m2 = samples
+        self.e1 = _next_smooth_length(2 * m1 - 1)
+        self.e2 = _next_smooth_length(2 * m2 - 1)
+        for length in (n1, n2, self.e1, self.e2):
+            if length > 1024:
+                raise NotImplementedError(
+                    f"FFT length {length} exceeds the kernels' 1024 limit")
+        self.codes = {L: lib.fbme_radix_code(L) for L in (n1, n2, self.e1, self.e2)}
+        if any(c == 0xFFFFFFFF for c in self.codes.values()):
+            raise NotImplementedError("grid lengths must factor into 2, 3 and 5")
+        stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        self.tw = {}
+        for L in (n1, n2, self.e1, self.e2):
+            t = torch.empty(L, dtype=torch.complex64, device=device)
+            _lib.check(lib.fbme_twiddle_table(stream, L, _ptr(t)), "twiddle")
+            self.tw[L] = t
+        d1 = np.zeros(2 * n1)
+        d2 = np.zeros(2 * n2)
+        _lib.check(lib.fbme_dirichlet_table(m1, n1, d1.ctypes.data_as(ctypes.c_void_p)), "dirichlet")
+        _lib.check(lib.fbme_dirichlet_table(m2, n2, d2.ctypes.data_as(ctypes.c_void_p)), "dirichlet")
+        self.dir1 = torch.from_numpy(d1).to(device)
+        self.dir2 = torch.from_numpy(d2).to(device)
+
+
+def _tables(grid, samples, device):
+    key = (tuple(grid), tuple(samples), str(device))
+    if key not in _TABLES:
+        _TABLES[key] = _Tables(grid, samples, device)
+    return _TABLES[key]
+
+
+def _probe_count(n, m, cfg: SBLConfig) -> int:
+    """solver.py:1194-1208"""
+    if not cfg.auto_probe_scaling:
+        return cfg.hutchinson_probes
+    under = n / m
+    rec = 8.0 * under ** 0.5
+    if under > 16.0:
+        rec = max(rec, 2.0 * under)
+    rec = int(rec + 0.999999)
+    return min(cfg.max_hutchinson_probes, max(cfg.hutchinson_probes, rec))
+
+
+def _schedule(it: int, cfg: SBLConfig) -> float:
+    """solver.py:6086-6118 (non-adaptive probe count)."""
+    if not cfg.adaptive_cg:
+        return cfg.cg_tolerance
+    if it <= 5:
+        return max(cfg.cg_initial_tolerance, cfg.cg_tolerance)
+    if it <= 15:
+        return max(cfg.cg_tolerance,
+                   (cfg.cg_initial_tolerance * cfg.cg_tolerance) ** 0.5)
+    return cfg.cg_tolerance
+
+
+def _check_supported(cfg: SBLConfig) -> None:
+    bad = []
+    if cfg.cg_preconditioner != "low_rank":
+        bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
+    if cfg.cg_preconditioner_rank != RANK:
+        bad.append("cg_preconditioner_rank != 32")
+    if cfg.cg_matvec != "tbt":
+        bad.append("cg_matvec='fourier'")
+    if cfg.probe_type != "rademacher":
+        bad.append("probe_type='hadamard'")
+    if cfg.adaptive_probes:
+        bad.append("adaptive_probes")
+    if cfg.joint_sparsity:
+        bad.append("joint_sparsity")
+    if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
+        bad.append("spatial coupling")
+    if cfg.fixed_noise_variance is not None:
+        bad.append("fixed_noise_variance")
+    if cfg.initialization != "backprojection":
+        bad.append("initialization='uniform'")
+    if cfg.offgrid_refinement:
+        bad.append("offgrid_refinement")
+    if bad:
+        raise NotImplementedError(
+            "mode='cg' on MI355X does not implement: " + ", ".join(bad))
+
+
+class _Solver:
+    """Device buffers and kernel calls of one (B, grid, samples, P) problem."""
+
+    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None):
+        self.lib = _lib.load()
+        self.dev = y.device
+        self.cfg = cfg
+        self.b = y.shape[0]
+        self.samples = tuple(int(v) for v in y.shape[-2:])
+        self.grid = tuple(grid)
+        self.n = prod(self.grid)
+        self.m = prod(self.samples)
+        self.t = _tables(self.grid, self.samples, self.dev)
+        self.p = _probe_count(self.n, self.m, cfg)
+        self.r = 1 + self.p
+        self.lanes = self.b * self.r
+        self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+        dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
+        c64, f32, f64 = torch.complex64, torch.float32, torch.float64
+        self.y = y.reshape(b, m).contiguous()
+        n1, n2 = self.grid
+        m1, m2 = self.samples
+        e1, e2 = self.t.e1, self.t.e2
+        # probes from torch's CPU generator (solver.py:5979-6000)
+        if probes_override is not None:
+            probes = probes_override
+        else:
+            g = torch.Generator(device="cpu")
+            g.manual_seed(cfg.seed)
+            probes = (2 * torch.randint(0, 2, (b, m, self.p), generator=g) - 1).to(c64)
+        self.rhs = torch.empty((b, r, m), dtype=c64, device=dev)
+        self.rhs[:, 0] = self.y
+        self.rhs[:, 1:] = probes.permute(0, 2, 1).to(dev)
+        self.lprobes = torch.empty((b, self.p, n), dtype=c64, device=dev)
+        self.mid = torch.empty(b * max(r, 1) * n1 * max(m2, 1), dtype=c64, device=dev)
+        self._adjoint(self.rhs[:, 1:].reshape(b * self.p, m), self.lprobes.view(b * self.p, n))
+        # state
+        self.gamma = torch.empty((b, n), dtype=f32, device=dev)
+        self.power = torch.empty(b, dtype=f32, device=dev)
+        self.floor = torch.empty(b, dtype=f32, device=dev)
+        self.noise = torch.empty(b, dtype=f32, device=dev)
+        self.noise_new = torch.empty(b, dtype=f32, device=dev)
+        self.bp = torch.empty((b, n), dtype=c64, device=dev)
+        self.ye = torch.empty(b, dtype=f64, device=dev)
+        self.bpe = torch.empty(b, dtype=f64, device=dev)
+        self.mean = torch.zeros((b, n), dtype=c64, device=dev)
+        self.prev = torch.zeros((b, n), dtype=c64, device=dev)
+        self.var = torch.empty((b, n), dtype=f32, device=dev)
+        self.dd = torch.empty((b, n), dtype=f32, device=dev)
+        self.pred = torch.empty((b, m), dtype=c64, device=dev)
+        self.lifted = torch.empty((b, r, n), dtype=c64, device=dev)
+        # PCG vectors
+        self.x = torch.zeros((b, r, m), dtype=c64, device=dev)
+        self.res = torch.empty_like(self.x)
+        self.d = torch.empty_like(self.x)
+        self.q = torch.empty_like(self.x)
+        self.z = torch.empty_like(self.x)
+        self.best_x = torch.empty_like(self.x)
+        self.have_solution = False
+        # matvec / spectrum work
+        self.work_a = torch.empty(self.lanes * m1 * e2, dtype=c64, device=dev)
+        self.work_b = torch.empty(self.lanes * m1 * e2, dtype=c64, device=dev)
+        big = b * max(n, e1 * e2)
+        self.spec_a = torch.empty(big, dtype=c64, device=dev)
+        self.spec_b = torch.empty(big, dtype=c64, device=dev)
+        self.spectrum = torch.empty((b, e1, e2), dtype=c64, device=dev)
+        # preconditioner
+        tiles = (n + 1023) // 1024
+        if tiles * RANK > 2048:
+            raise NotImplementedError("grid too large for the low-rank top-32 merge")
+        self.cand = torch.empty(b * tiles * RANK, dtype=torch.int64, device=dev)
+        self.top_idx = torch.empty((b, RANK), dtype=torch.int32, device=dev)
+        self.top_val = torch.empty((b, RANK), dtype=f32, device=dev)
+        self.factors = torch.empty((b, m, RANK), dtype=c64, device=dev)
+        self.sinv = torch.empty((b, RANK, RANK), dtype=c64, device=dev)
+        self.bg = torch.empty(b, dtype=f32, device=dev)
+        self.lr_part = torch.empty(self.lanes * ((m + 1023) // 1024) * RANK * 2, dtype=f64, device=dev)
+        self.corr = torch.empty((self.lanes, RANK), dtype=c64, device=dev)
+        # lane scalars
+        self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
+        self.rho = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.nrho = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.curv = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.rr = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.rhs_rr = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.active = torch.empty(self.lanes, dtype=torch.int32, device=dev)
+        self.best_rel = torch.empty(self.lanes, dtype=f64, device=dev)
+        self.improved = torch.empty(self.lanes, dtype=torch.int32, device=dev)
+        blocks_n = (n + 255) // 256
+        self.grid_part = torch.empty(b * blocks_n * 3, dtype=f64, device=dev)
+        self.sums = torch.empty((b, 3), dtype=f64, device=dev)
+        self.rel_sum = torch.empty(b, dtype=f64, device=dev)
+        self.dof = torch.empty(b, dtype=f64, device=dev)
+        self.dof_f = torch.empty(b, dtype=f32, device=dev)
+        self.noise_part = torch.empty(b * ((m + 255) // 256), dtype=f64, device=dev)
+        self.status = torch.zeros(b, dtype=torch.int32, device=dev)
+        self.support = None
+        self.mask = None
+        self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
+
+    # ------------------------------------------------------------ primitives
+    def _chk(self, rc, what):
+        _lib.check(rc, what)
+
+    def _adjoint(self, v, out):
+        batch = v.shape[0]
+        need = batch * self.grid[0] * self.samples[1]
+        if self.mid.numel() < need:
+            self.mid = torch.empty(need, dtype=torch.complex64, device=self.dev)
+        self._chk(self.lib.fbme_prefix_adjoint(self.stream, batch, *self.grid, *self.samples,
+                                               _ptr(v), _ptr(out), _ptr(self.mid)), "adjoint")
+
+    def _forward(self, x, out):
+        batch = x.shape[0]
+        need = batch * self.grid[0] * self.samples[1]
+        if self.mid.numel() < need:
+            self.mid = torch.empty(need, dtype=torch.complex64, device=self.dev)
+        self._chk(self.lib.fbme_prefix_forward(self.stream, batch, *self.grid, *self.samples,
+                                               _ptr(x), _ptr(out), _ptr(self.mid)), "forward")
+
+    def lane_dot(self, a, bvec, out):
+        lanes = a.numel() // a.shape[-1]
+        self._chk(self.lib.fbme_lane_dot(self.stream, lanes, a.shape[-1], _ptr(a), _ptr(bvec),
+                                         _ptr(self.dot_part), _ptr(out)), "lane_dot")
+
+    def matvec(self, d, out):
+        t = self.t
+        m1, m2 = self.samples
+        self._chk(self.lib.fbme_toeplitz_matvec(
+            self.stream, self.lanes, self.r, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
+            _ptr(self.spectrum), _ptr(d), _ptr(out), _ptr(self.work_a), _ptr(self.work_b),
+            _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])), "toeplitz_matvec")
+
+    def precondition(self, v, out):
+        self._chk(self.lib.fbme_lowrank_apply(
+            self.stream, self.lanes, self.r, self.m, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
+            _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr)), "lowrank_apply")
+
+    # ------------------------------------------------------------- pipeline
+    def init(self):
+        cfg = self.cfg
+        self._adjoint(self.y, self.bp)
+        self._chk(self.lib.fbme_lane_dot(self.stream, self.b, self.m, _ptr(self.y), _ptr(self.y),
+                                         _ptr(self.dot_part), _ptr(self.ye)), "ye")
+        self._chk(self.lib.fbme_lane_dot(self.stream, self.b, self.n, _ptr(self.bp), _ptr(self.bp),
+                                         _ptr(self.dot_part), _ptr(self.bpe)), "bpe")
+        self._chk(self.lib.fbme_cg_init(self.stream, self.b, self.n, self.m, _ptr(self.bp), _ptr(self.ye),
+                                        _ptr(self.bpe), float(cfg.min_variance),
+                                        float(cfg.initial_noise_fraction), float(cfg.initialization_blend),
+                                        _ptr(self.power), _ptr(self.floor), _ptr(self.noise),
+                                        _ptr(self.gamma)), "cg_init")
+        self.grid_sums(with_mean=False)
+
+    def grid_sums(self, with_mean=True):
+        self._chk(self.lib.fbme_grid_sums(self.stream, self.b, self.n, _ptr(self.gamma),
+                                          _ptr(self.mean) if with_mean else None,
+                                          _ptr(self.prev) if with_mean else None,
+                                          _ptr(self.grid_part), _ptr(self.sums)), "grid_sums")
+
+    def setup_operators(self):
+        t = self.t
+        n1, n2 = self.grid
+        m1, m2 = self.samples
+        self._chk(self.lib.fbme_gamma_spectrum(
+            self.stream, self.b, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
+            t.codes[t.e2], _ptr(self.gamma), _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b),
+            _ptr(self.spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
+            "gamma_spectrum")
+        self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
+                                       _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
+        gsum = self.sums[:, 0].contiguous()
+        self._chk(self.lib.fbme_lowrank_setup(
+            self.stream, self.b, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
+            _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
+            _ptr(self.status)), "lowrank_setup")
+
+    def pcg(self, tol: float):
+        """Batched complex PCG (solver.py:5582-5723).  Returns iterations."""
+        cfg = self.cfg
+        lib = self.lib
+        lanes, m = self.lanes, self.m
+        fixed = int(cfg.cg_fixed_iterations)
+        warm = cfg.cg_warm_start and self.have_solution
+        if warm:
+            self.matvec(self.x, self.q)
+            self._chk(lib.fbme_sub(self.stream, self.x.numel(), _ptr(self.rhs), _ptr(self.q), _ptr(self.res)), "sub")
+        else:
+            self.x.zero_()
+            self.res.copy_(self.rhs)
+        self.precondition(self.res, self.z)
+        self.lane_dot(self.res, self.z, self.rho)
+        self.lane_dot(self.res, self.res, self.rr)
+        self.active.fill_(1)
+        self.best_rel.fill_(float("inf"))
+        self._chk(lib.fbme_pcg_check(self.stream, lanes, _ptr(self.rr), _ptr(self.rhs_rr), _ptr(self.active),
+                                     _ptr(self.best_rel), _ptr(self.improved), float(tol), 0), "check")
+        if fixed > 0:
+            self._chk(lib.fbme_pcg_check(self.stream, lanes, _ptr(self.rr), _ptr(self.rhs_rr),
+                                         _ptr(torch.ones_like(self.active)), _ptr(self.best_rel),
+                                         _ptr(self.improved), float(tol), 1), "check")
+            self.best_x.copy_(self.x)
+        self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z), _ptr(self.rho),
+                                         _ptr(self.rho), _ptr(self.active), 1), "direction")
+        limit = fixed if fixed > 0 else cfg.cg_max_iter
+        it = 0
+        for it in range(1, limit + 1):
+            self.matvec(self.d, self.q)
+            self.lane_dot(self.d, self.q, self.curv)
+            self._chk(lib.fbme_pcg_update(self.stream, lanes, m, _ptr(self.x), _ptr(self.res), _ptr(self.d),
+                                          _ptr(self.q), _ptr(self.rho), _ptr(self.curv), _ptr(self.active)),
+                      "update")
+            if it % cfg.cg_check_interval == 0 or it == limit:
+                self.matvec(self.x, self.q)
+                self._chk(lib.fbme_sub(self.stream, self.x.numel(), _ptr(self.rhs), _ptr(self.q),
+                                       _ptr(self.res)), "sub")
+                self.lane_dot(self.res, self.res, self.rr)
+                self._chk(lib.fbme_pcg_check(self.stream, lanes, _ptr(self.rr), _ptr(self.rhs_rr),
+                                             _ptr(self.active), _ptr(self.best_rel), _ptr(self.improved),
+                                             float(tol), 1 if fixed > 0 else 0), "check")
+                if fixed > 0:
+                    self._chk(lib.fbme_masked_copy(self.stream, lanes, m, _ptr(self.best_x), _ptr(self.x),
+                                                   _ptr(self.improved)), "masked_copy")
+                self.precondition(self.res, self.z)
+                self.lane_dot(self.res, self.z, self.nrho)
+                self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z),
+                                                 _ptr(self.rho), _ptr(self.nrho), _ptr(self.active), 1),
+                          "direction")
+                if fixed == 0 and not bool(self.active.any()):
+                    break
+                continue
+            self.precondition(self.res, self.z)
+            self.lane_dot(self.res, self.z, self.nrho)
+            self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z), _ptr(self.rho),
+                                             _ptr(self.nrho), _ptr(self.active), 0), "direction")
+        if fixed > 0:
+            self.x.copy_(self.best_x)
+        self.have_solution = True
+        return it
+
+    def posterior(self, tol: float) -> int:
+        """CG posterior (solver.py:2991-3036) or the reduced active-set
+        posterior; leaves mean, var, dd and rel_sum."""
+        if self.support is not None:
+            t = self.t
+            gsum = self.sums[:, 0].contiguous()
+            self._chk(self.lib.fbme_reduced_posterior(
+                self.stream, self.b, self.support.shape[1], *self.grid, *self.samples, _ptr(self.support),
+                _ptr(self.gamma), _ptr(gsum), _ptr(self.noise), _ptr(self.bp), _ptr(self.ye), _ptr(t.dir1),
+                _ptr(t.dir2), float(self.cfg.cuda_cholesky_jitter_multiplier), _ptr(self.mean),
+                _ptr(self.var), _ptr(self.dd), _ptr(self.status)), "reduced_posterior")
+            self._chk(self.lib.fbme_relevance_sum(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.dd),
+                                                  _ptr(self.grid_part), _ptr(self.rel_sum)), "relevance")
+            return 0
+        self.setup_operators()
+        iters = self.pcg(tol)
+        self._adjoint(self.x.view(self.lanes, self.m), self.lifted.view(self.lanes, self.n))
+        self._chk(self.lib.fbme_hutchinson(self.stream, self.b, self.n, self.p, _ptr(self.lifted),
+                                           _ptr(self.lprobes), _ptr(self.gamma), _ptr(self.mean), _ptr(self.var),
+                                           _ptr(self.dd), _ptr(self.grid_part), _ptr(self.rel_sum)), "hutchinson")
+        return iters
+
+    def em(self, update=True):
+        cfg = self.cfg
+        keep = max(cfg.damping, cfg.cg_damping)
+        self._chk(self.lib.fbme_cg_em(self.stream, self.b, self.n, self.m, _ptr(self.gamma), _ptr(self.mean),
+                                      _ptr(self.dd), _ptr(self.rel_sum), _ptr(self.floor), _ptr(self.mask),
+                                      float(cfg.hyper_shape), float(cfg.hyper_rate), float(cfg.max_precision),
+                                      float(keep), 1 if update else 0, _ptr(self.grid_part), _ptr(self.dof),
+                                      _ptr(self.dof_f)), "cg_em")
+        if not update:
+            return
+        self._forward(self.mean, self.pred)
+        self._chk(self.lib.fbme_uamp_noise_update(
+            self.stream, self.b, self.m, _ptr(self.y), _ptr(self.pred), _ptr(self.noise), _ptr(self.dof_f),
+            _ptr(self.floor), _ptr(self.noise_new), float(cfg.noise_shape), float(cfg.noise_rate), float(keep),
+            1 if cfg.noise_update == "evidence" else 0, float(F32_EPS), _ptr(self.noise_part)), "noise")
+        self.noise.copy_(self.noise_new)
+
+    def try_active_set(self):
+        """Switch to the exact reduced posterior when the 32 strongest
+        variances explain the tail fraction (solver.py:3184-3216)."""
+        cfg = self.cfg
+        self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
+                                       _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
+        total = float(self.sums[0, 0])
+        if not np.isfinite(total):
+            return
+        top = self.top_val[0].double().cpu().numpy()
+        target = (1.0 - cfg.active_set_tail_fraction) * total
+        if top.sum() < target:
+            return
+        cum = np.cumsum(top)
+        required = int(np.searchsorted(cum, target)) + 1
+        size = min(RANK, max(cfg.active_set_min_size, required))
+        if size > RANK:
+            return
+        self.support = self.top_idx[:, :size].contiguous()
+        self.mask = torch.zeros((self.b, self.n), dtype=torch.uint8, device=self.dev)
+        self.mask[0, self.support[0].long()] = 1
+        self.have_solution = False
+
+
+def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
+           probes_override=None) -> SBLResult:
+    """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64)."""
+    _check_supported(cfg)
+    with torch.cuda.device(y.device):
+        return _run(y, grid, cfg, batch_shape, probes_override)
+
+
+def _run(y, grid, cfg, batch_shape, probes_override):
+    s = _Solver(y, grid, cfg, probes_override)
+    s.init()
+    min_iters = max(cfg.min_iter, cfg.cg_min_iter)
+    convergence = cfg.tol > 0.0 and min_iters < cfg.max_iter
+    tol_floor = 3.0 * F32_EPS ** 0.5
+    history = {k: [] for k in ("relative_change", "noise_variance", "effective_dof", "cg_iterations",
+                               "hutchinson_probes", "cg_tolerance", "active_set_size",
+                               "spatial_vertical_coupling", "spatial_horizontal_coupling")}
+    streak = 0
+    converged = False
+    change = float("inf")
+    it = 0
+    for it in range(1, cfg.max_iter + 1):
+        if (cfg.active_set and s.b == 1 and s.support is None and it >= cfg.active_set_start
+                and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
+            s.try_active_set()
+        tol = max(_schedule(it, cfg), tol_floor)
+        iters = s.posterior(tol)
+        s.em(update=True)
+        s.grid_sums(with_mean=True)
+        check = (convergence and it >= min_iters
+                 and ((it - min_iters) % cfg.convergence_check_interval == 0 or it == cfg.max_iter))
+        need = cfg.return_history or check or cfg.verbose or it == cfg.max_iter
+        if need:
+            sums = s.sums.cpu().numpy()
+            change = float(np.max(np.sqrt(sums[:, 1]) / np.maximum(np.sqrt(sums[:, 2]), F32_EPS)))
+        if cfg.return_history:
+            history["relative_change"].append(change)
+            history["noise_variance"].append(float(s.noise.mean()))
+            history["effective_dof"].append(float(s.dof.mean()))
+            history["cg_iterations"].append(float(iters))
+            history["hutchinson_probes"].append(float(s.p))
+            history["cg_tolerance"].append(float(tol))
+            history["active_set_size"].append(float(0 if s.support is None else s.support.shape[1]))
+            history["spatial_vertical_coupling"].append(float(cfg.spatial_coupling))
+            history["spatial_horizontal_coupling"].append(float(cfg.spatial_coupling))
+        s.prev.copy_(s.mean)
+        if check:
+            streak = streak + 1 if change <= cfg.tol else 0
+            if streak >= cfg.convergence_patience:
+                converged = True
+                break
+    final_tol = max(cfg.cg_tolerance, tol_floor)
+    s.posterior(final_tol)
+    s.em(update=False)
+    precision = torch.empty_like(s.gamma)
+    _lib.check(s.lib.fbme_reciprocal(s.stream, s.gamma.numel(), _ptr(s.gamma), _ptr(precision)), "reciprocal")
+    bshape = tuple(batch_shape)
+    oshape = (*bshape, *s.grid)
+    rel = change if (cfg.return_history or cfg.tol > 0.0 or cfg.verbose
+                     or cfg.eager_change_tracking) else float("nan")
+    return SBLResult(
+        x=s.mean.reshape(oshape).clone(),
+        posterior_variance=s.var.reshape(oshape).clone(),
+        precision=precision.reshape(oshape),
+        noise_variance=s.noise.reshape(bshape).clone(),
+        effective_dof=s.dof.float().reshape(bshape),
+        iterations=it,
+        converged=converged,
+        relative_change=rel,
+        mode="cg",
+        arithmetic_dtype="complex64",
+        uamp_backend_used=None,
+        active_set_size=0 if s.support is None else int(s.support.shape[1]),
+        history=history if cfg.return_history else {},
+    )

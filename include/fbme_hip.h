@@ -169,6 +169,112 @@ int fbme_uamp_noise_update(void* stream, int batch, int m, const void* data, con
                            float* output, float noise_shape, float noise_rate, float damping, int evidence_update,
                            float epsilon, double* partial);
 
+/* ------------------------------------------------------------------------
+ * General solver, mode="cg": covariance-free PCG + Hutchinson
+ * (torch_sbl/solver.py:1315-1904 and 2023-2080).  Vectors of the 1 + P
+ * right-hand sides of scene b are lanes b * R + r, each M complex64
+ * contiguous.  Workspace sizes are stated per call; every pointer is device
+ * memory unless noted; all calls are asynchronous on `stream`.
+ * ------------------------------------------------------------------------ */
+
+/* Radix digit code of an FFT length (factors 2, 3, 4, 5, 8), or 0xFFFFFFFF
+ * when the length has another prime factor or exceeds 1024.  Host only. */
+unsigned fbme_radix_code(int n);
+/* Column FFT of [batch, rows_in (zero padded to n), cols] -> first rows_out
+ * rows of the length-n (inverse when `inverse`) transform. */
+int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,
+                    int inverse);
+/* Host table out[2 * n] (double re, im): D[d] = sum_{a < m} exp(2 pi i a d / n),
+ * the Dirichlet kernel of the Gram closed form (operators.py:444-467 columns,
+ * Gram of the prefix rows). */
+int fbme_dirichlet_table(int m, int n, void* out);
+/* exp(-2 pi i k / len), k < len <= 1024, complex64 (FFT twiddles). */
+int fbme_twiddle_table(void* stream, int len, void* out);
+/* BTTB covariance spectrum of Q = noise I + A Gamma A^H on the circulant
+ * embedding E1 x E2 (operators.py:299-326).  work_a, work_b: batch *
+ * max(N1 N2, E1 E2) complex64 each; spectrum: batch * E1 * E2. */
+int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2, int e1, int e2, unsigned code1,
+                        unsigned code2, unsigned codee1, unsigned codee2, const float* gamma, const float* noise,
+                        void* work_a, void* work_b, void* spectrum, const void* tw_n1, const void* tw_n2,
+                        const void* tw_e1, const void* tw_e2);
+/* out = Q d for `lanes` lanes, `rhs` lanes per scene (operators.py:328-355).
+ * work_a, work_b: lanes * M1 * E2 complex64 each. */
+int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
+                         unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
+                         void* work_b, const void* tw_e1, const void* tw_e2);
+/* 32 largest values of each x[b, :n] sorted descending, ties to the lower
+ * index (the stable argsort of solver.py:5797-5800).  cand: batch *
+ * ceil(n / 1024) * 32 uint64; requires n <= 65536. */
+int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val);
+/* Low-rank preconditioner (_make_low_rank_preconditioner,
+ * solver.py:5786-5833): factors [batch, M, 32] = columns * sqrt(gamma_k),
+ * sinv [batch, 32, 32] = (I + F^H F / bg)^-1, bg = max(noise + omitted,
+ * eps * sum gamma).  dir1, dir2: fbme_dirichlet_table(M1, N1), (M2, N2) on
+ * the device.  status[b] = 1 when the Cholesky factorisation fails. */
+int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx, const float* val,
+                       const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,
+                       void* factors, void* sinv, float* bg, int* status);
+/* out = v / bg - F S^-1 F^H v / bg^2 (_apply_low_rank_preconditioner,
+ * solver.py:5836-5848).  part: lanes * ceil(M / 1024) * 64 doubles; corr:
+ * lanes * 32 complex64. */
+int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m, const void* factors, const void* sinv,
+                       const float* bg, const void* v, void* out, void* part, void* corr);
+/* out[l] = sum Re(conj(a_l) b_l) in FP64 (the PCG inner products of
+ * _pcg, solver.py:5582-5723).  part: lanes * 64 doubles. */
+int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out);
+/* _pcg step (solver.py:5644-5662): valid = active & finite & curv, rho >
+ * tiny; x += alpha d, r -= alpha q; active = valid. */
+int fbme_pcg_update(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                    const double* rho, const double* curv, int* active);
+/* _pcg direction (solver.py:5692-5719): d = z + beta d (restart: d = z),
+ * rho = nrho and active = active & nrho > tiny. */
+int fbme_pcg_direction(void* stream, int lanes, int m, void* d, const void* z, double* rho, const double* nrho,
+                       int* active, int restart);
+/* out = a - b over n complex64 (true residual, solver.py:5663-5671). */
+int fbme_sub(void* stream, long long n, const void* a, const void* b, void* out);
+/* _pcg residual test (solver.py:5672-5690): rel = sqrt(rr) / max(sqrt(init),
+ * eps); active = rel > tol (fixed budget: active &= rel > tol and the best
+ * relative residual is tracked in best_rel / improved). */
+int fbme_pcg_check(void* stream, int lanes, const double* rr, const double* init, int* active, double* best_rel,
+                   int* improved, float tol, int fixed);
+/* dst_l = src_l where mask[l] (best iterate of a fixed budget). */
+int fbme_masked_copy(void* stream, int lanes, int m, void* dst, const void* src, const int* mask);
+/* Hutchinson posterior (_posterior cg branch, solver.py:2991-3036):
+ * dd = max(mean_p Re(lifted_p conj(A^H p)), 0), mean = gamma lifted_0,
+ * var = max(gamma - gamma^2 dd, 0), rel_sum[b] = sum clamp(gamma dd, 0, 1).
+ * part: batch * ceil(N / 256) doubles. */
+int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
+                    const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum);
+/* Relevance projection (_project_relevance, solver.py:3156-3181), MacKay
+ * update (1669-1677) and damping with floor (1776-1798); mask (may be NULL)
+ * keeps gamma outside an active set.  dof[b] = projected relevance sum. */
+int fbme_cg_em(void* stream, int batch, int n, int m, float* gamma, const void* mean, const float* dd,
+               const double* rel_sum, const float* floor_, const unsigned char* mask, float hyper_shape,
+               float hyper_rate, float max_prec, float keep, int update, double* part, double* dof,
+               float* dof_f);
+/* Initial power, floor, noise and backprojection gamma
+ * (solver.py:1063-1089, 1151-1156); ye = ||y||^2, bpe = ||A^H y||^2. */
+int fbme_cg_init(void* stream, int batch, int n, int m, const void* bp, const double* ye, const double* bpe,
+                 float min_variance, float noise_fraction, float blend, float* power, float* floor_, float* noise,
+                 float* gamma);
+/* out[b] = sum clamp(gamma dd, 0, 1) (solver.py:3156-3164). */
+int fbme_relevance_sum(void* stream, int batch, int n, const float* gamma, const float* dd, double* part,
+                       double* out);
+/* out = 1 / in (precision output, solver.py:2089). */
+int fbme_reciprocal(void* stream, long long n, const float* in, float* out);
+/* out[b] = (sum gamma, ||mean - prev||^2, ||mean||^2) in FP64 (relative
+ * change, solver.py:1813-1826); mean and prev may be NULL. */
+int fbme_grid_sums(void* stream, int batch, int n, const float* gamma, const void* mean, const void* prev,
+                   double* part, double* out);
+/* Reduced active-set posterior with diffuse-tail loading nu = noise +
+ * omitted variance (_active_posterior, solver.py:3897-3924 with
+ * 3591-3779), K <= 32: mean, var, dd on the support; elsewhere mean = 0,
+ * var = gamma, dd = 0. */
+int fbme_reduced_posterior(void* stream, int batch, int k, int n1, int n2, int m1, int m2, const int* support,
+                           const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                           const double* ye, const void* dir1, const void* dir2, float jitter_multiplier,
+                           void* mean, float* var, float* dd, int* status);
+
 #ifdef __cplusplus
 }
 #endif

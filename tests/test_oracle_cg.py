@@ -1,0 +1,131 @@
+"""Pins of the float64 CG oracle (oracle/cg_oracle.py).
+
+Each piece is checked against an independent computation: the Toeplitz
+operator against the dense Q = noise I + A Gamma A^H, PCG against a dense
+solve, the low-rank preconditioner against the dense Woodbury inverse, the
+Hutchinson estimate against the exact diagonal (unbiasedness over many
+probes), and the whole solver against the reference's own golden run
+(tests/golden/general_32x32_24x24.npz, written by
+tests/golden/make_golden.py from torch_sbl.sbl_2d(mode='cg') on CPU).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from conftest import golden_path
+
+from oracle import cg_oracle as co
+from oracle.fbme_oracle import F32_EPS, F32_TINY, dense_prefix_matrix, fourier_columns
+
+
+def _problem(seed=0, grid=(12, 10), samples=(7, 6), batch=2):
+    rng = np.random.default_rng(seed)
+    n, m = grid[0] * grid[1], samples[0] * samples[1]
+    gamma = rng.gamma(0.5, 1.0, (batch, n))
+    noise = rng.uniform(0.05, 0.2, batch)
+    a = dense_prefix_matrix(grid, samples)
+    return rng, gamma, noise, a, co.Operator(grid, samples)
+
+
+def test_operator_matches_dense_covariance():
+    rng, gamma, noise, a, op = _problem()
+    d = rng.standard_normal((2, op.m, 3)) + 1j * rng.standard_normal((2, op.m, 3))
+    got = op.q(d, gamma, noise)
+    for b in range(2):
+        q = noise[b] * np.eye(op.m) + (a * gamma[b]) @ a.conj().T
+        np.testing.assert_allclose(got[b], q @ d[b], rtol=1e-12, atol=1e-12)
+
+
+def test_preconditioner_is_dense_woodbury_inverse():
+    rng, gamma, noise, a, op = _problem(1)
+    bg, factors, inv = co.low_rank_preconditioner(gamma, noise, op, 8, 0.0, F32_TINY)
+    v = rng.standard_normal((2, op.m, 2)) + 1j * rng.standard_normal((2, op.m, 2))
+    got = co.apply_preconditioner(v, (bg, factors, inv))
+    for b in range(2):
+        order = np.argsort(-gamma[b], kind="stable")[:8]
+        cols = a[:, order]
+        approx = bg[b] * np.eye(op.m) + (cols * gamma[b, order]) @ cols.conj().T
+        np.testing.assert_allclose(got[b], np.linalg.solve(approx, v[b]), rtol=1e-9, atol=1e-11)
+        omitted = gamma[b].sum() - gamma[b, order].sum()
+        assert bg[b] == pytest.approx(noise[b] + omitted, rel=1e-12)
+
+
+def test_fourier_factor_columns_are_dense_columns():
+    _, gamma, _, a, op = _problem(2)
+    idx = np.array([[0, 5, 17, 119]])
+    np.testing.assert_allclose(fourier_columns(idx, op.grid, op.samples)[0], a[:, idx[0]], atol=1e-12)
+
+
+@pytest.mark.parametrize("fixed", [0, 12])
+def test_pcg_solves_dense_system(fixed):
+    rng, gamma, noise, a, op = _problem(3)
+    rhs = rng.standard_normal((2, op.m, 4)) + 1j * rng.standard_normal((2, op.m, 4))
+    x, it = co.pcg(rhs, gamma, noise, op, x0=None, tol=1e-10, max_iter=200, check=8, rank=8,
+                   fixed=fixed, eps=F32_EPS, tiny=F32_TINY)
+    for b in range(2):
+        q = noise[b] * np.eye(op.m) + (a * gamma[b]) @ a.conj().T
+        exact = np.linalg.solve(q, rhs[b])
+        err = np.linalg.norm(x[b] - exact) / np.linalg.norm(exact)
+        assert err < (1e-8 if fixed == 0 else 1e-2)
+    if fixed:
+        assert it == fixed
+
+
+def test_pcg_warm_start_from_solution_stops_at_first_check():
+    rng, gamma, noise, a, op = _problem(4)
+    rhs = rng.standard_normal((2, op.m, 2)) + 1j * rng.standard_normal((2, op.m, 2))
+    x, _ = co.pcg(rhs, gamma, noise, op, x0=None, tol=1e-12, max_iter=300, check=8, rank=8)
+    _, it = co.pcg(rhs, gamma, noise, op, x0=x, tol=1e-6, max_iter=300, check=8, rank=8)
+    assert it == 8
+
+
+def test_hutchinson_is_unbiased_for_posterior_diagonal():
+    """mean_p Re(A^H Q^-1 p . conj(A^H p)) -> diag(A^H Q^-1 A) as P grows."""
+    rng, gamma, noise, a, op = _problem(5, batch=1)
+    q = noise[0] * np.eye(op.m) + (a * gamma[0]) @ a.conj().T
+    exact = np.real(np.sum(a.conj() * np.linalg.solve(q, a), 0))
+    count = 4000
+    probes = rng.choice([-1.0, 1.0], (1, op.m, count)).astype(np.complex128)
+    sol = np.linalg.solve(q, probes[0])[None]
+    est = np.mean(np.real(op.adj(sol) * np.conj(op.adj(probes))), 2)[0]
+    # Rademacher estimator variance ~ sum_j |off-diagonal|^2 / P
+    assert np.max(np.abs(est - exact) / exact) < 0.15
+    assert abs(est.mean() / exact.mean() - 1) < 0.02
+
+
+def test_probe_count_rule():
+    cfg = co.CGConfig()
+    assert co.probe_count(1024, 576, cfg) == 32               # 8 sqrt(1.78) = 10.7 -> 32
+    assert co.probe_count(65536, 1024, cfg) == 128             # 2 * 64 = 128
+    assert co.probe_count(65536, 25600, cfg) == 32
+    assert co.probe_count(65536, 2048, cfg) == 64              # max(8 sqrt 32, 2 * 32)
+    cfg.auto_probe_scaling = False
+    assert co.probe_count(65536, 1024, cfg) == 32
+
+
+def test_rademacher_probes_are_signs():
+    p = co.rademacher_probes(2, 50, 8, 0)
+    assert p.shape == (2, 50, 8)
+    assert set(np.unique(p)) == {-1.0, 1.0}
+    np.testing.assert_array_equal(p, co.rademacher_probes(2, 50, 8, 0))
+
+
+def test_schedule():
+    cfg = co.CGConfig()
+    assert co.cg_schedule(1, cfg) == 1e-2
+    assert co.cg_schedule(6, cfg) == pytest.approx(np.sqrt(1e-7))
+    assert co.cg_schedule(16, cfg) == 1e-5
+
+
+def test_sbl_cg_matches_reference_golden_run():
+    g = np.load(golden_path("general_32x32_24x24.npz"))
+    out = co.sbl_cg(g["y"], (32, 32))
+    assert out["iterations"] == int(g["cg_iterations"])
+    assert out["active_set_size"] == int(g["cg_active_set_size"])
+    ref = g["cg_x"].reshape(-1)
+    rel = np.linalg.norm(out["x"].reshape(-1) - ref) / np.linalg.norm(ref)
+    assert rel < 1e-4
+    assert out["noise_variance"] == pytest.approx(float(g["cg_noise_variance"]), rel=1e-3)
+    assert out["effective_dof"] == pytest.approx(float(g["cg_effective_dof"]), rel=1e-3)
+    np.testing.assert_allclose(out["cg_iterations"][:29], g["cg_history_cg_iterations"][:29], atol=8)
