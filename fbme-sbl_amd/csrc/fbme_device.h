@@ -206,11 +206,15 @@ __device__ __forceinline__ void fft_lines(float2* buf, int len, int nlines, int 
 
 // ----------------------------------------------------------------- reductions
 // Deterministic block sum of one double per thread (fixed shuffle/LDS tree).
+// Sum over the 64 lanes; the result is valid in lane 0 only.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
+
+// Sum over the 64 lanes, broadcast to every lane (same value as wave_sum).
+__device__ __forceinline__ double wave_sum_all(double v) { return __shfl(wave_sum(v), 0, 64); }
 
 __device__ __forceinline__ double block_sum(double v, double* scratch /* >= 4 */) {
   v = wave_sum(v);

@@ -204,7 +204,7 @@ k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const dou
     sq[lane] = sqrt(fmax((double)val[b * RANK + lane], 0.0));
     vsum = (double)val[b * RANK + lane];
   }
-  vsum = wave_sum(vsum);
+  vsum = wave_sum_all(vsum);
   const double total = sums[b];
   const double omitted = fmax(total - vsum, 0.0);
   const double bg = fmax((double)noise[b] + omitted, F32_EPS_D * total);
@@ -234,7 +234,7 @@ k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const dou
   // mean |diag| of the symmetrised system (read from LDS: row[lane] would
   // need a dynamic register index)
   double md = (lane < RANK) ? fabs(a[lane][lane].x) : 0.0;
-  md = wave_sum(md) / RANK;
+  md = wave_sum_all(md) / RANK;
   const double jit = 10.0 * F32_EPS_D * fmax(md, F32_TINY_D);
   __syncthreads();
   int fail = 0;

@@ -38,8 +38,8 @@ def sources():
 def compile_one(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
-    if src.endswith(".hip"):
-        cmd[1:1] = ["-x", "hip", *DEVICE]
+    # hipcc compiles .cpp as HIP as well: give every file the same target.
+    cmd[1:1] = ["-x", "hip", *DEVICE] if src.endswith(".hip") else DEVICE
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{res.stderr}")

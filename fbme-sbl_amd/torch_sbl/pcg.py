@@ -152,6 +152,9 @@ class _Solver:
         self.grid = tuple(grid)
         self.n = prod(self.grid)
         self.m = prod(self.samples)
+        if min(self.n, self.m) < RANK:
+            raise NotImplementedError("mode='cg' needs at least 32 samples and grid cells "
+                                      "(rank-32 preconditioner)")
         self.t = _tables(self.grid, self.samples, self.dev)
         self.p = _probe_count(self.n, self.m, cfg)
         self.r = 1 + self.p

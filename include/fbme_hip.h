@@ -187,7 +187,7 @@ int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, i
 /* Host table out[2 * n] (double re, im): D[d] = sum_{a < m} exp(2 pi i a d / n),
  * the Dirichlet kernel of the Gram closed form (operators.py:444-467 columns,
  * Gram of the prefix rows). */
-int fbme_dirichlet_table(int m, int n, void* out);
+int fbme_dirichlet_table(int m, int n, double* out);
 /* exp(-2 pi i k / len), k < len <= 1024, complex64 (FFT twiddles). */
 int fbme_twiddle_table(void* stream, int len, void* out);
 /* BTTB covariance spectrum of Q = noise I + A Gamma A^H on the circulant
