@@ -61,7 +61,11 @@ struct fbme_plan {
   int max_iters = 0;
   std::vector<FbmeScene> scenes;            // sorted by iteration budget (desc)
   std::vector<FbmeWork> col_work, row_work, tile_work;
+  // Work lists hold the LDS-Stockham items first and the register 16 x 16
+  // items (scenes with a 256-point pass axis) after col_split / row_split.
+  int col_split = 0, row_split = 0;
   std::vector<int> col_prefix, row_prefix, scene_prefix;  // per UAMP step t (index t-1)
+  std::vector<int> col16_prefix, row16_prefix;
   std::vector<double> dir;                  // Dirichlet tables, interleaved (re, im)
   std::vector<float> twid;                  // twiddle tables, interleaved (re, im)
   std::vector<long long> grid_off, samp_off; // caller order
@@ -178,6 +182,7 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
   o->noise_shape = 0.0f;
   o->noise_rate = 0.0f;
   o->polish_damping = 0.0f;
+  o->fft_kernel = 0;
 }
 
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
@@ -272,6 +277,28 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
     tw_off.push_back(off);
     return off;
   };
+  // FFT kernel variant of the column (length N1) and row (length N2) passes.
+  if (opt->fft_kernel < 0 || opt->fft_kernel > 2) {
+    delete p;
+    return fail("fft_kernel must be 0 (auto), 1 (LDS Stockham) or 2 (register 16 x 16)");
+  }
+  // Per scene: the register kernels take every pass whose axis has length
+  // 256 (the row pass also needs its four-row items to keep the candidate
+  // merge within one tile); the rest use the LDS Stockham kernels.
+  auto r16_col_ok = [](const FbmeScene& s) { return s.n1 == 256; };
+  auto r16_row_ok = [](const FbmeScene& s) {
+    return s.n2 == 256 && ((s.n1 + FBME_R16_ROWS - 1) / FBME_R16_ROWS) * s.k <= FBME_TILE_MAX;
+  };
+  if (opt->fft_kernel == 2) {
+    for (const FbmeScene& s : p->scenes) {
+      if (!r16_col_ok(s) || !r16_row_ok(s)) {
+        delete p;
+        return fail("fft_kernel 2 needs N1 = N2 = 256 in every scene");
+      }
+    }
+  }
+  const bool use_r16 = opt->fft_kernel != 1;
+  std::vector<FbmeWork> col16, row16;
   // work tables
   int slots = 0, cand = 0, xcand = 0;
   const int exch_cfg = opt->exchange_size;
@@ -281,11 +308,14 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
     s.dir2_off = table(s.m2, s.n2);
     s.tw1_off = twiddle(s.n1);
     s.tw2_off = twiddle(s.n2);
-    const int cl = std::max(1, std::min({s.m2, FBME_TILE_MAX / s.n1, std::max(1, opt->col_lines)}));
+    const bool r16_col = use_r16 && r16_col_ok(s);
+    const bool r16_row = use_r16 && r16_row_ok(s);
+    const int cl = r16_col ? std::min(s.m2, FBME_R16_COLS)
+                           : std::max(1, std::min({s.m2, FBME_TILE_MAX / s.n1, std::max(1, opt->col_lines)}));
     // Row tiles hold rl rows of n2 + 1 (padded) complex values in LDS, so
     // rl * (n2 + 1) <= FBME_TILE_MAX + 64 requires rl <= 64 as well.
     const int rl_max = std::max(1, std::min({s.n1, FBME_TILE_MAX / s.n2, 64}));
-    int rl = std::max(1, std::min(rl_max, std::max(1, opt->row_lines)));
+    int rl = r16_row ? FBME_R16_ROWS : std::max(1, std::min(rl_max, std::max(1, opt->row_lines)));
     // keep the candidate merge within one tile of LDS
     while (((s.n1 + rl - 1) / rl) * s.k > FBME_TILE_MAX && rl < rl_max) ++rl;
     s.col_items = (s.m2 + cl - 1) / cl;
@@ -310,9 +340,9 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
       return fail("grid too large for the exchange candidate merge");
     }
     for (int c = 0; c < s.col_items; ++c)
-      p->col_work.push_back({si, c * cl, std::min(cl, s.m2 - c * cl), c});
+      (r16_col ? col16 : p->col_work).push_back({si, c * cl, std::min(cl, s.m2 - c * cl), c});
     for (int r = 0; r < s.row_items; ++r)
-      p->row_work.push_back({si, r * rl, std::min(rl, s.n1 - r * rl), r});
+      (r16_row ? row16 : p->row_work).push_back({si, r * rl, std::min(rl, s.n1 - r * rl), r});
     for (int t = 0; t < s.xtile_items; ++t) {
       const long long first = (long long)t * FBME_XTILE;
       p->tile_work.push_back({si, (int)first, (int)std::min<long long>(FBME_XTILE, n - first), t});
@@ -324,6 +354,10 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
   for (const FbmeWork& w : p->row_work) max_row = std::max(max_row, w.count * p->scenes[w.scene].n2);
   p->ept_col = ept_for(max_col);
   p->ept_row = ept_for(max_row);
+  p->col_split = (int)p->col_work.size();
+  p->row_split = (int)p->row_work.size();
+  p->col_work.insert(p->col_work.end(), col16.begin(), col16.end());
+  p->row_work.insert(p->row_work.end(), row16.begin(), row16.end());
   p->total_slots = slots;
   p->total_cand = cand;
   p->total_xcand = std::max(xcand, 1);
@@ -332,12 +366,17 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
   for (int t = 1; t <= p->max_iters; ++t) {
     int ns = 0;
     while (ns < nscenes && p->scenes[ns].iters >= t) ++ns;
-    int nc = 0, nr = 0;
-    while (nc < (int)p->col_work.size() && p->col_work[nc].scene < ns) ++nc;
-    while (nr < (int)p->row_work.size() && p->row_work[nr].scene < ns) ++nr;
+    // scenes are sorted, so the live items of each list form a prefix of it
+    auto live = [&](const std::vector<FbmeWork>& v, int lo, int hi) {
+      int n = 0;
+      while (lo + n < hi && v[lo + n].scene < ns) ++n;
+      return n;
+    };
     p->scene_prefix.push_back(ns);
-    p->col_prefix.push_back(nc);
-    p->row_prefix.push_back(nr);
+    p->col_prefix.push_back(live(p->col_work, 0, p->col_split));
+    p->row_prefix.push_back(live(p->row_work, 0, p->row_split));
+    p->col16_prefix.push_back(live(p->col_work, p->col_split, (int)p->col_work.size()));
+    p->row16_prefix.push_back(live(p->row_work, p->row_split, (int)p->row_work.size()));
   }
   // support-mask threshold log(N / budget) is per scene in general; the
   // locked plans use one grid, uneven batches get the mask per scene shape
@@ -476,27 +515,51 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
     rc = (x);                  \
     if (rc) return FBME_ERR_HIP; \
   } while (0)
+  // Each pass launches its LDS-Stockham list and its register 16 x 16 list.
+  const FbmeWork* colw16 = colw + p->col_split;
+  const FbmeWork* roww16 = roww + p->row_split;
+  const int ncol16 = ncol - p->col_split, nrow16 = nrow - p->row_split;
+  auto col_adjoint = [&](int ng, int n16, int mode, int store) {
+    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, pred, msg, mid, sc, part, ps, mode,
+                                    store, o.uamp_damping, tw, st);
+    if (!r)
+      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, pred, msg, mid, sc, part, ps, mode,
+                                  store, o.uamp_damping, tw, st);
+    return r;
+  };
+  auto row = [&](int ng, int n16, int mode, uint32_t* key, int* idx, int store) {
+    int r = fbme_launch_row(scenes, roww, ng, ng, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, mode,
+                            o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, tw, st);
+    if (!r)
+      r = fbme_launch_row(scenes, roww16, n16, n16, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps, mode,
+                          o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, tw, st);
+    return r;
+  };
+  auto col_forward = [&](int ng, int n16, int first) {
+    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, pred, msg, sc, part, ps, first,
+                                    o.uamp_damping, tw, st);
+    if (!r)
+      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, pred, msg, sc, part, ps, first,
+                                  o.uamp_damping, tw, st);
+    return r;
+  };
   // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
-  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 0, 0, o.uamp_damping, tw, st));
-  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 0, o.uamp_damping, o.damping,
-                           o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, tw, st));
+  FBME_TRY(col_adjoint(p->col_split, ncol16, 0, 0));
+  FBME_TRY(row(p->row_split, nrow16, 0, nullptr, nullptr, 0));
   FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction, o.blend,
                                o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
   for (int t = 1; t <= p->max_iters; ++t) {
     const int nc = p->col_prefix[t - 1], nr = p->row_prefix[t - 1], ns = p->scene_prefix[t - 1];
-    if (t > 1)
-      FBME_TRY(fbme_launch_col_adjoint(scenes, colw, nc, nc, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 1, o.uamp_damping, tw, st));
-    FBME_TRY(fbme_launch_row(scenes, roww, nr, nr, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, t == 1 ? 1 : 2,
-                             o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, nullptr, nullptr, 0, tw, st));
-    FBME_TRY(fbme_launch_col_forward(scenes, colw, nc, nc, p->ept_col, mid, io->y, pred, msg, sc, part, ps, t == 1 ? 1 : 0,
-                                     o.uamp_damping, tw, st));
+    const int nc16 = p->col16_prefix[t - 1], nr16 = p->row16_prefix[t - 1];
+    if (t > 1) FBME_TRY(col_adjoint(nc, nc16, 1, 1));
+    FBME_TRY(row(nr, nr16, t == 1 ? 1 : 2, nullptr, nullptr, 0));
+    FBME_TRY(col_forward(nc, nc16, t == 1 ? 1 : 0));
     FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, 1, nullptr, o.min_variance, o.noise_fraction, o.blend, o.damping,
                                  o.noise_shape, o.noise_rate, o.evidence_noise, st));
   }
   // Final consistency E-step (no prediction) + per-tile top-k candidates.
-  FBME_TRY(fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 0, o.uamp_damping, tw, st));
-  FBME_TRY(fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 3, o.uamp_damping, o.damping,
-                           o.max_precision, hs, hr, o.blend, ckey, cidx, io->store_uamp ? 1 : 0, tw, st));
+  FBME_TRY(col_adjoint(p->col_split, ncol16, 1, 0));
+  FBME_TRY(row(p->row_split, nrow16, 3, ckey, cidx, io->store_uamp ? 1 : 0));
   // Polish.
   FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
                               io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
@@ -533,17 +596,34 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   float* gamma = (float*)(w + p->r_gamma.off);
   void* mean = w + p->r_mean.off;
   float* var = (float*)(w + p->r_var.off);
+  const FbmeWork* colw16 = colw + p->col_split;
+  const FbmeWork* roww16 = roww + p->row_split;
+  const int ng_col = p->col_split, n16_col = ncol - p->col_split;
+  const int ng_row = p->row_split, n16_row = nrow - p->row_split;
   switch (stage) {
     case FBME_STAGE_COL_ADJOINT:
-      return fbme_launch_col_adjoint(scenes, colw, ncol, ncol, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1, 1,
-                                     o.uamp_damping, tw, st) ? FBME_ERR_HIP : 0;
+      return (fbme_launch_col_adjoint(scenes, colw, ng_col, ng_col, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1,
+                                      1, o.uamp_damping, tw, st) ||
+              fbme_launch_col_adjoint(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, io->y, pred, msg, mid, sc, part,
+                                      ps, 1, 1, o.uamp_damping, tw, st))
+                 ? FBME_ERR_HIP
+                 : 0;
     case FBME_STAGE_ROW:
-      return fbme_launch_row(scenes, roww, nrow, nrow, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 2, o.uamp_damping,
-                             o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr, nullptr, 0, tw,
-                             st) ? FBME_ERR_HIP : 0;
+      return (fbme_launch_row(scenes, roww, ng_row, ng_row, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 2,
+                              o.uamp_damping, o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr,
+                              nullptr, 0, tw, st) ||
+              fbme_launch_row(scenes, roww16, n16_row, n16_row, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps, 2,
+                              o.uamp_damping, o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr,
+                              nullptr, 0, tw, st))
+                 ? FBME_ERR_HIP
+                 : 0;
     case FBME_STAGE_COL_FORWARD:
-      return fbme_launch_col_forward(scenes, colw, ncol, ncol, p->ept_col, mid, io->y, pred, msg, sc, part, ps, 0, o.uamp_damping, tw,
-                                     st) ? FBME_ERR_HIP : 0;
+      return (fbme_launch_col_forward(scenes, colw, ng_col, ng_col, p->ept_col, mid, io->y, pred, msg, sc, part, ps, 0,
+                                      o.uamp_damping, tw, st) ||
+              fbme_launch_col_forward(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, mid, io->y, pred, msg, sc, part,
+                                      ps, 0, o.uamp_damping, tw, st))
+                 ? FBME_ERR_HIP
+                 : 0;
     case FBME_STAGE_OUTPUT:
       return fbme_launch_output(scenes, tilew, ntile, ntile, sc, w + p->r_result.off, io->x, io->posterior_variance,
                                 io->precision, io->support_mask, o.false_alarm,

@@ -29,6 +29,14 @@ struct FbmeScene {
   int pad;
 };
 
+// Kernel variant carried in the `ept` argument of the pass launchers: 4, 8 or
+// 16 select the LDS Stockham kernels with that many elements per thread;
+// FBME_EPT_R16 selects the register-resident 16 x 16 FFT kernels (transform
+// length 256 along the pass axis), FBME_R16_COLS columns per column workgroup.
+#define FBME_EPT_R16 256
+#define FBME_R16_COLS 16
+#define FBME_R16_ROWS 4
+
 // One unit of work of a pass: `count` consecutive lines (columns for the
 // column passes, rows for the row passes) of scene `scene`, starting at
 // `first`.  `slot` is the scene-local partial-sum slot.

@@ -444,11 +444,22 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
 
 extern "C" {
 
+int fbme_launch_row16(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*, const double*,
+                      double*, int, int, float, float, float, float, float, float, uint32_t*, int*, int, const void*,
+                      hipStream_t);
+int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
+                              const double*, double*, int, int, int, float, const void*, hipStream_t);
+int fbme_launch_col16_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
+                              const double*, double*, int, int, float, const void*, hipStream_t);
+
 int fbme_launch_col_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
                             const void* y, const void* pred, void* msg, void* mid, const double* sc,
                             double* part, int pstride, int mode, int store_msg, float damping,
                             const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
+  if (ept == FBME_EPT_R16)
+    return fbme_launch_col16_adjoint(scenes, work, nwork, grid, y, pred, msg, mid, sc, part, pstride, mode, store_msg,
+                                     damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     if (mode == 0)
       hipLaunchKernelGGL((k_col_adjoint<E, 0>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
@@ -468,6 +479,10 @@ int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, in
                     float hyper_rate, float blend, uint32_t* cand_key, int* cand_idx, int store_final,
                     const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
+  if (ept == FBME_EPT_R16)
+    return fbme_launch_row16(scenes, work, nwork, grid, mid, bp, gamma, mean, var, sc, part, pstride, mode, damping,
+                             prior_damping, max_precision, hyper_shape, hyper_rate, blend, cand_key, cand_idx,
+                             store_final, twtab, stream);
   UampArgs a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
 #define FBME_ROW(M)                                                                                      \
   hipLaunchKernelGGL((k_row<E, M>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,       \
@@ -490,6 +505,9 @@ int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int n
                             double* part, int pstride, int first_message, float damping,
                             const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
+  if (ept == FBME_EPT_R16)
+    return fbme_launch_col16_forward(scenes, work, nwork, grid, mid, y, pred, msg, sc, part, pstride, first_message,
+                                     damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     hipLaunchKernelGGL((k_col_forward<E>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
                        (const float2*)mid, (const float2*)y, (float2*)pred, (float2*)msg, sc, part, pstride,

@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 Q_WORKSPACE_BYTES = 0
 Q_TOTAL_GRID = 1
@@ -45,6 +45,7 @@ class FbmePlanOptions(Structure):
         ("hyper_shape", c_float), ("hyper_rate", c_float),
         ("noise_shape", c_float), ("noise_rate", c_float),
         ("polish_damping", c_float),
+        ("fft_kernel", c_int),
     ]
 
 

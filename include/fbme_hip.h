@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 1
+#define FBME_ABI_VERSION 2
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -56,6 +56,8 @@ typedef struct FbmePlanOptions {
   float noise_shape;       /* 0 noise hyperprior c                         */
   float noise_rate;        /* 0 noise hyperprior d                         */
   float polish_damping;    /* 0 damping of the polish EM updates           */
+  int fft_kernel;          /* 0 auto, 1 LDS Stockham passes, 2 register
+                              16 x 16 passes (requires N1 or N2 = 256)     */
 } FbmePlanOptions;
 
 /* Outputs of one plan replay; per-scene arrays are in the caller's scene

@@ -146,3 +146,52 @@ def test_uneven_batch_equals_single(device):
         assert torch.equal(out.noise[i], o1.noise[0])
         orc = fo.fbme_plan(s.y, s.grid, "stable")
         assert _rel(o1.x.cpu().numpy(), orc["x"]) < 1e-4
+
+
+MIXED = [(256, 256, 160, 160), (128, 256, 64, 200), (256, 128, 180, 100), (256, 256, 140, 256)]
+
+
+@pytest.mark.parametrize("profile", ["fast", "stable"])
+def test_register_fft_passes_match_lds_passes(device, profile):
+    """The register 16 x 16 passes (every 256-point axis) against the LDS
+    Stockham passes on the same batch, and both against the oracle.  The
+    batch mixes scenes where only the rows, only the columns, or both axes
+    have length 256."""
+    from fbme_amd import _lib
+    from fbme_amd.plan import NativePlan
+    batch = [scenes.ellipse_scene(5000 + i, shp[:2], shp[2:], k=60, snr_db=20.0)
+             for i, shp in enumerate(MIXED)]
+    ys = torch.from_numpy(np.concatenate([s.y.reshape(-1) for s in batch])).to(device)
+    outs = {}
+    for kernel in (1, 0):
+        print("fft_kernel", kernel, profile, flush=True)
+        opt = _lib.default_options(profile == "stable")
+        opt.fft_kernel = kernel
+        plan = NativePlan(MIXED, opt, device)
+        o = plan.run(ys)
+        torch.cuda.synchronize()
+        outs[kernel] = (plan.scene_slices(), o.x.cpu().numpy(), o.noise.cpu().numpy(),
+                        o.support_mask.cpu().numpy(), o.status.cpu().numpy())
+    (sl, x1, n1, m1, s1), (_, x0, n0, m0, s0) = outs[1], outs[0]
+    assert not s0.any() and not s1.any()
+    for i, s in enumerate(batch):
+        go = sl[i][0]
+        n = s.grid[0] * s.grid[1]
+        orc = fo.fbme_plan(s.y, s.grid, profile)
+        print("scene", i, MIXED[i], "register vs oracle", _rel(x0[go:go + n], orc["x"]),
+              "lds vs oracle", _rel(x1[go:go + n], orc["x"]), flush=True)
+        assert _rel(x0[go:go + n], x1[go:go + n]) < 1e-4
+        assert _rel(x0[go:go + n], orc["x"]) < 1e-4
+        assert set(np.flatnonzero(x0[go:go + n])) == set(np.flatnonzero(orc["x"]))
+        assert np.array_equal(m0[go:go + n], orc["support_mask"])
+        assert abs(n0[i] / n1[i] - 1) < 1e-4
+
+
+def test_register_fft_is_selected_and_required(device):
+    from fbme_amd import _lib
+    from fbme_amd.plan import NativePlan
+    opt = _lib.default_options(False)
+    opt.fft_kernel = 2
+    NativePlan([(256, 256, 160, 160)], opt, device)
+    with pytest.raises(ValueError):
+        NativePlan([(128, 256, 64, 160)], opt, device)
