@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--profile", choices=("fast", "stable"), default="fast")
     ap.add_argument("--plain-steps", type=int, default=64)
     ap.add_argument("--kernel-reps", type=int, default=20)

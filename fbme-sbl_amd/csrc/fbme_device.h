@@ -398,4 +398,27 @@ __device__ int block_select_topk(const uint32_t* keys, int n, int k, int* out_po
   return k;
 }
 
+// Write the `got` selected tile positions (pos[0..got)) as the tile's K
+// candidates sorted by (key desc, position asc), padded with key 0 / index
+// -1; `base` converts a tile position to a grid index.  Threads < K work.
+__device__ __forceinline__ void write_sorted_candidates(const uint32_t* keys, const int* pos, int got, int k,
+                                                        int base, uint32_t* out_key, int* out_idx) {
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    if (i < got) {
+      const uint32_t ki = keys[pos[i]];
+      const int pi = pos[i];
+      int rank = 0;
+      for (int j = 0; j < got; ++j) {
+        const uint32_t kj = keys[pos[j]];
+        rank += (kj > ki) || (kj == ki && pos[j] < pi);
+      }
+      out_key[rank] = ki;
+      out_idx[rank] = base + pi;
+    } else {
+      out_key[i] = 0u;
+      out_idx[i] = -1;
+    }
+  }
+}
+
 }  // namespace fbme

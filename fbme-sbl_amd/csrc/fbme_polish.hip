@@ -112,69 +112,115 @@ struct PolishLds {
   double2 z[KM];
   double2 mu[KM];
   double dsc[KM];
+  double idsc[KM];    // 1 / dsc
+  double ipiv[KM];    // 1 / diag(L)
+  double iwg[KM];     // 1 / gamma_S
+  int wk1[KM], wk2[KM];  // grid coordinates of the support
   double sig[KM];
   double dd[KM];
   double red[16];
   int flag[4];
 };
 
+// Grid coordinates of the K support atoms (one integer division each, instead
+// of four per Gram entry).
+__device__ __forceinline__ void support_coords(const SceneCtx& c, const int* w, PolishLds& L) {
+  const int tid = threadIdx.x;
+  if (tid < c.k) {
+    const int r1 = w[tid] / c.n2;
+    L.wk1[tid] = r1;
+    L.wk2[tid] = w[tid] - r1 * c.n2;
+  }
+}
+
+// G[r][q] from the Dirichlet tables and the support coordinates.
+__device__ __forceinline__ double2 gram_rq(const SceneCtx& c, const PolishLds& L, int r, int q) {
+  int d1 = L.wk1[r] - L.wk1[q];
+  int d2 = L.wk2[r] - L.wk2[q];
+  d1 += (d1 < 0) ? c.n1 : 0;
+  d2 += (d2 < 0) ? c.n2 : 0;
+  return zmul(c.d1[d1], c.d2[d2]);
+}
+
 // Exact reduced posterior and negative log evidence of the zero-tail model
 // on support w with prior variances wg and noise nu (solver.py:3591-3779).
 // Leaves G(w) in L.g.  Results in L.mu / L.sig / L.dd; returns the nle.
+// Called by one wavefront (blockDim.x == 64); K <= 32.
 __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, double nu, PolishLds& L,
                            float jitter_multiplier) {
   const int K = c.k, tid = threadIdx.x;
-  for (int e = tid; e < K * K; e += blockDim.x) {
-    const int r = e / K, q = e - r * K;
-    const double2 gv = gram_entry(c, w[r], w[q]);
-    L.g[r][q] = gv;
-    double2 v = make_double2(gv.x / nu, gv.y / nu);
-    if (r == q) v.x += 1.0 / wg[r];
-    L.a[r][q] = v;
-  }
+  const int lane = tid & 63;
+  const double inu = 1.0 / nu;
+  support_coords(c, w, L);
+  if (tid < K) L.iwg[tid] = 1.0 / wg[tid];
   __syncthreads();
-  // Hermitian part, then the diagonal equilibration scale.
-  for (int e = tid; e < K * K; e += blockDim.x) {
-    const int r = e / K, q = e - r * K;
-    if (r < q) {
-      const double2 u = L.a[r][q], l = L.a[q][r];
-      const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
-      L.a[r][q] = h;
-      L.a[q][r] = make_double2(h.x, -h.y);
-    } else if (r == q) {
-      L.a[r][r].y = 0.0;
+  // S = diag(1/gamma) + G / nu, two rows per pass (lanes 0-31, 32-63).
+  const int q = lane & 31, half = lane >> 5;
+#pragma unroll
+  for (int r0 = 0; r0 < KM; r0 += 2) {
+    const int r = r0 + half;
+    if (r < K && q < K) {
+      const double2 gv = gram_rq(c, L, r, q);
+      L.g[r][q] = gv;
+      double2 v = make_double2(gv.x * inu, gv.y * inu);
+      if (r == q) v.x += L.iwg[r];
+      L.a[r][q] = v;
     }
   }
   __syncthreads();
-  if (tid < K) L.dsc[tid] = sqrt(fmax(L.a[tid][tid].x, F32_TINY_D));
+  // Hermitian part.
+#pragma unroll
+  for (int r0 = 0; r0 < KM; r0 += 2) {
+    const int r = r0 + half;
+    if (r < K && q < K) {
+      if (r < q) {
+        const double2 u = L.a[r][q], l = L.a[q][r];
+        const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+        L.a[r][q] = h;
+        L.a[q][r] = make_double2(h.x, -h.y);
+      } else if (r == q) {
+        L.a[r][r].y = 0.0;
+      }
+    }
+  }
   __syncthreads();
-  for (int e = tid; e < K * K; e += blockDim.x) {
-    const int r = e / K, q = e - r * K;
-    const double s = L.dsc[r] * L.dsc[q];
-    L.a[r][q] = make_double2(L.a[r][q].x / s, L.a[r][q].y / s);
+  // Diagonal equilibration D = sqrt(diag).
+  if (tid < K) {
+    const double d = sqrt(fmax(L.a[tid][tid].x, F32_TINY_D));
+    L.dsc[tid] = d;
+    L.idsc[tid] = 1.0 / d;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r0 = 0; r0 < KM; r0 += 2) {
+    const int r = r0 + half;
+    if (r < K && q < K) {
+      const double sc = L.idsc[r] * L.idsc[q];
+      L.a[r][q] = make_double2(L.a[r][q].x * sc, L.a[r][q].y * sc);
+    }
   }
   __syncthreads();
   // Deterministic loading of the GPU branch of _stable_cholesky_raw
   // (solver.py:2907-2920): jitter_multiplier * eps_f32 * mean|diag|.
-  if (tid == 0) {
-    double mean_diag = 0.0;
-    for (int r = 0; r < K; ++r) mean_diag += fabs(L.a[r][r].x);
-    mean_diag = fmax(mean_diag / K, F32_TINY_D);
+  {
+    double md = (tid < K) ? fabs(L.a[tid][tid].x) : 0.0;
+    md = wave_sum_all(md);
+    const double mean_diag = fmax(md / K, F32_TINY_D);
     const double jit = (double)jitter_multiplier * F32_EPS_D * mean_diag;
-    for (int r = 0; r < K; ++r) L.a[r][r].x += jit;
+    __syncthreads();
+    if (tid < K) L.a[tid][tid].x += jit;
   }
   __syncthreads();
-  // Cholesky and inverse in registers: lane r owns row r of the (padded to
-  // 32 with identity) equilibrated system; rows are exchanged by shuffles.
-  const int lane = tid & 63;
+  // Cholesky in registers: lane r owns row r of the (padded to 32 with
+  // identity) equilibrated system; rows are exchanged by shuffles.
   int fail = 0;
   {
     double2 a[32];
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      double2 v = make_double2(lane == q ? 1.0 : 0.0, 0.0);
-      if (lane < K && q < K) v = L.a[lane][q];
-      a[q] = v;
+    for (int qq = 0; qq < 32; ++qq) {
+      double2 v = make_double2(lane == qq ? 1.0 : 0.0, 0.0);
+      if (lane < K && qq < K) v = L.a[lane][qq];
+      a[qq] = v;
     }
     __syncthreads();
 #pragma unroll
@@ -182,24 +228,26 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
       const double d = __shfl(a[j].x, j, 64);
       fail |= !(d > 0.0);
       const double piv = sqrt(fmax(d, 0.0));
+      const double ip = 1.0 / piv;
       if (lane > j) {
-        a[j].x /= piv;
-        a[j].y /= piv;
+        a[j].x *= ip;
+        a[j].y *= ip;
       } else if (lane == j) {
         a[j] = make_double2(piv, 0.0);
+        L.ipiv[j] = ip;
       }
 #pragma unroll
-      for (int q = j + 1; q < 32; ++q) {
-        const double lx = __shfl(a[j].x, q, 64), ly = __shfl(a[j].y, q, 64);
-        if (lane >= q) {  // a[r][q] -= l[r][j] * conj(l[q][j])
-          a[q].x -= a[j].x * lx + a[j].y * ly;
-          a[q].y -= a[j].y * lx - a[j].x * ly;
+      for (int qq = j + 1; qq < 32; ++qq) {
+        const double lx = __shfl(a[j].x, qq, 64), ly = __shfl(a[j].y, qq, 64);
+        if (lane >= qq) {  // a[r][q] -= l[r][j] * conj(l[q][j])
+          a[qq].x -= a[j].x * lx + a[j].y * ly;
+          a[qq].y -= a[j].y * lx - a[j].x * ly;
         }
       }
     }
     if (lane < 32) {
 #pragma unroll
-      for (int q = 0; q < 32; ++q) L.a[lane][q] = a[q];
+      for (int qq = 0; qq < 32; ++qq) L.a[lane][qq] = a[qq];
     }
   }
   if (tid == 0) L.flag[0] = 0;
@@ -216,8 +264,8 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
       acc.x -= l.x * x[p].x - l.y * x[p].y;
       acc.y -= l.x * x[p].y + l.y * x[p].x;
     }
-    const double dg = L.a[r][r].x;
-    x[r] = make_double2(acc.x / dg, acc.y / dg);
+    const double ig = L.ipiv[r];
+    x[r] = make_double2(acc.x * ig, acc.y * ig);
   }
   double ev = 0.0;
 #pragma unroll
@@ -228,14 +276,14 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
   }
   if (tid < K) {
     const float2 b = c.bp[w[tid]];
-    L.rhs[tid] = make_double2((double)b.x / nu, (double)b.y / nu);
+    L.rhs[tid] = make_double2((double)b.x * inu, (double)b.y * inu);
   }
   __syncthreads();
   // z = X (rhs / d), lane r owns row r.
   if (tid < K) {
     double2 acc = make_double2(0.0, 0.0);
     for (int p = 0; p <= tid; ++p) {
-      const double2 sv = make_double2(L.rhs[p].x / L.dsc[p], L.rhs[p].y / L.dsc[p]);
+      const double2 sv = make_double2(L.rhs[p].x * L.idsc[p], L.rhs[p].y * L.idsc[p]);
       const double2 t = zmul(L.x[tid][p], sv);
       acc.x += t.x;
       acc.y += t.y;
@@ -244,6 +292,7 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
   }
   __syncthreads();
   // mu_eq = X^H z and Sigma_eq = ||X[:, c]||^2, lane c owns column c.
+  double terms = 0.0, proj = 0.0;
   if (tid < K) {
     double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
@@ -254,25 +303,23 @@ __device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, do
         acc.y += x[r].x * zr.y - x[r].y * zr.x;
       }
     }
-    const double d = L.dsc[tid];
-    L.mu[tid] = make_double2(acc.x / d, acc.y / d);
-    const double sg = fmax(ev / (d * d), 0.0);
+    const double id = L.idsc[tid];
+    const double2 mu = make_double2(acc.x * id, acc.y * id);
+    L.mu[tid] = mu;
+    const double sg = fmax(ev * id * id, 0.0);
     L.sig[tid] = sg;
-    const double gk = wg[tid];
-    L.dd[tid] = fmax((gk - sg) / (gk * gk), 0.0);
+    const double igk = L.iwg[tid];
+    L.dd[tid] = fmax((wg[tid] - sg) * igk * igk, 0.0);
+    // per-atom terms of log det and of the data projection
+    terms = log(wg[tid]) + 2.0 * (log(fmax(L.a[tid][tid].x, F32_TINY_D)) + log(L.dsc[tid]));
+    proj = L.rhs[tid].x * mu.x + L.rhs[tid].y * mu.y;
   }
-  __syncthreads();
+  terms = wave_sum(terms);
+  proj = wave_sum(proj);
   double nle = 0.0;
   if (tid == 0) {
-    double logdiag = 0.0, logd = 0.0, loggam = 0.0, proj = 0.0;
-    for (int r = 0; r < K; ++r) {
-      logdiag += log(fmax(L.a[r][r].x, F32_TINY_D));
-      logd += log(L.dsc[r]);
-      loggam += log(wg[r]);
-      proj += L.rhs[r].x * L.mu[r].x + L.rhs[r].y * L.mu[r].y;
-    }
-    const double logdet = (double)c.m_total * log(nu) + loggam + 2.0 * (logdiag + logd);
-    const double quad = fmax(c.ye / nu - proj, 0.0);
+    const double logdet = (double)c.m_total * log(nu) + terms;
+    const double quad = fmax(c.ye * inu - proj, 0.0);
     nle = L.flag[0] ? NAN : logdet + quad;
     L.red[0] = nle;
   }
@@ -313,9 +360,13 @@ __device__ double residual_energy(const SceneCtx& c, const int* s, const double2
 
 __device__ void load_gram(const SceneCtx& c, const int* s, PolishLds& L) {
   const int K = c.k;
-  for (int e = threadIdx.x; e < K * K; e += blockDim.x) {
-    const int r = e / K, q = e - r * K;
-    L.g[r][q] = gram_entry(c, s[r], s[q]);
+  support_coords(c, s, L);
+  __syncthreads();
+  const int q = threadIdx.x & 31, half = threadIdx.x >> 5;
+#pragma unroll
+  for (int r0 = 0; r0 < KM; r0 += 2) {
+    const int r = r0 + half;
+    if (r < K && q < K) L.g[r][q] = gram_rq(c, L, r, q);
   }
   __syncthreads();
 }
@@ -410,21 +461,35 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   if (phase == 0) {
     // Top-k of the candidate scores (solver.py:1931).
     const int ncand = s.row_items * K;
-    if (ncand <= 64 * 32) {
-      // Wave arg-max selection over register-held candidates (in order).
-      unsigned long long v[32];
-#pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const int slot = tid + 64 * i;
-        v[i] = (slot < ncand) ? pack_key(cand_key[s.cand_off + slot], slot) : 0ull;
-        if (slot < ncand && cand_key[s.cand_off + slot] == 0u) v[i] = 0ull;
+    if (s.row_items <= 64) {
+      // Every row tile left its K candidates sorted by (score desc, grid
+      // index asc): merge the sorted lists, lane l owning list l, by K
+      // rounds of wave arg-max over the list heads.
+      for (int i = tid; i < ncand; i += blockDim.x) {
+        keys[i] = cand_key[s.cand_off + i];
+        kidx[i] = cand_idx[s.cand_off + i];
       }
-      wave_topk<32>(v, K, packed);
+      __syncthreads();
+      auto entry = [&](int p) -> unsigned long long {
+        if (tid >= s.row_items || p >= K) return 0ull;
+        const uint32_t kk = keys[tid * K + p];
+        return kk ? pack_key(kk, kidx[tid * K + p]) : 0ull;
+      };
+      int ptr = 0;
+      unsigned long long head = entry(0), next = entry(1);
+      for (int r = 0; r < K; ++r) {
+        const unsigned long long w = wave_max_u64(head);
+        if (tid == 0) packed[r] = w;
+        if (w != 0ull && head == w) {
+          head = next;
+          ++ptr;
+          next = entry(ptr + 1);
+        }
+      }
       __syncthreads();
       if (tid < K) {
-        const int slot = packed_pos(packed[tid]);
         srt_key[tid] = (uint32_t)(packed[tid] >> 32);
-        srt_idx[tid] = cand_idx[s.cand_off + slot];
+        srt_idx[tid] = packed[tid] ? packed_pos(packed[tid]) : -1;
       }
       __syncthreads();
     } else {

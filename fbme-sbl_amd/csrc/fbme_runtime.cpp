@@ -31,7 +31,7 @@ int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, int, co
                             void*, const double*, double*, int, int, int, float, const void*, hipStream_t);
 int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, int, void*, void*, float*, void*, float*,
                     const double*, double*, int, int, float, float, float, float, float, float, uint32_t*, int*,
-                    int, const void*, hipStream_t);
+                    int, uint32_t*, const void*, hipStream_t);
 int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*, void*,
                             void*, const double*, double*, int, int, float, const void*, hipStream_t);
 int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
@@ -75,7 +75,7 @@ struct fbme_plan {
   float threshold = 0.f;
   // workspace layout
   Region r_scenes, r_col, r_row, r_tile, r_dir, r_tw, r_sc, r_part, r_mid, r_msg, r_pred, r_bp, r_gamma, r_mean, r_var,
-      r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result;
+      r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result, r_score;
   size_t workspace_bytes = 0;
   char* ws = nullptr;
   int bound = 0;
@@ -411,6 +411,12 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
   reg(p->r_xidx, 4 * (size_t)p->total_xcand);
   reg(p->r_state, fbme_polish_state_bytes() * nscenes);
   reg(p->r_result, fbme_polish_result_bytes() * nscenes);
+  // polish scores of the register-FFT row tiles (selected by a separate pass)
+  long long score_cells = 0;
+  for (size_t i = p->row_split; i < p->row_work.size(); ++i)
+    score_cells = std::max<long long>(score_cells, p->scenes[p->row_work[i].scene].grid_off +
+                                                       (long long)p->scenes[p->row_work[i].scene].n1 * 256);
+  reg(p->r_score, 4 * (size_t)score_cells);
   p->workspace_bytes = align_up(off, 256);
   return p;
 }
@@ -527,12 +533,15 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
                                   store, o.uamp_damping, tw, st);
     return r;
   };
+  uint32_t* score = p->r_score.bytes ? (uint32_t*)(w + p->r_score.off) : nullptr;
   auto row = [&](int ng, int n16, int mode, uint32_t* key, int* idx, int store) {
     int r = fbme_launch_row(scenes, roww, ng, ng, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, mode,
-                            o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, tw, st);
+                            o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, nullptr,
+                            tw, st);
     if (!r)
       r = fbme_launch_row(scenes, roww16, n16, n16, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps, mode,
-                          o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, tw, st);
+                          o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, score, tw,
+                          st);
     return r;
   };
   auto col_forward = [&](int ng, int n16, int first) {
@@ -611,10 +620,10 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
     case FBME_STAGE_ROW:
       return (fbme_launch_row(scenes, roww, ng_row, ng_row, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, 2,
                               o.uamp_damping, o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr,
-                              nullptr, 0, tw, st) ||
+                              nullptr, 0, nullptr, tw, st) ||
               fbme_launch_row(scenes, roww16, n16_row, n16_row, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps, 2,
                               o.uamp_damping, o.damping, o.max_precision, o.hyper_shape, o.hyper_rate, o.blend, nullptr,
-                              nullptr, 0, tw, st))
+                              nullptr, 0, nullptr, tw, st))
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_COL_FORWARD:

@@ -273,16 +273,9 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
       // Top-k candidates of this tile for the polish selection (solver.py:1926-1933).
       __syncthreads();
       const int got = block_select_topk(keys, nel, s.k, pos, hist, misc);
-      const long long cbase = (long long)s.cand_off + (long long)wk.slot * s.k;
-      for (int i = threadIdx.x; i < s.k; i += FBME_BLOCK) {
-        if (i < got) {
-          cand_key[cbase + i] = keys[pos[i]];
-          cand_idx[cbase + i] = wk.first * len + pos[i];
-        } else {
-          cand_key[cbase + i] = 0u;
-          cand_idx[cbase + i] = -1;
-        }
-      }
+      write_sorted_candidates(keys, pos, got, s.k, wk.first * len,
+                              cand_key + (long long)s.cand_off + (long long)wk.slot * s.k,
+                              cand_idx + (long long)s.cand_off + (long long)wk.slot * s.k);
       __syncthreads();
       continue;
     }
@@ -445,8 +438,8 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
 extern "C" {
 
 int fbme_launch_row16(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*, const double*,
-                      double*, int, int, float, float, float, float, float, float, uint32_t*, int*, int, const void*,
-                      hipStream_t);
+                      double*, int, int, float, float, float, float, float, float, uint32_t*, int*, int, uint32_t*,
+                      const void*, hipStream_t);
 int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
                               const double*, double*, int, int, int, float, const void*, hipStream_t);
 int fbme_launch_col16_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
@@ -477,12 +470,12 @@ int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, in
                     float* gamma, void* mean, float* var, const double* sc, double* part, int pstride,
                     int mode, float damping, float prior_damping, float max_precision, float hyper_shape,
                     float hyper_rate, float blend, uint32_t* cand_key, int* cand_idx, int store_final,
-                    const void* twtab, hipStream_t stream) {
+                    uint32_t* score, const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   if (ept == FBME_EPT_R16)
     return fbme_launch_row16(scenes, work, nwork, grid, mid, bp, gamma, mean, var, sc, part, pstride, mode, damping,
                              prior_damping, max_precision, hyper_shape, hyper_rate, blend, cand_key, cand_idx,
-                             store_final, twtab, stream);
+                             store_final, score, twtab, stream);
   UampArgs a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
 #define FBME_ROW(M)                                                                                      \
   hipLaunchKernelGGL((k_row<E, M>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,       \

@@ -91,15 +91,15 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
 }
 
 // 256-point transform of the line owned by the 16 lanes of a group.  On entry
-// v[j] = a[t + 16 j]; on exit v[n] = b[t + 16 n].  w[n] = exp(-2 pi i t n / 256).
-// x: this group's LDS transpose buffer; every lane of the workgroup must call
-// (the barriers are workgroup-wide).
+// v[j] = a[t + 16 j]; on exit v[n] = b[t + 16 n].  tw: LDS table
+// exp(-2 pi i k / 256); x: this group's LDS transpose buffer; every lane of
+// the workgroup must call (the barriers are workgroup-wide).
 template <bool INV>
-__device__ __forceinline__ void fft256(float2 (&v)[16], const float2 (&w)[16], float2* x, int t) {
+__device__ __forceinline__ void fft256(float2 (&v)[16], const float2* tw, float2* x, int t) {
   dft16<INV>(v);
 #pragma unroll
   for (int n = 1; n < 16; ++n) {
-    float2 ww = w[n];
+    float2 ww = tw[t * n];
     if (INV) ww.y = -ww.y;
     v[n] = cmul(v[n], ww);
   }
@@ -112,9 +112,124 @@ __device__ __forceinline__ void fft256(float2 (&v)[16], const float2 (&w)[16], f
   dft16<INV>(v);
 }
 
-__device__ __forceinline__ void lane_twiddles(float2 (&w)[16], const float2* __restrict__ table, int t) {
+// Copy the 256-entry twiddle table to LDS; the caller synchronises before use.
+__device__ __forceinline__ void load_tw256(float2* tw, const float2* __restrict__ table) {
+  for (int i = threadIdx.x; i < 256; i += blockDim.x) tw[i] = table[i];
+}
+
+// Top-k (k <= 32) of the keys a wavefront holds 16 per lane (lane (g, t)
+// holds tile position g * 256 + 16 n + t; key 0 marks an empty slot),
+// written sorted by (key desc, position asc) to out_key / out_idx with
+// `base` added to positions.
+//  1. The k largest lane maxima are k distinct keys >= L, the k-th largest
+//     lane maximum, so every top-k key is >= L.
+//  2. While more than 64 keys survive, bisection raises the threshold lo
+//     keeping count(key >= lo) >= k.
+//  3. At most `cap` survivors are ranked directly.  Otherwise the bisection
+//     ended on the exact k-th largest key T with massive ties: keys > T and
+//     the lowest positions among keys == T are taken.
+// Called by every lane of a single-wavefront workgroup.
+__device__ void wave_tile_topk(const uint32_t (&kv)[16], int g, int t, int k, int base, uint32_t* sk, int* sp,
+                               int cap, uint32_t* lmax, uint32_t* out_key, int* out_idx) {
+  const int lane = threadIdx.x & 63;
+  uint32_t v = 0u;
 #pragma unroll
-  for (int n = 0; n < 16; ++n) w[n] = table[t * n];
+  for (int n = 0; n < 16; ++n) v = max(v, kv[n]);
+  lmax[lane] = v;
+  __syncthreads();
+  int rank = 0;
+#pragma unroll 16
+  for (int j = 0; j < 64; ++j) {
+    const uint32_t u = lmax[j];
+    rank += (u > v) || (u == v && j < lane);
+  }
+  __syncthreads();
+  if (rank == k - 1) lmax[0] = v;
+  __syncthreads();
+  uint32_t lo = lmax[0];
+  __syncthreads();
+  auto count_ge = [&](uint32_t th) {
+    int cnt = 0;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) cnt += __popcll(__ballot(kv[n] != 0u && kv[n] >= th));
+    return cnt;
+  };
+  unsigned long long hi = (unsigned long long)wave_max_u64((unsigned long long)v) + 1ull;  // count(>= hi) = 0
+  int cnt = count_ge(lo);
+  while (cnt > 64 && hi > (unsigned long long)lo + 1ull) {
+    const uint32_t mid = (uint32_t)((unsigned long long)lo + (hi - (unsigned long long)lo) / 2ull);
+    const int c2 = count_ge(mid);
+    if (c2 >= k) {
+      lo = mid;
+      cnt = c2;
+    } else {
+      hi = mid;
+    }
+  }
+  const unsigned long long below = __lanemask_lt();
+  int total = 0;
+  if (cnt <= cap) {
+    // all survivors (keys >= lo), compacted in (n, lane) order
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const bool keep = kv[n] != 0u && kv[n] >= lo;
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int o = total + __popcll(m & below);
+        sk[o] = kv[n];
+        sp[o] = g * 256 + 16 * n + t;
+      }
+      total += __popcll(m);
+    }
+  } else {
+    // lo is the k-th largest key T: take keys > T, then the lowest
+    // positions among keys == T (position order is (g, n, t)).
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const bool keep = kv[n] > lo;
+      const unsigned long long m = __ballot(keep);
+      if (keep) {
+        const int o = total + __popcll(m & below);
+        sk[o] = kv[n];
+        sp[o] = g * 256 + 16 * n + t;
+      }
+      total += __popcll(m);
+    }
+    for (int gg = 0; gg < 4 && total < k; ++gg) {
+#pragma unroll
+      for (int n = 0; n < 16; ++n) {
+        if (total >= k) break;
+        const bool tie = g == gg && kv[n] == lo;
+        const unsigned long long m = __ballot(tie);
+        if (tie) {
+          const int o = total + __popcll(m & below);
+          if (o < k) {
+            sk[o] = kv[n];
+            sp[o] = g * 256 + 16 * n + t;
+          }
+        }
+        total = min(k, total + __popcll(m));
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = lane; i < total; i += 64) {
+    const uint32_t ki = sk[i];
+    const int pi = sp[i];
+    int r = 0;
+    for (int j = 0; j < total; ++j) {
+      const uint32_t kj = sk[j];
+      r += (kj > ki) || (kj == ki && sp[j] < pi);
+    }
+    if (r < k) {
+      out_key[r] = ki;
+      out_idx[r] = base + pi;
+    }
+  }
+  for (int i = total + lane; i < k; i += 64) {
+    out_key[i] = 0u;
+    out_idx[i] = -1;
+  }
 }
 
 // ------------------------------------------------------------------- row pass
@@ -126,10 +241,11 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
         float2* __restrict__ mid, float2* __restrict__ bp, float* __restrict__ gamma,
         float2* __restrict__ mean, float* __restrict__ var, const double* __restrict__ sc,
         double* __restrict__ part, int pstride, UampArgs16 a, uint32_t* __restrict__ cand_key,
-        int* __restrict__ cand_idx, int store_final, const float2* __restrict__ twtab) {
+        int* __restrict__ cand_idx, int store_final, uint32_t* __restrict__ score,
+        const float2* __restrict__ twtab) {
   __shared__ float2 xs[4][XS];
+  __shared__ float2 tw[256];
   __shared__ double red[8];
-  __shared__ unsigned long long top[MODE == 3 ? FBME_KMAX : 1];
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   float2* x = xs[g];
   for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
@@ -146,31 +262,17 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const float noise = (float)scs[SC_NOISE];
     const float floor_v = (float)scs[SC_FLOOR];
     const float power = (float)scs[SC_POWER];
-    float2 w[16];
-    lane_twiddles(w, twtab + s.tw2_off, t);
-    float gv[16], vv[16];
-    float2 mv[16], v[16];
-    if (MODE >= 1) {
-#pragma unroll
-      for (int n = 0; n < 16; ++n) {
-        const long long k = gb + t + 16 * n;
-        if (MODE == 1) {
-          mv[n] = act ? bp[k] : make_float2(0.f, 0.f);
-        } else {
-          gv[n] = act ? gamma[k] : 1.0f;
-          mv[n] = act ? mean[k] : make_float2(0.f, 0.f);
-          vv[n] = act ? var[k] : 1.0f;
-        }
-      }
-    }
+    load_tw256(tw, twtab + s.tw2_off);
+    float2 v[16];
     if (MODE != 1) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) {
         const int c = t + 16 * j;
         v[j] = (act && c < s.m2) ? mid[mb + c] : make_float2(0.f, 0.f);
       }
-      fft256<true>(v, w, x, t);
     }
+    __syncthreads();
+    if (MODE != 1) fft256<true>(v, tw, x, t);
     if (MODE == 0) {
       double acc = 0.0;
 #pragma unroll
@@ -182,6 +284,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
       }
       acc = block_sum(acc, red);
       if (threadIdx.x == 0) part[(long long)PT_BPE * pstride + s.row_part + wk.slot] = acc;
+      __syncthreads();
       continue;
     }
     const float tau = fmaxf((dv + noise) / (float)n_samp, F32_TINY);
@@ -190,59 +293,82 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const float bpe_clamped = fmaxf((float)scs[SC_BPE], F32_EPS);
     const float uniform = power / (float)n_grid;
     double dof_acc = 0.0, vs_acc = 0.0;
-    unsigned long long pk[16];
+    // The per-element state is loaded after the inverse transform, in two
+    // halves, so that little of it is live at once (fewer registers, more
+    // resident waves to cover the HBM latency).
 #pragma unroll
-    for (int n = 0; n < 16; ++n) {
-      const long long k = gb + t + 16 * n;
-      float2 lifted, mean_old;
-      float prior, v_old;
-      if (MODE == 1) {
-        const float2 b = mv[n];
-        lifted = cscale(b, first_scale);
-        const float energy = b.x * b.x + b.y * b.y;
-        const float structured = energy / bpe_clamped * power;
-        prior = fmaxf(a.blend * structured + (1.0f - a.blend) * uniform, floor_v);
-        mean_old = make_float2(0.f, 0.f);
-        v_old = fmaxf(prior, F32_TINY);
-      } else {
-        lifted = v[n];
-        prior = gv[n];
-        mean_old = mv[n];
-        v_old = fmaxf(vv[n], F32_TINY);
-      }
-      // Denoise (triton_kernels.py:131-158).
-      const float denom = fmaxf(prior + tau, F32_TINY);
-      const float gain = prior / denom;
-      const float2 pm = make_float2(mean_old.x + tau * lifted.x, mean_old.y + tau * lifted.y);
-      const float2 cm = make_float2(gain * pm.x, gain * pm.y);
-      const float cvar = prior * tau / denom;
-      const float2 mu = make_float2(keep * mean_old.x + update * cm.x, keep * mean_old.y + update * cm.y);
-      const float vn = fmaxf(keep * v_old + update * cvar, F32_TINY);
-      float rel = 1.0f - vn / fmaxf(prior, F32_TINY);
-      rel = fmaxf(0.0f, fminf(1.0f, rel));
-      if (act) {
-        dof_acc += (double)rel;
-        vs_acc += (double)vn;
-      }
-      if (MODE == 3) {
-        pk[n] = act ? pack_key(score_key(mu.x * mu.x + mu.y * mu.y + vn), g * 256 + t + 16 * n) : 0ull;
-        if (store_final && act) {
-          mean[k] = mu;
-          var[k] = vn;
+    for (int h = 0; h < 16; h += 8) {
+      float gv[8], vv[8];
+      float2 mv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const long long k = gb + t + 16 * (h + q);
+        if (MODE == 1) {
+          mv[q] = act ? bp[k] : make_float2(0.f, 0.f);
+        } else {
+          gv[q] = act ? gamma[k] : 1.0f;
+          mv[q] = act ? mean[k] : make_float2(0.f, 0.f);
+          vv[q] = act ? var[k] : 1.0f;
         }
-      } else {
-        // Fused ARD update (triton_kernels.py:159-180).
-        const float second = mu.x * mu.x + mu.y * mu.y + vn;
-        float cp = (second + a.hyper_rate) / (1.0f + a.hyper_shape);
-        cp = fmaxf(floor_v, fminf(a.max_precision, cp));
-        const float nxt = a.prior_damping * prior + (1.0f - a.prior_damping) * cp;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int n = h + q;
+        const long long k = gb + t + 16 * n;
+        float2 lifted, mean_old;
+        float prior, v_old;
+        if (MODE == 1) {
+          const float2 b = mv[q];
+          lifted = cscale(b, first_scale);
+          const float energy = b.x * b.x + b.y * b.y;
+          const float structured = energy / bpe_clamped * power;
+          prior = fmaxf(a.blend * structured + (1.0f - a.blend) * uniform, floor_v);
+          mean_old = make_float2(0.f, 0.f);
+          v_old = fmaxf(prior, F32_TINY);
+        } else {
+          lifted = v[n];
+          prior = gv[q];
+          mean_old = mv[q];
+          v_old = fmaxf(vv[q], F32_TINY);
+        }
+        // Denoise (triton_kernels.py:131-158).
+        const float denom = fmaxf(prior + tau, F32_TINY);
+        const float gain = prior / denom;
+        const float2 pm = make_float2(mean_old.x + tau * lifted.x, mean_old.y + tau * lifted.y);
+        const float2 cm = make_float2(gain * pm.x, gain * pm.y);
+        const float cvar = prior * tau / denom;
+        const float2 mu = make_float2(keep * mean_old.x + update * cm.x, keep * mean_old.y + update * cm.y);
+        const float vn = fmaxf(keep * v_old + update * cvar, F32_TINY);
+        float rel = 1.0f - vn / fmaxf(prior, F32_TINY);
+        rel = fmaxf(0.0f, fminf(1.0f, rel));
         if (act) {
-          gamma[k] = fmaxf(nxt, floor_v);
-          mean[k] = mu;
-          var[k] = vn;
+          dof_acc += (double)rel;
+          vs_acc += (double)vn;
         }
-        v[n] = act ? mu : make_float2(0.f, 0.f);
+        if (MODE == 3) {
+          // polish score |mu|^2 + v as an order-preserving key; the tile's
+          // top-k runs in k_tile_topk16 (a latency-bound select would hold
+          // this bandwidth-bound pass's registers and waves)
+          if (act) score[k] = score_key(mu.x * mu.x + mu.y * mu.y + vn);
+          if (store_final && act) {
+            mean[k] = mu;
+            var[k] = vn;
+          }
+        } else {
+          // Fused ARD update (triton_kernels.py:159-180).
+          const float second = mu.x * mu.x + mu.y * mu.y + vn;
+          float cp = (second + a.hyper_rate) / (1.0f + a.hyper_shape);
+          cp = fmaxf(floor_v, fminf(a.max_precision, cp));
+          const float nxt = a.prior_damping * prior + (1.0f - a.prior_damping) * cp;
+          if (act) {
+            gamma[k] = fmaxf(nxt, floor_v);
+            mean[k] = mu;
+            var[k] = vn;
+          }
+          v[n] = act ? mu : make_float2(0.f, 0.f);
+        }
       }
+      __builtin_amdgcn_sched_barrier(0);
     }
     block_sum2(dof_acc, vs_acc, red);
     if (threadIdx.x == 0) {
@@ -250,26 +376,42 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
       part[(long long)PT_VS * pstride + s.row_part + wk.slot] = vs_acc;
     }
     if (MODE == 3) {
-      // Top-k candidates of the tile for the polish selection
-      // (solver.py:1926-1933): score |mu|^2 + v, ties to the lower index.
-      wave_topk<16>(pk, s.k, top);
-      __syncthreads();
-      const long long cbase = (long long)s.cand_off + (long long)wk.slot * s.k;
-      for (int i = threadIdx.x; i < s.k; i += 64) {
-        const unsigned long long p = top[i];
-        cand_key[cbase + i] = (uint32_t)(p >> 32);
-        cand_idx[cbase + i] = p ? wk.first * 256 + packed_pos(p) : -1;
-      }
       __syncthreads();
       continue;
     }
     // Row pass of A mu keeping the M2 leading outputs.
-    fft256<false>(v, w, x, t);
+    fft256<false>(v, tw, x, t);
 #pragma unroll
     for (int n = 0; n < 16; ++n) {
       const int c = t + 16 * n;
       if (act && c < s.m2) mid[mb + c] = v[n];
     }
+    __syncthreads();
+  }
+}
+
+// Top-k candidates of each four-row tile for the polish selection
+// (solver.py:1926-1933): score |mu|^2 + v written by k_row16<3>, sorted by
+// (score desc, grid index asc), one wavefront per tile.
+__global__ void __launch_bounds__(64)
+k_tile_topk16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
+              const uint32_t* __restrict__ score, uint32_t* __restrict__ cand_key, int* __restrict__ cand_idx) {
+  __shared__ uint32_t surv_key[64];
+  __shared__ int surv_pos[64];
+  __shared__ uint32_t lane_max[64];
+  const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const FbmeWork wk = work[wi];
+    const FbmeScene s = scenes[wk.scene];
+    const bool act = g < wk.count;
+    const long long gb = s.grid_off + (long long)(wk.first + g) * 256;
+    uint32_t kv[16];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) kv[n] = act ? score[gb + t + 16 * n] : 0u;
+    uint32_t* ck = cand_key + (long long)s.cand_off + (long long)wk.slot * s.k;
+    int* ci = cand_idx + (long long)s.cand_off + (long long)wk.slot * s.k;
+    wave_tile_topk(kv, g, t, s.k, wk.first * 256, surv_key, surv_pos, 64, lane_max, ck, ci);
+    __syncthreads();
   }
 }
 
@@ -282,6 +424,7 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
                 float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[COLS][XS];
+  __shared__ float2 tw[256];
   __shared__ double red[8];
   const int c = threadIdx.x % COLS, t = threadIdx.x / COLS;
   float2* x = xs[c];
@@ -294,8 +437,7 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float precision = 1.0f / (dv + (float)scs[SC_NOISE]);
     const float keep = damping, update = 1.0f - damping;
-    float2 w[16];
-    lane_twiddles(w, twtab + s.tw1_off, t);
+    load_tw256(tw, twtab + s.tw1_off);
     float2 v[16];
     double acc = 0.0;
 #pragma unroll
@@ -321,11 +463,13 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       acc = block_sum(acc, red);
       if (threadIdx.x == 0) part[(long long)PT_YE * pstride + s.col_part + wk.slot] = acc;
     }
-    fft256<true>(v, w, x, t);
+    __syncthreads();
+    fft256<true>(v, tw, x, t);
     if (act) {
 #pragma unroll
       for (int n = 0; n < 16; ++n) mid[s.mid_off + (long long)(t + 16 * n) * s.m2 + col] = v[n];
     }
+    __syncthreads();
   }
 }
 
@@ -336,6 +480,7 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
                 float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[COLS][XS];
+  __shared__ float2 tw[256];
   __shared__ double red[8];
   const int c = threadIdx.x % COLS, t = threadIdx.x / COLS;
   float2* x = xs[c];
@@ -344,16 +489,21 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
     const FbmeScene s = scenes[wk.scene];
     const bool act = c < wk.count;
     const int col = wk.first + c;
-    float2 w[16];
-    lane_twiddles(w, twtab + s.tw1_off, t);
+    load_tw256(tw, twtab + s.tw1_off);
     float2 v[16], yv[16];
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
       const int i = t + 16 * j;
       v[j] = act ? mid[s.mid_off + (long long)i * s.m2 + col] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft256<false>(v, tw, x, t);
+    // data loaded after the transform (not live across it)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const int i = t + 16 * j;
       yv[j] = (act && i < s.m1) ? y[s.samp_off + (long long)i * s.m2 + col] : make_float2(0.f, 0.f);
     }
-    fft256<false>(v, w, x, t);
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float precision = 1.0f / (dv + (float)scs[SC_NOISE]);
@@ -376,6 +526,7 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
     }
     acc = block_sum(acc, red);
     if (threadIdx.x == 0) part[(long long)PT_RES * pstride + s.col_part + wk.slot] = acc;
+    __syncthreads();
   }
 }
 
@@ -386,14 +537,15 @@ extern "C" {
 int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, void* mid, void* bp,
                       float* gamma, void* mean, float* var, const double* sc, double* part, int pstride, int mode,
                       float damping, float prior_damping, float max_precision, float hyper_shape, float hyper_rate,
-                      float blend, uint32_t* cand_key, int* cand_idx, int store_final, const void* twtab,
-                      hipStream_t stream) {
+                      float blend, uint32_t* cand_key, int* cand_idx, int store_final, uint32_t* score,
+                      const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
+  if (mode == 3 && !score) return FBME_ERR_ARG;
   UampArgs16 a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
 #define FBME_ROW16(M)                                                                                        \
   hipLaunchKernelGGL((k_row16<M>), dim3(grid), dim3(64), 0, stream, scenes, work, nwork, (float2*)mid,       \
                      (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, cand_key, cand_idx,        \
-                     store_final, (const float2*)twtab)
+                     store_final, score, (const float2*)twtab)
   switch (mode) {
     case 0: FBME_ROW16(0); break;
     case 1: FBME_ROW16(1); break;
@@ -401,6 +553,12 @@ int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, 
     default: FBME_ROW16(3); break;
   }
 #undef FBME_ROW16
+  if (mode == 3) {
+    const int e = (int)hipGetLastError();
+    if (e) return e;
+    hipLaunchKernelGGL(k_tile_topk16, dim3(grid), dim3(64), 0, stream, scenes, work, nwork, score, cand_key,
+                       cand_idx);
+  }
   return (int)hipGetLastError();
 }
 
