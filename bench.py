@@ -138,16 +138,18 @@ def main():
         stage_ms[name] = start.elapsed_time(stop) / args.kernel_reps
     row_s = stage_ms["row"] / 1e3
     achieved = kb["row"] * b / row_s
+    kname = "k_row16<2>" if native.r16_row_items > 0 else "k_row<EPT, 2>"
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "row_pass_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)
-        if t.get("batch") == b and t.get("workload") == "256x256/160x160":
+        if (t.get("batch") == b and t.get("workload") == "256x256/160x160"
+                and t.get("kernel") == kname):
             traffic = t.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
-                "kernel": "k_row (fused A^H row pass + denoise/ARD + A row pass)",
+                "kernel": kname + " (fused A^H row pass + denoise/ARD + A row pass)",
                 "algorithmic_bytes_per_launch": kb["row"] * b,
                 "avg_launch_ms": stage_ms["row"]}
     per_kernel = {k: {"ms": v, "GB/s": kb[k] * b / (v / 1e3) / 1e9}

@@ -441,6 +441,8 @@ long long fbme_plan_query(const fbme_plan* p, int what) {
       n += 1;           // output
       return n;
     }
+    case FBME_Q_R16_COL_ITEMS: return (long long)p->col_work.size() - p->col_split;
+    case FBME_Q_R16_ROW_ITEMS: return (long long)p->row_work.size() - p->row_split;
     default: return -1;
   }
 }

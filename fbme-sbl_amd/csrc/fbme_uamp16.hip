@@ -92,9 +92,11 @@ __device__ __forceinline__ void dft16(float2 (&v)[16]) {
 
 // 256-point transform of the line owned by the 16 lanes of a group.  On entry
 // v[j] = a[t + 16 j]; on exit v[n] = b[t + 16 n].  tw: LDS table
-// exp(-2 pi i k / 256); x: this group's LDS transpose buffer; every lane of
-// the workgroup must call (the barriers are workgroup-wide).
-template <bool INV>
+// exp(-2 pi i k / 256); x: this group's LDS transpose buffer, element i at
+// x[i * XSTRIDE] (groups interleaved with stride XSTRIDE keep the 16 lanes
+// of one LDS pass on consecutive slots); every lane of the workgroup must
+// call (the barriers are workgroup-wide).
+template <bool INV, int XSTRIDE = 1>
 __device__ __forceinline__ void fft256(float2 (&v)[16], const float2* tw, float2* x, int t) {
   dft16<INV>(v);
 #pragma unroll
@@ -104,10 +106,10 @@ __device__ __forceinline__ void fft256(float2 (&v)[16], const float2* tw, float2
     v[n] = cmul(v[n], ww);
   }
 #pragma unroll
-  for (int n = 0; n < 16; ++n) x[n * 17 + t] = v[n];
+  for (int n = 0; n < 16; ++n) x[(n * 17 + t) * XSTRIDE] = v[n];
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = x[t * 17 + j];
+  for (int j = 0; j < 16; ++j) v[j] = x[(t * 17 + j) * XSTRIDE];
   __syncthreads();
   dft16<INV>(v);
 }
@@ -292,6 +294,8 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const float first_scale = update / (dv + noise);
     const float bpe_clamped = fmaxf((float)scs[SC_BPE], F32_EPS);
     const float uniform = power / (float)n_grid;
+    const float struct_scale = power / bpe_clamped;
+    const float ard_scale = 1.0f / (1.0f + a.hyper_shape);
     double dof_acc = 0.0, vs_acc = 0.0;
     // The per-element state is loaded after the inverse transform, in two
     // halves, so that little of it is live at once (fewer registers, more
@@ -321,7 +325,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
           const float2 b = mv[q];
           lifted = cscale(b, first_scale);
           const float energy = b.x * b.x + b.y * b.y;
-          const float structured = energy / bpe_clamped * power;
+          const float structured = energy * struct_scale;
           prior = fmaxf(a.blend * structured + (1.0f - a.blend) * uniform, floor_v);
           mean_old = make_float2(0.f, 0.f);
           v_old = fmaxf(prior, F32_TINY);
@@ -331,15 +335,18 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
           mean_old = mv[q];
           v_old = fmaxf(vv[q], F32_TINY);
         }
-        // Denoise (triton_kernels.py:131-158).
+        // Denoise (triton_kernels.py:131-158).  Quotients use the hardware
+        // reciprocal (1 ulp), the precision class of the reference's Triton
+        // float division.
         const float denom = fmaxf(prior + tau, F32_TINY);
-        const float gain = prior / denom;
+        const float rden = __builtin_amdgcn_rcpf(denom);
+        const float gain = prior * rden;
         const float2 pm = make_float2(mean_old.x + tau * lifted.x, mean_old.y + tau * lifted.y);
         const float2 cm = make_float2(gain * pm.x, gain * pm.y);
-        const float cvar = prior * tau / denom;
+        const float cvar = prior * tau * rden;
         const float2 mu = make_float2(keep * mean_old.x + update * cm.x, keep * mean_old.y + update * cm.y);
         const float vn = fmaxf(keep * v_old + update * cvar, F32_TINY);
-        float rel = 1.0f - vn / fmaxf(prior, F32_TINY);
+        float rel = 1.0f - vn * __builtin_amdgcn_rcpf(fmaxf(prior, F32_TINY));
         rel = fmaxf(0.0f, fminf(1.0f, rel));
         if (act) {
           dof_acc += (double)rel;
@@ -357,7 +364,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
         } else {
           // Fused ARD update (triton_kernels.py:159-180).
           const float second = mu.x * mu.x + mu.y * mu.y + vn;
-          float cp = (second + a.hyper_rate) / (1.0f + a.hyper_shape);
+          float cp = (second + a.hyper_rate) * ard_scale;
           cp = fmaxf(floor_v, fminf(a.max_precision, cp));
           const float nxt = a.prior_damping * prior + (1.0f - a.prior_damping) * cp;
           if (act) {
@@ -423,11 +430,11 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
                 const float2* __restrict__ y, const float2* __restrict__ pred, float2* __restrict__ msg,
                 float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
-  __shared__ float2 xs[COLS][XS];
+  __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
   __shared__ float2 tw[256];
   __shared__ double red[8];
   const int c = threadIdx.x % COLS, t = threadIdx.x / COLS;
-  float2* x = xs[c];
+  float2* x = xs + c;
   for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
     const FbmeWork wk = work[wi];
     const FbmeScene s = scenes[wk.scene];
@@ -464,7 +471,7 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       if (threadIdx.x == 0) part[(long long)PT_YE * pstride + s.col_part + wk.slot] = acc;
     }
     __syncthreads();
-    fft256<true>(v, tw, x, t);
+    fft256<true, COLS>(v, tw, x, t);
     if (act) {
 #pragma unroll
       for (int n = 0; n < 16; ++n) mid[s.mid_off + (long long)(t + 16 * n) * s.m2 + col] = v[n];
@@ -479,11 +486,11 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
                 const float2* __restrict__ mid, const float2* __restrict__ y, float2* __restrict__ pred,
                 float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
-  __shared__ float2 xs[COLS][XS];
+  __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
   __shared__ float2 tw[256];
   __shared__ double red[8];
   const int c = threadIdx.x % COLS, t = threadIdx.x / COLS;
-  float2* x = xs[c];
+  float2* x = xs + c;
   for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
     const FbmeWork wk = work[wi];
     const FbmeScene s = scenes[wk.scene];
@@ -497,7 +504,7 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       v[j] = act ? mid[s.mid_off + (long long)i * s.m2 + col] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft256<false>(v, tw, x, t);
+    fft256<false, COLS>(v, tw, x, t);
     // data loaded after the transform (not live across it)
 #pragma unroll
     for (int j = 0; j < 16; ++j) {

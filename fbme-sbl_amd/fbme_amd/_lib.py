@@ -23,6 +23,8 @@ Q_MAX_ITERS = 3
 Q_NSCENES = 4
 Q_EVALUATIONS = 5
 Q_LAUNCHES = 6
+Q_R16_COL_ITEMS = 7
+Q_R16_ROW_ITEMS = 8
 
 STAGE_COL_ADJOINT = 0
 STAGE_ROW = 1

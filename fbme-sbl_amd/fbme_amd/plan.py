@@ -71,6 +71,8 @@ class NativePlan:
         self.max_iters = q(_lib.Q_MAX_ITERS)
         self.evaluations = q(_lib.Q_EVALUATIONS)
         self.launches = q(_lib.Q_LAUNCHES)
+        self.r16_col_items = q(_lib.Q_R16_COL_ITEMS)
+        self.r16_row_items = q(_lib.Q_R16_ROW_ITEMS)
         self.nscenes = len(self.shapes)
         info = (ctypes.c_longlong * 8)()
         self.scene_info = []

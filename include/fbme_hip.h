@@ -89,7 +89,9 @@ enum {
   FBME_Q_MAX_ITERS = 3,
   FBME_Q_NSCENES = 4,
   FBME_Q_EVALUATIONS = 5,
-  FBME_Q_LAUNCHES = 6
+  FBME_Q_LAUNCHES = 6,
+  FBME_Q_R16_COL_ITEMS = 7,   /* column work items on the register 16 x 16 passes */
+  FBME_Q_R16_ROW_ITEMS = 8    /* row work items on the register 16 x 16 passes    */
 };
 
 int fbme_abi_version(void);
