@@ -153,6 +153,25 @@ _SIGNATURES = {
     "fbme_reciprocal": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p]),
     "fbme_grid_sums": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                c_void_p, c_void_p, c_void_p]),
+    # learned warm start (convolutional network on BF16 MFMA)
+    "fbme_cnn_param_count": (c_longlong, []),
+    "fbme_cnn_weight_count": (c_longlong, []),
+    "fbme_cnn_prepare": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_features": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_grad_rows": (c_longlong, [c_int]),
+    "fbme_cnn_part_floats": (c_longlong, [c_int, c_int, c_int]),
+    "fbme_cnn_loss": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_backward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p]),
+    "fbme_cnn_adam": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p,
+                              c_void_p, c_void_p, c_float, c_int]),
+    "fbme_cnn_apply_warm_start": (c_int, [c_void_p, c_int, c_int, c_void_p,
+                                          c_void_p, c_void_p, c_float]),
     "fbme_reduced_posterior": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                        c_int, c_int, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p,

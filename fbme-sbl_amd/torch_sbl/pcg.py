@@ -445,17 +445,27 @@ class _Solver:
         self.have_solution = False
 
 
+class SolverStall(RuntimeError):
+    """Raised when a stall check rejects a warm-started solve."""
+
+
 def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
-           probes_override=None) -> SBLResult:
-    """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64)."""
+           probes_override=None, warm_start=None, stall_check=None) -> SBLResult:
+    """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64).
+
+    warm_start(solver) may replace the initial variances after the plain
+    initialisation; stall_check(solver, pcg_iterations, cfg, sum_gamma_before)
+    is consulted after every EM iteration and raises SolverStall when true."""
     _check_supported(cfg)
     with torch.cuda.device(y.device):
-        return _run(y, grid, cfg, batch_shape, probes_override)
+        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check)
 
 
-def _run(y, grid, cfg, batch_shape, probes_override):
+def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None):
     s = _Solver(y, grid, cfg, probes_override)
     s.init()
+    if warm_start is not None:
+        warm_start(s)
     min_iters = max(cfg.min_iter, cfg.cg_min_iter)
     convergence = cfg.tol > 0.0 and min_iters < cfg.max_iter
     tol_floor = 3.0 * F32_EPS ** 0.5
@@ -471,9 +481,12 @@ def _run(y, grid, cfg, batch_shape, probes_override):
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             s.try_active_set()
         tol = max(_schedule(it, cfg), tol_floor)
+        sum_before = s.sums[:, 0].cpu().numpy() if stall_check is not None else None
         iters = s.posterior(tol)
         s.em(update=True)
         s.grid_sums(with_mean=True)
+        if stall_check is not None and stall_check(s, iters, cfg, sum_before):
+            raise SolverStall(f"warm-started solve stalled at iteration {it}")
         check = (convergence and it >= min_iters
                  and ((it - min_iters) % cfg.convergence_check_interval == 0 or it == cfg.max_iter))
         need = cfg.return_history or check or cfg.verbose or it == cfg.max_iter

@@ -279,6 +279,45 @@ int fbme_reduced_posterior(void* stream, int batch, int k, int n1, int n2, int m
                            const double* ye, const void* dir1, const void* dir2, float jitter_multiplier,
                            void* mean, float* var, float* dd, int* status);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start of the general solver (no counterpart in the
+ * reference, which has no learned component): a 4-layer 3x3 convolutional
+ * network, channels 16 -> 32 -> 32 -> 32 -> 16 (input channels 0, 1 =
+ * log|A^H y|^2 and log gamma, output channel 0 = per-cell log-variance
+ * correction), BF16 weights and activations on MFMA with FP32 accumulation,
+ * trained on the GPU by these kernels.  Activations are [B, H, W, C] bf16;
+ * parameters fp32, per layer W[9][ci][co] then b[co].
+ * ------------------------------------------------------------------------ */
+long long fbme_cnn_param_count(void);
+long long fbme_cnn_weight_count(void);
+/* bf16 forward (wf) and backward (wb) weights from the fp32 parameters,
+ * fbme_cnn_weight_count() values each. */
+int fbme_cnn_prepare(void* stream, const float* params, void* wf, void* wb);
+/* x [B, n, 16] bf16 from the backprojection bp [B, n] complex64 and gamma
+ * [B, n]; bpe = sum |bp|^2 and gsum = sum gamma per scene (FP64). */
+int fbme_cnn_features(void* stream, int batch, int n, const void* bp, const float* gamma, const double* bpe,
+                      const double* gsum, void* x);
+/* out [B, H, W] fp32; act: 3 * B * H * W * 32 bf16 hidden activations. */
+int fbme_cnn_forward(void* stream, int batch, int h, int w, const void* x, const void* wf, const float* params,
+                     void* act, float* out);
+long long fbme_cnn_grad_rows(int h);
+/* fp32 partial-sum workspace of fbme_cnn_backward. */
+long long fbme_cnn_part_floats(int batch, int h, int w);
+/* Mean squared error of out against target over `count` cells, its gradient
+ * dz4 [count, 16] bf16, loss (FP64) via part (ceil(count / 256) doubles). */
+int fbme_cnn_loss(void* stream, long long count, const float* out, const float* target, void* dz4, double* part,
+                  double* loss);
+/* Parameter gradients (fp32, parameter layout) from the forward
+ * activations; dz: 2 * B * H * W * 32 bf16 workspace. */
+int fbme_cnn_backward(void* stream, int batch, int h, int w, const void* x, const void* act, const void* wb,
+                      const void* dz4, void* dz, float* part, float* grads);
+/* Adam step (beta 0.9 / 0.999, eps 1e-8, bias-corrected at `step` >= 1). */
+int fbme_cnn_adam(void* stream, long long n, float* params, const float* grads, float* m, float* v, float lr,
+                  int step);
+/* gamma <- max(gamma exp(clamp(out, +-clampv)), floor[b]). */
+int fbme_cnn_apply_warm_start(void* stream, int batch, int n, float* gamma, const float* out, const float* floor_,
+                              float clampv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,5 +121,34 @@ def main(reference_root: str) -> None:
     print("wrote general_32x32_24x24")
 
 
+def long_cg(reference_root: str) -> None:
+    """A batch of four 64x64 / 40x40 ellipse scenes (10 dB) through 100
+    iterations of the CG path with 16 probes: the long-run regime where
+    float32 Hutchinson estimates meet strongly pruned variances."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    batch = [scenes.ellipse_scene(1000 + i, (64, 64), (40, 40), k=30, snr_db=10.0)
+             for i in range(4)]
+    y = np.stack([s.y for s in batch]).astype(np.complex64)
+    cfg = SBLConfig(device="cpu", mode="cg", hutchinson_probes=16,
+                    auto_probe_scaling=False, cg_tolerance=1e-4, max_iter=100)
+    res = sbl_2d(torch.from_numpy(y), (64, 64), config=cfg)
+    arrays = dict(y=y, x_true=np.stack([s.x for s in batch]),
+                  grid=np.array((64, 64)), samples=np.array((40, 40)))
+    arrays.update(_result_arrays(res, "cg_"))
+    for key in ("relative_change", "cg_iterations", "noise_variance"):
+        arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+    np.savez_compressed(os.path.join(HERE, "general_64x64_40x40_b4.npz"), **arrays)
+    print("wrote general_64x64_40x40_b4")
+
+
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    if len(sys.argv) > 2 and sys.argv[2] == "long_cg":
+        long_cg(root)
+    else:
+        main(root)
+        long_cg(root)
