@@ -5,10 +5,12 @@ B synthetic scenes (256x256 grid, 160x160 prefix, K=300 log-normal point
 scatterers in a random ellipse, 10 dB SNR, polish_size 32), inputs resident
 in HBM.  value = B * steps / elapsed seconds (scenes/s).
 
-Also reported: the plain single-scene plan's scenes/s on the same GPU (the
-north-star comparison), the NMSE of both on the same scenes, the roofline of
-the dominant kernel (the fused row pass) timed with events on the stream it
-runs on, and the float64 CPU oracle timed on the host.
+Also reported: the plain single-scene plan's scenes/s on the same GPU, the
+NMSE of both on the same scenes, the roofline of the dominant kernel (the
+fused row pass) timed with events on the stream it runs on, the float64 CPU
+oracle timed on the host, a 64-scene uneven workload in one graph-captured
+plan (autotuned pass shapes against the defaults), and the learned warm start
+against the plain general solver (PCG iterations, scenes/s, NMSE).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -42,7 +44,92 @@ def parse():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (counter calibration)")
+    ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--learned-scenes", type=int, default=4)
     return ap.parse_args()
+
+
+def uneven_workload(device, reps=20):
+    """64 scenes, grids from {64, 128, 192, 256}^2, per-axis sampling ratios
+    in [0.3, 0.8], K in [0.5 %, 3 %] of the cells, SNR in [5, 25] dB: one
+    graph-captured plan, pass shapes from the autotuner's table against the
+    default shapes."""
+    import numpy as np
+    import torch
+
+    from fbme_amd import _lib, scenes
+    from fbme_amd.batched import UnevenPlan
+    from fbme_amd.bytes_model import survey_scene_bytes
+    batch = scenes.uneven_batch(2024, count=64)
+    shapes = [(s.grid[0], s.grid[1], s.samples[0], s.samples[1]) for s in batch]
+    ys = [torch.from_numpy(s.y).to(device) for s in batch]
+    out = {"workload": "64 uneven scenes (grids 64^2..256^2, ratios 0.3-0.8), FBME-Fast, one plan + hipGraph"}
+    for name, opt in (("tuned", None), ("default", _lib.default_options(False))):
+        plan = UnevenPlan(shapes, device=device, options=opt)
+        plan.replay(ys)
+        for _ in range(3):
+            plan.replay()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            plan.replay()
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / reps
+        nm = [scenes.nmse_db(plan.scene(i)["x"].reshape(-1).cpu().numpy(), s.x.reshape(-1))
+              for i, s in enumerate(batch)]
+        t_us = [plan.plan.scene_info[i][2] for i in range(len(batch))]
+        model = sum(survey_scene_bytes(a * b, c * d, t) for (a, b, c, d), t in zip(shapes, t_us))
+        out[name] = {"ms_per_replay": ms, "scenes_per_s": len(batch) / (ms / 1e3),
+                     "nmse_db": float(np.mean(nm)), "model_GB_per_s": model / (ms / 1e3) / 1e9,
+                     "options": {"fft_kernel": plan.options.fft_kernel, "row_lines": plan.options.row_lines,
+                                 "col_lines": plan.options.col_lines}}
+        del plan
+    return out
+
+
+def learned_comparison(device, count):
+    """Learned warm start against the plain general solver (mode='cg', 16
+    probes, PCG tolerance 1e-4), one 256x256/160x160 K=300 10 dB scene per
+    solve, held-out seeds."""
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, learned, pcg
+    if not os.path.exists(learned.WEIGHTS):
+        return None
+    net = learned.WarmStartNet.load(device)
+    sc = [scenes.ellipse_scene(1300 + i, (256, 256), (160, 160), k=300, snr_db=10.0) for i in range(count)]
+    runs = [("plain_reference_budget", "plain", 100), ("plain_target_iter", "plain", net.target_iter),
+            ("plain_warm_budget", "plain", net.warm_iter), ("learned", "learned", net.warm_iter)]
+    out = {"workload": f"{count} single-scene solves, 256x256/160x160, K=300, 10 dB",
+           "target_iter": net.target_iter, "warm_iter": net.warm_iter}
+    for name, kind, iters in runs:
+        cfg = SBLConfig(mode="cg", device=str(device), max_iter=iters, min_iter=1, tol=0.0,
+                        hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
+        y0 = torch.from_numpy(sc[0].y[None]).to(device)
+        (learned.learned_sbl_2d(y0, (256, 256), net, cfg) if kind == "learned"
+         else pcg.sbl_cg(y0, (256, 256), cfg, batch_shape=(1,)))
+        torch.cuda.synchronize()
+        secs, its, nm, fb = [], [], [], 0.0
+        for s in sc:
+            y = torch.from_numpy(s.y[None]).to(device)
+            t0 = time.perf_counter()
+            r = (learned.learned_sbl_2d(y, (256, 256), net, cfg) if kind == "learned"
+                 else pcg.sbl_cg(y, (256, 256), cfg, batch_shape=(1,)))
+            torch.cuda.synchronize()
+            secs.append(time.perf_counter() - t0)
+            its.append(float(np.sum(r.history["cg_iterations"])))
+            nm.append(scenes.nmse_db(r.x.reshape(-1).cpu().numpy(), s.x.reshape(-1)))
+            fb += r.history.get("learned_fallback", [0.0])[0]
+        out[name] = {"em_iterations": iters, "pcg_iterations": float(np.mean(its)),
+                     "scenes_per_s": 1.0 / float(np.mean(secs)), "nmse_db": float(np.mean(nm)),
+                     "median_nmse_db": float(np.median(nm)), "fallbacks": fb}
+        print(f"[bench] learned comparison {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -174,6 +261,15 @@ def main():
                "sample": f"{count} scenes of the same 256x256/160x160 workload, "
                          "float64 NumPy oracle (single-threaded pocketfft)"}
 
+    uneven = None
+    if not args.skip_uneven:
+        print("[bench] uneven workload", file=sys.stderr, flush=True)
+        uneven = uneven_workload(device)
+        print(f"[bench] uneven: {uneven}", file=sys.stderr, flush=True)
+    learned_cmp = None
+    if not args.skip_learned:
+        learned_cmp = learned_comparison(device, args.learned_scenes)
+
     t_u = native.max_iters
     line = {
         "metric": "scenes/s at 256x256 (160x160 prefix), batched FBME-"
@@ -193,6 +289,8 @@ def main():
         "speedup_vs_plain": (value / plain["scenes_per_s"]) if plain else None,
         "nmse_db_batched": nmse_batched,
         "per_kernel": per_kernel,
+        "uneven_workload": uneven,
+        "learned_warm_start": learned_cmp,
         "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),
         "wall_s": wall,

@@ -698,10 +698,18 @@ __global__ void k_reciprocal(const float* __restrict__ in, float* __restrict__ o
 
 inline int nblk(long long n, int bs = FBME_BLOCK) { return (int)((n + bs - 1) / bs); }
 
+// Rows per workgroup of the row passes: as many padded rows as the LDS tile
+// holds (at most 64), or `want` when it is positive and smaller.
+template <int EPT>
+int rowpass_rows(int n, int want) {
+  const int cap = std::max(1, std::min<int>(EPT * FBME_BLOCK / (n + 1), 64));
+  return want > 0 ? std::min(want, cap) : cap;
+}
+
 template <int EPT>
 void launch_rowpass(bool inv, const float2* in, float2* out, int n, int len_in, int len_out, long long rows,
-                    unsigned code, const float2* tw, hipStream_t st) {
-  const int tr = std::max(1, std::min<int>(EPT * FBME_BLOCK / (n + 1), 64));
+                    unsigned code, const float2* tw, hipStream_t st, int want_rows = 0) {
+  const int tr = rowpass_rows<EPT>(n, want_rows);
   const int grid = (int)((rows + tr - 1) / tr);
   if (inv)
     hipLaunchKernelGGL((k_rowpass<EPT, true>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out,
@@ -744,15 +752,17 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
   return (int)hipGetLastError();
 }
 
-int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
-                         unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
-                         void* work_b, const void* tw_e1, const void* tw_e2) {
+int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
+                               unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
+                               void* work_b, const void* tw_e1, const void* tw_e2, int row_tile, int col_tile) {
   // work_a, work_b: lanes * M1 * E2 complex each.
+  if (row_tile < 0 || col_tile < 0) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   launch_rowpass<4>(false, (const float2*)d, (float2*)work_a, e2, m2, e2, (long long)lanes * m1, code_e2,
-                    (const float2*)tw_e2, st);
+                    (const float2*)tw_e2, st, row_tile);
   const int ept = (e1 <= 1024 / 8) ? 4 : (e1 <= 512) ? 8 : 16;
-  const int tc = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
+  const int tc_cap = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
+  const int tc = col_tile > 0 ? std::min(col_tile, tc_cap) : tc_cap;
   dim3 grid((e2 + tc - 1) / tc, lanes);
   if (ept == 4)
     hipLaunchKernelGGL((k_toeplitz_col<4>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
@@ -764,8 +774,23 @@ int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e
     hipLaunchKernelGGL((k_toeplitz_col<16>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
                        (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
   launch_rowpass<4>(true, (const float2*)work_b, (float2*)out, e2, e2, m2, (long long)lanes * m1, code_e2,
-                    (const float2*)tw_e2, st);
+                    (const float2*)tw_e2, st, row_tile);
   return (int)hipGetLastError();
+}
+
+int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
+                         unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
+                         void* work_b, const void* tw_e1, const void* tw_e2) {
+  return fbme_toeplitz_matvec_tiled(stream, lanes, rhs, m1, m2, e1, e2, code_e1, code_e2, spectrum, d, out, work_a,
+                                    work_b, tw_e1, tw_e2, 0, 0);
+}
+
+int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max) {
+  if (e1 < 1 || e2 < 1 || e1 > 1024 || e2 > 1024) return FBME_ERR_ARG;
+  const int ept = (e1 <= 1024 / 8) ? 4 : (e1 <= 512) ? 8 : 16;
+  *row_tile_max = rowpass_rows<4>(e2, 0);
+  *col_tile_max = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
+  return 0;
 }
 
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val) {

@@ -116,6 +116,11 @@ _SIGNATURES = {
                                      c_int, c_int, c_uint, c_uint, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p]),
+    "fbme_toeplitz_matvec_tiled": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                           c_int, c_int, c_uint, c_uint, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, c_int, c_int]),
+    "fbme_toeplitz_tiles": (c_int, [c_int, c_int, c_void_p, c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
     "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,

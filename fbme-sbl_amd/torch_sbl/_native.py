@@ -15,7 +15,7 @@ from math import ceil, prod
 
 import torch
 
-from fbme_amd import _lib
+from fbme_amd import _lib, autotune
 from fbme_amd.plan import NativePlan
 
 from .config import SBLConfig, SBLResult
@@ -123,7 +123,14 @@ def uamp_budget(cfg: SBLConfig, n: int, m: int) -> int:
     return min(cfg.max_iter, max(minimum, budget))
 
 
+def tuned(shapes, opt: _lib.FbmePlanOptions, device) -> _lib.FbmePlanOptions:
+    """opt with the autotuned FFT pass shapes of this workload on this GPU
+    (unchanged when the workload was never tuned)."""
+    return autotune.tuned_plan_options(shapes, opt.exchange_size > 0, device, opt)
+
+
 def cached_plan(shapes, opt, device) -> NativePlan:
+    opt = tuned(shapes, opt, device)
     key = (tuple(shapes), options_key(opt), str(device))
     plan = _PLAN_CACHE.get(key)
     if plan is None:

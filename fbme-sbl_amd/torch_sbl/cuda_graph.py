@@ -70,6 +70,7 @@ class CUDAGraphSBLPlan:
         opt = _native.plan_options(self.config, self.grid_shape, samples)
         shapes = [(self.grid_shape[0], self.grid_shape[1], samples[0],
                    samples[1])] * batch
+        opt = _native.tuned(shapes, opt, self.device)
         with torch.cuda.device(self.device):
             self._plan = _native.NativePlan(shapes, opt, self.device)
             self._views = _native.ResultViews(self._plan, self.config,

@@ -10,7 +10,8 @@ tail plus the strongest atoms), also precondition the first PCG solves.
 The network (4 layers, 3x3, 32 channels, BF16 weights on MFMA with FP32
 accumulation) runs and trains on the GPU through the fbme_cnn_* entry points
 of libfbme_hip.so.  Training data are trajectories of the plain solver:
-inputs from its initial state, targets from its converged variances.
+inputs from its initial state, targets from its variances after target_iter
+EM iterations, so a warm-started solve skips that part of the trajectory.
 
 The plain solver is the fallback: if a warm-started solve stalls (a PCG solve
 exhausts cg_max_iter, the variances blow up, or the noise becomes
@@ -90,13 +91,22 @@ class WarmStartNet:
         self.m = torch.zeros_like(self.params)
         self.v = torch.zeros_like(self.params)
         self.step = 0
+        self.target_iter = 10
+        self.warm_iter = 2
         self._ws = {}
         self.prepare()
 
     @classmethod
     def load(cls, device, path: str = WEIGHTS) -> "WarmStartNet":
+        """Trained weights plus the trajectory iteration they imitate
+        (target_iter) and the EM budget they were evaluated with (warm_iter)."""
         with np.load(path) as z:
-            return cls(device, z["params"])
+            net = cls(device, z["params"])
+            if "target_iter" in z:
+                net.target_iter = int(z["target_iter"])
+            if "warm_iter" in z:
+                net.warm_iter = int(z["warm_iter"])
+        return net
 
     def save(self, path: str, **meta) -> None:
         os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
@@ -170,8 +180,10 @@ def _stalled(solver, iters, cfg, sum_before) -> bool:
 
 def learned_sbl_2d(data, grid_shape, net: WarmStartNet, config: SBLConfig | None = None) -> SBLResult:
     """sbl_2d(mode='cg') started from the network's variances, with the plain
-    solver as fallback (see the module docstring)."""
-    cfg = config or SBLConfig(mode="cg", device=str(net.device))
+    solver as fallback (see the module docstring).  Without a config the solve
+    runs net.warm_iter EM iterations (16 probes, PCG tolerance 1e-4)."""
+    cfg = config or SBLConfig(mode="cg", device=str(net.device), max_iter=net.warm_iter, min_iter=1, tol=0.0,
+                              hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
     y = torch.as_tensor(data)
     batch_shape = tuple(y.shape[:-2])
     b = prod(batch_shape) if batch_shape else 1
@@ -197,13 +209,14 @@ def learned_sbl_2d(data, grid_shape, net: WarmStartNet, config: SBLConfig | None
 
 def trajectory_pairs(y: torch.Tensor, grid, cfg: SBLConfig, net: WarmStartNet):
     """Run the plain solver on y [B, M1, M2]; return network inputs from its
-    initial state and targets log(gamma_final / gamma_0), clamped."""
-    res = pcg.sbl_cg(y, grid, cfg)
+    initial state, targets log(gamma_final / gamma_0) (clamped), the result
+    and a per-scene validity mask (finite targets and noise)."""
+    res = pcg.sbl_cg(y, grid, cfg, batch_shape=(y.shape[0],))
     with torch.cuda.device(y.device):
         s = pcg._Solver(y, grid, cfg)
         s.init()
         x = net.features(s.bp, s.gamma, s.bpe, s.sums[:, 0].contiguous(), grid)
         gfinal = 1.0 / res.precision.reshape(s.b, -1)
         target = torch.log(gfinal / s.gamma).clamp(-CLAMP, CLAMP).reshape(s.b, *grid).float().contiguous()
-    ok = torch.isfinite(target).all(dim=(1, 2)) & torch.isfinite(res.noise_variance.reshape(-1))
-    return x[ok].contiguous(), target[ok].contiguous(), res
+    valid = torch.isfinite(target).all(dim=(1, 2)) & torch.isfinite(res.noise_variance.reshape(-1))
+    return x, torch.nan_to_num(target), res, valid

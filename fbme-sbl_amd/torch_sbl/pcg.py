@@ -32,7 +32,7 @@ from math import prod
 import numpy as np
 import torch
 
-from fbme_amd import _lib
+from fbme_amd import _lib, autotune
 
 from .config import SBLConfig, SBLResult
 from .operators import _next_smooth_length
@@ -159,6 +159,7 @@ class _Solver:
         self.p = _probe_count(self.n, self.m, cfg)
         self.r = 1 + self.p
         self.lanes = self.b * self.r
+        self.tiles = autotune.cg_tiles(self.dev, self.lanes, *self.samples, self.t.e1, self.t.e2)
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
@@ -271,10 +272,10 @@ class _Solver:
     def matvec(self, d, out):
         t = self.t
         m1, m2 = self.samples
-        self._chk(self.lib.fbme_toeplitz_matvec(
+        self._chk(self.lib.fbme_toeplitz_matvec_tiled(
             self.stream, self.lanes, self.r, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
             _ptr(self.spectrum), _ptr(d), _ptr(out), _ptr(self.work_a), _ptr(self.work_b),
-            _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])), "toeplitz_matvec")
+            _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2]), self.tiles[0], self.tiles[1]), "toeplitz_matvec")
 
     def precondition(self, v, out):
         self._chk(self.lib.fbme_lowrank_apply(

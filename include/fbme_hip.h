@@ -206,6 +206,16 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
 int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
                          unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
                          void* work_b, const void* tw_e1, const void* tw_e2);
+/* fbme_toeplitz_matvec with explicit workgroup tiles (same arithmetic, same
+ * results): row_tile rows per workgroup of the two E2-point row passes,
+ * col_tile columns per workgroup of the E1-point column pass; 0 takes the
+ * largest tile LDS holds, larger values are capped to it.  Tuning hook of
+ * the same operator (operators.py:328-355); fbme_toeplitz_tiles reports the
+ * caps. */
+int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
+                               unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
+                               void* work_b, const void* tw_e1, const void* tw_e2, int row_tile, int col_tile);
+int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
  * index (the stable argsort of solver.py:5797-5800).  cand: batch *
  * ceil(n / 1024) * 32 uint64; requires n <= 65536. */

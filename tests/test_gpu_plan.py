@@ -99,6 +99,23 @@ def test_full_size_scene_vs_oracle(device, profile):
     assert abs(float(res.noise_variance) / float(orc["noise_variance"]) - 1) < 1e-4
 
 
+@pytest.mark.parametrize("snr_db", [0.0, 10.0, 20.0])
+def test_stable_512_nonsquare_prefix_vs_oracle(device, snr_db):
+    """512^2 grid, 384 x 256 prefix, K=1200 (six clustered ellipses plus 10 %
+    isolated points), Stable plan, SNR swept over {0, 10, 20} dB."""
+    sc = scenes.clustered_scene(1000, (512, 512), (384, 256), k=1200, snr_db=snr_db)
+    data = torch.from_numpy(sc.y).to(device)
+    print("512x512/384x256 stable snr", snr_db, flush=True)
+    plan = _make("stable", data, (512, 512), device)
+    res = plan.replay(synchronize=True)
+    x = res.x.cpu().numpy().reshape(-1)
+    orc = fo.fbme_plan(sc.y, (512, 512), "stable")
+    assert _rel(x, orc["x"]) < 1e-4
+    assert np.array_equal(res.support_mask.cpu().numpy().reshape(-1), orc["support_mask"])
+    assert abs(float(res.noise_variance) / float(orc["noise_variance"]) - 1) < 1e-4
+    assert int(plan.native_plan.out.status[0]) == 0
+
+
 def test_graph_replay_new_data_matches_eager(device):
     from torch_sbl import SBLConfig, fast_sbl
     a = scenes.uniform_scene(3001, (64, 64), (40, 40), k=15, snr_db=15.0)
