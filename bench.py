@@ -45,9 +45,88 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--skip-pcg", action="store_true")
+    ap.add_argument("--pcg-batch", type=int, default=8)
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--learned-scenes", type=int, default=4)
     return ap.parse_args()
+
+
+def pcg_kernels(device, batch=8, reps=20):
+    """The PCG entry points of the general solver at 256x256/160x160 with 16
+    Hutchinson probes (lanes L = 17 B): the operators are set up from the
+    solver's state after one EM iteration, then each entry point is timed
+    alone with events on its stream.  Algorithmic bytes per call (complex64
+    = 8 B; M = M1 M2): covariance matvec on the 256 x 256 grid 16 L M +
+    32 L 256 M2 + 4 B 65536 (column pass of A^H, row pass with Gamma,
+    column pass of A; the [256, M2] intermediate round-trips twice; gamma
+    once per scene), or through the BTTB embedding E1 x E2 16 L M +
+    32 L M1 E2 + 8 B E1 E2; low-rank preconditioner 16 L M (the factor is generated from
+    its separable phase tables, 80 KB per scene); lane dot 16 L M; update
+    48 L M; direction 24 L M; one PCG iteration = matvec + precondition + 2
+    dots + update + direction."""
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, pcg
+    grid, samples = (256, 256), (160, 160)
+    sc = [scenes.ellipse_scene(1400 + i, grid, samples, k=300, snr_db=10.0) for i in range(batch)]
+    y = torch.from_numpy(np.stack([s.y for s in sc]).astype(np.complex64)).to(device)
+    cfg = SBLConfig(mode="cg", device=str(device), hutchinson_probes=16, auto_probe_scaling=False,
+                    cg_tolerance=1e-4, max_iter=3, min_iter=1, tol=0.0)
+    s = pcg._Solver(y, grid, cfg)
+    s.init()
+    s.posterior(1e-3)
+    s.em(update=True)
+    s.grid_sums(with_mean=True)
+    s.setup_operators()
+    s.d.copy_(s.rhs)
+    s.res.copy_(s.rhs)
+    lib = s.lib
+    L, M = s.lanes, s.m
+    m1 = samples[0]
+    e1, e2 = s.t.e1, s.t.e2
+    calls = {
+        "matvec": (lambda: s.matvec(s.d, s.q),
+                   (16 * L * M + 32 * L * 256 * samples[1] + 4 * batch * 65536) if s.grid_op
+                   else (16 * L * M + 32 * L * m1 * e2 + 8 * batch * e1 * e2)),
+        "precondition": (lambda: s.precondition(s.res, s.z), 16 * L * M),
+        "lane_dot": (lambda: s.lane_dot(s.d, s.q, s.curv), 16 * L * M),
+        "update": (lambda: lib.fbme_pcg_update(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.d),
+                                               pcg._ptr(s.q), pcg._ptr(s.rho), pcg._ptr(s.curv),
+                                               pcg._ptr(s.improved)), 48 * L * M),
+        "direction": (lambda: lib.fbme_pcg_direction(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z),
+                                                     pcg._ptr(s.nrho), pcg._ptr(s.nrho), pcg._ptr(s.improved), 1),
+                      24 * L * M),
+    }
+    s.rho.fill_(1.0)
+    s.curv.fill_(2.0)
+    s.nrho.fill_(1.0)
+    s.improved.fill_(1)
+    out = {"workload": f"{batch} scenes x 17 lanes (data + 16 probes), 256x256/160x160",
+           "lanes": L, "matvec_path": "grid 256x256 (register FFT)" if s.grid_op else f"BTTB {e1}x{e2}"}
+    total_ms = total_bytes = 0.0
+    for name, (fn, nbytes) in calls.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            fn()
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / reps
+        out[name] = {"ms": ms, "bytes": nbytes, "GB/s": nbytes / (ms / 1e3) / 1e9,
+                     "frac": nbytes / (ms / 1e3) / HBM_PEAK}
+        weight = 2 if name == "lane_dot" else 1
+        total_ms += weight * ms
+        total_bytes += weight * nbytes
+    out["pcg_iteration"] = {"ms": total_ms, "bytes": total_bytes, "GB/s": total_bytes / (total_ms / 1e3) / 1e9,
+                            "frac": total_bytes / (total_ms / 1e3) / HBM_PEAK,
+                            "us_per_scene": total_ms * 1e3 / batch}
+    return out
 
 
 def uneven_workload(device, reps=20):
@@ -266,6 +345,11 @@ def main():
         print("[bench] uneven workload", file=sys.stderr, flush=True)
         uneven = uneven_workload(device)
         print(f"[bench] uneven: {uneven}", file=sys.stderr, flush=True)
+    pcg_cmp = None
+    if not args.skip_pcg:
+        print("[bench] pcg kernels", file=sys.stderr, flush=True)
+        pcg_cmp = pcg_kernels(device, args.pcg_batch)
+        print(f"[bench] pcg: {pcg_cmp}", file=sys.stderr, flush=True)
     learned_cmp = None
     if not args.skip_learned:
         learned_cmp = learned_comparison(device, args.learned_scenes)
@@ -290,6 +374,7 @@ def main():
         "nmse_db_batched": nmse_batched,
         "per_kernel": per_kernel,
         "uneven_workload": uneven,
+        "pcg_kernels": pcg_cmp,
         "learned_warm_start": learned_cmp,
         "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),

@@ -165,24 +165,31 @@ k_topk_merge(const unsigned long long* __restrict__ cand, int tiles, const float
 }
 
 // ------------------------------------------------------ low-rank factors
-// F[b, m, k] = exp(-2 pi i (m1 k1 / N1 + m2 k2 / N2)) sqrt(gamma_k), phase
-// reduced modulo N in integers.
-__global__ void k_lr_factors(const int* __restrict__ idx, const float* __restrict__ val, float2* __restrict__ f,
-                             int n1, int n2, int m1, int m2) {
+// The factor matrix F[m, k] = exp(-2 pi i (m1 k1 / N1 + m2 k2 / N2))
+// sqrt(gamma_k) is separable: F[m1 M2 + m2, k] = U[m1, k] V[m2, k] with
+// U[m1, k] = exp(-2 pi i (m1 k1 mod N1) / N1) and
+// V[m2, k] = exp(-2 pi i (m2 k2 mod N2) / N2) sqrt(gamma_k) (phases reduced
+// modulo N in integers).  tab[b] = U [M1, RANK] followed by V [M2, RANK];
+// the M x RANK matrix itself is never formed.
+__global__ void k_lr_tables(const int* __restrict__ idx, const float* __restrict__ val, float2* __restrict__ tab,
+                            int n1, int n2, int m1, int m2) {
   const int b = blockIdx.y;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = m1 * m2;
-  if (i >= (long long)m * RANK) return;
-  const int row = (int)(i / RANK), k = (int)(i - (long long)row * RANK);
-  const int a1 = row / m2, a2 = row - a1 * m2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (m1 + m2) * RANK) return;
+  const int row = i / RANK, k = i - row * RANK;
   const int g = idx[b * RANK + k];
   const int k1 = g / n2, k2 = g - k1 * n2;
-  const long long p1 = ((long long)a1 * k1) % n1, p2 = ((long long)a2 * k2) % n2;
-  const double ph = (double)p1 / n1 + (double)p2 / n2;   // in [0, 2)
+  double ph;
+  float amp = 1.f;
+  if (row < m1) {
+    ph = (double)(((long long)row * k1) % n1) / n1;
+  } else {
+    ph = (double)(((long long)(row - m1) * k2) % n2) / n2;
+    amp = sqrtf(fmaxf(val[b * RANK + k], 0.f));
+  }
   double s, c;
   sincospi(-2.0 * ph, &s, &c);
-  const float sq = sqrtf(fmaxf(val[b * RANK + k], 0.f));
-  f[((long long)b * m + row) * RANK + k] = make_float2((float)c * sq, (float)s * sq);
+  tab[(long long)b * (m1 + m2) * RANK + i] = make_float2((float)c * amp, (float)s * amp);
 }
 
 // One wavefront per scene: small = I + F^H F / bg with F^H F from the
@@ -301,57 +308,83 @@ k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const dou
   }
 }
 
-// proj[(b, r), k] partial over a chunk of m: sum conj(F[b, m, k]) v[(b, r), m].
+// Projection partials proj[(b, r), k] = sum_m conj(F[m, k]) v[(b, r), m] over
+// the rows [row0, row0 + rb) of scene b, for the LG lanes r0 .. r0 + LG - 1
+// of one lane group: F is generated from the U / V tables once per sample
+// and applied to every lane of the group.  Thread (g, k) = (tid / 32,
+// tid % 32) takes the samples m2 = g (mod 8) of each row; the V values of
+// its samples stay in registers across the rows of the block.
+constexpr int LR_TILE = 128;
+
+template <int LG>
 __global__ void __launch_bounds__(FBME_BLOCK)
-k_lr_project(const float2* __restrict__ f, const float2* __restrict__ v, int m, int rhs, int chunk,
+k_lr_project(const float2* __restrict__ tab, const float2* __restrict__ v, int m1, int m2, int rhs, int rb,
              double2* __restrict__ part) {
-  __shared__ float2 fs[64][RANK + 1];
-  __shared__ float2 vs[64];
-  __shared__ double2 red[4][RANK];
-  const int lanev = blockIdx.y;
-  const int b = lanev / rhs;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = blockIdx.x * chunk;
-  const int m1 = min(m, m0 + chunk);
-  double2 acc = make_double2(0.0, 0.0);
-  const int k = lane & 31, half = lane >> 5;   // 2 halves x 32 atoms per wave
-  for (int base = m0; base < m1; base += 64) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * RANK; e += FBME_BLOCK) {
-      const int mm = e / RANK, kk = e - mm * RANK;
-      fs[mm][kk] = (base + mm < m1) ? f[((long long)b * m + base + mm) * RANK + kk] : make_float2(0.f, 0.f);
-    }
-    if (threadIdx.x < 64)
-      vs[threadIdx.x] = (base + threadIdx.x < m1) ? v[(long long)lanev * m + base + threadIdx.x]
-                                                  : make_float2(0.f, 0.f);
-    __syncthreads();
-    // 8 groups (4 waves x 2 halves) split the 64 samples of the chunk.
-    const int g = wave * 2 + half;
+  constexpr int LDS = (LG * LR_TILE > 8 * LG * RANK) ? LG * LR_TILE : 8 * LG * RANK;
+  __shared__ float2 buf[LDS];
+  const int b = blockIdx.z;
+  const int r0 = blockIdx.y * LG;
+  const int nl = min(LG, rhs - r0);
+  const int k = threadIdx.x & (RANK - 1), g = threadIdx.x >> 5;
+  const int row0 = blockIdx.x * rb, row1 = min(m1, row0 + rb);
+  const long long m = (long long)m1 * m2;
+  const float2* U = tab + (long long)b * (m1 + m2) * RANK;
+  const float2* V = U + (long long)m1 * RANK;
+  const float2* vb = v + ((long long)b * rhs + r0) * m;
+  float2 acc[LG];
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int mm = g * 8 + t;
-      const float2 fv = fs[mm][k], vv = vs[mm];
-      acc.x += (double)fv.x * vv.x + (double)fv.y * vv.y;
-      acc.y += (double)fv.x * vv.y - (double)fv.y * vv.x;
+  for (int j = 0; j < LG; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (int t0 = 0; t0 < m2; t0 += LR_TILE) {
+    const int tl = min(LR_TILE, m2 - t0);
+    float2 cv[LR_TILE / 8];   // conj(V) of this thread's samples in the tile
+#pragma unroll
+    for (int i = 0; i < LR_TILE / 8; ++i) {
+      const int c = g + 8 * i;
+      const float2 w = (c < tl) ? V[(long long)(t0 + c) * RANK + k] : make_float2(0.f, 0.f);
+      cv[i] = make_float2(w.x, -w.y);
+    }
+    for (int row = row0; row < row1; ++row) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < LG * tl; e += FBME_BLOCK) {
+        const int j = e / tl, c = e - j * tl;
+        buf[j * LR_TILE + c] = (j < nl) ? vb[(long long)j * m + (long long)row * m2 + t0 + c] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      const float2 u = U[(long long)row * RANK + k];
+      const float2 cu = make_float2(u.x, -u.y);
+#pragma unroll
+      for (int i = 0; i < LR_TILE / 8; ++i) {
+        const int c = g + 8 * i;
+        if (c < tl) {
+          const float2 f = cmul(cu, cv[i]);
+#pragma unroll
+          for (int j = 0; j < LG; ++j) {
+            const float2 w = buf[j * LR_TILE + c];
+            acc[j].x += f.x * w.x - f.y * w.y;
+            acc[j].y += f.x * w.y + f.y * w.x;
+          }
+        }
+      }
     }
   }
-  // reduce the 8 groups: halves within a wave by shuffle, waves through LDS
-  acc.x += __shfl_down(acc.x, 32, 64);
-  acc.y += __shfl_down(acc.y, 32, 64);
+  // reduce the 8 sample groups through LDS, in FP64
   __syncthreads();
-  if (half == 0) red[wave][k] = acc;
+#pragma unroll
+  for (int j = 0; j < LG; ++j) buf[(g * LG + j) * RANK + k] = acc[j];
   __syncthreads();
-  if (threadIdx.x < RANK) {
-    double2 s = red[0][threadIdx.x];
-    for (int w = 1; w < 4; ++w) {
-      s.x += red[w][threadIdx.x].x;
-      s.y += red[w][threadIdx.x].y;
+  for (int e = threadIdx.x; e < nl * RANK; e += FBME_BLOCK) {
+    const int j = e / RANK, kk = e - j * RANK;
+    double2 sum = make_double2(0.0, 0.0);
+    for (int q = 0; q < 8; ++q) {
+      const float2 w = buf[(q * LG + j) * RANK + kk];
+      sum.x += w.x;
+      sum.y += w.y;
     }
-    part[((long long)lanev * gridDim.x + blockIdx.x) * RANK + threadIdx.x] = s;
+    part[(((long long)b * rhs + r0 + j) * gridDim.x + blockIdx.x) * RANK + kk] = sum;
   }
 }
 
-// corr[(b, r), :] = S^-1 (proj / bg) / bg, from the chunk partials.
+// corr[(b, r), :] = S^-1 (proj / bg) / bg, from the row-chunk partials.
 __global__ void __launch_bounds__(64)
 k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict__ sinv,
           const float* __restrict__ bg, int rhs, float2* __restrict__ corr) {
@@ -381,28 +414,49 @@ k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict
   }
 }
 
-// out = v / bg - F corr  (corr already carries the 1 / bg^2 factor).
+// out = v / bg - F corr (corr already carries the 1 / bg^2 factor): one
+// sample per thread, the 32 factor entries of the sample generated once from
+// U / V and applied to every lane of the group.
+template <int LG>
 __global__ void __launch_bounds__(FBME_BLOCK)
-k_lr_apply(const float2* __restrict__ f, const float2* __restrict__ v, const float2* __restrict__ corr,
-           const float* __restrict__ bg, int m, int rhs, float2* __restrict__ out) {
-  __shared__ float2 cs[RANK];
-  const int lanev = blockIdx.y;
-  const int b = lanev / rhs;
-  if (threadIdx.x < RANK) cs[threadIdx.x] = corr[(long long)lanev * RANK + threadIdx.x];
+k_lr_apply(const float2* __restrict__ tab, const float2* __restrict__ v, const float2* __restrict__ corr,
+           const float* __restrict__ bg, int m1, int m2, int rhs, float2* __restrict__ out) {
+  __shared__ float2 cs[LG][RANK];
+  const int b = blockIdx.z;
+  const int r0 = blockIdx.y * LG;
+  const int nl = min(LG, rhs - r0);
+  for (int e = threadIdx.x; e < nl * RANK; e += FBME_BLOCK)
+    cs[e / RANK][e % RANK] = corr[((long long)b * rhs + r0) * RANK + e];
   __syncthreads();
-  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  const long long m = (long long)m1 * m2;
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
   if (i >= m) return;
-  const float ib = 1.0f / bg[b];
-  const float2* fr = f + ((long long)b * m + i) * RANK;
-  float ax = 0.f, ay = 0.f;
-#pragma unroll 8
-  for (int k = 0; k < RANK; ++k) {
-    const float2 fv = fr[k], c = cs[k];
-    ax += fv.x * c.x - fv.y * c.y;
-    ay += fv.x * c.y + fv.y * c.x;
+  const int a1 = (int)(i / m2), a2 = (int)(i - (long long)a1 * m2);
+  const float2* U = tab + (long long)b * (m1 + m2) * RANK;
+  const float2* V = U + (long long)m1 * RANK;
+  float2 f[RANK];
+  const float4* u4 = reinterpret_cast<const float4*>(U + (long long)a1 * RANK);
+  const float4* v4 = reinterpret_cast<const float4*>(V + (long long)a2 * RANK);
+#pragma unroll
+  for (int q = 0; q < RANK / 2; ++q) {
+    const float4 uu = u4[q], vv = v4[q];
+    f[2 * q] = cmul(make_float2(uu.x, uu.y), make_float2(vv.x, vv.y));
+    f[2 * q + 1] = cmul(make_float2(uu.z, uu.w), make_float2(vv.z, vv.w));
   }
-  const float2 vv = v[(long long)lanev * m + i];
-  out[(long long)lanev * m + i] = make_float2(vv.x * ib - ax, vv.y * ib - ay);
+  const float ib = 1.0f / bg[b];
+  const float2* vb = v + ((long long)b * rhs + r0) * m + i;
+  float2* ob = out + ((long long)b * rhs + r0) * m + i;
+  for (int j = 0; j < nl; ++j) {
+    float ax = 0.f, ay = 0.f;
+#pragma unroll
+    for (int q = 0; q < RANK; ++q) {
+      const float2 c = cs[j][q];
+      ax += f[q].x * c.x - f[q].y * c.y;
+      ay += f[q].x * c.y + f[q].y * c.x;
+    }
+    const float2 vv = vb[(long long)j * m];
+    ob[(long long)j * m] = make_float2(vv.x * ib - ax, vv.y * ib - ay);
+  }
 }
 
 // ------------------------------------------------------------- lane algebra
@@ -698,26 +752,32 @@ __global__ void k_reciprocal(const float* __restrict__ in, float* __restrict__ o
 
 inline int nblk(long long n, int bs = FBME_BLOCK) { return (int)((n + bs - 1) / bs); }
 
-// Rows per workgroup of the row passes: as many padded rows as the LDS tile
-// holds (at most 64), or `want` when it is positive and smaller.
-template <int EPT>
-int rowpass_rows(int n, int want) {
-  const int cap = std::max(1, std::min<int>(EPT * FBME_BLOCK / (n + 1), 64));
-  return want > 0 ? std::min(want, cap) : cap;
+// Row passes: `want` rows per workgroup (0: as many as a 4-element-per-thread
+// LDS tile holds), at most 64 and at most what a 16-element tile holds; the
+// kernel instance is the smallest tile that fits the rows.
+int rowpass_cap(int n) { return std::max(1, std::min(16 * FBME_BLOCK / (n + 1), 64)); }
+
+template <bool INV>
+void launch_rowpass(const float2* in, float2* out, int n, int len_in, int len_out, long long rows, unsigned code,
+                    const float2* tw, hipStream_t st, int want_rows = 0) {
+  const int def = std::max(1, std::min(4 * FBME_BLOCK / (n + 1), 64));
+  const int tr = want_rows > 0 ? std::min(want_rows, rowpass_cap(n)) : def;
+  const int grid = (int)((rows + tr - 1) / tr);
+  const int need = tr * (n + 1);
+  if (need <= 4 * FBME_BLOCK)
+    hipLaunchKernelGGL((k_rowpass<4, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
+                       tr, code, tw);
+  else if (need <= 8 * FBME_BLOCK)
+    hipLaunchKernelGGL((k_rowpass<8, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
+                       tr, code, tw);
+  else
+    hipLaunchKernelGGL((k_rowpass<16, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
+                       tr, code, tw);
 }
 
-template <int EPT>
-void launch_rowpass(bool inv, const float2* in, float2* out, int n, int len_in, int len_out, long long rows,
-                    unsigned code, const float2* tw, hipStream_t st, int want_rows = 0) {
-  const int tr = rowpass_rows<EPT>(n, want_rows);
-  const int grid = (int)((rows + tr - 1) / tr);
-  if (inv)
-    hipLaunchKernelGGL((k_rowpass<EPT, true>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out,
-                       rows, tr, code, tw);
-  else
-    hipLaunchKernelGGL((k_rowpass<EPT, false>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out,
-                       rows, tr, code, tw);
-}
+// Columns per workgroup of the fused Toeplitz column pass: at most 16 and at
+// most what a 24-element-per-thread LDS tile holds.
+int toeplitz_col_cap(int e1, int e2) { return std::max(1, std::min({e2, 16, 24 * FBME_BLOCK / (e1 + 1)})); }
 
 }  // namespace
 
@@ -737,12 +797,12 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
   float2* bwork = (float2*)work_b;
   hipLaunchKernelGGL(k_real_to_complex, dim3(nblk(batch * n)), dim3(FBME_BLOCK), 0, st, gamma, a, batch * n);
   // lags = fft2(gamma): rows then columns (full lengths)
-  launch_rowpass<4>(false, a, bwork, n2, n2, n2, (long long)batch * n1, code2, (const float2*)tw_n2, st);
+  launch_rowpass<false>(a, bwork, n2, n2, n2, (long long)batch * n1, code2, (const float2*)tw_n2, st);
   int rc = fbme_column_fft(stream, batch, n1, n1, n1, n2, bwork, a, 0);
   if (rc) return rc;
   hipLaunchKernelGGL(k_embed, dim3(nblk((long long)e1 * e2), batch), dim3(FBME_BLOCK), 0, st, a, noise, bwork, n1, n2,
                      m1, m2, e1, e2);
-  launch_rowpass<4>(false, bwork, a, e2, e2, e2, (long long)batch * e1, codee2, (const float2*)tw_e2, st);
+  launch_rowpass<false>(bwork, a, e2, e2, e2, (long long)batch * e1, codee2, (const float2*)tw_e2, st);
   rc = fbme_column_fft(stream, batch, e1, e1, e1, e2, a, spectrum, 0);
   (void)code1;
   (void)codee1;
@@ -758,23 +818,24 @@ int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2,
   // work_a, work_b: lanes * M1 * E2 complex each.
   if (row_tile < 0 || col_tile < 0) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  launch_rowpass<4>(false, (const float2*)d, (float2*)work_a, e2, m2, e2, (long long)lanes * m1, code_e2,
-                    (const float2*)tw_e2, st, row_tile);
-  const int ept = (e1 <= 1024 / 8) ? 4 : (e1 <= 512) ? 8 : 16;
-  const int tc_cap = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
-  const int tc = col_tile > 0 ? std::min(col_tile, tc_cap) : tc_cap;
+  launch_rowpass<false>((const float2*)d, (float2*)work_a, e2, m2, e2, (long long)lanes * m1, code_e2,
+                        (const float2*)tw_e2, st, row_tile);
+  const int cap = toeplitz_col_cap(e1, e2);
+  const int tc = col_tile > 0 ? std::min(col_tile, cap) : cap;
+  const int need = tc * (e1 + 1);
   dim3 grid((e2 + tc - 1) / tc, lanes);
-  if (ept == 4)
-    hipLaunchKernelGGL((k_toeplitz_col<4>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
-                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
-  else if (ept == 8)
-    hipLaunchKernelGGL((k_toeplitz_col<8>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
-                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
+#define FBME_TCOL(EPT)                                                                                          \
+  hipLaunchKernelGGL((k_toeplitz_col<EPT>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a,              \
+                     (float2*)work_b, (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1)
+  if (need <= 8 * FBME_BLOCK)
+    FBME_TCOL(8);
+  else if (need <= 16 * FBME_BLOCK)
+    FBME_TCOL(16);
   else
-    hipLaunchKernelGGL((k_toeplitz_col<16>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a, (float2*)work_b,
-                       (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1);
-  launch_rowpass<4>(true, (const float2*)work_b, (float2*)out, e2, e2, m2, (long long)lanes * m1, code_e2,
-                    (const float2*)tw_e2, st, row_tile);
+    FBME_TCOL(24);
+#undef FBME_TCOL
+  launch_rowpass<true>((const float2*)work_b, (float2*)out, e2, e2, m2, (long long)lanes * m1, code_e2,
+                       (const float2*)tw_e2, st, row_tile);
   return (int)hipGetLastError();
 }
 
@@ -787,9 +848,8 @@ int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e
 
 int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max) {
   if (e1 < 1 || e2 < 1 || e1 > 1024 || e2 > 1024) return FBME_ERR_ARG;
-  const int ept = (e1 <= 1024 / 8) ? 4 : (e1 <= 512) ? 8 : 16;
-  *row_tile_max = rowpass_rows<4>(e2, 0);
-  *col_tile_max = std::max(1, std::min(e2, std::min(16, ept * FBME_BLOCK / (e1 + 1))));
+  *row_tile_max = rowpass_cap(e2);
+  *col_tile_max = toeplitz_col_cap(e1, e2);
   return 0;
 }
 
@@ -809,26 +869,43 @@ int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, 
                        const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,
                        void* factors, void* sinv, float* bg, int* status) {
   hipStream_t st = (hipStream_t)stream;
-  const long long m = (long long)m1 * m2;
-  hipLaunchKernelGGL(k_lr_factors, dim3(nblk(m * RANK), batch), dim3(FBME_BLOCK), 0, st, idx, val, (float2*)factors,
-                     n1, n2, m1, m2);
+  hipLaunchKernelGGL(k_lr_tables, dim3(nblk((long long)(m1 + m2) * RANK), batch), dim3(FBME_BLOCK), 0, st, idx, val,
+                     (float2*)factors, n1, n2, m1, m2);
   hipLaunchKernelGGL(k_lr_setup, dim3(batch), dim3(64), 0, st, idx, val, gamma_sum, noise, (const double2*)dir1,
                      (const double2*)dir2, n1, n2, (float2*)sinv, bg, status);
   return (int)hipGetLastError();
 }
 
-int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m, const void* factors, const void* sinv,
+int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
                        const float* bg, const void* v, void* out, void* part, void* corr) {
-  // part: lanes * ceil(m / 1024) * 32 double2; corr: lanes * 32 complex.
+  // part: lanes * 64 * 32 double2; corr: lanes * 32 complex.
+  if (rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  const int chunk = 1024;
-  const int chunks = (m + chunk - 1) / chunk;
-  hipLaunchKernelGGL(k_lr_project, dim3(chunks, lanes), dim3(FBME_BLOCK), 0, st, (const float2*)factors,
-                     (const float2*)v, m, rhs, chunk, (double2*)part);
-  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(64), 0, st, (const double2*)part, chunks, (const float2*)sinv, bg,
-                     rhs, (float2*)corr);
-  hipLaunchKernelGGL(k_lr_apply, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (const float2*)factors,
-                     (const float2*)v, (const float2*)corr, bg, m, rhs, (float2*)out);
+  const int batch = lanes / rhs;
+  const int rb = (m1 + 63) / 64;
+  const int chunks = (m1 + rb - 1) / rb;
+  const int groups = (rhs + 23) / 24;
+  const int per = (rhs + groups - 1) / groups;
+  const long long m = (long long)m1 * m2;
+  const dim3 pg(chunks, groups, batch), ag((unsigned)nblk(m), groups, batch);
+  const float2* tab = (const float2*)factors;
+#define FBME_LR_LAUNCH(LG)                                                                                     \
+  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, m1, m2, rhs, rb,  \
+                     (double2*)part);                                                                          \
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(64), 0, st, (const double2*)part, chunks, (const float2*)sinv,  \
+                     bg, rhs, (float2*)corr);                                                                  \
+  hipLaunchKernelGGL((k_lr_apply<LG>), ag, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, (const float2*)corr,   \
+                     bg, m1, m2, rhs, (float2*)out)
+  if (per <= 8) {
+    FBME_LR_LAUNCH(8);
+  } else if (per <= 12) {
+    FBME_LR_LAUNCH(12);
+  } else if (per <= 17) {
+    FBME_LR_LAUNCH(17);
+  } else {
+    FBME_LR_LAUNCH(24);
+  }
+#undef FBME_LR_LAUNCH
   return (int)hipGetLastError();
 }
 

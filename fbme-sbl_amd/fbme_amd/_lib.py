@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 Q_WORKSPACE_BYTES = 0
 Q_TOTAL_GRID = 1
@@ -121,15 +121,18 @@ _SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_int, c_int]),
     "fbme_toeplitz_tiles": (c_int, [c_int, c_int, c_void_p, c_void_p]),
+    "fbme_grid_matvec": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
     "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                    c_int, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p, c_void_p,
                                    c_void_p, c_void_p, c_void_p]),
-    "fbme_lowrank_apply": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+    "fbme_lowrank_apply": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                    c_void_p, c_void_p, c_void_p, c_void_p,
-                                   c_void_p, c_void_p]),
+                                   c_void_p, c_void_p, c_void_p]),
     "fbme_lane_dot": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                               c_void_p, c_void_p]),
     "fbme_pcg_update": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,

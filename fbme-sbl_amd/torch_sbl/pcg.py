@@ -159,7 +159,9 @@ class _Solver:
         self.p = _probe_count(self.n, self.m, cfg)
         self.r = 1 + self.p
         self.lanes = self.b * self.r
-        self.tiles = autotune.cg_tiles(self.dev, self.lanes, *self.samples, self.t.e1, self.t.e2)
+        self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
+        # 256 x 256 grids apply Q through the grid on the register FFT passes
+        self.grid_op = self.grid == (256, 256)
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
@@ -210,6 +212,7 @@ class _Solver:
         self.spec_a = torch.empty(big, dtype=c64, device=dev)
         self.spec_b = torch.empty(big, dtype=c64, device=dev)
         self.spectrum = torch.empty((b, e1, e2), dtype=c64, device=dev)
+        self.gmid = (torch.empty(self.lanes * 256 * m2, dtype=c64, device=dev) if self.grid_op else None)
         # preconditioner
         tiles = (n + 1023) // 1024
         if tiles * RANK > 2048:
@@ -217,10 +220,10 @@ class _Solver:
         self.cand = torch.empty(b * tiles * RANK, dtype=torch.int64, device=dev)
         self.top_idx = torch.empty((b, RANK), dtype=torch.int32, device=dev)
         self.top_val = torch.empty((b, RANK), dtype=f32, device=dev)
-        self.factors = torch.empty((b, m, RANK), dtype=c64, device=dev)
+        self.factors = torch.empty((b, m1 + m2, RANK), dtype=c64, device=dev)
         self.sinv = torch.empty((b, RANK, RANK), dtype=c64, device=dev)
         self.bg = torch.empty(b, dtype=f32, device=dev)
-        self.lr_part = torch.empty(self.lanes * ((m + 1023) // 1024) * RANK * 2, dtype=f64, device=dev)
+        self.lr_part = torch.empty(self.lanes * 64 * RANK * 2, dtype=f64, device=dev)
         self.corr = torch.empty((self.lanes, RANK), dtype=c64, device=dev)
         # lane scalars
         self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
@@ -272,6 +275,11 @@ class _Solver:
     def matvec(self, d, out):
         t = self.t
         m1, m2 = self.samples
+        if self.grid_op:
+            self._chk(self.lib.fbme_grid_matvec(self.stream, self.lanes, self.r, m1, m2, _ptr(self.gamma),
+                                                _ptr(self.noise), _ptr(d), _ptr(out), _ptr(self.gmid),
+                                                _ptr(t.tw[256])), "grid_matvec")
+            return
         self._chk(self.lib.fbme_toeplitz_matvec_tiled(
             self.stream, self.lanes, self.r, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
             _ptr(self.spectrum), _ptr(d), _ptr(out), _ptr(self.work_a), _ptr(self.work_b),
@@ -279,7 +287,7 @@ class _Solver:
 
     def precondition(self, v, out):
         self._chk(self.lib.fbme_lowrank_apply(
-            self.stream, self.lanes, self.r, self.m, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
+            self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
             _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr)), "lowrank_apply")
 
     # ------------------------------------------------------------- pipeline
@@ -307,11 +315,8 @@ class _Solver:
         t = self.t
         n1, n2 = self.grid
         m1, m2 = self.samples
-        self._chk(self.lib.fbme_gamma_spectrum(
-            self.stream, self.b, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
-            t.codes[t.e2], _ptr(self.gamma), _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b),
-            _ptr(self.spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
-            "gamma_spectrum")
+        if not self.grid_op:
+            self.spectrum_setup()
         self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
         gsum = self.sums[:, 0].contiguous()
@@ -319,6 +324,16 @@ class _Solver:
             self.stream, self.b, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
             _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
             _ptr(self.status)), "lowrank_setup")
+
+    def spectrum_setup(self):
+        t = self.t
+        n1, n2 = self.grid
+        m1, m2 = self.samples
+        self._chk(self.lib.fbme_gamma_spectrum(
+            self.stream, self.b, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
+            t.codes[t.e2], _ptr(self.gamma), _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b),
+            _ptr(self.spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
+            "gamma_spectrum")
 
     def pcg(self, tol: float):
         """Batched complex PCG (solver.py:5582-5723).  Returns iterations."""

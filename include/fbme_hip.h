@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 2
+#define FBME_ABI_VERSION 3
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -216,12 +216,23 @@ int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2,
                                unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
                                void* work_b, const void* tw_e1, const void* tw_e2, int row_tile, int col_tile);
 int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max);
+/* out = noise d + A Gamma A^H d for 256 x 256 grids: the same covariance
+ * matvec (operators.py:328-355) evaluated through the grid with three
+ * register 256-point FFT passes instead of the BTTB embedding.  lanes =
+ * batch * rhs; d, out [lanes, M1, M2] (distinct); gamma [batch, 65536];
+ * noise [batch]; mid: lanes * 256 * M2 complex64; tw256:
+ * fbme_twiddle_table(256).  Requires N1 = N2 = 256 and M1, M2 <= 256. */
+int fbme_grid_matvec(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
+                     const void* d, void* out, void* mid, const void* tw256);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
  * index (the stable argsort of solver.py:5797-5800).  cand: batch *
  * ceil(n / 1024) * 32 uint64; requires n <= 65536. */
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val);
 /* Low-rank preconditioner (_make_low_rank_preconditioner,
- * solver.py:5786-5833): factors [batch, M, 32] = columns * sqrt(gamma_k),
+ * solver.py:5786-5833).  The factor matrix F = columns * sqrt(gamma_k)
+ * [M, 32] is separable and kept as its two phase tables: factors [batch,
+ * M1 + M2, 32] complex64 = U [M1, 32] (row phases) then V [M2, 32] (column
+ * phases times sqrt(gamma_k)), F[m1 M2 + m2, k] = U[m1, k] V[m2, k].
  * sinv [batch, 32, 32] = (I + F^H F / bg)^-1, bg = max(noise + omitted,
  * eps * sum gamma).  dir1, dir2: fbme_dirichlet_table(M1, N1), (M2, N2) on
  * the device.  status[b] = 1 when the Cholesky factorisation fails. */
@@ -229,9 +240,10 @@ int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, 
                        const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,
                        void* factors, void* sinv, float* bg, int* status);
 /* out = v / bg - F S^-1 F^H v / bg^2 (_apply_low_rank_preconditioner,
- * solver.py:5836-5848).  part: lanes * ceil(M / 1024) * 64 doubles; corr:
- * lanes * 32 complex64. */
-int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m, const void* factors, const void* sinv,
+ * solver.py:5836-5848), F generated from the phase tables of
+ * fbme_lowrank_setup once per sample for all `rhs` lanes of a scene.
+ * part: lanes * 64 * 64 doubles; corr: lanes * 32 complex64. */
+int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
                        const float* bg, const void* v, void* out, void* part, void* corr);
 /* out[l] = sum Re(conj(a_l) b_l) in FP64 (the PCG inner products of
  * _pcg, solver.py:5582-5723).  part: lanes * 64 doubles. */

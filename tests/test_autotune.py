@@ -42,7 +42,8 @@ def test_shipped_table_is_well_formed():
     for arch, entries in table.items():
         for sig, entry in entries.items():
             if sig.startswith("cg|"):
-                assert entry["row_tile"] >= 1 and entry["col_tile"] >= 1
+                assert entry["row_tile"] >= 0 and entry["col_tile"] >= 0
+                assert entry["ms"] <= entry["default_ms"] * 1.0001
             else:
                 assert sig.split("|")[0] in ("fast", "stable")
                 assert entry["fft_kernel"] in (0, 1)

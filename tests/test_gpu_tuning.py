@@ -72,6 +72,7 @@ def test_tiled_toeplitz_matvec_is_bitwise_equal(device, grid, samples):
     ys = [scenes.uniform_scene(i, grid, samples, k=10, snr_db=20.0).y for i in range(2)]
     y = torch.from_numpy(np.stack(ys).astype(np.complex64)).to(device)
     s = pcg._Solver(y, grid, SBLConfig(mode="cg", device=str(device)))
+    s.grid_op = False   # the BTTB passes
     s.init()
     s.setup_operators()
     lib = _lib.load()
@@ -86,3 +87,32 @@ def test_tiled_toeplitz_matvec_is_bitwise_equal(device, grid, samples):
         s.matvec(s.rhs, out)
         torch.cuda.synchronize()
         assert torch.equal(out, ref), (rt, ct)
+
+
+@pytest.mark.parametrize("samples", [(160, 160), (96, 200), (256, 40)])
+def test_grid_matvec_matches_bttb_and_oracle(device, samples):
+    """Q d through the 256 x 256 grid (register FFT passes) against the BTTB
+    passes and the float64 operator, on the solver's own state and lanes."""
+    from oracle import cg_oracle as co
+    from torch_sbl import SBLConfig, pcg
+    grid = (256, 256)
+    ys = [scenes.ellipse_scene(20 + i, grid, samples, k=100, snr_db=10.0).y for i in range(2)]
+    y = torch.from_numpy(np.stack(ys).astype(np.complex64)).to(device)
+    s = pcg._Solver(y, grid, SBLConfig(mode="cg", device=str(device), hutchinson_probes=4,
+                                        auto_probe_scaling=False))
+    assert s.grid_op
+    s.init()
+    s.setup_operators()
+    s.spectrum_setup()
+    got = torch.empty_like(s.rhs)
+    s.matvec(s.rhs, got)
+    s.grid_op = False
+    ref = torch.empty_like(s.rhs)
+    s.matvec(s.rhs, ref)
+    torch.cuda.synchronize()
+    g, r = got.cpu().numpy(), ref.cpu().numpy()
+    assert _rel(g, r) < 2e-5
+    op = co.Operator(grid, samples)
+    d = np.moveaxis(s.rhs.cpu().numpy().astype(np.complex128), 1, 2)      # [B, M, R]
+    q = op.q(d, s.gamma.cpu().numpy().astype(np.float64), s.noise.cpu().numpy().astype(np.float64))
+    assert _rel(g, np.moveaxis(q, 2, 1)) < 2e-5
