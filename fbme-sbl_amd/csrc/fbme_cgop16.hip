@@ -83,11 +83,16 @@ k_gop_row(float2* __restrict__ mid, const float* __restrict__ gamma, int m2, int
   }
 }
 
+// DOT: also part[lane, block] = sum Re(conj(d) out) over the block's
+// outputs (FP64, fixed order), the PCG curvature <d, Q d>.
+template <bool DOT>
 __global__ void __launch_bounds__(16 * GC)
 k_gop_col_forward(const float2* __restrict__ mid, const float2* __restrict__ d, const float* __restrict__ noise,
-                  float2* __restrict__ out, int m1, int m2, int rhs, const float2* __restrict__ twtab) {
+                  float2* __restrict__ out, int m1, int m2, int rhs, const float2* __restrict__ twtab,
+                  double* __restrict__ part) {
   __shared__ float2 xs[16 * 17 * GC];
   __shared__ float2 tw[256];
+  __shared__ double red[8];
   const int c = threadIdx.x % GC, t = threadIdx.x / GC;
   const int lane = blockIdx.y;
   const int col = blockIdx.x * GC + c;
@@ -101,33 +106,63 @@ k_gop_col_forward(const float2* __restrict__ mid, const float2* __restrict__ d, 
   fft256<false, GC>(v, tw, xs + c, t);
   const float nv = noise[lane / rhs];
   const long long base = (long long)lane * m1 * m2 + col;
+  double acc = 0.0;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const int k = t + 16 * n;
     if (act && k < m1) {
       const float2 dv = d[base + (long long)k * m2];
-      out[base + (long long)k * m2] = make_float2(v[n].x + nv * dv.x, v[n].y + nv * dv.y);
+      const float2 o = make_float2(v[n].x + nv * dv.x, v[n].y + nv * dv.y);
+      out[base + (long long)k * m2] = o;
+      if (DOT) acc += (double)dv.x * o.x + (double)dv.y * o.y;
     }
   }
+  if (DOT) {
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) part[(long long)lane * gridDim.x + blockIdx.x] = acc;
+  }
+}
+
+// out[l] = sum_i part[l, i], one wavefront per lane (fixed shuffle tree).
+__global__ void __launch_bounds__(64) k_gop_finish(const double* __restrict__ part, int blocks, int lanes,
+                                                   double* __restrict__ out) {
+  const int l = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < blocks; i += 64) s += part[(long long)l * blocks + i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[l] = s;
 }
 
 }  // namespace
 
 extern "C" {
 
-int fbme_grid_matvec(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
-                     const void* d, void* out, void* mid, const void* tw256) {
+int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
+                         const void* d, void* out, void* mid, const void* tw256, double* part, double* dot) {
   if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 256 || m2 > 256 || d == out)
     return FBME_ERR_ARG;
+  if ((part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const dim3 cg((m2 + GC - 1) / GC, lanes);
   hipLaunchKernelGGL(k_gop_col_adjoint, cg, dim3(16 * GC), 0, st, (const float2*)d, (float2*)mid, m1, m2,
                      (const float2*)tw256);
   hipLaunchKernelGGL(k_gop_row, dim3(64, lanes), dim3(64), 0, st, (float2*)mid, gamma, m2, rhs,
                      (const float2*)tw256);
-  hipLaunchKernelGGL(k_gop_col_forward, cg, dim3(16 * GC), 0, st, (const float2*)mid, (const float2*)d, noise,
-                     (float2*)out, m1, m2, rhs, (const float2*)tw256);
+  if (dot) {
+    hipLaunchKernelGGL(k_gop_col_forward<true>, cg, dim3(16 * GC), 0, st, (const float2*)mid, (const float2*)d, noise,
+                       (float2*)out, m1, m2, rhs, (const float2*)tw256, part);
+    hipLaunchKernelGGL(k_gop_finish, dim3(lanes), dim3(64), 0, st, (const double*)part, (int)cg.x, lanes,
+                       dot);
+  } else {
+    hipLaunchKernelGGL(k_gop_col_forward<false>, cg, dim3(16 * GC), 0, st, (const float2*)mid, (const float2*)d,
+                       noise, (float2*)out, m1, m2, rhs, (const float2*)tw256, (double*)nullptr);
+  }
   return (int)hipGetLastError();
+}
+
+int fbme_grid_matvec(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
+                     const void* d, void* out, void* mid, const void* tw256) {
+  return fbme_grid_matvec_dot(stream, lanes, rhs, m1, m2, gamma, noise, d, out, mid, tw256, nullptr, nullptr);
 }
 
 }  // extern "C"

@@ -384,43 +384,59 @@ k_lr_project(const float2* __restrict__ tab, const float2* __restrict__ v, int m
   }
 }
 
-// corr[(b, r), :] = S^-1 (proj / bg) / bg, from the row-chunk partials.
-__global__ void __launch_bounds__(64)
+// corr[(b, r), :] = S^-1 (proj / bg) / bg, from the row-chunk partials:
+// thread (q, k) = (tid / 32, tid % 32) sums chunks q, q + 8, ... of atom k,
+// the eight partial sums are added in a fixed order, then thread k < 32
+// forms row k of S^-1 proj.
+__global__ void __launch_bounds__(FBME_BLOCK)
 k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict__ sinv,
           const float* __restrict__ bg, int rhs, float2* __restrict__ corr) {
+  __shared__ double2 acc8[8][RANK];
   __shared__ double2 proj[RANK];
   const int lanev = blockIdx.x;
   const int b = lanev / rhs;
-  const int k = threadIdx.x;
-  if (k < RANK) {
-    double2 s = make_double2(0.0, 0.0);
-    for (int c = 0; c < chunks; ++c) {
-      const double2 v = part[((long long)lanev * chunks + c) * RANK + k];
-      s.x += v.x;
-      s.y += v.y;
+  const int k = threadIdx.x & (RANK - 1), q = threadIdx.x >> 5;
+  double2 s = make_double2(0.0, 0.0);
+  for (int c = q; c < chunks; c += 8) {
+    const double2 v = part[((long long)lanev * chunks + c) * RANK + k];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  acc8[q][k] = s;
+  __syncthreads();
+  if (threadIdx.x < RANK) {
+    double2 t = acc8[0][k];
+    for (int w = 1; w < 8; ++w) {
+      t.x += acc8[w][k].x;
+      t.y += acc8[w][k].y;
     }
-    proj[k] = s;
+    proj[k] = t;
   }
   __syncthreads();
-  if (k < RANK) {
+  if (threadIdx.x < RANK) {
     const double ib = 1.0 / (double)bg[b];
-    double2 acc = make_double2(0.0, 0.0);
-    for (int j = 0; j < RANK; ++j) {
-      const float2 w = sinv[((long long)b * RANK + k) * RANK + j];
-      acc.x += (double)w.x * proj[j].x - (double)w.y * proj[j].y;
-      acc.y += (double)w.x * proj[j].y + (double)w.y * proj[j].x;
+    double2 a = make_double2(0.0, 0.0);
+    for (int jj = 0; jj < RANK; ++jj) {
+      const float2 w = sinv[((long long)b * RANK + k) * RANK + jj];
+      a.x += (double)w.x * proj[jj].x - (double)w.y * proj[jj].y;
+      a.y += (double)w.x * proj[jj].y + (double)w.y * proj[jj].x;
     }
-    corr[(long long)lanev * RANK + k] = make_float2((float)(acc.x * ib * ib), (float)(acc.y * ib * ib));
+    corr[(long long)lanev * RANK + k] = make_float2((float)(a.x * ib * ib), (float)(a.y * ib * ib));
   }
 }
 
 // out = v / bg - F corr (corr already carries the 1 / bg^2 factor): one
 // sample per thread, the 32 factor entries of the sample generated once from
-// U / V and applied to every lane of the group.
-template <int LG>
+// U / V and applied to every lane of the group (corr staged in LDS).  DOT:
+// also part[lane, block] = sum Re(conj(v) out)
+// over the block's samples (FP64, fixed order), the PCG inner product
+// <r, M^-1 r>.
+template <int LG, bool DOT>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_lr_apply(const float2* __restrict__ tab, const float2* __restrict__ v, const float2* __restrict__ corr,
-           const float* __restrict__ bg, int m1, int m2, int rhs, float2* __restrict__ out) {
+           const float* __restrict__ bg, int m1, int m2, int rhs, float2* __restrict__ out,
+           double* __restrict__ part) {
+  __shared__ double red[FBME_BLOCK / 64][LG];
   __shared__ float2 cs[LG][RANK];
   const int b = blockIdx.z;
   const int r0 = blockIdx.y * LG;
@@ -430,8 +446,9 @@ k_lr_apply(const float2* __restrict__ tab, const float2* __restrict__ v, const f
   __syncthreads();
   const long long m = (long long)m1 * m2;
   const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
-  if (i >= m) return;
-  const int a1 = (int)(i / m2), a2 = (int)(i - (long long)a1 * m2);
+  const bool live = i < m;
+  const int a1 = live ? (int)(i / m2) : 0;
+  const int a2 = live ? (int)(i - (long long)a1 * m2) : 0;
   const float2* U = tab + (long long)b * (m1 + m2) * RANK;
   const float2* V = U + (long long)m1 * RANK;
   float2 f[RANK];
@@ -444,18 +461,38 @@ k_lr_apply(const float2* __restrict__ tab, const float2* __restrict__ v, const f
     f[2 * q + 1] = cmul(make_float2(uu.z, uu.w), make_float2(vv.z, vv.w));
   }
   const float ib = 1.0f / bg[b];
-  const float2* vb = v + ((long long)b * rhs + r0) * m + i;
-  float2* ob = out + ((long long)b * rhs + r0) * m + i;
-  for (int j = 0; j < nl; ++j) {
-    float ax = 0.f, ay = 0.f;
+  const long long lane0 = (long long)b * rhs + r0;
+  double dacc[LG];
 #pragma unroll
-    for (int q = 0; q < RANK; ++q) {
-      const float2 c = cs[j][q];
-      ax += f[q].x * c.x - f[q].y * c.y;
-      ay += f[q].x * c.y + f[q].y * c.x;
+  for (int j = 0; j < LG; ++j) {
+    dacc[j] = 0.0;
+    if (j < nl && live) {
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int q = 0; q < RANK; ++q) {
+        const float2 c = cs[j][q];
+        ax += f[q].x * c.x - f[q].y * c.y;
+        ay += f[q].x * c.y + f[q].y * c.x;
+      }
+      const float2 vv = v[(lane0 + j) * m + i];
+      const float2 o = make_float2(vv.x * ib - ax, vv.y * ib - ay);
+      out[(lane0 + j) * m + i] = o;
+      if (DOT) dacc[j] = (double)vv.x * o.x + (double)vv.y * o.y;
     }
-    const float2 vv = vb[(long long)j * m];
-    ob[(long long)j * m] = make_float2(vv.x * ib - ax, vv.y * ib - ay);
+  }
+  if (DOT) {
+    const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < LG; ++j) {
+      const double w = wave_sum(dacc[j]);
+      if (wl == 0) red[wave][j] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < nl) {
+      double sum = 0.0;
+      for (int w = 0; w < FBME_BLOCK / 64; ++w) sum += red[w][threadIdx.x];
+      part[(lane0 + threadIdx.x) * gridDim.x + blockIdx.x] = sum;
+    }
   }
 }
 
@@ -474,12 +511,15 @@ k_lane_dot(const float2* __restrict__ a, const float2* __restrict__ b, int m, do
   if (threadIdx.x == 0) part[(long long)lanev * gridDim.x + blockIdx.x] = acc;
 }
 
-__global__ void k_lane_finish(const double* __restrict__ part, int blocks, int lanes, double* __restrict__ out) {
-  const int l = blockIdx.x * blockDim.x + threadIdx.x;
-  if (l >= lanes) return;
+// out[l] = sum_i part[l, i]: one wavefront per lane, lane t of the wave
+// summing i = t, t + 64, ... and a fixed shuffle tree (deterministic).
+__global__ void __launch_bounds__(64) k_lane_finish(const double* __restrict__ part, int blocks, int lanes,
+                                                    double* __restrict__ out) {
+  const int l = blockIdx.x;
   double s = 0.0;
-  for (int i = 0; i < blocks; ++i) s += part[(long long)l * blocks + i];
-  out[l] = s;
+  for (int i = threadIdx.x; i < blocks; i += 64) s += part[(long long)l * blocks + i];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) out[l] = s;
 }
 
 // x += alpha d, r -= alpha q with alpha from (rho, curv, active)
@@ -876,10 +916,12 @@ int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, 
   return (int)hipGetLastError();
 }
 
-int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
-                       const float* bg, const void* v, void* out, void* part, void* corr) {
-  // part: lanes * 64 * 32 double2; corr: lanes * 32 complex.
+int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
+                           const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
+                           double* dot) {
+  // part: lanes * 64 * 32 double2; corr: lanes * 32 complex; dot_part: lanes * ceil(M / 256) doubles.
   if (rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1) return FBME_ERR_ARG;
+  if ((dot_part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int batch = lanes / rhs;
   const int rb = (m1 + 63) / 64;
@@ -889,13 +931,18 @@ int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const v
   const long long m = (long long)m1 * m2;
   const dim3 pg(chunks, groups, batch), ag((unsigned)nblk(m), groups, batch);
   const float2* tab = (const float2*)factors;
-#define FBME_LR_LAUNCH(LG)                                                                                     \
-  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, m1, m2, rhs, rb,  \
-                     (double2*)part);                                                                          \
-  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(64), 0, st, (const double2*)part, chunks, (const float2*)sinv,  \
-                     bg, rhs, (float2*)corr);                                                                  \
-  hipLaunchKernelGGL((k_lr_apply<LG>), ag, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, (const float2*)corr,   \
-                     bg, m1, m2, rhs, (float2*)out)
+#define FBME_LR_LAUNCH(LG)                                                                                      \
+  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, m1, m2, rhs, rb,   \
+                     (double2*)part);                                                                           \
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,             \
+                     (const float2*)sinv,                                                                       \
+                     bg, rhs, (float2*)corr);                                                                   \
+  if (dot)                                                                                                      \
+    hipLaunchKernelGGL((k_lr_apply<LG, true>), ag, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v,              \
+                       (const float2*)corr, bg, m1, m2, rhs, (float2*)out, dot_part);                           \
+  else                                                                                                          \
+    hipLaunchKernelGGL((k_lr_apply<LG, false>), ag, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v,             \
+                       (const float2*)corr, bg, m1, m2, rhs, (float2*)out, (double*)nullptr)
   if (per <= 8) {
     FBME_LR_LAUNCH(8);
   } else if (per <= 12) {
@@ -906,7 +953,16 @@ int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const v
     FBME_LR_LAUNCH(24);
   }
 #undef FBME_LR_LAUNCH
+  if (dot)
+    hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)ag.x,
+                       lanes, dot);
   return (int)hipGetLastError();
+}
+
+int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
+                       const float* bg, const void* v, void* out, void* part, void* corr) {
+  return fbme_lowrank_apply_dot(stream, lanes, rhs, m1, m2, factors, sinv, bg, v, out, part, corr, nullptr,
+                                nullptr);
 }
 
 int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out) {
@@ -915,7 +971,7 @@ int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, 
   const int blocks = std::min(64, nblk(m));
   hipLaunchKernelGGL(k_lane_dot, dim3(blocks, lanes), dim3(FBME_BLOCK), 0, st, (const float2*)a, (const float2*)b, m,
                      part);
-  hipLaunchKernelGGL(k_lane_finish, dim3(nblk(lanes, 64)), dim3(64), 0, st, (const double*)part, blocks, lanes, out);
+  hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)part, blocks, lanes, out);
   return (int)hipGetLastError();
 }
 

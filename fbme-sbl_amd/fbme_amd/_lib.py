@@ -124,6 +124,13 @@ _SIGNATURES = {
     "fbme_grid_matvec": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p,
                                  c_void_p]),
+    "fbme_grid_matvec_dot": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
+    "fbme_lowrank_apply_dot": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                       c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_void_p,
+                                       c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
     "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,

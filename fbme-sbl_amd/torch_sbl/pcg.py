@@ -140,6 +140,11 @@ def _check_supported(cfg: SBLConfig) -> None:
             "mode='cg' on MI355X does not implement: " + ", ".join(bad))
 
 
+# Replay the PCG iterations between two host convergence checks as one
+# captured hipGraph (bitwise the same work as the eager loop).
+USE_GRAPHS = True
+
+
 class _Solver:
     """Device buffers and kernel calls of one (B, grid, samples, P) problem."""
 
@@ -160,6 +165,7 @@ class _Solver:
         self.r = 1 + self.p
         self.lanes = self.b * self.r
         self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
+        self._graphs = {}
         # 256 x 256 grids apply Q through the grid on the register FFT passes
         self.grid_op = self.grid == (256, 256)
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
@@ -227,6 +233,8 @@ class _Solver:
         self.corr = torch.empty((self.lanes, RANK), dtype=c64, device=dev)
         # lane scalars
         self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
+        self.mv_part = torch.empty(self.lanes * ((m2 + 15) // 16), dtype=f64, device=dev)
+        self.pc_part = torch.empty(self.lanes * ((m + 255) // 256), dtype=f64, device=dev)
         self.rho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.nrho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.curv = torch.empty(self.lanes, dtype=f64, device=dev)
@@ -272,23 +280,29 @@ class _Solver:
         self._chk(self.lib.fbme_lane_dot(self.stream, lanes, a.shape[-1], _ptr(a), _ptr(bvec),
                                          _ptr(self.dot_part), _ptr(out)), "lane_dot")
 
-    def matvec(self, d, out):
+    def matvec(self, d, out, dot=None):
+        """out = Q d; with `dot`, also dot[lane] = Re <d, out> (FP64)."""
         t = self.t
         m1, m2 = self.samples
         if self.grid_op:
-            self._chk(self.lib.fbme_grid_matvec(self.stream, self.lanes, self.r, m1, m2, _ptr(self.gamma),
-                                                _ptr(self.noise), _ptr(d), _ptr(out), _ptr(self.gmid),
-                                                _ptr(t.tw[256])), "grid_matvec")
+            self._chk(self.lib.fbme_grid_matvec_dot(
+                self.stream, self.lanes, self.r, m1, m2, _ptr(self.gamma), _ptr(self.noise), _ptr(d), _ptr(out),
+                _ptr(self.gmid), _ptr(t.tw[256]), _ptr(self.mv_part) if dot is not None else None, _ptr(dot)),
+                "grid_matvec")
             return
         self._chk(self.lib.fbme_toeplitz_matvec_tiled(
             self.stream, self.lanes, self.r, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
             _ptr(self.spectrum), _ptr(d), _ptr(out), _ptr(self.work_a), _ptr(self.work_b),
             _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2]), self.tiles[0], self.tiles[1]), "toeplitz_matvec")
+        if dot is not None:
+            self.lane_dot(d, out, dot)
 
-    def precondition(self, v, out):
-        self._chk(self.lib.fbme_lowrank_apply(
+    def precondition(self, v, out, dot=None):
+        """out = M^-1 v; with `dot`, also dot[lane] = Re <v, out> (FP64)."""
+        self._chk(self.lib.fbme_lowrank_apply_dot(
             self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
-            _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr)), "lowrank_apply")
+            _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr),
+            _ptr(self.pc_part) if dot is not None else None, _ptr(dot)), "lowrank_apply")
 
     # ------------------------------------------------------------- pipeline
     def init(self):
@@ -348,8 +362,7 @@ class _Solver:
         else:
             self.x.zero_()
             self.res.copy_(self.rhs)
-        self.precondition(self.res, self.z)
-        self.lane_dot(self.res, self.z, self.rho)
+        self.precondition(self.res, self.z, dot=self.rho)
         self.lane_dot(self.res, self.res, self.rr)
         self.active.fill_(1)
         self.best_rel.fill_(float("inf"))
@@ -363,14 +376,17 @@ class _Solver:
         self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z), _ptr(self.rho),
                                          _ptr(self.rho), _ptr(self.active), 1), "direction")
         limit = fixed if fixed > 0 else cfg.cg_max_iter
+        ci = cfg.cg_check_interval
         it = 0
-        for it in range(1, limit + 1):
-            self.matvec(self.d, self.q)
-            self.lane_dot(self.d, self.q, self.curv)
-            self._chk(lib.fbme_pcg_update(self.stream, lanes, m, _ptr(self.x), _ptr(self.res), _ptr(self.d),
-                                          _ptr(self.q), _ptr(self.rho), _ptr(self.curv), _ptr(self.active)),
-                      "update")
-            if it % cfg.cg_check_interval == 0 or it == limit:
+        while it < limit:
+            if USE_GRAPHS and fixed == 0 and it % ci == 0 and it + ci <= limit:
+                # iterations it+1 .. it+ci up to the check of the last one
+                self._replay_block(ci)
+                it += ci
+            else:
+                it += 1
+                self._iter_first()
+            if it % ci == 0 or it == limit:
                 self.matvec(self.x, self.q)
                 self._chk(lib.fbme_sub(self.stream, self.x.numel(), _ptr(self.rhs), _ptr(self.q),
                                        _ptr(self.res)), "sub")
@@ -381,22 +397,53 @@ class _Solver:
                 if fixed > 0:
                     self._chk(lib.fbme_masked_copy(self.stream, lanes, m, _ptr(self.best_x), _ptr(self.x),
                                                    _ptr(self.improved)), "masked_copy")
-                self.precondition(self.res, self.z)
-                self.lane_dot(self.res, self.z, self.nrho)
+                self.precondition(self.res, self.z, dot=self.nrho)
                 self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z),
                                                  _ptr(self.rho), _ptr(self.nrho), _ptr(self.active), 1),
                           "direction")
                 if fixed == 0 and not bool(self.active.any()):
                     break
                 continue
-            self.precondition(self.res, self.z)
-            self.lane_dot(self.res, self.z, self.nrho)
-            self._chk(lib.fbme_pcg_direction(self.stream, lanes, m, _ptr(self.d), _ptr(self.z), _ptr(self.rho),
-                                             _ptr(self.nrho), _ptr(self.active), 0), "direction")
+            self._iter_second()
         if fixed > 0:
             self.x.copy_(self.best_x)
         self.have_solution = True
         return it
+
+    def _iter_first(self):
+        """q = Q d with the curvature <d, q>; x += alpha d, r -= alpha q."""
+        self.matvec(self.d, self.q, dot=self.curv)
+        self._chk(self.lib.fbme_pcg_update(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
+                                           _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.curv),
+                                           _ptr(self.active)), "update")
+
+    def _iter_second(self):
+        """z = M^-1 r with <r, z>; d = z + beta d (no restart)."""
+        self.precondition(self.res, self.z, dot=self.nrho)
+        self._chk(self.lib.fbme_pcg_direction(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),
+                                              _ptr(self.rho), _ptr(self.nrho), _ptr(self.active), 0),
+                  "direction")
+
+    def _replay_block(self, ci: int):
+        """ci - 1 full PCG iterations and the first half of the next, the
+        work between two host convergence checks, as one hipGraph captured
+        on first use (same kernels and buffers as the eager loop)."""
+        key = (ci, self.grid_op, tuple(self.tiles))
+        graph = self._graphs.get(key)
+        if graph is None:
+            graph = torch.cuda.CUDAGraph()
+            eager_stream = self.stream
+            with torch.cuda.graph(graph):
+                self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
+                try:
+                    for _ in range(ci - 1):
+                        self._iter_first()
+                        self._iter_second()
+                    self._iter_first()
+                finally:
+                    self.stream = eager_stream
+            self._graphs[key] = graph
+        graph.replay()
 
     def posterior(self, tol: float) -> int:
         """CG posterior (solver.py:2991-3036) or the reduced active-set

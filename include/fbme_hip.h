@@ -224,6 +224,11 @@ int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max);
  * fbme_twiddle_table(256).  Requires N1 = N2 = 256 and M1, M2 <= 256. */
 int fbme_grid_matvec(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
                      const void* d, void* out, void* mid, const void* tw256);
+/* fbme_grid_matvec that also leaves dot[lane] = Re <d, out> (FP64), the PCG
+ * curvature of solver.py:5645-5655, from the last pass's registers.  part:
+ * lanes * ceil(M2 / 16) doubles. */
+int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
+                         const void* d, void* out, void* mid, const void* tw256, double* part, double* dot);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
  * index (the stable argsort of solver.py:5797-5800).  cand: batch *
  * ceil(n / 1024) * 32 uint64; requires n <= 65536. */
@@ -245,6 +250,12 @@ int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, 
  * part: lanes * 64 * 64 doubles; corr: lanes * 32 complex64. */
 int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
                        const float* bg, const void* v, void* out, void* part, void* corr);
+/* fbme_lowrank_apply that also leaves dot[lane] = Re <v, out> (FP64), the
+ * PCG inner product <r, M^-1 r> of solver.py:5700-5708.  dot_part: lanes *
+ * ceil(M1 M2 / 256) doubles. */
+int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
+                           const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
+                           double* dot);
 /* out[l] = sum Re(conj(a_l) b_l) in FP64 (the PCG inner products of
  * _pcg, solver.py:5582-5723).  part: lanes * 64 doubles. */
 int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out);

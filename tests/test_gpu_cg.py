@@ -165,3 +165,29 @@ def test_sbl_cg_unsupported_options_raise(device):
     y = torch.zeros((24, 24), dtype=torch.complex64, device=device)
     with pytest.raises(NotImplementedError):
         sbl_2d(y, (32, 32), config=SBLConfig(mode="cg", device=str(device), cg_preconditioner="nystrom"))
+
+
+@pytest.mark.parametrize("grid,samples", [((64, 48), (45, 40)), ((256, 256), (160, 160))])
+def test_pcg_graph_blocks_equal_eager(device, grid, samples):
+    """The PCG iterations between host checks replayed as a captured hipGraph
+    (many replays per solve, eager check / EM work in between) give bitwise
+    the eager loop's solve."""
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, pcg
+    ys = [scenes.ellipse_scene(40 + i, grid, samples, k=20, snr_db=15.0).y for i in range(2)]
+    y = torch.from_numpy(np.stack(ys).astype(np.complex64)).to(device)
+    cfg = SBLConfig(mode="cg", device=str(device), max_iter=4, min_iter=1, tol=0.0)
+    saved = pcg.USE_GRAPHS
+    runs = []
+    try:
+        for flag in (False, True):
+            pcg.USE_GRAPHS = flag
+            runs.append(pcg.sbl_cg(y, grid, cfg, batch_shape=(2,)))
+    finally:
+        pcg.USE_GRAPHS = saved
+    eager, graph = runs
+    print("pcg iterations", eager.history["cg_iterations"], flush=True)
+    assert max(eager.history["cg_iterations"]) >= 8   # at least one captured block ran
+    assert eager.history["cg_iterations"] == graph.history["cg_iterations"]
+    assert torch.equal(eager.x, graph.x)
+    assert torch.equal(eager.noise_variance, graph.noise_variance)
