@@ -57,12 +57,12 @@ def pcg_kernels(device, batch=8, reps=20):
     Hutchinson probes (lanes L = 17 B): the operators are set up from the
     solver's state after one EM iteration, then each entry point is timed
     alone with events on its stream.  Algorithmic bytes per call (complex64
-    = 8 B; M = M1 M2): covariance matvec on the 256 x 256 grid 16 L M +
-    32 L 256 M2 + 4 B 65536 (column pass of A^H, row pass with Gamma,
-    column pass of A; the [256, M2] intermediate round-trips twice; gamma
-    once per scene), or through the BTTB embedding E1 x E2 16 L M +
+    = 8 B; M = M1 M2): covariance matvec on the 256 x 256 grid 24 L M +
+    32 L 256 M2 + 4 B 65536 (column pass of A^H reads d, row pass with
+    Gamma, column pass of A reads d again for the noise term and writes Q d;
+    the [256, M2] intermediate round-trips twice; gamma once per scene), or through the BTTB embedding E1 x E2 16 L M +
     32 L M1 E2 + 8 B E1 E2; low-rank preconditioner 16 L M (the factor is generated from
-    its separable phase tables, 80 KB per scene); lane dot 16 L M; update
+    its separable phase tables, 80 KB per scene); residual norm <r, r> 8 L M; update
     48 L M; direction 24 L M; one PCG iteration = matvec + precondition + 2
     dots + update + direction."""
     import numpy as np
@@ -89,10 +89,10 @@ def pcg_kernels(device, batch=8, reps=20):
     e1, e2 = s.t.e1, s.t.e2
     calls = {
         "matvec": (lambda: s.matvec(s.d, s.q, dot=s.curv),
-                   (16 * L * M + 32 * L * 256 * samples[1] + 4 * batch * 65536) if s.grid_op
+                   (24 * L * M + 32 * L * 256 * samples[1] + 4 * batch * 65536) if s.grid_op
                    else (16 * L * M + 32 * L * m1 * e2 + 8 * batch * e1 * e2)),
         "precondition": (lambda: s.precondition(s.res, s.z, dot=s.nrho), 16 * L * M),
-        "lane_dot": (lambda: s.lane_dot(s.res, s.res, s.rr), 16 * L * M),
+        "lane_dot": (lambda: s.lane_dot(s.res, s.res, s.rr), 8 * L * M),
         "update": (lambda: lib.fbme_pcg_update(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.d),
                                                pcg._ptr(s.q), pcg._ptr(s.rho), pcg._ptr(s.curv),
                                                pcg._ptr(s.improved)), 48 * L * M),
