@@ -168,6 +168,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
   __shared__ double red[8];
   const int lane = threadIdx.x, g = lane >> 4, t = lane & 15;
   float2* x = xs[g];
+  int tw_loaded = -1;   // twiddle table in LDS (all 256-point axes share one)
   for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
     const FbmeWork wk = work[wi];
     const FbmeScene s = scenes[wk.scene];
@@ -182,7 +183,10 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const float noise = (float)scs[SC_NOISE];
     const float floor_v = (float)scs[SC_FLOOR];
     const float power = (float)scs[SC_POWER];
-    load_tw256(tw, twtab + s.tw2_off);
+    if (s.tw2_off != tw_loaded) {   // uniform: every lane reads the same scene
+      load_tw256(tw, twtab + s.tw2_off);
+      tw_loaded = s.tw2_off;
+    }
     float2 v[16];
     if (MODE != 1) {
 #pragma unroll
