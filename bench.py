@@ -60,11 +60,17 @@ def pcg_kernels(device, batch=8, reps=20):
     = 8 B; M = M1 M2): covariance matvec on the 256 x 256 grid 24 L M +
     32 L 256 M2 + 4 B 65536 (column pass of A^H reads d, row pass with
     Gamma, column pass of A reads d again for the noise term and writes Q d;
-    the [256, M2] intermediate round-trips twice; gamma once per scene), or through the BTTB embedding E1 x E2 16 L M +
-    32 L M1 E2 + 8 B E1 E2; low-rank preconditioner 16 L M (the factor is generated from
-    its separable phase tables, 80 KB per scene); residual norm <r, r> 8 L M; update
-    48 L M; direction 24 L M; one PCG iteration = matvec + precondition + 2
-    dots + update + direction."""
+    the [256, M2] intermediate round-trips twice; gamma once per scene), or
+    through the BTTB embedding E1 x E2 16 L M + 32 L M1 E2 + 8 B E1 E2;
+    low-rank preconditioner through the 256-point column transform (N1 =
+    256) 24 L M + 16 L 32 M2 (v read by the projection and again by the
+    correction, z written; the [32, M2] slot rows of the projection written
+    and read once),
+    otherwise 16 L M (factor generated from its separable phase tables);
+    residual norm <r, r> 8 L M; update 48 L M; direction 24 L M; one PCG
+    iteration = matvec (with <d, Q d>) + precondition (with <r, M^-1 r>) +
+    update + direction + the residual norm (counted once, although only
+    check iterations need it)."""
     import numpy as np
     import torch
 
@@ -91,7 +97,8 @@ def pcg_kernels(device, batch=8, reps=20):
         "matvec": (lambda: s.matvec(s.d, s.q, dot=s.curv),
                    (24 * L * M + 32 * L * 256 * samples[1] + 4 * batch * 65536) if s.grid_op
                    else (16 * L * M + 32 * L * m1 * e2 + 8 * batch * e1 * e2)),
-        "precondition": (lambda: s.precondition(s.res, s.z, dot=s.nrho), 16 * L * M),
+        "precondition": (lambda: s.precondition(s.res, s.z, dot=s.nrho),
+                         (24 * L * M + 16 * L * 32 * samples[1]) if s.lr_fft else 16 * L * M),
         "lane_dot": (lambda: s.lane_dot(s.res, s.res, s.rr), 8 * L * M),
         "update": (lambda: lib.fbme_pcg_update(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.d),
                                                pcg._ptr(s.q), pcg._ptr(s.rho), pcg._ptr(s.curv),

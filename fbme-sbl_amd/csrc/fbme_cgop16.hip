@@ -133,9 +133,228 @@ __global__ void __launch_bounds__(64) k_gop_finish(const double* __restrict__ pa
   if (threadIdx.x == 0) out[l] = s;
 }
 
+// ------------------------------------------ low-rank preconditioner, N1 = 256
+// F[m, k] = U[m1, k] V[m2, k] with U[m1, k] = e^{-2 pi i m1 k1 / 256}: the
+// projection F^H v and the synthesis F c factor through the 256-point column
+// transform.  With the atoms' distinct rows as "slots":
+//   F^H v [k] = sum_{m2} conj(V[m2, k]) W[slot(k), m2],
+//     W[s, m2] = sum_{m1} e^{+2 pi i m1 row(s) / 256} v[m1, m2]   (column IFFT, slot rows kept);
+//   (F c)[m1, m2] = sum_s e^{-2 pi i m1 row(s) / 256} T[s, m2],
+//     T[s, m2] = sum_{k in s} V[m2, k] c[k]       (slot rows formed in LDS, then their column FFT).
+// map[b] = slot of each atom [32], row of each slot [32], slot count [1],
+// slot of each grid row (or -1) [256].
+constexpr int RANK = 32;
+constexpr int MAP_INTS = 32 + 32 + 1 + 256;
+
+__global__ void __launch_bounds__(64)
+k_lrf_map(const int* __restrict__ idx, int n2, int* __restrict__ map) {
+  const int b = blockIdx.x;
+  int* m = map + (long long)b * MAP_INTS;
+  for (int r = threadIdx.x; r < 256; r += 64) m[65 + r] = -1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int rows[RANK];
+    int ns = 0;
+    for (int k = 0; k < RANK; ++k) {
+      const int r = idx[b * RANK + k] / n2;
+      int sl = -1;
+      for (int q = 0; q < ns; ++q)
+        if (rows[q] == r) sl = q;
+      if (sl < 0) {
+        sl = ns;
+        rows[ns++] = r;
+      }
+      m[k] = sl;
+    }
+    for (int q = 0; q < RANK; ++q) m[32 + q] = q < ns ? rows[q] : -1;
+    m[64] = ns;
+    for (int q = 0; q < ns; ++q) m[65 + rows[q]] = q;
+  }
+}
+
+// W[lane, s, c] for the slot rows of the column IFFT of v[lane, :, c].
+__global__ void __launch_bounds__(16 * GC)
+k_lrf_col_project(const float2* __restrict__ v, const int* __restrict__ map, float2* __restrict__ w, int m1, int m2,
+                  int rhs, const float2* __restrict__ twtab) {
+  __shared__ float2 xs[16 * 17 * GC];
+  __shared__ float2 tw[256];
+  __shared__ int rowslot[256];
+  const int c = threadIdx.x % GC, t = threadIdx.x / GC;
+  const int lane = blockIdx.y;
+  const int col = blockIdx.x * GC + c;
+  const bool act = col < m2;
+  rowslot[threadIdx.x] = map[(long long)(lane / rhs) * MAP_INTS + 65 + threadIdx.x];
+  load_tw256(tw, twtab);
+  const float2* src = v + (long long)lane * m1 * m2;
+  float2 x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int i = t + 16 * j;
+    x[j] = (act && i < m1) ? src[(long long)i * m2 + col] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft256<true, GC>(x, tw, xs + c, t);
+  if (act) {
+    float2* dst = w + (long long)lane * RANK * m2 + col;
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+      const int sl = rowslot[t + 16 * n];
+      if (sl >= 0) dst[(long long)sl * m2] = x[n];
+    }
+  }
+}
+
+// Partial projections over 16 samples per workgroup: part[lane, chunk, k] =
+// sum_{m2 in chunk} conj(V[m2, k]) W[lane, slot(k), m2]; thread (g, k) takes
+// samples 2g, 2g + 1 of the chunk, the eight groups are added in FP64 (the
+// chunks are summed by k_lr_corr).
+constexpr int LRF_CHUNK = 16;
+
+__global__ void __launch_bounds__(256)
+k_lrf_row_project(const float2* __restrict__ w, const float2* __restrict__ tab, const int* __restrict__ map,
+                  int m1, int m2, int rhs, double2* __restrict__ part) {
+  __shared__ float2 red[8][RANK];
+  const int lane = blockIdx.y;
+  const int b = lane / rhs;
+  const int k = threadIdx.x & (RANK - 1), g = threadIdx.x >> 5;
+  const float2* V = tab + ((long long)b * (m1 + m2) + m1) * RANK;
+  const int sl = map[(long long)b * MAP_INTS + k];
+  const float2* wr = w + ((long long)lane * RANK + sl) * m2;
+  float ax = 0.f, ay = 0.f;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int c = blockIdx.x * LRF_CHUNK + 2 * g + q;
+    if (c < m2) {
+      const float2 vv = V[(long long)c * RANK + k], ww = wr[c];
+      ax += vv.x * ww.x + vv.y * ww.y;
+      ay += vv.x * ww.y - vv.y * ww.x;
+    }
+  }
+  red[g][k] = make_float2(ax, ay);
+  __syncthreads();
+  if (threadIdx.x < RANK) {
+    double2 sum = make_double2(0.0, 0.0);
+    for (int q = 0; q < 8; ++q) {
+      sum.x += red[q][k].x;
+      sum.y += red[q][k].y;
+    }
+    part[((long long)lane * gridDim.x + blockIdx.x) * RANK + k] = sum;
+  }
+}
+
+// out[lane, m1, c] = v / bg - sum_s e^{-2 pi i m1 row(s) / 256} T[s, c] with
+// T[s, c] = sum_{k in s} V[c, k] corr[lane, k] formed in LDS for the
+// workgroup's columns (each atom feeds one slot; fixed summation order),
+// then the column FFT of the slot rows; part[lane, block] = sum Re(conj(v) out).
+__global__ void __launch_bounds__(16 * GC)
+k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab, const int* __restrict__ map,
+                const float2* __restrict__ v, const float* __restrict__ bg, float2* __restrict__ out, int m1,
+                int m2, int rhs, const float2* __restrict__ twtab, double* __restrict__ part) {
+  __shared__ float2 xs[16 * 17 * GC];
+  __shared__ float2 tw[256];
+  __shared__ double red[8];
+  __shared__ int rowslot[256];
+  __shared__ int aslot[RANK];
+  __shared__ float2 cs[RANK];
+  __shared__ float2 vsm[GC][RANK];
+  __shared__ float2 ts[RANK][GC];
+  const int c = threadIdx.x % GC, t = threadIdx.x / GC;
+  const int lane = blockIdx.y;
+  const int b = lane / rhs;
+  const int col = blockIdx.x * GC + c;
+  const bool act = col < m2;
+  const int* mb = map + (long long)b * MAP_INTS;
+  const float2* V = tab + ((long long)b * (m1 + m2) + m1) * RANK;
+  rowslot[threadIdx.x] = mb[65 + threadIdx.x];
+  if (threadIdx.x < RANK) {
+    aslot[threadIdx.x] = mb[threadIdx.x];
+    cs[threadIdx.x] = corr[(long long)lane * RANK + threadIdx.x];
+  }
+  for (int e = threadIdx.x; e < GC * RANK; e += 16 * GC) {
+    const int cc = e / RANK, k = e - cc * RANK;
+    const int c2 = blockIdx.x * GC + cc;
+    vsm[cc][k] = (c2 < m2) ? V[(long long)c2 * RANK + k] : make_float2(0.f, 0.f);
+  }
+  load_tw256(tw, twtab);
+  __syncthreads();
+  const int ns = mb[64];
+  for (int e = threadIdx.x; e < RANK * GC; e += 16 * GC) {
+    const int sl = e / GC, cc = e - sl * GC;
+    float ax = 0.f, ay = 0.f;
+    if (sl < ns) {
+      for (int k = 0; k < RANK; ++k) {
+        if (aslot[k] != sl) continue;
+        const float2 vv = vsm[cc][k], w = cs[k];
+        ax += vv.x * w.x - vv.y * w.y;
+        ay += vv.x * w.y + vv.y * w.x;
+      }
+    }
+    ts[sl][cc] = make_float2(ax, ay);
+  }
+  __syncthreads();
+  float2 x[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int sl = rowslot[t + 16 * j];
+    x[j] = (act && sl >= 0) ? ts[sl][c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft256<false, GC>(x, tw, xs + c, t);
+  const float ib = 1.0f / bg[b];
+  const long long base = (long long)lane * m1 * m2 + col;
+  double acc = 0.0;
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int k = t + 16 * n;
+    if (act && k < m1) {
+      const float2 vv = v[base + (long long)k * m2];
+      const float2 o = make_float2(vv.x * ib - x[n].x, vv.y * ib - x[n].y);
+      out[base + (long long)k * m2] = o;
+      if (part) acc += (double)vv.x * o.x + (double)vv.y * o.y;
+    }
+  }
+  if (part) {
+    acc = block_sum(acc, red);
+    if (threadIdx.x == 0) part[(long long)lane * gridDim.x + blockIdx.x] = acc;
+  }
+}
+
 }  // namespace
 
+// k_lr_corr of fbme_pcg.hip: corr = S^-1 (proj / bg) / bg from chunk partials.
+int fbme_lr_corr_launch(int lanes, int rhs, int chunks, const void* part, const void* sinv, const float* bg,
+                        void* corr, hipStream_t st);
+
 extern "C" {
+
+int fbme_lowrank_map(void* stream, int batch, int n1, int n2, const int* idx, int* map) {
+  if (batch < 1 || n1 != 256 || n2 < 1) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_lrf_map, dim3(batch), dim3(64), 0, (hipStream_t)stream, idx, n2, map);
+  return (int)hipGetLastError();
+}
+
+int fbme_lowrank_apply_fft_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                               const void* sinv, const float* bg, const int* map, const void* v, void* out,
+                               void* wbuf, void* part, void* corr, const void* tw256, double* dot_part,
+                               double* dot) {
+  if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 256 || v == out) return FBME_ERR_ARG;
+  if ((dot_part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 cg((m2 + GC - 1) / GC, lanes);
+  const float2* tab = (const float2*)factors;
+  hipLaunchKernelGGL(k_lrf_col_project, cg, dim3(16 * GC), 0, st, (const float2*)v, map, (float2*)wbuf, m1, m2, rhs,
+                     (const float2*)tw256);
+  const int chunks = (m2 + LRF_CHUNK - 1) / LRF_CHUNK;
+  hipLaunchKernelGGL(k_lrf_row_project, dim3(chunks, lanes), dim3(256), 0, st, (const float2*)wbuf, tab, map, m1,
+                     m2, rhs, (double2*)part);
+  int rc = fbme_lr_corr_launch(lanes, rhs, chunks, part, sinv, bg, corr, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lrf_col_apply, cg, dim3(16 * GC), 0, st, (const float2*)corr, tab, map, (const float2*)v,
+                     bg, (float2*)out, m1, m2, rhs, (const float2*)tw256, dot_part);
+  if (dot)
+    hipLaunchKernelGGL(k_gop_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)cg.x, lanes, dot);
+  return (int)hipGetLastError();
+}
 
 int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
                          const void* d, void* out, void* mid, const void* tw256, double* part, double* dot) {

@@ -821,6 +821,14 @@ int toeplitz_col_cap(int e1, int e2) { return std::max(1, std::min({e2, 16, 24 *
 
 }  // namespace
 
+// Launcher of k_lr_corr shared with the FFT-based preconditioner passes.
+int fbme_lr_corr_launch(int lanes, int rhs, int chunks, const void* part, const void* sinv, const float* bg,
+                        void* corr, hipStream_t st) {
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,
+                     (const float2*)sinv, bg, rhs, (float2*)corr);
+  return (int)hipGetLastError();
+}
+
 extern "C" {
 
 int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,

@@ -131,6 +131,11 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
+    "fbme_lowrank_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "fbme_lowrank_apply_fft_dot": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                           c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
     "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,

@@ -166,8 +166,11 @@ class _Solver:
         self.lanes = self.b * self.r
         self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
         self._graphs = {}
-        # 256 x 256 grids apply Q through the grid on the register FFT passes
+        # 256 x 256 grids apply Q through the grid on the register FFT passes;
+        # N1 = 256 grids project / synthesise the preconditioner's atoms
+        # through the 256-point column transform
         self.grid_op = self.grid == (256, 256)
+        self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
@@ -234,7 +237,9 @@ class _Solver:
         # lane scalars
         self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
         self.mv_part = torch.empty(self.lanes * ((m2 + 15) // 16), dtype=f64, device=dev)
-        self.pc_part = torch.empty(self.lanes * ((m + 255) // 256), dtype=f64, device=dev)
+        self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
+        self.lr_map = torch.empty((b, 321), dtype=torch.int32, device=dev)
+        self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
         self.rho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.nrho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.curv = torch.empty(self.lanes, dtype=f64, device=dev)
@@ -299,6 +304,13 @@ class _Solver:
 
     def precondition(self, v, out, dot=None):
         """out = M^-1 v; with `dot`, also dot[lane] = Re <v, out> (FP64)."""
+        if self.lr_fft:
+            self._chk(self.lib.fbme_lowrank_apply_fft_dot(
+                self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv),
+                _ptr(self.bg), _ptr(self.lr_map), _ptr(v), _ptr(out), _ptr(self.lr_buf), _ptr(self.lr_part),
+                _ptr(self.corr), _ptr(self.t.tw[256]), _ptr(self.pc_part) if dot is not None else None,
+                _ptr(dot)), "lowrank_apply_fft")
+            return
         self._chk(self.lib.fbme_lowrank_apply_dot(
             self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
             _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr),
@@ -338,6 +350,9 @@ class _Solver:
             self.stream, self.b, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
             _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
             _ptr(self.status)), "lowrank_setup")
+        if self.lr_fft:
+            self._chk(self.lib.fbme_lowrank_map(self.stream, self.b, n1, n2, _ptr(self.top_idx), _ptr(self.lr_map)),
+                      "lowrank_map")
 
     def spectrum_setup(self):
         t = self.t
@@ -428,7 +443,7 @@ class _Solver:
         """ci - 1 full PCG iterations and the first half of the next, the
         work between two host convergence checks, as one hipGraph captured
         on first use (same kernels and buffers as the eager loop)."""
-        key = (ci, self.grid_op, tuple(self.tiles))
+        key = (ci, self.grid_op, self.lr_fft, tuple(self.tiles))
         graph = self._graphs.get(key)
         if graph is None:
             graph = torch.cuda.CUDAGraph()

@@ -256,6 +256,21 @@ int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const v
 int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
                            const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
                            double* dot);
+/* The same preconditioner for N1 = 256 grids through the 256-point column
+ * transform: the projection is the column IFFT of v restricted to the
+ * atoms' distinct rows, then 32 row sums; the correction is a row synthesis
+ * on those rows, then one column FFT (same operator as fbme_lowrank_apply,
+ * solver.py:5836-5848, with about a quarter of the flops).
+ * fbme_lowrank_map builds map [batch, 321] ints from the atoms of
+ * fbme_topk32 (slot of each atom, row of each slot, slot count, slot of each
+ * grid row).  wbuf: lanes * 32 * M2 complex64; part: lanes * ceil(M2 / 16)
+ * * 32 double2;
+ * dot_part: lanes * ceil(M2 / 16) doubles; tw256: fbme_twiddle_table(256). */
+int fbme_lowrank_map(void* stream, int batch, int n1, int n2, const int* idx, int* map);
+int fbme_lowrank_apply_fft_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                               const void* sinv, const float* bg, const int* map, const void* v, void* out,
+                               void* wbuf, void* part, void* corr, const void* tw256, double* dot_part,
+                               double* dot);
 /* out[l] = sum Re(conj(a_l) b_l) in FP64 (the PCG inner products of
  * _pcg, solver.py:5582-5723).  part: lanes * 64 doubles. */
 int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out);
