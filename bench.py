@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--skip-pcg", action="store_true")
     ap.add_argument("--pcg-batch", type=int, default=8)
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--learned-scenes", type=int, default=4)
+    ap.add_argument("--learned-scenes", type=int, default=16)
     return ap.parse_args()
 
 
@@ -189,8 +189,10 @@ def learned_comparison(device, count):
         return None
     net = learned.WarmStartNet.load(device)
     sc = [scenes.ellipse_scene(1300 + i, (256, 256), (160, 160), k=300, snr_db=10.0) for i in range(count)]
+    w = net.warm_iter
     runs = [("plain_reference_budget", "plain", 100), ("plain_target_iter", "plain", net.target_iter),
-            ("plain_warm_budget", "plain", net.warm_iter), ("learned", "learned", net.warm_iter)]
+            ("plain_warm_budget", "plain", w), ("learned", "learned", w),
+            (f"plain_{w - 1}", "plain", w - 1), (f"learned_{w - 1}", "learned", w - 1)]
     out = {"workload": f"{count} single-scene solves, 256x256/160x160, K=300, 10 dB",
            "target_iter": net.target_iter, "warm_iter": net.warm_iter}
     for name, kind, iters in runs:
@@ -215,6 +217,28 @@ def learned_comparison(device, count):
                      "scenes_per_s": 1.0 / float(np.mean(secs)), "nmse_db": float(np.mean(nm)),
                      "median_nmse_db": float(np.median(nm)), "fallbacks": fb}
         print(f"[bench] learned comparison {name}: {out[name]}", file=sys.stderr, flush=True)
+    # batched: all scenes of the sample in one solve (probes in the lane
+    # dimension of every scene); a stalled warm start reruns the batch plain
+    yb = torch.from_numpy(np.stack([s.y for s in sc])).to(device)
+    for name, kind, iters in [runs[1], runs[3], runs[5]]:
+        cfg = SBLConfig(mode="cg", device=str(device), max_iter=iters, min_iter=1, tol=0.0,
+                        hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
+        solve = ((lambda: learned.learned_sbl_2d(yb, (256, 256), net, cfg)) if kind == "learned"
+                 else (lambda: pcg.sbl_cg(yb, (256, 256), cfg, batch_shape=(count,))))
+        solve()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = solve()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        x = r.x.reshape(count, -1).cpu().numpy()
+        nm = [scenes.nmse_db(x[i], s.x.reshape(-1)) for i, s in enumerate(sc)]
+        out[name + "_batched"] = {"em_iterations": iters, "batch": count,
+                                  "pcg_iterations": float(np.sum(r.history["cg_iterations"])),
+                                  "scenes_per_s": count / dt, "nmse_db": float(np.mean(nm)),
+                                  "fallbacks": r.history.get("learned_fallback", [0.0])[0]}
+        print(f"[bench] learned comparison {name} batched: {out[name + '_batched']}", file=sys.stderr,
+              flush=True)
     return out
 
 

@@ -134,13 +134,13 @@ def main(argv=None):
     ap.add_argument("--k", type=int, default=300)
     ap.add_argument("--snr", type=float, default=10.0)
     ap.add_argument("--target-iter", type=int, default=10)
-    ap.add_argument("--warm-iter", type=int, default=2)
+    ap.add_argument("--warm-iter", type=int, default=3)
     ap.add_argument("--plain-iter", type=int, default=100)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--minibatch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval", type=int, default=16)
-    ap.add_argument("--extra-iters", default="3,5", help="further budgets evaluated for both solvers")
+    ap.add_argument("--extra-iters", default="2,5", help="further budgets evaluated for both solvers")
     args = ap.parse_args(argv)
     device = torch.device("cuda", 0)
     grid, samples = (args.grid, args.grid), (args.samples, args.samples)

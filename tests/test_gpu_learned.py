@@ -74,7 +74,7 @@ def test_warm_start_gains_over_plain(device, net):
     # NMSE in dB: lower is better.  Gain at the same EM budget:
     assert learned_b[0] < plain_b[0] - 0.5
     # the imitated trajectory point's quality with far fewer PCG iterations
-    assert learned_b[0] < plain_t[0] + 0.3
+    assert learned_b[0] < plain_t[0] + 0.15
     assert learned_b[1] < 0.5 * plain_t[1]
 
 
