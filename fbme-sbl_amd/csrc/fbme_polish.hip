@@ -845,16 +845,22 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
   __shared__ int sidx[KM];
   __shared__ float2 smu[KM];
   __shared__ float ssig[KM], sgam[KM];
+  __shared__ unsigned int hitbits[FBME_XTILE / 32];
   for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
     const FbmeWork wk = work[w];
     const FbmeScene s = scenes[wk.scene];
     const PolishResult* r = results + wk.scene;
     const int K = s.k;
+    if (threadIdx.x < FBME_XTILE / 32) hitbits[threadIdx.x] = 0u;
+    __syncthreads();
     if (threadIdx.x < K) {
-      sidx[threadIdx.x] = r->idx[threadIdx.x];
+      const int id = r->idx[threadIdx.x];
+      sidx[threadIdx.x] = id;
       smu[threadIdx.x] = r->mu[threadIdx.x];
       ssig[threadIdx.x] = r->sig[threadIdx.x];
       sgam[threadIdx.x] = r->gam[threadIdx.x];
+      const int rel = id - wk.first;
+      if (rel >= 0 && rel < wk.count) atomicOr(&hitbits[rel >> 5], 1u << (rel & 31));
     }
     __syncthreads();
     const float floor_v = (float)sc[(long long)wk.scene * SC_COUNT + SC_FLOOR];
@@ -862,33 +868,65 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
     // log(N / false_alarm_budget) (solver.py:2108-2126), float32 like the
     // reference's torch.full_like(noise_variance, value).
     const float thr = (float)log((double)s.n1 * (double)s.n2 / (double)threshold);
-    for (int e = threadIdx.x; e < wk.count; e += FBME_BLOCK) {
-      const int n = wk.first + e;
-      int lo = 0, hi = K - 1, hit = -1;
-      while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = sidx[mid];
-        if (v == n) {
-          hit = mid;
-          break;
+    // cell n: the polished atom's mean / variance / precision or the pruned
+    // tail's zero / floor / 1 / floor; the LDS bitmap marks the (at most K)
+    // cells of this tile that hold an atom
+    auto cell = [&](int e, float2& xv, float& pv, float& pr, unsigned char& mk) {
+      xv = make_float2(0.f, 0.f);
+      pv = floor_v;
+      pr = inv_floor;
+      if ((hitbits[e >> 5] >> (e & 31)) & 1u) {
+        const int n = wk.first + e;
+        int lo = 0, hi = K - 1, hit = -1;
+        while (lo <= hi) {
+          const int mid = (lo + hi) >> 1;
+          const int v = sidx[mid];
+          if (v == n) {
+            hit = mid;
+            break;
+          }
+          if (v < n) lo = mid + 1;
+          else hi = mid - 1;
         }
-        if (v < n) lo = mid + 1;
-        else hi = mid - 1;
+        if (hit >= 0) {
+          xv = smu[hit];
+          pv = ssig[hit];
+          pr = 1.0f / sgam[hit];
+        }
       }
-      float2 xv = make_float2(0.f, 0.f);
-      float pv = floor_v, pr = inv_floor;
-      if (hit >= 0) {
-        xv = smu[hit];
-        pv = ssig[hit];
-        pr = 1.0f / sgam[hit];
+      const float score = (xv.x * xv.x + xv.y * xv.y) / fmaxf(pv, 1.17549435e-38f);
+      mk = score > thr ? 1 : 0;
+    };
+    const long long g0 = s.grid_off + wk.first;
+    if ((g0 & 3) == 0 && (wk.count & 3) == 0) {
+      // four consecutive cells per thread: 16-byte stores, one mask word
+      const int e = 4 * threadIdx.x;
+      for (int e0 = e; e0 < wk.count; e0 += 4 * FBME_BLOCK) {
+        float2 xv[4];
+        float pv[4], pr[4];
+        unsigned char mk[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) cell(e0 + q, xv[q], pv[q], pr[q], mk[q]);
+        const long long gi = g0 + e0;
+        reinterpret_cast<float4*>(x + gi)[0] = make_float4(xv[0].x, xv[0].y, xv[1].x, xv[1].y);
+        reinterpret_cast<float4*>(x + gi)[1] = make_float4(xv[2].x, xv[2].y, xv[3].x, xv[3].y);
+        *reinterpret_cast<float4*>(pvar + gi) = make_float4(pv[0], pv[1], pv[2], pv[3]);
+        *reinterpret_cast<float4*>(prec + gi) = make_float4(pr[0], pr[1], pr[2], pr[3]);
+        if (with_mask)
+          *reinterpret_cast<unsigned int*>(mask + gi) =
+              (unsigned)mk[0] | ((unsigned)mk[1] << 8) | ((unsigned)mk[2] << 16) | ((unsigned)mk[3] << 24);
       }
-      const long long gi = s.grid_off + n;
-      x[gi] = xv;
-      pvar[gi] = pv;
-      prec[gi] = pr;
-      if (with_mask) {
-        const float score = (xv.x * xv.x + xv.y * xv.y) / fmaxf(pv, 1.17549435e-38f);
-        mask[gi] = score > thr ? 1 : 0;
+    } else {
+      for (int e = threadIdx.x; e < wk.count; e += FBME_BLOCK) {
+        float2 xv;
+        float pv, pr;
+        unsigned char mk;
+        cell(e, xv, pv, pr, mk);
+        const long long gi = g0 + e;
+        x[gi] = xv;
+        pvar[gi] = pv;
+        prec[gi] = pr;
+        if (with_mask) mask[gi] = mk;
       }
     }
     __syncthreads();
