@@ -338,7 +338,7 @@ int fbme_lowrank_apply_fft_dot(void* stream, int lanes, int rhs, int m1, int m2,
                                void* wbuf, void* part, void* corr, const void* tw256, double* dot_part,
                                double* dot) {
   if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 256 || v == out) return FBME_ERR_ARG;
-  if ((dot_part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
+  if (dot != nullptr && dot_part == nullptr) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const dim3 cg((m2 + GC - 1) / GC, lanes);
   const float2* tab = (const float2*)factors;
@@ -360,18 +360,18 @@ int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const
                          const void* d, void* out, void* mid, const void* tw256, double* part, double* dot) {
   if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 256 || m2 > 256 || d == out)
     return FBME_ERR_ARG;
-  if ((part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
+  if (dot != nullptr && part == nullptr) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const dim3 cg((m2 + GC - 1) / GC, lanes);
   hipLaunchKernelGGL(k_gop_col_adjoint, cg, dim3(16 * GC), 0, st, (const float2*)d, (float2*)mid, m1, m2,
                      (const float2*)tw256);
   hipLaunchKernelGGL(k_gop_row, dim3(64, lanes), dim3(64), 0, st, (float2*)mid, gamma, m2, rhs,
                      (const float2*)tw256);
-  if (dot) {
+  if (part) {
     hipLaunchKernelGGL(k_gop_col_forward<true>, cg, dim3(16 * GC), 0, st, (const float2*)mid, (const float2*)d, noise,
                        (float2*)out, m1, m2, rhs, (const float2*)tw256, part);
-    hipLaunchKernelGGL(k_gop_finish, dim3(lanes), dim3(64), 0, st, (const double*)part, (int)cg.x, lanes,
-                       dot);
+    if (dot)
+      hipLaunchKernelGGL(k_gop_finish, dim3(lanes), dim3(64), 0, st, (const double*)part, (int)cg.x, lanes, dot);
   } else {
     hipLaunchKernelGGL(k_gop_col_forward<false>, cg, dim3(16 * GC), 0, st, (const float2*)mid, (const float2*)d,
                        noise, (float2*)out, m1, m2, rhs, (const float2*)tw256, (double*)nullptr);

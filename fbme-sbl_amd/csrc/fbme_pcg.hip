@@ -586,6 +586,86 @@ __global__ void k_pcg_commit_rho(double* __restrict__ rho, const double* __restr
   active[l] = vu;
 }
 
+// Variants of the update and direction passes that take their lane inner
+// product as per-block partials of the producing pass (summed here in a fixed
+// order, so no separate reduction launch) and fold the validity commit of
+// k_pcg_commit_valid into the rho commit: the lane logic is the same as
+// update -> commit_valid -> direction -> commit_rho.
+__device__ __forceinline__ double lane_parts(const double* __restrict__ part, int nparts, int lanev) {
+  // up to 16 partials (M2 <= 256) loaded together, then added in index order
+  const double* p = part + (long long)lanev * nparts;
+  double v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = i < nparts ? p[i] : 0.0;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) s += v[i];
+  for (int i = 16; i < nparts; ++i) s += p[i];
+  return s;
+}
+
+__device__ __forceinline__ bool pcg_valid(double cv, double rh) {
+  return isfinite(cv) && isfinite(rh) && cv > F32_TINY_D && rh > F32_TINY_D;
+}
+
+// x += alpha d, r -= alpha q with curv = sum of the matvec's partials; block
+// (0, lane) stores curv[lane].
+__global__ void k_pcg_update_parts(float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ d,
+                                   const float2* __restrict__ q, const double* __restrict__ rho,
+                                   const double* __restrict__ part, int nparts, double* __restrict__ curv,
+                                   const int* __restrict__ active, int m) {
+  const int lanev = blockIdx.y;
+  const double cv = lane_parts(part, nparts, lanev), rh = rho[lanev];
+  if (blockIdx.x == 0 && threadIdx.x == 0) curv[lanev] = cv;
+  const bool valid = active[lanev] && pcg_valid(cv, rh);
+  const float alpha = valid ? (float)(rh / fmax(cv, F32_TINY_D)) : 0.f;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && alpha != 0.f) {
+    const long long g = (long long)lanev * m + i;
+    const float2 dv = d[g], qv = q[g];
+    float2 xv = x[g], rv = r[g];
+    xv.x += alpha * dv.x;
+    xv.y += alpha * dv.y;
+    rv.x -= alpha * qv.x;
+    rv.y -= alpha * qv.y;
+    x[g] = xv;
+    r[g] = rv;
+  }
+}
+
+// d = z + beta d with nrho = sum of the preconditioner's partials and the
+// lane active iff it was, its update was valid and nrho is; block (0, lane)
+// stores nrho[lane].
+__global__ void k_pcg_direction_parts(float2* __restrict__ d, const float2* __restrict__ z,
+                                      const double* __restrict__ rho, const double* __restrict__ curv,
+                                      const double* __restrict__ part, int nparts, double* __restrict__ nrho,
+                                      const int* __restrict__ active, int m) {
+  const int lanev = blockIdx.y;
+  const double nr = lane_parts(part, nparts, lanev), rh = rho[lanev];
+  if (blockIdx.x == 0 && threadIdx.x == 0) nrho[lanev] = nr;
+  const bool vu = active[lanev] && pcg_valid(curv[lanev], rh) && isfinite(nr) && nr > F32_TINY_D;
+  const float beta = vu ? (float)(nr / fmax(rh, F32_TINY_D)) : 0.f;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const long long g = (long long)lanev * m + i;
+  if (vu) {
+    const float2 zv = z[g], dv = d[g];
+    d[g] = make_float2(zv.x + beta * dv.x, zv.y + beta * dv.y);
+  } else {
+    d[g] = make_float2(0.f, 0.f);
+  }
+}
+
+__global__ void k_pcg_commit_parts(double* __restrict__ rho, const double* __restrict__ curv,
+                                   const double* __restrict__ nrho, int* __restrict__ active, int lanes) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= lanes) return;
+  const double nr = nrho[l];
+  const bool vu = active[l] && pcg_valid(curv[l], rho[l]) && isfinite(nr) && nr > F32_TINY_D;
+  if (vu) rho[l] = nr;
+  active[l] = vu;
+}
+
 // r = rhs - Qx (true residual at a check iteration).
 __global__ void k_sub(const float2* __restrict__ a, const float2* __restrict__ b, float2* __restrict__ out,
                       long long n) {
@@ -989,6 +1069,30 @@ int fbme_pcg_update(void* stream, int lanes, int m, void* x, void* r, const void
   hipLaunchKernelGGL(k_pcg_update, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (float2*)x, (float2*)r,
                      (const float2*)d, (const float2*)q, rho, curv, active, m);
   hipLaunchKernelGGL(k_pcg_commit_valid, dim3(nblk(lanes, 64)), dim3(64), 0, st, rho, curv, active, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_update_parts(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                          const double* rho, const double* part, int nparts, double* curv, const int* active) {
+  if (lanes < 1 || m < 1 || nparts < 1) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_pcg_update_parts, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (float2*)x,
+                     (float2*)r, (const float2*)d, (const float2*)q, rho, part, nparts, curv, active, m);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_direction_parts(void* stream, int lanes, int m, void* d, const void* z, double* rho,
+                             const double* curv, const double* part, int nparts, double* nrho, int* active) {
+  if (lanes < 1 || m < 1 || nparts < 1) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_pcg_direction_parts, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (float2*)d,
+                     (const float2*)z, rho, curv, part, nparts, nrho, active, m);
+  hipLaunchKernelGGL(k_pcg_commit_parts, dim3(nblk(lanes, 64)), dim3(64), 0, st, rho, curv, nrho, active, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const double* curv, int* active) {
+  hipLaunchKernelGGL(k_pcg_commit_valid, dim3(nblk(lanes, 64)), dim3(64), 0, (hipStream_t)stream, rho, curv, active,
+                     lanes);
   return (int)hipGetLastError();
 }
 

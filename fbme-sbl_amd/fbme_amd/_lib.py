@@ -132,6 +132,13 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
     "fbme_lowrank_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "fbme_pcg_update_parts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_void_p, c_void_p, c_void_p,
+                                      c_int, c_void_p, c_void_p]),
+    "fbme_pcg_direction_parts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_int,
+                                         c_void_p, c_void_p]),
+    "fbme_pcg_commit_valid": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fbme_lowrank_apply_fft_dot": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                            c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p,

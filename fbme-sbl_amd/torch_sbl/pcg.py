@@ -190,6 +190,10 @@ class _Solver:
         # through the 256-point column transform
         self.grid_op = self.grid == (256, 256)
         self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256
+        # both on the column-transform passes: the PCG update / direction sum
+        # the inner-product partials of those passes themselves
+        self.fused = self.grid_op and self.lr_fft
+        self.nparts = (self.samples[1] + 15) // 16
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
@@ -419,6 +423,7 @@ class _Solver:
                 it += 1
                 self._iter_first()
             if it % ci == 0 or it == limit:
+                self._commit_valid()
                 self.matvec(self.x, self.q)
                 self._chk(lib.fbme_sub(self.stream, self.x.numel(), _ptr(self.rhs), _ptr(self.q),
                                        _ptr(self.res)), "sub")
@@ -443,24 +448,53 @@ class _Solver:
         return it
 
     def _iter_first(self):
-        """q = Q d with the curvature <d, q>; x += alpha d, r -= alpha q."""
+        """q = Q d with the curvature <d, q>; x += alpha d, r -= alpha q (the
+        validity commit of the lanes is left to the direction pass or to
+        commit_valid before a convergence check)."""
+        lib, t = self.lib, self.t
+        if self.fused:
+            m1, m2 = self.samples
+            self._chk(lib.fbme_grid_matvec_dot(
+                self.stream, self.lanes, self.r, m1, m2, _ptr(self.gamma), _ptr(self.noise), _ptr(self.d),
+                _ptr(self.q), _ptr(self.gmid), _ptr(t.tw[256]), _ptr(self.mv_part), None), "grid_matvec")
+            self._chk(lib.fbme_pcg_update_parts(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
+                                                _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.mv_part),
+                                                self.nparts, _ptr(self.curv), _ptr(self.active)), "update")
+            return
         self.matvec(self.d, self.q, dot=self.curv)
-        self._chk(self.lib.fbme_pcg_update(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
-                                           _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.curv),
-                                           _ptr(self.active)), "update")
+        self._chk(lib.fbme_pcg_update(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
+                                      _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.curv),
+                                      _ptr(self.active)), "update")
+
+    def _commit_valid(self):
+        """Lane validity after _iter_first, before a convergence check."""
+        if self.fused:
+            self._chk(self.lib.fbme_pcg_commit_valid(self.stream, self.lanes, _ptr(self.rho), _ptr(self.curv),
+                                                     _ptr(self.active)), "commit_valid")
 
     def _iter_second(self):
         """z = M^-1 r with <r, z>; d = z + beta d (no restart)."""
+        lib, t = self.lib, self.t
+        if self.fused:
+            m1, m2 = self.samples
+            self._chk(lib.fbme_lowrank_apply_fft_dot(
+                self.stream, self.lanes, self.r, m1, m2, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
+                _ptr(self.lr_map), _ptr(self.res), _ptr(self.z), _ptr(self.lr_buf), _ptr(self.lr_part),
+                _ptr(self.corr), _ptr(t.tw[256]), _ptr(self.pc_part), None), "lowrank_apply_fft")
+            self._chk(lib.fbme_pcg_direction_parts(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),
+                                                   _ptr(self.rho), _ptr(self.curv), _ptr(self.pc_part),
+                                                   self.nparts, _ptr(self.nrho), _ptr(self.active)),
+                      "direction")
+            return
         self.precondition(self.res, self.z, dot=self.nrho)
-        self._chk(self.lib.fbme_pcg_direction(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),
-                                              _ptr(self.rho), _ptr(self.nrho), _ptr(self.active), 0),
-                  "direction")
+        self._chk(lib.fbme_pcg_direction(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),
+                                         _ptr(self.rho), _ptr(self.nrho), _ptr(self.active), 0), "direction")
 
     def _replay_block(self, ci: int):
         """ci - 1 full PCG iterations and the first half of the next, the
         work between two host convergence checks, as one hipGraph captured
         on first use (same kernels and buffers as the eager loop)."""
-        key = (ci, self.grid_op, self.lr_fft, tuple(self.tiles))
+        key = (ci, self.grid_op, self.lr_fft, self.fused, tuple(self.tiles))
         graph = self._graphs.get(key)
         if graph is None:
             # capture on a side stream (nothing is allocated inside, so the

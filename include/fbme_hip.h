@@ -229,6 +229,20 @@ int fbme_grid_matvec(void* stream, int lanes, int rhs, int m1, int m2, const flo
  * lanes * ceil(M2 / 16) doubles. */
 int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,
                          const void* d, void* out, void* mid, const void* tw256, double* part, double* dot);
+/* PCG update and direction that take their inner product as the per-block
+ * partials of the producing pass (fbme_grid_matvec_dot /
+ * fbme_lowrank_apply_fft_dot with part but dot = NULL: no reduction launch)
+ * and fold the validity commit into the rho commit; same lane logic as
+ * fbme_pcg_update + fbme_pcg_direction (solver.py:5645-5719).
+ * fbme_pcg_update_parts stores curv[lane]; fbme_pcg_direction_parts stores
+ * nrho[lane] and commits rho / active (restart = 0).  fbme_pcg_commit_valid
+ * is the validity commit alone, for a convergence check after
+ * fbme_pcg_update_parts. */
+int fbme_pcg_update_parts(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                          const double* rho, const double* part, int nparts, double* curv, const int* active);
+int fbme_pcg_direction_parts(void* stream, int lanes, int m, void* d, const void* z, double* rho,
+                             const double* curv, const double* part, int nparts, double* nrho, int* active);
+int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const double* curv, int* active);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
  * index (the stable argsort of solver.py:5797-5800).  cand: batch *
  * ceil(n / 1024) * 32 uint64; requires n <= 65536. */
