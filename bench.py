@@ -426,14 +426,15 @@ def main():
 
     t_u = native.max_iters
     line = {
-        "metric": "scenes/s at 256x256 (160x160 prefix), batched FBME-"
-                  f"{args.profile.capitalize()} plan",
+        "metric": "scenes/s at 256² at fixed NMSE, batched+learned vs plain FBME; % HBM peak",
         "value": value, "unit": "scenes/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "c64/f32 (f64 polish)", "data": "synthetic",
-        "config": {"workload": "FBME-%s plan, %d scenes/step, grid 256x256, "
-                               "prefix 160x160, K=300 ellipse scatterers, 10 dB"
+        "config": {"workload": "batched FBME-%s plan (value), %d scenes/step, grid 256x256, "
+                               "prefix 160x160, K=300 ellipse scatterers, 10 dB; plain single-scene "
+                               "plan in plain_single_scene; learned warm start of the general solver "
+                               "against the plain solver in learned_warm_start"
                                % (args.profile.capitalize(), b),
                    "batch": b, "grid": list(grid), "prefix": list(samples),
                    "polish_size": 32, "uamp_iterations": t_u},

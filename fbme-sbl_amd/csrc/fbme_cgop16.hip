@@ -142,34 +142,55 @@ __global__ void __launch_bounds__(64) k_gop_finish(const double* __restrict__ pa
 //   (F c)[m1, m2] = sum_s e^{-2 pi i m1 row(s) / 256} T[s, m2],
 //     T[s, m2] = sum_{k in s} V[m2, k] c[k]       (slot rows formed in LDS, then their column FFT).
 // map[b] = slot of each atom [32], row of each slot [32], slot count [1],
-// slot of each grid row (or -1) [256].
+// slot of each grid row (or -1) [256], atoms ordered by (slot, index) [32],
+// first position of each slot in that order [33].
 constexpr int RANK = 32;
-constexpr int MAP_INTS = 32 + 32 + 1 + 256;
+constexpr int MAP_INTS = 32 + 32 + 1 + 256 + 32 + 33;
+constexpr int MAP_PERM = 321, MAP_START = 353;
 
+// One wavefront per scene, lane k = atom k: slots are the distinct rows in
+// order of first occurrence among the atoms (ballots instead of a serial
+// scan); the atom order within a slot is increasing k.
 __global__ void __launch_bounds__(64)
 k_lrf_map(const int* __restrict__ idx, int n2, int* __restrict__ map) {
   const int b = blockIdx.x;
+  const int k = threadIdx.x;
   int* m = map + (long long)b * MAP_INTS;
-  for (int r = threadIdx.x; r < 256; r += 64) m[65 + r] = -1;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int rows[RANK];
-    int ns = 0;
-    for (int k = 0; k < RANK; ++k) {
-      const int r = idx[b * RANK + k] / n2;
-      int sl = -1;
-      for (int q = 0; q < ns; ++q)
-        if (rows[q] == r) sl = q;
-      if (sl < 0) {
-        sl = ns;
-        rows[ns++] = r;
-      }
-      m[k] = sl;
-    }
-    for (int q = 0; q < RANK; ++q) m[32 + q] = q < ns ? rows[q] : -1;
-    m[64] = ns;
-    for (int q = 0; q < ns; ++q) m[65 + rows[q]] = q;
+  for (int r = k; r < 256; r += 64) m[65 + r] = -1;
+  const bool atom = k < RANK;
+  const int row = atom ? idx[b * RANK + k] / n2 : -1;
+  int first = k;
+  for (int j = RANK - 1; j >= 0; --j) {
+    const int rj = __shfl(row, j, 64);
+    if (atom && rj == row) first = j;
   }
+  const bool is_first = atom && first == k;
+  const unsigned long long firsts = __ballot(is_first);
+  const int slot_of_first = __popcll(firsts & ((1ull << k) - 1ull));
+  const int slot = __shfl(slot_of_first, atom ? first : 0, 64);   // slot of the first atom with this row
+  const int ns = __popcll(firsts);
+  // rank of atom k among the atoms of its slot, and the slot's first position
+  int before = 0, within = 0;
+  for (int j = 0; j < RANK; ++j) {
+    const int sj = __shfl(slot, j, 64);
+    if (atom) {
+      before += sj < slot;
+      within += (sj == slot) && (j < k);
+    }
+  }
+  __syncthreads();   // the row map is cleared before it is filled
+  if (atom) {
+    m[k] = slot;
+    m[MAP_PERM + before + within] = k;
+    if (is_first) {
+      m[32 + slot] = row;
+      m[65 + row] = slot;
+      m[MAP_START + slot] = before;
+    }
+    if (k >= ns) m[32 + k] = -1;
+  }
+  if (k >= ns && k <= RANK) m[MAP_START + k] = RANK;
+  if (k == 0) m[64] = ns;
 }
 
 // W[lane, s, c] for the slot rows of the column IFFT of v[lane, :, c].
@@ -254,7 +275,8 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
   __shared__ float2 tw[256];
   __shared__ double red[8];
   __shared__ int rowslot[256];
-  __shared__ int aslot[RANK];
+  __shared__ int perm[RANK];
+  __shared__ int start[RANK + 1];
   __shared__ float2 cs[RANK];
   __shared__ float2 vsm[GC][RANK];
   __shared__ float2 ts[RANK][GC];
@@ -267,9 +289,10 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
   const float2* V = tab + ((long long)b * (m1 + m2) + m1) * RANK;
   rowslot[threadIdx.x] = mb[65 + threadIdx.x];
   if (threadIdx.x < RANK) {
-    aslot[threadIdx.x] = mb[threadIdx.x];
+    perm[threadIdx.x] = mb[MAP_PERM + threadIdx.x];
     cs[threadIdx.x] = corr[(long long)lane * RANK + threadIdx.x];
   }
+  if (threadIdx.x <= RANK) start[threadIdx.x] = mb[MAP_START + threadIdx.x];
   for (int e = threadIdx.x; e < GC * RANK; e += 16 * GC) {
     const int cc = e / RANK, k = e - cc * RANK;
     const int c2 = blockIdx.x * GC + cc;
@@ -282,8 +305,8 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
     const int sl = e / GC, cc = e - sl * GC;
     float ax = 0.f, ay = 0.f;
     if (sl < ns) {
-      for (int k = 0; k < RANK; ++k) {
-        if (aslot[k] != sl) continue;
+      for (int j = start[sl]; j < start[sl + 1]; ++j) {
+        const int k = perm[j];
         const float2 vv = vsm[cc][k], w = cs[k];
         ax += vv.x * w.x - vv.y * w.y;
         ay += vv.x * w.y + vv.y * w.x;

@@ -259,7 +259,7 @@ class _Solver:
         self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
         self.mv_part = torch.empty(self.lanes * ((m2 + 15) // 16), dtype=f64, device=dev)
         self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
-        self.lr_map = torch.empty((b, 321), dtype=torch.int32, device=dev)
+        self.lr_map = torch.empty((b, 386), dtype=torch.int32, device=dev)
         self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
         self.rho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.nrho = torch.empty(self.lanes, dtype=f64, device=dev)

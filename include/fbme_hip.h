@@ -275,9 +275,9 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
  * atoms' distinct rows, then 32 row sums; the correction is a row synthesis
  * on those rows, then one column FFT (same operator as fbme_lowrank_apply,
  * solver.py:5836-5848, with about a quarter of the flops).
- * fbme_lowrank_map builds map [batch, 321] ints from the atoms of
+ * fbme_lowrank_map builds map [batch, 386] ints from the atoms of
  * fbme_topk32 (slot of each atom, row of each slot, slot count, slot of each
- * grid row).  wbuf: lanes * 32 * M2 complex64; part: lanes * ceil(M2 / 16)
+ * grid row, atoms ordered by slot, first position of each slot).  wbuf: lanes * 32 * M2 complex64; part: lanes * ceil(M2 / 16)
  * * 32 double2;
  * dot_part: lanes * ceil(M2 / 16) doubles; tw256: fbme_twiddle_table(256). */
 int fbme_lowrank_map(void* stream, int batch, int n1, int n2, const int* idx, int* map);
