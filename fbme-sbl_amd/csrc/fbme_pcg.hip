@@ -615,15 +615,23 @@ __global__ void k_pcg_update_parts(float2* __restrict__ x, float2* __restrict__ 
                                    const double* __restrict__ part, int nparts, double* __restrict__ curv,
                                    const int* __restrict__ active, int m) {
   const int lanev = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = (long long)lanev * m + i;
+  // vector loads issued before the lane scalars are reduced (lanes already
+  // converged load nothing)
+  const bool act = active[lanev] != 0;
+  float2 dv = make_float2(0.f, 0.f), qv = dv, xv = dv, rv = dv;
+  if (i < m && act) {
+    dv = d[g];
+    qv = q[g];
+    xv = x[g];
+    rv = r[g];
+  }
   const double cv = lane_parts(part, nparts, lanev), rh = rho[lanev];
   if (blockIdx.x == 0 && threadIdx.x == 0) curv[lanev] = cv;
-  const bool valid = active[lanev] && pcg_valid(cv, rh);
+  const bool valid = act && pcg_valid(cv, rh);
   const float alpha = valid ? (float)(rh / fmax(cv, F32_TINY_D)) : 0.f;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m && alpha != 0.f) {
-    const long long g = (long long)lanev * m + i;
-    const float2 dv = d[g], qv = q[g];
-    float2 xv = x[g], rv = r[g];
     xv.x += alpha * dv.x;
     xv.y += alpha * dv.y;
     rv.x -= alpha * qv.x;
@@ -641,19 +649,22 @@ __global__ void k_pcg_direction_parts(float2* __restrict__ d, const float2* __re
                                       const double* __restrict__ part, int nparts, double* __restrict__ nrho,
                                       const int* __restrict__ active, int m) {
   const int lanev = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const long long g = (long long)lanev * m + i;
+  // vector loads issued before the lane scalars are reduced (lanes already
+  // converged load nothing)
+  const bool act = active[lanev] != 0;
+  float2 zv = make_float2(0.f, 0.f), dv = zv;
+  if (i < m && act) {
+    zv = z[g];
+    dv = d[g];
+  }
   const double nr = lane_parts(part, nparts, lanev), rh = rho[lanev];
   if (blockIdx.x == 0 && threadIdx.x == 0) nrho[lanev] = nr;
-  const bool vu = active[lanev] && pcg_valid(curv[lanev], rh) && isfinite(nr) && nr > F32_TINY_D;
+  const bool vu = act && pcg_valid(curv[lanev], rh) && isfinite(nr) && nr > F32_TINY_D;
   const float beta = vu ? (float)(nr / fmax(rh, F32_TINY_D)) : 0.f;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  const long long g = (long long)lanev * m + i;
-  if (vu) {
-    const float2 zv = z[g], dv = d[g];
-    d[g] = make_float2(zv.x + beta * dv.x, zv.y + beta * dv.y);
-  } else {
-    d[g] = make_float2(0.f, 0.f);
-  }
+  d[g] = vu ? make_float2(zv.x + beta * dv.x, zv.y + beta * dv.y) : make_float2(0.f, 0.f);
 }
 
 __global__ void k_pcg_commit_parts(double* __restrict__ rho, const double* __restrict__ curv,
