@@ -11,11 +11,17 @@ TEST INFRASTRUCTURE ONLY.  Follows torch_sbl/solver.py:
   final posterior 2023-2080, PCG 5582-5723, low-rank preconditioner
   5786-5848, reduced active posterior with diffuse-tail loading 3591-3779.
 The Toeplitz/BTTB matvec of operators.py:299-371 is evaluated as the
-mathematically identical sigma^2 d + A (gamma . A^H d).
+mathematically identical sigma^2 d + A (gamma . A^H d).  An observation
+mask (solver.py:986-1024, operators.py:57-75 and 473-496) keeps the
+observed samples only: data, probes, residuals and the preconditioner's
+factor columns live on those rows, and every "M" of the loop is their
+count.
 
 Pins: tests/test_oracle_cg.py (PCG against a dense solve, Hutchinson
 unbiasedness against the exact diagonal, preconditioner against the dense
-Woodbury inverse, end-to-end against the reference's golden run).
+Woodbury inverse, the masked operator against the row-restricted dense
+matrix, end-to-end against the reference's golden runs with and without
+a mask).
 """
 
 from __future__ import annotations
@@ -104,23 +110,51 @@ def cg_schedule(it: int, cfg: CGConfig) -> float:
 
 
 class Operator:
-    """A, A^H and Q = noise I + A Gamma A^H on [B, M, R] blocks."""
+    """A, A^H and Q = noise I + A Gamma A^H on [B, M, R] blocks; with an
+    observation mask [M1, M2] the sample dimension holds the observed
+    samples only (restrict after A, expand before A^H;
+    operators.py:57-75, 473-496)."""
 
-    def __init__(self, grid, samples):
+    def __init__(self, grid, samples, mask=None):
         self.grid = grid
         self.samples = samples
         self.n = grid[0] * grid[1]
-        self.m = samples[0] * samples[1]
+        self.m_full = samples[0] * samples[1]
+        if mask is None:
+            self.idx = None
+            self.m = self.m_full
+        else:
+            mask = np.asarray(mask, bool)
+            if mask.shape != tuple(samples):
+                raise ValueError("observation_mask must match the sample shape")
+            self.idx = np.flatnonzero(mask.reshape(-1))
+            if self.idx.size == 0:
+                raise ValueError("observation_mask must retain at least one sample")
+            self.m = int(self.idx.size)
+
+    def restrict(self, full):   # [B, M_full, ...] -> [B, M, ...]
+        return full if self.idx is None else full[:, self.idx]
+
+    def expand(self, part):     # [B, M, ...] -> [B, M_full, ...]
+        if self.idx is None:
+            return part
+        out = np.zeros((part.shape[0], self.m_full) + part.shape[2:], part.dtype)
+        out[:, self.idx] = part
+        return out
+
+    def columns(self, indices):   # A[:, S] on the observed rows -> [B, M, K]
+        return self.restrict(fourier_columns(indices, self.grid, self.samples))
 
     def fwd(self, x):      # [B, N, R] -> [B, M, R]
         b, _, r = x.shape
         flat = np.moveaxis(x, 2, 1).reshape(b * r, self.n)
         out = prefix_forward(flat, self.grid, self.samples)
-        return np.moveaxis(out.reshape(b, r, self.m), 1, 2)
+        return self.restrict(np.moveaxis(out.reshape(b, r, self.m_full), 1, 2))
 
     def adj(self, v):      # [B, M, R] -> [B, N, R]
+        v = self.expand(v)
         b, _, r = v.shape
-        flat = np.moveaxis(v, 2, 1).reshape(b * r, self.m)
+        flat = np.moveaxis(v, 2, 1).reshape(b * r, self.m_full)
         out = prefix_adjoint(flat, self.grid, self.samples)
         return np.moveaxis(out.reshape(b, r, self.n), 1, 2)
 
@@ -133,7 +167,7 @@ def low_rank_preconditioner(gamma, noise, op: Operator, rank, eps, tiny):
     rank = min(rank, op.m, op.n)
     order = np.argsort(-gamma, axis=1, kind="stable")[:, :rank]
     vals = np.take_along_axis(gamma, order, 1)
-    cols = fourier_columns(order, op.grid, op.samples)
+    cols = op.columns(order)
     factors = cols * np.sqrt(vals)[:, None, :]
     total = gamma.sum(1)
     omitted = np.maximum(total - vals.sum(1), 0.0)
@@ -250,8 +284,12 @@ def active_posterior(y, gamma, noise, sup, op: Operator, eps, tiny):
     gs = gamma[rows, sup]
     omitted = np.maximum(gamma.sum(1) - gs.sum(1), 0.0)
     nu = noise + omitted
-    gram = gram_closed_form(sup, op.grid, op.samples)
-    cdat = prefix_adjoint(y, op.grid, op.samples)[rows, sup]
+    if op.idx is None:
+        gram = gram_closed_form(sup, op.grid, op.samples)
+    else:
+        cols = op.columns(sup)
+        gram = np.einsum("bmk,bml->bkl", np.conj(cols), cols)
+    cdat = op.adj(y[:, :, None])[:, :, 0][rows, sup]
     mu, sig, dd, _ = reduced_posterior(gram, cdat, np.sum(np.abs(y) ** 2, 1),
                                        gs, nu, op.m, eps=eps, tiny=tiny,
                                        jitter_multiplier=1.0)
@@ -265,8 +303,9 @@ def active_posterior(y, gamma, noise, sup, op: Operator, eps, tiny):
 
 
 def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
-           tiny=F32_TINY, trace=None) -> dict:
-    """Default ``sbl_2d`` in mode='cg' on [B, M1, M2] (or [M1, M2]) data."""
+           tiny=F32_TINY, trace=None, mask=None) -> dict:
+    """Default ``sbl_2d`` in mode='cg' on [B, M1, M2] (or [M1, M2]) data,
+    optionally on the samples an observation mask [M1, M2] keeps."""
     cfg = cfg or CGConfig()
     data = np.asarray(data)
     single = data.ndim == 2
@@ -275,12 +314,12 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     b = data.shape[0]
     samples = tuple(data.shape[-2:])
     grid = tuple(int(v) for v in grid)
-    op = Operator(grid, samples)
+    op = Operator(grid, samples, mask)
     n, m = op.n, op.m
-    y = data.reshape(b, m).astype(np.complex128)
+    y = op.restrict(data.reshape(b, op.m_full).astype(np.complex128))
     power = np.maximum(np.mean(np.abs(y) ** 2, 1), eps)
     floor = np.maximum(power * cfg.min_variance, tiny)
-    bp = prefix_adjoint(y, grid, samples)
+    bp = op.adj(y[:, :, None])[:, :, 0]
     energy = np.abs(bp) ** 2
     structured = energy / np.maximum(energy.sum(1, keepdims=True), eps) * power[:, None]
     gamma = np.maximum(cfg.initialization_blend * structured
@@ -333,7 +372,7 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             mask = np.zeros_like(gamma, bool)
             mask[np.arange(b)[:, None], sup] = True
             new_gamma = np.where(mask, new_gamma, gamma)
-        resid = y - prefix_forward(mean, grid, samples)
+        resid = y - op.fwd(mean[:, :, None])[:, :, 0]
         e = np.sum(np.abs(resid) ** 2, 1)
         if cfg.noise_update == "em":
             new_noise = (e + noise * dof + cfg.noise_rate) / (m + cfg.noise_shape)

@@ -14,6 +14,8 @@ What it pins (functions no closed-form check fixes end to end):
   * ``sbl_2d`` with the default ``SBLConfig`` on CPU (mode "auto" -> "cg",
     solver.py:1025-1036) on the 32x32 / 24x24 workload.
   * ``dense_em_sbl`` in complex128 on the same workload.
+  * ``sbl_2d`` in mode "cg" with an ``observation_mask`` (a single 32x32 /
+    24x24 scene and a batch of two 64x64 / 40x40 scenes).
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -145,10 +147,60 @@ def long_cg(reference_root: str) -> None:
     print("wrote general_64x64_40x40_b4")
 
 
+def observation_mask(seed: int, samples, keep: float) -> np.ndarray:
+    """A seeded random observation mask over the sample block."""
+    rng = np.random.default_rng(seed)
+    return rng.random(samples) < keep
+
+
+def masked_cg(reference_root: str) -> None:
+    """``sbl_2d(mode='cg', observation_mask=...)`` on CPU: a 32x32 / 24x24
+    scene with 70% of its samples observed (default configuration), and a
+    batch of two 64x64 / 40x40 ellipse scenes with 60% observed (16 probes,
+    tolerance 1e-4, 30 iterations)."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    scene = scenes.uniform_scene(1010, (32, 32), (24, 24), k=12, snr_db=20.0)
+    mask = observation_mask(1010, (24, 24), 0.7)
+    y = np.where(mask, scene.y, 0).astype(np.complex64)
+    res = sbl_2d(torch.from_numpy(y), (32, 32), config=SBLConfig(device="cpu", mode="cg"),
+                 observation_mask=torch.from_numpy(mask))
+    arrays = dict(y=y, mask=mask, x_true=scene.x, grid=np.array((32, 32)),
+                  samples=np.array((24, 24)))
+    arrays.update(_result_arrays(res, "cg_"))
+    for key in ("relative_change", "cg_iterations", "noise_variance"):
+        arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+    np.savez_compressed(os.path.join(HERE, "masked_32x32_24x24.npz"), **arrays)
+    print("wrote masked_32x32_24x24")
+
+    batch = [scenes.ellipse_scene(1020 + i, (64, 64), (40, 40), k=30, snr_db=15.0)
+             for i in range(2)]
+    mask = observation_mask(1020, (40, 40), 0.6)
+    y = np.stack([np.where(mask, s.y, 0) for s in batch]).astype(np.complex64)
+    cfg = SBLConfig(device="cpu", mode="cg", hutchinson_probes=16,
+                    auto_probe_scaling=False, cg_tolerance=1e-4, max_iter=30)
+    res = sbl_2d(torch.from_numpy(y), (64, 64), config=cfg,
+                 observation_mask=torch.from_numpy(mask))
+    arrays = dict(y=y, mask=mask, x_true=np.stack([s.x for s in batch]),
+                  grid=np.array((64, 64)), samples=np.array((40, 40)))
+    arrays.update(_result_arrays(res, "cg_"))
+    for key in ("relative_change", "cg_iterations", "noise_variance"):
+        arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+    np.savez_compressed(os.path.join(HERE, "masked_64x64_40x40_b2.npz"), **arrays)
+    print("wrote masked_64x64_40x40_b2")
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
-    if len(sys.argv) > 2 and sys.argv[2] == "long_cg":
+    which = sys.argv[2] if len(sys.argv) > 2 else "all"
+    if which == "long_cg":
         long_cg(root)
+    elif which == "masked":
+        masked_cg(root)
     else:
         main(root)
         long_cg(root)
+        masked_cg(root)

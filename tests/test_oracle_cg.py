@@ -129,3 +129,55 @@ def test_sbl_cg_matches_reference_golden_run():
     assert out["noise_variance"] == pytest.approx(float(g["cg_noise_variance"]), rel=1e-3)
     assert out["effective_dof"] == pytest.approx(float(g["cg_effective_dof"]), rel=1e-3)
     np.testing.assert_allclose(out["cg_iterations"][:29], g["cg_history_cg_iterations"][:29], atol=8)
+
+
+def test_masked_operator_is_dense_row_restriction():
+    """With a mask, Q acts on the observed rows of A only."""
+    rng, gamma, noise, a, _ = _problem(6)
+    mask = rng.random((7, 6)) < 0.6
+    op = co.Operator((12, 10), (7, 6), mask)
+    ar = a[mask.reshape(-1)]
+    assert op.m == int(mask.sum())
+    d = rng.standard_normal((2, op.m, 2)) + 1j * rng.standard_normal((2, op.m, 2))
+    got = op.q(d, gamma, noise)
+    for b in range(2):
+        q = noise[b] * np.eye(op.m) + (ar * gamma[b]) @ ar.conj().T
+        np.testing.assert_allclose(got[b], q @ d[b], rtol=1e-12, atol=1e-12)
+    bg, factors, inv = co.low_rank_preconditioner(gamma, noise, op, 8, 0.0, F32_TINY)
+    order = np.argsort(-gamma[0], kind="stable")[:8]
+    np.testing.assert_allclose(factors[0], ar[:, order] * np.sqrt(gamma[0, order]), atol=1e-12)
+
+
+def test_full_mask_is_unmasked_solver():
+    g = np.load(golden_path("general_32x32_24x24.npz"))
+    cfg = co.CGConfig(max_iter=6)
+    a = co.sbl_cg(g["y"], (32, 32), cfg)
+    b = co.sbl_cg(g["y"], (32, 32), cfg, mask=np.ones((24, 24), bool))
+    np.testing.assert_allclose(b["x"], a["x"], rtol=1e-12, atol=1e-14)
+    assert b["noise_variance"] == pytest.approx(a["noise_variance"], rel=1e-12)
+
+
+def test_mask_validation():
+    with pytest.raises(ValueError):
+        co.Operator((8, 8), (4, 4), np.ones((4, 5), bool))
+    with pytest.raises(ValueError):
+        co.Operator((8, 8), (4, 4), np.zeros((4, 4), bool))
+
+
+@pytest.mark.parametrize("name,grid,kw", [
+    ("masked_32x32_24x24", (32, 32), {}),
+    ("masked_64x64_40x40_b2", (64, 64), dict(hutchinson_probes=16, auto_probe_scaling=False,
+                                             cg_tolerance=1e-4, max_iter=30)),
+])
+def test_masked_sbl_cg_matches_reference_golden_run(name, grid, kw):
+    """sbl_2d(mode='cg', observation_mask=mask) of the reference
+    (solver.py:986-1024, operators.py:57-75) on 67% / 58% observed samples."""
+    g = np.load(golden_path(name + ".npz"))
+    out = co.sbl_cg(g["y"], grid, co.CGConfig(**kw), mask=g["mask"])
+    assert out["iterations"] == int(g["cg_iterations"])
+    assert out["active_set_size"] == int(g["cg_active_set_size"])
+    ref = g["cg_x"].reshape(out["x"].shape)
+    assert np.linalg.norm(out["x"] - ref) / np.linalg.norm(ref) < 1e-4
+    np.testing.assert_allclose(out["noise_variance"], g["cg_noise_variance"], rtol=1e-3)
+    np.testing.assert_allclose(out["effective_dof"], g["cg_effective_dof"], rtol=1e-3)
+    np.testing.assert_allclose(out["cg_iterations"], g["cg_history_cg_iterations"], atol=8)
