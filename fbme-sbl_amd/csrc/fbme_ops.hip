@@ -186,12 +186,12 @@ __global__ void k_prior(const float2* __restrict__ mean, const float* __restrict
 
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_residual_partial(const float2* __restrict__ data, const float2* __restrict__ pred, int m,
-                   double* __restrict__ partial) {
+                   double* __restrict__ partial, const unsigned char* __restrict__ mask) {
   __shared__ double red[8];
   const int b = blockIdx.y;
   const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
   double acc = 0.0;
-  if (i < m) {
+  if (i < m && (!mask || mask[i])) {
     const long long gi = (long long)b * m + i;
     const float rx = data[gi].x - pred[gi].x, ry = data[gi].y - pred[gi].y;
     acc = (double)rx * rx + (double)ry * ry;
@@ -326,7 +326,7 @@ int fbme_complex_residual_energy(void* stream, int batch, int m, const void* dat
   hipStream_t st = (hipStream_t)stream;
   const int nb = nblk(m);
   hipLaunchKernelGGL(k_residual_partial, dim3(nb, batch), dim3(FBME_BLOCK), 0, st, (const float2*)data,
-                     (const float2*)prediction, m, partial);
+                     (const float2*)prediction, m, partial, (const unsigned char*)nullptr);
   hipLaunchKernelGGL(k_residual_finish, dim3(batch), dim3(64), 0, st, (const double*)partial, nb, output,
                      (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, m, 0.f, 0.f, 0.f, 0,
                      0.f, 0);
@@ -337,13 +337,24 @@ int fbme_uamp_noise_update(void* stream, int batch, int m, const void* data, con
                            const float* old_noise, const float* effective_dof, const float* variance_floor,
                            float* output, float noise_shape, float noise_rate, float damping, int evidence_update,
                            float epsilon, double* partial) {
-  if (batch < 1 || m < 1 || !partial) return FBME_ERR_ARG;
+  return fbme_uamp_noise_update_masked(stream, batch, m, nullptr, m, data, prediction, old_noise, effective_dof,
+                                       variance_floor, output, noise_shape, noise_rate, damping, evidence_update,
+                                       epsilon, partial);
+}
+
+int fbme_uamp_noise_update_masked(void* stream, int batch, int m, const unsigned char* mask, int observed,
+                                  const void* data, const void* prediction, const float* old_noise,
+                                  const float* effective_dof, const float* variance_floor, float* output,
+                                  float noise_shape, float noise_rate, float damping, int evidence_update,
+                                  float epsilon, double* partial) {
+  if (batch < 1 || m < 1 || observed < 1 || observed > m || !partial) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int nb = nblk(m);
   hipLaunchKernelGGL(k_residual_partial, dim3(nb, batch), dim3(FBME_BLOCK), 0, st, (const float2*)data,
-                     (const float2*)prediction, m, partial);
+                     (const float2*)prediction, m, partial, mask);
   hipLaunchKernelGGL(k_residual_finish, dim3(batch), dim3(64), 0, st, (const double*)partial, nb, output, old_noise,
-                     effective_dof, variance_floor, m, noise_shape, noise_rate, damping, evidence_update, epsilon, 1);
+                     effective_dof, variance_floor, observed, noise_shape, noise_rate, damping, evidence_update,
+                     epsilon, 1);
   return (int)hipGetLastError();
 }
 

@@ -193,12 +193,15 @@ __global__ void k_lr_tables(const int* __restrict__ idx, const float* __restrict
 }
 
 // One wavefront per scene: small = I + F^H F / bg with F^H F from the
-// Dirichlet Gram, GPU Cholesky loading (10 eps mean|diag|,
-// solver.py:5827-5833), inverse S^-1 = L^-H L^-1 (solver.py:5848).
+// Dirichlet Gram (or, under an observation mask, from the mask spectrum mh:
+// the Gram of the factor columns on the observed rows), GPU Cholesky loading
+// (10 eps mean|diag|, solver.py:5827-5833), inverse S^-1 = L^-H L^-1
+// (solver.py:5848).
 __global__ void __launch_bounds__(64)
 k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const double* __restrict__ sums,
            const float* __restrict__ noise, const double2* __restrict__ d1, const double2* __restrict__ d2,
-           int n1, int n2, float2* __restrict__ sinv, float* __restrict__ bg_out, int* __restrict__ status) {
+           const double2* __restrict__ mh, int n1, int n2, float2* __restrict__ sinv, float* __restrict__ bg_out,
+           int* __restrict__ status) {
   __shared__ double2 a[RANK][RANK + 1];
   __shared__ double2 xs[RANK][RANK + 1];
   __shared__ double sq[RANK];
@@ -222,8 +225,13 @@ k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const dou
     int dd2 = gi[r] % n2 - gi[q] % n2;
     dd1 += dd1 < 0 ? n1 : 0;
     dd2 += dd2 < 0 ? n2 : 0;
-    const double2 u = d1[dd1], w = d2[dd2];
-    const double2 gv = make_double2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x);
+    double2 gv;
+    if (mh) {
+      gv = mh[(long long)dd1 * n2 + dd2];
+    } else {
+      const double2 u = d1[dd1], w = d2[dd2];
+      gv = make_double2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x);
+    }
     const double s = sq[r] * sq[q] / bg;
     a[r][q] = make_double2((r == q ? 1.0 : 0.0) + gv.x * s, gv.y * s);
   }
@@ -520,6 +528,69 @@ __global__ void __launch_bounds__(64) k_lane_finish(const double* __restrict__ p
   for (int i = threadIdx.x; i < blocks; i += 64) s += part[(long long)l * blocks + i];
   s = wave_sum(s);
   if (threadIdx.x == 0) out[l] = s;
+}
+
+// ---------------------------------------------------------- observation masks
+// Vectors keep all M1 M2 samples with zeros on the unobserved ones; the
+// operator restricted to the observed rows (operators.py:57-75, 473-496) is
+// the full one followed by the mask.
+
+// out[lane, i] *= mask[i]; with part, also the block partials of
+// Re <a, out> for k_lane_finish.
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_mask_lanes(const unsigned char* __restrict__ mask, const float2* __restrict__ a, float2* __restrict__ out, int m,
+             double* __restrict__ part) {
+  __shared__ double red[8];
+  const int lanev = blockIdx.y;
+  double acc = 0.0;
+  for (int i = blockIdx.x * FBME_BLOCK + threadIdx.x; i < m; i += gridDim.x * FBME_BLOCK) {
+    const long long g = (long long)lanev * m + i;
+    if (!mask[i]) {
+      out[g] = make_float2(0.f, 0.f);
+    } else if (part) {
+      const float2 u = a[g], w = out[g];
+      acc += (double)u.x * w.x + (double)u.y * w.y;
+    }
+  }
+  if (!part) return;
+  acc = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long long)lanev * gridDim.x + blockIdx.x] = acc;
+}
+
+// t[m1, d2] = sum over observed m2 of exp(+2 pi i m2 d2 / N2), FP64, the
+// phase reduced modulo N2 exactly in integers.  One thread per (m1, d2).
+__global__ void k_mask_rows(const unsigned char* __restrict__ mask, int m1, int m2, int n2, double2* __restrict__ t) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)m1 * n2) return;
+  const int r = (int)(e / n2), d2 = (int)(e - (long long)r * n2);
+  double sx = 0.0, sy = 0.0;
+  for (int j = 0; j < m2; ++j) {
+    if (!mask[(long long)r * m2 + j]) continue;
+    const long long ph = ((long long)j * d2) % n2;
+    double sn, cs;
+    sincospi(2.0 * (double)ph / (double)n2, &sn, &cs);
+    sx += cs;
+    sy += sn;
+  }
+  t[e] = make_double2(sx, sy);
+}
+
+// s[d1, d2] = sum_m1 exp(+2 pi i m1 d1 / N1) t[m1, d2].  One thread per
+// (d1, d2); consecutive threads read consecutive d2.
+__global__ void k_mask_cols(const double2* __restrict__ t, int m1, int n1, int n2, double2* __restrict__ s) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)n1 * n2) return;
+  const int d1 = (int)(e / n2), d2 = (int)(e - (long long)d1 * n2);
+  double sx = 0.0, sy = 0.0;
+  for (int r = 0; r < m1; ++r) {
+    const long long ph = ((long long)r * d1) % n1;
+    double sn, cs;
+    sincospi(2.0 * (double)ph / (double)n1, &sn, &cs);
+    const double2 v = t[(long long)r * n2 + d2];
+    sx += cs * v.x - sn * v.y;
+    sy += cs * v.y + sn * v.x;
+  }
+  s[e] = make_double2(sx, sy);
 }
 
 // x += alpha d, r -= alpha q with alpha from (rho, curv, active)
@@ -1007,11 +1078,21 @@ int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand
 int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx, const float* val,
                        const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,
                        void* factors, void* sinv, float* bg, int* status) {
+  return fbme_lowrank_setup_masked(stream, batch, n1, n2, m1, m2, idx, val, gamma_sum, noise, dir1, dir2, nullptr,
+                                   factors, sinv, bg, status);
+}
+
+int fbme_lowrank_setup_masked(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx,
+                              const float* val, const double* gamma_sum, const float* noise, const void* dir1,
+                              const void* dir2, const void* mask_spectrum, void* factors, void* sinv, float* bg,
+                              int* status) {
+  if (batch < 1 || n1 < 1 || n2 < 1 || m1 < 1 || m2 < 1) return FBME_ERR_ARG;
+  if (!mask_spectrum && (!dir1 || !dir2)) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_lr_tables, dim3(nblk((long long)(m1 + m2) * RANK), batch), dim3(FBME_BLOCK), 0, st, idx, val,
                      (float2*)factors, n1, n2, m1, m2);
   hipLaunchKernelGGL(k_lr_setup, dim3(batch), dim3(64), 0, st, idx, val, gamma_sum, noise, (const double2*)dir1,
-                     (const double2*)dir2, n1, n2, (float2*)sinv, bg, status);
+                     (const double2*)dir2, (const double2*)mask_spectrum, n1, n2, (float2*)sinv, bg, status);
   return (int)hipGetLastError();
 }
 
@@ -1062,6 +1143,30 @@ int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const v
                        const float* bg, const void* v, void* out, void* part, void* corr) {
   return fbme_lowrank_apply_dot(stream, lanes, rhs, m1, m2, factors, sinv, bg, v, out, part, corr, nullptr,
                                 nullptr);
+}
+
+int fbme_mask_spectrum(void* stream, int n1, int n2, int m1, int m2, const unsigned char* mask, void* spectrum,
+                       void* work) {
+  if (n1 < 1 || n2 < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2 || !mask || !spectrum || !work) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_mask_rows, dim3(nblk((long long)m1 * n2)), dim3(FBME_BLOCK), 0, st, mask, m1, m2, n2,
+                     (double2*)work);
+  hipLaunchKernelGGL(k_mask_cols, dim3(nblk((long long)n1 * n2)), dim3(FBME_BLOCK), 0, st, (const double2*)work, m1,
+                     n1, n2, (double2*)spectrum);
+  return (int)hipGetLastError();
+}
+
+int fbme_mask_lanes_dot(void* stream, int lanes, int m, const unsigned char* mask, const void* a, void* out,
+                        double* part, double* dot) {
+  if (lanes < 1 || m < 1 || !mask || !out) return FBME_ERR_ARG;
+  if (dot && (!part || !a)) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = std::min(64, nblk(m));
+  hipLaunchKernelGGL(k_mask_lanes, dim3(blocks, lanes), dim3(FBME_BLOCK), 0, st, mask, (const float2*)a,
+                     (float2*)out, m, dot ? part : (double*)nullptr);
+  if (dot)
+    hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)part, blocks, lanes, dot);
+  return (int)hipGetLastError();
 }
 
 int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out) {

@@ -87,20 +87,28 @@ __device__ __forceinline__ float key_score(uint32_t key) {
   return (key & 0x80000000u) ? __uint_as_float(key & 0x7FFFFFFFu) : __uint_as_float(~key);
 }
 
+// m_total counts the observed samples.  mh, when set, is the [N1, N2]
+// spectrum of an observation mask (the Gram entry of two atoms on the
+// observed rows is mh[lag1, lag2]); unset, the Gram is the product of the
+// Dirichlet tables d1, d2 of the full prefix block.
 struct SceneCtx {
   int n1, n2, m_total, k;
   double floor_v, ye;
   const double2* d1;
   const double2* d2;
+  const double2* mh;
   const float2* bp;
 };
 
-__device__ __forceinline__ double2 gram_entry(const SceneCtx& c, int a, int b) {
-  int d1 = (a / c.n2) - (b / c.n2);
-  int d2 = (a % c.n2) - (b % c.n2);
+__device__ __forceinline__ double2 lag_gram(const SceneCtx& c, int d1, int d2) {
   d1 += (d1 < 0) ? c.n1 : 0;
   d2 += (d2 < 0) ? c.n2 : 0;
+  if (c.mh) return c.mh[(long long)d1 * c.n2 + d2];
   return zmul(c.d1[d1], c.d2[d2]);
+}
+
+__device__ __forceinline__ double2 gram_entry(const SceneCtx& c, int a, int b) {
+  return lag_gram(c, (a / c.n2) - (b / c.n2), (a % c.n2) - (b % c.n2));
 }
 
 // LDS workspace of one polish workgroup.
@@ -135,11 +143,7 @@ __device__ __forceinline__ void support_coords(const SceneCtx& c, const int* w, 
 
 // G[r][q] from the Dirichlet tables and the support coordinates.
 __device__ __forceinline__ double2 gram_rq(const SceneCtx& c, const PolishLds& L, int r, int q) {
-  int d1 = L.wk1[r] - L.wk1[q];
-  int d2 = L.wk2[r] - L.wk2[q];
-  d1 += (d1 < 0) ? c.n1 : 0;
-  d2 += (d2 < 0) ? c.n2 : 0;
-  return zmul(c.d1[d1], c.d2[d2]);
+  return lag_gram(c, L.wk1[r] - L.wk1[q], L.wk2[r] - L.wk2[q]);
 }
 
 // Exact reduced posterior and negative log evidence of the zero-tail model
@@ -432,6 +436,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   c.k = s.k;
   c.floor_v = scs[SC_FLOOR];
   c.ye = scs[SC_YE];
+  c.mh = nullptr;
   // Dirichlet tables staged in LDS (the Gram gathers hit them K^2 times per
   // evaluation); grids wider than the staging area read them from global.
   if (s.n1 + s.n2 <= DIR_LDS) {
@@ -940,10 +945,11 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
 // 3897-3924).  One wavefront per scene; outputs over the full grid:
 // mean (0 off S), variance (gamma off S) and data diagonal (0 off S).
 __global__ void __launch_bounds__(64)
-k_reduced_posterior(int k, int n1, int n2, int m1, int m2, const int* __restrict__ support,
+k_reduced_posterior(int k, int n1, int n2, int observed, const int* __restrict__ support,
                     const float* __restrict__ gamma, const double* __restrict__ gamma_sum,
                     const float* __restrict__ noise, const float2* __restrict__ bp, const double* __restrict__ ye,
-                    const double2* __restrict__ d1, const double2* __restrict__ d2, float jitter_multiplier,
+                    const double2* __restrict__ d1, const double2* __restrict__ d2,
+                    const double2* __restrict__ mh, float jitter_multiplier,
                     float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd,
                     int* __restrict__ status) {
   __shared__ PolishLds L;
@@ -966,12 +972,13 @@ k_reduced_posterior(int k, int n1, int n2, int m1, int m2, const int* __restrict
   SceneCtx c;
   c.n1 = n1;
   c.n2 = n2;
-  c.m_total = m1 * m2;
+  c.m_total = observed;
   c.k = k;
   c.floor_v = 0.0;
   c.ye = ye[b];
   c.d1 = d1;
   c.d2 = d2;
+  c.mh = mh;
   c.bp = bp + b * n;
   const double nle = evaluate(c, w, wg, nu, L, jitter_multiplier);
   if (tid < k) {
@@ -1003,14 +1010,23 @@ int fbme_reduced_posterior(void* stream, int batch, int k, int n1, int n2, int m
                            const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
                            const double* ye, const void* dir1, const void* dir2, float jitter_multiplier,
                            void* mean, float* var, float* dd, int* status) {
-  if (batch < 1 || k < 1 || k > FBME_KMAX) return FBME_ERR_ARG;
+  return fbme_reduced_posterior_masked(stream, batch, k, n1, n2, m1 * m2, support, gamma, gamma_sum, noise, bp, ye,
+                                       dir1, dir2, nullptr, jitter_multiplier, mean, var, dd, status);
+}
+
+int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2, int observed, const int* support,
+                                  const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                                  const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
+                                  float jitter_multiplier, void* mean, float* var, float* dd, int* status) {
+  if (batch < 1 || k < 1 || k > FBME_KMAX || observed < 1) return FBME_ERR_ARG;
+  if (!mask_spectrum && (!dir1 || !dir2)) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const long long total = (long long)batch * n1 * n2;
   hipLaunchKernelGGL(k_reduced_defaults, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gamma,
                      (float2*)mean, var, dd, total);
-  hipLaunchKernelGGL(k_reduced_posterior, dim3(batch), dim3(64), 0, st, k, n1, n2, m1, m2, support, gamma,
+  hipLaunchKernelGGL(k_reduced_posterior, dim3(batch), dim3(64), 0, st, k, n1, n2, observed, support, gamma,
                      gamma_sum, noise, (const float2*)bp, ye, (const double2*)dir1, (const double2*)dir2,
-                     jitter_multiplier, (float2*)mean, var, dd, status);
+                     (const double2*)mask_spectrum, jitter_multiplier, (float2*)mean, var, dd, status);
   return (int)hipGetLastError();
 }
 size_t fbme_polish_result_bytes(void) { return sizeof(PolishResult); }

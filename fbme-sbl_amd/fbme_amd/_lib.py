@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 Q_WORKSPACE_BYTES = 0
 Q_TOTAL_GRID = 1
@@ -205,6 +205,29 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p,
                                        c_float, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
+    # observation masks of the general solver
+    "fbme_mask_spectrum": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                   c_void_p, c_void_p, c_void_p]),
+    "fbme_mask_lanes_dot": (c_int, [c_void_p, c_int, c_int, c_void_p,
+                                    c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_lowrank_setup_masked": (c_int, [c_void_p, c_int, c_int, c_int,
+                                          c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p]),
+    "fbme_reduced_posterior_masked": (c_int, [c_void_p, c_int, c_int, c_int,
+                                              c_int, c_int, c_void_p,
+                                              c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_float,
+                                              c_void_p, c_void_p, c_void_p,
+                                              c_void_p]),
+    "fbme_uamp_noise_update_masked": (c_int, [c_void_p, c_int, c_int,
+                                              c_void_p, c_int, c_void_p,
+                                              c_void_p, c_void_p, c_void_p,
+                                              c_void_p, c_void_p, c_float,
+                                              c_float, c_float, c_int,
+                                              c_float, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

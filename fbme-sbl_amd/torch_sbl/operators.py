@@ -4,7 +4,9 @@ torch_sbl/operators.py:23-501 for the methods the FBME hot path uses).
 ``forward_flat`` / ``adjoint_flat`` and the ``[B, N, R]`` forms run the
 pruned two-pass FFT kernels of libfbme_hip.so (fbme_prefix_forward /
 fbme_prefix_adjoint); the DFT is unnormalised, A = fft2(.)[:M1, :M2] and
-A^H = N ifft2(zero-pad(.)).
+A^H = N ifft2(zero-pad(.)).  With an observation mask the operator keeps
+the observed rows (operators.py:57-75, 473-496): forward gathers them
+from the full prefix block, adjoint scatters them into zeros first.
 """
 
 from __future__ import annotations
@@ -47,8 +49,6 @@ class PrefixFourierOperator:
             raise ValueError("Only 1-D and 2-D Fourier grids are supported.")
         if any(m <= 0 or m > n for m, n in zip(samples, grid)):
             raise ValueError("Each sample dimension must satisfy 0 < M <= N.")
-        if observation_mask is not None:
-            raise NotImplementedError("observation masks are not supported yet.")
         if complex_dtype != torch.complex64:
             raise NotImplementedError("the HIP operator is complex64 only.")
         self.grid_shape = grid
@@ -59,7 +59,18 @@ class PrefixFourierOperator:
         self.n_grid = prod(grid)
         self.n_samples = prod(samples)
         self.full_n_samples = self.n_samples
+        self.observation_mask = None
         self.observed_indices = None
+        if observation_mask is not None:
+            mask = torch.as_tensor(observation_mask, dtype=torch.bool).to(self.device)
+            if tuple(mask.shape) != samples:
+                raise ValueError(f"observation_mask must have shape {samples}, got {tuple(mask.shape)}.")
+            indices = mask.reshape(-1).nonzero(as_tuple=False).squeeze(1)
+            if indices.numel() == 0:
+                raise ValueError("observation_mask must retain at least one sample.")
+            self.observation_mask = mask
+            self.observed_indices = indices
+            self.n_samples = int(indices.numel())
         self.embedding_shape = tuple(_next_smooth_length(2 * m - 1)
                                      for m in samples)
         self._mid: Tensor | None = None
@@ -71,7 +82,23 @@ class PrefixFourierOperator:
 
     @property
     def is_complete(self) -> bool:
-        return True
+        return self.observed_indices is None
+
+    def restrict(self, full_data: Tensor) -> Tensor:
+        """[B, M1 M2, ...] -> [B, M_observed, ...] (operators.py:473-482)."""
+        if self.observed_indices is None:
+            return full_data
+        return full_data.index_select(1, self.observed_indices)
+
+    def expand(self, observed_data: Tensor) -> Tensor:
+        """[B, M_observed, ...] -> [B, M1 M2, ...] with zeros on the
+        unobserved samples (operators.py:484-496)."""
+        if self.observed_indices is None:
+            return observed_data
+        full = torch.zeros((observed_data.shape[0], self.full_n_samples, *observed_data.shape[2:]),
+                           dtype=observed_data.dtype, device=observed_data.device)
+        full.index_copy_(1, self.observed_indices, observed_data)
+        return full
 
     def _workspace(self, batch: int) -> Tensor:
         need = batch * self.grid_shape[0] * self.sample_shape[1]
@@ -91,19 +118,19 @@ class PrefixFourierOperator:
         self._check(coefficients, self.n_grid, "coefficients")
         x = coefficients.contiguous()
         b = x.shape[0]
-        y = torch.empty((b, self.n_samples), dtype=torch.complex64,
+        y = torch.empty((b, self.full_n_samples), dtype=torch.complex64,
                         device=self.device)
         mid = self._workspace(b)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(_lib.load().fbme_prefix_forward(
             ctypes.c_void_p(stream), b, *self.grid_shape, *self.sample_shape,
             x.data_ptr(), y.data_ptr(), mid.data_ptr()), "fbme_prefix_forward")
-        return y
+        return self.restrict(y)
 
     def adjoint_flat(self, data: Tensor) -> Tensor:
         """A^H v for v [B, M] -> [B, N] (operators.py:255-280)."""
         self._check(data, self.n_samples, "data")
-        v = data.contiguous()
+        v = self.expand(data).contiguous()
         b = v.shape[0]
         x = torch.empty((b, self.n_grid), dtype=torch.complex64,
                         device=self.device)
@@ -149,4 +176,4 @@ class PrefixFourierOperator:
         phase = (r1[:, :, None, :].double() / n1
                  + r2[:, None, :, :].double() / n2)
         cols = torch.exp(-2j * torch.pi * phase).to(torch.complex64)
-        return cols.reshape(flat_indices.shape[0], m1 * m2, -1)
+        return self.restrict(cols.reshape(flat_indices.shape[0], m1 * m2, -1))

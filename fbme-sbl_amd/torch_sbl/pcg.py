@@ -14,6 +14,15 @@ switch.  Probes are drawn from torch's CPU generator seeded with
 ``config.seed`` (the stream the reference draws on CPU) and copied to the GPU
 once.
 
+An observation mask (solver.py:986-1024) keeps every vector at M1 M2
+samples with zeros on the unobserved ones: data and probes are scattered
+onto the observed samples, the matvec and preconditioner outputs are
+multiplied by the mask before their inner products, the preconditioner's
+and the reduced posterior's Gram come from the mask's 2-D spectrum, and
+the observed count replaces M in the probe rule, data power, degrees of
+freedom and noise update.  This is the reference's restricted operator
+(operators.py:57-75, 473-496) without the gather.
+
 Differences from the reference, all documented:
   * an active-set switch is taken when the 32 strongest variances explain
     95% of the total (the reference then also selects 32 atoms); cases where
@@ -167,7 +176,7 @@ USE_GRAPHS = True
 class _Solver:
     """Device buffers and kernel calls of one (B, grid, samples, P) problem."""
 
-    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None):
+    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None, observation_mask=None):
         self.lib = _lib.load()
         self.dev = y.device
         self.cfg = cfg
@@ -176,11 +185,25 @@ class _Solver:
         self.grid = tuple(grid)
         self.n = prod(self.grid)
         self.m = prod(self.samples)
-        if min(self.n, self.m) < RANK:
-            raise NotImplementedError("mode='cg' needs at least 32 samples and grid cells "
+        # observed samples: obs is one byte per sample, obs_idx their indices
+        self.obs = self.obs_idx = None
+        self.m_obs = self.m
+        if observation_mask is not None:
+            keep = torch.as_tensor(observation_mask, dtype=torch.bool).to(self.dev)
+            if tuple(keep.shape) != self.samples:
+                raise ValueError("observation_mask must match the trailing data dimensions.")
+            keep = keep.reshape(-1)
+            self.m_obs = int(keep.sum())
+            if self.m_obs == 0:
+                raise ValueError("observation_mask must retain at least one sample.")
+            if self.m_obs < self.m:
+                self.obs = keep.to(torch.uint8).contiguous()
+                self.obs_idx = torch.nonzero(keep).reshape(-1)
+        if min(self.n, self.m_obs) < RANK:
+            raise NotImplementedError("mode='cg' needs at least 32 (observed) samples and grid cells "
                                       "(rank-32 preconditioner)")
         self.t = _tables(self.grid, self.samples, self.dev)
-        self.p = _probe_count(self.n, self.m, cfg)
+        self.p = _probe_count(self.n, self.m_obs, cfg)
         self.r = 1 + self.p
         self.lanes = self.b * self.r
         self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
@@ -192,12 +215,14 @@ class _Solver:
         self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256
         # both on the column-transform passes: the PCG update / direction sum
         # the inner-product partials of those passes themselves
-        self.fused = self.grid_op and self.lr_fft
+        self.fused = self.grid_op and self.lr_fft and self.obs is None
         self.nparts = (self.samples[1] + 15) // 16
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
         self.y = y.reshape(b, m).contiguous()
+        if self.obs is not None:
+            self.y = self.y * self.obs
         n1, n2 = self.grid
         m1, m2 = self.samples
         e1, e2 = self.t.e1, self.t.e2
@@ -206,9 +231,15 @@ class _Solver:
         self.rhs = torch.empty((b, r, m), dtype=c64, device=dev)
         self.rhs[:, 0] = self.y
         if probes_override is not None:
-            self.rhs[:, 1:] = probes_override.permute(0, 2, 1).to(dev)
+            probes = probes_override.permute(0, 2, 1).to(dev)
         else:
-            self.rhs[:, 1:] = _probes(cfg.seed, b, m, self.p, dev)
+            probes = _probes(cfg.seed, b, self.m_obs, self.p, dev)
+        if self.obs is None:
+            self.rhs[:, 1:] = probes
+        else:
+            # the reference draws [B, M_observed, P] (solver.py:1209-1216)
+            self.rhs[:, 1:] = 0
+            self.rhs[:, 1:, self.obs_idx] = probes
         self.lprobes = torch.empty((b, self.p, n), dtype=c64, device=dev)
         self.mid = torch.empty(b * max(r, 1) * n1 * max(m2, 1), dtype=c64, device=dev)
         self._adjoint(self.rhs[:, 1:].reshape(b * self.p, m), self.lprobes.view(b * self.p, n))
@@ -261,6 +292,12 @@ class _Solver:
         self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
         self.lr_map = torch.empty((b, 386), dtype=torch.int32, device=dev)
         self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
+        self.mask_spec = None
+        if self.obs is not None:
+            self.mask_spec = torch.empty(n, dtype=torch.complex128, device=dev)
+            work = torch.empty(m1 * n2, dtype=torch.complex128, device=dev)
+            self._chk(self.lib.fbme_mask_spectrum(self.stream, n1, n2, m1, m2, _ptr(self.obs), _ptr(self.mask_spec),
+                                                  _ptr(work)), "mask_spectrum")
         self.rho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.nrho = torch.empty(self.lanes, dtype=f64, device=dev)
         self.curv = torch.empty(self.lanes, dtype=f64, device=dev)
@@ -306,8 +343,21 @@ class _Solver:
         self._chk(self.lib.fbme_lane_dot(self.stream, lanes, a.shape[-1], _ptr(a), _ptr(bvec),
                                          _ptr(self.dot_part), _ptr(out)), "lane_dot")
 
+    def _restrict(self, a, out, dot):
+        """out *= mask (the observed rows of the operator), then the inner
+        product with `a` when asked."""
+        self._chk(self.lib.fbme_mask_lanes_dot(self.stream, self.lanes, self.m, _ptr(self.obs), _ptr(a), _ptr(out),
+                                               _ptr(self.dot_part), _ptr(dot)), "mask_lanes")
+
     def matvec(self, d, out, dot=None):
         """out = Q d; with `dot`, also dot[lane] = Re <d, out> (FP64)."""
+        if self.obs is not None:
+            self._matvec(d, out)
+            self._restrict(d, out, dot)
+            return
+        self._matvec(d, out, dot)
+
+    def _matvec(self, d, out, dot=None):
         t = self.t
         m1, m2 = self.samples
         if self.grid_op:
@@ -325,6 +375,13 @@ class _Solver:
 
     def precondition(self, v, out, dot=None):
         """out = M^-1 v; with `dot`, also dot[lane] = Re <v, out> (FP64)."""
+        if self.obs is not None:
+            self._precondition(v, out)
+            self._restrict(v, out, dot)
+            return
+        self._precondition(v, out, dot)
+
+    def _precondition(self, v, out, dot=None):
         if self.lr_fft:
             self._chk(self.lib.fbme_lowrank_apply_fft_dot(
                 self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv),
@@ -345,7 +402,7 @@ class _Solver:
                                          _ptr(self.dot_part), _ptr(self.ye)), "ye")
         self._chk(self.lib.fbme_lane_dot(self.stream, self.b, self.n, _ptr(self.bp), _ptr(self.bp),
                                          _ptr(self.dot_part), _ptr(self.bpe)), "bpe")
-        self._chk(self.lib.fbme_cg_init(self.stream, self.b, self.n, self.m, _ptr(self.bp), _ptr(self.ye),
+        self._chk(self.lib.fbme_cg_init(self.stream, self.b, self.n, self.m_obs, _ptr(self.bp), _ptr(self.ye),
                                         _ptr(self.bpe), float(cfg.min_variance),
                                         float(cfg.initial_noise_fraction), float(cfg.initialization_blend),
                                         _ptr(self.power), _ptr(self.floor), _ptr(self.noise),
@@ -367,10 +424,10 @@ class _Solver:
         self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
         gsum = self.sums[:, 0].contiguous()
-        self._chk(self.lib.fbme_lowrank_setup(
+        self._chk(self.lib.fbme_lowrank_setup_masked(
             self.stream, self.b, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
-            _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
-            _ptr(self.status)), "lowrank_setup")
+            _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.mask_spec), _ptr(self.factors),
+            _ptr(self.sinv), _ptr(self.bg), _ptr(self.status)), "lowrank_setup")
         if self.lr_fft:
             self._chk(self.lib.fbme_lowrank_map(self.stream, self.b, n1, n2, _ptr(self.top_idx), _ptr(self.lr_map)),
                       "lowrank_map")
@@ -525,11 +582,11 @@ class _Solver:
         if self.support is not None:
             t = self.t
             gsum = self.sums[:, 0].contiguous()
-            self._chk(self.lib.fbme_reduced_posterior(
-                self.stream, self.b, self.support.shape[1], *self.grid, *self.samples, _ptr(self.support),
+            self._chk(self.lib.fbme_reduced_posterior_masked(
+                self.stream, self.b, self.support.shape[1], *self.grid, self.m_obs, _ptr(self.support),
                 _ptr(self.gamma), _ptr(gsum), _ptr(self.noise), _ptr(self.bp), _ptr(self.ye), _ptr(t.dir1),
-                _ptr(t.dir2), float(self.cfg.cuda_cholesky_jitter_multiplier), _ptr(self.mean),
-                _ptr(self.var), _ptr(self.dd), _ptr(self.status)), "reduced_posterior")
+                _ptr(t.dir2), _ptr(self.mask_spec), float(self.cfg.cuda_cholesky_jitter_multiplier),
+                _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.status)), "reduced_posterior")
             self._chk(self.lib.fbme_relevance_sum(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.dd),
                                                   _ptr(self.grid_part), _ptr(self.rel_sum)), "relevance")
             return 0
@@ -544,7 +601,7 @@ class _Solver:
     def em(self, update=True):
         cfg = self.cfg
         keep = max(cfg.damping, cfg.cg_damping)
-        self._chk(self.lib.fbme_cg_em(self.stream, self.b, self.n, self.m, _ptr(self.gamma), _ptr(self.mean),
+        self._chk(self.lib.fbme_cg_em(self.stream, self.b, self.n, self.m_obs, _ptr(self.gamma), _ptr(self.mean),
                                       _ptr(self.dd), _ptr(self.rel_sum), _ptr(self.floor), _ptr(self.mask),
                                       float(cfg.hyper_shape), float(cfg.hyper_rate), float(cfg.max_precision),
                                       float(keep), 1 if update else 0, _ptr(self.grid_part), _ptr(self.dof),
@@ -552,8 +609,9 @@ class _Solver:
         if not update:
             return
         self._forward(self.mean, self.pred)
-        self._chk(self.lib.fbme_uamp_noise_update(
-            self.stream, self.b, self.m, _ptr(self.y), _ptr(self.pred), _ptr(self.noise), _ptr(self.dof_f),
+        self._chk(self.lib.fbme_uamp_noise_update_masked(
+            self.stream, self.b, self.m, _ptr(self.obs), self.m_obs, _ptr(self.y), _ptr(self.pred), _ptr(self.noise),
+            _ptr(self.dof_f),
             _ptr(self.floor), _ptr(self.noise_new), float(cfg.noise_shape), float(cfg.noise_rate), float(keep),
             1 if cfg.noise_update == "evidence" else 0, float(F32_EPS), _ptr(self.noise_part)), "noise")
         self.noise.copy_(self.noise_new)
@@ -587,19 +645,21 @@ class SolverStall(RuntimeError):
 
 
 def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
-           probes_override=None, warm_start=None, stall_check=None) -> SBLResult:
-    """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64).
+           probes_override=None, warm_start=None, stall_check=None,
+           observation_mask=None) -> SBLResult:
+    """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64),
+    optionally on the samples of a boolean observation_mask [M1, M2].
 
     warm_start(solver) may replace the initial variances after the plain
     initialisation; stall_check(solver, pcg_iterations, cfg, sum_gamma_before)
     is consulted after every EM iteration and raises SolverStall when true."""
     _check_supported(cfg)
     with torch.cuda.device(y.device):
-        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check)
+        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check, observation_mask)
 
 
-def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None):
-    s = _Solver(y, grid, cfg, probes_override)
+def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None, observation_mask=None):
+    s = _Solver(y, grid, cfg, probes_override, observation_mask)
     s.init()
     if warm_start is not None:
         warm_start(s)

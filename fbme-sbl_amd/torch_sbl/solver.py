@@ -121,16 +121,26 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
         raise RuntimeError(
             "This build runs FBME on MI355X GPUs only; pass a GPU tensor or "
             "device (the CPU oracle lives in the test suite).")
+    mask = None
     if observation_mask is not None:
-        raise NotImplementedError(
-            "observation_mask is not supported by the MI355X plan yet.")
+        # solver.py:986-1000
+        mask = torch.as_tensor(observation_mask, dtype=torch.bool)
+        if tuple(mask.shape) != samples:
+            raise ValueError(
+                "observation_mask must match the trailing data dimensions.")
+        if not bool(mask.any()):
+            raise ValueError(
+                "observation_mask must retain at least one sample.")
+        if cfg.offgrid_refinement:
+            raise ValueError(
+                "Run off-grid refinement after masked reconstruction.")
     if tensor.dtype in (torch.complex128, torch.float64):
         raise NotImplementedError(
             "complex128 arithmetic is not provided by the MI355X kernels; "
             "pass complex64 data.")
     y = tensor.to(device=device, dtype=torch.complex64)
     batch = prod(batch_shape) if batch_shape else 1
-    m = prod(samples)
+    m = prod(samples) if mask is None else int(mask.sum())
     threshold = cfg.dense_threshold if device.type == "cuda" else min(
         cfg.dense_threshold, 512)
     mode = ("dense" if cfg.mode == "auto" and m <= threshold else
@@ -142,7 +152,11 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
     if mode == "cg":
         from . import pcg
         return pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
-                          batch_shape=batch_shape)
+                          batch_shape=batch_shape, observation_mask=mask)
+    if mask is not None:
+        raise NotImplementedError(
+            "observation_mask is implemented for mode='cg'; the UAMP plans "
+            "run on full prefix blocks.")
     if cfg.uamp_backend == "torch":
         raise NotImplementedError(
             "uamp_backend='torch' selects the reference's PyTorch UAMP; this "

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 3
+#define FBME_ABI_VERSION 4
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -340,6 +340,54 @@ int fbme_reduced_posterior(void* stream, int batch, int k, int n1, int n2, int m
                            const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
                            const double* ye, const void* dir1, const void* dir2, float jitter_multiplier,
                            void* mean, float* var, float* dd, int* status);
+
+/* ------------------------------------------------------------------------
+ * Observation masks of the general solver: sbl_2d(..., observation_mask=)
+ * in mode="cg" (torch_sbl/solver.py:986-1024).  The reference restricts the
+ * prefix operator to the observed samples (PrefixFourierOperator.restrict /
+ * expand, torch_sbl/operators.py:57-75 and 473-496).  These kernels keep
+ * every vector at M1 M2 samples with zeros on the unobserved ones, so the
+ * restricted operator is the full one followed by the mask; `mask` is one
+ * byte per sample (nonzero = observed), shared by the batch.
+ * ------------------------------------------------------------------------ */
+/* spectrum[d1, d2] = sum over observed (m1, m2) of
+ * exp(+2 pi i (m1 d1 / N1 + m2 d2 / N2)), complex128 [N1, N2]: the Gram entry
+ * a_k^H a_l of two atoms on the observed rows is
+ * spectrum[(k1 - l1) mod N1, (k2 - l2) mod N2]; for a full mask it is the
+ * product of the fbme_dirichlet_table tables.  work: M1 * N2 complex128.
+ * (The reference forms the same Gram from the restricted columns,
+ * solver.py:5786-5848 and 3591-3779.) */
+int fbme_mask_spectrum(void* stream, int n1, int n2, int m1, int m2, const unsigned char* mask, void* spectrum,
+                       void* work);
+/* out[lane, i] *= mask[i] for every lane, the restriction of Q d or M^-1 v
+ * to the observed samples (operators.py:57-75); with dot (and a), also
+ * dot[lane] = Re <a, out> (FP64), as fbme_lane_dot.  part: lanes * 64
+ * doubles. */
+int fbme_mask_lanes_dot(void* stream, int lanes, int m, const unsigned char* mask, const void* a, void* out,
+                        double* part, double* dot);
+/* fbme_lowrank_setup with the factor Gram taken on the observed rows
+ * (_build_low_rank_preconditioner on the restricted operator,
+ * solver.py:5786-5848); mask_spectrum from fbme_mask_spectrum (NULL: the
+ * Dirichlet tables dir1, dir2 as in fbme_lowrank_setup). */
+int fbme_lowrank_setup_masked(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx,
+                              const float* val, const double* gamma_sum, const float* noise, const void* dir1,
+                              const void* dir2, const void* mask_spectrum, void* factors, void* sinv, float* bg,
+                              int* status);
+/* fbme_reduced_posterior on the observed samples (solver.py:3591-3779 on the
+ * restricted operator): Gram from mask_spectrum (NULL: Dirichlet tables)
+ * and `observed` samples in the evidence's log-determinant. */
+int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2, int observed, const int* support,
+                                  const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                                  const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
+                                  float jitter_multiplier, void* mean, float* var, float* dd, int* status);
+/* fbme_uamp_noise_update over the observed samples: the residual energy of
+ * the samples with mask[i] nonzero (mask NULL: all) and `observed` in place
+ * of M in the update (solver.py:1737-1768 on the restricted operator). */
+int fbme_uamp_noise_update_masked(void* stream, int batch, int m, const unsigned char* mask, int observed,
+                                  const void* data, const void* prediction, const float* old_noise,
+                                  const float* effective_dof, const float* variance_floor, float* output,
+                                  float noise_shape, float noise_rate, float damping, int evidence_update,
+                                  float epsilon, double* partial);
 
 /* ------------------------------------------------------------------------
  * Learned warm start of the general solver (no counterpart in the

@@ -1,0 +1,198 @@
+"""Observation masks on the general solver (sbl_2d(..., observation_mask=),
+mode='cg') against the float64 oracle and the reference's golden runs.
+
+The GPU path keeps full-length vectors with zeros on the unobserved samples
+(the reference gathers the observed rows, operators.py:57-75 and 473-496).
+Pieces: the mask spectrum against a direct float64 sum (and, for a full
+mask, against the Dirichlet tables it generalises), the masked matvec and
+preconditioner against the oracle's row-restricted operator, the masked
+operator's forward / adjoint, and the whole solver against the reference's
+golden runs (tests/golden/masked_*.npz, written by
+tests/golden/make_golden.py).  Tolerances as for the unmasked solver:
+float32 matvec relative L2 2e-5, preconditioner 1e-4, whole solver 1e-3
+(single scene) / 2e-2 (batch) with NMSE within 0.1 dB.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden_path
+from oracle import cg_oracle as co
+from oracle.fbme_oracle import F32_EPS, F32_TINY
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [((32, 32), (24, 24)), ((40, 30), (20, 15)), ((64, 48), (45, 40)), ((256, 256), (160, 160))]
+
+
+def _mask(seed, samples, keep=0.6):
+    return np.random.default_rng(seed).random(samples) < keep
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _solver(device, grid, samples, mask, batch=2, seed=0):
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, pcg
+    ys = [np.where(mask, scenes.uniform_scene(seed + i, grid, samples, k=10, snr_db=20.0).y, 0)
+          for i in range(batch)]
+    y = torch.from_numpy(np.stack(ys).astype(np.complex64)).to(device)
+    s = pcg._Solver(y, grid, SBLConfig(mode="cg", device=str(device)), None, torch.from_numpy(mask))
+    s.init()
+    torch.cuda.synchronize()
+    return s
+
+
+@pytest.mark.parametrize("grid,samples", [((40, 30), (20, 15)), ((256, 256), (160, 160))])
+def test_mask_spectrum_vs_direct_sum(device, grid, samples):
+    from fbme_amd import _lib
+    from torch_sbl import pcg
+    print("shape", grid, samples, flush=True)
+    lib = _lib.load()
+    n1, n2 = grid
+    m1, m2 = samples
+    stream = torch.cuda.current_stream(device).cuda_stream
+    spec = torch.empty(n1 * n2, dtype=torch.complex128, device=device)
+    work = torch.empty(m1 * n2, dtype=torch.complex128, device=device)
+    mask = _mask(1, samples)
+    mt = torch.from_numpy(mask.astype(np.uint8)).to(device)
+    _lib.check(lib.fbme_mask_spectrum(stream, n1, n2, m1, m2, mt.data_ptr(), spec.data_ptr(), work.data_ptr()),
+               "mask_spectrum")
+    got = spec.cpu().numpy().reshape(n1, n2)
+    e1 = np.exp(2j * np.pi * np.outer(np.arange(m1), np.arange(n1)) / n1)   # [m1, d1]
+    e2 = np.exp(2j * np.pi * np.outer(np.arange(m2), np.arange(n2)) / n2)   # [m2, d2]
+    want = e1.T @ mask.astype(np.float64) @ e2
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * mask.sum())
+    # a full mask gives the product of the Dirichlet tables
+    full = torch.ones(m1 * m2, dtype=torch.uint8, device=device)
+    _lib.check(lib.fbme_mask_spectrum(stream, n1, n2, m1, m2, full.data_ptr(), spec.data_ptr(), work.data_ptr()),
+               "mask_spectrum")
+    t = pcg._tables(grid, samples, device)
+    d1 = t.dir1.cpu().numpy().view(np.complex128)
+    d2 = t.dir2.cpu().numpy().view(np.complex128)
+    np.testing.assert_allclose(spec.cpu().numpy().reshape(n1, n2), np.outer(d1, d2), rtol=0,
+                               atol=1e-9 * m1 * m2)
+
+
+@pytest.mark.parametrize("grid,samples", SHAPES)
+def test_masked_matvec_and_preconditioner_vs_oracle(device, grid, samples):
+    print("shape", grid, samples, flush=True)
+    mask = _mask(2, samples)
+    s = _solver(device, grid, samples, mask, seed=30)
+    assert s.obs is not None and not s.fused
+    s.setup_operators()
+    torch.cuda.synchronize()
+    assert int(s.status.sum()) == 0
+    keep = mask.reshape(-1)
+    op = co.Operator(grid, samples, mask)
+    gamma = s.gamma.cpu().numpy().astype(np.float64)
+    noise = s.noise.cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(6)
+    d = (rng.standard_normal((s.b, s.r, s.m)) + 1j * rng.standard_normal((s.b, s.r, s.m))) * keep
+    dt = torch.from_numpy(d.astype(np.complex64)).to(device)
+    out = torch.empty_like(dt)
+    dot = torch.empty(s.lanes, dtype=torch.float64, device=device)
+    s.matvec(dt, out, dot=dot)
+    got = out.cpu().numpy()
+    assert not np.any(got[:, :, ~keep])
+    ref = np.moveaxis(op.q(np.moveaxis(d[:, :, keep], 1, 2), gamma, noise), 2, 1)
+    assert _rel(got[:, :, keep], ref) < 2e-5
+    np.testing.assert_allclose(dot.cpu().numpy(), np.sum(np.real(np.conj(d) * got), -1).reshape(-1), rtol=1e-5)
+    pre = co.low_rank_preconditioner(gamma, noise, op, 32, F32_EPS, F32_TINY)
+    np.testing.assert_allclose(s.bg.cpu().numpy(), pre[0], rtol=1e-5)
+    s.precondition(dt, out, dot=dot)
+    got = out.cpu().numpy()
+    assert not np.any(got[:, :, ~keep])
+    ref = np.moveaxis(co.apply_preconditioner(np.moveaxis(d[:, :, keep], 1, 2), pre), 2, 1)
+    assert _rel(got[:, :, keep], ref) < 1e-4
+    np.testing.assert_allclose(dot.cpu().numpy(), np.sum(np.real(np.conj(d) * got), -1).reshape(-1), rtol=1e-5)
+
+
+def test_masked_operator_forward_adjoint(device):
+    from torch_sbl.operators import PrefixFourierOperator
+    grid, samples = (40, 30), (20, 15)
+    mask = _mask(3, samples)
+    op = PrefixFourierOperator(grid, samples, device=device, observation_mask=torch.from_numpy(mask))
+    ref = co.Operator(grid, samples, mask)
+    assert op.n_samples == ref.m and not op.is_complete
+    rng = np.random.default_rng(7)
+    x = (rng.standard_normal((2, op.n_grid)) + 1j * rng.standard_normal((2, op.n_grid)))
+    v = (rng.standard_normal((2, op.n_samples)) + 1j * rng.standard_normal((2, op.n_samples)))
+    fx = op.forward_flat(torch.from_numpy(x.astype(np.complex64)).to(device)).cpu().numpy()
+    av = op.adjoint_flat(torch.from_numpy(v.astype(np.complex64)).to(device)).cpu().numpy()
+    assert _rel(fx, ref.fwd(x[:, :, None])[:, :, 0]) < 2e-6
+    assert _rel(av, ref.adj(v[:, :, None])[:, :, 0]) < 2e-6
+    idx = torch.tensor([[0, 7, 599, 1199]], device=device)
+    cols = op.fourier_columns(idx).cpu().numpy()
+    assert _rel(cols, ref.columns(idx.cpu().numpy())) < 1e-6
+
+
+@pytest.mark.parametrize("name,grid,kw,rel_x,its", [
+    ("masked_32x32_24x24", (32, 32), {}, 1e-3, 6),
+    ("masked_64x64_40x40_b2", (64, 64), dict(hutchinson_probes=16, auto_probe_scaling=False,
+                                             cg_tolerance=1e-4, max_iter=30), 2e-2, 0),
+])
+def test_masked_sbl_2d_matches_reference_golden(device, name, grid, kw, rel_x, its):
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, sbl_2d
+    print("golden", name, flush=True)
+    g = np.load(golden_path(name + ".npz"))
+    y = torch.from_numpy(g["y"]).to(device)
+    res = sbl_2d(y, grid, config=SBLConfig(mode="cg", device=str(device), **kw),
+                 observation_mask=torch.from_numpy(g["mask"]))
+    assert res.mode == "cg"
+    assert abs(res.iterations - int(g["cg_iterations"])) <= its
+    assert res.active_set_size == int(g["cg_active_set_size"])
+    x = res.x.cpu().numpy()
+    ref = g["cg_x"].reshape(x.shape)
+    truth = g["x_true"].reshape(x.shape)
+    xs, rs, ts = x.reshape(-1, *grid), ref.reshape(-1, *grid), truth.reshape(-1, *grid)
+    for i in range(xs.shape[0]):
+        assert _rel(xs[i], rs[i]) < rel_x
+        assert abs(scenes.nmse_db(xs[i], ts[i]) - scenes.nmse_db(rs[i], ts[i])) < 0.1
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), g["cg_noise_variance"], rtol=2e-2)
+    np.testing.assert_allclose(res.effective_dof.cpu().numpy(), g["cg_effective_dof"], rtol=2e-2)
+
+
+def test_masked_256_grid_vs_oracle(device):
+    """256 x 256 grid, 160 x 160 prefix with 60% observed: three EM
+    iterations of the masked solver against the float64 oracle (the grid
+    matvec and FFT preconditioner with the mask applied to their outputs)."""
+    from fbme_amd import scenes
+    from torch_sbl import pcg, SBLConfig
+    grid, samples = (256, 256), (160, 160)
+    mask = _mask(4, samples)
+    sc = scenes.ellipse_scene(1300, grid, samples, k=300, snr_db=10.0)
+    y = np.where(mask, sc.y, 0).astype(np.complex64)
+    kw = dict(max_iter=3, min_iter=1, tol=0.0, hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
+    res = pcg.sbl_cg(torch.from_numpy(y[None]).to(device), grid, SBLConfig(mode="cg", device=str(device), **kw),
+                     batch_shape=(1,), observation_mask=torch.from_numpy(mask))
+    ref = co.sbl_cg(y[None], grid, co.CGConfig(**{k: v for k, v in kw.items() if k not in ("min_iter", "tol")}),
+                    mask=mask)
+    x = res.x.cpu().numpy().reshape(-1)
+    xr = ref["x"].reshape(-1)
+    print("masked 256:", _rel(x, xr), scenes.nmse_db(x, sc.x.reshape(-1)), scenes.nmse_db(xr, sc.x.reshape(-1)),
+          flush=True)
+    assert _rel(x, xr) < 2e-2
+    assert abs(scenes.nmse_db(x, sc.x.reshape(-1)) - scenes.nmse_db(xr, sc.x.reshape(-1))) < 0.1
+    assert float(res.noise_variance[0]) == pytest.approx(float(ref["noise_variance"][0]), rel=2e-2)
+
+
+def test_full_mask_is_unmasked_and_validation(device):
+    from torch_sbl import SBLConfig, sbl_2d
+    g = np.load(golden_path("general_32x32_24x24.npz"))
+    y = torch.from_numpy(g["y"]).to(device)
+    cfg = SBLConfig(mode="cg", device=str(device), max_iter=4)
+    a = sbl_2d(y, (32, 32), config=cfg)
+    b = sbl_2d(y, (32, 32), config=cfg, observation_mask=torch.ones(24, 24, dtype=torch.bool))
+    assert torch.equal(a.x, b.x)
+    with pytest.raises(ValueError):
+        sbl_2d(y, (32, 32), config=cfg, observation_mask=torch.ones(24, 23, dtype=torch.bool))
+    with pytest.raises(ValueError):
+        sbl_2d(y, (32, 32), config=cfg, observation_mask=torch.zeros(24, 24, dtype=torch.bool))
+    with pytest.raises(NotImplementedError):
+        sbl_2d(y, (32, 32), config=SBLConfig(mode="uamp", device=str(device)),
+               observation_mask=torch.from_numpy(_mask(5, (24, 24))))
