@@ -19,7 +19,14 @@ Pins (see tests/test_oracle_*.py):
     independently in tests (closed-form elementwise maps on random inputs),
     invariants 0 < v, 0 <= dof <= M.
   * end-to-end Fast/Stable: golden vectors written by
-    tests/golden/make_golden.py from the reference's CPU Torch backend.
+    tests/golden/make_golden.py from the reference's CPU Torch backend,
+    with and without an observation mask.
+
+An observation mask [M1, M2] (solver.py:986-1024) keeps the observed rows
+of the operator (operators.py:57-75, 473-496): data and messages live on
+the observed samples, A gathers them after the transform and A^H scatters
+them into zeros before it, Gram matrices are formed from the restricted
+columns, and M is the observed count throughout.
 """
 
 from __future__ import annotations
@@ -56,6 +63,39 @@ def prefix_adjoint(v: np.ndarray, grid, samples) -> np.ndarray:
     n = grid[0] * grid[1]
     out = np.fft.ifft2(field_, s=grid, axes=(1, 2)) * n
     return out.reshape(b, -1)
+
+
+def restrict(v: np.ndarray, mask) -> np.ndarray:
+    """[B, M1 M2, ...] -> [B, M_observed, ...] (operators.py:473-482)."""
+    return v if mask is None else v[:, np.asarray(mask, bool).reshape(-1)]
+
+
+def expand(v: np.ndarray, mask) -> np.ndarray:
+    """[B, M_observed, ...] -> [B, M1 M2, ...], zeros on the unobserved
+    samples (operators.py:484-496)."""
+    if mask is None:
+        return v
+    keep = np.asarray(mask, bool).reshape(-1)
+    out = np.zeros((v.shape[0], keep.size) + v.shape[2:], np.result_type(v, np.complex128))
+    out[:, keep] = v
+    return out
+
+
+def masked_forward(x, grid, samples, mask=None):
+    return restrict(prefix_forward(x, grid, samples), mask)
+
+
+def masked_adjoint(v, grid, samples, mask=None):
+    return prefix_adjoint(expand(v, mask), grid, samples)
+
+
+def masked_gram(indices, grid, samples, mask=None):
+    """C^H C of the selected columns on the observed rows (closed form
+    without a mask)."""
+    if mask is None:
+        return gram_closed_form(indices, grid, samples)
+    cols = restrict(fourier_columns(indices, grid, samples), mask)
+    return np.einsum("bmk,bml->bkl", np.conj(cols), cols)
 
 
 def dense_prefix_matrix(grid, samples) -> np.ndarray:
@@ -176,13 +216,13 @@ class UampTrace:
     dof: list = field(default_factory=list)
 
 
-def uamp_initial_state(y, grid, samples, cfg: PlanConfig, eps, tiny):
+def uamp_initial_state(y, grid, samples, cfg: PlanConfig, eps, tiny, mask=None):
     """Backprojection initialisation (solver.py:1063-1189)."""
     n = grid[0] * grid[1]
     m = y.shape[1]
     power = np.maximum(np.mean(np.abs(y) ** 2, axis=1), eps)
     floor = np.maximum(power * cfg.min_variance, tiny)
-    bp = prefix_adjoint(y, grid, samples)
+    bp = masked_adjoint(y, grid, samples, mask)
     energy = np.abs(bp) ** 2
     structured = (energy / np.maximum(energy.sum(axis=1, keepdims=True), eps)
                   * power[:, None])
@@ -201,7 +241,7 @@ def uamp_initial_state(y, grid, samples, cfg: PlanConfig, eps, tiny):
 
 def uamp_step(y, gamma, noise, state, grid, samples, *, damping, tiny,
               backprojection=None, prior_floor=None, cfg=None,
-              compute_prediction=True):
+              compute_prediction=True, mask=None):
     """One UAMP step: message (triton_kernels.py:36-89), lift (A^H or the
     scaled backprojection at the first step, solver.py:2775-2782), denoise +
     optional fused ARD update (triton_kernels.py:92-192), prediction A mu
@@ -218,7 +258,7 @@ def uamp_step(y, gamma, noise, state, grid, samples, *, damping, tiny,
     if backprojection is not None:
         lifted = backprojection * ((1.0 - damping) / total)[:, None]
     else:
-        lifted = prefix_adjoint(message, grid, samples)
+        lifted = masked_adjoint(message, grid, samples, mask)
     denom = np.maximum(gamma + tau[:, None], tiny)
     gain = gamma / denom
     cand_mean = gain * (state["mean"] + tau[:, None] * lifted)
@@ -235,7 +275,7 @@ def uamp_step(y, gamma, noise, state, grid, samples, *, damping, tiny,
                                 np.minimum(cfg.max_precision, cand_prior))
         nxt = cfg.damping * gamma + (1.0 - cfg.damping) * cand_prior
         next_prior = np.maximum(nxt, prior_floor[:, None])
-    prediction = (prefix_forward(mean, grid, samples) if compute_prediction
+    prediction = (masked_forward(mean, grid, samples, mask) if compute_prediction
                   else state["prediction"])
     new_state = dict(mean=mean, variance=var, message=message,
                      prediction=prediction, data_variance=var.sum(axis=1))
@@ -243,19 +283,20 @@ def uamp_step(y, gamma, noise, state, grid, samples, *, damping, tiny,
 
 
 def uamp_sweep(y, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
-               tiny=F32_TINY, trace: UampTrace | None = None):
-    """Init + T_u UAMP iterations + final E-step (solver.py:1063-1925)."""
+               tiny=F32_TINY, trace: UampTrace | None = None, mask=None):
+    """Init + T_u UAMP iterations + final E-step (solver.py:1063-1925);
+    y holds the observed samples."""
     n = grid[0] * grid[1]
     m = y.shape[1]
     power, floor, bp, gamma, noise, state = uamp_initial_state(
-        y, grid, samples, cfg, eps, tiny)
+        y, grid, samples, cfg, eps, tiny, mask)
     t_u = uamp_iteration_budget(n, m, cfg)
     max_dof = m * (1.0 - 10.0 * eps)
     for it in range(1, t_u + 1):
         _, _, dof, next_prior, state = uamp_step(
             y, gamma, noise, state, grid, samples, damping=cfg.uamp_damping,
             tiny=tiny, backprojection=bp if it == 1 else None,
-            prior_floor=floor, cfg=cfg)
+            prior_floor=floor, cfg=cfg, mask=mask)
         dof = np.minimum(dof, max_dof)
         energy = np.sum(np.abs(y - state["prediction"]) ** 2, axis=1)
         if cfg.noise_update == "em":
@@ -274,7 +315,7 @@ def uamp_sweep(y, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
             trace.dof.append(dof.copy())
     mean, var, _, _, state = uamp_step(
         y, gamma, noise, state, grid, samples, damping=cfg.uamp_damping,
-        tiny=tiny, compute_prediction=False)
+        tiny=tiny, compute_prediction=False, mask=mask)
     return dict(mean=mean, variance=var, gamma=gamma, noise=noise,
                 floor=floor, power=power, backprojection=bp, iterations=t_u)
 
@@ -342,12 +383,14 @@ def _topk_desc(values, k):
 
 
 def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
-           tiny=F32_TINY, gram_fn=None):
-    """Evidence-gated active-set EM polish with optional residual exchange."""
+           tiny=F32_TINY, gram_fn=None, mask=None):
+    """Evidence-gated active-set EM polish with optional residual exchange;
+    y holds the observed samples."""
     b, m = y.shape
     n = grid[0] * grid[1]
     if gram_fn is None:
-        gram_fn = gram_closed_form
+        def gram_fn(sup, grid_, samples_):
+            return masked_gram(sup, grid_, samples_, mask)
     k = min(cfg.polish_size, n)
     floor = sweep["floor"]
     score = np.abs(sweep["mean"]) ** 2 + sweep["variance"]
@@ -435,7 +478,7 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
         if exch > 0 and pit == 0:
             # |A^H r|^2 with r = y - C mu_S; A^H C mu_S via the Dirichlet
             # kernel: (A^H A)[n, k] = D1[n1-k1] D2[n2-k2].
-            resid_bp = bp - _gram_apply(mu_s, support, grid, samples)
+            resid_bp = bp - _gram_apply(mu_s, support, grid, samples, mask)
             rscore = np.abs(resid_bp) ** 2
             rscore[rows, support] = -np.inf
             add_idx = _topk_desc(rscore, exch)
@@ -450,8 +493,8 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
                                 ratio - 1.0 - np.log(np.maximum(ratio, tiny)),
                                 0.0)
             del_gain = dgain[rows, rem_pos]
-            mask = (add_gain + del_gain) > 0.0
-            repl_idx = np.where(mask, add_idx, rem_idx)
+            swap = (add_gain + del_gain) > 0.0
+            repl_idx = np.where(swap, add_idx, rem_idx)
             new_support = support.copy()
             new_support[rows, rem_pos] = repl_idx
             add_gamma = (np.maximum(add_energy - base_noise[:, None]
@@ -459,10 +502,10 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
             add_gamma = np.minimum(np.maximum(add_gamma, floor[:, None]),
                                    cfg.max_precision)
             retained = working[rows, rem_idx]
-            working[rows, rem_idx] = np.where(mask, floor[:, None], retained)
-            working[rows, repl_idx] = np.where(mask, add_gamma, retained)
+            working[rows, rem_idx] = np.where(swap, floor[:, None], retained)
+            working[rows, repl_idx] = np.where(swap, add_gamma, retained)
             support = new_support
-            exchange_pending = mask.any(axis=1)
+            exchange_pending = swap.any(axis=1)
     out = dict(
         mean=best["mean"], posterior_variance=best["pvar"],
         data_diagonal=best["ddiag"], gamma=best["prior"],
@@ -474,8 +517,12 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
     return out
 
 
-def _gram_apply(mu_s, support, grid, samples):
-    """(A^H A x_S)[n] = sum_k mu_k D1[(n1-k1) mod N1] D2[(n2-k2) mod N2]."""
+def _gram_apply(mu_s, support, grid, samples, mask=None):
+    """(A^H A x_S)[n] = sum_k mu_k D1[(n1-k1) mod N1] D2[(n2-k2) mod N2];
+    with a mask, A^H (A x_S) on the observed rows through the transforms."""
+    if mask is not None:
+        cols = restrict(fourier_columns(support, grid, samples), mask)
+        return masked_adjoint(np.einsum("bmk,bk->bm", cols, mu_s), grid, samples, mask)
     n1, n2 = grid
     d1 = dirichlet(samples[0], n1)
     d2 = dirichlet(samples[1], n2)
@@ -493,7 +540,7 @@ def _gram_apply(mu_s, support, grid, samples):
 
 
 def fixed_budget_residuals(y, result, floor, grid, samples, cfg, *, eps,
-                           tiny):
+                           tiny, mask=None):
     """Stable's EM fixed-point residuals (solver.py:3408-3496) and the
     evidence-trace monotonicity flags (solver.py:1989-2021)."""
     b, m = y.shape
@@ -507,7 +554,7 @@ def fixed_budget_residuals(y, result, floor, grid, samples, cfg, *, eps,
     scale = np.maximum(np.maximum(np.abs(gamma_s), np.abs(target)), tiny)
     gamma_res = np.max(np.abs(gamma_s - target) / scale, axis=1)
     dof = np.clip(1.0 - sig_s / gamma_s, 0.0, 1.0).sum(1)
-    cols = fourier_columns(sup, grid, samples)
+    cols = restrict(fourier_columns(sup, grid, samples), mask)
     pred = np.einsum("bmk,bk->bm", cols, mu_s)
     energy = np.sum(np.abs(y - pred) ** 2, axis=1)
     noise = result["noise"]
@@ -536,8 +583,10 @@ def project_relevance_dof(gamma, data_diag, m, eps):
 
 
 def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
-              polish_size: int = 32, *, eps=F32_EPS, tiny=F32_TINY) -> dict:
-    """End-to-end Fast/Stable plan on [B, M1, M2] (or [M1, M2]) data.
+              polish_size: int = 32, *, eps=F32_EPS, tiny=F32_TINY,
+              mask=None) -> dict:
+    """End-to-end Fast/Stable plan on [B, M1, M2] (or [M1, M2]) data,
+    optionally on the samples an observation mask [M1, M2] keeps.
 
     Returns the SBLResult fields the locked plans populate
     (solver.py:2081-2126, 2154-2393)."""
@@ -549,12 +598,16 @@ def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
     samples = tuple(int(v) for v in data.shape[-2:])
     grid = tuple(int(v) for v in grid)
     n = grid[0] * grid[1]
-    y = data.reshape(b, -1).astype(np.complex128)
+    if mask is not None:
+        mask = np.asarray(mask, bool)
+        if mask.shape != samples or not mask.any():
+            raise ValueError("observation_mask must match the samples and keep one")
+    y = restrict(data.reshape(b, -1).astype(np.complex128), mask)
     m = y.shape[1]
     cfg = (PlanConfig.fast if profile == "fast" else PlanConfig.stable)(
         polish_size)
-    sweep = uamp_sweep(y, grid, samples, cfg, eps=eps, tiny=tiny)
-    res = polish(y, sweep, grid, samples, cfg, eps=eps, tiny=tiny)
+    sweep = uamp_sweep(y, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
+    res = polish(y, sweep, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
     out = dict(
         x=res["mean"], posterior_variance=res["posterior_variance"],
         precision=1.0 / res["gamma"], noise_variance=res["noise"],
@@ -572,7 +625,7 @@ def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
     if cfg.fixed_budget_diagnostics:
         out.update({f"fixed_budget_{k}": v for k, v in fixed_budget_residuals(
             y, res, sweep["floor"], grid, samples, cfg, eps=eps,
-            tiny=tiny).items()})
+            tiny=tiny, mask=mask).items()})
     if cfg.false_alarm_budget is not None:
         thr = log(n / cfg.false_alarm_budget)
         score = np.abs(out["x"]) ** 2 / np.maximum(out["posterior_variance"],

@@ -15,7 +15,8 @@ What it pins (functions no closed-form check fixes end to end):
     solver.py:1025-1036) on the 32x32 / 24x24 workload.
   * ``dense_em_sbl`` in complex128 on the same workload.
   * ``sbl_2d`` in mode "cg" with an ``observation_mask`` (a single 32x32 /
-    24x24 scene and a batch of two 64x64 / 40x40 scenes).
+    24x24 scene and a batch of two 64x64 / 40x40 scenes), and ``fast_sbl``
+    under the Fast / Stable numerics with an ``observation_mask``.
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -193,6 +194,44 @@ def masked_cg(reference_root: str) -> None:
     print("wrote masked_64x64_40x40_b2")
 
 
+def masked_plans(reference_root: str) -> None:
+    """``fast_sbl`` under the Fast and Stable numerics with an observation
+    mask: a 64x64 / 40x40 uniform scene with 70% observed and a 96x80 /
+    60x48 ellipse scene with 60% observed."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, fast_sbl
+
+    torch.set_num_threads(8)
+    cases = [
+        ("masked_plan_64x64_40x40", scenes.uniform_scene(1030, (64, 64), (40, 40), k=20, snr_db=15.0), 0.7),
+        ("masked_plan_96x80_60x48", scenes.ellipse_scene(1031, (96, 80), (60, 48), k=24, snr_db=10.0), 0.6),
+    ]
+    for name, scene, keep in cases:
+        mask = observation_mask(1030 + len(name), scene.samples, keep)
+        y = np.where(mask, scene.y, 0).astype(np.complex64)
+        fast_cfg = SBLConfig.paper_hybrid_uamp(
+            device="cpu", uamp_backend="torch",
+            uamp_polish_size=32, uamp_polish_iterations=2,
+            uamp_polish_residual_exchange_size=0,
+            uamp_polish_fixed_budget_diagnostics=False,
+            uamp_polish_compute_truncation_diagnostic=False)
+        stable_cfg = SBLConfig.realtime_hybrid_uamp(
+            device="cpu", uamp_backend="torch", uamp_polish_size=32,
+            uamp_polish_iterations=3, uamp_polish_residual_exchange_size=12,
+            uamp_polish_fixed_budget_diagnostics=True,
+            uamp_polish_compute_truncation_diagnostic=False)
+        arrays = dict(y=y, mask=mask, x_true=scene.x, grid=np.array(scene.grid),
+                      samples=np.array(scene.samples))
+        m = torch.from_numpy(mask)
+        arrays.update(_result_arrays(fast_sbl(torch.from_numpy(y), scene.grid, config=fast_cfg,
+                                              observation_mask=m), "fast_"))
+        arrays.update(_result_arrays(fast_sbl(torch.from_numpy(y), scene.grid, config=stable_cfg,
+                                              observation_mask=m), "stable_"))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
@@ -200,7 +239,9 @@ if __name__ == "__main__":
         long_cg(root)
     elif which == "masked":
         masked_cg(root)
+        masked_plans(root)
     else:
         main(root)
         long_cg(root)
         masked_cg(root)
+        masked_plans(root)

@@ -14,7 +14,9 @@ from conftest import golden_path
 from oracle import fbme_oracle as fo
 
 CASES = ["plan_32x32_24x24", "plan_40x32_20x16", "plan_64x64_40x40",
-         "plan_96x80_60x48"]
+         "plan_96x80_60x48",
+         # observation masks (fast_sbl(..., observation_mask=), 70% / 60%)
+         "masked_plan_64x64_40x40", "masked_plan_96x80_60x48"]
 
 
 def _rel(a, b):
@@ -26,7 +28,7 @@ def _rel(a, b):
 def test_oracle_matches_reference_golden(case, profile):
     g = np.load(golden_path(case + ".npz"))
     grid = tuple(int(v) for v in g["grid"])
-    out = fo.fbme_plan(g["y"], grid, profile)
+    out = fo.fbme_plan(g["y"], grid, profile, mask=g["mask"] if "mask" in g else None)
     xr = g[profile + "_x"].reshape(-1)
     assert _rel(out["x"], xr) < 1e-4
     assert set(np.flatnonzero(out["x"])) == set(np.flatnonzero(xr))
@@ -67,3 +69,25 @@ def test_planted_mistake_is_caught():
     sweep = fo.uamp_sweep(y, grid, samples, cfg)
     res = fo.polish(y, sweep, grid, samples, cfg)
     assert _rel(res["mean"][0], g["fast_x"].reshape(-1)) > 1e-4
+
+
+def test_full_mask_is_unmasked_plan():
+    g = np.load(golden_path("plan_40x32_20x16.npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    a = fo.fbme_plan(g["y"], grid, "stable")
+    b = fo.fbme_plan(g["y"], grid, "stable", mask=np.ones(g["y"].shape, bool))
+    np.testing.assert_allclose(b["x"], a["x"], rtol=1e-12, atol=1e-14)
+    assert np.array_equal(b["support"], a["support"])
+
+
+def test_masked_gram_apply_is_dense():
+    """The masked Gram pieces against the dense row-restricted matrix."""
+    grid, samples = (12, 10), (7, 6)
+    mask = np.random.default_rng(8).random(samples) < 0.6
+    a = fo.dense_prefix_matrix(grid, samples)[mask.reshape(-1)]
+    sup = np.array([[3, 17, 40, 99]])
+    np.testing.assert_allclose(fo.masked_gram(sup, grid, samples, mask)[0],
+                               a[:, sup[0]].conj().T @ a[:, sup[0]], atol=1e-12)
+    mu = np.array([[1 + 2j, -0.5j, 0.25, 3.0]])
+    np.testing.assert_allclose(fo._gram_apply(mu, sup, grid, samples, mask)[0],
+                               a.conj().T @ (a[:, sup[0]] @ mu[0]), atol=1e-11)
