@@ -389,9 +389,14 @@ __device__ void rank_sort_desc(const uint32_t* key, const int* idx, int k, uint3
   __syncthreads();
 }
 
+// MASKED: some scenes carry an observation mask; their Gram entries come
+// from the mask spectrum at mh + mh_off (the unmasked instantiation keeps
+// the Dirichlet-product Gram with no branch).
+template <bool MASKED>
 __global__ void __launch_bounds__(64)
 k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const float2* __restrict__ bp_all,
-         const double* __restrict__ sc, const double2* __restrict__ dir, const uint32_t* __restrict__ cand_key,
+         const double* __restrict__ sc, const double2* __restrict__ dir, const double2* __restrict__ mh,
+         const uint32_t* __restrict__ cand_key,
          const int* __restrict__ cand_idx, const uint32_t* __restrict__ xcand_key,
          const int* __restrict__ xcand_idx, PolishState* __restrict__ states, PolishResult* __restrict__ results,
          float* __restrict__ out_noise, float* __restrict__ out_dof, float* __restrict__ out_nle,
@@ -432,11 +437,11 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   SceneCtx c;
   c.n1 = s.n1;
   c.n2 = s.n2;
-  c.m_total = s.m1 * s.m2;
+  c.m_total = s.m_obs;
   c.k = s.k;
   c.floor_v = scs[SC_FLOOR];
   c.ye = scs[SC_YE];
-  c.mh = nullptr;
+  c.mh = (MASKED && s.mh_off >= 0) ? mh + s.mh_off : nullptr;
   // Dirichlet tables staged in LDS (the Gram gathers hit them K^2 times per
   // evaluation); grids wider than the staging area read them from global.
   if (s.n1 + s.n2 <= DIR_LDS) {
@@ -764,9 +769,10 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
 
 // |A^H (y - C mu)|^2 over one tile of the grid with the support excluded,
 // reduced to the tile's top `exch` candidates (solver.py:4396-4413).
+template <bool MASKED>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_exchange_score(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-                 const float2* __restrict__ bp_all, const double2* __restrict__ dir,
+                 const float2* __restrict__ bp_all, const double2* __restrict__ dir, const double2* __restrict__ mh,
                  const PolishState* __restrict__ states, uint32_t* __restrict__ xcand_key,
                  int* __restrict__ xcand_idx, int exchange_size) {
   __shared__ float2 d1[1024];
@@ -815,7 +821,13 @@ k_exchange_score(const FbmeScene* __restrict__ scenes, const FbmeWork* __restric
         int i2 = r2 - sk2[k];
         i1 += (i1 < 0) ? s.n1 : 0;
         i2 += (i2 < 0) ? s.n2 : 0;
-        const float2 g = cmul(d1[i1], d2[i2]);
+        float2 g;
+        if (MASKED && s.mh_off >= 0) {
+          const double2 gv = mh[s.mh_off + (long long)i1 * s.n2 + i2];
+          g = make_float2((float)gv.x, (float)gv.y);
+        } else {
+          g = cmul(d1[i1], d2[i2]);
+        }
         const float2 t = cmul(smu[k], g);
         ax -= t.x;
         ay -= t.y;
@@ -1032,7 +1044,7 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
 size_t fbme_polish_result_bytes(void) { return sizeof(PolishResult); }
 
 int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const void* bp, const double* sc,
-                       const void* dir, const uint32_t* cand_key, const int* cand_idx, const uint32_t* xcand_key,
+                       const void* dir, const void* mh, const uint32_t* cand_key, const int* cand_idx, const uint32_t* xcand_key,
                        const int* xcand_idx, void* states, void* results, float* out_noise, float* out_dof,
                        float* out_nle, float* out_trace, long long* out_exchanges, float* out_diag, int* status,
                        const FbmePlanOptions* o, hipStream_t stream) {
@@ -1053,20 +1065,31 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
   a.evidence_noise = o->evidence_noise;
   // One wavefront per scene: the K x K work is latency bound, and with a
   // single wave every barrier is cheap.
-  hipLaunchKernelGGL(k_polish, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
-                     (const float2*)bp, sc, (const double2*)dir, cand_key, cand_idx, xcand_key, xcand_idx,
-                     (PolishState*)states, (PolishResult*)results, out_noise, out_dof, out_nle, out_trace,
-                     out_exchanges, out_diag, status, a);
+  if (mh)
+    hipLaunchKernelGGL(k_polish<true>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
+                       (const float2*)bp, sc, (const double2*)dir, (const double2*)mh, cand_key, cand_idx,
+                       xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
+                       out_nle, out_trace, out_exchanges, out_diag, status, a);
+  else
+    hipLaunchKernelGGL(k_polish<false>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
+                       (const float2*)bp, sc, (const double2*)dir, (const double2*)nullptr, cand_key, cand_idx,
+                       xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
+                       out_nle, out_trace, out_exchanges, out_diag, status, a);
   return (int)hipGetLastError();
 }
 
 int fbme_launch_exchange_score(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* bp,
-                               const void* dir, const void* states, uint32_t* xcand_key, int* xcand_idx,
-                               int exchange_size, hipStream_t stream) {
+                               const void* dir, const void* mh, const void* states, uint32_t* xcand_key,
+                               int* xcand_idx, int exchange_size, hipStream_t stream) {
   if (nwork <= 0) return 0;
-  hipLaunchKernelGGL(k_exchange_score, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                     (const float2*)bp, (const double2*)dir, (const PolishState*)states, xcand_key, xcand_idx,
-                     exchange_size);
+  if (mh)
+    hipLaunchKernelGGL(k_exchange_score<true>, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                       (const float2*)bp, (const double2*)dir, (const double2*)mh, (const PolishState*)states,
+                       xcand_key, xcand_idx, exchange_size);
+  else
+    hipLaunchKernelGGL(k_exchange_score<false>, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
+                       (const float2*)bp, (const double2*)dir, (const double2*)nullptr, (const PolishState*)states,
+                       xcand_key, xcand_idx, exchange_size);
   return (int)hipGetLastError();
 }
 

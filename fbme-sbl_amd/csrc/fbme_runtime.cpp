@@ -27,22 +27,25 @@
 #include "fbme_types.h"
 
 extern "C" {
-int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*, void*,
-                            void*, const double*, double*, int, int, int, float, const void*, hipStream_t);
+int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const unsigned char*,
+                            const void*, void*, void*, const double*, double*, int, int, int, float, const void*,
+                            hipStream_t);
 int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, int, void*, void*, float*, void*, float*,
                     const double*, double*, int, int, float, float, float, float, float, float, uint32_t*, int*,
                     int, uint32_t*, const void*, hipStream_t);
-int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*, void*,
-                            void*, const double*, double*, int, int, float, const void*, hipStream_t);
+int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*,
+                            const unsigned char*, void*, void*, const double*, double*, int, int, float, const void*,
+                            hipStream_t);
 int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
                         float, float, float, float, float, int, hipStream_t);
 size_t fbme_polish_state_bytes(void);
 size_t fbme_polish_result_bytes(void);
-int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, const void*, const uint32_t*,
+int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, const void*, const void*,
+                       const uint32_t*,
                        const int*, const uint32_t*, const int*, void*, void*, float*, float*, float*, float*,
                        long long*, float*, int*, const FbmePlanOptions*, hipStream_t);
 int fbme_launch_exchange_score(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, const void*,
-                               uint32_t*, int*, int, hipStream_t);
+                               const void*, uint32_t*, int*, int, hipStream_t);
 int fbme_launch_output(const FbmeScene*, const FbmeWork*, int, int, const double*, const void*, void*, float*,
                        float*, unsigned char*, float, int, hipStream_t);
 }
@@ -76,6 +79,11 @@ struct fbme_plan {
   // workspace layout
   Region r_scenes, r_col, r_row, r_tile, r_dir, r_tw, r_sc, r_part, r_mid, r_msg, r_pred, r_bp, r_gamma, r_mean, r_var,
       r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result, r_score;
+  // observation masks: spectra of the masked scenes and the scratch of
+  // their computation; the caller's mask bytes (packed like the data)
+  Region r_mh, r_mhwork;
+  int masked = 0;                           // scenes with an observation mask
+  const unsigned char* mask = nullptr;
   size_t workspace_bytes = 0;
   char* ws = nullptr;
   int bound = 0;
@@ -186,6 +194,11 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
 }
 
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
+  return fbme_plan_create_masked(nscenes, shapes, nullptr, opt, err, errlen);
+}
+
+fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed, const FbmePlanOptions* opt,
+                                   char* err, int errlen) {
   auto fail = [&](const std::string& msg) -> fbme_plan* {
     if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", msg.c_str());
     return nullptr;
@@ -221,7 +234,14 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
       return fail("scene " + std::to_string(i) + ": " + e);
     }
     const long long n = (long long)n1 * n2, m = (long long)m1 * m2;
-    s.iters = iteration_budget(n, m, *opt);
+    const long long mo = observed ? observed[i] : m;
+    if (mo < 1 || mo > m) {
+      delete p;
+      return fail("scene " + std::to_string(i) + ": observed samples must be in [1, M1 M2]");
+    }
+    s.m_obs = (int)mo;
+    s.mh_off = -1;
+    s.iters = iteration_budget(n, mo, *opt);
     s.k = (int)std::min<long long>(opt->polish_size, n);
     s.orig = i;
   }
@@ -242,6 +262,15 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
   p->total_grid = go;
   p->total_samp = so;
   p->total_mid = mo;
+  // mask spectra [N1*N2] (double2) of the masked scenes, caller order
+  long long mh_total = 0, mh_work = 0;
+  for (int i = 0; i < nscenes; ++i) {
+    if (sc[i].m_obs == sc[i].m1 * sc[i].m2) continue;
+    sc[i].mh_off = mh_total;
+    mh_total += (long long)sc[i].n1 * sc[i].n2;
+    mh_work = std::max<long long>(mh_work, (long long)sc[i].m1 * sc[i].n2);
+    ++p->masked;
+  }
   // sort by iteration budget, descending (stable -> deterministic)
   std::vector<int> order(nscenes);
   std::iota(order.begin(), order.end(), 0);
@@ -417,6 +446,8 @@ fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOption
     score_cells = std::max<long long>(score_cells, p->scenes[p->row_work[i].scene].grid_off +
                                                        (long long)p->scenes[p->row_work[i].scene].n1 * 256);
   reg(p->r_score, 4 * (size_t)score_cells);
+  reg(p->r_mh, 16 * (size_t)mh_total);
+  reg(p->r_mhwork, 16 * (size_t)mh_work);
   p->workspace_bytes = align_up(off, 256);
   return p;
 }
@@ -484,9 +515,23 @@ int fbme_plan_bind(fbme_plan* p, void* workspace, void* stream) {
   return 0;
 }
 
+int fbme_plan_set_mask(fbme_plan* p, const unsigned char* mask, void* stream) {
+  if (!p || !p->bound || !mask || !p->masked) return FBME_ERR_ARG;
+  char* w = p->ws;
+  for (const FbmeScene& s : p->scenes) {
+    if (s.mh_off < 0) continue;
+    const int rc = fbme_mask_spectrum(stream, s.n1, s.n2, s.m1, s.m2, mask + s.samp_off,
+                                      w + p->r_mh.off + 16 * (size_t)s.mh_off, w + p->r_mhwork.off);
+    if (rc) return rc;
+  }
+  p->mask = mask;
+  return 0;
+}
+
 int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold, void* stream) {
   (void)support_threshold;
   if (!p || !io || !p->bound) return FBME_ERR_ARG;
+  if (p->masked && !p->mask) return FBME_ERR_ARG;
   if (!io->y || !io->x || !io->posterior_variance || !io->precision || !io->noise || !io->dof || !io->nle ||
       !io->trace)
     return FBME_ERR_ARG;
@@ -497,6 +542,8 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   const FbmeWork* roww = (const FbmeWork*)(w + p->r_row.off);
   const FbmeWork* tilew = (const FbmeWork*)(w + p->r_tile.off);
   const void* dir = w + p->r_dir.off;
+  const void* mh = p->masked ? (const void*)(w + p->r_mh.off) : nullptr;
+  const unsigned char* mask = p->masked ? p->mask : nullptr;
   const void* tw = w + p->r_tw.off;
   double* sc = (double*)(w + p->r_sc.off);
   double* part = (double*)(w + p->r_part.off);
@@ -528,11 +575,11 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   const FbmeWork* roww16 = roww + p->row_split;
   const int ncol16 = ncol - p->col_split, nrow16 = nrow - p->row_split;
   auto col_adjoint = [&](int ng, int n16, int mode, int store) {
-    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, pred, msg, mid, sc, part, ps, mode,
-                                    store, o.uamp_damping, tw, st);
+    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, mask, pred, msg, mid, sc, part, ps,
+                                    mode, store, o.uamp_damping, tw, st);
     if (!r)
-      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, pred, msg, mid, sc, part, ps, mode,
-                                  store, o.uamp_damping, tw, st);
+      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, mask, pred, msg, mid, sc, part, ps,
+                                  mode, store, o.uamp_damping, tw, st);
     return r;
   };
   uint32_t* score = p->r_score.bytes ? (uint32_t*)(w + p->r_score.off) : nullptr;
@@ -547,11 +594,11 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
     return r;
   };
   auto col_forward = [&](int ng, int n16, int first) {
-    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, pred, msg, sc, part, ps, first,
-                                    o.uamp_damping, tw, st);
+    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, mask, pred, msg, sc, part, ps,
+                                    first, o.uamp_damping, tw, st);
     if (!r)
-      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, pred, msg, sc, part, ps, first,
-                                  o.uamp_damping, tw, st);
+      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, mask, pred, msg, sc, part, ps,
+                                  first, o.uamp_damping, tw, st);
     return r;
   };
   // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
@@ -572,12 +619,13 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   FBME_TRY(col_adjoint(p->col_split, ncol16, 1, 0));
   FBME_TRY(row(p->row_split, nrow16, 3, ckey, cidx, io->store_uamp ? 1 : 0));
   // Polish.
-  FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
+  FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, mh, ckey, cidx, xkey, xidx, states, results, io->noise,
                               io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
   const bool exch = o.exchange_size > 0 && p->total_xcand > 1;
   if (exch) {
-    FBME_TRY(fbme_launch_exchange_score(scenes, tilew, ntile, ntile, bp, dir, states, xkey, xidx, o.exchange_size, st));
-    FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 1, bp, sc, dir, ckey, cidx, xkey, xidx, states, results, io->noise,
+    FBME_TRY(fbme_launch_exchange_score(scenes, tilew, ntile, ntile, bp, dir, mh, states, xkey, xidx, o.exchange_size,
+                                        st));
+    FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 1, bp, sc, dir, mh, ckey, cidx, xkey, xidx, states, results, io->noise,
                                 io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
   }
   FBME_TRY(fbme_launch_output(scenes, tilew, ntile, ntile, sc, results, io->x, io->posterior_variance, io->precision,
@@ -588,6 +636,8 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
 
 int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* stream) {
   if (!p || !io || !p->bound || !io->y) return FBME_ERR_ARG;
+  if (p->masked && !p->mask) return FBME_ERR_ARG;
+  const unsigned char* mask = p->masked ? p->mask : nullptr;
   hipStream_t st = (hipStream_t)stream;
   char* w = p->ws;
   const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
@@ -613,10 +663,10 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   const int ng_row = p->row_split, n16_row = nrow - p->row_split;
   switch (stage) {
     case FBME_STAGE_COL_ADJOINT:
-      return (fbme_launch_col_adjoint(scenes, colw, ng_col, ng_col, p->ept_col, io->y, pred, msg, mid, sc, part, ps, 1,
-                                      1, o.uamp_damping, tw, st) ||
-              fbme_launch_col_adjoint(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, io->y, pred, msg, mid, sc, part,
-                                      ps, 1, 1, o.uamp_damping, tw, st))
+      return (fbme_launch_col_adjoint(scenes, colw, ng_col, ng_col, p->ept_col, io->y, mask, pred, msg, mid, sc, part,
+                                      ps, 1, 1, o.uamp_damping, tw, st) ||
+              fbme_launch_col_adjoint(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, io->y, mask, pred, msg, mid, sc,
+                                      part, ps, 1, 1, o.uamp_damping, tw, st))
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_ROW:
@@ -629,10 +679,10 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_COL_FORWARD:
-      return (fbme_launch_col_forward(scenes, colw, ng_col, ng_col, p->ept_col, mid, io->y, pred, msg, sc, part, ps, 0,
-                                      o.uamp_damping, tw, st) ||
-              fbme_launch_col_forward(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, mid, io->y, pred, msg, sc, part,
-                                      ps, 0, o.uamp_damping, tw, st))
+      return (fbme_launch_col_forward(scenes, colw, ng_col, ng_col, p->ept_col, mid, io->y, mask, pred, msg, sc, part,
+                                      ps, 0, o.uamp_damping, tw, st) ||
+              fbme_launch_col_forward(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, mid, io->y, mask, pred, msg, sc,
+                                      part, ps, 0, o.uamp_damping, tw, st))
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_OUTPUT:

@@ -8,7 +8,9 @@
 // Per-scene descriptor of a (possibly uneven) batch.  Scenes are stored back
 // to back in every buffer; the offsets below are element offsets.
 //   grid buffers   [N1*N2]   gamma, mean, variance, backprojection, score, x
-//   sample buffers [M1*M2]   data, message, prediction
+//   sample buffers [M1*M2]   data, message, prediction, observation mask
+//                            (one byte per sample; unobserved samples carry
+//                            zero messages and no residual)
 //   mid buffers    [N1*M2]   column-pass <-> row-pass intermediate
 struct FbmeScene {
   int n1, n2, m1, m2;
@@ -26,7 +28,8 @@ struct FbmeScene {
   int xcand_off;            // first exchange candidate slot
   int orig;                 // caller's scene index (per-scene outputs)
   int tw1_off, tw2_off;     // twiddle tables (float2) of lengths n1 and n2
-  int pad;
+  int m_obs;                // observed samples (M1 * M2 without a mask)
+  long long mh_off;         // mask spectrum (double2 [N1*N2]) or -1 without a mask
 };
 
 // Kernel variant carried in the `ept` argument of the pass launchers: 4, 8 or

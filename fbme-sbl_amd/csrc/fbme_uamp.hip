@@ -43,11 +43,14 @@ __device__ __forceinline__ double part_get(const double* part, int pstride, int 
 // ------------------------------------------------------------- column adjoint
 // MODE 0: input is the data y (backprojection); partial sum |y|^2.
 // MODE 1: input is the UAMP message update of (y, pred, msg).
+// With an observation mask, unobserved samples contribute zero data and a
+// zero message (the restricted operator, operators.py:57-75).
 // All global loads of the tile are issued before the first LDS store.
 template <int EPT, int MODE>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-              const float2* __restrict__ y, const float2* __restrict__ pred, float2* __restrict__ msg,
+              const float2* __restrict__ y, const unsigned char* __restrict__ mask,
+              const float2* __restrict__ pred, float2* __restrict__ msg,
               float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
               int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
@@ -64,12 +67,14 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
     const float precision = 1.0f / (dv + noise);
     const float keep = damping, update = 1.0f - damping;
     float2 yv[EPT], pv[EPT], mv[EPT];
+    bool ob[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
       const int e = threadIdx.x + q * FBME_BLOCK;
       if (e < nin) {
         const int i = e / cols, c = e - i * cols;
         const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
+        ob[q] = !mask || mask[gi];
         yv[q] = y[gi];
         if (MODE == 1) {
           pv[q] = pred[gi];
@@ -86,12 +91,13 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
         const int i = e / cols, c = e - i * cols;
         float2 v;
         if (MODE == 0) {
-          v = yv[q];
+          v = ob[q] ? yv[q] : make_float2(0.f, 0.f);
           acc += (double)v.x * v.x + (double)v.y * v.y;
         } else {
           const float2 pseudo = make_float2(pv[q].x - dv * mv[q].x, pv[q].y - dv * mv[q].y);
           const float2 cand = make_float2(precision * (yv[q].x - pseudo.x), precision * (yv[q].y - pseudo.y));
-          v = make_float2(keep * mv[q].x + update * cand.x, keep * mv[q].y + update * cand.y);
+          v = ob[q] ? make_float2(keep * mv[q].x + update * cand.x, keep * mv[q].y + update * cand.y)
+                    : make_float2(0.f, 0.f);
           if (store_msg) msg[s.samp_off + (long long)i * s.m2 + wk.first + c] = v;
         }
         buf[lds_idx(c * stride + i)] = v;
@@ -148,7 +154,7 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
     const int len = s.n2, rows = wk.count, stride = len + 1;
     const int nel = rows * len;
     const long long n_grid = (long long)s.n1 * s.n2;
-    const int n_samp = s.m1 * s.m2;
+    const int n_samp = s.m_obs;
     const long long gbase = s.grid_off + (long long)wk.first * len;
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
@@ -297,7 +303,8 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
 template <int EPT>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-              const float2* __restrict__ mid, const float2* __restrict__ y, float2* __restrict__ pred,
+              const float2* __restrict__ mid, const float2* __restrict__ y,
+              const unsigned char* __restrict__ mask, float2* __restrict__ pred,
               float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
               int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
@@ -336,13 +343,15 @@ k_col_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
         const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
         const float2 p = buf[lds_idx(c * stride + i)];
         pred[gi] = p;
+        const bool ob = !mask || mask[gi];
         const float rx = yv[q].x - p.x, ry = yv[q].y - p.y;
-        acc += (double)rx * rx + (double)ry * ry;
+        if (ob) acc += (double)rx * rx + (double)ry * ry;
         if (first_message) {
           // First message from zero state (triton_kernels.py:66-77 with
           // prediction = message = 0).
-          msg[gi] = make_float2((1.0f - damping) * (precision * yv[q].x),
-                                (1.0f - damping) * (precision * yv[q].y));
+          msg[gi] = ob ? make_float2((1.0f - damping) * (precision * yv[q].x),
+                                     (1.0f - damping) * (precision * yv[q].y))
+                       : make_float2(0.f, 0.f);
         }
       }
     }
@@ -374,7 +383,7 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
   if (active && !active[si]) return;
   const FbmeScene s = scenes[si];
   double* scs = sc + (long long)si * SC_COUNT;
-  const int n_samp = s.m1 * s.m2;
+  const int n_samp = s.m_obs;
   const double n_grid = (double)s.n1 * s.n2;
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (mode == 0) {
@@ -440,27 +449,29 @@ extern "C" {
 int fbme_launch_row16(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*, const double*,
                       double*, int, int, float, float, float, float, float, float, uint32_t*, int*, int, uint32_t*,
                       const void*, hipStream_t);
-int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
-                              const double*, double*, int, int, int, float, const void*, hipStream_t);
-int fbme_launch_col16_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, void*, void*,
-                              const double*, double*, int, int, float, const void*, hipStream_t);
+int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const unsigned char*,
+                              const void*, void*, void*, const double*, double*, int, int, int, float, const void*,
+                              hipStream_t);
+int fbme_launch_col16_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*,
+                              const unsigned char*, void*, void*, const double*, double*, int, int, float,
+                              const void*, hipStream_t);
 
 int fbme_launch_col_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
-                            const void* y, const void* pred, void* msg, void* mid, const double* sc,
-                            double* part, int pstride, int mode, int store_msg, float damping,
+                            const void* y, const unsigned char* mask, const void* pred, void* msg, void* mid,
+                            const double* sc, double* part, int pstride, int mode, int store_msg, float damping,
                             const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   if (ept == FBME_EPT_R16)
-    return fbme_launch_col16_adjoint(scenes, work, nwork, grid, y, pred, msg, mid, sc, part, pstride, mode, store_msg,
-                                     damping, twtab, stream);
+    return fbme_launch_col16_adjoint(scenes, work, nwork, grid, y, mask, pred, msg, mid, sc, part, pstride, mode,
+                                     store_msg, damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     if (mode == 0)
       hipLaunchKernelGGL((k_col_adjoint<E, 0>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                         (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                         (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                          store_msg, damping, (const float2*)twtab);
     else
       hipLaunchKernelGGL((k_col_adjoint<E, 1>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                         (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                         (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                          store_msg, damping, (const float2*)twtab);
   })
   return (int)hipGetLastError();
@@ -494,16 +505,16 @@ int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, in
 }
 
 int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
-                            const void* mid, const void* y, void* pred, void* msg, const double* sc,
-                            double* part, int pstride, int first_message, float damping,
+                            const void* mid, const void* y, const unsigned char* mask, void* pred, void* msg,
+                            const double* sc, double* part, int pstride, int first_message, float damping,
                             const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   if (ept == FBME_EPT_R16)
-    return fbme_launch_col16_forward(scenes, work, nwork, grid, mid, y, pred, msg, sc, part, pstride, first_message,
-                                     damping, twtab, stream);
+    return fbme_launch_col16_forward(scenes, work, nwork, grid, mid, y, mask, pred, msg, sc, part, pstride,
+                                     first_message, damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     hipLaunchKernelGGL((k_col_forward<E>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
-                       (const float2*)mid, (const float2*)y, (float2*)pred, (float2*)msg, sc, part, pstride,
+                       (const float2*)mid, (const float2*)y, mask, (float2*)pred, (float2*)msg, sc, part, pstride,
                        first_message, damping, (const float2*)twtab);
   })
   return (int)hipGetLastError();

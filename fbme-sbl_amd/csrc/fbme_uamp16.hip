@@ -177,7 +177,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const long long gb = s.grid_off + (long long)row * 256;
     const long long mb = s.mid_off + (long long)row * s.m2;
     const long long n_grid = (long long)s.n1 * 256;
-    const int n_samp = s.m1 * s.m2;
+    const int n_samp = s.m_obs;
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float noise = (float)scs[SC_NOISE];
@@ -346,10 +346,13 @@ k_tile_topk16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
 
 // ------------------------------------------------------------- column passes
 // COLS columns per workgroup, 16 lanes per column: lane = c + COLS * t.
+// With an observation mask, unobserved samples contribute zero data, a zero
+// message and no residual.
 template <int COLS, int MODE>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-                const float2* __restrict__ y, const float2* __restrict__ pred, float2* __restrict__ msg,
+                const float2* __restrict__ y, const unsigned char* __restrict__ mask,
+                const float2* __restrict__ pred, float2* __restrict__ msg,
                 float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
@@ -375,15 +378,16 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       v[j] = make_float2(0.f, 0.f);
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
+        const bool ob = !mask || mask[gi];
         const float2 yv = y[gi];
         if (MODE == 0) {
-          v[j] = yv;
-          acc += (double)yv.x * yv.x + (double)yv.y * yv.y;
+          if (ob) v[j] = yv;
+          acc += (double)v[j].x * v[j].x + (double)v[j].y * v[j].y;
         } else {
           const float2 pv = pred[gi], mv = msg[gi];
           const float2 pseudo = make_float2(pv.x - dv * mv.x, pv.y - dv * mv.y);
           const float2 cand = make_float2(precision * (yv.x - pseudo.x), precision * (yv.y - pseudo.y));
-          v[j] = make_float2(keep * mv.x + update * cand.x, keep * mv.y + update * cand.y);
+          if (ob) v[j] = make_float2(keep * mv.x + update * cand.x, keep * mv.y + update * cand.y);
           if (store_msg) msg[gi] = v[j];
         }
       }
@@ -405,7 +409,8 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
 template <int COLS>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-                const float2* __restrict__ mid, const float2* __restrict__ y, float2* __restrict__ pred,
+                const float2* __restrict__ mid, const float2* __restrict__ y,
+                const unsigned char* __restrict__ mask, float2* __restrict__ pred,
                 float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
@@ -443,13 +448,15 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
         pred[gi] = v[n];
+        const bool ob = !mask || mask[gi];
         const float rx = yv[n].x - v[n].x, ry = yv[n].y - v[n].y;
-        acc += (double)rx * rx + (double)ry * ry;
+        if (ob) acc += (double)rx * rx + (double)ry * ry;
         if (first_message) {
           // First message from zero state (triton_kernels.py:66-77 with
           // prediction = message = 0).
-          msg[gi] = make_float2((1.0f - damping) * (precision * yv[n].x),
-                                (1.0f - damping) * (precision * yv[n].y));
+          msg[gi] = ob ? make_float2((1.0f - damping) * (precision * yv[n].x),
+                                     (1.0f - damping) * (precision * yv[n].y))
+                       : make_float2(0.f, 0.f);
         }
       }
     }
@@ -492,28 +499,30 @@ int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, 
 }
 
 int fbme_launch_col16_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* y,
-                              const void* pred, void* msg, void* mid, const double* sc, double* part, int pstride,
-                              int mode, int store_msg, float damping, const void* twtab, hipStream_t stream) {
+                              const unsigned char* mask, const void* pred, void* msg, void* mid, const double* sc,
+                              double* part, int pstride, int mode, int store_msg, float damping, const void* twtab,
+                              hipStream_t stream) {
   if (nwork <= 0) return 0;
   constexpr int C = FBME_R16_COLS;
   if (mode == 0)
     hipLaunchKernelGGL((k_col16_adjoint<C, 0>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                       (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                       (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                        store_msg, damping, (const float2*)twtab);
   else
     hipLaunchKernelGGL((k_col16_adjoint<C, 1>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                       (const float2*)y, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                       (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                        store_msg, damping, (const float2*)twtab);
   return (int)hipGetLastError();
 }
 
 int fbme_launch_col16_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* mid,
-                              const void* y, void* pred, void* msg, const double* sc, double* part, int pstride,
-                              int first_message, float damping, const void* twtab, hipStream_t stream) {
+                              const void* y, const unsigned char* mask, void* pred, void* msg, const double* sc,
+                              double* part, int pstride, int first_message, float damping, const void* twtab,
+                              hipStream_t stream) {
   if (nwork <= 0) return 0;
   constexpr int C = FBME_R16_COLS;
   hipLaunchKernelGGL((k_col16_forward<C>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                     (const float2*)mid, (const float2*)y, (float2*)pred, (float2*)msg, sc, part, pstride,
+                     (const float2*)mid, (const float2*)y, mask, (float2*)pred, (float2*)msg, sc, part, pstride,
                      first_message, damping, (const float2*)twtab);
   return (int)hipGetLastError();
 }

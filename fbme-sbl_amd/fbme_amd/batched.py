@@ -6,7 +6,8 @@ and UAMP budgets in the scene table; scenes whose budget T_u is spent drop
 out of the later sweeps, which is the per-scene convergence mask), and the
 whole solve is captured in one hipGraph.  The pass shapes come from the
 autotuner's table for this workload and GPU (``tune=True`` runs the tuner
-first).  Outputs are plan-owned and overwritten by the next replay, as in
+first).  ``masks`` optionally gives scenes observation masks [M1, M2]
+(None for a fully observed scene).  Outputs are plan-owned and overwritten by the next replay, as in
 the reference's graph plans (torch_sbl/cuda_graph.py:17-19).
 """
 
@@ -24,7 +25,8 @@ class UnevenPlan:
     """Graph-captured solve of a fixed list of (N1, N2, M1, M2) scenes."""
 
     def __init__(self, shapes: Sequence[Sequence[int]], *, device, profile: str = "fast", warmup: int = 2,
-                 tune: bool = False, options: _lib.FbmePlanOptions | None = None) -> None:
+                 tune: bool = False, options: _lib.FbmePlanOptions | None = None,
+                 masks: Sequence | None = None) -> None:
         if profile not in ("fast", "stable"):
             raise ValueError("profile must be 'fast' or 'stable'")
         if warmup < 1:
@@ -41,7 +43,7 @@ class UnevenPlan:
             opt = autotune.tuned_plan_options(self.shapes, stable, self.device, opt)
         self.options = opt
         with torch.cuda.device(self.device):
-            self.plan = NativePlan(self.shapes, opt, self.device)
+            self.plan = NativePlan(self.shapes, opt, self.device, masks)
             self.input = torch.zeros(self.plan.total_samples, dtype=torch.complex64, device=self.device)
             side = torch.cuda.Stream(device=self.device)
             side.wait_stream(torch.cuda.current_stream(self.device))

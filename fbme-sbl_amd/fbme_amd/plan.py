@@ -44,10 +44,16 @@ def options_from_fields(stable: bool, **fields) -> _lib.FbmePlanOptions:
 
 
 class NativePlan:
-    """Fixed-shape native solve for a list of (N1, N2, M1, M2) scene shapes."""
+    """Fixed-shape native solve for a list of (N1, N2, M1, M2) scene shapes.
+
+    ``masks`` optionally gives each scene a boolean observation mask
+    [M1, M2] (None: all samples observed); the plan keeps a packed device
+    copy (operators.py:57-75 restrict the operator to the observed
+    samples)."""
 
     def __init__(self, shapes: Sequence[Sequence[int]],
-                 options: _lib.FbmePlanOptions, device: torch.device) -> None:
+                 options: _lib.FbmePlanOptions, device: torch.device,
+                 masks: Sequence | None = None) -> None:
         lib = _lib.load()
         device = torch.device(device)
         if device.type != "cuda":
@@ -58,8 +64,32 @@ class NativePlan:
         flat = [v for s in self.shapes for v in s]
         arr = (ctypes.c_int * len(flat))(*flat)
         err = ctypes.create_string_buffer(512)
-        handle = lib.fbme_plan_create(len(self.shapes), arr,
-                                      ctypes.byref(options), err, 512)
+        packed = None
+        if masks is not None and any(mk is not None for mk in masks):
+            if len(masks) != len(self.shapes):
+                raise ValueError("one mask (or None) per scene")
+            parts, observed = [], []
+            for i, (mk, (_, _, m1, m2)) in enumerate(zip(masks, self.shapes)):
+                if mk is None:
+                    mk = torch.ones((m1, m2), dtype=torch.bool)
+                mk = torch.as_tensor(mk, dtype=torch.bool).cpu()
+                if tuple(mk.shape) != (m1, m2):
+                    raise ValueError(f"scene {i}: observation_mask must have shape {(m1, m2)}")
+                count = int(mk.sum())
+                if count == 0:
+                    raise ValueError(f"scene {i}: observation_mask must retain at least one sample")
+                parts.append(mk.reshape(-1).to(torch.uint8))
+                observed.append(count)
+            if all(c == s[2] * s[3] for c, s in zip(observed, self.shapes)):
+                parts = None        # every sample observed: the unmasked plan
+            packed = torch.cat(parts) if parts else None
+        if packed is not None:
+            obs = (ctypes.c_int * len(observed))(*observed)
+            handle = lib.fbme_plan_create_masked(len(self.shapes), arr, obs,
+                                                 ctypes.byref(options), err, 512)
+        else:
+            handle = lib.fbme_plan_create(len(self.shapes), arr,
+                                          ctypes.byref(options), err, 512)
         if not handle:
             raise ValueError(err.value.decode() or "fbme_plan_create failed")
         self._handle = handle
@@ -86,6 +116,12 @@ class NativePlan:
             _lib.check(lib.fbme_plan_bind(handle, self.workspace.data_ptr(),
                                           ctypes.c_void_p(stream)),
                        "fbme_plan_bind")
+            self.mask = None
+            if packed is not None:
+                self.mask = packed.to(device)
+                _lib.check(lib.fbme_plan_set_mask(handle, self.mask.data_ptr(),
+                                                  ctypes.c_void_p(stream)),
+                           "fbme_plan_set_mask")
             g, b = self.total_grid, self.nscenes
             self.out = PlanOutputs(
                 x=torch.zeros(g, dtype=torch.complex64, device=device),

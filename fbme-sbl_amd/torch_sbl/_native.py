@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import math
 from collections import OrderedDict
-from math import ceil, prod
+from math import prod
 
 import torch
 
@@ -78,10 +78,12 @@ def check_supported(cfg: SBLConfig) -> None:
             "options: " + ", ".join(bad))
 
 
-def plan_options(cfg: SBLConfig, grid_shape, sample_shape) -> _lib.FbmePlanOptions:
-    """FbmePlanOptions equivalent of a UAMP-mode SBLConfig."""
+def plan_options(cfg: SBLConfig, grid_shape, sample_shape, observed: int | None = None
+                 ) -> _lib.FbmePlanOptions:
+    """FbmePlanOptions equivalent of a UAMP-mode SBLConfig (`observed`: the
+    sample count an observation mask keeps)."""
     n = prod(grid_shape)
-    m = prod(sample_shape)
+    m = prod(sample_shape) if observed is None else int(observed)
     damping = cfg.uamp_damping
     if cfg.uamp_adaptive_damping:
         # solver.py:1180-1186
@@ -117,24 +119,24 @@ def options_key(opt: _lib.FbmePlanOptions) -> tuple:
     return tuple(getattr(opt, name) for name, _ in opt._fields_)
 
 
-def uamp_budget(cfg: SBLConfig, n: int, m: int) -> int:
-    minimum = max(cfg.min_iter, cfg.uamp_min_iter)
-    budget = ceil(cfg.uamp_iteration_offset + cfg.uamp_iteration_scale * n / m)
-    return min(cfg.max_iter, max(minimum, budget))
-
-
 def tuned(shapes, opt: _lib.FbmePlanOptions, device) -> _lib.FbmePlanOptions:
     """opt with the autotuned FFT pass shapes of this workload on this GPU
     (unchanged when the workload was never tuned)."""
     return autotune.tuned_plan_options(shapes, opt.exchange_size > 0, device, opt)
 
 
-def cached_plan(shapes, opt, device) -> NativePlan:
+def cached_plan(shapes, opt, device, mask=None) -> NativePlan:
+    """Plan of these shapes and options (one observation mask, shared by
+    every scene, or None), built once per key."""
     opt = tuned(shapes, opt, device)
-    key = (tuple(shapes), options_key(opt), str(device))
+    mkey = None
+    if mask is not None:
+        mask = torch.as_tensor(mask, dtype=torch.bool).cpu()
+        mkey = (tuple(mask.shape), mask.numpy().tobytes())
+    key = (tuple(shapes), options_key(opt), str(device), mkey)
     plan = _PLAN_CACHE.get(key)
     if plan is None:
-        plan = NativePlan(shapes, opt, device)
+        plan = NativePlan(shapes, opt, device, None if mask is None else [mask] * len(shapes))
         _PLAN_CACHE[key] = plan
         while len(_PLAN_CACHE) > _PLAN_CACHE_SIZE:
             _PLAN_CACHE.popitem(last=False)
@@ -186,7 +188,8 @@ class ResultViews:
         else:
             self.mask = None
             self.threshold = None
-        self.iterations = uamp_budget(cfg, n, plan.shapes[0][2] * plan.shapes[0][3])
+        # T_u of the plan's first scene (N / observed samples, solver.py:1296-1309)
+        self.iterations = int(plan.scene_info[0][2])
         self.k = int(plan.options.polish_size)
 
     def refresh(self) -> None:

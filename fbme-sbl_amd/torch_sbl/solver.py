@@ -153,18 +153,14 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
         from . import pcg
         return pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
                           batch_shape=batch_shape, observation_mask=mask)
-    if mask is not None:
-        raise NotImplementedError(
-            "observation_mask is implemented for mode='cg'; the UAMP plans "
-            "run on full prefix blocks.")
     if cfg.uamp_backend == "torch":
         raise NotImplementedError(
             "uamp_backend='torch' selects the reference's PyTorch UAMP; this "
             "build always runs the HIP kernels.")
     _native.check_supported(cfg)
-    opt = _native.plan_options(cfg, grid, samples)
+    opt = _native.plan_options(cfg, grid, samples, None if mask is None else m)
     shapes = [(grid[0], grid[1], samples[0], samples[1])] * batch
-    plan = _native.cached_plan(shapes, opt, device)
+    plan = _native.cached_plan(shapes, opt, device, mask)
     views = _native.ResultViews(plan, cfg, batch_shape, grid)
     with torch.cuda.device(device):
         plan.run(y.reshape(-1).contiguous())

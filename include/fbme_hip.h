@@ -106,6 +106,20 @@ void fbme_default_options(FbmePlanOptions* options, int stable);
  * returns NULL and writes a message into err. */
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* options, char* err,
                             int errlen);
+/* fbme_plan_create with observation masks: observed[i] is the number of
+ * samples scene i keeps (M1 M2 for an unmasked scene).  The UAMP budget
+ * uses N / observed (solver.py:1296-1309 on the restricted operator); the
+ * mask bytes themselves are attached after fbme_plan_bind by
+ * fbme_plan_set_mask.  The reference builds the restricted operator per
+ * call (solver.py:986-1024, operators.py:57-75). */
+fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed,
+                                   const FbmePlanOptions* options, char* err, int errlen);
+/* Attach the observation masks of a masked plan: `mask` is a device buffer
+ * of FBME_Q_TOTAL_SAMPLES bytes packed like the data (nonzero = observed),
+ * read by every later run, so it must stay allocated and unchanged while
+ * the plan runs.  Also computes each masked scene's mask spectrum into the
+ * workspace (enqueued on `stream`; call before graph capture). */
+int fbme_plan_set_mask(fbme_plan* plan, const unsigned char* mask, void* stream);
 void fbme_plan_destroy(fbme_plan* plan);
 long long fbme_plan_query(const fbme_plan* plan, int what);
 /* info[0..7] = grid offset, sample offset, T_u, K, radix codes (2),

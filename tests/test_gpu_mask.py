@@ -1,5 +1,6 @@
-"""Observation masks on the general solver (sbl_2d(..., observation_mask=),
-mode='cg') against the float64 oracle and the reference's golden runs.
+"""Observation masks (sbl_2d / fast_sbl(..., observation_mask=)) on the
+general solver (mode='cg') and the native UAMP plans, against the float64
+oracles and the reference's golden runs.
 
 The GPU path keeps full-length vectors with zeros on the unobserved samples
 (the reference gathers the observed rows, operators.py:57-75 and 473-496).
@@ -32,6 +33,11 @@ def _mask(seed, samples, keep=0.6):
 
 def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def scenes_mod():
+    from fbme_amd import scenes
+    return scenes
 
 
 def _solver(device, grid, samples, mask, batch=2, seed=0):
@@ -193,6 +199,106 @@ def test_full_mask_is_unmasked_and_validation(device):
         sbl_2d(y, (32, 32), config=cfg, observation_mask=torch.ones(24, 23, dtype=torch.bool))
     with pytest.raises(ValueError):
         sbl_2d(y, (32, 32), config=cfg, observation_mask=torch.zeros(24, 24, dtype=torch.bool))
-    with pytest.raises(NotImplementedError):
-        sbl_2d(y, (32, 32), config=SBLConfig(mode="uamp", device=str(device)),
-               observation_mask=torch.from_numpy(_mask(5, (24, 24))))
+
+
+# ---------------------------------------------------------------- UAMP plans
+def _plan_cfg(profile, device):
+    from torch_sbl import SBLConfig
+    if profile == "fast":
+        return SBLConfig.paper_hybrid_uamp(
+            device=str(device), uamp_polish_size=32, uamp_polish_iterations=2,
+            uamp_polish_residual_exchange_size=0, uamp_polish_fixed_budget_diagnostics=False,
+            uamp_polish_compute_truncation_diagnostic=False)
+    return SBLConfig.realtime_hybrid_uamp(
+        device=str(device), uamp_polish_size=32, uamp_polish_iterations=3,
+        uamp_polish_residual_exchange_size=12, uamp_polish_fixed_budget_diagnostics=True,
+        uamp_polish_compute_truncation_diagnostic=False)
+
+
+@pytest.mark.parametrize("case", ["masked_plan_64x64_40x40", "masked_plan_96x80_60x48"])
+@pytest.mark.parametrize("profile", ["fast", "stable"])
+def test_masked_plan_vs_golden_and_oracle(device, case, profile):
+    """fast_sbl(..., observation_mask=) on the native plan against the
+    reference's masked golden run and the float64 oracle, to the unmasked
+    plans' bar (relative L2 1e-4, identical supports and support mask)."""
+    from oracle import fbme_oracle as fo
+    from torch_sbl import fast_sbl
+    print("case", case, profile, flush=True)
+    g = np.load(golden_path(case + ".npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    res = fast_sbl(torch.from_numpy(g["y"]).to(device), grid, config=_plan_cfg(profile, device),
+                   observation_mask=torch.from_numpy(g["mask"]))
+    x = res.x.cpu().numpy().reshape(-1)
+    orc = fo.fbme_plan(g["y"], grid, profile, mask=g["mask"])
+    xr = g[profile + "_x"].reshape(-1)
+    assert _rel(x, orc["x"]) < 1e-4
+    assert _rel(x, xr) < 1e-4
+    assert set(np.flatnonzero(x)) == set(np.flatnonzero(orc["x"]))
+    assert np.array_equal(res.support_mask.cpu().numpy().reshape(-1), g[profile + "_support_mask"].reshape(-1))
+    assert abs(float(res.noise_variance) / float(orc["noise_variance"]) - 1) < 1e-4
+    assert abs(float(res.effective_dof) - float(orc["effective_dof"])) < 1e-3
+    assert _rel(res.posterior_variance.cpu().numpy().reshape(-1), orc["posterior_variance"]) < 1e-4
+    np.testing.assert_allclose(res.evidence_trace.cpu().numpy().reshape(-1), orc["evidence_trace"],
+                               rtol=1e-5, atol=1e-2)
+    assert res.iterations == int(g[profile + "_iterations"])
+    if profile == "stable":
+        assert int(res.support_exchanges_accepted) == int(orc["support_exchanges_accepted"])
+        assert abs(float(res.fixed_budget_gamma_residual) - float(orc["fixed_budget_gamma_residual"])) < 1e-4
+        assert bool(res.fixed_budget_monotone) == bool(orc["fixed_budget_monotone"])
+
+
+@pytest.mark.parametrize("profile", ["fast", "stable"])
+def test_masked_plan_256_register_kernels_vs_oracle(device, profile):
+    """256 x 256 grid, 160 x 160 prefix with 60% observed (the register
+    16 x 16 FFT passes with the mask)."""
+    from oracle import fbme_oracle as fo
+    from torch_sbl import fast_sbl
+    grid, samples = (256, 256), (160, 160)
+    mask = _mask(9, samples)
+    sc = scenes_mod().ellipse_scene(1400, grid, samples, k=300, snr_db=10.0)
+    res = fast_sbl(torch.from_numpy(sc.y).to(device), grid, config=_plan_cfg(profile, device),
+                   observation_mask=torch.from_numpy(mask))
+    x = res.x.cpu().numpy().reshape(-1)
+    orc = fo.fbme_plan(sc.y, grid, profile, mask=mask)
+    print("masked 256 plan", profile, _rel(x, orc["x"]), res.iterations, flush=True)
+    assert res.iterations == orc["iterations"]
+    assert _rel(x, orc["x"]) < 1e-4
+    assert np.array_equal(res.support_mask.cpu().numpy().reshape(-1), orc["support_mask"])
+    assert abs(float(res.noise_variance) / float(orc["noise_variance"]) - 1) < 1e-4
+
+
+def test_uneven_masked_plan_equals_single_plans(device):
+    """Masked and unmasked scenes of different shapes in one graph-captured
+    plan give the single-scene plans' results exactly."""
+    from fbme_amd.batched import UnevenPlan
+    from fbme_amd.plan import NativePlan
+    sc = scenes_mod()
+    shapes = [(64, 64, 40, 40), (96, 80, 60, 48), (256, 256, 160, 160), (64, 64, 40, 40)]
+    masks = [_mask(10, (40, 40)), None, _mask(11, (160, 160)), _mask(12, (40, 40), keep=0.8)]
+    ys = [sc.uniform_scene(5000 + i, s[:2], s[2:], k=15, snr_db=15.0).y.astype(np.complex64)
+          for i, s in enumerate(shapes)]
+    plan = UnevenPlan(shapes, device=device, profile="stable", masks=masks, warmup=1)
+    plan.replay([torch.from_numpy(y).to(device) for y in ys], synchronize=True)
+    opt = plan.options
+    for i, (shape, mk, y) in enumerate(zip(shapes, masks, ys)):
+        single = NativePlan([shape], opt, device, [mk])
+        single.run(torch.from_numpy(y.reshape(-1)).to(device))
+        torch.cuda.synchronize()
+        got = plan.scene(i)
+        assert torch.equal(got["x"].reshape(-1), single.out.x), i
+        assert torch.equal(got["noise_variance"], single.out.noise[0]), i
+    # unobserved data values are ignored
+    ys[0] = np.where(masks[0], ys[0], 1e3).astype(np.complex64)
+    before = plan.scene(0)["x"].clone()
+    plan.replay([torch.from_numpy(y).to(device) for y in ys], synchronize=True)
+    assert torch.equal(plan.scene(0)["x"], before)
+
+
+def test_full_mask_plan_is_unmasked_plan(device):
+    from torch_sbl import fast_sbl
+    g = np.load(golden_path("plan_64x64_40x40.npz"))
+    y = torch.from_numpy(g["y"]).to(device)
+    a = fast_sbl(y, (64, 64), config=_plan_cfg("stable", device))
+    b = fast_sbl(y, (64, 64), config=_plan_cfg("stable", device),
+                 observation_mask=torch.ones(40, 40, dtype=torch.bool))
+    assert torch.equal(a.x, b.x)
