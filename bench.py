@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--skip-pcg", action="store_true")
     ap.add_argument("--pcg-batch", type=int, default=8)
     ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--skip-masked", action="store_true")
     ap.add_argument("--learned-scenes", type=int, default=16)
     return ap.parse_args()
 
@@ -212,6 +213,70 @@ def uneven_workload(device, reps=20):
                      "options": {"fft_kernel": plan.options.fft_kernel, "row_lines": plan.options.row_lines,
                                  "col_lines": plan.options.col_lines}}
         del plan
+    return out
+
+
+def masked_workload(device, batch, reps=20):
+    """Observation masks (60 % of the 160 x 160 samples observed, one seeded
+    mask): the batched FBME-Fast plan over the bench's scenes in one
+    graph-captured replay, and the general solver's PCG iteration at 8
+    scenes, each against the same work without the mask."""
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from fbme_amd.batched import UnevenPlan
+    from torch_sbl import SBLConfig, pcg
+    grid, samples = (256, 256), (160, 160)
+    mask = np.random.default_rng(60).random(samples) < 0.6
+    shapes = [(256, 256, 160, 160)] * len(batch)
+    packed = torch.from_numpy(np.stack([np.where(mask, s.y, 0) for s in batch]).astype(np.complex64)
+                              .reshape(-1)).to(device)
+    out = {"workload": f"{len(batch)} scenes 256x256/160x160 K=300 10 dB, FBME-Fast, mask keeps "
+                       f"{int(mask.sum())} of {mask.size} samples; PCG iteration at 8 scenes x 17 lanes"}
+    for name, masks in (("masked", [mask] * len(batch)), ("unmasked", None)):
+        plan = UnevenPlan(shapes, device=device, masks=masks)
+        plan.replay(packed)
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            plan.replay()
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / reps
+        nm = [scenes.nmse_db(plan.scene(i)["x"].reshape(-1).cpu().numpy(), batch[i].x.reshape(-1))
+              for i in range(32)]
+        out["plan_" + name] = {"ms_per_replay": ms, "scenes_per_s": len(batch) / (ms / 1e3),
+                               "nmse_db_first_32": float(np.mean(nm))}
+        del plan
+    y = torch.from_numpy(np.stack([np.where(mask, s.y, 0) for s in batch[:8]]).astype(np.complex64)).to(device)
+    cfg = SBLConfig(mode="cg", device=str(device), hutchinson_probes=16, auto_probe_scaling=False,
+                    cg_tolerance=1e-4, max_iter=3, min_iter=1, tol=0.0)
+    for name, mk in (("masked", torch.from_numpy(mask)), ("unmasked", None)):
+        s = pcg._Solver(y, grid, cfg, None, mk)
+        s.init()
+        s.posterior(1e-3)
+        s.em(update=True)
+        s.grid_sums(with_mean=True)
+        s.setup_operators()
+        s.d.copy_(s.rhs)
+        s.res.copy_(s.rhs)
+        s.active.fill_(1)
+        s.rho.fill_(1.0)
+        for _ in range(3):
+            s._iter_first()
+            s._iter_second()
+        torch.cuda.synchronize()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(reps):
+            s._iter_first()
+            s._iter_second()
+        stop.record()
+        torch.cuda.synchronize()
+        out["pcg_iteration_" + name] = {"ms": start.elapsed_time(stop) / reps}
+        del s
     return out
 
 
@@ -423,6 +488,11 @@ def main():
     learned_cmp = None
     if not args.skip_learned:
         learned_cmp = learned_comparison(device, args.learned_scenes)
+    masked = None
+    if not args.skip_masked:
+        print("[bench] masked workload", file=sys.stderr, flush=True)
+        masked = masked_workload(device, batch)
+        print(f"[bench] masked: {masked}", file=sys.stderr, flush=True)
 
     t_u = native.max_iters
     line = {
@@ -447,6 +517,7 @@ def main():
         "uneven_workload": uneven,
         "pcg_kernels": pcg_cmp,
         "learned_warm_start": learned_cmp,
+        "masked_workload": masked,
         "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),
         "wall_s": wall,

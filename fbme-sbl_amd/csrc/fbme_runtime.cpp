@@ -83,6 +83,7 @@ struct fbme_plan {
   // their computation; the caller's mask bytes (packed like the data)
   Region r_mh, r_mhwork;
   int masked = 0;                           // scenes with an observation mask
+  std::vector<int> mh_leaders;              // caller indices owning a spectrum
   const unsigned char* mask = nullptr;
   size_t workspace_bytes = 0;
   char* ws = nullptr;
@@ -194,11 +195,11 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
 }
 
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
-  return fbme_plan_create_masked(nscenes, shapes, nullptr, opt, err, errlen);
+  return fbme_plan_create_masked(nscenes, shapes, nullptr, nullptr, opt, err, errlen);
 }
 
-fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed, const FbmePlanOptions* opt,
-                                   char* err, int errlen) {
+fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed, const int* mask_group,
+                                   const FbmePlanOptions* opt, char* err, int errlen) {
   auto fail = [&](const std::string& msg) -> fbme_plan* {
     if (err && errlen > 0) std::snprintf(err, (size_t)errlen, "%s", msg.c_str());
     return nullptr;
@@ -262,14 +263,32 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   p->total_grid = go;
   p->total_samp = so;
   p->total_mid = mo;
-  // mask spectra [N1*N2] (double2) of the masked scenes, caller order
+  // mask spectra [N1*N2] (double2) of the masked scenes, caller order; a
+  // scene whose mask_group names an earlier scene of the same shape shares
+  // that scene's mask and spectrum
   long long mh_total = 0, mh_work = 0;
   for (int i = 0; i < nscenes; ++i) {
     if (sc[i].m_obs == sc[i].m1 * sc[i].m2) continue;
+    ++p->masked;
+    const int g = mask_group ? mask_group[i] : i;
+    if (g < 0 || g > i) {
+      delete p;
+      return fail("scene " + std::to_string(i) + ": mask_group must name this or an earlier scene");
+    }
+    if (g != i) {
+      const FbmeScene& a = sc[g];
+      if (a.n1 != sc[i].n1 || a.n2 != sc[i].n2 || a.m1 != sc[i].m1 || a.m2 != sc[i].m2 ||
+          a.m_obs != sc[i].m_obs || a.mh_off < 0) {
+        delete p;
+        return fail("scene " + std::to_string(i) + ": mask_group scene differs in shape or observed count");
+      }
+      sc[i].mh_off = a.mh_off;
+      continue;
+    }
     sc[i].mh_off = mh_total;
+    p->mh_leaders.push_back(i);
     mh_total += (long long)sc[i].n1 * sc[i].n2;
     mh_work = std::max<long long>(mh_work, (long long)sc[i].m1 * sc[i].n2);
-    ++p->masked;
   }
   // sort by iteration budget, descending (stable -> deterministic)
   std::vector<int> order(nscenes);
@@ -519,7 +538,8 @@ int fbme_plan_set_mask(fbme_plan* p, const unsigned char* mask, void* stream) {
   if (!p || !p->bound || !mask || !p->masked) return FBME_ERR_ARG;
   char* w = p->ws;
   for (const FbmeScene& s : p->scenes) {
-    if (s.mh_off < 0) continue;
+    if (s.mh_off < 0 || std::find(p->mh_leaders.begin(), p->mh_leaders.end(), s.orig) == p->mh_leaders.end())
+      continue;
     const int rc = fbme_mask_spectrum(stream, s.n1, s.n2, s.m1, s.m2, mask + s.samp_off,
                                       w + p->r_mh.off + 16 * (size_t)s.mh_off, w + p->r_mhwork.off);
     if (rc) return rc;

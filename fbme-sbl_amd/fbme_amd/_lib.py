@@ -71,7 +71,8 @@ _SIGNATURES = {
     "fbme_plan_create": (c_void_p, [c_int, POINTER(c_int),
                                     POINTER(FbmePlanOptions), c_char_p, c_int]),
     "fbme_plan_create_masked": (c_void_p, [c_int, POINTER(c_int), POINTER(c_int),
-                                           POINTER(FbmePlanOptions), c_char_p, c_int]),
+                                           POINTER(c_int), POINTER(FbmePlanOptions),
+                                           c_char_p, c_int]),
     "fbme_plan_set_mask": (c_int, [c_void_p, c_void_p, c_void_p]),
     "fbme_plan_destroy": (None, [c_void_p]),
     "fbme_plan_query": (c_longlong, [c_void_p, c_int]),

@@ -68,7 +68,7 @@ class NativePlan:
         if masks is not None and any(mk is not None for mk in masks):
             if len(masks) != len(self.shapes):
                 raise ValueError("one mask (or None) per scene")
-            parts, observed = [], []
+            parts, observed, group, seen = [], [], [], {}
             for i, (mk, (_, _, m1, m2)) in enumerate(zip(masks, self.shapes)):
                 if mk is None:
                     mk = torch.ones((m1, m2), dtype=torch.bool)
@@ -80,12 +80,16 @@ class NativePlan:
                     raise ValueError(f"scene {i}: observation_mask must retain at least one sample")
                 parts.append(mk.reshape(-1).to(torch.uint8))
                 observed.append(count)
+                # scenes repeating a mask share its spectrum
+                key = (self.shapes[i], mk.numpy().tobytes())
+                group.append(seen.setdefault(key, i))
             if all(c == s[2] * s[3] for c, s in zip(observed, self.shapes)):
                 parts = None        # every sample observed: the unmasked plan
             packed = torch.cat(parts) if parts else None
         if packed is not None:
             obs = (ctypes.c_int * len(observed))(*observed)
-            handle = lib.fbme_plan_create_masked(len(self.shapes), arr, obs,
+            grp = (ctypes.c_int * len(group))(*group)
+            handle = lib.fbme_plan_create_masked(len(self.shapes), arr, obs, grp,
                                                  ctypes.byref(options), err, 512)
         else:
             handle = lib.fbme_plan_create(len(self.shapes), arr,

@@ -107,12 +107,14 @@ void fbme_default_options(FbmePlanOptions* options, int stable);
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* options, char* err,
                             int errlen);
 /* fbme_plan_create with observation masks: observed[i] is the number of
- * samples scene i keeps (M1 M2 for an unmasked scene).  The UAMP budget
- * uses N / observed (solver.py:1296-1309 on the restricted operator); the
- * mask bytes themselves are attached after fbme_plan_bind by
- * fbme_plan_set_mask.  The reference builds the restricted operator per
+ * samples scene i keeps (M1 M2 for an unmasked scene).  mask_group[i] (or
+ * NULL: every scene its own) names this or an earlier scene of the same
+ * shape whose mask scene i repeats, so the two share one mask spectrum.
+ * The UAMP budget uses N / observed (solver.py:1296-1309 on the restricted
+ * operator); the mask bytes themselves are attached after fbme_plan_bind
+ * by fbme_plan_set_mask.  The reference builds the restricted operator per
  * call (solver.py:986-1024, operators.py:57-75). */
-fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed,
+fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* observed, const int* mask_group,
                                    const FbmePlanOptions* options, char* err, int errlen);
 /* Attach the observation masks of a masked plan: `mask` is a device buffer
  * of FBME_Q_TOTAL_SAMPLES bytes packed like the data (nonzero = observed),

@@ -273,8 +273,9 @@ def test_uneven_masked_plan_equals_single_plans(device):
     from fbme_amd.batched import UnevenPlan
     from fbme_amd.plan import NativePlan
     sc = scenes_mod()
-    shapes = [(64, 64, 40, 40), (96, 80, 60, 48), (256, 256, 160, 160), (64, 64, 40, 40)]
-    masks = [_mask(10, (40, 40)), None, _mask(11, (160, 160)), _mask(12, (40, 40), keep=0.8)]
+    shapes = [(64, 64, 40, 40), (96, 80, 60, 48), (256, 256, 160, 160), (64, 64, 40, 40), (64, 64, 40, 40)]
+    # the last scene repeats the first scene's mask (one shared mask spectrum)
+    masks = [_mask(10, (40, 40)), None, _mask(11, (160, 160)), _mask(12, (40, 40), keep=0.8), _mask(10, (40, 40))]
     ys = [sc.uniform_scene(5000 + i, s[:2], s[2:], k=15, snr_db=15.0).y.astype(np.complex64)
           for i, s in enumerate(shapes)]
     plan = UnevenPlan(shapes, device=device, profile="stable", masks=masks, warmup=1)
