@@ -127,6 +127,22 @@ def clustered_scene(seed: int, grid=(512, 512), samples=(384, 256), k=1200,
     return _observe(rng, x.reshape(grid), samples, snr_db, seed, grid)
 
 
+def joint_scenes(seed: int, count=3, grid=(32, 32), samples=(24, 24), k=12,
+                 snr_db=20.0) -> list[Scene]:
+    """`count` snapshots sharing one support of K uniform positions, each with
+    its own CN(0, 1) amplitudes and noise (the multiple-measurement-vector
+    model of joint sparsity)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = grid[0] * grid[1]
+    pos = rng.choice(n, size=min(k, n), replace=False)
+    out = []
+    for i in range(count):
+        x = np.zeros(n, dtype=np.complex128)
+        x[pos] = np.sqrt(0.5) * (rng.standard_normal(pos.size) + 1j * rng.standard_normal(pos.size))
+        out.append(_observe(rng, x.reshape(grid), samples, snr_db, seed + i, grid))
+    return out
+
+
 def uneven_batch(seed: int, count=64, grids=(64, 128, 192, 256),
                  ratio=(0.3, 0.8), k_fraction=(0.005, 0.03),
                  snr_db=(5.0, 25.0)) -> list[Scene]:

@@ -17,11 +17,17 @@ observed samples only: data, probes, residuals and the preconditioner's
 factor columns live on those rows, and every "M" of the loop is their
 count.
 
+Joint sparsity (joint=True, solver.py:1090-1099, 1157-1159, 1653-1666,
+1769-1774, 3039-3153) shares one variance per cell across the snapshots
+and updates it by the M-SBL EM rule; with shared noise the posterior is one
+PCG solve whose right-hand sides are the B snapshots and the P probes.
+
 Pins: tests/test_oracle_cg.py (PCG against a dense solve, Hutchinson
 unbiasedness against the exact diagonal, preconditioner against the dense
 Woodbury inverse, the masked operator against the row-restricted dense
-matrix, end-to-end against the reference's golden runs with and without
-a mask).
+matrix, the shared multi-right-hand-side posterior against the dense
+solve, end-to-end against the reference's golden runs with and without a
+mask and with joint sparsity).
 """
 
 from __future__ import annotations
@@ -302,10 +308,33 @@ def active_posterior(y, gamma, noise, sup, op: Operator, eps, tiny):
     return mean, var, ddiag
 
 
+def _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg, tol, eps, tiny):
+    """One system for all snapshots (shared gamma and noise): the B data
+    vectors and the P probes are the right-hand sides of one PCG solve
+    (_shared_mmv_posterior, solver.py:3039-3153)."""
+    b = y.shape[0]
+    rhs = np.concatenate([y.T[None], probes], axis=2)           # [1, M, B + P]
+    sol, nit = pcg(rhs, gamma[:1], noise[:1], op, x0=sol if cfg.cg_warm_start else None,
+                   tol=tol, max_iter=cfg.cg_max_iter, check=cfg.cg_check_interval,
+                   rank=cfg.cg_preconditioner_rank, fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
+    lifted = op.adj(sol)                                        # [1, N, B + P]
+    dd1 = np.maximum(np.mean(np.real(lifted[:, :, b:] * np.conj(lprobes)), 2), 0.0)   # [1, N]
+    mean = gamma[:1] * lifted[0, :, :b].T                       # [B, N]
+    var1 = np.maximum(gamma[:1] - gamma[:1] ** 2 * dd1, 0.0)
+    return mean, np.repeat(var1, b, 0), np.repeat(dd1, b, 0), sol, nit
+
+
 def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
-           tiny=F32_TINY, trace=None, mask=None) -> dict:
+           tiny=F32_TINY, trace=None, mask=None, joint=False, joint_noise=True) -> dict:
     """Default ``sbl_2d`` in mode='cg' on [B, M1, M2] (or [M1, M2]) data,
-    optionally on the samples an observation mask [M1, M2] keeps."""
+    optionally on the samples an observation mask [M1, M2] keeps.
+
+    joint: joint sparsity (M-SBL, solver.py:1090-1099, 1157-1159,
+    1653-1666, 1769-1774): one variance per grid cell for every snapshot,
+    initialised to the snapshots' mean, floored at the largest floor and
+    updated by the EM rule on the snapshots' mean second moment; with
+    joint_noise the noise variance is shared too and the posterior is one
+    multi-right-hand-side solve."""
     cfg = cfg or CGConfig()
     data = np.asarray(data)
     single = data.ndim == 2
@@ -322,12 +351,22 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     bp = op.adj(y[:, :, None])[:, :, 0]
     energy = np.abs(bp) ** 2
     structured = energy / np.maximum(energy.sum(1, keepdims=True), eps) * power[:, None]
-    gamma = np.maximum(cfg.initialization_blend * structured
-                       + (1 - cfg.initialization_blend) * power[:, None] / n,
-                       floor[:, None])
+    gamma = (cfg.initialization_blend * structured
+             + (1 - cfg.initialization_blend) * power[:, None] / n)
+    shared = joint and joint_noise
+    if joint:
+        if b < 2:
+            raise ValueError("joint_sparsity=True requires at least two leading batch items.")
+        gamma = np.repeat(gamma.mean(0, keepdims=True), b, 0)
+        prior_floor = np.full(b, floor.max())
+    else:
+        prior_floor = floor
+    gamma = np.maximum(gamma, prior_floor[:, None])
     noise = np.maximum(cfg.initial_noise_fraction * power, floor)
+    if shared:
+        noise = np.full(b, noise.mean())
     pcount = probe_count(n, m, cfg)
-    probes = rademacher_probes(b, m, pcount, cfg.seed).astype(np.complex128)
+    probes = rademacher_probes(1 if shared else b, m, pcount, cfg.seed).astype(np.complex128)
     lprobes = op.adj(probes)
     prev = np.zeros((b, n), np.complex128)
     keep = max(cfg.damping, cfg.cg_damping)
@@ -340,7 +379,7 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     it = 0
     cg_iters = []
     for it in range(1, cfg.max_iter + 1):
-        if (cfg.active_set and b == 1 and sup is None
+        if (cfg.active_set and b == 1 and not joint and sup is None
                 and it >= cfg.active_set_start
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             sup = select_active(gamma, cfg)
@@ -351,6 +390,10 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             mean, var, dd = active_posterior(y, gamma, noise, sup, op, eps, tiny)
             cg_iters.append(0)
             sol = None
+        elif shared:
+            mean, var, dd, sol, nit = _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg,
+                                                        tol, eps, tiny)
+            cg_iters.append(nit)
         else:
             rhs = np.concatenate([y[:, :, None], probes], axis=2)
             sol, nit = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
@@ -365,9 +408,14 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             var = np.maximum(gamma - gamma ** 2 * dd, 0.0)
         rel = project_relevance(gamma, dd, m, eps)
         dof = rel.sum(1)
-        prec = np.clip((rel + cfg.hyper_shape) / (np.abs(mean) ** 2 + cfg.hyper_rate),
-                       1.0 / cfg.max_precision, cfg.max_precision)
-        new_gamma = 1.0 / prec
+        if joint:
+            second = np.mean(np.abs(mean) ** 2 + var, 0, keepdims=True)
+            new_gamma = np.repeat(np.clip((second + cfg.hyper_rate) / (1.0 + cfg.hyper_shape),
+                                          prior_floor.max(), cfg.max_precision), b, 0)
+        else:
+            prec = np.clip((rel + cfg.hyper_shape) / (np.abs(mean) ** 2 + cfg.hyper_rate),
+                           1.0 / cfg.max_precision, cfg.max_precision)
+            new_gamma = 1.0 / prec
         if sup is not None:
             mask = np.zeros_like(gamma, bool)
             mask[np.arange(b)[:, None], sup] = True
@@ -379,7 +427,9 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
         else:
             new_noise = (e + cfg.noise_rate) / np.maximum(m - dof + cfg.noise_shape, eps)
         new_noise = np.maximum(new_noise, floor)
-        gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, floor[:, None])
+        if shared:
+            new_noise = np.full(b, max(new_noise.mean(), floor.max()))
+        gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, prior_floor[:, None])
         noise = keep * noise + (1 - keep) * new_noise
         change = np.max(np.linalg.norm(mean - prev, axis=1)
                         / np.maximum(np.linalg.norm(mean, axis=1), eps))
@@ -397,6 +447,9 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     final_tol = max(cfg.cg_tolerance, tol_floor)
     if sup is not None:
         mean, var, dd = active_posterior(y, gamma, noise, sup, op, eps, tiny)
+    elif shared:
+        mean, var, dd, sol, _ = _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg,
+                                                  final_tol, eps, tiny)
     else:
         rhs = np.concatenate([y[:, :, None], probes], axis=2)
         sol, _ = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,

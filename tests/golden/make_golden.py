@@ -17,6 +17,8 @@ What it pins (functions no closed-form check fixes end to end):
   * ``sbl_2d`` in mode "cg" with an ``observation_mask`` (a single 32x32 /
     24x24 scene and a batch of two 64x64 / 40x40 scenes), and ``fast_sbl``
     under the Fast / Stable numerics with an ``observation_mask``.
+  * ``sbl_2d`` in mode "cg" with ``joint_sparsity=True`` (shared and
+    per-snapshot noise).
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -232,6 +234,37 @@ def masked_plans(reference_root: str) -> None:
         print("wrote", name)
 
 
+def joint_cg(reference_root: str) -> None:
+    """``sbl_2d(mode='cg', joint_sparsity=True)`` on CPU: three 32x32 / 24x24
+    snapshots sharing a 12-atom support (default configuration, shared noise),
+    the same with per-snapshot noise, and four 64x64 / 40x40 snapshots sharing
+    30 atoms (16 probes, tolerance 1e-4, 30 iterations)."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    cases = [
+        ("joint_32x32_24x24_b3", scenes.joint_scenes(1040, 3, (32, 32), (24, 24), k=12, snr_db=20.0),
+         dict(), True),
+        ("joint_32x32_24x24_b3_noise", scenes.joint_scenes(1041, 3, (32, 32), (24, 24), k=12, snr_db=15.0),
+         dict(), False),
+        ("joint_64x64_40x40_b4", scenes.joint_scenes(1042, 4, (64, 64), (40, 40), k=30, snr_db=10.0),
+         dict(hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4, max_iter=30), True),
+    ]
+    for name, batch, kw, shared_noise in cases:
+        y = np.stack([s.y for s in batch]).astype(np.complex64)
+        cfg = SBLConfig(device="cpu", mode="cg", joint_sparsity=True, joint_noise_variance=shared_noise, **kw)
+        res = sbl_2d(torch.from_numpy(y), batch[0].grid, config=cfg)
+        arrays = dict(y=y, x_true=np.stack([s.x for s in batch]), grid=np.array(batch[0].grid),
+                      samples=np.array(batch[0].samples), joint_noise_variance=np.array(shared_noise))
+        arrays.update(_result_arrays(res, "cg_"))
+        for key in ("relative_change", "cg_iterations", "noise_variance"):
+            arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
@@ -240,8 +273,11 @@ if __name__ == "__main__":
     elif which == "masked":
         masked_cg(root)
         masked_plans(root)
+    elif which == "joint":
+        joint_cg(root)
     else:
         main(root)
         long_cg(root)
         masked_cg(root)
         masked_plans(root)
+        joint_cg(root)

@@ -181,3 +181,47 @@ def test_masked_sbl_cg_matches_reference_golden_run(name, grid, kw):
     np.testing.assert_allclose(out["noise_variance"], g["cg_noise_variance"], rtol=1e-3)
     np.testing.assert_allclose(out["effective_dof"], g["cg_effective_dof"], rtol=1e-3)
     np.testing.assert_allclose(out["cg_iterations"], g["cg_history_cg_iterations"], atol=8)
+
+
+def test_shared_posterior_is_dense_multi_rhs_solve():
+    """The joint-noise posterior (one system, snapshots and probes as
+    right-hand sides) against the dense solve with the same probes."""
+    rng, gamma, noise, a, op = _problem(9, batch=3)
+    gamma = np.repeat(gamma[:1], 3, 0)
+    noise = np.full(3, noise[0])
+    y = rng.standard_normal((3, op.m)) + 1j * rng.standard_normal((3, op.m))
+    probes = co.rademacher_probes(1, op.m, 64, 0).astype(np.complex128)
+    mean, var, dd, _, _ = co._posterior_shared(y, gamma, noise, probes, op.adj(probes), op, None,
+                                               co.CGConfig(cg_warm_start=False), 1e-12, F32_EPS, F32_TINY)
+    q = noise[0] * np.eye(op.m) + (a * gamma[0]) @ a.conj().T
+    np.testing.assert_allclose(mean, gamma[:1] * (a.conj().T @ np.linalg.solve(q, y.T)).T, rtol=1e-8, atol=1e-10)
+    lifted = a.conj().T @ np.linalg.solve(q, probes[0])
+    want = np.maximum(np.mean(np.real(lifted * np.conj(a.conj().T @ probes[0])), 1), 0.0)
+    np.testing.assert_allclose(dd[0], want, rtol=1e-8, atol=1e-12)
+    assert np.array_equal(dd[0], dd[2]) and np.array_equal(var[0], var[1])
+
+
+def test_joint_requires_two_snapshots():
+    g = np.load(golden_path("general_32x32_24x24.npz"))
+    with pytest.raises(ValueError):
+        co.sbl_cg(g["y"], (32, 32), co.CGConfig(max_iter=2), joint=True)
+
+
+@pytest.mark.parametrize("name,grid,kw", [
+    ("joint_32x32_24x24_b3", (32, 32), {}),
+    ("joint_32x32_24x24_b3_noise", (32, 32), {}),
+    ("joint_64x64_40x40_b4", (64, 64), dict(hutchinson_probes=16, auto_probe_scaling=False,
+                                            cg_tolerance=1e-4, max_iter=30)),
+])
+def test_joint_sbl_cg_matches_reference_golden_run(name, grid, kw):
+    """sbl_2d(mode='cg', joint_sparsity=True) of the reference, with shared
+    noise (one multi-right-hand-side system) and per-snapshot noise."""
+    g = np.load(golden_path(name + ".npz"))
+    out = co.sbl_cg(g["y"], grid, co.CGConfig(**kw), joint=True,
+                    joint_noise=bool(g["joint_noise_variance"]))
+    assert out["iterations"] == int(g["cg_iterations"])
+    ref = g["cg_x"].reshape(out["x"].shape)
+    assert np.linalg.norm(out["x"] - ref) / np.linalg.norm(ref) < 1e-3
+    np.testing.assert_allclose(out["noise_variance"], g["cg_noise_variance"], rtol=1e-3)
+    np.testing.assert_allclose(out["effective_dof"], g["cg_effective_dof"], rtol=1e-3)
+    np.testing.assert_allclose(out["cg_iterations"], g["cg_history_cg_iterations"], atol=8)
