@@ -225,6 +225,35 @@ __global__ void k_residual_finish(const double* __restrict__ partial, int nblock
   out[b] = damping * old + (1.0f - damping) * cand;
 }
 
+// Shared noise of joint sparsity: each snapshot's update (EM or evidence
+// form, floored at its own floor), averaged over the snapshots, floored at
+// the largest floor, then damped (solver.py:1737-1774).  One wavefront.
+__global__ void __launch_bounds__(64)
+k_joint_noise_finish(const double* __restrict__ partial, int nblocks, int batch, float* __restrict__ out,
+                     const float* __restrict__ old_noise, const float* __restrict__ dof_,
+                     const float* __restrict__ floor_, int m, float noise_shape, float noise_rate, float damping,
+                     int evidence, float epsilon) {
+  float acc = 0.f, fmax_ = 0.f;
+  for (int b = 0; b < batch; ++b) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nblocks; i += 64) a += partial[(long long)b * nblocks + i];
+    a = wave_sum_all(a);
+    const float energy = (float)a;
+    const float old = old_noise[b], dof = dof_[b], fl = floor_[b];
+    float cand;
+    if (evidence) {
+      cand = (energy + noise_rate) / fmaxf((float)m - dof + noise_shape, epsilon);
+    } else {
+      cand = (energy + old * dof + noise_rate) / ((float)m + noise_shape);
+    }
+    acc += fmaxf(cand, fl);
+    fmax_ = fmaxf(fmax_, fl);
+  }
+  const float shared = fmaxf(acc / (float)batch, fmax_);
+  if (threadIdx.x == 0)
+    for (int b = 0; b < batch; ++b) out[b] = damping * old_noise[b] + (1.0f - damping) * shared;
+}
+
 inline int nblk(int n) { return (n + FBME_BLOCK - 1) / FBME_BLOCK; }
 
 }  // namespace
@@ -340,6 +369,21 @@ int fbme_uamp_noise_update(void* stream, int batch, int m, const void* data, con
   return fbme_uamp_noise_update_masked(stream, batch, m, nullptr, m, data, prediction, old_noise, effective_dof,
                                        variance_floor, output, noise_shape, noise_rate, damping, evidence_update,
                                        epsilon, partial);
+}
+
+int fbme_joint_noise_update(void* stream, int batch, int m, const unsigned char* mask, int observed,
+                            const void* data, const void* prediction, const float* old_noise,
+                            const float* effective_dof, const float* variance_floor, float* output, float noise_shape,
+                            float noise_rate, float damping, int evidence_update, float epsilon, double* partial) {
+  if (batch < 1 || m < 1 || observed < 1 || observed > m || !partial) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = nblk(m);
+  hipLaunchKernelGGL(k_residual_partial, dim3(nb, batch), dim3(FBME_BLOCK), 0, st, (const float2*)data,
+                     (const float2*)prediction, m, partial, mask);
+  hipLaunchKernelGGL(k_joint_noise_finish, dim3(1), dim3(64), 0, st, (const double*)partial, nb, batch, output,
+                     old_noise, effective_dof, variance_floor, observed, noise_shape, noise_rate, damping,
+                     evidence_update, epsilon);
+  return (int)hipGetLastError();
 }
 
 int fbme_uamp_noise_update_masked(void* stream, int batch, int m, const unsigned char* mask, int observed,

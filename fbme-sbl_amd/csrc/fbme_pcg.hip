@@ -834,6 +834,112 @@ k_cg_em(float* __restrict__ gamma, const float2* __restrict__ mean, const float*
   if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = dof;
 }
 
+// ------------------------------------------------------------- joint sparsity
+// Snapshots b = 0..B-1 share one variance per cell (M-SBL).  gamma is kept
+// as B identical rows so the per-snapshot kernels read it unchanged.
+
+// gamma[b, i] = max(mean_l(blend |bp_l|^2 / ||bp_l||^2 P_l + (1 - blend) P_l / N),
+// max_l floor_l) for every b (solver.py:1071-1099).
+__global__ void k_joint_gamma0(const float2* __restrict__ bp, const double* __restrict__ bpe,
+                               const float* __restrict__ power, const float* __restrict__ floor_, float blend,
+                               int batch, int n, float* __restrict__ gamma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f, fmax_ = 0.f;
+  for (int b = 0; b < batch; ++b) {
+    const float2 v = bp[(long long)b * n + i];
+    const float p = power[b];
+    const float structured = (v.x * v.x + v.y * v.y) / fmaxf((float)bpe[b], (float)F32_EPS_D) * p;
+    acc += blend * structured + (1.0f - blend) * (p / (float)n);
+    fmax_ = fmaxf(fmax_, floor_[b]);
+  }
+  const float g = fmaxf(acc / (float)batch, fmax_);
+  for (int b = 0; b < batch; ++b) gamma[(long long)b * n + i] = g;
+}
+
+// noise[b] = mean_l noise_l for every b (solver.py:1157-1159).
+__global__ void k_joint_noise0(float* __restrict__ noise, int batch) {
+  if (threadIdx.x != 0) return;
+  float s = 0.f;
+  for (int b = 0; b < batch; ++b) s += noise[b];
+  s /= (float)batch;
+  for (int b = 0; b < batch; ++b) noise[b] = s;
+}
+
+// Shared-noise posterior: lanes 0..B-1 of `lifted` are A^H Q^-1 y_b, lanes
+// B..B+P-1 the probes' A^H Q^-1 p (one system, solver.py:3039-3153).
+// mean[b] = gamma lifted[b]; the Hutchinson diagonal, variance and relevance
+// partials are the same for every snapshot and written for each.
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_hutch_shared(const float2* __restrict__ lifted, const float2* __restrict__ lprobes,
+               const float* __restrict__ gamma, int batch, int n, int probes, float2* __restrict__ mean,
+               float* __restrict__ var, float* __restrict__ dd_out, double* __restrict__ part) {
+  __shared__ double red[8];
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  double rel = 0.0;
+  if (i < n) {
+    double acc = 0.0;
+    for (int p = 0; p < probes; ++p) {
+      const float2 u = lifted[((long long)batch + p) * n + i];
+      const float2 w = lprobes[(long long)p * n + i];
+      acc += (double)u.x * w.x + (double)u.y * w.y;
+    }
+    const float dd = fmaxf((float)(acc / probes), 0.f);
+    const float g = gamma[i];
+    const float v = fmaxf(g - g * g * dd, 0.f);
+    for (int b = 0; b < batch; ++b) {
+      const long long gi = (long long)b * n + i;
+      const float2 l = lifted[gi];
+      mean[gi] = make_float2(g * l.x, g * l.y);
+      var[gi] = v;
+      dd_out[gi] = dd;
+    }
+    rel = fminf(fmaxf(g * dd, 0.f), 1.f);
+  }
+  rel = block_sum(rel, red);
+  if (threadIdx.x == 0)
+    for (int b = 0; b < batch; ++b) part[(long long)b * gridDim.x + blockIdx.x] = rel;
+}
+
+// Projected relevance (dof partials per snapshot) and the complex M-SBL EM
+// update gamma <- (mean_b(|mu_b|^2 + v_b) + rate) / (1 + shape), clamped to
+// [max_b floor_b, max_prec], damped and floored (solver.py:1653-1666,
+// 1776-1798).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_cg_em_joint(float* __restrict__ gamma, const float2* __restrict__ mean, const float* __restrict__ var,
+              const float* __restrict__ dd, const double* __restrict__ rel_sum, const float* __restrict__ floor_,
+              int batch, int n, int m, float hyper_shape, float hyper_rate, float max_prec, float keep, int update,
+              double* __restrict__ part) {
+  __shared__ double red[8];
+  const int i = blockIdx.x * FBME_BLOCK + threadIdx.x;
+  const double cap = (double)m * (1.0 - 10.0 * F32_EPS_D);
+  float fmax_ = 0.f;
+  for (int b = 0; b < batch; ++b) fmax_ = fmaxf(fmax_, floor_[b]);
+  float gm = 0.f, second = 0.f;
+  if (i < n) {
+    gm = gamma[i];
+    for (int b = 0; b < batch; ++b) {
+      const float2 mu = mean[(long long)b * n + i];
+      second += mu.x * mu.x + mu.y * mu.y + var[(long long)b * n + i];
+    }
+    second /= (float)batch;
+  }
+  for (int b = 0; b < batch; ++b) {
+    double rel = 0.0;
+    if (i < n) {
+      const float scale = (float)fmin(1.0, cap / fmax(rel_sum[b], F32_EPS_D));
+      rel = fminf(fmaxf(gamma[(long long)b * n + i] * dd[(long long)b * n + i], 0.f), 1.f) * scale;
+    }
+    rel = block_sum(rel, red);
+    if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = rel;
+  }
+  if (update && i < n) {
+    const float ng = fminf(fmaxf((second + hyper_rate) / (1.0f + hyper_shape), fmax_), max_prec);
+    const float g = fmaxf(keep * gm + (1.0f - keep) * ng, fmax_);
+    for (int b = 0; b < batch; ++b) gamma[(long long)b * n + i] = g;
+  }
+}
+
 __global__ void k_seg_sum(const double* __restrict__ part, int blocks, double* __restrict__ out,
                           float* __restrict__ out_f) {
   const int b = blockIdx.x;
@@ -1242,6 +1348,40 @@ int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lift
                      (const float2*)lprobes, gamma, n, probes, (float2*)mean, var, dd, part);
   hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum,
                      (float*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int fbme_joint_init(void* stream, int batch, int n, const void* bp, const double* bpe, const float* power,
+                    const float* floor_, float blend, int shared_noise, float* gamma, float* noise) {
+  if (batch < 1 || n < 1) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_joint_gamma0, dim3(nblk(n)), dim3(FBME_BLOCK), 0, st, (const float2*)bp, bpe, power, floor_,
+                     blend, batch, n, gamma);
+  if (shared_noise) hipLaunchKernelGGL(k_joint_noise0, dim3(1), dim3(64), 0, st, noise, batch);
+  return (int)hipGetLastError();
+}
+
+int fbme_hutchinson_shared(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
+                           const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum) {
+  if (batch < 1 || n < 1 || probes < 1) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_hutch_shared, dim3(blocks), dim3(FBME_BLOCK), 0, st, (const float2*)lifted,
+                     (const float2*)lprobes, gamma, batch, n, probes, (float2*)mean, var, dd, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum,
+                     (float*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int fbme_cg_em_joint(void* stream, int batch, int n, int m, float* gamma, const void* mean, const float* var,
+                     const float* dd, const double* rel_sum, const float* floor_, float hyper_shape, float hyper_rate,
+                     float max_prec, float keep, int update, double* part, double* dof, float* dof_f) {
+  if (batch < 1 || n < 1 || m < 1) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_cg_em_joint, dim3(blocks), dim3(FBME_BLOCK), 0, st, gamma, (const float2*)mean, var, dd,
+                     rel_sum, floor_, batch, n, m, hyper_shape, hyper_rate, max_prec, keep, update, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, dof, dof_f);
   return (int)hipGetLastError();
 }
 

@@ -209,6 +209,23 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p,
                                        c_float, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
+    # joint sparsity of the general solver
+    "fbme_joint_init": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_float, c_int, c_void_p,
+                                c_void_p]),
+    "fbme_hutchinson_shared": (c_int, [c_void_p, c_int, c_int, c_int,
+                                       c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p]),
+    "fbme_cg_em_joint": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_float, c_float, c_float, c_float,
+                                 c_int, c_void_p, c_void_p, c_void_p]),
+    "fbme_joint_noise_update": (c_int, [c_void_p, c_int, c_int, c_void_p,
+                                        c_int, c_void_p, c_void_p, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_float,
+                                        c_float, c_float, c_int, c_float,
+                                        c_void_p]),
     # observation masks of the general solver
     "fbme_mask_spectrum": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                    c_void_p, c_void_p, c_void_p]),

@@ -29,8 +29,12 @@ Differences from the reference, all documented:
     the reference would switch to 33-256 atoms keep the PCG posterior;
   * the low-rank preconditioner requires ceil(N / 1024) <= 64;
   * 'circulant', 'nystrom' and 'none' preconditioners, Hadamard probes,
-    adaptive probe counts, joint sparsity and spatial coupling raise
-    NotImplementedError.
+    adaptive probe counts and spatial coupling raise NotImplementedError.
+
+Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
+3039-3153) keeps one variance per cell for all snapshots (B identical rows)
+updated by the M-SBL EM rule; with shared noise the snapshots are the first
+B lanes of one PCG system, followed by the P probes.
 """
 
 from __future__ import annotations
@@ -134,8 +138,6 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append("probe_type='hadamard'")
     if cfg.adaptive_probes:
         bad.append("adaptive_probes")
-    if cfg.joint_sparsity:
-        bad.append("joint_sparsity")
     if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
         bad.append("spatial coupling")
     if cfg.fixed_noise_variance is not None:
@@ -202,10 +204,18 @@ class _Solver:
         if min(self.n, self.m_obs) < RANK:
             raise NotImplementedError("mode='cg' needs at least 32 (observed) samples and grid cells "
                                       "(rank-32 preconditioner)")
+        # joint sparsity: the snapshots share gamma (B identical rows); with
+        # shared noise they are the right-hand sides of one system of
+        # B + P lanes (solver.py:1157-1159, 3039-3153)
+        self.joint = bool(cfg.joint_sparsity)
+        self.shared = self.joint and bool(cfg.joint_noise_variance)
+        if self.joint and self.b < 2:
+            raise ValueError("joint_sparsity=True requires at least two leading batch items.")
+        self.nsys = 1 if self.shared else self.b
         self.t = _tables(self.grid, self.samples, self.dev)
         self.p = _probe_count(self.n, self.m_obs, cfg)
-        self.r = 1 + self.p
-        self.lanes = self.b * self.r
+        self.r = (self.b if self.shared else 1) + self.p
+        self.lanes = self.nsys * self.r
         self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
         self._graphs = {}
         # 256 x 256 grids apply Q through the grid on the register FFT passes;
@@ -218,7 +228,7 @@ class _Solver:
         self.fused = self.grid_op and self.lr_fft and self.obs is None
         self.nparts = (self.samples[1] + 15) // 16
         self.stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
-        dev, b, n, m, r = self.dev, self.b, self.n, self.m, self.r
+        dev, b, n, m, r, ns = self.dev, self.b, self.n, self.m, self.r, self.nsys
         c64, f32, f64 = torch.complex64, torch.float32, torch.float64
         self.y = y.reshape(b, m).contiguous()
         if self.obs is not None:
@@ -228,21 +238,25 @@ class _Solver:
         e1, e2 = self.t.e1, self.t.e2
         # probes from torch's CPU generator (solver.py:5979-6000), kept on
         # the device per (seed, batch, samples, probes)
-        self.rhs = torch.empty((b, r, m), dtype=c64, device=dev)
-        self.rhs[:, 0] = self.y
+        self.rhs = torch.empty((ns, r, m), dtype=c64, device=dev)
+        nd = r - self.p            # data lanes per system
+        if self.shared:
+            self.rhs[0, :b] = self.y
+        else:
+            self.rhs[:, 0] = self.y
         if probes_override is not None:
             probes = probes_override.permute(0, 2, 1).to(dev)
         else:
-            probes = _probes(cfg.seed, b, self.m_obs, self.p, dev)
+            probes = _probes(cfg.seed, ns, self.m_obs, self.p, dev)
         if self.obs is None:
-            self.rhs[:, 1:] = probes
+            self.rhs[:, nd:] = probes
         else:
             # the reference draws [B, M_observed, P] (solver.py:1209-1216)
-            self.rhs[:, 1:] = 0
-            self.rhs[:, 1:, self.obs_idx] = probes
-        self.lprobes = torch.empty((b, self.p, n), dtype=c64, device=dev)
-        self.mid = torch.empty(b * max(r, 1) * n1 * max(m2, 1), dtype=c64, device=dev)
-        self._adjoint(self.rhs[:, 1:].reshape(b * self.p, m), self.lprobes.view(b * self.p, n))
+            self.rhs[:, nd:] = 0
+            self.rhs[:, nd:, self.obs_idx] = probes
+        self.lprobes = torch.empty((ns, self.p, n), dtype=c64, device=dev)
+        self.mid = torch.empty(self.lanes * n1 * max(m2, 1), dtype=c64, device=dev)
+        self._adjoint(self.rhs[:, nd:].reshape(ns * self.p, m), self.lprobes.view(ns * self.p, n))
         # state
         self.gamma = torch.empty((b, n), dtype=f32, device=dev)
         self.power = torch.empty(b, dtype=f32, device=dev)
@@ -257,9 +271,9 @@ class _Solver:
         self.var = torch.empty((b, n), dtype=f32, device=dev)
         self.dd = torch.empty((b, n), dtype=f32, device=dev)
         self.pred = torch.empty((b, m), dtype=c64, device=dev)
-        self.lifted = torch.empty((b, r, n), dtype=c64, device=dev)
+        self.lifted = torch.empty((ns, r, n), dtype=c64, device=dev)
         # PCG vectors
-        self.x = torch.zeros((b, r, m), dtype=c64, device=dev)
+        self.x = torch.zeros((ns, r, m), dtype=c64, device=dev)
         self.res = torch.empty_like(self.x)
         self.d = torch.empty_like(self.x)
         self.q = torch.empty_like(self.x)
@@ -269,10 +283,10 @@ class _Solver:
         # matvec / spectrum work
         self.work_a = torch.empty(self.lanes * m1 * e2, dtype=c64, device=dev)
         self.work_b = torch.empty(self.lanes * m1 * e2, dtype=c64, device=dev)
-        big = b * max(n, e1 * e2)
+        big = ns * max(n, e1 * e2)
         self.spec_a = torch.empty(big, dtype=c64, device=dev)
         self.spec_b = torch.empty(big, dtype=c64, device=dev)
-        self.spectrum = torch.empty((b, e1, e2), dtype=c64, device=dev)
+        self.spectrum = torch.empty((ns, e1, e2), dtype=c64, device=dev)
         self.gmid = (torch.empty(self.lanes * 256 * m2, dtype=c64, device=dev) if self.grid_op else None)
         # preconditioner
         tiles = (n + 1023) // 1024
@@ -281,16 +295,16 @@ class _Solver:
         self.cand = torch.empty(b * tiles * RANK, dtype=torch.int64, device=dev)
         self.top_idx = torch.empty((b, RANK), dtype=torch.int32, device=dev)
         self.top_val = torch.empty((b, RANK), dtype=f32, device=dev)
-        self.factors = torch.empty((b, m1 + m2, RANK), dtype=c64, device=dev)
-        self.sinv = torch.empty((b, RANK, RANK), dtype=c64, device=dev)
-        self.bg = torch.empty(b, dtype=f32, device=dev)
+        self.factors = torch.empty((ns, m1 + m2, RANK), dtype=c64, device=dev)
+        self.sinv = torch.empty((ns, RANK, RANK), dtype=c64, device=dev)
+        self.bg = torch.empty(ns, dtype=f32, device=dev)
         self.lr_part = torch.empty(self.lanes * 64 * RANK * 2, dtype=f64, device=dev)
         self.corr = torch.empty((self.lanes, RANK), dtype=c64, device=dev)
         # lane scalars
         self.dot_part = torch.empty(self.lanes * 64, dtype=f64, device=dev)
         self.mv_part = torch.empty(self.lanes * ((m2 + 15) // 16), dtype=f64, device=dev)
         self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
-        self.lr_map = torch.empty((b, 386), dtype=torch.int32, device=dev)
+        self.lr_map = torch.empty((ns, 386), dtype=torch.int32, device=dev)
         self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
         self.mask_spec = None
         if self.obs is not None:
@@ -407,6 +421,11 @@ class _Solver:
                                         float(cfg.initial_noise_fraction), float(cfg.initialization_blend),
                                         _ptr(self.power), _ptr(self.floor), _ptr(self.noise),
                                         _ptr(self.gamma)), "cg_init")
+        if self.joint:
+            self._chk(self.lib.fbme_joint_init(self.stream, self.b, self.n, _ptr(self.bp), _ptr(self.bpe),
+                                               _ptr(self.power), _ptr(self.floor),
+                                               float(cfg.initialization_blend), 1 if self.shared else 0,
+                                               _ptr(self.gamma), _ptr(self.noise)), "joint_init")
         self.grid_sums(with_mean=False)
 
     def grid_sums(self, with_mean=True):
@@ -421,15 +440,16 @@ class _Solver:
         m1, m2 = self.samples
         if not self.grid_op:
             self.spectrum_setup()
-        self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
+        self._chk(self.lib.fbme_topk32(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
         gsum = self.sums[:, 0].contiguous()
         self._chk(self.lib.fbme_lowrank_setup_masked(
-            self.stream, self.b, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
+            self.stream, self.nsys, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),
             _ptr(self.noise), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.mask_spec), _ptr(self.factors),
             _ptr(self.sinv), _ptr(self.bg), _ptr(self.status)), "lowrank_setup")
         if self.lr_fft:
-            self._chk(self.lib.fbme_lowrank_map(self.stream, self.b, n1, n2, _ptr(self.top_idx), _ptr(self.lr_map)),
+            self._chk(self.lib.fbme_lowrank_map(self.stream, self.nsys, n1, n2, _ptr(self.top_idx),
+                                                _ptr(self.lr_map)),
                       "lowrank_map")
 
     def spectrum_setup(self):
@@ -437,7 +457,7 @@ class _Solver:
         n1, n2 = self.grid
         m1, m2 = self.samples
         self._chk(self.lib.fbme_gamma_spectrum(
-            self.stream, self.b, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
+            self.stream, self.nsys, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
             t.codes[t.e2], _ptr(self.gamma), _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b),
             _ptr(self.spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
             "gamma_spectrum")
@@ -593,28 +613,41 @@ class _Solver:
         self.setup_operators()
         iters = self.pcg(tol)
         self._adjoint(self.x.view(self.lanes, self.m), self.lifted.view(self.lanes, self.n))
-        self._chk(self.lib.fbme_hutchinson(self.stream, self.b, self.n, self.p, _ptr(self.lifted),
-                                           _ptr(self.lprobes), _ptr(self.gamma), _ptr(self.mean), _ptr(self.var),
-                                           _ptr(self.dd), _ptr(self.grid_part), _ptr(self.rel_sum)), "hutchinson")
+        hutch = self.lib.fbme_hutchinson_shared if self.shared else self.lib.fbme_hutchinson
+        self._chk(hutch(self.stream, self.b, self.n, self.p, _ptr(self.lifted), _ptr(self.lprobes), _ptr(self.gamma),
+                        _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.grid_part), _ptr(self.rel_sum)),
+                  "hutchinson")
         return iters
 
     def em(self, update=True):
         cfg = self.cfg
         keep = max(cfg.damping, cfg.cg_damping)
+        if self.joint:
+            self._chk(self.lib.fbme_cg_em_joint(
+                self.stream, self.b, self.n, self.m_obs, _ptr(self.gamma), _ptr(self.mean), _ptr(self.var),
+                _ptr(self.dd), _ptr(self.rel_sum), _ptr(self.floor), float(cfg.hyper_shape), float(cfg.hyper_rate),
+                float(cfg.max_precision), float(keep), 1 if update else 0, _ptr(self.grid_part), _ptr(self.dof),
+                _ptr(self.dof_f)), "cg_em_joint")
+        else:
+            self._cg_em(keep, update)
+        if not update:
+            return
+        self._forward(self.mean, self.pred)
+        noise = self.lib.fbme_joint_noise_update if self.shared else self.lib.fbme_uamp_noise_update_masked
+        self._chk(noise(
+            self.stream, self.b, self.m, _ptr(self.obs), self.m_obs, _ptr(self.y), _ptr(self.pred), _ptr(self.noise),
+            _ptr(self.dof_f), _ptr(self.floor), _ptr(self.noise_new), float(cfg.noise_shape), float(cfg.noise_rate),
+            float(keep), 1 if cfg.noise_update == "evidence" else 0, float(F32_EPS), _ptr(self.noise_part)), "noise")
+        self.noise.copy_(self.noise_new)
+
+    def _cg_em(self, keep, update):
+        """Relevance projection and MacKay update (independent snapshots)."""
+        cfg = self.cfg
         self._chk(self.lib.fbme_cg_em(self.stream, self.b, self.n, self.m_obs, _ptr(self.gamma), _ptr(self.mean),
                                       _ptr(self.dd), _ptr(self.rel_sum), _ptr(self.floor), _ptr(self.mask),
                                       float(cfg.hyper_shape), float(cfg.hyper_rate), float(cfg.max_precision),
                                       float(keep), 1 if update else 0, _ptr(self.grid_part), _ptr(self.dof),
                                       _ptr(self.dof_f)), "cg_em")
-        if not update:
-            return
-        self._forward(self.mean, self.pred)
-        self._chk(self.lib.fbme_uamp_noise_update_masked(
-            self.stream, self.b, self.m, _ptr(self.obs), self.m_obs, _ptr(self.y), _ptr(self.pred), _ptr(self.noise),
-            _ptr(self.dof_f),
-            _ptr(self.floor), _ptr(self.noise_new), float(cfg.noise_shape), float(cfg.noise_rate), float(keep),
-            1 if cfg.noise_update == "evidence" else 0, float(F32_EPS), _ptr(self.noise_part)), "noise")
-        self.noise.copy_(self.noise_new)
 
     def try_active_set(self):
         """Switch to the exact reduced posterior when the 32 strongest

@@ -396,6 +396,39 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
                                   const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
                                   const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
                                   float jitter_multiplier, void* mean, float* var, float* dd, int* status);
+/* ------------------------------------------------------------------------
+ * Joint sparsity of the general solver: SBLConfig(joint_sparsity=True)
+ * (torch_sbl/solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
+ * 3039-3153).  The B snapshots share one variance per grid cell, kept as B
+ * identical rows of gamma [B, N]; with shared noise the posterior is one
+ * PCG system whose lanes are the B snapshots followed by the P probes.
+ * ------------------------------------------------------------------------ */
+/* After fbme_cg_init: gamma[b] = max(mean over snapshots of the
+ * backprojection initialisation, max_b floor_b) for every b
+ * (solver.py:1071-1099); with shared_noise also noise[b] = mean_b noise_b
+ * (solver.py:1157-1159). */
+int fbme_joint_init(void* stream, int batch, int n, const void* bp, const double* bpe, const float* power,
+                    const float* floor_, float blend, int shared_noise, float* gamma, float* noise);
+/* Shared-noise posterior moments (_shared_mmv_posterior,
+ * solver.py:3120-3153): lifted holds B + probes lanes of A^H Q^-1 [y, p];
+ * mean[b] = gamma lifted[b], and the Hutchinson diagonal dd, variance and
+ * relevance sum (rel_sum[b]) written for every snapshot.  part: B *
+ * ceil(N / 256) doubles. */
+int fbme_hutchinson_shared(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
+                           const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum);
+/* Projected relevance per snapshot (dof) and the complex M-SBL EM update of
+ * the shared variances (solver.py:1653-1666, 1776-1798); part: B *
+ * ceil(N / 256) doubles. */
+int fbme_cg_em_joint(void* stream, int batch, int n, int m, float* gamma, const void* mean, const float* var,
+                     const float* dd, const double* rel_sum, const float* floor_, float hyper_shape, float hyper_rate,
+                     float max_prec, float keep, int update, double* part, double* dof, float* dof_f);
+/* Shared noise update (solver.py:1737-1774): each snapshot's update over its
+ * observed samples (mask may be NULL), floored at its floor, averaged,
+ * floored at the largest floor and damped; written for every snapshot. */
+int fbme_joint_noise_update(void* stream, int batch, int m, const unsigned char* mask, int observed,
+                            const void* data, const void* prediction, const float* old_noise,
+                            const float* effective_dof, const float* variance_floor, float* output, float noise_shape,
+                            float noise_rate, float damping, int evidence_update, float epsilon, double* partial);
 /* fbme_uamp_noise_update over the observed samples: the residual energy of
  * the samples with mask[i] nonzero (mask NULL: all) and `observed` in place
  * of M in the update (solver.py:1737-1768 on the restricted operator). */

@@ -1,0 +1,91 @@
+"""Joint sparsity (SBLConfig(joint_sparsity=True), mode='cg') on the HIP
+kernels against the float64 oracle and the reference's golden runs.
+
+Snapshots share one variance per grid cell (M-SBL EM update); with shared
+noise (joint_noise_variance=True, the default) the posterior is one PCG
+system whose lanes are the snapshots and the probes
+(solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774, 3039-3153).
+Tolerances as for the independent-snapshot solver: relative L2 2e-2 of the
+estimate (float32 PCG stopped at a relative residual of 1e-3 by design),
+NMSE within 0.1 dB, noise and dof within 2 %.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden_path
+from oracle import cg_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _cfg(device, **kw):
+    from torch_sbl import SBLConfig
+    return SBLConfig(mode="cg", device=str(device), joint_sparsity=True, **kw)
+
+
+def _compare(x, ref, truth, grid):
+    from fbme_amd import scenes
+    xs, rs, ts = x.reshape(-1, *grid), ref.reshape(-1, *grid), truth.reshape(-1, *grid)
+    for i in range(xs.shape[0]):
+        assert _rel(xs[i], rs[i]) < 2e-2, i
+        assert abs(scenes.nmse_db(xs[i], ts[i]) - scenes.nmse_db(rs[i], ts[i])) < 0.1, i
+
+
+@pytest.mark.parametrize("name,grid,kw", [
+    ("joint_32x32_24x24_b3", (32, 32), {}),
+    ("joint_32x32_24x24_b3_noise", (32, 32), {}),
+    ("joint_64x64_40x40_b4", (64, 64), dict(hutchinson_probes=16, auto_probe_scaling=False,
+                                            cg_tolerance=1e-4, max_iter=30)),
+])
+def test_joint_sbl_2d_matches_reference_golden(device, name, grid, kw):
+    from torch_sbl import sbl_2d
+    print("golden", name, flush=True)
+    g = np.load(golden_path(name + ".npz"))
+    shared = bool(g["joint_noise_variance"])
+    res = sbl_2d(torch.from_numpy(g["y"]).to(device), grid,
+                 config=_cfg(device, joint_noise_variance=shared, **kw))
+    assert res.mode == "cg" and res.active_set_size == 0
+    assert res.iterations == int(g["cg_iterations"])
+    x = res.x.cpu().numpy()
+    _compare(x, g["cg_x"].reshape(x.shape), g["x_true"].reshape(x.shape), grid)
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), g["cg_noise_variance"], rtol=2e-2)
+    np.testing.assert_allclose(res.effective_dof.cpu().numpy(), g["cg_effective_dof"], rtol=2e-2)
+    if shared:
+        # one system: the same noise and posterior variances for every snapshot
+        assert torch.all(res.noise_variance == res.noise_variance[0])
+        assert torch.equal(res.posterior_variance[0], res.posterior_variance[-1])
+
+
+@pytest.mark.parametrize("grid,samples,masked", [((64, 48), (45, 40), False), ((40, 30), (20, 15), True),
+                                                 ((256, 256), (160, 160), False)])
+def test_joint_vs_oracle(device, grid, samples, masked):
+    """Fresh snapshot batches (a mask on the 40 x 30 one) against the
+    oracle, shared noise, a few EM iterations."""
+    from fbme_amd import scenes
+    from torch_sbl import pcg
+    print("joint", grid, samples, masked, flush=True)
+    k = 300 if grid == (256, 256) else 20
+    batch = scenes.joint_scenes(1500, 4, grid, samples, k=k, snr_db=12.0)
+    mask = np.random.default_rng(15).random(samples) < 0.7 if masked else None
+    y = np.stack([s.y if mask is None else np.where(mask, s.y, 0) for s in batch]).astype(np.complex64)
+    kw = dict(max_iter=4, min_iter=1, tol=0.0, hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
+    res = pcg.sbl_cg(torch.from_numpy(y).to(device), grid, _cfg(device, **kw), batch_shape=(4,),
+                     observation_mask=None if mask is None else torch.from_numpy(mask))
+    ref = co.sbl_cg(y, grid, co.CGConfig(**{k_: v for k_, v in kw.items() if k_ not in ("min_iter", "tol")}),
+                    joint=True, mask=mask)
+    x = res.x.cpu().numpy()
+    _compare(x, ref["x"].reshape(x.shape), np.stack([s.x for s in batch]), grid)
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=2e-2)
+
+
+def test_joint_requires_two_snapshots(device):
+    from torch_sbl import sbl_2d
+    g = np.load(golden_path("general_32x32_24x24.npz"))
+    with pytest.raises(ValueError):
+        sbl_2d(torch.from_numpy(g["y"]).to(device), (32, 32), config=_cfg(device, max_iter=2))
