@@ -28,14 +28,15 @@
 
 extern "C" {
 int fbme_launch_col_adjoint(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const unsigned char*,
-                            const void*, void*, void*, const double*, double*, int, int, int, float, const void*,
-                            hipStream_t);
+                            const void*, const void*, void*, void*, const double*, double*, int, int, int, float,
+                            const void*, hipStream_t);
 int fbme_launch_row(const FbmeScene*, const FbmeWork*, int, int, int, void*, void*, float*, void*, float*,
                     const double*, double*, int, int, float, float, float, float, float, float, uint32_t*, int*,
                     int, uint32_t*, const void*, hipStream_t);
 int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, const void*, const void*,
-                            const unsigned char*, void*, void*, const double*, double*, int, int, float, const void*,
-                            hipStream_t);
+                            const unsigned char*, const void*, void*, void*, const double*, double*, int, int, float,
+                            const void*, hipStream_t);
+int fbme_launch_mask_bits16(const unsigned char*, int, int, void*, hipStream_t);
 int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
                         float, float, float, float, float, int, hipStream_t);
 size_t fbme_polish_state_bytes(void);
@@ -81,7 +82,7 @@ struct fbme_plan {
       r_ckey, r_cidx, r_xkey, r_xidx, r_state, r_result, r_score;
   // observation masks: spectra of the masked scenes and the scratch of
   // their computation; the caller's mask bytes (packed like the data)
-  Region r_mh, r_mhwork;
+  Region r_mh, r_mhwork, r_mbits;
   int masked = 0;                           // scenes with an observation mask
   std::vector<int> mh_leaders;              // caller indices owning a spectrum
   const unsigned char* mask = nullptr;
@@ -242,6 +243,7 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
     }
     s.m_obs = (int)mo;
     s.mh_off = -1;
+    s.mbits_off = -1;
     s.iters = iteration_budget(n, mo, *opt);
     s.k = (int)std::min<long long>(opt->polish_size, n);
     s.orig = i;
@@ -266,7 +268,7 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   // mask spectra [N1*N2] (double2) of the masked scenes, caller order; a
   // scene whose mask_group names an earlier scene of the same shape shares
   // that scene's mask and spectrum
-  long long mh_total = 0, mh_work = 0;
+  long long mh_total = 0, mh_work = 0, mbits_total = 0;
   for (int i = 0; i < nscenes; ++i) {
     if (sc[i].m_obs == sc[i].m1 * sc[i].m2) continue;
     ++p->masked;
@@ -283,9 +285,14 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
         return fail("scene " + std::to_string(i) + ": mask_group scene differs in shape or observed count");
       }
       sc[i].mh_off = a.mh_off;
+      sc[i].mbits_off = a.mbits_off;
       continue;
     }
     sc[i].mh_off = mh_total;
+    if (sc[i].n1 == 256) {   // the register column passes read the mask as 16-bit words
+      sc[i].mbits_off = (int)mbits_total;
+      mbits_total += (long long)sc[i].m2 * 16;
+    }
     p->mh_leaders.push_back(i);
     mh_total += (long long)sc[i].n1 * sc[i].n2;
     mh_work = std::max<long long>(mh_work, (long long)sc[i].m1 * sc[i].n2);
@@ -467,6 +474,7 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   reg(p->r_score, 4 * (size_t)score_cells);
   reg(p->r_mh, 16 * (size_t)mh_total);
   reg(p->r_mhwork, 16 * (size_t)mh_work);
+  reg(p->r_mbits, 2 * (size_t)mbits_total);
   p->workspace_bytes = align_up(off, 256);
   return p;
 }
@@ -540,8 +548,11 @@ int fbme_plan_set_mask(fbme_plan* p, const unsigned char* mask, void* stream) {
   for (const FbmeScene& s : p->scenes) {
     if (s.mh_off < 0 || std::find(p->mh_leaders.begin(), p->mh_leaders.end(), s.orig) == p->mh_leaders.end())
       continue;
-    const int rc = fbme_mask_spectrum(stream, s.n1, s.n2, s.m1, s.m2, mask + s.samp_off,
-                                      w + p->r_mh.off + 16 * (size_t)s.mh_off, w + p->r_mhwork.off);
+    int rc = fbme_mask_spectrum(stream, s.n1, s.n2, s.m1, s.m2, mask + s.samp_off,
+                                w + p->r_mh.off + 16 * (size_t)s.mh_off, w + p->r_mhwork.off);
+    if (!rc && s.mbits_off >= 0)
+      rc = fbme_launch_mask_bits16(mask + s.samp_off, s.m1, s.m2, w + p->r_mbits.off + 2 * (size_t)s.mbits_off,
+                                   (hipStream_t)stream);
     if (rc) return rc;
   }
   p->mask = mask;
@@ -564,6 +575,7 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   const void* dir = w + p->r_dir.off;
   const void* mh = p->masked ? (const void*)(w + p->r_mh.off) : nullptr;
   const unsigned char* mask = p->masked ? p->mask : nullptr;
+  const void* mbits = (p->masked && p->r_mbits.bytes) ? (const void*)(w + p->r_mbits.off) : nullptr;
   const void* tw = w + p->r_tw.off;
   double* sc = (double*)(w + p->r_sc.off);
   double* part = (double*)(w + p->r_part.off);
@@ -595,11 +607,11 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   const FbmeWork* roww16 = roww + p->row_split;
   const int ncol16 = ncol - p->col_split, nrow16 = nrow - p->row_split;
   auto col_adjoint = [&](int ng, int n16, int mode, int store) {
-    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, mask, pred, msg, mid, sc, part, ps,
-                                    mode, store, o.uamp_damping, tw, st);
+    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, mask, mbits, pred, msg, mid, sc, part,
+                                    ps, mode, store, o.uamp_damping, tw, st);
     if (!r)
-      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, mask, pred, msg, mid, sc, part, ps,
-                                  mode, store, o.uamp_damping, tw, st);
+      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, mask, mbits, pred, msg, mid, sc,
+                                  part, ps, mode, store, o.uamp_damping, tw, st);
     return r;
   };
   uint32_t* score = p->r_score.bytes ? (uint32_t*)(w + p->r_score.off) : nullptr;
@@ -614,11 +626,11 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
     return r;
   };
   auto col_forward = [&](int ng, int n16, int first) {
-    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, mask, pred, msg, sc, part, ps,
-                                    first, o.uamp_damping, tw, st);
+    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, mask, mbits, pred, msg, sc, part,
+                                    ps, first, o.uamp_damping, tw, st);
     if (!r)
-      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, mask, pred, msg, sc, part, ps,
-                                  first, o.uamp_damping, tw, st);
+      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, mask, mbits, pred, msg, sc,
+                                  part, ps, first, o.uamp_damping, tw, st);
     return r;
   };
   // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
@@ -660,6 +672,7 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   const unsigned char* mask = p->masked ? p->mask : nullptr;
   hipStream_t st = (hipStream_t)stream;
   char* w = p->ws;
+  const void* mbits = (p->masked && p->r_mbits.bytes) ? (const void*)(w + p->r_mbits.off) : nullptr;
   const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
   const FbmeWork* colw = (const FbmeWork*)(w + p->r_col.off);
   const FbmeWork* roww = (const FbmeWork*)(w + p->r_row.off);
@@ -683,10 +696,10 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
   const int ng_row = p->row_split, n16_row = nrow - p->row_split;
   switch (stage) {
     case FBME_STAGE_COL_ADJOINT:
-      return (fbme_launch_col_adjoint(scenes, colw, ng_col, ng_col, p->ept_col, io->y, mask, pred, msg, mid, sc, part,
-                                      ps, 1, 1, o.uamp_damping, tw, st) ||
-              fbme_launch_col_adjoint(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, io->y, mask, pred, msg, mid, sc,
-                                      part, ps, 1, 1, o.uamp_damping, tw, st))
+      return (fbme_launch_col_adjoint(scenes, colw, ng_col, ng_col, p->ept_col, io->y, mask, mbits, pred, msg, mid, sc,
+                                      part, ps, 1, 1, o.uamp_damping, tw, st) ||
+              fbme_launch_col_adjoint(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, io->y, mask, mbits, pred, msg,
+                                      mid, sc, part, ps, 1, 1, o.uamp_damping, tw, st))
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_ROW:
@@ -699,10 +712,10 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_COL_FORWARD:
-      return (fbme_launch_col_forward(scenes, colw, ng_col, ng_col, p->ept_col, mid, io->y, mask, pred, msg, sc, part,
-                                      ps, 0, o.uamp_damping, tw, st) ||
-              fbme_launch_col_forward(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, mid, io->y, mask, pred, msg, sc,
-                                      part, ps, 0, o.uamp_damping, tw, st))
+      return (fbme_launch_col_forward(scenes, colw, ng_col, ng_col, p->ept_col, mid, io->y, mask, mbits, pred, msg, sc,
+                                      part, ps, 0, o.uamp_damping, tw, st) ||
+              fbme_launch_col_forward(scenes, colw16, n16_col, n16_col, FBME_EPT_R16, mid, io->y, mask, mbits, pred,
+                                      msg, sc, part, ps, 0, o.uamp_damping, tw, st))
                  ? FBME_ERR_HIP
                  : 0;
     case FBME_STAGE_OUTPUT:

@@ -29,6 +29,7 @@ struct FbmeScene {
   int orig;                 // caller's scene index (per-scene outputs)
   int tw1_off, tw2_off;     // twiddle tables (float2) of lengths n1 and n2
   int m_obs;                // observed samples (M1 * M2 without a mask)
+  int mbits_off;            // register-pass mask bits (uint16 [M2*16]) or -1
   long long mh_off;         // mask spectrum (double2 [N1*N2]) or -1 without a mask
 };
 

@@ -449,21 +449,21 @@ extern "C" {
 int fbme_launch_row16(const FbmeScene*, const FbmeWork*, int, int, void*, void*, float*, void*, float*, const double*,
                       double*, int, int, float, float, float, float, float, float, uint32_t*, int*, int, uint32_t*,
                       const void*, hipStream_t);
-int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const unsigned char*,
+int fbme_launch_col16_adjoint(const FbmeScene*, const FbmeWork*, int, int, const void*, const uint16_t*,
                               const void*, void*, void*, const double*, double*, int, int, int, float, const void*,
                               hipStream_t);
 int fbme_launch_col16_forward(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*,
-                              const unsigned char*, void*, void*, const double*, double*, int, int, float,
+                              const uint16_t*, void*, void*, const double*, double*, int, int, float,
                               const void*, hipStream_t);
 
 int fbme_launch_col_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
-                            const void* y, const unsigned char* mask, const void* pred, void* msg, void* mid,
-                            const double* sc, double* part, int pstride, int mode, int store_msg, float damping,
-                            const void* twtab, hipStream_t stream) {
+                            const void* y, const unsigned char* mask, const void* mbits, const void* pred, void* msg,
+                            void* mid, const double* sc, double* part, int pstride, int mode, int store_msg,
+                            float damping, const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   if (ept == FBME_EPT_R16)
-    return fbme_launch_col16_adjoint(scenes, work, nwork, grid, y, mask, pred, msg, mid, sc, part, pstride, mode,
-                                     store_msg, damping, twtab, stream);
+    return fbme_launch_col16_adjoint(scenes, work, nwork, grid, y, (const uint16_t*)mbits, pred, msg, mid, sc, part,
+                                     pstride, mode, store_msg, damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     if (mode == 0)
       hipLaunchKernelGGL((k_col_adjoint<E, 0>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
@@ -505,13 +505,13 @@ int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, in
 }
 
 int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, int ept,
-                            const void* mid, const void* y, const unsigned char* mask, void* pred, void* msg,
-                            const double* sc, double* part, int pstride, int first_message, float damping,
+                            const void* mid, const void* y, const unsigned char* mask, const void* mbits, void* pred,
+                            void* msg, const double* sc, double* part, int pstride, int first_message, float damping,
                             const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
   if (ept == FBME_EPT_R16)
-    return fbme_launch_col16_forward(scenes, work, nwork, grid, mid, y, mask, pred, msg, sc, part, pstride,
-                                     first_message, damping, twtab, stream);
+    return fbme_launch_col16_forward(scenes, work, nwork, grid, mid, y, (const uint16_t*)mbits, pred, msg, sc, part,
+                                     pstride, first_message, damping, twtab, stream);
   FBME_EPT_DISPATCH(ept, {
     hipLaunchKernelGGL((k_col_forward<E>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,
                        (const float2*)mid, (const float2*)y, mask, (float2*)pred, (float2*)msg, sc, part, pstride,

@@ -347,11 +347,13 @@ k_tile_topk16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
 // ------------------------------------------------------------- column passes
 // COLS columns per workgroup, 16 lanes per column: lane = c + COLS * t.
 // With an observation mask, unobserved samples contribute zero data, a zero
-// message and no residual.
+// message and no residual.  The mask reaches these kernels as 16-bit words
+// (bit j of word [col][t] = sample (t + 16 j, col) observed), one load per
+// lane and work item.
 template <int COLS, int MODE>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
-                const float2* __restrict__ y, const unsigned char* __restrict__ mask,
+                const float2* __restrict__ y, const uint16_t* __restrict__ mbits,
                 const float2* __restrict__ pred, float2* __restrict__ msg,
                 float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
@@ -370,6 +372,7 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
     const float precision = 1.0f / (dv + (float)scs[SC_NOISE]);
     const float keep = damping, update = 1.0f - damping;
     load_tw256(tw, twtab + s.tw1_off);
+    const unsigned bits = (mbits && s.mbits_off >= 0 && act) ? mbits[s.mbits_off + col * 16 + t] : 0xFFFFu;
     float2 v[16];
     double acc = 0.0;
 #pragma unroll
@@ -378,7 +381,7 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       v[j] = make_float2(0.f, 0.f);
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
-        const bool ob = !mask || mask[gi];
+        const bool ob = (bits >> j) & 1u;
         const float2 yv = y[gi];
         if (MODE == 0) {
           if (ob) v[j] = yv;
@@ -410,7 +413,7 @@ template <int COLS>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
                 const float2* __restrict__ mid, const float2* __restrict__ y,
-                const unsigned char* __restrict__ mask, float2* __restrict__ pred,
+                const uint16_t* __restrict__ mbits, float2* __restrict__ pred,
                 float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
@@ -438,6 +441,7 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       const int i = t + 16 * j;
       yv[j] = (act && i < s.m1) ? y[s.samp_off + (long long)i * s.m2 + col] : make_float2(0.f, 0.f);
     }
+    const unsigned bits = (mbits && s.mbits_off >= 0 && act) ? mbits[s.mbits_off + col * 16 + t] : 0xFFFFu;
     const double* scs = sc + (long long)wk.scene * SC_COUNT;
     const float dv = fmaxf((float)scs[SC_DV], F32_TINY);
     const float precision = 1.0f / (dv + (float)scs[SC_NOISE]);
@@ -448,7 +452,7 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
         pred[gi] = v[n];
-        const bool ob = !mask || mask[gi];
+        const bool ob = (bits >> n) & 1u;
         const float rx = yv[n].x - v[n].x, ry = yv[n].y - v[n].y;
         if (ob) acc += (double)rx * rx + (double)ry * ry;
         if (first_message) {
@@ -466,9 +470,30 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
   }
 }
 
+// Mask words of the register column passes: word [col][t] bit j = sample
+// (t + 16 j, col) observed (rows t + 16 j >= M1 give 0).  One thread per word.
+__global__ void k_mask_bits16(const unsigned char* __restrict__ mask, int m1, int m2, uint16_t* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m2 * 16) return;
+  const int col = e >> 4, t = e & 15;
+  unsigned bits = 0u;
+  for (int j = 0; j < 16; ++j) {
+    const int i = t + 16 * j;
+    if (i < m1 && mask[(long long)i * m2 + col]) bits |= 1u << j;
+  }
+  out[e] = (uint16_t)bits;
+}
+
 }  // namespace
 
 extern "C" {
+
+int fbme_launch_mask_bits16(const unsigned char* mask, int m1, int m2, void* out, hipStream_t stream) {
+  if (m1 < 1 || m1 > 256 || m2 < 1) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_mask_bits16, dim3((m2 * 16 + 255) / 256), dim3(256), 0, stream, mask, m1, m2, (uint16_t*)out);
+  return (int)hipGetLastError();
+}
+
 
 int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, void* mid, void* bp,
                       float* gamma, void* mean, float* var, const double* sc, double* part, int pstride, int mode,
@@ -499,30 +524,30 @@ int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, 
 }
 
 int fbme_launch_col16_adjoint(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* y,
-                              const unsigned char* mask, const void* pred, void* msg, void* mid, const double* sc,
+                              const uint16_t* mbits, const void* pred, void* msg, void* mid, const double* sc,
                               double* part, int pstride, int mode, int store_msg, float damping, const void* twtab,
                               hipStream_t stream) {
   if (nwork <= 0) return 0;
   constexpr int C = FBME_R16_COLS;
   if (mode == 0)
     hipLaunchKernelGGL((k_col16_adjoint<C, 0>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                       (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                       (const float2*)y, mbits, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                        store_msg, damping, (const float2*)twtab);
   else
     hipLaunchKernelGGL((k_col16_adjoint<C, 1>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                       (const float2*)y, mask, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
+                       (const float2*)y, mbits, (const float2*)pred, (float2*)msg, (float2*)mid, sc, part, pstride,
                        store_msg, damping, (const float2*)twtab);
   return (int)hipGetLastError();
 }
 
 int fbme_launch_col16_forward(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const void* mid,
-                              const void* y, const unsigned char* mask, void* pred, void* msg, const double* sc,
+                              const void* y, const uint16_t* mbits, void* pred, void* msg, const double* sc,
                               double* part, int pstride, int first_message, float damping, const void* twtab,
                               hipStream_t stream) {
   if (nwork <= 0) return 0;
   constexpr int C = FBME_R16_COLS;
   hipLaunchKernelGGL((k_col16_forward<C>), dim3(grid), dim3(16 * C), 0, stream, scenes, work, nwork,
-                     (const float2*)mid, (const float2*)y, mask, (float2*)pred, (float2*)msg, sc, part, pstride,
+                     (const float2*)mid, (const float2*)y, mbits, (float2*)pred, (float2*)msg, sc, part, pstride,
                      first_message, damping, (const float2*)twtab);
   return (int)hipGetLastError();
 }
