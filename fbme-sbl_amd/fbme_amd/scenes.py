@@ -143,6 +143,44 @@ def joint_scenes(seed: int, count=3, grid=(32, 32), samples=(24, 24), k=12,
     return out
 
 
+def offgrid_scene(seed: int, grid=(64, 64), samples=(40, 40), k=6, snr_db=25.0,
+                  min_separation=4.0) -> tuple[Scene, np.ndarray, np.ndarray]:
+    """K point sources at continuous (off-grid) positions, at least
+    `min_separation` bins apart (periodic), amplitudes CN(0, 1) rescaled to
+    modulus in [0.5, 1.5].  Returns the scene (x holds each source's
+    amplitude at its nearest cell, plus 1e-3-level clutter elsewhere, as a
+    stand-in for an on-grid estimate), the positions [K, 2] and the
+    amplitudes [K]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n1, n2 = grid
+    pos = []
+    while len(pos) < k:
+        p = np.array([rng.uniform(0, n1), rng.uniform(0, n2)])
+        ok = True
+        for q in pos:
+            d = np.abs(p - q)
+            d = np.minimum(d, np.array(grid) - d)
+            if np.hypot(*d) < min_separation:
+                ok = False
+        if ok:
+            pos.append(p)
+    pos = np.stack(pos)
+    amp = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+    amp = amp / np.abs(amp) * rng.uniform(0.5, 1.5, k)
+    m1, m2 = samples
+    mm1, mm2 = np.meshgrid(np.arange(m1), np.arange(m2), indexing="ij")
+    ph = mm1.reshape(-1, 1) * pos[None, :, 0] / n1 + mm2.reshape(-1, 1) * pos[None, :, 1] / n2
+    clean = (np.exp(-2j * np.pi * ph) @ amp).reshape(samples)
+    sigma2 = float(np.sum(np.abs(clean) ** 2) / (clean.size * 10.0 ** (snr_db / 10.0)))
+    y = clean + np.sqrt(sigma2 / 2.0) * (rng.standard_normal(samples) + 1j * rng.standard_normal(samples))
+    x = 1e-3 * (rng.standard_normal(grid) + 1j * rng.standard_normal(grid))
+    cells = np.round(pos).astype(int) % np.array(grid)
+    x[cells[:, 0], cells[:, 1]] = amp
+    sc = Scene(x=x, y=y.astype(np.complex64), noise_variance=sigma2, grid=tuple(grid),
+               samples=tuple(samples), snr_db=float(snr_db), seed=int(seed))
+    return sc, pos, amp
+
+
 def uneven_batch(seed: int, count=64, grids=(64, 128, 192, 256),
                  ratio=(0.3, 0.8), k_fraction=(0.005, 0.03),
                  snr_db=(5.0, 25.0)) -> list[Scene]:

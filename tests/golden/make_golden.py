@@ -19,6 +19,7 @@ What it pins (functions no closed-form check fixes end to end):
     under the Fast / Stable numerics with an ``observation_mask``.
   * ``sbl_2d`` in mode "cg" with ``joint_sparsity=True`` (shared and
     per-snapshot noise).
+  * ``refine_offgrid_fourier`` on scenes with off-grid point sources.
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -265,6 +266,39 @@ def joint_cg(reference_root: str) -> None:
         print("wrote", name)
 
 
+def offgrid(reference_root: str) -> None:
+    """``refine_offgrid_fourier`` on CPU: 64x64 / 40x40 scenes with six
+    off-grid sources (model order by BIC) in complex128 and complex64, a
+    batch of two with a fixed component count, and a 128x96 / 80x60 scene
+    with ten sources."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl.offgrid import refine_offgrid_fourier
+
+    torch.set_num_threads(8)
+    cases = [
+        ("offgrid_64x64_40x40_c128", [1050], (64, 64), (40, 40), 6, 25.0, np.complex128, None),
+        ("offgrid_64x64_40x40_c64", [1050], (64, 64), (40, 40), 6, 25.0, np.complex64, None),
+        ("offgrid_64x64_40x40_b2_k5", [1051, 1052], (64, 64), (40, 40), 5, 20.0, np.complex64, 5),
+        ("offgrid_128x96_80x60", [1053], (128, 96), (80, 60), 10, 15.0, np.complex64, None),
+    ]
+    for name, seeds, grid, samples, k, snr, dtype, count in cases:
+        made = [scenes.offgrid_scene(s, grid, samples, k=k, snr_db=snr) for s in seeds]
+        y = np.stack([m[0].y.astype(np.complex128) if dtype == np.complex128 else m[0].y for m in made])
+        x = np.stack([m[0].x for m in made]).astype(dtype)
+        res = refine_offgrid_fourier(torch.from_numpy(y.astype(dtype)), torch.from_numpy(x), grid,
+                                     component_count=count)
+        arrays = dict(y=y.astype(dtype), x=x, grid=np.array(grid), samples=np.array(samples),
+                      true_positions=np.stack([m[1] for m in made]),
+                      true_amplitudes=np.stack([m[2] for m in made]),
+                      component_count=np.array(-1 if count is None else count))
+        for field in ("initial_positions", "positions", "amplitudes", "position_std",
+                      "initial_residual_norm", "residual_norm", "iterations", "converged"):
+            arrays[field] = getattr(res, field).detach().cpu().numpy()   # leading batch dimension
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
@@ -275,9 +309,12 @@ if __name__ == "__main__":
         masked_plans(root)
     elif which == "joint":
         joint_cg(root)
+    elif which == "offgrid":
+        offgrid(root)
     else:
         main(root)
         long_cg(root)
         masked_cg(root)
         masked_plans(root)
         joint_cg(root)
+        offgrid(root)
