@@ -209,6 +209,13 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p,
                                        c_float, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
+    # off-grid refinement
+    "fbme_offgrid_refine": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                    c_int, c_void_p, c_void_p, c_int, c_int,
+                                    c_int, c_float, c_float, c_int, c_double,
+                                    c_double, c_double, c_double, c_double,
+                                    c_double, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
     # joint sparsity of the general solver
     "fbme_joint_init": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_float, c_int, c_void_p,

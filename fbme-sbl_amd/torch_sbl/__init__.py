@@ -24,11 +24,17 @@ from .cuda_graph import (  # noqa: E402
     make_realtime_sbl_plan,
 )
 from .hip_kernels import reset_triton_runtime_error  # noqa: E402
+from .offgrid import (  # noqa: E402
+    OffGridResult,
+    continuous_fourier_dictionary,
+    refine_offgrid_fourier,
+)
 
 __all__ = [
     "PAPER_RELEASE_ID", "__version__", "SBLConfig", "SBLResult",
     "clear_operator_cache", "fast_sbl", "sbl_1d", "sbl_2d",
-    "PrefixFourierOperator", "CUDAGraphSBLPlan", "RealtimeSBLPlan",
+    "PrefixFourierOperator", "OffGridResult", "continuous_fourier_dictionary",
+    "refine_offgrid_fourier", "CUDAGraphSBLPlan", "RealtimeSBLPlan",
     "make_cuda_graph_sbl_plan", "make_realtime_sbl_plan",
     "reset_triton_runtime_error",
 ]

@@ -57,7 +57,6 @@ def _unsupported(cfg: SBLConfig) -> list[str]:
         "initialization='uniform'": cfg.initialization != "backprojection",
         "uamp_auto_iterations=False": not cfg.uamp_auto_iterations,
         "tol>0 (eager convergence checks)": cfg.tol > 0.0,
-        "offgrid_refinement": cfg.offgrid_refinement,
         "phase_error_correction": cfg.phase_error_correction,
         "range_migration_correction": cfg.range_migration_correction,
         "learn_snapshot_covariance": cfg.learn_snapshot_covariance,

@@ -144,8 +144,6 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append("fixed_noise_variance")
     if cfg.initialization != "backprojection":
         bad.append("initialization='uniform'")
-    if cfg.offgrid_refinement:
-        bad.append("offgrid_refinement")
     if bad:
         raise NotImplementedError(
             "mode='cg' on MI355X does not implement: " + ", ".join(bad))

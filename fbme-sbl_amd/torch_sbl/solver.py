@@ -151,8 +151,9 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
             "build; use mode='cg' or mode='uamp'.")
     if mode == "cg":
         from . import pcg
-        return pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
-                          batch_shape=batch_shape, observation_mask=mask)
+        res = pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
+                         batch_shape=batch_shape, observation_mask=mask)
+        return _with_offgrid(res, y, grid, cfg)
     if cfg.uamp_backend == "torch":
         raise NotImplementedError(
             "uamp_backend='torch' selects the reference's PyTorch UAMP; this "
@@ -166,4 +167,20 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
         plan.run(y.reshape(-1).contiguous())
         views.refresh()
         # Eager calls return fresh tensors; graph plans return plan storage.
-        return views.result(clone=True)
+        return _with_offgrid(views.result(clone=True), y, grid, cfg)
+
+
+def _with_offgrid(res: SBLResult, y: torch.Tensor, grid, cfg: SBLConfig) -> SBLResult:
+    """Off-grid refinement of the reconstruction (solver.py:2139-2153)."""
+    if not cfg.offgrid_refinement:
+        return res
+    from .offgrid import refine_offgrid_fourier
+    res.offgrid = refine_offgrid_fourier(
+        y, res.x, grid, component_count=cfg.offgrid_components,
+        max_components=cfg.offgrid_max_components,
+        relative_threshold=cfg.offgrid_relative_threshold,
+        minimum_separation=cfg.offgrid_minimum_separation,
+        max_iter=cfg.offgrid_max_iter, tolerance=cfg.offgrid_tolerance,
+        max_step=cfg.offgrid_max_step, damping=cfg.offgrid_damping,
+        ridge=cfg.offgrid_ridge)
+    return res

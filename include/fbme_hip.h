@@ -397,6 +397,30 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
                                   const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
                                   float jitter_multiplier, void* mean, float* var, float* dd, int* status);
 /* ------------------------------------------------------------------------
+ * Off-grid refinement: refine_offgrid_fourier (torch_sbl/offgrid.py:70-226)
+ * for 2-D grids.  One workgroup per scene runs the whole refinement:
+ * top max(64, 16 max_components) magnitudes of the coefficients, periodic
+ * non-maximum suppression (offgrid.py:229-295), the extended BIC model order
+ * when component_count <= 0 (offgrid.py:318-361), projected Gauss-Newton
+ * with Levenberg damping, trust bound and backtracking (offgrid.py:364-476)
+ * and the position standard deviations (offgrid.py:538-564), all in FP64.
+ * data [B, M1*M2] and coefficients [B, N1*N2] are complex64 (complex128 when
+ * `complex128` is nonzero).  Limits: max_components <= 64, at most 32
+ * refined atoms (counts[b*3+1] = -1 when the model order exceeds 32).
+ * Outputs, with Kout = component_count > 0 ? component_count :
+ * max_components rows per scene (rows past the scene's count are NaN):
+ * initial_positions / positions / position_std [B, Kout, 2] double,
+ * amplitudes [B, Kout] complex128, norms [B, 2] (initial, final residual
+ * norm), counts [B, 3] (atoms, iterations, converged).  eps / tiny are the
+ * data precision's (the reference's finfo of the real dtype). */
+int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                        const void* coefficients, int complex128, int component_count, int max_components,
+                        float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                        double max_step, double damping, double ridge, double eps, double tiny,
+                        double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                        double* norms, int* counts);
+
+/* ------------------------------------------------------------------------
  * Joint sparsity of the general solver: SBLConfig(joint_sparsity=True)
  * (torch_sbl/solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
  * 3039-3153).  The B snapshots share one variance per grid cell, kept as B
