@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--skip-masked", action="store_true")
     ap.add_argument("--skip-joint", action="store_true")
+    ap.add_argument("--skip-offgrid", action="store_true")
     ap.add_argument("--learned-scenes", type=int, default=16)
     return ap.parse_args()
 
@@ -336,6 +337,49 @@ def joint_workload(device, count=8, reps=20):
     return out
 
 
+def offgrid_workload(device, count=64, reps=5, cpu=True):
+    """Off-grid refinement of `count` 256x256/160x160 scenes with 20 off-grid
+    sources each (20 dB, fixed component count 20) in one launch, against
+    the float64 oracle of the same algorithm on one scene (NumPy, host)."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from torch_sbl import refine_offgrid_fourier
+    made = [scenes.offgrid_scene(2000 + i, (256, 256), (160, 160), k=20, snr_db=20.0) for i in range(count)]
+    y = torch.from_numpy(np.stack([m[0].y for m in made])).to(device)
+    x = torch.from_numpy(np.stack([m[0].x for m in made]).astype(np.complex64)).to(device)
+    refine_offgrid_fourier(y, x, (256, 256), component_count=20)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        res = refine_offgrid_fourier(y, x, (256, 256), component_count=20)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    pos = res.positions.cpu().numpy()
+    err = []
+    for i, m in enumerate(made):
+        d = np.abs(pos[i][:, None, :] - m[1][None, :, :])
+        d = np.minimum(d, 256 - d)
+        err.append(float(np.max(np.min(np.hypot(d[..., 0], d[..., 1]), axis=0))))
+    out = {"workload": f"{count} scenes 256x256/160x160, 20 off-grid sources, 20 dB, component_count=20",
+           "ms": ms, "scenes_per_s": count / (ms / 1e3), "max_position_error_bins": max(err),
+           "mean_iterations": float(res.iterations.float().mean())}
+    if cpu:
+        # CPU baseline of this workload: the float64 oracle on one scene
+        from oracle import offgrid_oracle as oo
+        t0 = time.perf_counter()
+        oo.refine_offgrid(made[0][0].y, made[0][0].x.astype(np.complex64), (256, 256), component_count=20)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / el, "unit": "scenes/s", "kind": "port",
+                               "sample": "1 scene, float64 NumPy oracle of the same algorithm"}
+    return out
+
+
 def learned_comparison(device, count):
     """Learned warm start against the plain general solver (mode='cg', 16
     probes, PCG tolerance 1e-4), one 256x256/160x160 K=300 10 dB scene per
@@ -544,6 +588,11 @@ def main():
     learned_cmp = None
     if not args.skip_learned:
         learned_cmp = learned_comparison(device, args.learned_scenes)
+    offgrid = None
+    if not args.skip_offgrid:
+        print("[bench] off-grid refinement", file=sys.stderr, flush=True)
+        offgrid = offgrid_workload(device, cpu=not args.no_cpu_baseline)
+        print(f"[bench] offgrid: {offgrid}", file=sys.stderr, flush=True)
     joint = None
     if not args.skip_joint:
         print("[bench] joint sparsity", file=sys.stderr, flush=True)
@@ -580,6 +629,7 @@ def main():
         "learned_warm_start": learned_cmp,
         "masked_workload": masked,
         "joint_sparsity": joint,
+        "offgrid_refinement": offgrid,
         "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),
         "wall_s": wall,
