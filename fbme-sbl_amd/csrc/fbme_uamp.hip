@@ -42,7 +42,8 @@ __device__ __forceinline__ double part_get(const double* part, int pstride, int 
 
 // ------------------------------------------------------------- column adjoint
 // MODE 0: input is the data y (backprojection); partial sum |y|^2.
-// MODE 1: input is the UAMP message update of (y, pred, msg).
+// MODE 1: input is the UAMP message update of (r, msg), r = y - A mu being
+// the residual the forward column pass stored.
 // With an observation mask, unobserved samples contribute zero data and a
 // zero message (the restricted operator, operators.py:57-75).
 // All global loads of the tile are issued before the first LDS store.
@@ -50,7 +51,7 @@ template <int EPT, int MODE>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
               const float2* __restrict__ y, const unsigned char* __restrict__ mask,
-              const float2* __restrict__ pred, float2* __restrict__ msg,
+              const float2* __restrict__ resid, float2* __restrict__ msg,
               float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
               int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
@@ -75,9 +76,9 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
         const int i = e / cols, c = e - i * cols;
         const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
         ob[q] = !mask || mask[gi];
-        yv[q] = y[gi];
+        if (MODE == 0) yv[q] = y[gi];
         if (MODE == 1) {
-          pv[q] = pred[gi];
+          pv[q] = resid[gi];
           mv[q] = msg[gi];
         }
       }
@@ -94,8 +95,8 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
           v = ob[q] ? yv[q] : make_float2(0.f, 0.f);
           acc += (double)v.x * v.x + (double)v.y * v.y;
         } else {
-          const float2 pseudo = make_float2(pv[q].x - dv * mv[q].x, pv[q].y - dv * mv[q].y);
-          const float2 cand = make_float2(precision * (yv[q].x - pseudo.x), precision * (yv[q].y - pseudo.y));
+          // y - (pred - dv msg) from the stored residual y - pred
+          const float2 cand = make_float2(precision * (pv[q].x + dv * mv[q].x), precision * (pv[q].y + dv * mv[q].y));
           v = ob[q] ? make_float2(keep * mv[q].x + update * cand.x, keep * mv[q].y + update * cand.y)
                     : make_float2(0.f, 0.f);
           if (store_msg) msg[s.samp_off + (long long)i * s.m2 + wk.first + c] = v;
@@ -304,7 +305,7 @@ template <int EPT>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_col_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
               const float2* __restrict__ mid, const float2* __restrict__ y,
-              const unsigned char* __restrict__ mask, float2* __restrict__ pred,
+              const unsigned char* __restrict__ mask, float2* __restrict__ resid,
               float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
               int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
@@ -342,9 +343,9 @@ k_col_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
         const int i = e / cols, c = e - i * cols;
         const long long gi = s.samp_off + (long long)i * s.m2 + wk.first + c;
         const float2 p = buf[lds_idx(c * stride + i)];
-        pred[gi] = p;
         const bool ob = !mask || mask[gi];
         const float rx = yv[q].x - p.x, ry = yv[q].y - p.y;
+        resid[gi] = make_float2(rx, ry);
         if (ob) acc += (double)rx * rx + (double)ry * ry;
         if (first_message) {
           // First message from zero state (triton_kernels.py:66-77 with

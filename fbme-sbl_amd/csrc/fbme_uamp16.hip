@@ -354,7 +354,7 @@ template <int COLS, int MODE>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
                 const float2* __restrict__ y, const uint16_t* __restrict__ mbits,
-                const float2* __restrict__ pred, float2* __restrict__ msg,
+                const float2* __restrict__ resid, float2* __restrict__ msg,
                 float2* __restrict__ mid, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int store_msg, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
@@ -382,14 +382,14 @@ k_col16_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
         const bool ob = (bits >> j) & 1u;
-        const float2 yv = y[gi];
         if (MODE == 0) {
-          if (ob) v[j] = yv;
+          if (ob) v[j] = y[gi];
           acc += (double)v[j].x * v[j].x + (double)v[j].y * v[j].y;
         } else {
-          const float2 pv = pred[gi], mv = msg[gi];
-          const float2 pseudo = make_float2(pv.x - dv * mv.x, pv.y - dv * mv.y);
-          const float2 cand = make_float2(precision * (yv.x - pseudo.x), precision * (yv.y - pseudo.y));
+          // y - (pred - dv msg) with the residual y - pred stored by the
+          // forward pass (one sample array less to read)
+          const float2 rv = resid[gi], mv = msg[gi];
+          const float2 cand = make_float2(precision * (rv.x + dv * mv.x), precision * (rv.y + dv * mv.y));
           if (ob) v[j] = make_float2(keep * mv.x + update * cand.x, keep * mv.y + update * cand.y);
           if (store_msg) msg[gi] = v[j];
         }
@@ -413,7 +413,7 @@ template <int COLS>
 __global__ void __launch_bounds__(16 * COLS)
 k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
                 const float2* __restrict__ mid, const float2* __restrict__ y,
-                const uint16_t* __restrict__ mbits, float2* __restrict__ pred,
+                const uint16_t* __restrict__ mbits, float2* __restrict__ resid,
                 float2* __restrict__ msg, const double* __restrict__ sc, double* __restrict__ part,
                 int pstride, int first_message, float damping, const float2* __restrict__ twtab) {
   __shared__ float2 xs[16 * 17 * COLS];   // column c's element i at xs[i * COLS + c]
@@ -451,9 +451,9 @@ k_col16_forward(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict
       const int i = t + 16 * n;
       if (act && i < s.m1) {
         const long long gi = s.samp_off + (long long)i * s.m2 + col;
-        pred[gi] = v[n];
         const bool ob = (bits >> n) & 1u;
         const float rx = yv[n].x - v[n].x, ry = yv[n].y - v[n].y;
+        resid[gi] = make_float2(rx, ry);
         if (ob) acc += (double)rx * rx + (double)ry * ry;
         if (first_message) {
           // First message from zero state (triton_kernels.py:66-77 with

@@ -27,11 +27,11 @@ def kernel_bytes(n1: int, n2: int, m1: int, m2: int) -> dict[str, int]:
     m = m1 * m2
     mid = n1 * m2
     return {
-        # y, prediction, old message in; new message and [N1, M2] out
-        "col_adjoint": 24 * m + 8 * m + 8 * mid,
+        # residual y - A mu and old message in; new message and [N1, M2] out
+        "col_adjoint": 16 * m + 8 * m + 8 * mid,
         # [N1, M2] in; gamma, mean, variance in and out; [N1, M2] out
         "row": 8 * mid + 16 * n + 16 * n + 8 * mid,
-        # [N1, M2] in, data in, prediction out
+        # [N1, M2] in, data in, residual out
         "col_forward": 8 * mid + 8 * m + 8 * m,
         # x, posterior variance, precision, support mask out
         "output": 8 * n + 4 * n + 4 * n + n,

@@ -185,7 +185,7 @@ class NativePlan:
 
     def debug_tensor(self, which: int, dtype: torch.dtype, count: int):
         """View of internal plan state (tests only): 0 gamma, 1 mean,
-        2 variance, 3 backprojection, 4 scalars, 5 prediction, 6 message."""
+        2 variance, 3 backprojection, 4 scalars, 5 residual y - A mu, 6 message."""
         ptr = ctypes.c_void_p()
         _lib.check(self._lib.fbme_plan_debug_pointer(self._handle, which,
                                                       ctypes.byref(ptr)),

@@ -147,7 +147,7 @@ enum {
 };
 int fbme_plan_run_stage(const fbme_plan* plan, int stage, const FbmeIO* io, void* stream);
 /* Device pointers of internal state for tests: 0 gamma, 1 mean, 2 variance,
- * 3 backprojection, 4 per-scene scalars (double[8]), 5 prediction,
+ * 3 backprojection, 4 per-scene scalars (double[8]), 5 residual y - A mu,
  * 6 message. */
 int fbme_plan_debug_pointer(const fbme_plan* plan, int which, void** out);
 
