@@ -209,6 +209,13 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p,
                                        c_float, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
+    # exact dense posterior (mode="dense")
+    "fbme_dense_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int, c_int]),
+    "fbme_dense_posterior": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                     c_int, c_int, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_double, c_double,
+                                     c_void_p, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p]),
     # off-grid refinement
     "fbme_offgrid_refine": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                     c_int, c_void_p, c_void_p, c_int, c_int,

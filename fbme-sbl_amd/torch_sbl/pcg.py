@@ -52,7 +52,9 @@ from .operators import _next_smooth_length
 
 F32_EPS = float(np.finfo(np.float32).eps)
 F32_TINY = float(np.finfo(np.float32).tiny)
+F64_EPS = float(np.finfo(np.float64).eps)
 RANK = 32
+DENSE_MAX = 2048
 
 _TABLES: dict = {}
 
@@ -142,8 +144,6 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append("spatial coupling")
     if cfg.fixed_noise_variance is not None:
         bad.append("fixed_noise_variance")
-    if cfg.initialization != "backprojection":
-        bad.append("initialization='uniform'")
     if bad:
         raise NotImplementedError(
             "mode='cg' on MI355X does not implement: " + ", ".join(bad))
@@ -176,8 +176,11 @@ USE_GRAPHS = True
 class _Solver:
     """Device buffers and kernel calls of one (B, grid, samples, P) problem."""
 
-    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None, observation_mask=None):
+    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None, observation_mask=None, dense=False):
         self.lib = _lib.load()
+        # mode="dense": exact posterior through the M x M covariance, no
+        # probes and no Krylov solve (solver.py:2962-2990)
+        self.dense = bool(dense)
         self.dev = y.device
         self.cfg = cfg
         self.b = y.shape[0]
@@ -199,7 +202,10 @@ class _Solver:
             if self.m_obs < self.m:
                 self.obs = keep.to(torch.uint8).contiguous()
                 self.obs_idx = torch.nonzero(keep).reshape(-1)
-        if min(self.n, self.m_obs) < RANK:
+        if self.dense:
+            if self.m_obs > DENSE_MAX:
+                raise NotImplementedError(f"mode='dense' on MI355X takes at most {DENSE_MAX} observed samples.")
+        elif min(self.n, self.m_obs) < RANK:
             raise NotImplementedError("mode='cg' needs at least 32 (observed) samples and grid cells "
                                       "(rank-32 preconditioner)")
         # joint sparsity: the snapshots share gamma (B identical rows); with
@@ -209,9 +215,11 @@ class _Solver:
         self.shared = self.joint and bool(cfg.joint_noise_variance)
         if self.joint and self.b < 2:
             raise ValueError("joint_sparsity=True requires at least two leading batch items.")
+        if self.joint and self.dense:
+            raise NotImplementedError("mode='dense' with joint_sparsity is not implemented on MI355X.")
         self.nsys = 1 if self.shared else self.b
         self.t = _tables(self.grid, self.samples, self.dev)
-        self.p = _probe_count(self.n, self.m_obs, cfg)
+        self.p = 0 if self.dense else _probe_count(self.n, self.m_obs, cfg)
         self.r = (self.b if self.shared else 1) + self.p
         self.lanes = self.nsys * self.r
         self.tiles = autotune.cg_tiles(self.dev, *self.samples, self.t.e1, self.t.e2)
@@ -254,7 +262,12 @@ class _Solver:
             self.rhs[:, nd:, self.obs_idx] = probes
         self.lprobes = torch.empty((ns, self.p, n), dtype=c64, device=dev)
         self.mid = torch.empty(self.lanes * n1 * max(m2, 1), dtype=c64, device=dev)
-        self._adjoint(self.rhs[:, nd:].reshape(ns * self.p, m), self.lprobes.view(ns * self.p, n))
+        if self.p:
+            self._adjoint(self.rhs[:, nd:].reshape(ns * self.p, m), self.lprobes.view(ns * self.p, n))
+        if self.dense:
+            nbytes = int(self.lib.fbme_dense_workspace_bytes(b, n1, n2, m1, m2, self.m_obs))
+            self.dense_work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+            self.dense_obs = (self.obs_idx.to(torch.int32).contiguous() if self.obs_idx is not None else None)
         # state
         self.gamma = torch.empty((b, n), dtype=f32, device=dev)
         self.power = torch.empty(b, dtype=f32, device=dev)
@@ -416,15 +429,20 @@ class _Solver:
                                          _ptr(self.dot_part), _ptr(self.bpe)), "bpe")
         self._chk(self.lib.fbme_cg_init(self.stream, self.b, self.n, self.m_obs, _ptr(self.bp), _ptr(self.ye),
                                         _ptr(self.bpe), float(cfg.min_variance),
-                                        float(cfg.initial_noise_fraction), float(cfg.initialization_blend),
+                                        float(cfg.initial_noise_fraction), self._blend(),
                                         _ptr(self.power), _ptr(self.floor), _ptr(self.noise),
                                         _ptr(self.gamma)), "cg_init")
         if self.joint:
             self._chk(self.lib.fbme_joint_init(self.stream, self.b, self.n, _ptr(self.bp), _ptr(self.bpe),
                                                _ptr(self.power), _ptr(self.floor),
-                                               float(cfg.initialization_blend), 1 if self.shared else 0,
+                                               self._blend(), 1 if self.shared else 0,
                                                _ptr(self.gamma), _ptr(self.noise)), "joint_init")
         self.grid_sums(with_mean=False)
+
+    def _blend(self):
+        """Backprojection blend of the initial variances; 'uniform' is the
+        blend 0 (solver.py:1070-1089)."""
+        return 0.0 if self.cfg.initialization == "uniform" else float(self.cfg.initialization_blend)
 
     def grid_sums(self, with_mean=True):
         self._chk(self.lib.fbme_grid_sums(self.stream, self.b, self.n, _ptr(self.gamma),
@@ -608,6 +626,17 @@ class _Solver:
             self._chk(self.lib.fbme_relevance_sum(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.dd),
                                                   _ptr(self.grid_part), _ptr(self.rel_sum)), "relevance")
             return 0
+        if self.dense:
+            n1, n2 = self.grid
+            m1, m2 = self.samples
+            self._chk(self.lib.fbme_dense_posterior(
+                self.stream, self.b, n1, n2, m1, m2, self.m_obs, _ptr(self.dense_obs), _ptr(self.y),
+                # the factorisation runs in FP64, so the loading uses the FP64 eps
+                # (the reference loads with the eps of its matrix dtype)
+                _ptr(self.gamma), _ptr(self.noise), float(self.cfg.cuda_cholesky_jitter_multiplier) * F64_EPS,
+                F32_TINY, _ptr(self.dense_work), _ptr(self.mean), _ptr(self.var), _ptr(self.dd),
+                _ptr(self.grid_part), _ptr(self.rel_sum), _ptr(self.status)), "dense_posterior")
+            return 0
         self.setup_operators()
         iters = self.pcg(tol)
         self._adjoint(self.x.view(self.lanes, self.m), self.lifted.view(self.lanes, self.n))
@@ -619,7 +648,8 @@ class _Solver:
 
     def em(self, update=True):
         cfg = self.cfg
-        keep = max(cfg.damping, cfg.cg_damping)
+        # solver.py:1483-1487: the CG damping applies to mode="cg" only
+        keep = cfg.damping if self.dense else max(cfg.damping, cfg.cg_damping)
         if self.joint:
             self._chk(self.lib.fbme_cg_em_joint(
                 self.stream, self.b, self.n, self.m_obs, _ptr(self.gamma), _ptr(self.mean), _ptr(self.var),
@@ -677,7 +707,7 @@ class SolverStall(RuntimeError):
 
 def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
            probes_override=None, warm_start=None, stall_check=None,
-           observation_mask=None) -> SBLResult:
+           observation_mask=None, dense=False) -> SBLResult:
     """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64),
     optionally on the samples of a boolean observation_mask [M1, M2].
 
@@ -686,15 +716,17 @@ def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
     is consulted after every EM iteration and raises SolverStall when true."""
     _check_supported(cfg)
     with torch.cuda.device(y.device):
-        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check, observation_mask)
+        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check, observation_mask, dense)
 
 
-def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None, observation_mask=None):
-    s = _Solver(y, grid, cfg, probes_override, observation_mask)
+def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None, observation_mask=None,
+         dense=False):
+    s = _Solver(y, grid, cfg, probes_override, observation_mask, dense)
     s.init()
     if warm_start is not None:
         warm_start(s)
-    min_iters = max(cfg.min_iter, cfg.cg_min_iter)
+    # solver.py:1285-1292
+    min_iters = cfg.min_iter if dense else max(cfg.min_iter, cfg.cg_min_iter)
     convergence = cfg.tol > 0.0 and min_iters < cfg.max_iter
     tol_floor = 3.0 * F32_EPS ** 0.5
     history = {k: [] for k in ("relative_change", "noise_variance", "effective_dof", "cg_iterations",
@@ -705,7 +737,7 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
     change = float("inf")
     it = 0
     for it in range(1, cfg.max_iter + 1):
-        if (cfg.active_set and s.b == 1 and s.support is None and it >= cfg.active_set_start
+        if (cfg.active_set and not dense and s.b == 1 and s.support is None and it >= cfg.active_set_start
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             s.try_active_set()
         tol = max(_schedule(it, cfg), tol_floor)
@@ -740,6 +772,8 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
     final_tol = max(cfg.cg_tolerance, tol_floor)
     s.posterior(final_tol)
     s.em(update=False)
+    if dense and bool(s.status.any()):
+        raise RuntimeError("CUDA Cholesky failed after deterministic diagonal loading.")
     precision = torch.empty_like(s.gamma)
     _lib.check(s.lib.fbme_reciprocal(s.stream, s.gamma.numel(), _ptr(s.gamma), _ptr(precision)), "reciprocal")
     bshape = tuple(batch_shape)
@@ -755,7 +789,7 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
         iterations=it,
         converged=converged,
         relative_change=rel,
-        mode="cg",
+        mode="dense" if dense else "cg",
         arithmetic_dtype="complex64",
         uamp_backend_used=None,
         active_set_size=0 if s.support is None else int(s.support.shape[1]),

@@ -145,14 +145,11 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
         cfg.dense_threshold, 512)
     mode = ("dense" if cfg.mode == "auto" and m <= threshold else
             "cg" if cfg.mode == "auto" else cfg.mode)
-    if mode == "dense":
-        raise NotImplementedError(
-            "mode='dense' (exact M x M EM) has no GPU implementation in this "
-            "build; use mode='cg' or mode='uamp'.")
-    if mode == "cg":
+    if mode in ("cg", "dense"):
         from . import pcg
         res = pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
-                         batch_shape=batch_shape, observation_mask=mask)
+                         batch_shape=batch_shape, observation_mask=mask,
+                         dense=mode == "dense")
         return _with_offgrid(res, y, grid, cfg)
     if cfg.uamp_backend == "torch":
         raise NotImplementedError(

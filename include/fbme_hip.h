@@ -397,6 +397,25 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
                                   const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
                                   float jitter_multiplier, void* mean, float* var, float* dd, int* status);
 /* ------------------------------------------------------------------------
+ * Exact dense posterior of the general solver, mode="dense": _posterior's
+ * dense branch (torch_sbl/solver.py:2962-2990) with Q = noise I + A Gamma
+ * A^H built from the Fourier lags of gamma (operators.py:282-297), the GPU
+ * Cholesky loading jitter * mean|diag| (solver.py:2898-2920; jitter =
+ * cuda_cholesky_jitter_multiplier * eps) and the data diagonal
+ * diag(A^H Q^-1 A) (operators.py:418-442), in FP64, one workgroup per scene
+ * for the factorisation.  observed <= 2048 samples; obs_idx [observed]
+ * int32 lists their sample indices (NULL: all M1 M2).  y is [B, M1 M2]
+ * complex64 in the full sample layout.  Writes mean [B, N] complex64, var
+ * and dd [B, N] float32, rel_sum[b] = sum clamp(gamma dd, 0, 1); part:
+ * B * ceil(N / 256) doubles; status[b] = 1 on a failed pivot.  work:
+ * fbme_dense_workspace_bytes(batch, n1, n2, m1, m2, observed) bytes. */
+size_t fbme_dense_workspace_bytes(int batch, int n1, int n2, int m1, int m2, int observed);
+int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2, int observed, const int* obs_idx,
+                         const void* y, const float* gamma, const float* noise, double jitter, double tiny,
+                         void* work, void* mean, float* var, float* dd, double* part, double* rel_sum,
+                         int* status);
+
+/* ------------------------------------------------------------------------
  * Off-grid refinement: refine_offgrid_fourier (torch_sbl/offgrid.py:70-226)
  * for 2-D grids.  One workgroup per scene runs the whole refinement:
  * top max(64, 16 max_components) magnitudes of the coefficients, periodic
