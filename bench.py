@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--skip-masked", action="store_true")
     ap.add_argument("--skip-joint", action="store_true")
     ap.add_argument("--skip-offgrid", action="store_true")
+    ap.add_argument("--skip-dense", action="store_true")
     ap.add_argument("--learned-scenes", type=int, default=16)
     return ap.parse_args()
 
@@ -380,6 +381,52 @@ def offgrid_workload(device, count=64, reps=5, cpu=True):
     return out
 
 
+def dense_workload(device, cpu=True):
+    """mode='dense' under the dense EM configuration
+    (literature_reproduction/baselines.py:254-278), 30 EM iterations:
+    one 32x32/24x24 scene (M = 576), a batch of 16 such scenes, and one
+    64x64/40x40 scene (M = 1600); CPU baseline: the float64 oracle on the
+    32x32 scene."""
+    import time
+
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, sbl_2d
+
+    def cfg(iters=30):
+        return SBLConfig(mode="dense", max_iter=iters, min_iter=iters, tol=0.0, hyper_shape=1e-6,
+                         hyper_rate=1e-6, noise_shape=1e-6, noise_rate=1e-6, initialization="uniform",
+                         damping=0.0, noise_update="evidence", active_set=False, device=str(device),
+                         return_history=False, eager_change_tracking=False)
+
+    out = {"workload": "dense EM, 30 iterations"}
+    cases = (("32x32_24x24", (32, 32), (24, 24), 1), ("32x32_24x24_b16", (32, 32), (24, 24), 16),
+             ("64x64_40x40", (64, 64), (40, 40), 1))
+    for name, grid, samples, b in cases:
+        sc = [scenes.uniform_scene(2100 + i, grid, samples, k=20, snr_db=20.0) for i in range(b)]
+        y = torch.from_numpy(np.stack([s.y for s in sc]).astype(np.complex64)).to(device)
+        y = y if b > 1 else y[0]
+        sbl_2d(y, grid, config=cfg(2))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sbl_2d(y, grid, config=cfg())
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        nm = scenes.nmse_db(res.x.reshape(b, -1)[0].cpu().numpy(), sc[0].x.reshape(-1))
+        out[name] = {"s": el, "scenes_per_s": b / el, "nmse_db_first": nm}
+    if cpu:
+        from oracle import dense_em
+        sc = scenes.uniform_scene(2100, (32, 32), (24, 24), k=20, snr_db=20.0)
+        t0 = time.perf_counter()
+        dense_em.dense_em_sbl(sc.y.astype(np.complex128), (32, 32), 30)
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": 1.0 / el, "unit": "scenes/s", "kind": "port",
+                               "sample": "1 scene 32x32/24x24, float64 NumPy dense EM oracle, 30 iterations"}
+    return out
+
+
 def learned_comparison(device, count):
     """Learned warm start against the plain general solver (mode='cg', 16
     probes, PCG tolerance 1e-4), one 256x256/160x160 K=300 10 dB scene per
@@ -588,6 +635,11 @@ def main():
     learned_cmp = None
     if not args.skip_learned:
         learned_cmp = learned_comparison(device, args.learned_scenes)
+    dense = None
+    if not args.skip_dense:
+        print("[bench] dense mode", file=sys.stderr, flush=True)
+        dense = dense_workload(device, cpu=not args.no_cpu_baseline)
+        print(f"[bench] dense: {dense}", file=sys.stderr, flush=True)
     offgrid = None
     if not args.skip_offgrid:
         print("[bench] off-grid refinement", file=sys.stderr, flush=True)
@@ -630,6 +682,7 @@ def main():
         "masked_workload": masked,
         "joint_sparsity": joint,
         "offgrid_refinement": offgrid,
+        "dense_mode": dense,
         "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),
         "wall_s": wall,

@@ -6,8 +6,11 @@
 //     exp(-2 pi i l.n / N) at the (2 M1 - 1)(2 M2 - 1) sample-index
 //     differences, a separable FP64 DFT;
 //   * Cholesky with the GPU's one deterministic loading jitter * mean|diag|
-//     (solver.py:2898-2920), then the triangular inverse W = L^-1, FP64, one
-//     workgroup per scene (left-looking, the pivot row staged in LDS);
+//     (solver.py:2898-2920), then the triangular inverse W = L^-1, FP64, in
+//     32 x 32 tiles staged in LDS: per block column a diagonal-tile kernel
+//     (factor and inverse), a panel kernel and a trailing-update kernel over
+//     all tile pairs, then one launch per block row of W; the matrix is padded
+//     to a multiple of 32 with an identity block;
 //   * Q^-1 y = W^H (W y);
 //   * the data diagonal diag(A^H Q^-1 A)_n = sum_k |(A^H conj(w_k))_n|^2 over
 //     the rows w_k of W (a_n^H Q^-1 a_n = ||W a_n||^2; the reference
@@ -71,20 +74,28 @@ __global__ void k_lag_cols(const double2* __restrict__ t, int n1, int m1, int m2
 }
 
 // Q[b, r, c] = lags[s_r - s_c] + noise delta_rc over the observed samples
-// (obs[r] = sample index m1 * M2 + m2 of row r; NULL: all samples).
-__global__ void k_dense_build(const double2* __restrict__ lags, const int* __restrict__ obs, int m, int m1, int m2,
-                              const float* __restrict__ noise, double2* __restrict__ q) {
+// (obs[r] = sample index m1 * M2 + m2 of row r; NULL: all samples), plus
+// the loading jitter * mean|diag| on the diagonal (every diagonal entry is
+// c[0] + noise); rows and columns m..mp-1 are an identity block.
+__global__ void k_dense_build(const double2* __restrict__ lags, const int* __restrict__ obs, int m, int mp, int m1,
+                              int m2, const float* __restrict__ noise, double jitter, double tiny,
+                              double2* __restrict__ q) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
-  if (e >= (long long)m * m) return;
-  const int r = (int)(e / m), c = (int)(e - (long long)r * m);
-  const int sr = obs ? obs[r] : r, sc = obs ? obs[c] : c;
-  const int d1 = sr / m2 - sc / m2, d2 = sr % m2 - sc % m2;
-  const long long li = (long long)b * (2 * m1 - 1) * (2 * m2 - 1) + (long long)(d1 + m1 - 1) * (2 * m2 - 1) +
-                       (d2 + m2 - 1);
-  double2 v = lags[li];
-  if (r == c) v.x += (double)noise[b];
-  q[(long long)b * m * m + e] = v;
+  if (e >= (long long)mp * mp) return;
+  const int r = (int)(e / mp), c = (int)(e - (long long)r * mp);
+  double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  if (r < m && c < m) {
+    const long long base = (long long)b * (2 * m1 - 1) * (2 * m2 - 1);
+    const int sr = obs ? obs[r] : r, sc = obs ? obs[c] : c;
+    const int d1 = sr / m2 - sc / m2, d2 = sr % m2 - sc % m2;
+    v = lags[base + (long long)(d1 + m1 - 1) * (2 * m2 - 1) + (d2 + m2 - 1)];
+    if (r == c) {
+      const double diag = lags[base + (long long)(m1 - 1) * (2 * m2 - 1) + (m2 - 1)].x + (double)noise[b];
+      v.x += (double)noise[b] + jitter * fmax(fabs(diag), tiny);
+    }
+  }
+  q[(long long)b * mp * mp + e] = v;
 }
 
 __device__ double block_sum_dn(double v, double* red) {
@@ -100,89 +111,217 @@ __device__ double block_sum_dn(double v, double* red) {
   return r;
 }
 
-// In place: Q + jitter * mean|Re diag Q| I = L L^H (left-looking, row j of
-// L staged in LDS), then W = L^-1 (row by row, row i of L staged in LDS);
-// the upper triangle is zeroed.  One workgroup per scene.
+// ------------------------------------------------ tiled FP64 factorisation
+constexpr int TS = 32;
+
+__device__ __forceinline__ double2 zm(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zcm(double2 a, double2 b) {   // a * conj(b)
+  return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// Diagonal tile jb: in-LDS Cholesky L_jj and its inverse, L_jj written back
+// to Q and the inverse to linv[b, jb].
 __global__ void __launch_bounds__(DN_THREADS)
-k_dense_factor(double2* __restrict__ qs, int m, double jitter, double tiny, int* __restrict__ status) {
-  __shared__ double2 row[DN_MMAX];
-  __shared__ double red[DN_THREADS];
-  __shared__ double piv_s;
-  __shared__ int bad;
-  const int t = threadIdx.x, b = blockIdx.x;
-  double2* q = qs + (long long)b * m * m;
-  double s = 0.0;
-  for (int i = t; i < m; i += DN_THREADS) s += fabs(q[(long long)i * m + i].x);
-  const double load = jitter * fmax(block_sum_dn(s, red) / m, tiny);
-  if (t == 0) bad = 0;
-  for (int i = t; i < m; i += DN_THREADS) q[(long long)i * m + i].x += load;
+k_tile_diag(double2* __restrict__ qs, int mp, int jb, double tiny, double2* __restrict__ linv,
+            int* __restrict__ status) {
+  __shared__ double2 a[TS][TS + 1];
+  __shared__ double2 x[TS][TS + 1];
+  const int t = threadIdx.x, b = blockIdx.x, nb = mp / TS;
+  double2* q = qs + (long long)b * mp * mp;
+  const long long o = (long long)jb * TS * mp + (long long)jb * TS;
+  for (int e = t; e < TS * TS; e += DN_THREADS) a[e / TS][e % TS] = q[o + (long long)(e / TS) * mp + e % TS];
   __syncthreads();
-  for (int j = 0; j < m; ++j) {
-    for (int k = t; k < j; k += DN_THREADS) row[k] = q[(long long)j * m + k];
-    __syncthreads();
-    double d = 0.0;
-    for (int k = t; k < j; k += DN_THREADS) d += row[k].x * row[k].x + row[k].y * row[k].y;
-    d = block_sum_dn(d, red);
+  for (int j = 0; j < TS; ++j) {
     if (t == 0) {
-      const double v = q[(long long)j * m + j].x - d;
-      if (!(v > 0.0) || !isfinite(v)) bad = 1;
-      piv_s = sqrt(fmax(v, tiny));
-      q[(long long)j * m + j] = make_double2(piv_s, 0.0);
+      const double d = a[j][j].x;
+      if ((!(d > 0.0) || !isfinite(d)) && status) status[b] = 1;
+      a[j][j] = make_double2(sqrt(fmax(d, tiny)), 0.0);
     }
     __syncthreads();
-    const double ip = 1.0 / piv_s;
-    for (int i = j + 1 + t; i < m; i += DN_THREADS) {
-      const double2* ri = q + (long long)i * m;
-      double ax = ri[j].x, ay = ri[j].y;
-      for (int k = 0; k < j; ++k) {
-        // a -= L[i][k] conj(L[j][k])
-        const double2 u = ri[k], w = row[k];
-        ax -= u.x * w.x + u.y * w.y;
-        ay -= u.y * w.x - u.x * w.y;
+    const double ip = 1.0 / a[j][j].x;
+    for (int i = j + 1 + t; i < TS; i += DN_THREADS) a[i][j] = make_double2(a[i][j].x * ip, a[i][j].y * ip);
+    __syncthreads();
+    const int rem = TS - j - 1;
+    for (int e = t; e < rem * rem; e += DN_THREADS) {
+      const int i = j + 1 + e / rem, k = j + 1 + e % rem;
+      if (k <= i) {
+        const double2 v = zcm(a[i][j], a[k][j]);
+        a[i][k] = make_double2(a[i][k].x - v.x, a[i][k].y - v.y);
       }
-      q[(long long)i * m + j] = make_double2(ax * ip, ay * ip);
     }
     __syncthreads();
   }
-  // W = L^-1: W[i][c] = -(sum_{k=c}^{i-1} L[i][k] W[k][c]) / L[i][i]
-  for (int i = 0; i < m; ++i) {
-    for (int k = t; k <= i; k += DN_THREADS) row[k] = q[(long long)i * m + k];
-    __syncthreads();
-    const double il = 1.0 / row[i].x;
-    for (int c = t; c < i; c += DN_THREADS) {
-      double ax = 0.0, ay = 0.0;
+  // inverse of the lower factor, column c by thread c
+  if (t < TS) {
+    const int c = t;
+    for (int i = 0; i < TS; ++i) {
+      if (i < c) {
+        x[i][c] = make_double2(0.0, 0.0);
+        continue;
+      }
+      double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
       for (int k = c; k < i; ++k) {
-        const double2 l = row[k], w = q[(long long)k * m + c];
-        ax += l.x * w.x - l.y * w.y;
-        ay += l.x * w.y + l.y * w.x;
+        const double2 v = zm(a[i][k], x[k][c]);
+        s = make_double2(s.x - v.x, s.y - v.y);
       }
-      q[(long long)i * m + c] = make_double2(-ax * il, -ay * il);
+      const double il = 1.0 / a[i][i].x;
+      x[i][c] = make_double2(s.x * il, s.y * il);
     }
-    if (t == 0) q[(long long)i * m + i] = make_double2(il, 0.0);
+  }
+  __syncthreads();
+  double2* li = linv + ((long long)b * nb + jb) * TS * TS;
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    const int i = e / TS, k = e % TS;
+    q[o + (long long)i * mp + k] = k <= i ? a[i][k] : make_double2(0.0, 0.0);
+    li[e] = x[i][k];
+  }
+}
+
+// Panel: L_ib,jb = A_ib,jb L_jj^-H for ib = jb + 1 + blockIdx.x.
+__global__ void __launch_bounds__(DN_THREADS)
+k_tile_panel(double2* __restrict__ qs, int mp, int jb, const double2* __restrict__ linv) {
+  __shared__ double2 a[TS][TS + 1];
+  __shared__ double2 x[TS][TS + 1];
+  const int t = threadIdx.x, b = blockIdx.y, nb = mp / TS;
+  const int ib = jb + 1 + blockIdx.x;
+  double2* q = qs + (long long)b * mp * mp;
+  const long long o = (long long)ib * TS * mp + (long long)jb * TS;
+  const double2* li = linv + ((long long)b * nb + jb) * TS * TS;
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    a[e / TS][e % TS] = q[o + (long long)(e / TS) * mp + e % TS];
+    x[e / TS][e % TS] = li[e];
+  }
+  __syncthreads();
+  const int r = t >> 3, c0 = t & 7;
+  double2 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < TS; ++k) {
+    const double2 v = a[r][k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 p = zcm(v, x[c0 + 8 * u][k]);   // A[r][k] conj(Linv[c][k])
+      acc[u] = make_double2(acc[u].x + p.x, acc[u].y + p.y);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) q[o + (long long)r * mp + c0 + 8 * u] = acc[u];
+}
+
+// Trailing update: A_ib,kb -= L_ib,jb L_kb,jb^H for jb < kb <= ib, one
+// workgroup per tile pair (blockIdx.x enumerates the lower-triangular pairs).
+__global__ void __launch_bounds__(DN_THREADS)
+k_tile_update(double2* __restrict__ qs, int mp, int jb) {
+  __shared__ double2 l1[TS][TS + 1];
+  __shared__ double2 l2[TS][TS + 1];
+  const int t = threadIdx.x, b = blockIdx.y;
+  // pair index p -> (i, k) with 0 <= k <= i over the remaining tiles
+  int p = blockIdx.x, i = 0;
+  while (p > i) {
+    p -= i + 1;
+    ++i;
+  }
+  const int ib = jb + 1 + i, kb = jb + 1 + p;
+  double2* q = qs + (long long)b * mp * mp;
+  const long long oi = (long long)ib * TS * mp + (long long)jb * TS, ok = (long long)kb * TS * mp + (long long)jb * TS;
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    l1[e / TS][e % TS] = q[oi + (long long)(e / TS) * mp + e % TS];
+    l2[e / TS][e % TS] = q[ok + (long long)(e / TS) * mp + e % TS];
+  }
+  __syncthreads();
+  const int r = t >> 3, c0 = t & 7;
+  double2 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < TS; ++k) {
+    const double2 v = l1[r][k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 w = zcm(v, l2[c0 + 8 * u][k]);
+      acc[u] = make_double2(acc[u].x + w.x, acc[u].y + w.y);
+    }
+  }
+  const long long o = (long long)ib * TS * mp + (long long)kb * TS;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    double2* d = q + o + (long long)r * mp + c0 + 8 * u;
+    *d = make_double2(d->x - acc[u].x, d->y - acc[u].y);
+  }
+}
+
+// Block row i of W = L^-1 (into w, separate from L): W_ii = L_ii^-1 and, for
+// j = blockIdx.x < i, W_ij = -L_ii^-1 sum_{k=j}^{i-1} L_ik W_kj.
+__global__ void __launch_bounds__(DN_THREADS)
+k_tile_inverse_row(const double2* __restrict__ qs, int mp, int ib, const double2* __restrict__ linv,
+                   double2* __restrict__ ws) {
+  __shared__ double2 l[TS][TS + 1];
+  __shared__ double2 x[TS][TS + 1];
+  __shared__ double2 s[TS][TS + 1];
+  const int t = threadIdx.x, b = blockIdx.y, nb = mp / TS, jb = blockIdx.x;
+  const double2* q = qs + (long long)b * mp * mp;
+  double2* w = ws + (long long)b * mp * mp;
+  const double2* lii = linv + ((long long)b * nb + ib) * TS * TS;
+  const int r = t >> 3, c0 = t & 7;
+  if (jb == ib) {
+    for (int e = t; e < TS * TS; e += DN_THREADS)
+      w[(long long)(ib * TS + e / TS) * mp + ib * TS + e % TS] = lii[e];
+    return;
+  }
+  double2 acc[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int kb = jb; kb < ib; ++kb) {
+    for (int e = t; e < TS * TS; e += DN_THREADS) {
+      l[e / TS][e % TS] = q[(long long)(ib * TS + e / TS) * mp + kb * TS + e % TS];
+      x[e / TS][e % TS] = w[(long long)(kb * TS + e / TS) * mp + jb * TS + e % TS];
+    }
+    __syncthreads();
+    for (int k = 0; k < TS; ++k) {
+      const double2 v = l[r][k];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double2 p = zm(v, x[k][c0 + 8 * u]);
+        acc[u] = make_double2(acc[u].x + p.x, acc[u].y + p.y);
+      }
+    }
     __syncthreads();
   }
-  for (long long e = t; e < (long long)m * m; e += DN_THREADS) {
-    const int r = (int)(e / m), c = (int)(e - (long long)r * m);
-    if (c > r) q[e] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) s[r][c0 + 8 * u] = acc[u];
+  for (int e = t; e < TS * TS; e += DN_THREADS) l[e / TS][e % TS] = lii[e];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
+  for (int k = 0; k < TS; ++k) {
+    const double2 v = l[r][k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 p = zm(v, s[k][c0 + 8 * u]);
+      acc[u] = make_double2(acc[u].x + p.x, acc[u].y + p.y);
+    }
   }
-  if (t == 0 && bad && status) status[b] = 1;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    w[(long long)(ib * TS + r) * mp + jb * TS + c0 + 8 * u] = make_double2(-acc[u].x, -acc[u].y);
 }
 
 // Rows of the transforms, FP64 in the full sample layout (zeros on the
 // unobserved samples): rows 0..m-1 = conj(W[k, :]), row m = z = W^H (W y).
 __global__ void __launch_bounds__(DN_THREADS)
 k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, const int* __restrict__ obs, int m,
-              int mfull, double2* __restrict__ rows) {
+              int mp, int mfull, double2* __restrict__ rows) {
   __shared__ double2 u[DN_MMAX];
   const int t = threadIdx.x, b = blockIdx.x;
-  const double2* w = ws + (long long)b * m * m;
+  const double2* w = ws + (long long)b * mp * mp;
   const float2* yb = y + (long long)b * mfull;
   double2* rb = rows + (long long)b * (m + 1) * mfull;
   for (long long e = t; e < (long long)(m + 1) * mfull; e += DN_THREADS) rb[e] = make_double2(0.0, 0.0);
   for (int i = t; i < m; i += DN_THREADS) {
     double ax = 0.0, ay = 0.0;
     for (int k = 0; k <= i; ++k) {
-      const double2 a = w[(long long)i * m + k];
+      const double2 a = w[(long long)i * mp + k];
       const float2 v = yb[obs ? obs[k] : k];
       ax += a.x * v.x - a.y * v.y;
       ay += a.x * v.y + a.y * v.x;
@@ -193,7 +332,7 @@ k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, cons
   for (int k = t; k < m; k += DN_THREADS) {
     double ax = 0.0, ay = 0.0;
     for (int i = k; i < m; ++i) {
-      const double2 a = w[(long long)i * m + k];   // conj(W[i][k]) u_i
+      const double2 a = w[(long long)i * mp + k];   // conj(W[i][k]) u_i
       ax += a.x * u[i].x + a.y * u[i].y;
       ay += a.x * u[i].y - a.y * u[i].x;
     }
@@ -201,7 +340,8 @@ k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, cons
   }
   for (long long e = t; e < (long long)m * m; e += DN_THREADS) {
     const int r = (int)(e / m), c = (int)(e - (long long)r * m);
-    const double2 a = w[e];
+    if (c > r) continue;   // W is lower triangular
+    const double2 a = w[(long long)r * mp + c];
     rb[(long long)r * mfull + (obs ? obs[c] : c)] = make_double2(a.x, -a.y);
   }
 }
@@ -294,7 +434,7 @@ __global__ void k_dense_seg(const double* __restrict__ part, int blocks, double*
 }
 
 struct DenseLayout {
-  size_t t, lags, q, rows, t1, total;
+  size_t t, lags, q, w, linv, rows, t1, total;
 };
 
 inline size_t align256(size_t v) { return (v + 255) / 256 * 256; }
@@ -310,7 +450,10 @@ DenseLayout dense_layout(int batch, int n1, int n2, int m1, int m2, int m) {
   };
   l.t = take(16 * b * n1 * (2 * m2 - 1));
   l.lags = take(16 * b * (2 * m1 - 1) * (2 * m2 - 1));
-  l.q = take(16 * b * m * m);
+  const size_t mp = (size_t)((m + 31) / 32) * 32;
+  l.q = take(16 * b * mp * mp);
+  l.w = take(16 * b * mp * mp);
+  l.linv = take(16 * b * mp * 32);
   l.rows = take(16 * b * (m + 1) * mf);
   l.t1 = take(16 * b * (m + 1) * (size_t)m1 * n2);
   l.total = off;
@@ -342,6 +485,9 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
   double2* t = (double2*)(w + l.t);
   double2* lags = (double2*)(w + l.lags);
   double2* q = (double2*)(w + l.q);
+  double2* wmat = (double2*)(w + l.w);
+  double2* linv = (double2*)(w + l.linv);
+  const int mp = (m + TS - 1) / TS * TS, nb = mp / TS;
   double2* rows = (double2*)(w + l.rows);
   double2* t1 = (double2*)(w + l.t1);
   const long long rows_e = (long long)n1 * (2 * m2 - 1), lag_e = (long long)(2 * m1 - 1) * (2 * m2 - 1);
@@ -349,11 +495,21 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
                      t);
   hipLaunchKernelGGL(k_lag_cols, dim3((unsigned)((lag_e + 255) / 256), batch), dim3(256), 0, st, t, n1, m1, m2,
                      lags);
-  hipLaunchKernelGGL(k_dense_build, dim3((unsigned)(((long long)m * m + 255) / 256), batch), dim3(256), 0, st, lags,
-                     obs_idx, m, m1, m2, noise, q);
-  hipLaunchKernelGGL(k_dense_factor, dim3(batch), dim3(DN_THREADS), 0, st, q, m, jitter, tiny, status);
-  hipLaunchKernelGGL(k_dense_apply, dim3(batch), dim3(DN_THREADS), 0, st, q, (const float2*)y, obs_idx, m, mf,
-                     rows);
+  hipLaunchKernelGGL(k_dense_build, dim3((unsigned)(((long long)mp * mp + 255) / 256), batch), dim3(256), 0, st,
+                     lags, obs_idx, m, mp, m1, m2, noise, jitter, tiny, q);
+  for (int jb = 0; jb < nb; ++jb) {
+    hipLaunchKernelGGL(k_tile_diag, dim3(batch), dim3(DN_THREADS), 0, st, q, mp, jb, tiny, linv, status);
+    const int rest = nb - jb - 1;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_tile_panel, dim3(rest, batch), dim3(DN_THREADS), 0, st, q, mp, jb, (const double2*)linv);
+      hipLaunchKernelGGL(k_tile_update, dim3(rest * (rest + 1) / 2, batch), dim3(DN_THREADS), 0, st, q, mp, jb);
+    }
+  }
+  for (int ib = 0; ib < nb; ++ib)
+    hipLaunchKernelGGL(k_tile_inverse_row, dim3(ib + 1, batch), dim3(DN_THREADS), 0, st, (const double2*)q, mp, ib,
+                       (const double2*)linv, wmat);
+  hipLaunchKernelGGL(k_dense_apply, dim3(batch), dim3(DN_THREADS), 0, st, (const double2*)wmat, (const float2*)y,
+                     obs_idx, m, mp, mf, rows);
   const long long t1_e = (long long)(m + 1) * m1 * n2;
   hipLaunchKernelGGL(k_dense_t1, dim3((unsigned)((t1_e + DN_THREADS - 1) / DN_THREADS), batch), dim3(DN_THREADS), 0,
                      st, rows, m + 1, m1, m2, n2, t1);

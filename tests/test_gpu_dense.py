@@ -8,9 +8,10 @@ with the GPU's deterministic loading (solver.py:2898-2920) at the FP64 eps,
 transforms run in float32.  Against the
 float64 oracle the bar is 1e-4 relative L2 on the posterior mean, variance
 and data diagonal of one step; against the reference's complex128
-dense_em_sbl runs (30 and 200 EM iterations) 1e-5 on x, 1e-4 on the noise
-variance (the runs end with M - dof = 0.2, which amplifies any difference)
-and 1e-5 on the dof.
+dense_em_sbl runs (30 and 200 EM iterations) 1e-5 on x, 1e-3 on the noise
+variance (the runs end with M - dof = 0.2 in its denominator, which turns a
+1e-6 relative difference in dof into 3e-4 in the noise) and 1e-5 on the
+dof.
 """
 
 import numpy as np
@@ -47,7 +48,7 @@ def test_dense_em_matches_reference_golden(device, iters):
     x = res.x.cpu().numpy()
     print("dense", iters, _rel(x, g[pre + "x"]), flush=True)
     assert _rel(x, g[pre + "x"]) < 1e-5
-    assert float(res.noise_variance) == pytest.approx(float(g[pre + "noise_variance"]), rel=1e-4)
+    assert float(res.noise_variance) == pytest.approx(float(g[pre + "noise_variance"]), rel=1e-3)
     assert float(res.effective_dof) == pytest.approx(float(g[pre + "effective_dof"]), rel=1e-5)
 
 
