@@ -491,7 +491,7 @@ long long fbme_plan_query(const fbme_plan* p, int what) {
     case FBME_Q_NSCENES: return p->nscenes;
     case FBME_Q_EVALUATIONS: return p->opt.polish_iterations;
     case FBME_Q_LAUNCHES: {
-      long long n = 3;  // init
+      long long n = p->row_split > 0 ? 3 : 2;  // init
       for (int t = 1; t <= p->max_iters; ++t) n += (t > 1) ? 4 : 3;
       n += 2 + 1;       // final E-step + polish
       const bool exch = p->opt.exchange_size > 0 && p->total_xcand > 1;
@@ -635,7 +635,9 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   };
   // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
   FBME_TRY(col_adjoint(p->col_split, ncol16, 0, 0));
-  FBME_TRY(row(p->row_split, nrow16, 0, nullptr, nullptr, 0));
+  // Register-FFT rows form A^H y inside their first UAMP step (every scene
+  // runs at least one: min_iters >= 1); the LDS kernels write it here.
+  FBME_TRY(row(p->row_split, 0, 0, nullptr, nullptr, 0));
   FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction, o.blend,
                                o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
   for (int t = 1; t <= p->max_iters; ++t) {

@@ -64,11 +64,10 @@ enum FbmeScalar {
 // Per-scene partial sums (doubles), `FBME_PART_STRIDE` slots per quantity.
 enum FbmePart {
   PT_YE = 0,   // sum |y|^2                      (column pass, init)
-  PT_BPE = 1,  // sum |A^H y|^2                  (row pass, init)
-  PT_DOF = 2,  // sum relevance                  (row pass)
-  PT_VS = 3,   // sum posterior variance         (row pass)
-  PT_RES = 4,  // sum |y - A mu|^2               (column pass)
-  PT_COUNT = 5
+  PT_DOF = 1,  // sum relevance                  (row pass)
+  PT_VS = 2,   // sum posterior variance         (row pass)
+  PT_RES = 3,  // sum |y - A mu|^2               (column pass)
+  PT_COUNT = 4
 };
 
 

@@ -128,7 +128,7 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
 }
 
 // -------------------------------------------------------------------- row pass
-// MODE 0: backprojection rows (A^H y), partial sum |A^H y|^2.
+// MODE 0: backprojection rows (A^H y).
 // MODE 1: first UAMP step: lifted = bp * (1-d)/(dv+sigma^2), gamma0 built from
 //         the backprojection (solver.py:1063-1099), mean_old = 0, v_old = gamma0.
 // MODE 2: UAMP step: lifted = row pass of A^H of the column output.
@@ -197,19 +197,14 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
     __syncthreads();
     if (MODE != 1) fft_lines<EPT, true>(buf, len, rows, stride, s.rad2, tw);
     if (MODE == 0) {
-      double acc = 0.0;
 #pragma unroll
       for (int q = 0; q < EPT; ++q) {
         const int e = threadIdx.x + q * FBME_BLOCK;
         if (e < nel) {
           const int r = e / len, j = e - r * len;
-          const float2 v = buf[lds_idx(r * stride + j)];
-          bp[gbase + e] = v;
-          acc += (double)v.x * v.x + (double)v.y * v.y;
+          bp[gbase + e] = buf[lds_idx(r * stride + j)];
         }
       }
-      acc = block_sum(acc, red);
-      if (threadIdx.x == 0) part[(long long)PT_BPE * pstride + s.row_part + wk.slot] = acc;
       __syncthreads();
       continue;
     }
@@ -372,6 +367,9 @@ struct ScalarArgs {
 // mode 0: initialisation (solver.py:1063-1189): power, floor, sigma^2_0,
 //         sum |A^H y|^2 and sum gamma0 (closed form, the floor clamp is
 //         inactive because gamma0 >= 0.9 P / N > max(1e-12 P, tiny)).
+//         A^H y is the unnormalised 2-D inverse DFT of the zero-filled
+//         samples, so by Parseval sum |A^H y|^2 = N1 N2 sum |y|^2 (masked
+//         samples are zero); no pass over the backprojection is needed.
 // mode 1: end of a UAMP iteration: dof clamp, noise EM/evidence update with
 //         damping (triton_kernels.py:375-417, solver.py:1764-1798), data
 //         variance = sum v for the next message update.
@@ -389,7 +387,6 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
   double q0 = 0.0, q1 = 0.0, q2 = 0.0;
   if (mode == 0) {
     for (int i = threadIdx.x; i < s.col_items; i += 64) q0 += part_get(part, pstride, PT_YE, s.col_part + i);
-    for (int i = threadIdx.x; i < s.row_items; i += 64) q1 += part_get(part, pstride, PT_BPE, s.row_part + i);
   } else {
     for (int i = threadIdx.x; i < s.row_items; i += 64) {
       q0 += part_get(part, pstride, PT_DOF, s.row_part + i);
@@ -404,7 +401,7 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
   if (mode == 0) {
     const float power = fmaxf((float)(q0 / n_samp), F32_EPS);
     const float floor_v = fmaxf(power * a.min_variance, F32_TINY);
-    const float bpe = (float)q1;
+    const float bpe = (float)(n_grid * q0);
     const double ratio = (double)bpe / (double)fmaxf(bpe, F32_EPS);
     const float dv0 = (float)(a.blend * (double)power * ratio + (1.0 - a.blend) * (double)power);
     scs[SC_YE] = q0;
@@ -414,7 +411,6 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
     scs[SC_NOISE] = fmaxf(a.noise_fraction * power, floor_v);
     scs[SC_DV] = dv0;
     scs[SC_DOF] = 0.0;
-    (void)n_grid;
   } else {
     const double max_dof = (double)n_samp * (1.0 - 10.0 * (double)F32_EPS);
     const double dof = fmin(q0, max_dof);

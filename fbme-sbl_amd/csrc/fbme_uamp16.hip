@@ -153,8 +153,11 @@ __device__ void wave_tile_topk(const uint32_t (&kv)[16], int g, int t, int k, in
 }
 
 // ------------------------------------------------------------------- row pass
-// Modes as k_row in fbme_uamp.hip: 0 backprojection rows, 1 first UAMP step
-// (no inverse FFT), 2 UAMP step, 3 final E-step with top-k candidates.
+// Modes as k_row in fbme_uamp.hip: 0 backprojection rows, 1 first UAMP step,
+// 2 UAMP step, 3 final E-step with top-k candidates.  Mode 1 transforms the
+// column output itself and stores the backprojection A^H y (read by the
+// polish) on the way, so the plans launch no separate mode-0 pass for
+// register-FFT rows.
 template <int MODE>
 __global__ void __launch_bounds__(64)
 k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
@@ -188,26 +191,19 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
       tw_loaded = s.tw2_off;
     }
     float2 v[16];
-    if (MODE != 1) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
-        const int c = t + 16 * j;
-        v[j] = (act && c < s.m2) ? mid[mb + c] : make_float2(0.f, 0.f);
-      }
+    for (int j = 0; j < 16; ++j) {
+      const int c = t + 16 * j;
+      v[j] = (act && c < s.m2) ? mid[mb + c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    if (MODE != 1) fft256<true>(v, tw, x, t);
-    if (MODE == 0) {
-      double acc = 0.0;
+    fft256<true>(v, tw, x, t);
+    if (MODE <= 1) {
 #pragma unroll
-      for (int n = 0; n < 16; ++n) {
-        if (act) {
-          bp[gb + t + 16 * n] = v[n];
-          acc += (double)v[n].x * v[n].x + (double)v[n].y * v[n].y;
-        }
-      }
-      acc = block_sum(acc, red);
-      if (threadIdx.x == 0) part[(long long)PT_BPE * pstride + s.row_part + wk.slot] = acc;
+      for (int n = 0; n < 16; ++n)
+        if (act) bp[gb + t + 16 * n] = v[n];
+    }
+    if (MODE == 0) {
       __syncthreads();
       continue;
     }
@@ -229,9 +225,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         const long long k = gb + t + 16 * (h + q);
-        if (MODE == 1) {
-          mv[q] = act ? bp[k] : make_float2(0.f, 0.f);
-        } else {
+        if (MODE != 1) {
           gv[q] = act ? gamma[k] : 1.0f;
           mv[q] = act ? mean[k] : make_float2(0.f, 0.f);
           vv[q] = act ? var[k] : 1.0f;
@@ -244,7 +238,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
         float2 lifted, mean_old;
         float prior, v_old;
         if (MODE == 1) {
-          const float2 b = mv[q];
+          const float2 b = v[n];
           lifted = cscale(b, first_scale);
           const float energy = b.x * b.x + b.y * b.y;
           const float structured = energy * struct_scale;
