@@ -7,7 +7,8 @@ in HBM.  value = B * steps / elapsed seconds (scenes/s).
 
 Also reported: the plain single-scene plan's scenes/s on the same GPU, the
 NMSE of both on the same scenes, the roofline of the dominant kernel (the
-fused row pass) timed with events on the stream it runs on, the float64 CPU
+fused row pass) timed with events on the stream it runs on, beside the
+bandwidth plain streaming kernels reach on the same GPU, the float64 CPU
 oracle timed on the host, a 64-scene uneven workload in one graph-captured
 plan (autotuned pass shapes against the defaults), and the learned warm start
 against the plain general solver (PCG iterations, scenes/s, NMSE).
@@ -603,6 +604,28 @@ def main():
     per_kernel = {k: {"ms": v, "GB/s": kb[k] * b / (v / 1e3) / 1e9}
                   for k, v in stage_ms.items()}
 
+    # What this GPU delivers to plain streaming kernels (torch's own copy,
+    # fill and sum over 1 GiB), beside the datasheet peak the fraction uses.
+    def stream_gbs(fn, nbytes, reps=10):
+        fn()
+        torch.cuda.synchronize()
+        start.record()
+        for _ in range(reps):
+            fn()
+        stop.record()
+        torch.cuda.synchronize()
+        return nbytes / (start.elapsed_time(stop) / reps / 1e3) / 1e9
+
+    src = torch.empty(1 << 28, dtype=torch.float32, device=device).uniform_()
+    dst = torch.empty_like(src)
+    gib = src.numel() * src.element_size()
+    hbm_reference = {"copy_GB_per_s": stream_gbs(lambda: dst.copy_(src), 2 * gib),
+                     "write_GB_per_s": stream_gbs(lambda: dst.fill_(1.0), gib),
+                     "read_GB_per_s": stream_gbs(lambda: src.sum(), gib),
+                     "buffer_bytes": gib}
+    roofline["frac_of_copy"] = achieved / 1e9 / hbm_reference["copy_GB_per_s"]
+    del src, dst
+
     if args.calibrate_copy:
         src = torch.empty(64 << 20, dtype=torch.float32, device=device)
         dst = torch.empty_like(src)
@@ -671,6 +694,7 @@ def main():
                    "batch": b, "grid": list(grid), "prefix": list(samples),
                    "polish_size": 32, "uamp_iterations": t_u},
         "roofline": roofline,
+        "hbm_reference": hbm_reference,
         "cpu_baseline": cpu,
         "plain_single_scene": plain,
         "speedup_vs_plain": (value / plain["scenes_per_s"]) if plain else None,
