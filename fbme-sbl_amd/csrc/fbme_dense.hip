@@ -77,17 +77,26 @@ __global__ void k_lag_cols(const double2* __restrict__ t, int n1, int m1, int m2
 // (obs[r] = sample index m1 * M2 + m2 of row r; NULL: all samples), plus
 // the loading jitter * mean|diag| on the diagonal (every diagonal entry is
 // c[0] + noise); rows and columns m..mp-1 are an identity block.
+// Sample index of observed sample k, or -1 when the caller's index lies
+// outside [0, M1 M2) (such a sample is left out and status flags it).
+__device__ __forceinline__ int obs_at(const int* __restrict__ obs, int k, int mfull) {
+  if (!obs) return k;
+  const int v = obs[k];
+  return (v >= 0 && v < mfull) ? v : -1;
+}
+
 __global__ void k_dense_build(const double2* __restrict__ lags, const int* __restrict__ obs, int m, int mp, int m1,
                               int m2, const float* __restrict__ noise, double jitter, double tiny,
-                              double2* __restrict__ q) {
+                              double2* __restrict__ q, int* __restrict__ status) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (e >= (long long)mp * mp) return;
   const int r = (int)(e / mp), c = (int)(e - (long long)r * mp);
   double2 v = make_double2(r == c ? 1.0 : 0.0, 0.0);
-  if (r < m && c < m) {
+  const int sr = r < m ? obs_at(obs, r, m1 * m2) : 0, sc = c < m ? obs_at(obs, c, m1 * m2) : 0;
+  if (r < m && r == c && sr < 0 && status) status[b] = 2;
+  if (r < m && c < m && sr >= 0 && sc >= 0) {
     const long long base = (long long)b * (2 * m1 - 1) * (2 * m2 - 1);
-    const int sr = obs ? obs[r] : r, sc = obs ? obs[c] : c;
     const int d1 = sr / m2 - sc / m2, d2 = sr % m2 - sc % m2;
     v = lags[base + (long long)(d1 + m1 - 1) * (2 * m2 - 1) + (d2 + m2 - 1)];
     if (r == c) {
@@ -322,7 +331,8 @@ k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, cons
     double ax = 0.0, ay = 0.0;
     for (int k = 0; k <= i; ++k) {
       const double2 a = w[(long long)i * mp + k];
-      const float2 v = yb[obs ? obs[k] : k];
+      const int sk = obs_at(obs, k, mfull);
+      const float2 v = sk >= 0 ? yb[sk] : make_float2(0.f, 0.f);
       ax += a.x * v.x - a.y * v.y;
       ay += a.x * v.y + a.y * v.x;
     }
@@ -336,13 +346,15 @@ k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, cons
       ax += a.x * u[i].x + a.y * u[i].y;
       ay += a.x * u[i].y - a.y * u[i].x;
     }
-    rb[(long long)m * mfull + (obs ? obs[k] : k)] = make_double2(ax, ay);
+    const int sk = obs_at(obs, k, mfull);
+    if (sk >= 0) rb[(long long)m * mfull + sk] = make_double2(ax, ay);
   }
   for (long long e = t; e < (long long)m * m; e += DN_THREADS) {
     const int r = (int)(e / m), c = (int)(e - (long long)r * m);
     if (c > r) continue;   // W is lower triangular
     const double2 a = w[(long long)r * mp + c];
-    rb[(long long)r * mfull + (obs ? obs[c] : c)] = make_double2(a.x, -a.y);
+    const int sc = obs_at(obs, c, mfull);
+    if (sc >= 0) rb[(long long)r * mfull + sc] = make_double2(a.x, -a.y);
   }
 }
 
@@ -465,19 +477,26 @@ DenseLayout dense_layout(int batch, int n1, int n2, int m1, int m2, int m) {
 
 extern "C" {
 
+static bool dense_args_ok(int batch, int n1, int n2, int m1, int m2, int observed) {
+  // scenes run on grid.y (at most 65535); grid indices stay in int range
+  return batch >= 1 && batch <= 65535 && m1 >= 1 && m2 >= 1 && m1 <= n1 && m2 <= n2 &&
+         (long long)n1 * n2 <= (1ll << 28) && observed >= 1 && observed <= m1 * m2 && observed <= DN_MMAX;
+}
+
 size_t fbme_dense_workspace_bytes(int batch, int n1, int n2, int m1, int m2, int observed) {
-  if (batch < 1 || observed < 1 || observed > DN_MMAX) return 0;
+  if (!dense_args_ok(batch, n1, n2, m1, m2, observed)) return 0;
   return dense_layout(batch, n1, n2, m1, m2, observed).total;
 }
 
 int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2, int observed, const int* obs_idx,
                          const void* y, const float* gamma, const float* noise, double jitter, double tiny,
-                         void* work, void* mean, float* var, float* dd, double* part, double* rel_sum,
-                         int* status) {
-  if (batch < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2 || observed < 1 || observed > m1 * m2 ||
-      observed > DN_MMAX || !work)
+                         void* work, size_t work_bytes, void* mean, float* var, float* dd, double* part,
+                         double* rel_sum, int* status) {
+  if (!dense_args_ok(batch, n1, n2, m1, m2, observed) || !work || !y || !gamma || !noise || !mean || !var || !dd ||
+      !part || !rel_sum)
     return FBME_ERR_ARG;
   if (!obs_idx && observed != m1 * m2) return FBME_ERR_ARG;
+  if (work_bytes < dense_layout(batch, n1, n2, m1, m2, observed).total) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int m = observed, n = n1 * n2, mf = m1 * m2;
   const DenseLayout l = dense_layout(batch, n1, n2, m1, m2, m);
@@ -496,7 +515,7 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
   hipLaunchKernelGGL(k_lag_cols, dim3((unsigned)((lag_e + 255) / 256), batch), dim3(256), 0, st, t, n1, m1, m2,
                      lags);
   hipLaunchKernelGGL(k_dense_build, dim3((unsigned)(((long long)mp * mp + 255) / 256), batch), dim3(256), 0, st,
-                     lags, obs_idx, m, mp, m1, m2, noise, jitter, tiny, q);
+                     lags, obs_idx, m, mp, m1, m2, noise, jitter, tiny, q, status);
   for (int jb = 0; jb < nb; ++jb) {
     hipLaunchKernelGGL(k_tile_diag, dim3(batch), dim3(DN_THREADS), 0, st, q, mp, jb, tiny, linv, status);
     const int rest = nb - jb - 1;

@@ -746,6 +746,10 @@ int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2,
                         double ridge, double eps, double tiny, double* initial_positions, double* positions,
                         void* amplitudes, double* position_std, double* norms, int* counts) {
   if (batch < 1 || n1 < 1 || n2 < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2) return FBME_ERR_ARG;
+  if ((long long)m1 * m2 < 2 || (long long)n1 * n2 > (1ll << 28)) return FBME_ERR_ARG;
+  if (!data || !coefficients || !initial_positions || !positions || !amplitudes || !position_std || !norms ||
+      !counts)
+    return FBME_ERR_ARG;
   if (max_components < 1 || max_components > OG_BICMAX || component_count > OG_KMAX) return FBME_ERR_ARG;
   if (max_iter < 1 || !(tolerance > 0.0) || !(max_step > 0.0) || !(damping > 0.0) || ridge < 0.0) return FBME_ERR_ARG;
   OgArgs a;

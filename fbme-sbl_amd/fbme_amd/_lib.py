@@ -14,7 +14,10 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
+# return codes (include/fbme_hip.h)
+ERR_ARG = 1
+ERR_HIP = 2
 
 Q_WORKSPACE_BYTES = 0
 Q_TOTAL_GRID = 1
@@ -214,8 +217,8 @@ _SIGNATURES = {
     "fbme_dense_posterior": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                      c_int, c_int, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_double, c_double,
-                                     c_void_p, c_void_p, c_void_p, c_void_p,
-                                     c_void_p, c_void_p, c_void_p]),
+                                     c_void_p, c_size_t, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
     # off-grid refinement
     "fbme_offgrid_refine": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                     c_int, c_void_p, c_void_p, c_int, c_int,

@@ -266,6 +266,9 @@ class _Solver:
             self._adjoint(self.rhs[:, nd:].reshape(ns * self.p, m), self.lprobes.view(ns * self.p, n))
         if self.dense:
             nbytes = int(self.lib.fbme_dense_workspace_bytes(b, n1, n2, m1, m2, self.m_obs))
+            if nbytes == 0:
+                raise ValueError(f"dense mode does not take {b} scenes of {n1}x{n2} / {m1}x{m2} "
+                                 f"with {self.m_obs} observed samples")
             self.dense_work = torch.empty(nbytes, dtype=torch.uint8, device=dev)
             self.dense_obs = (self.obs_idx.to(torch.int32).contiguous() if self.obs_idx is not None else None)
         # state
@@ -634,7 +637,7 @@ class _Solver:
                 # the factorisation runs in FP64, so the loading uses the FP64 eps
                 # (the reference loads with the eps of its matrix dtype)
                 _ptr(self.gamma), _ptr(self.noise), float(self.cfg.cuda_cholesky_jitter_multiplier) * F64_EPS,
-                F32_TINY, _ptr(self.dense_work), _ptr(self.mean), _ptr(self.var), _ptr(self.dd),
+                F32_TINY, _ptr(self.dense_work), self.dense_work.numel(), _ptr(self.mean), _ptr(self.var), _ptr(self.dd),
                 _ptr(self.grid_part), _ptr(self.rel_sum), _ptr(self.status)), "dense_posterior")
             return 0
         self.setup_operators()
@@ -773,6 +776,8 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
     s.posterior(final_tol)
     s.em(update=False)
     if dense and bool(s.status.any()):
+        if bool((s.status == 2).any()):
+            raise RuntimeError("an observed-sample index lies outside the sample grid.")
         raise RuntimeError("CUDA Cholesky failed after deterministic diagonal loading.")
     precision = torch.empty_like(s.gamma)
     _lib.check(s.lib.fbme_reciprocal(s.stream, s.gamma.numel(), _ptr(s.gamma), _ptr(precision)), "reciprocal")

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 4
+#define FBME_ABI_VERSION 5
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -402,18 +402,22 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
  * A^H built from the Fourier lags of gamma (operators.py:282-297), the GPU
  * Cholesky loading jitter * mean|diag| (solver.py:2898-2920; jitter =
  * cuda_cholesky_jitter_multiplier * eps) and the data diagonal
- * diag(A^H Q^-1 A) (operators.py:418-442), in FP64, one workgroup per scene
- * for the factorisation.  observed <= 2048 samples; obs_idx [observed]
- * int32 lists their sample indices (NULL: all M1 M2).  y is [B, M1 M2]
- * complex64 in the full sample layout.  Writes mean [B, N] complex64, var
- * and dd [B, N] float32, rel_sum[b] = sum clamp(gamma dd, 0, 1); part:
- * B * ceil(N / 256) doubles; status[b] = 1 on a failed pivot.  work:
- * fbme_dense_workspace_bytes(batch, n1, n2, m1, m2, observed) bytes. */
+ * diag(A^H Q^-1 A) (operators.py:418-442), in FP64: tiled 32 x 32 Cholesky
+ * and triangular inverse over many workgroups per scene.  1 <= batch <=
+ * 65535; observed <= 2048 samples; obs_idx [observed] int32 lists their
+ * distinct sample indices in [0, M1 M2) (NULL: all M1 M2).  y is
+ * [B, M1 M2] complex64 in the full sample layout.  Writes mean [B, N]
+ * complex64, var and dd [B, N] float32, rel_sum[b] = sum clamp(gamma dd,
+ * 0, 1); part: B * ceil(N / 256) doubles.  status[b] (zeroed by the caller,
+ * may be NULL): 1 on a failed pivot, 2 when an obs_idx entry lies outside
+ * [0, M1 M2) (that sample is left out).  work: work_bytes >=
+ * fbme_dense_workspace_bytes(batch, n1, n2, m1, m2, observed), which
+ * returns 0 for arguments the posterior rejects. */
 size_t fbme_dense_workspace_bytes(int batch, int n1, int n2, int m1, int m2, int observed);
 int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2, int observed, const int* obs_idx,
                          const void* y, const float* gamma, const float* noise, double jitter, double tiny,
-                         void* work, void* mean, float* var, float* dd, double* part, double* rel_sum,
-                         int* status);
+                         void* work, size_t work_bytes, void* mean, float* var, float* dd, double* part,
+                         double* rel_sum, int* status);
 
 /* ------------------------------------------------------------------------
  * Off-grid refinement: refine_offgrid_fourier (torch_sbl/offgrid.py:70-226)
@@ -424,8 +428,9 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
  * with Levenberg damping, trust bound and backtracking (offgrid.py:364-476)
  * and the position standard deviations (offgrid.py:538-564), all in FP64.
  * data [B, M1*M2] and coefficients [B, N1*N2] are complex64 (complex128 when
- * `complex128` is nonzero).  Limits: max_components <= 64, at most 32
- * refined atoms (counts[b*3+1] = -1 when the model order exceeds 32).
+ * `complex128` is nonzero).  Limits: M1 M2 >= 2, max_components <= 64, at
+ * most 32 refined atoms (counts[b*3+1] = -1 when the model order exceeds
+ * 32); every pointer is required (FBME_ERR_ARG otherwise).
  * Outputs, with Kout = component_count > 0 ? component_count :
  * max_components rows per scene (rows past the scene's count are NaN):
  * initial_positions / positions / position_std [B, Kout, 2] double,

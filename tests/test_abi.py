@@ -31,6 +31,22 @@ def test_abi_version():
     assert _lib.load().fbme_abi_version() == _lib.ABI_VERSION
 
 
+def test_dense_workspace_rejects_bad_shapes():
+    """fbme_dense_workspace_bytes is host-only: 0 for every shape the dense
+    posterior rejects, a positive size otherwise."""
+    lib = _lib.load()
+    assert lib.fbme_dense_workspace_bytes(2, 32, 32, 24, 24, 576) > 0
+    assert lib.fbme_dense_workspace_bytes(1, 40, 30, 20, 15, 200) > 0
+    for args in [(0, 32, 32, 24, 24, 576),        # no scenes
+                 (70000, 32, 32, 24, 24, 576),    # beyond grid.y
+                 (1, 32, 32, 40, 24, 576),        # M1 > N1
+                 (1, 32, 32, 24, 24, 577),        # more observed than samples
+                 (1, 64, 64, 48, 48, 2304),       # observed > 2048
+                 (1, 32, 32, 24, 24, 0),
+                 (1, -32, 32, 24, 24, 576)]:
+        assert lib.fbme_dense_workspace_bytes(*args) == 0, args
+
+
 def _plan(shapes, stable=False, **kw):
     lib = _lib.load()
     opt = _lib.default_options(stable)

@@ -4,8 +4,8 @@ runs.
 
 The GPU path forms Q from the Fourier lags of gamma and factors it in FP64
 with the GPU's deterministic loading (solver.py:2898-2920) at the FP64 eps,
-8 eps_f64 mean|diag|; the data enters in complex64 and the prefix
-transforms run in float32.  Against the
+8 eps_f64 mean|diag|; the data enters in complex64, the prefix transforms of
+the inverse factor's rows run in FP64.  Against the
 float64 oracle the bar is 1e-4 relative L2 on the posterior mean, variance
 and data diagonal of one step; against the reference's complex128
 dense_em_sbl runs (30 and 200 EM iterations) 1e-5 on x, 1e-3 on the noise
@@ -95,3 +95,45 @@ def test_auto_mode_small_problem_runs_dense(device):
     res = sbl_2d(torch.from_numpy(sc.y).to(device), (32, 32), config=SBLConfig(device=str(device), max_iter=20))
     assert res.mode == "dense"
     assert scenes.nmse_db(res.x.cpu().numpy().reshape(-1), sc.x.reshape(-1)) < -15.0
+
+
+def test_dense_posterior_rejects_bad_arguments(device):
+    """C-ABI error paths: a workspace smaller than the layout is refused
+    before any launch, and an observed-sample index outside [0, M1 M2) is
+    left out by the kernels and reported as status 2."""
+    import ctypes
+    from fbme_amd import _lib
+    lib = _lib.load()
+    b, n1, n2, m1, m2 = 1, 16, 16, 8, 8
+    n, mf, m = n1 * n2, m1 * m2, 30
+    nbytes = int(lib.fbme_dense_workspace_bytes(b, n1, n2, m1, m2, m))
+    assert nbytes > 0
+    work = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    y = torch.randn(b, mf, dtype=torch.complex64, device=device)
+    gamma = torch.full((b, n), 0.1, dtype=torch.float32, device=device)
+    noise = torch.full((b,), 0.01, dtype=torch.float32, device=device)
+    mean = torch.empty(b, n, dtype=torch.complex64, device=device)
+    var = torch.empty(b, n, dtype=torch.float32, device=device)
+    dd = torch.empty_like(var)
+    part = torch.empty(b * ((n + 255) // 256), dtype=torch.float64, device=device)
+    rel = torch.empty(b, dtype=torch.float64, device=device)
+    status = torch.zeros(b, dtype=torch.int32, device=device)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+    def call(obs, size):
+        return lib.fbme_dense_posterior(stream, b, n1, n2, m1, m2, m, p(obs), p(y), p(gamma), p(noise),
+                                        8.0 * 2.0 ** -52, 1e-38, p(work), size, p(mean), p(var), p(dd), p(part),
+                                        p(rel), p(status))
+
+    good = torch.arange(0, 2 * m, 2, dtype=torch.int32, device=device)   # 0, 2, .., 58 < 64
+    assert call(good, nbytes - 1) == _lib.ERR_ARG
+    assert call(good, nbytes) == 0
+    torch.cuda.synchronize()
+    assert int(status.sum()) == 0
+    assert bool(torch.isfinite(var).all())
+    bad = good.clone()
+    bad[5] = mf + 3
+    assert call(bad, nbytes) == 0
+    torch.cuda.synchronize()
+    assert int(status[0]) == 2
