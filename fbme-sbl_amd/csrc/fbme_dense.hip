@@ -11,12 +11,16 @@
 //     (factor and inverse), a panel kernel and a trailing-update kernel over
 //     all tile pairs, then one launch per block row of W; the matrix is padded
 //     to a multiple of 32 with an identity block;
-//   * Q^-1 y = W^H (W y);
+//   * Q^-1 y = W^H (W y): one wavefront per row for W y, then column sums
+//     of W over row chunks (coalesced, fixed-order partials);
 //   * the data diagonal diag(A^H Q^-1 A)_n = sum_k |(A^H conj(w_k))_n|^2 over
 //     the rows w_k of W (a_n^H Q^-1 a_n = ||W a_n||^2; the reference
 //     accumulates Q^-1 along its Fourier lags, operators.py:418-442, equally
 //     exact), and A^H Q^-1 y, from separable FP64 prefix transforms of those
-//     rows (twiddles from per-block tables, phases reduced mod N).
+//     rows: the first axis reads W in place through the sample -> observed
+//     index map, the second runs over chunks of rows in parallel with
+//     fixed-order partial sums (twiddles from per-block LDS tables, phases
+//     reduced mod N; grid axes up to 1024).
 #include "fbme_device.h"
 
 using namespace fbme;
@@ -316,52 +320,104 @@ k_tile_inverse_row(const double2* __restrict__ qs, int mp, int ib, const double2
     w[(long long)(ib * TS + r) * mp + jb * TS + c0 + 8 * u] = make_double2(-acc[u].x, -acc[u].y);
 }
 
-// Rows of the transforms, FP64 in the full sample layout (zeros on the
+// Rows of the transforms, FP64, in the full sample layout (zeros on the
 // unobserved samples): rows 0..m-1 = conj(W[k, :]), row m = z = W^H (W y).
+constexpr int DN_VCH = 128;   // rows of W per partial sum of W^H u
+constexpr int DN_RCH = 64;    // rows of conj(W) per partial data diagonal
+constexpr int DN_AXMAX = 1024;   // longest grid axis (LDS twiddle tables)
+
+// pos[j] = observed index of sample j, -1 where sample j is not observed:
+// identity without a list, else filled with -1 and scattered from the list.
+__global__ void k_dense_pos_fill(int listed, int mfull, int* __restrict__ pos) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < mfull) pos[j] = listed ? -1 : j;
+}
+
+__global__ void k_dense_pos_scatter(const int* __restrict__ obs, int m, int mfull, int* __restrict__ pos) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  const int s = obs_at(obs, k, mfull);
+  if (s >= 0) pos[s] = k;
+}
+
+// u[b, i] = sum_{k <= i} W[i][k] y[obs k]: one wavefront per row of W.
 __global__ void __launch_bounds__(DN_THREADS)
-k_dense_apply(const double2* __restrict__ ws, const float2* __restrict__ y, const int* __restrict__ obs, int m,
-              int mp, int mfull, double2* __restrict__ rows) {
-  __shared__ double2 u[DN_MMAX];
-  const int t = threadIdx.x, b = blockIdx.x;
-  const double2* w = ws + (long long)b * mp * mp;
+k_dense_u(const double2* __restrict__ ws, const float2* __restrict__ y, const int* __restrict__ obs, int m, int mp,
+          int mfull, double2* __restrict__ u) {
+  const int lane = threadIdx.x & 63, b = blockIdx.y;
+  const int i = blockIdx.x * (DN_THREADS / 64) + (threadIdx.x >> 6);
+  if (i >= m) return;   // uniform per wavefront
+  const double2* w = ws + (long long)b * mp * mp + (long long)i * mp;
   const float2* yb = y + (long long)b * mfull;
-  double2* rb = rows + (long long)b * (m + 1) * mfull;
-  for (long long e = t; e < (long long)(m + 1) * mfull; e += DN_THREADS) rb[e] = make_double2(0.0, 0.0);
-  for (int i = t; i < m; i += DN_THREADS) {
-    double ax = 0.0, ay = 0.0;
-    for (int k = 0; k <= i; ++k) {
-      const double2 a = w[(long long)i * mp + k];
-      const int sk = obs_at(obs, k, mfull);
-      const float2 v = sk >= 0 ? yb[sk] : make_float2(0.f, 0.f);
-      ax += a.x * v.x - a.y * v.y;
-      ay += a.x * v.y + a.y * v.x;
-    }
-    u[i] = make_double2(ax, ay);
-  }
-  __syncthreads();
-  for (int k = t; k < m; k += DN_THREADS) {
-    double ax = 0.0, ay = 0.0;
-    for (int i = k; i < m; ++i) {
-      const double2 a = w[(long long)i * mp + k];   // conj(W[i][k]) u_i
-      ax += a.x * u[i].x + a.y * u[i].y;
-      ay += a.x * u[i].y - a.y * u[i].x;
-    }
+  double ax = 0.0, ay = 0.0;
+  for (int k = lane; k <= i; k += 64) {
     const int sk = obs_at(obs, k, mfull);
-    if (sk >= 0) rb[(long long)m * mfull + sk] = make_double2(ax, ay);
+    if (sk < 0) continue;
+    const double2 a = w[k];
+    const float2 v = yb[sk];
+    ax += a.x * v.x - a.y * v.y;
+    ay += a.x * v.y + a.y * v.x;
   }
-  for (long long e = t; e < (long long)m * m; e += DN_THREADS) {
-    const int r = (int)(e / m), c = (int)(e - (long long)r * m);
-    if (c > r) continue;   // W is lower triangular
-    const double2 a = w[(long long)r * mp + c];
-    const int sc = obs_at(obs, c, mfull);
-    if (sc >= 0) rb[(long long)r * mfull + sc] = make_double2(a.x, -a.y);
+  ax = wave_sum(ax);
+  ay = wave_sum(ay);
+  if (lane == 0) u[(long long)b * m + i] = make_double2(ax, ay);
+}
+
+// vpart[b, c, k] = sum over rows i of chunk c with i >= k of conj(W[i][k]) u_i:
+// lane l owns column k (row-contiguous loads), the four wavefronts split the
+// chunk's rows, fixed-order sums.
+__global__ void __launch_bounds__(DN_THREADS)
+k_dense_wh(const double2* __restrict__ ws, const double2* __restrict__ u, int m, int mp, int nch,
+           double2* __restrict__ vpart) {
+  __shared__ double2 red[DN_THREADS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, c = blockIdx.y, b = blockIdx.z;
+  const int k = blockIdx.x * 64 + lane;
+  const double2* w = ws + (long long)b * mp * mp;
+  const double2* ub = u + (long long)b * m;
+  const int i1 = min(m, (c + 1) * DN_VCH);
+  double ax = 0.0, ay = 0.0;
+  if (k < m) {
+    for (int i = c * DN_VCH + wave; i < i1; i += DN_THREADS / 64) {
+      if (i < k) continue;   // W is lower triangular
+      const double2 a = w[(long long)i * mp + k], v = ub[i];
+      ax += a.x * v.x + a.y * v.y;
+      ay += a.x * v.y - a.y * v.x;
+    }
+  }
+  red[threadIdx.x] = make_double2(ax, ay);
+  __syncthreads();
+  if (wave == 0 && k < m) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int q = 0; q < DN_THREADS / 64; ++q) {
+      sx += red[q * 64 + lane].x;
+      sy += red[q * 64 + lane].y;
+    }
+    vpart[((long long)b * nch + c) * m + k] = make_double2(sx, sy);
   }
 }
 
-// t1[b, r, m1, n2] = sum_m2 rows[b, r, m1 M2 + m2] exp(+2 pi i m2 n2 / N2).
+// v[b, k] = sum_c vpart[b, c, k] (W^H W y over the observed samples).
+__global__ void k_dense_vsum(const double2* __restrict__ vpart, int m, int nch, double2* __restrict__ v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x, b = blockIdx.y;
+  if (k >= m) return;
+  double ax = 0.0, ay = 0.0;
+  for (int c = 0; c < nch; ++c) {
+    const double2 p = vpart[((long long)b * nch + c) * m + k];
+    ax += p.x;
+    ay += p.y;
+  }
+  v[(long long)b * m + k] = make_double2(ax, ay);
+}
+
+// t1[b, r, a, c] = sum_m2 R[r, a M2 + m2] exp(+2 pi i m2 c / N2), where R in
+// the full sample layout holds the rows of conj(W) (r < m: R[r, j] =
+// conj(W[r][pos j]) for 0 <= pos j <= r) and W^H W y (r = m: v[pos j]),
+// zero elsewhere.  W is read in place through pos.
 __global__ void __launch_bounds__(DN_THREADS)
-k_dense_t1(const double2* __restrict__ rows, int nrows, int m1, int m2, int n2, double2* __restrict__ t1) {
-  __shared__ double2 tw[1024];
+k_dense_t1(const double2* __restrict__ ws, const double2* __restrict__ v, const int* __restrict__ pos, int m,
+           int mp, int m1, int m2, int n2, double2* __restrict__ t1) {
+  __shared__ double2 tw[DN_AXMAX];
   for (int i = threadIdx.x; i < n2; i += DN_THREADS) {
     double sn, cs;
     sincospi(2.0 * (double)i / n2, &sn, &cs);
@@ -370,30 +426,84 @@ k_dense_t1(const double2* __restrict__ rows, int nrows, int m1, int m2, int n2, 
   __syncthreads();
   const int b = blockIdx.y;
   const long long e = (long long)blockIdx.x * DN_THREADS + threadIdx.x;
-  if (e >= (long long)nrows * m1 * n2) return;
+  if (e >= (long long)(m + 1) * m1 * n2) return;
   const int c = (int)(e % n2);
-  const long long rr = e / n2;                       // r * m1 + row
-  const double2* src = rows + ((long long)b * nrows * m1 + rr) * m2;
+  const long long rr = e / n2;   // r * m1 + a
+  const int r = (int)(rr / m1), a = (int)(rr - (long long)r * m1);
+  const double2* wr = ws + (long long)b * mp * mp + (long long)r * mp;
+  const double2* vb = v + (long long)b * m;
+  const int* pa = pos + a * m2;
   double ax = 0.0, ay = 0.0;
   int ph = 0;
   for (int k = 0; k < m2; ++k) {
-    const double2 v = src[k], w = tw[ph];
-    ax += v.x * w.x - v.y * w.y;
-    ay += v.x * w.y + v.y * w.x;
+    const int pk = pa[k];
+    if (pk >= 0 && (r == m || pk <= r)) {
+      double2 val;
+      if (r == m) {
+        val = vb[pk];
+      } else {
+        const double2 wv = wr[pk];
+        val = make_double2(wv.x, -wv.y);
+      }
+      const double2 w = tw[ph];
+      ax += val.x * w.x - val.y * w.y;
+      ay += val.x * w.y + val.y * w.x;
+    }
     ph += c;
     if (ph >= n2) ph -= n2;
   }
-  t1[(long long)b * nrows * m1 * n2 + e] = make_double2(ax, ay);
+  t1[(long long)b * (m + 1) * m1 * n2 + e] = make_double2(ax, ay);
 }
 
-// Per grid point n = (n1, n2): u_r = sum_m1 t1[b, r, m1, n2] exp(+2 pi i m1 n1 / N1);
-// dd = sum_{r<m} |u_r|^2, lifted = u_m; then mean = gamma lifted, variance,
-// data diagonal and relevance partials (solver.py:2985-2990, 3016-3036).
+// Row transform of t1 row r at grid point (n1, n2):
+// sum_m1 t1[r, m1, n2] exp(+2 pi i m1 n1 / N1).
+__device__ __forceinline__ double2 dense_row_at(const double2* __restrict__ tb, int r, int m1, int n1, int n2, int r1,
+                                                int c2, const double2* tw) {
+  double ux = 0.0, uy = 0.0;
+  int ph = 0;
+  for (int k = 0; k < m1; ++k) {
+    const double2 v = tb[((long long)r * m1 + k) * n2 + c2], w = tw[ph];
+    ux += v.x * w.x - v.y * w.y;
+    uy += v.x * w.y + v.y * w.x;
+    ph += r1;
+    if (ph >= n1) ph -= n1;
+  }
+  return make_double2(ux, uy);
+}
+
+// ddpart[b, ch, n] = sum over rows r < m of chunk ch of |row transform|^2.
 __global__ void __launch_bounds__(DN_THREADS)
-k_dense_moments(const double2* __restrict__ t1, int m, int m1, int n1, int n2, const float* __restrict__ gamma,
-                float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd_out,
-                double* __restrict__ part) {
-  __shared__ double2 tw[1024];
+k_dense_ddpart(const double2* __restrict__ t1, int m, int m1, int n1, int n2, int nrch, double* __restrict__ ddpart) {
+  __shared__ double2 tw[DN_AXMAX];
+  for (int i = threadIdx.x; i < n1; i += DN_THREADS) {
+    double sn, cs;
+    sincospi(2.0 * (double)i / n1, &sn, &cs);
+    tw[i] = make_double2(cs, sn);
+  }
+  __syncthreads();
+  const int ch = blockIdx.y, b = blockIdx.z;
+  const int n = n1 * n2;
+  const int i = blockIdx.x * DN_THREADS + threadIdx.x;
+  if (i >= n) return;
+  const int r1 = i / n2, c2 = i - r1 * n2;
+  const double2* tb = t1 + (long long)b * (m + 1) * m1 * n2;
+  const int r_end = min(m, (ch + 1) * DN_RCH);
+  double acc = 0.0;
+  for (int r = ch * DN_RCH; r < r_end; ++r) {
+    const double2 u = dense_row_at(tb, r, m1, n1, n2, r1, c2, tw);
+    acc += u.x * u.x + u.y * u.y;
+  }
+  ddpart[((long long)b * nrch + ch) * n + i] = acc;
+}
+
+// Per grid point: dd = sum of the chunk partials, lifted = row transform of
+// t1 row m; then mean = gamma lifted, variance, data diagonal and relevance
+// partials (solver.py:2985-2990, 3016-3036).
+__global__ void __launch_bounds__(DN_THREADS)
+k_dense_moments(const double2* __restrict__ t1, const double* __restrict__ ddpart, int m, int m1, int n1, int n2,
+                int nrch, const float* __restrict__ gamma, float2* __restrict__ mean, float* __restrict__ var,
+                float* __restrict__ dd_out, double* __restrict__ part) {
+  __shared__ double2 tw[DN_AXMAX];
   __shared__ double red[DN_THREADS];
   for (int i = threadIdx.x; i < n1; i += DN_THREADS) {
     double sn, cs;
@@ -408,27 +518,12 @@ k_dense_moments(const double2* __restrict__ t1, int m, int m1, int n1, int n2, c
   if (i < n) {
     const int r1 = i / n2, c2 = i - r1 * n2;
     const double2* tb = t1 + (long long)b * (m + 1) * m1 * n2;
-    double acc = 0.0, lx = 0.0, ly = 0.0;
-    for (int r = 0; r <= m; ++r) {
-      double ux = 0.0, uy = 0.0;
-      int ph = 0;
-      for (int k = 0; k < m1; ++k) {
-        const double2 v = tb[((long long)r * m1 + k) * n2 + c2], w = tw[ph];
-        ux += v.x * w.x - v.y * w.y;
-        uy += v.x * w.y + v.y * w.x;
-        ph += r1;
-        if (ph >= n1) ph -= n1;
-      }
-      if (r < m) {
-        acc += ux * ux + uy * uy;
-      } else {
-        lx = ux;
-        ly = uy;
-      }
-    }
+    double acc = 0.0;
+    for (int ch = 0; ch < nrch; ++ch) acc += ddpart[((long long)b * nrch + ch) * n + i];
+    const double2 l = dense_row_at(tb, m, m1, n1, n2, r1, c2, tw);
     const long long g = (long long)b * n + i;
     const double gm = gamma[g];
-    mean[g] = make_float2((float)(gm * lx), (float)(gm * ly));
+    mean[g] = make_float2((float)(gm * l.x), (float)(gm * l.y));
     var[g] = (float)fmax(gm - gm * gm * acc, 0.0);
     dd_out[g] = (float)acc;
     rel = fmin(fmax(gm * acc, 0.0), 1.0);
@@ -446,7 +541,7 @@ __global__ void k_dense_seg(const double* __restrict__ part, int blocks, double*
 }
 
 struct DenseLayout {
-  size_t t, lags, q, w, linv, rows, t1, total;
+  size_t t, lags, q, w, linv, pos, u, vpart, v, t1, ddpart, total;
 };
 
 inline size_t align256(size_t v) { return (v + 255) / 256 * 256; }
@@ -466,10 +561,14 @@ DenseLayout dense_layout(int batch, int n1, int n2, int m1, int m2, int m) {
   l.q = take(16 * b * mp * mp);
   l.w = take(16 * b * mp * mp);
   l.linv = take(16 * b * mp * 32);
-  l.rows = take(16 * b * (m + 1) * mf);
+  const size_t nch = (size_t)((m + DN_VCH - 1) / DN_VCH), nrch = (size_t)((m + DN_RCH - 1) / DN_RCH);
+  l.pos = take(4 * mf);
+  l.u = take(16 * b * m);
+  l.vpart = take(16 * b * nch * m);
+  l.v = take(16 * b * m);
   l.t1 = take(16 * b * (m + 1) * (size_t)m1 * n2);
+  l.ddpart = take(8 * b * nrch * n);
   l.total = off;
-  (void)n;
   return l;
 }
 
@@ -479,8 +578,8 @@ extern "C" {
 
 static bool dense_args_ok(int batch, int n1, int n2, int m1, int m2, int observed) {
   // scenes run on grid.y (at most 65535); grid indices stay in int range
-  return batch >= 1 && batch <= 65535 && m1 >= 1 && m2 >= 1 && m1 <= n1 && m2 <= n2 &&
-         (long long)n1 * n2 <= (1ll << 28) && observed >= 1 && observed <= m1 * m2 && observed <= DN_MMAX;
+  return batch >= 1 && batch <= 65535 && m1 >= 1 && m2 >= 1 && m1 <= n1 && m2 <= n2 && n1 <= DN_AXMAX &&
+         n2 <= DN_AXMAX && observed >= 1 && observed <= m1 * m2 && observed <= DN_MMAX;
 }
 
 size_t fbme_dense_workspace_bytes(int batch, int n1, int n2, int m1, int m2, int observed) {
@@ -507,8 +606,13 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
   double2* wmat = (double2*)(w + l.w);
   double2* linv = (double2*)(w + l.linv);
   const int mp = (m + TS - 1) / TS * TS, nb = mp / TS;
-  double2* rows = (double2*)(w + l.rows);
+  int* pos = (int*)(w + l.pos);
+  double2* u = (double2*)(w + l.u);
+  double2* vpart = (double2*)(w + l.vpart);
+  double2* v = (double2*)(w + l.v);
   double2* t1 = (double2*)(w + l.t1);
+  double* ddpart = (double*)(w + l.ddpart);
+  const int nch = (m + DN_VCH - 1) / DN_VCH, nrch = (m + DN_RCH - 1) / DN_RCH;
   const long long rows_e = (long long)n1 * (2 * m2 - 1), lag_e = (long long)(2 * m1 - 1) * (2 * m2 - 1);
   hipLaunchKernelGGL(k_lag_rows, dim3((unsigned)((rows_e + 255) / 256), batch), dim3(256), 0, st, gamma, n1, n2, m2,
                      t);
@@ -527,14 +631,22 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
   for (int ib = 0; ib < nb; ++ib)
     hipLaunchKernelGGL(k_tile_inverse_row, dim3(ib + 1, batch), dim3(DN_THREADS), 0, st, (const double2*)q, mp, ib,
                        (const double2*)linv, wmat);
-  hipLaunchKernelGGL(k_dense_apply, dim3(batch), dim3(DN_THREADS), 0, st, (const double2*)wmat, (const float2*)y,
-                     obs_idx, m, mp, mf, rows);
+  // Q^-1 y = W^H W y and the rows of conj(W), both in the full sample layout
+  hipLaunchKernelGGL(k_dense_pos_fill, dim3((mf + 255) / 256), dim3(256), 0, st, obs_idx ? 1 : 0, mf, pos);
+  if (obs_idx) hipLaunchKernelGGL(k_dense_pos_scatter, dim3((m + 255) / 256), dim3(256), 0, st, obs_idx, m, mf, pos);
+  hipLaunchKernelGGL(k_dense_u, dim3((m + DN_THREADS / 64 - 1) / (DN_THREADS / 64), batch), dim3(DN_THREADS), 0, st,
+                     (const double2*)wmat, (const float2*)y, obs_idx, m, mp, mf, u);
+  hipLaunchKernelGGL(k_dense_wh, dim3((m + 63) / 64, nch, batch), dim3(DN_THREADS), 0, st, (const double2*)wmat,
+                     (const double2*)u, m, mp, nch, vpart);
+  hipLaunchKernelGGL(k_dense_vsum, dim3((m + 255) / 256, batch), dim3(256), 0, st, (const double2*)vpart, m, nch, v);
   const long long t1_e = (long long)(m + 1) * m1 * n2;
   hipLaunchKernelGGL(k_dense_t1, dim3((unsigned)((t1_e + DN_THREADS - 1) / DN_THREADS), batch), dim3(DN_THREADS), 0,
-                     st, rows, m + 1, m1, m2, n2, t1);
+                     st, (const double2*)wmat, (const double2*)v, (const int*)pos, m, mp, m1, m2, n2, t1);
   const int blocks = (n + DN_THREADS - 1) / DN_THREADS;
-  hipLaunchKernelGGL(k_dense_moments, dim3(blocks, batch), dim3(DN_THREADS), 0, st, t1, m, m1, n1, n2, gamma,
-                     (float2*)mean, var, dd, part);
+  hipLaunchKernelGGL(k_dense_ddpart, dim3(blocks, nrch, batch), dim3(DN_THREADS), 0, st, (const double2*)t1, m, m1,
+                     n1, n2, nrch, ddpart);
+  hipLaunchKernelGGL(k_dense_moments, dim3(blocks, batch), dim3(DN_THREADS), 0, st, (const double2*)t1,
+                     (const double*)ddpart, m, m1, n1, n2, nrch, gamma, (float2*)mean, var, dd, part);
   hipLaunchKernelGGL(k_dense_seg, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum);
   return (int)hipGetLastError();
 }
