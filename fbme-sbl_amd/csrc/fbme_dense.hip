@@ -141,47 +141,60 @@ k_tile_diag(double2* __restrict__ qs, int mp, int jb, double tiny, double2* __re
             int* __restrict__ status) {
   __shared__ double2 a[TS][TS + 1];
   __shared__ double2 x[TS][TS + 1];
+  __shared__ double piv[TS];
   const int t = threadIdx.x, b = blockIdx.x, nb = mp / TS;
   double2* q = qs + (long long)b * mp * mp;
   const long long o = (long long)jb * TS * mp + (long long)jb * TS;
   for (int e = t; e < TS * TS; e += DN_THREADS) a[e / TS][e % TS] = q[o + (long long)(e / TS) * mp + e % TS];
   __syncthreads();
+  // Right-looking factorisation, one barrier per column: column j stays
+  // unscaled (L[i][j] = a[i][j] / piv_j is applied at the end), so step j
+  // reads only column j and writes only columns > j.
   for (int j = 0; j < TS; ++j) {
+    const double d = a[j][j].x;
+    const double dc = fmax(d, tiny);
     if (t == 0) {
-      const double d = a[j][j].x;
       if ((!(d > 0.0) || !isfinite(d)) && status) status[b] = 1;
-      a[j][j] = make_double2(sqrt(fmax(d, tiny)), 0.0);
+      piv[j] = sqrt(dc);
     }
-    __syncthreads();
-    const double ip = 1.0 / a[j][j].x;
-    for (int i = j + 1 + t; i < TS; i += DN_THREADS) a[i][j] = make_double2(a[i][j].x * ip, a[i][j].y * ip);
-    __syncthreads();
+    const double id = 1.0 / dc;
     const int rem = TS - j - 1;
     for (int e = t; e < rem * rem; e += DN_THREADS) {
       const int i = j + 1 + e / rem, k = j + 1 + e % rem;
       if (k <= i) {
         const double2 v = zcm(a[i][j], a[k][j]);
-        a[i][k] = make_double2(a[i][k].x - v.x, a[i][k].y - v.y);
+        a[i][k] = make_double2(a[i][k].x - v.x * id, a[i][k].y - v.y * id);
       }
     }
     __syncthreads();
   }
-  // inverse of the lower factor, column c by thread c
-  if (t < TS) {
-    const int c = t;
-    for (int i = 0; i < TS; ++i) {
-      if (i < c) {
-        x[i][c] = make_double2(0.0, 0.0);
-        continue;
-      }
-      double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
-      for (int k = c; k < i; ++k) {
-        const double2 v = zm(a[i][k], x[k][c]);
-        s = make_double2(s.x - v.x, s.y - v.y);
-      }
-      const double il = 1.0 / a[i][i].x;
-      x[i][c] = make_double2(s.x * il, s.y * il);
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    const int i = e / TS, j = e % TS;
+    if (i > j) {
+      const double sj = 1.0 / piv[j];
+      a[i][j] = make_double2(a[i][j].x * sj, a[i][j].y * sj);
+    } else if (i == j) {
+      a[i][i] = make_double2(piv[i], 0.0);
     }
+    x[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  // X = L^-1 for all 32 columns at once, one barrier per row step: row k of
+  // X stays unscaled (divided by L[k][k] at the end), so step k reads row k
+  // and writes rows > k.
+  for (int k = 0; k < TS - 1; ++k) {
+    const double il = 1.0 / a[k][k].x;
+    for (int e = t; e < (TS - 1 - k) * TS; e += DN_THREADS) {
+      const int i = k + 1 + e / TS, c = e % TS;
+      const double2 v = zm(a[i][k], x[k][c]);
+      x[i][c] = make_double2(x[i][c].x - v.x * il, x[i][c].y - v.y * il);
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    const int i = e / TS, c = e % TS;
+    const double il = 1.0 / a[i][i].x;
+    x[i][c] = make_double2(x[i][c].x * il, x[i][c].y * il);
   }
   __syncthreads();
   double2* li = linv + ((long long)b * nb + jb) * TS * TS;
