@@ -277,28 +277,24 @@ k_tile_update(double2* __restrict__ qs, int mp, int jb) {
   }
 }
 
-// Block row i of W = L^-1 (into w, separate from L): W_ii = L_ii^-1 and, for
-// j = blockIdx.x < i, W_ij = -L_ii^-1 sum_{k=j}^{i-1} L_ik W_kj.
+constexpr int DN_KS = 8;   // interleaved partial sums per tile of W
+
+// Partial sums of block row ib of W = L^-1: for j = blockIdx.x < ib and
+// p = blockIdx.y, spart[j, p] = sum over k = j + p, j + p + KS, .. < ib of
+// L_ik W_kj (the row's serial chain split KS ways).
 __global__ void __launch_bounds__(DN_THREADS)
-k_tile_inverse_row(const double2* __restrict__ qs, int mp, int ib, const double2* __restrict__ linv,
-                   double2* __restrict__ ws) {
+k_tile_inverse_part(const double2* __restrict__ qs, int mp, int ib, const double2* __restrict__ ws,
+                    double2* __restrict__ spart) {
   __shared__ double2 l[TS][TS + 1];
   __shared__ double2 x[TS][TS + 1];
-  __shared__ double2 s[TS][TS + 1];
-  const int t = threadIdx.x, b = blockIdx.y, nb = mp / TS, jb = blockIdx.x;
+  const int t = threadIdx.x, jb = blockIdx.x, pk = blockIdx.y, b = blockIdx.z, nb = mp / TS;
   const double2* q = qs + (long long)b * mp * mp;
-  double2* w = ws + (long long)b * mp * mp;
-  const double2* lii = linv + ((long long)b * nb + ib) * TS * TS;
+  const double2* w = ws + (long long)b * mp * mp;
   const int r = t >> 3, c0 = t & 7;
-  if (jb == ib) {
-    for (int e = t; e < TS * TS; e += DN_THREADS)
-      w[(long long)(ib * TS + e / TS) * mp + ib * TS + e % TS] = lii[e];
-    return;
-  }
   double2 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
-  for (int kb = jb; kb < ib; ++kb) {
+  for (int kb = jb + pk; kb < ib; kb += DN_KS) {
     for (int e = t; e < TS * TS; e += DN_THREADS) {
       l[e / TS][e % TS] = q[(long long)(ib * TS + e / TS) * mp + kb * TS + e % TS];
       x[e / TS][e % TS] = w[(long long)(kb * TS + e / TS) * mp + jb * TS + e % TS];
@@ -314,10 +310,41 @@ k_tile_inverse_row(const double2* __restrict__ qs, int mp, int ib, const double2
     }
     __syncthreads();
   }
+  double2* sp = spart + (((long long)b * nb + jb) * DN_KS + pk) * TS * TS;
 #pragma unroll
-  for (int u = 0; u < 4; ++u) s[r][c0 + 8 * u] = acc[u];
-  for (int e = t; e < TS * TS; e += DN_THREADS) l[e / TS][e % TS] = lii[e];
+  for (int u = 0; u < 4; ++u) sp[r * TS + c0 + 8 * u] = acc[u];
+}
+
+// Block row i of W = L^-1 (into w, separate from L): W_ii = L_ii^-1 and, for
+// j = blockIdx.x < i, W_ij = -L_ii^-1 sum_{k=j}^{i-1} L_ik W_kj from the KS
+// partial sums (added in a fixed order).
+__global__ void __launch_bounds__(DN_THREADS)
+k_tile_inverse_row(int mp, int ib, const double2* __restrict__ linv, const double2* __restrict__ spart,
+                   double2* __restrict__ ws) {
+  __shared__ double2 l[TS][TS + 1];
+  __shared__ double2 s[TS][TS + 1];
+  const int t = threadIdx.x, b = blockIdx.y, nb = mp / TS, jb = blockIdx.x;
+  double2* w = ws + (long long)b * mp * mp;
+  const double2* lii = linv + ((long long)b * nb + ib) * TS * TS;
+  const int r = t >> 3, c0 = t & 7;
+  if (jb == ib) {
+    for (int e = t; e < TS * TS; e += DN_THREADS)
+      w[(long long)(ib * TS + e / TS) * mp + ib * TS + e % TS] = lii[e];
+    return;
+  }
+  const double2* sp = spart + ((long long)b * nb + jb) * DN_KS * TS * TS;
+  for (int e = t; e < TS * TS; e += DN_THREADS) {
+    double sx = 0.0, sy = 0.0;
+    for (int p = 0; p < DN_KS; ++p) {
+      const double2 v = sp[(long long)p * TS * TS + e];
+      sx += v.x;
+      sy += v.y;
+    }
+    s[e / TS][e % TS] = make_double2(sx, sy);
+    l[e / TS][e % TS] = lii[e];
+  }
   __syncthreads();
+  double2 acc[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) acc[u] = make_double2(0.0, 0.0);
   for (int k = 0; k < TS; ++k) {
@@ -554,7 +581,7 @@ __global__ void k_dense_seg(const double* __restrict__ part, int blocks, double*
 }
 
 struct DenseLayout {
-  size_t t, lags, q, w, linv, pos, u, vpart, v, t1, ddpart, total;
+  size_t t, lags, q, w, linv, spart, pos, u, vpart, v, t1, ddpart, total;
 };
 
 inline size_t align256(size_t v) { return (v + 255) / 256 * 256; }
@@ -574,6 +601,7 @@ DenseLayout dense_layout(int batch, int n1, int n2, int m1, int m2, int m) {
   l.q = take(16 * b * mp * mp);
   l.w = take(16 * b * mp * mp);
   l.linv = take(16 * b * mp * 32);
+  l.spart = take(16 * b * (mp / 32) * DN_KS * 32 * 32);
   const size_t nch = (size_t)((m + DN_VCH - 1) / DN_VCH), nrch = (size_t)((m + DN_RCH - 1) / DN_RCH);
   l.pos = take(4 * mf);
   l.u = take(16 * b * m);
@@ -641,9 +669,14 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
       hipLaunchKernelGGL(k_tile_update, dim3(rest * (rest + 1) / 2, batch), dim3(DN_THREADS), 0, st, q, mp, jb);
     }
   }
-  for (int ib = 0; ib < nb; ++ib)
-    hipLaunchKernelGGL(k_tile_inverse_row, dim3(ib + 1, batch), dim3(DN_THREADS), 0, st, (const double2*)q, mp, ib,
-                       (const double2*)linv, wmat);
+  double2* spart = (double2*)(w + l.spart);
+  for (int ib = 0; ib < nb; ++ib) {
+    if (ib > 0)
+      hipLaunchKernelGGL(k_tile_inverse_part, dim3(ib, DN_KS, batch), dim3(DN_THREADS), 0, st, (const double2*)q, mp,
+                         ib, (const double2*)wmat, spart);
+    hipLaunchKernelGGL(k_tile_inverse_row, dim3(ib + 1, batch), dim3(DN_THREADS), 0, st, mp, ib,
+                       (const double2*)linv, (const double2*)spart, wmat);
+  }
   // Q^-1 y = W^H W y and the rows of conj(W), both in the full sample layout
   hipLaunchKernelGGL(k_dense_pos_fill, dim3((mf + 255) / 256), dim3(256), 0, st, obs_idx ? 1 : 0, mf, pos);
   if (obs_idx) hipLaunchKernelGGL(k_dense_pos_scatter, dim3((m + 255) / 256), dim3(256), 0, st, obs_idx, m, mf, pos);
