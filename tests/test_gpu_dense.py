@@ -86,6 +86,45 @@ def test_dense_posterior_step_vs_oracle(device, grid, samples, masked):
     assert _rel(s.dd.cpu().numpy()[0], diag) < 1e-4
 
 
+def test_dense_batch_matches_single_scenes(device):
+    """Three scenes in one launch (scene index on the grid's y / z axes of
+    every dense kernel) give each scene's single-launch posterior, and the
+    float64 dense posterior.  40x30 / 20x15: M = 300, ten block rows, so the
+    inverse's partial sums and the row chunks of the data diagonal are
+    exercised."""
+    from torch_sbl import SBLConfig, pcg
+    from fbme_amd import scenes
+    grid, samples, b = (40, 30), (20, 15), 3
+    n = grid[0] * grid[1]
+    rng = np.random.default_rng(5)
+    ys = np.stack([scenes.uniform_scene(80 + i, grid, samples, k=10, snr_db=20.0).y for i in range(b)])
+    gammas = (rng.gamma(0.5, 1.0, (b, n)) * 0.01).astype(np.float32)
+    noises = np.array([0.03, 0.05, 0.08], np.float32)
+    cfg = SBLConfig(mode="dense", device=str(device))
+
+    def run(y, g, nu):
+        s = pcg._Solver(torch.from_numpy(y.astype(np.complex64)).to(device), grid, cfg, None, None, dense=True)
+        s.init()
+        s.gamma.copy_(torch.from_numpy(g))
+        s.noise.copy_(torch.from_numpy(nu))
+        s.posterior(0.0)
+        torch.cuda.synchronize()
+        assert int(s.status.sum()) == 0
+        return s.mean.cpu().numpy(), s.var.cpu().numpy(), s.dd.cpu().numpy()
+
+    batch = run(ys, gammas, noises)
+    for i in range(b):
+        single = run(ys[i:i + 1], gammas[i:i + 1], noises[i:i + 1])
+        for got, want in zip(batch, single):
+            assert _rel(got[i], want[0]) < 1e-6
+    a = fo.dense_prefix_matrix(grid, samples)
+    mean, var, diag = dense_em.dense_posterior(a, ys[1].reshape(-1).astype(np.complex128),
+                                               gammas[1].astype(np.float64), float(noises[1]))
+    assert _rel(batch[0][1], mean) < 1e-4
+    assert _rel(batch[1][1], var) < 1e-4
+    assert _rel(batch[2][1], diag) < 1e-4
+
+
 def test_auto_mode_small_problem_runs_dense(device):
     """Default sbl_2d on the GPU with M <= dense_threshold takes the exact
     dense posterior, as the reference does (solver.py:1025-1036)."""
