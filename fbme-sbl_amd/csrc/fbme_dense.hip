@@ -126,6 +126,7 @@ __device__ double block_sum_dn(double v, double* red) {
 
 // ------------------------------------------------ tiled FP64 factorisation
 constexpr int TS = 32;
+static_assert(DN_THREADS * 4 == TS * TS, "tile kernels give every thread four elements");
 
 __device__ __forceinline__ double2 zm(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -158,12 +159,17 @@ k_tile_diag(double2* __restrict__ qs, int mp, int jb, double tiny, double2* __re
       piv[j] = sqrt(dc);
     }
     const double id = 1.0 / dc;
-    const int rem = TS - j - 1;
-    for (int e = t; e < rem * rem; e += DN_THREADS) {
-      const int i = j + 1 + e / rem, k = j + 1 + e % rem;
-      if (k <= i) {
-        const double2 v = zcm(a[i][j], a[k][j]);
-        a[i][k] = make_double2(a[i][k].x - v.x * id, a[i][k].y - v.y * id);
+    // thread t owns row t / 8, columns t % 8 + 8u (a fixed map: no
+    // division in the step loop)
+    const int i = t >> 3;
+    if (i > j) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = (t & 7) + 8 * u;
+        if (k > j && k <= i) {
+          const double2 v = zcm(a[i][j], a[k][j]);
+          a[i][k] = make_double2(a[i][k].x - v.x * id, a[i][k].y - v.y * id);
+        }
       }
     }
     __syncthreads();
@@ -184,10 +190,14 @@ k_tile_diag(double2* __restrict__ qs, int mp, int jb, double tiny, double2* __re
   // and writes rows > k.
   for (int k = 0; k < TS - 1; ++k) {
     const double il = 1.0 / a[k][k].x;
-    for (int e = t; e < (TS - 1 - k) * TS; e += DN_THREADS) {
-      const int i = k + 1 + e / TS, c = e % TS;
-      const double2 v = zm(a[i][k], x[k][c]);
-      x[i][c] = make_double2(x[i][c].x - v.x * il, x[i][c].y - v.y * il);
+    const int i = t >> 3;
+    if (i > k) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int c = (t & 7) + 8 * u;
+        const double2 v = zm(a[i][k], x[k][c]);
+        x[i][c] = make_double2(x[i][c].x - v.x * il, x[i][c].y - v.y * il);
+      }
     }
     __syncthreads();
   }
