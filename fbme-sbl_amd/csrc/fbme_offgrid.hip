@@ -618,7 +618,10 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   }
   // ----------------------------------------------- projected Gauss--Newton
   data_sums(y, L.pos, 0, k, a, true, L);
-  bool ok = fit(L.pos, k, a, L, L.amp, &L.obj);
+  // The ridge loading keeps the Gram positive definite; a failed pivot of
+  // the normal equations gives a non-finite step, which ends the iteration
+  // below (offgrid.py:408-409), so the factorisations' flags are not needed.
+  fit(L.pos, k, a, L, L.amp, &L.obj);
   if (t == 0) {
     L.lambda = a.damping;
     out_norm[2 * b] = sqrt(L.obj);
@@ -629,7 +632,7 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   for (int it = 1; it <= a.max_iter; ++it) {
     iteration = it;
     normal_eq(k, a, L);
-    ok = solve_step(n2k, a, L);
+    solve_step(n2k, a, L);
     if (t == 0) {
       int finite = 1;
       for (int i = 0; i < n2k; ++i) {

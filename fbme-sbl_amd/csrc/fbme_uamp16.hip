@@ -163,8 +163,7 @@ __global__ void __launch_bounds__(64)
 k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
         float2* __restrict__ mid, float2* __restrict__ bp, float* __restrict__ gamma,
         float2* __restrict__ mean, float* __restrict__ var, const double* __restrict__ sc,
-        double* __restrict__ part, int pstride, UampArgs16 a, uint32_t* __restrict__ cand_key,
-        int* __restrict__ cand_idx, int store_final, uint32_t* __restrict__ score,
+        double* __restrict__ part, int pstride, UampArgs16 a, int store_final, uint32_t* __restrict__ score,
         const float2* __restrict__ twtab) {
   __shared__ float2 xs[4][XS];
   __shared__ float2 tw[256];
@@ -499,8 +498,8 @@ int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, 
   UampArgs16 a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
 #define FBME_ROW16(M)                                                                                        \
   hipLaunchKernelGGL((k_row16<M>), dim3(grid), dim3(64), 0, stream, scenes, work, nwork, (float2*)mid,       \
-                     (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, cand_key, cand_idx,        \
-                     store_final, score, (const float2*)twtab)
+                     (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, store_final, score,       \
+                     (const float2*)twtab)
   switch (mode) {
     case 0: FBME_ROW16(0); break;
     case 1: FBME_ROW16(1); break;
