@@ -618,9 +618,10 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   }
   // ----------------------------------------------- projected Gauss--Newton
   data_sums(y, L.pos, 0, k, a, true, L);
-  // The ridge loading keeps the Gram positive definite; a failed pivot of
-  // the normal equations gives a non-finite step, which ends the iteration
-  // below (offgrid.py:408-409), so the factorisations' flags are not needed.
+  // The ridge loading keeps the Gram positive definite.  A failed pivot of
+  // the normal equations (no finite step exists) or a NaN step ends the
+  // iteration, as the non-finite test after the clamp does in the reference
+  // (offgrid.py:407-409).
   fit(L.pos, k, a, L, L.amp, &L.obj);
   if (t == 0) {
     L.lambda = a.damping;
@@ -632,11 +633,14 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   for (int it = 1; it <= a.max_iter; ++it) {
     iteration = it;
     normal_eq(k, a, L);
-    solve_step(n2k, a, L);
+    const bool solved = solve_step(n2k, a, L);
     if (t == 0) {
-      int finite = 1;
+      int finite = solved ? 1 : 0;
       for (int i = 0; i < n2k; ++i) {
-        L.step[i] = fmin(fmax(L.step[i], -a.max_step), a.max_step);
+        // torch.clamp semantics: NaN stays NaN (and ends the iteration),
+        // +-Inf clamps to +-max_step (fmin/fmax alone would drop the NaN).
+        const double v = L.step[i];
+        L.step[i] = isnan(v) ? v : fmin(fmax(v, -a.max_step), a.max_step);
         finite &= isfinite(L.step[i]) ? 1 : 0;
       }
       L.flag = finite;

@@ -31,18 +31,6 @@ constexpr int KP = KM + 1;   // padded row length in LDS
 constexpr int TE_MAX = 8;
 constexpr int DIR_LDS = 1024;   // double2 slots for the two Dirichlet tables
 
-// Phase timestamps for kernel analysis builds (-DFBME_PROBE): ticks of the
-// 100 MHz wall clock since kernel entry, written to status[scene * 16 + i].
-#ifdef FBME_PROBE
-#define FBME_MARK(i)                                                        \
-  do {                                                                     \
-    if (threadIdx.x == 0 && status) status[blockIdx.x * 16 + (i)] =        \
-        (int)(wall_clock64() - probe_t0);                                  \
-  } while (0)
-#else
-#define FBME_MARK(i) do {} while (0)
-#endif
-
 struct PolishArgs {
   float margin_factor, jitter_multiplier, max_precision, polish_damping;
   float hyper_shape, hyper_rate, noise_shape, noise_rate;
@@ -428,9 +416,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
 
   const int si = blockIdx.x;
   if (si >= nscenes) return;
-#ifdef FBME_PROBE
-  const long long probe_t0 = wall_clock64();
-#endif
   const FbmeScene s = scenes[si];
   const int tid = threadIdx.x;
   const double* scs = sc + (long long)si * SC_COUNT;
@@ -455,7 +440,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   }
   c.bp = bp_all + s.grid_off;
   __syncthreads();
-  FBME_MARK(0);
   const int K = s.k;
   const long long n_grid = (long long)s.n1 * s.n2;
   PolishState* st = states + si;
@@ -524,7 +508,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     }
     work_noise = scs[SC_NOISE];
     __syncthreads();
-    FBME_MARK(1);
   } else {
     // Scenes without an exchange finished in the first phase.
     if (min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K))) <= 0) return;
@@ -621,7 +604,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
 
   for (;;) {
     const double nle = evaluate(c, work_s, work_gam, work_noise, L, a.jitter_multiplier);
-    FBME_MARK(2 + 2 * pit);
     bool accept = true;
     if (have_best) {
       const double margin = (double)a.margin_factor * F32_EPS_D * (1.0 + fabs(nle) + fabs(best_nle));
@@ -673,7 +655,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     upd = fmax(upd, c.floor_v);
     work_noise = keep * best_noise + (1.0 - keep) * upd;
     __syncthreads();
-    FBME_MARK(3 + 2 * pit);
     const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
     if (exch > 0 && pit == 0 && phase == 0) {
       if (tid < K) {
@@ -712,9 +693,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     out_nle[oi] = (float)best_nle;
     for (int i = 0; i < a.evaluations; ++i) out_trace[(long long)oi * a.evaluations + i] = (float)trace[i];
     if (out_exchanges) out_exchanges[oi] = exchanges;
-#ifndef FBME_PROBE
     if (status) status[oi] = L.flag[0];
-#endif
   }
   if (a.diagnostics) {
     // Fixed-budget EM residuals (solver.py:3408-3496) and trace flags.
@@ -764,7 +743,6 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     res->gam[rank] = (float)best_gam[tid];
   }
   if (tid == 0) results[si].k = K;
-  FBME_MARK(15);
 }
 
 // |A^H (y - C mu)|^2 over one tile of the grid with the support excluded,

@@ -689,13 +689,17 @@ class _Solver:
         total = float(self.sums[0, 0])
         if not np.isfinite(total):
             return
-        top = self.top_val[0].double().cpu().numpy()
+        # solver.py:3200-3216 on the top min(max_size, N) variances; the
+        # kernel ranks the top RANK, so a switch that would need more atoms
+        # keeps the PCG posterior.
+        cap = min(cfg.active_set_max_size, self.n)
+        top = self.top_val[0].double().cpu().numpy()[:min(cap, RANK)]
         target = (1.0 - cfg.active_set_tail_fraction) * total
         if top.sum() < target:
             return
         cum = np.cumsum(top)
-        required = int(np.searchsorted(cum, target)) + 1
-        size = min(RANK, max(cfg.active_set_min_size, required))
+        required = min(int(np.searchsorted(cum, target)), cap - 1) + 1
+        size = min(cap, max(cfg.active_set_min_size, required))
         if size > RANK:
             return
         self.support = self.top_idx[:, :size].contiguous()

@@ -6,9 +6,9 @@ independent scenes solved together.  Execution modes:
 * ``mode="uamp"`` runs the native FBME plan (HIP kernels, one C call);
 * ``mode="cg"`` runs the covariance-free PCG + Hutchinson path on HIP
   kernels (torch_sbl.pcg);
-* ``mode="dense"`` (and ``"auto"`` resolving to it) is the exact
-  observation-domain EM of the reference, which this build keeps only as the
-  CPU float64 oracle; on the GPU it raises NotImplementedError.
+* ``mode="dense"`` (and ``"auto"`` resolving to it when M <= 2048) is the
+  exact observation-domain EM (FP64 tiled Cholesky of the M x M covariance on
+  HIP kernels; M <= 2048, grid axes <= 1024, no joint sparsity).
 
 Only GPU tensors are accepted: there is no CPU execution path.
 """

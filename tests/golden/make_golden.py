@@ -299,10 +299,90 @@ def offgrid(reference_root: str) -> None:
         print("wrote", name)
 
 
+HEADLINE_CASES = [
+    # name, generator, seed, grid, samples, k, snr_db, profiles
+    ("headline_256x256_160x160_s1000", "ellipse", 1000, (256, 256), (160, 160), 300, 10.0, ("fast", "stable")),
+    ("headline_256x256_160x160_s1001", "ellipse", 1001, (256, 256), (160, 160), 300, 10.0, ("fast", "stable")),
+    ("headline_256x256_160x160_s1002", "ellipse", 1002, (256, 256), (160, 160), 300, 10.0, ("fast", "stable")),
+    ("headline_512x512_384x256_0db", "clustered", 1000, (512, 512), (384, 256), 1200, 0.0, ("stable",)),
+    ("headline_512x512_384x256_10db", "clustered", 1000, (512, 512), (384, 256), 1200, 10.0, ("stable",)),
+    ("headline_512x512_384x256_20db", "clustered", 1000, (512, 512), (384, 256), 1200, 20.0, ("stable",)),
+]
+
+
+def headline_scene(generator, seed, grid, samples, k, snr_db):
+    """The scene of one headline fixture (regenerated by the tests from its
+    seed; the fixture pins the bytes of ``y`` by their SHA-256)."""
+    fn = scenes.ellipse_scene if generator == "ellipse" else scenes.clustered_scene
+    return fn(seed, grid, samples, k=k, snr_db=snr_db)
+
+
+def y_digest(y: np.ndarray) -> str:
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(y, dtype=np.complex64).tobytes()).hexdigest()
+
+
+def headline(reference_root: str) -> None:
+    """``fast_sbl`` under the FBME-Fast / FBME-Stable numerics at the
+    headline sizes, where K >> polish_size: 256x256 / 160x160 with a
+    300-scatterer ellipse at 10 dB (three seeds, both profiles), and
+    512x512 / 384x256 with 1200 clustered scatterers at 0 / 10 / 20 dB
+    (Stable).  The inputs are not stored: the fixture keeps their SHA-256
+    and the generator arguments.  ``x`` is exactly zero outside the polished
+    atoms (prune_tail), so it is stored sparsely; the posterior variance is
+    stored on the polished atoms plus its grid sum."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, fast_sbl
+
+    torch.set_num_threads(8)
+    fast_cfg = SBLConfig.paper_hybrid_uamp(
+        device="cpu", uamp_backend="torch",
+        uamp_polish_size=32, uamp_polish_iterations=2,
+        uamp_polish_residual_exchange_size=0,
+        uamp_polish_fixed_budget_diagnostics=False,
+        uamp_polish_compute_truncation_diagnostic=False)
+    stable_cfg = SBLConfig.realtime_hybrid_uamp(
+        device="cpu", uamp_backend="torch", uamp_polish_size=32,
+        uamp_polish_iterations=3, uamp_polish_residual_exchange_size=12,
+        uamp_polish_fixed_budget_diagnostics=True,
+        uamp_polish_compute_truncation_diagnostic=False)
+    for name, gen, seed, grid, samples, k, snr, profiles in HEADLINE_CASES:
+        scene = headline_scene(gen, seed, grid, samples, k, snr)
+        arrays = dict(generator=np.array(gen), seed=np.array(seed), grid=np.array(grid),
+                      samples=np.array(samples), k=np.array(k), snr_db=np.array(snr),
+                      y_sha256=np.array(y_digest(scene.y)))
+        data = torch.from_numpy(scene.y)
+        for prof in profiles:
+            res = fast_sbl(data, grid, config=fast_cfg if prof == "fast" else stable_cfg)
+            x = _np(res.x).reshape(-1)
+            nz = np.flatnonzero(x)
+            var = _np(res.posterior_variance).reshape(-1)
+            p = prof + "_"
+            arrays[p + "x_index"] = nz.astype(np.int64)
+            arrays[p + "x_value"] = x[nz]
+            arrays[p + "variance_on_support"] = var[nz]
+            arrays[p + "variance_sum"] = np.array(float(np.sum(var, dtype=np.float64)))
+            arrays[p + "support_mask_index"] = np.flatnonzero(_np(res.support_mask).reshape(-1)).astype(np.int64)
+            for field in ("noise_variance", "effective_dof", "negative_log_evidence",
+                          "evidence_trace", "support_exchanges_accepted",
+                          "fixed_budget_gamma_residual", "fixed_budget_noise_residual",
+                          "fixed_budget_monotone"):
+                value = getattr(res, field, None)
+                if value is not None:
+                    arrays[p + field] = _np(value)
+            arrays[p + "iterations"] = np.array(res.iterations)
+            arrays[p + "active_set_size"] = np.array(res.active_set_size)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, flush=True)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "long_cg":
+    if which == "headline":
+        headline(root)
+    elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
         masked_cg(root)
@@ -318,3 +398,4 @@ if __name__ == "__main__":
         masked_plans(root)
         joint_cg(root)
         offgrid(root)
+        headline(root)

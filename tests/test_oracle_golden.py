@@ -13,6 +13,8 @@ import pytest
 from conftest import golden_path
 from oracle import fbme_oracle as fo
 
+import headline_golden
+
 CASES = ["plan_32x32_24x24", "plan_40x32_20x16", "plan_64x64_40x40",
          "plan_96x80_60x48",
          # observation masks (fast_sbl(..., observation_mask=), 70% / 60%)
@@ -91,3 +93,15 @@ def test_masked_gram_apply_is_dense():
     mu = np.array([[1 + 2j, -0.5j, 0.25, 3.0]])
     np.testing.assert_allclose(fo._gram_apply(mu, sup, grid, samples, mask)[0],
                                a.conj().T @ (a[:, sup[0]] @ mu[0]), atol=1e-11)
+
+
+@pytest.mark.parametrize("case,profile", headline_golden.CASES)
+def test_oracle_matches_reference_at_headline_sizes(case, profile):
+    """256x256 / 160x160 (K=300 ellipse, 10 dB, three seeds) and 512x512 /
+    384x256 (K=1200 clustered, 0/10/20 dB): K >> polish_size, where top-k
+    boundaries, evidence gates and exchanges meet many strong off-support
+    atoms and the reference's float32 atom phases are least accurate."""
+    g, scene = headline_golden.load(case)
+    out = fo.fbme_plan(scene.y, scene.grid, profile)
+    rel = headline_golden.compare(g, profile, out, trace_rtol=2e-4, trace_atol=0.1)
+    print(case, profile, "oracle vs reference", rel)
