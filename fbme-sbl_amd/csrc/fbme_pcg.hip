@@ -142,6 +142,22 @@ k_topk_tile(const float* __restrict__ x, long long n, int tiles, unsigned long l
   }
 }
 
+// One level of the candidate merge for large grids: every group of up to
+// 2048 candidate keys of a scene (one wavefront) keeps its top RANK.  Keys
+// pack (value, position), so each level is exact and keeps the tie order.
+__global__ void __launch_bounds__(64)
+k_topk_merge_keys(const unsigned long long* __restrict__ cand_in, int count, int groups,
+                  unsigned long long* __restrict__ cand_out) {
+  const int g = blockIdx.x, b = blockIdx.y;
+  unsigned long long v[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int slot = g * 2048 + threadIdx.x + 64 * i;
+    v[i] = (slot < count) ? cand_in[(long long)b * count + slot] : 0ull;
+  }
+  wave_topk<32>(v, RANK, cand_out + ((long long)b * groups + g) * RANK);
+}
+
 // Merge the tile candidates of each scene into the sorted top RANK
 // (values and indices), one wavefront per scene (tiles * RANK <= 2048).
 __global__ void __launch_bounds__(64)
@@ -1169,17 +1185,37 @@ int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max) {
   return 0;
 }
 
+size_t fbme_topk32_scratch_bytes(int batch, long long n) {
+  if (batch < 1 || n < RANK || n > (1ll << 26)) return 0;
+  long long count = (n + 1023) / 1024 * RANK, total = count;
+  while (count > 2048) {
+    count = (count + 2047) / 2048 * RANK;
+    total += count;
+  }
+  return (size_t)batch * (size_t)total * sizeof(unsigned long long);
+}
+
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val) {
-  // cand: batch * ceil(n / 1024) * 32 uint64; requires ceil(n/1024) * 32 <= 2048.
+  // cand: fbme_topk32_scratch_bytes(batch, n) bytes.  Per-tile top 32, then
+  // merge levels of 2048 candidates until one wavefront holds them all.
+  if (fbme_topk32_scratch_bytes(batch, n) == 0 || !cand) return FBME_ERR_ARG;
   const int tiles = (int)((n + 1023) / 1024);
-  if (tiles * RANK > 2048) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_topk_tile, dim3(tiles, batch), dim3(FBME_BLOCK), 0, st, x, n, tiles,
-                     (unsigned long long*)cand);
-  hipLaunchKernelGGL(k_topk_merge, dim3(batch), dim3(64), 0, st, (const unsigned long long*)cand, tiles, x, n, idx,
-                     val);
+  unsigned long long* cur = (unsigned long long*)cand;
+  hipLaunchKernelGGL(k_topk_tile, dim3(tiles, batch), dim3(FBME_BLOCK), 0, st, x, n, tiles, cur);
+  int count = tiles * RANK;
+  while (count > 2048) {
+    const int groups = (count + 2047) / 2048;
+    unsigned long long* next = cur + (long long)batch * count;
+    hipLaunchKernelGGL(k_topk_merge_keys, dim3(groups, batch), dim3(64), 0, st, cur, count, groups, next);
+    cur = next;
+    count = groups * RANK;
+  }
+  hipLaunchKernelGGL(k_topk_merge, dim3(batch), dim3(64), 0, st, (const unsigned long long*)cur, count / RANK, x, n,
+                     idx, val);
   return (int)hipGetLastError();
 }
+
 
 int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx, const float* val,
                        const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,

@@ -27,7 +27,7 @@ Differences from the reference, all documented:
   * an active-set switch is taken when the 32 strongest variances explain
     95% of the total (the reference then also selects 32 atoms); cases where
     the reference would switch to 33-256 atoms keep the PCG posterior;
-  * the low-rank preconditioner requires ceil(N / 1024) <= 64;
+  * the low-rank preconditioner's top-32 selection takes N <= 2^26 cells;
   * 'circulant', 'nystrom' and 'none' preconditioners, Hadamard probes,
     adaptive probe counts and spatial coupling raise NotImplementedError.
 
@@ -303,10 +303,10 @@ class _Solver:
         self.spectrum = torch.empty((ns, e1, e2), dtype=c64, device=dev)
         self.gmid = (torch.empty(self.lanes * 256 * m2, dtype=c64, device=dev) if self.grid_op else None)
         # preconditioner
-        tiles = (n + 1023) // 1024
-        if tiles * RANK > 2048:
-            raise NotImplementedError("grid too large for the low-rank top-32 merge")
-        self.cand = torch.empty(b * tiles * RANK, dtype=torch.int64, device=dev)
+        cand_bytes = int(self.lib.fbme_topk32_scratch_bytes(b, n))
+        if cand_bytes == 0:
+            raise NotImplementedError(f"the top-32 selection takes 32 <= N <= 2^26 grid cells, not {n}")
+        self.cand = torch.empty(cand_bytes // 8, dtype=torch.int64, device=dev)
         self.top_idx = torch.empty((b, RANK), dtype=torch.int32, device=dev)
         self.top_val = torch.empty((b, RANK), dtype=f32, device=dev)
         self.factors = torch.empty((ns, m1 + m2, RANK), dtype=c64, device=dev)

@@ -260,8 +260,10 @@ int fbme_pcg_direction_parts(void* stream, int lanes, int m, void* d, const void
                              const double* curv, const double* part, int nparts, double* nrho, int* active);
 int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const double* curv, int* active);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
- * index (the stable argsort of solver.py:5797-5800).  cand: batch *
- * ceil(n / 1024) * 32 uint64; requires n <= 65536. */
+ * index (the stable argsort of solver.py:5797-5800).  cand: scratch of
+ * fbme_topk32_scratch_bytes(batch, n) bytes (per-tile candidates, then one
+ * level of merged candidates per 64x reduction); 32 <= n <= 2^26. */
+size_t fbme_topk32_scratch_bytes(int batch, long long n);
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val);
 /* Low-rank preconditioner (_make_low_rank_preconditioner,
  * solver.py:5786-5833).  The factor matrix F = columns * sqrt(gamma_k)

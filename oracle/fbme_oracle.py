@@ -35,6 +35,7 @@ from dataclasses import dataclass, field
 from math import ceil, log
 
 import numpy as np
+import scipy.fft as _sfft
 
 F32_EPS = float(np.finfo(np.float32).eps)
 F32_TINY = float(np.finfo(np.float32).tiny)
@@ -51,7 +52,7 @@ def prefix_forward(x: np.ndarray, grid, samples) -> np.ndarray:
     (operators.py:237-253)."""
     b = x.shape[0]
     field_ = x.reshape(b, *grid).astype(np.complex128)
-    spec = np.fft.fft2(field_, axes=(1, 2))
+    spec = _sfft.fft2(field_, axes=(1, 2), workers=-1)
     return spec[:, : samples[0], : samples[1]].reshape(b, -1)
 
 
@@ -61,7 +62,7 @@ def prefix_adjoint(v: np.ndarray, grid, samples) -> np.ndarray:
     b = v.shape[0]
     field_ = v.reshape(b, *samples).astype(np.complex128)
     n = grid[0] * grid[1]
-    out = np.fft.ifft2(field_, s=grid, axes=(1, 2)) * n
+    out = _sfft.ifft2(field_, s=grid, axes=(1, 2), workers=-1) * n
     return out.reshape(b, -1)
 
 

@@ -246,3 +246,63 @@ def test_fft_preconditioner_matches_direct_and_oracle(device, grid, samples):
     pre = co.low_rank_preconditioner(gamma, noise, co.Operator(grid, samples), 32, F32_EPS, F32_TINY)
     oref = np.moveaxis(co.apply_preconditioner(np.moveaxis(v.astype(np.complex128), 1, 2), pre), 2, 1)
     assert _rel(g, oref) < 1e-4
+
+
+@pytest.mark.parametrize("n", [65536 + 1000, 262144, 1 << 20])
+def test_topk32_multilevel_merge_vs_stable_argsort(device, n):
+    """Top-32 selection beyond one merge wavefront (N > 65,536: one or two
+    extra merge levels) against numpy's stable argsort, with many exact ties
+    (values quantised to 64 levels, so ties at the threshold are decided by
+    the lower index) and the maximum placed in the last tile."""
+    import ctypes
+    from fbme_amd import _lib
+    lib = _lib.load()
+    b = 3
+    rng = np.random.default_rng(n)
+    x = (rng.integers(0, 64, size=(b, n)) / 64.0).astype(np.float32)
+    x[1, n - 1] = 2.0
+    xt = torch.from_numpy(x).to(device)
+    nbytes = int(lib.fbme_topk32_scratch_bytes(b, n))
+    assert nbytes > b * ((n + 1023) // 1024) * 32 * 8
+    cand = torch.empty(nbytes // 8, dtype=torch.int64, device=device)
+    idx = torch.empty((b, 32), dtype=torch.int32, device=device)
+    val = torch.empty((b, 32), dtype=torch.float32, device=device)
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    _lib.check(lib.fbme_topk32(stream, b, n, p(xt), p(cand), p(idx), p(val)), "topk32")
+    torch.cuda.synchronize()
+    want = np.argsort(-x, axis=1, kind="stable")[:, :32]
+    np.testing.assert_array_equal(idx.cpu().numpy(), want)
+    np.testing.assert_array_equal(val.cpu().numpy(), np.take_along_axis(x, want, 1))
+
+
+def test_sbl_cg_512_nonsquare_vs_oracle(device):
+    """512x512 grid, 384x256 prefix, K=1200 clustered scatterers, 10 dB,
+    32 Rademacher probes, 3 EM iterations, against the float64 oracle's
+    run on the same scene and probes (stored by
+    tests/golden/make_oracle_fixtures.py: the oracle takes ~200 s here).
+    N = 262,144 takes two merge levels in the preconditioner's top-32.
+    Achieved: relative L2 6.1e-6 on x, PCG iterations 24/32/32 as the
+    oracle's."""
+    import hashlib
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, sbl_2d
+    g = np.load(golden_path("oracle_cg_512x512_384x256.npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    sc = scenes.clustered_scene(int(g["seed"]), grid, tuple(int(v) for v in g["samples"]),
+                                k=int(g["k"]), snr_db=float(g["snr_db"]))
+    assert hashlib.sha256(np.ascontiguousarray(sc.y, np.complex64).tobytes()).hexdigest() == str(g["y_sha256"])
+    cfg = SBLConfig(mode="cg", device=str(device), hutchinson_probes=int(g["probes"]), auto_probe_scaling=False,
+                    max_iter=int(g["em_iterations"]))
+    res = sbl_2d(torch.from_numpy(sc.y).to(device), grid, config=cfg)
+    x = res.x.cpu().numpy().reshape(-1)
+    rel = _rel(x, g["x"].astype(np.complex128))
+    print("512 cg: rel", rel, "pcg its", res.history["cg_iterations"], list(g["cg_iterations"]),
+          "nmse", scenes.nmse_db(x, sc.x.reshape(-1)), flush=True)
+    assert res.mode == "cg" and res.iterations == int(g["iterations"])
+    assert [int(v) for v in res.history["cg_iterations"]] == [int(v) for v in g["cg_iterations"]]
+    assert rel < 1e-4
+    truth = sc.x.reshape(-1)
+    assert abs(scenes.nmse_db(x, truth) - scenes.nmse_db(g["x"], truth)) < 0.01
+    assert float(res.noise_variance) == pytest.approx(float(g["noise_variance"][0]), rel=1e-4)
+    assert _rel(res.posterior_variance.cpu().numpy().reshape(-1), g["posterior_variance"].astype(np.float64)) < 1e-4
