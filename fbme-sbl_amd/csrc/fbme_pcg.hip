@@ -17,6 +17,7 @@
 //     damping (1776-1798).
 // Vectors of lane (b, r) are contiguous: X[(b * R + r) * M + m].
 #include <algorithm>
+#include <mutex>
 
 #include "fbme_device.h"
 
@@ -1105,6 +1106,34 @@ int toeplitz_col_cap(int e1, int e2) { return std::max(1, std::min({e2, 16, 24 *
 
 }  // namespace
 
+// Scratch of fbme_topk32_noalloc: a fixed device array instead of a
+// workspace argument, for callers that cannot allocate.  Each GPU holds its
+// own copy at its own address; host code uses the address that
+// hipGetSymbolAddress returns for the current device, never the array's name.
+static __device__ __align__(256) unsigned long long scratch[FBME_TOPK32_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace {
+
+constexpr int SCRATCH_DEVICES = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[SCRATCH_DEVICES] = {};
+
+// Device address of `scratch` on the current device, resolved once per device.
+int scratch_address(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= SCRATCH_DEVICES) return FBME_ERR_HIP;
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FBME_ERR_HIP;
+    scratch_ptr[dev] = p;
+  }
+  *out = scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+
 // Launcher of k_lr_corr shared with the FFT-based preconditioner passes.
 int fbme_lr_corr_launch(int lanes, int rhs, int chunks, const void* part, const void* sinv, const float* bg,
                         void* corr, hipStream_t st) {
@@ -1216,6 +1245,16 @@ int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand
   return (int)hipGetLastError();
 }
 
+int fbme_topk32_noalloc(void* stream, int batch, long long n, const float* x, int* idx, float* val) {
+  // Every candidate slot is written before it is read, so the shared scratch
+  // needs no clearing between calls.
+  const size_t need = fbme_topk32_scratch_bytes(batch, n);
+  if (need == 0 || need > FBME_TOPK32_NOALLOC_SCRATCH_BYTES) return FBME_ERR_ARG;
+  void* ptr = nullptr;
+  const int rc = scratch_address(&ptr);
+  if (rc) return rc;
+  return fbme_topk32(stream, batch, n, x, ptr, idx, val);
+}
 
 int fbme_lowrank_setup(void* stream, int batch, int n1, int n2, int m1, int m2, const int* idx, const float* val,
                        const double* gamma_sum, const float* noise, const void* dir1, const void* dir2,

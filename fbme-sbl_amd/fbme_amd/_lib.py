@@ -151,6 +151,7 @@ _SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32_scratch_bytes": (c_size_t, [c_int, c_longlong]),
+    "fbme_topk32_noalloc": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
     "fbme_lowrank_setup": (c_int, [c_void_p, c_int, c_int, c_int, c_int,

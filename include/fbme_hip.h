@@ -265,6 +265,18 @@ int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const doub
  * level of merged candidates per 64x reduction); 32 <= n <= 2^26. */
 size_t fbme_topk32_scratch_bytes(int batch, long long n);
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val);
+/* fbme_topk32 without a workspace argument and without any allocation: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The candidates live in a fixed per-GPU device array of
+ * FBME_TOPK32_NOALLOC_SCRATCH_BYTES, held until the process ends; inputs
+ * needing more (fbme_topk32_scratch_bytes(batch, n) larger) return
+ * FBME_ERR_ARG.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (concurrent streams or threads) race and must be serialised by the
+ * caller.  The first call on a device resolves the array's address there
+ * (FBME_ERR_HIP if that fails). */
+#define FBME_TOPK32_NOALLOC_SCRATCH_BYTES (4u << 20)
+int fbme_topk32_noalloc(void* stream, int batch, long long n, const float* x, int* idx, float* val);
 /* Low-rank preconditioner (_make_low_rank_preconditioner,
  * solver.py:5786-5833).  The factor matrix F = columns * sqrt(gamma_k)
  * [M, 32] is separable and kept as its two phase tables: factors [batch,
