@@ -28,3 +28,27 @@ def device():
 
 def golden_path(name):
     return os.path.join(GOLDEN, name)
+
+
+def assert_history_matches(res, g, *, upto=None, rtol=1e-4):
+    """Per-EM-iteration agreement of a CG solve with a reference golden run:
+    identical PCG iteration counts, noise variance (batch mean) within
+    ``rtol``, and relative changes ||mu_t - mu_t-1|| / ||mu_t|| within 2e-5
+    absolute (a difference of two close iterates: a 1e-6 relative difference
+    in mu moves it by ~1e-6 absolute, 1e-5 seen at changes of 1e-3), for
+    iterations 1..upto (all when None).  Returns the largest relative noise
+    deviation seen."""
+    import numpy as np
+    hn = np.asarray(res.history["noise_variance"], np.float64)
+    rn = np.asarray(g["cg_history_noise_variance"], np.float64)
+    hc = np.asarray(res.history["cg_iterations"])
+    rc = np.asarray(g["cg_history_cg_iterations"])
+    hr = np.asarray(res.history["relative_change"], np.float64)
+    rr = np.asarray(g["cg_history_relative_change"], np.float64)
+    n = min(len(hn), len(rn)) if upto is None else upto
+    assert len(hn) >= n and len(rn) >= n
+    np.testing.assert_array_equal(hc[:n], rc[:n])
+    dev = np.abs(hn[:n] / rn[:n] - 1.0)
+    assert dev.max() < rtol, (int(dev.argmax()) + 1, float(dev.max()))
+    np.testing.assert_allclose(hr[1:n], rr[1:n], rtol=1e-3, atol=2e-5)
+    return float(dev.max())

@@ -6,17 +6,20 @@ Q = noise I + A Gamma A^H, the low-rank preconditioner against the dense
 Woodbury form, the top-32 selection against a stable argsort, the
 Hutchinson posterior against the oracle's with the same probes, and the
 whole solver against the reference's golden run and the oracle on a batch.
-Tolerances: float32 FFT/matvec relative L2 error 2e-5; whole-solver
-relative L2 difference 2e-2 of the estimate (the PCG stops at a relative
-residual of 3 sqrt(eps_f32) = 1e-3 by design, so float32 and float64 runs
-agree to that order) and NMSE within 0.1 dB.
+Tolerances: float32 FFT/matvec relative L2 error 2e-5.  Whole solves
+against the reference's golden runs: relative L2 1e-4 on x, identical
+iteration counts, and per EM iteration identical PCG counts and noise
+within 1e-4, up to the point where a long float32 run diverges (stated per
+test with the achieved figures).  Whole solves against the float64 oracle:
+the PCG stops at a relative residual of 3 sqrt(eps_f32) = 1e-3, which bounds
+how closely float32 and float64 runs can agree.
 """
 
 import numpy as np
 import pytest
 import torch
 
-from conftest import golden_path
+from conftest import assert_history_matches, golden_path
 from oracle import cg_oracle as co
 from oracle.fbme_oracle import F32_EPS, F32_TINY
 
@@ -107,6 +110,10 @@ def test_pcg_and_hutchinson_posterior_vs_oracle(device):
 
 
 def test_sbl_cg_matches_reference_golden(device):
+    """Default SBLConfig on 32x32 / 24x24 (auto probes, adaptive tolerance,
+    active-set switch at iteration 30): achieved relative L2 2.3e-6 on x,
+    76/76 EM iterations, every PCG count and the noise at every EM iteration
+    within 1e-6."""
     from fbme_amd import scenes
     from torch_sbl import SBLConfig, sbl_2d
     g = np.load(golden_path("general_32x32_24x24.npz"))
@@ -114,18 +121,17 @@ def test_sbl_cg_matches_reference_golden(device):
     res = sbl_2d(y, (32, 32), config=SBLConfig(mode="cg", device=str(device)))
     x = res.x.cpu().numpy().reshape(-1)
     ref = g["cg_x"].reshape(-1)
+    dev = assert_history_matches(res, g)
     print("iterations", res.iterations, int(g["cg_iterations"]), "active", res.active_set_size,
-          "rel", _rel(x, ref), flush=True)
+          "rel", _rel(x, ref), "noise history", dev, flush=True)
     assert res.mode == "cg"
-    assert abs(res.iterations - int(g["cg_iterations"])) <= 6
+    assert res.iterations == int(g["cg_iterations"])
     assert res.active_set_size == int(g["cg_active_set_size"])
-    assert _rel(x, ref) < 1e-3
+    assert _rel(x, ref) < 1e-4
     truth = g["x_true"].reshape(-1)
-    assert abs(scenes.nmse_db(x, truth) - scenes.nmse_db(ref, truth)) < 0.1
-    assert float(res.noise_variance) == pytest.approx(float(g["cg_noise_variance"]), rel=2e-2)
-    assert float(res.effective_dof) == pytest.approx(float(g["cg_effective_dof"]), rel=2e-2)
-    hist = np.asarray(res.history["cg_iterations"])
-    np.testing.assert_allclose(hist[:20], g["cg_history_cg_iterations"][:20], atol=8)
+    assert abs(scenes.nmse_db(x, truth) - scenes.nmse_db(ref, truth)) < 0.01
+    assert float(res.noise_variance) == pytest.approx(float(g["cg_noise_variance"]), rel=1e-4)
+    assert float(res.effective_dof) == pytest.approx(float(g["cg_effective_dof"]), rel=1e-4)
 
 
 def test_sbl_cg_batch_vs_oracle(device):
@@ -140,13 +146,15 @@ def test_sbl_cg_batch_vs_oracle(device):
     ref = co.sbl_cg(data, grid, ocfg)
     x = res.x.cpu().numpy().reshape(3, -1)
     xr = ref["x"].reshape(3, -1)
-    print("iterations", res.iterations, ref["iterations"], flush=True)
-    assert abs(res.iterations - ref["iterations"]) <= 4
+    print("iterations", res.iterations, ref["iterations"], "rel", [_rel(x[i], xr[i]) for i in range(3)],
+          "noise", res.noise_variance.cpu().numpy() / ref["noise_variance"] - 1, flush=True)
+    # achieved: 40/40 iterations, relative L2 1.8e-6 .. 2.6e-6, noise within 9e-7
+    assert res.iterations == ref["iterations"]
     for i in range(3):
-        assert _rel(x[i], xr[i]) < 2e-2
+        assert _rel(x[i], xr[i]) < 1e-4
         truth = batch[i].x.reshape(-1)
-        assert abs(scenes.nmse_db(x[i], truth) - scenes.nmse_db(xr[i], truth)) < 0.1
-    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=2e-2)
+        assert abs(scenes.nmse_db(x[i], truth) - scenes.nmse_db(xr[i], truth)) < 0.01
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=1e-4)
 
 
 def test_sbl_cg_is_deterministic(device):
@@ -210,12 +218,19 @@ def test_long_run_matches_reference_including_float32_divergence(device):
     noise = res.noise_variance.cpu().numpy()
     ref_x = g["cg_x"].reshape(4, -1)
     truth = g["x_true"].reshape(4, -1)
+    # the trajectories agree until scene 1 starts to diverge at EM iteration 80
+    # (achieved: identical PCG counts, noise within 7e-7 through iteration 79)
+    dev = assert_history_matches(res, g, upto=79)
+    print("history noise deviation through iteration 79:", dev, flush=True)
     for i in range(4):
         print(i, "nmse", scenes.nmse_db(x[i], truth[i]), scenes.nmse_db(ref_x[i], truth[i]),
               "noise", noise[i], g["cg_noise_variance"][i], "rel", _rel(x[i], ref_x[i]), flush=True)
+    # the stable scenes after 100 iterations: achieved relative L2 1.1e-4 ..
+    # 6.7e-4 on x, NMSE within 0.002 dB, noise within 2e-5
     for i in (0, 2, 3):
-        assert abs(scenes.nmse_db(x[i], truth[i]) - scenes.nmse_db(ref_x[i], truth[i])) < 0.2
-        assert noise[i] == pytest.approx(float(g["cg_noise_variance"][i]), rel=5e-2)
+        assert _rel(x[i], ref_x[i]) < 2e-3
+        assert abs(scenes.nmse_db(x[i], truth[i]) - scenes.nmse_db(ref_x[i], truth[i])) < 0.01
+        assert noise[i] == pytest.approx(float(g["cg_noise_variance"][i]), rel=1e-4)
     assert g["cg_noise_variance"][1] > 1e3 and noise[1] > 1e3
 
 

@@ -5,16 +5,20 @@ Snapshots share one variance per grid cell (M-SBL EM update); with shared
 noise (joint_noise_variance=True, the default) the posterior is one PCG
 system whose lanes are the snapshots and the probes
 (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774, 3039-3153).
-Tolerances as for the independent-snapshot solver: relative L2 2e-2 of the
-estimate (float32 PCG stopped at a relative residual of 1e-3 by design),
-NMSE within 0.1 dB, noise and dof within 2 %.
+Against the reference's golden runs: identical iteration and PCG counts,
+noise within 1e-4 at every EM iteration and relative L2 1e-4 on x, except
+the 200-iteration shared-noise run, whose float32 noise estimate drifts
+slowly from the float64-summed one (noise within 1e-4 through iteration
+150, 1.6e-4 at 200; x within 1.3e-4).  Against the float64 oracle the PCG
+stopping tolerance (relative residual 3 sqrt(eps_f32) = 1e-3) bounds the
+agreement.
 """
 
 import numpy as np
 import pytest
 import torch
 
-from conftest import golden_path
+from conftest import assert_history_matches, golden_path
 from oracle import cg_oracle as co
 
 pytestmark = pytest.mark.gpu
@@ -33,17 +37,22 @@ def _compare(x, ref, truth, grid):
     from fbme_amd import scenes
     xs, rs, ts = x.reshape(-1, *grid), ref.reshape(-1, *grid), truth.reshape(-1, *grid)
     for i in range(xs.shape[0]):
-        assert _rel(xs[i], rs[i]) < 2e-2, i
-        assert abs(scenes.nmse_db(xs[i], ts[i]) - scenes.nmse_db(rs[i], ts[i])) < 0.1, i
+        print("snapshot", i, "rel", _rel(xs[i], rs[i]), flush=True)
+        assert _rel(xs[i], rs[i]) < 1e-4, i
+        assert abs(scenes.nmse_db(xs[i], ts[i]) - scenes.nmse_db(rs[i], ts[i])) < 0.01, i
 
 
-@pytest.mark.parametrize("name,grid,kw", [
-    ("joint_32x32_24x24_b3", (32, 32), {}),
-    ("joint_32x32_24x24_b3_noise", (32, 32), {}),
+@pytest.mark.parametrize("name,grid,kw,upto,rel_x", [
+    # achieved: x 1.34e-4, noise within 1e-4 through iteration 152
+    ("joint_32x32_24x24_b3", (32, 32), {}, 150, 3e-4),
+    # achieved: x 1.44e-5, 200/200 iterations
+    ("joint_32x32_24x24_b3_noise", (32, 32), {}, None, 1e-4),
+    # achieved: x 1.54e-6, 30/30 iterations
     ("joint_64x64_40x40_b4", (64, 64), dict(hutchinson_probes=16, auto_probe_scaling=False,
-                                            cg_tolerance=1e-4, max_iter=30)),
+                                            cg_tolerance=1e-4, max_iter=30), None, 1e-4),
 ])
-def test_joint_sbl_2d_matches_reference_golden(device, name, grid, kw):
+def test_joint_sbl_2d_matches_reference_golden(device, name, grid, kw, upto, rel_x):
+    from fbme_amd import scenes
     from torch_sbl import sbl_2d
     print("golden", name, flush=True)
     g = np.load(golden_path(name + ".npz"))
@@ -52,10 +61,17 @@ def test_joint_sbl_2d_matches_reference_golden(device, name, grid, kw):
                  config=_cfg(device, joint_noise_variance=shared, **kw))
     assert res.mode == "cg" and res.active_set_size == 0
     assert res.iterations == int(g["cg_iterations"])
+    dev = assert_history_matches(res, g, upto=upto)
     x = res.x.cpu().numpy()
-    _compare(x, g["cg_x"].reshape(x.shape), g["x_true"].reshape(x.shape), grid)
-    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), g["cg_noise_variance"], rtol=2e-2)
-    np.testing.assert_allclose(res.effective_dof.cpu().numpy(), g["cg_effective_dof"], rtol=2e-2)
+    ref, truth = g["cg_x"].reshape(x.shape), g["x_true"].reshape(x.shape)
+    xs, rs, ts = x.reshape(-1, *grid), ref.reshape(-1, *grid), truth.reshape(-1, *grid)
+    for i in range(xs.shape[0]):
+        print(name, i, "rel", _rel(xs[i], rs[i]), "noise history", dev, flush=True)
+        assert _rel(xs[i], rs[i]) < rel_x, i
+        assert abs(scenes.nmse_db(xs[i], ts[i]) - scenes.nmse_db(rs[i], ts[i])) < 0.01, i
+    tol = 1e-4 if upto is None else 3e-4
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), g["cg_noise_variance"], rtol=tol)
+    np.testing.assert_allclose(res.effective_dof.cpu().numpy(), g["cg_effective_dof"], rtol=tol)
     if shared:
         # one system: the same noise and posterior variances for every snapshot
         assert torch.all(res.noise_variance == res.noise_variance[0])
@@ -80,8 +96,9 @@ def test_joint_vs_oracle(device, grid, samples, masked):
     ref = co.sbl_cg(y, grid, co.CGConfig(**{k_: v for k_, v in kw.items() if k_ not in ("min_iter", "tol")}),
                     joint=True, mask=mask)
     x = res.x.cpu().numpy()
+    # achieved: relative L2 1.4e-7 .. 2.9e-7 per snapshot
     _compare(x, ref["x"].reshape(x.shape), np.stack([s.x for s in batch]), grid)
-    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=2e-2)
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=1e-4)
 
 
 def test_joint_requires_two_snapshots(device):
