@@ -50,7 +50,7 @@ __global__ void k_embed(const float2* __restrict__ lags, const float* __restrict
   else if (a2 > e2 - m2) l2 = a2 - e2 + n2;
   float2 v = make_float2(0.f, 0.f);
   if (l1 >= 0 && l2 >= 0) v = lags[((long long)b * n1 + l1) * n2 + l2];
-  if (i == 0) v.x += noise[b];
+  if (i == 0 && noise) v.x += noise[b];
   kern[(long long)b * e1 * e2 + i] = v;
 }
 

@@ -36,12 +36,14 @@ struct PolishArgs {
   float hyper_shape, hyper_rate, noise_shape, noise_rate;
   float false_alarm_threshold;
   int evaluations, exchange_size, diagnostics, evidence_noise;
+  int prune_tail, reinitialize, evidence_gate;
 };
 
 // Per-scene state carried from the first to the second polish phase (Stable).
 struct PolishState {
   int best_s[KM];
   int work_s[KM];
+  int init_s[KM];   // the initial top-k support (the truncation diagnostic's S)
   double2 best_mu[KM];
   double best_sig[KM];
   double best_dd[KM];
@@ -389,7 +391,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
          const int* __restrict__ xcand_idx, PolishState* __restrict__ states, PolishResult* __restrict__ results,
          float* __restrict__ out_noise, float* __restrict__ out_dof, float* __restrict__ out_nle,
          float* __restrict__ out_trace, long long* __restrict__ out_exchanges, float* __restrict__ out_diag,
-         int* __restrict__ status, PolishArgs a) {
+         int* __restrict__ status, const float* __restrict__ gamma_all, PolishArgs a) {
   // The candidate keys are consumed before the first evaluation, so they
   // share LDS with the matrices.
   __shared__ union {
@@ -451,6 +453,9 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   int exchanges = 0;
   double best_noise = 0.0, best_nle = 0.0, work_noise = 0.0;
   double trace[TE_MAX];
+  // Diffuse tail loading of the un-pruned model (solver.py:3627-3633): the
+  // variances off the support, which no evaluation changes.
+  double tail = 0.0;
 
   if (phase == 0) {
     // Top-k of the candidate scores (solver.py:1931).
@@ -501,11 +506,18 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
       rank_sort_desc(sel_key, sel_idx, K, srt_key, srt_idx);
     }
     // Zero-tail pruning + re-initialisation from UAMP second moments
-    // (solver.py:4087-4106): gamma_S = max(|mu|^2 + v, floor).
+    // (solver.py:4087-4106): gamma_S = max(|mu|^2 + v, floor), or the UAMP
+    // gamma_S without re-initialisation.
+    double gs = 0.0;
     if (tid < K) {
       work_s[tid] = srt_idx[tid];
-      work_gam[tid] = fmax((double)key_score(srt_key[tid]), c.floor_v);
+      const int id = srt_idx[tid];
+      gs = (id >= 0 && id < n_grid) ? (double)gamma_all[s.grid_off + id] : 0.0;
+      work_gam[tid] = a.reinitialize ? fmax((double)key_score(srt_key[tid]), c.floor_v) : gs;
+      st->init_s[tid] = srt_idx[tid];
     }
+    gs = wave_sum_all(gs);   // every lane builds rows of S with nu = noise + tail
+    if (!a.prune_tail) tail = fmax(scs[SC_GSUM] - gs, 0.0);
     work_noise = scs[SC_NOISE];
     __syncthreads();
   } else {
@@ -603,9 +615,9 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   }
 
   for (;;) {
-    const double nle = evaluate(c, work_s, work_gam, work_noise, L, a.jitter_multiplier);
+    const double nle = evaluate(c, work_s, work_gam, work_noise + tail, L, a.jitter_multiplier);
     bool accept = true;
-    if (have_best) {
+    if (have_best && a.evidence_gate) {
       const double margin = (double)a.margin_factor * F32_EPS_D * (1.0 + fabs(nle) + fabs(best_nle));
       accept = isfinite(nle) && (!isfinite(best_nle) || nle <= best_nle - margin);
       if (pending && accept) exchanges += 1;
@@ -646,11 +658,12 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
       work_gam[tid] = keep * best_gam[tid] + (1.0 - keep) * ng;
       work_s[tid] = best_s[tid];
     }
+    // EM on nu = noise + tail, then the tail removed (solver.py:4342-4383)
     double upd;
     if (a.evidence_noise) {
-      upd = (energy + a.noise_rate) / fmax((double)c.m_total - dof + a.noise_shape, F32_EPS_D);
+      upd = (energy + a.noise_rate) / fmax((double)c.m_total - dof + a.noise_shape, F32_EPS_D) - tail;
     } else {
-      upd = (energy + best_noise * dof + a.noise_rate) / ((double)c.m_total + a.noise_shape);
+      upd = (energy + (best_noise + tail) * dof + a.noise_rate) / ((double)c.m_total + a.noise_shape) - tail;
     }
     upd = fmax(upd, c.floor_v);
     work_noise = keep * best_noise + (1.0 - keep) * upd;
@@ -836,7 +849,7 @@ __global__ void __launch_bounds__(FBME_BLOCK)
 k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
          const double* __restrict__ sc, const PolishResult* __restrict__ results, float2* __restrict__ x,
          float* __restrict__ pvar, float* __restrict__ prec, unsigned char* __restrict__ mask, float threshold,
-         int with_mask) {
+         int with_mask, const float* __restrict__ tail_gamma) {
   __shared__ int sidx[KM];
   __shared__ float2 smu[KM];
   __shared__ float ssig[KM], sgam[KM];
@@ -866,10 +879,17 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
     // cell n: the polished atom's mean / variance / precision or the pruned
     // tail's zero / floor / 1 / floor; the LDS bitmap marks the (at most K)
     // cells of this tile that hold an atom
+    // tail_gamma (un-pruned model): the cells off the support keep their
+    // prior variance (solver.py:3766-3772) and its precision
     auto cell = [&](int e, float2& xv, float& pv, float& pr, unsigned char& mk) {
       xv = make_float2(0.f, 0.f);
-      pv = floor_v;
-      pr = inv_floor;
+      if (tail_gamma) {
+        pv = tail_gamma[s.grid_off + wk.first + e];
+        pr = 1.0f / pv;
+      } else {
+        pv = floor_v;
+        pr = inv_floor;
+      }
       if ((hitbits[e >> 5] >> (e & 31)) & 1u) {
         const int n = wk.first + e;
         int lo = 0, hi = K - 1, hit = -1;
@@ -990,9 +1010,99 @@ __global__ void k_reduced_defaults(const float* __restrict__ gamma, float2* __re
   dd[i] = 0.f;
 }
 
+// sc[SC_GSUM] = sum of the scene's variances after the sweep (the diffuse
+// tail loading of the un-pruned polish), fixed-order FP64 reduction.
+__global__ void __launch_bounds__(256)
+k_gamma_sum(const FbmeScene* __restrict__ scenes, const float* __restrict__ gamma, double* __restrict__ sc,
+            int nscenes) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  if (b >= nscenes) return;
+  const FbmeScene s = scenes[b];
+  const long long n = (long long)s.n1 * s.n2;
+  double acc = 0.0;
+  for (long long i = threadIdx.x; i < n; i += 256) acc += (double)gamma[s.grid_off + i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc[(long long)b * SC_COUNT + SC_GSUM] = red[0];
+}
+
+// Truncation diagnostic (solver.py:3927-3958): tail[b] = gamma of plan
+// scene b with its initial top-k support zeroed (plan order, dense [N]).
+__global__ void k_trunc_tail(const FbmeScene* __restrict__ scenes, const float* __restrict__ gamma,
+                             float* __restrict__ tail, long long n, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const long long b = i / n;
+  tail[i] = gamma[scenes[b].grid_off + (i - b * n)];
+}
+
+__global__ void k_trunc_zero(const FbmeScene* __restrict__ scenes, const PolishState* __restrict__ states,
+                             float* __restrict__ tail, long long n, int nscenes) {
+  const int b = blockIdx.x;
+  if (b >= nscenes) return;
+  const int k = scenes[b].k;
+  for (int t = threadIdx.x; t < k; t += blockDim.x) {
+    const int idx = states[b].init_s[t];
+    if (idx >= 0 && idx < n) tail[(long long)b * n + idx] = 0.f;
+  }
+}
+
+// bound[orig] = max |spectrum[b]| / max(noise, tiny), one workgroup per
+// scene; the noise is the UAMP estimate the polish starts from.
+__global__ void __launch_bounds__(256)
+k_trunc_max(const FbmeScene* __restrict__ scenes, const double* __restrict__ sc, const float2* __restrict__ spec,
+            long long e, float* __restrict__ bound, int nscenes) {
+  __shared__ float red[4];
+  const int b = blockIdx.x;
+  if (b >= nscenes) return;
+  const float2* v = spec + (long long)b * e;
+  float m = 0.f;
+  for (long long i = threadIdx.x; i < e; i += blockDim.x) {
+    const float2 z = v[i];
+    m = fmaxf(m, sqrtf(z.x * z.x + z.y * z.y));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    const float noise = (float)sc[(long long)b * SC_COUNT + SC_NOISE];
+    bound[scenes[b].orig] = mx / fmaxf(noise, (float)F32_TINY_D);
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int fbme_launch_gamma_sum(const FbmeScene* scenes, int nscenes, const float* gamma, double* sc, hipStream_t stream) {
+  if (nscenes <= 0) return 0;
+  hipLaunchKernelGGL(k_gamma_sum, dim3(nscenes), dim3(256), 0, stream, scenes, gamma, sc, nscenes);
+  return (int)hipGetLastError();
+}
+
+int fbme_launch_truncation(const FbmeScene* scenes, int nscenes, long long n, long long e, const float* gamma,
+                           const void* states, const double* sc, float* tail, const void* spectrum, float* bound,
+                           int phase, hipStream_t stream) {
+  // phase 0: the tail variances; phase 1: the bound from the spectrum.
+  if (nscenes <= 0) return 0;
+  if (phase == 0) {
+    const long long total = (long long)nscenes * n;
+    hipLaunchKernelGGL(k_trunc_tail, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, stream, scenes, gamma, tail,
+                       n, total);
+    hipLaunchKernelGGL(k_trunc_zero, dim3(nscenes), dim3(64), 0, stream, scenes, (const PolishState*)states, tail, n,
+                       nscenes);
+  } else {
+    hipLaunchKernelGGL(k_trunc_max, dim3(nscenes), dim3(256), 0, stream, scenes, sc, (const float2*)spectrum, e,
+                       bound, nscenes);
+  }
+  return (int)hipGetLastError();
+}
 
 size_t fbme_polish_state_bytes(void) { return sizeof(PolishState); }
 
@@ -1025,7 +1135,7 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
                        const void* dir, const void* mh, const uint32_t* cand_key, const int* cand_idx, const uint32_t* xcand_key,
                        const int* xcand_idx, void* states, void* results, float* out_noise, float* out_dof,
                        float* out_nle, float* out_trace, long long* out_exchanges, float* out_diag, int* status,
-                       const FbmePlanOptions* o, hipStream_t stream) {
+                       const float* gamma, const FbmePlanOptions* o, hipStream_t stream) {
   if (nscenes <= 0) return 0;
   PolishArgs a;
   a.margin_factor = o->margin_factor;
@@ -1041,18 +1151,21 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
   a.exchange_size = o->exchange_size;
   a.diagnostics = o->diagnostics;
   a.evidence_noise = o->evidence_noise;
+  a.prune_tail = o->prune_tail;
+  a.reinitialize = o->reinitialize;
+  a.evidence_gate = o->evidence_gate;
   // One wavefront per scene: the K x K work is latency bound, and with a
   // single wave every barrier is cheap.
   if (mh)
     hipLaunchKernelGGL(k_polish<true>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
                        (const float2*)bp, sc, (const double2*)dir, (const double2*)mh, cand_key, cand_idx,
                        xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
-                       out_nle, out_trace, out_exchanges, out_diag, status, a);
+                       out_nle, out_trace, out_exchanges, out_diag, status, gamma, a);
   else
     hipLaunchKernelGGL(k_polish<false>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
                        (const float2*)bp, sc, (const double2*)dir, (const double2*)nullptr, cand_key, cand_idx,
                        xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
-                       out_nle, out_trace, out_exchanges, out_diag, status, a);
+                       out_nle, out_trace, out_exchanges, out_diag, status, gamma, a);
   return (int)hipGetLastError();
 }
 
@@ -1073,10 +1186,10 @@ int fbme_launch_exchange_score(const FbmeScene* scenes, const FbmeWork* work, in
 
 int fbme_launch_output(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const double* sc,
                        const void* results, void* x, float* pvar, float* prec, unsigned char* mask,
-                       float threshold, int with_mask, hipStream_t stream) {
+                       float threshold, int with_mask, const float* tail_gamma, hipStream_t stream) {
   if (nwork <= 0) return 0;
   hipLaunchKernelGGL(k_output, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork, sc,
-                     (const PolishResult*)results, (float2*)x, pvar, prec, mask, threshold, with_mask);
+                     (const PolishResult*)results, (float2*)x, pvar, prec, mask, threshold, with_mask, tail_gamma);
   return (int)hipGetLastError();
 }
 

@@ -44,11 +44,14 @@ size_t fbme_polish_result_bytes(void);
 int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, const void*, const void*,
                        const uint32_t*,
                        const int*, const uint32_t*, const int*, void*, void*, float*, float*, float*, float*,
-                       long long*, float*, int*, const FbmePlanOptions*, hipStream_t);
+                       long long*, float*, int*, const float*, const FbmePlanOptions*, hipStream_t);
+int fbme_launch_gamma_sum(const FbmeScene*, int, const float*, double*, hipStream_t);
 int fbme_launch_exchange_score(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, const void*,
                                const void*, uint32_t*, int*, int, hipStream_t);
 int fbme_launch_output(const FbmeScene*, const FbmeWork*, int, int, const double*, const void*, void*, float*,
-                       float*, unsigned char*, float, int, hipStream_t);
+                       float*, unsigned char*, float, int, const float*, hipStream_t);
+int fbme_launch_truncation(const FbmeScene*, int, long long, long long, const float*, const void*, const double*,
+                           float*, const void*, float*, int, hipStream_t);
 }
 
 namespace {
@@ -83,6 +86,10 @@ struct fbme_plan {
   // observation masks: spectra of the masked scenes and the scratch of
   // their computation; the caller's mask bytes (packed like the data)
   Region r_mh, r_mhwork, r_mbits;
+  // truncation diagnostic: tail variances [nscenes, N] and the two work
+  // buffers of the BTTB spectrum [nscenes, max(N, E1 E2)] complex
+  Region r_ttail, r_twa, r_twb;
+  int te1 = 0, te2 = 0, ttw_n2 = 0, ttw_e2 = 0;
   int masked = 0;                           // scenes with an observation mask
   std::vector<int> mh_leaders;              // caller indices owning a spectrum
   const unsigned char* mask = nullptr;
@@ -193,6 +200,10 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
   o->noise_rate = 0.0f;
   o->polish_damping = 0.0f;
   o->fft_kernel = 0;
+  o->truncation_diagnostic = 0;
+  o->prune_tail = 1;
+  o->reinitialize = 1;
+  o->evidence_gate = 1;
 }
 
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
@@ -215,6 +226,12 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   if (opt->exchange_size > 0 && opt->polish_iterations < 2)
     return fail("residual exchange requires at least two polish evaluations");
   if (opt->min_iters < 1 || opt->max_iters < 1) return fail("iteration limits must be positive");
+  if (opt->truncation_diagnostic && !opt->prune_tail)
+    return fail("the truncation diagnostic certifies the zero-tail (pruned) model only");
+  if (opt->exchange_size > 0 && (!opt->prune_tail || !opt->evidence_gate))
+    return fail("the residual exchange runs on the pruned, evidence-gated polish only");
+  if (opt->diagnostics && !opt->prune_tail)
+    return fail("the fixed-budget diagnostics run on the pruned polish only");
   fbme_plan* p = new fbme_plan();
   p->opt = *opt;
   p->nscenes = nscenes;
@@ -403,6 +420,36 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
       p->tile_work.push_back({si, (int)first, (int)std::min<long long>(FBME_XTILE, n - first), t});
     }
   }
+  // Truncation diagnostic: the BTTB embedding of the full sample block
+  // (operators.py:10-20, 100-105: first 2/3/5/7-smooth length >= 2M - 1 per
+  // axis; an observation mask does not enter it, operators.py:299-326) on
+  // the FFT passes of the general solver, for one shared shape.
+  if (opt->truncation_diagnostic) {
+    const FbmeScene& a = p->scenes[0];
+    for (const FbmeScene& s2 : p->scenes) {
+      if (s2.n1 != a.n1 || s2.n2 != a.n2 || s2.m1 != a.m1 || s2.m2 != a.m2) {
+        delete p;
+        return fail("the truncation diagnostic needs one grid and sample shape for every scene");
+      }
+    }
+    auto smooth = [](int v) {
+      for (int c = v;; ++c) {
+        int r = c;
+        for (int f : {2, 3, 5, 7})
+          while (r % f == 0) r /= f;
+        if (r == 1) return c;
+      }
+    };
+    p->te1 = smooth(2 * a.m1 - 1);
+    p->te2 = smooth(2 * a.m2 - 1);
+    if (radix_code(p->te1, 1, &e) == 0xFFFFFFFFu || radix_code(p->te2, 1, &e) == 0xFFFFFFFFu) {
+      delete p;
+      return fail("truncation diagnostic: embedding " + std::to_string(p->te1) + " x " + std::to_string(p->te2) +
+                  ": " + e);
+    }
+    p->ttw_n2 = twiddle(a.n2);
+    p->ttw_e2 = twiddle(p->te2);
+  }
   auto ept_for = [](int elems) { return elems <= 1024 ? 4 : elems <= 2048 ? 8 : 16; };
   int max_col = 1, max_row = 1;
   for (const FbmeWork& w : p->col_work) max_col = std::max(max_col, w.count * p->scenes[w.scene].n1);
@@ -475,6 +522,13 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   reg(p->r_mh, 16 * (size_t)mh_total);
   reg(p->r_mhwork, 16 * (size_t)mh_work);
   reg(p->r_mbits, 2 * (size_t)mbits_total);
+  if (opt->truncation_diagnostic) {
+    const long long n = (long long)p->scenes[0].n1 * p->scenes[0].n2;
+    const long long big = std::max<long long>(n, (long long)p->te1 * p->te2);
+    reg(p->r_ttail, 4 * (size_t)(nscenes * n));
+    reg(p->r_twa, 8 * (size_t)(nscenes * big));
+    reg(p->r_twb, 8 * (size_t)(nscenes * big));
+  }
   p->workspace_bytes = align_up(off, 256);
   return p;
 }
@@ -497,6 +551,8 @@ long long fbme_plan_query(const fbme_plan* p, int what) {
       const bool exch = p->opt.exchange_size > 0 && p->total_xcand > 1;
       if (exch) n += 2;
       n += 1;           // output
+      if (p->opt.truncation_diagnostic) n += 9;
+      if (!p->opt.prune_tail) n += 1;
       return n;
     }
     case FBME_Q_R16_COL_ITEMS: return (long long)p->col_work.size() - p->col_split;
@@ -566,6 +622,7 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   if (!io->y || !io->x || !io->posterior_variance || !io->precision || !io->noise || !io->dof || !io->nle ||
       !io->trace)
     return FBME_ERR_ARG;
+  if (p->opt.truncation_diagnostic && !io->truncation) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   char* w = p->ws;
   const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
@@ -652,18 +709,35 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   // Final consistency E-step (no prediction) + per-tile top-k candidates.
   FBME_TRY(col_adjoint(p->col_split, ncol16, 1, 0));
   FBME_TRY(row(p->row_split, nrow16, 3, ckey, cidx, io->store_uamp ? 1 : 0));
-  // Polish.
+  // Polish (the un-pruned model first sums the variances for its tail loading).
+  if (!o.prune_tail) FBME_TRY(fbme_launch_gamma_sum(scenes, p->nscenes, gamma, sc, st));
   FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 0, bp, sc, dir, mh, ckey, cidx, xkey, xidx, states, results, io->noise,
-                              io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
+                              io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, gamma, &o, st));
   const bool exch = o.exchange_size > 0 && p->total_xcand > 1;
   if (exch) {
     FBME_TRY(fbme_launch_exchange_score(scenes, tilew, ntile, ntile, bp, dir, mh, states, xkey, xidx, o.exchange_size,
                                         st));
     FBME_TRY(fbme_launch_polish(scenes, p->nscenes, 1, bp, sc, dir, mh, ckey, cidx, xkey, xidx, states, results, io->noise,
-                                io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, &o, st));
+                                io->dof, io->nle, io->trace, io->exchanges, io->diag, io->status, gamma, &o, st));
   }
   FBME_TRY(fbme_launch_output(scenes, tilew, ntile, ntile, sc, results, io->x, io->posterior_variance, io->precision,
-                              io->support_mask, o.false_alarm, (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, st));
+                              io->support_mask, o.false_alarm, (io->support_mask && o.false_alarm > 0.f) ? 1 : 0,
+                              o.prune_tail ? nullptr : gamma, st));
+  if (o.truncation_diagnostic) {
+    // max |BTTB spectrum of the tail variances| / noise (solver.py:3945-3958)
+    const FbmeScene& a = p->scenes[0];
+    const long long n = (long long)a.n1 * a.n2, e = (long long)p->te1 * p->te2;
+    float* tail = (float*)(w + p->r_ttail.off);
+    void* wa = w + p->r_twa.off;
+    void* wb = w + p->r_twb.off;
+    const float2* twb = (const float2*)tw;
+    FBME_TRY(fbme_launch_truncation(scenes, p->nscenes, n, e, gamma, states, sc, tail, nullptr, nullptr, 0, st));
+    // the spectrum lands in wb (free again after the embedding pass)
+    FBME_TRY(fbme_gamma_spectrum(st, p->nscenes, a.n1, a.n2, a.m1, a.m2, p->te1, p->te2, fbme_radix_code(a.n1), fbme_radix_code(a.n2),
+                                 fbme_radix_code(p->te1), fbme_radix_code(p->te2), tail, nullptr, wa, wb, wb,
+                                 twb + p->ttw_n2, twb + p->ttw_n2, twb + p->ttw_e2, twb + p->ttw_e2));
+    FBME_TRY(fbme_launch_truncation(scenes, p->nscenes, n, e, gamma, states, sc, tail, wb, io->truncation, 1, st));
+  }
 #undef FBME_TRY
   return 0;
 }
@@ -723,7 +797,8 @@ int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* s
     case FBME_STAGE_OUTPUT:
       return fbme_launch_output(scenes, tilew, ntile, ntile, sc, w + p->r_result.off, io->x, io->posterior_variance,
                                 io->precision, io->support_mask, o.false_alarm,
-                                (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, st) ? FBME_ERR_HIP : 0;
+                                (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, o.prune_tail ? nullptr : gamma,
+                                st) ? FBME_ERR_HIP : 0;
     default:
       return FBME_ERR_ARG;
   }

@@ -58,6 +58,7 @@ enum FbmeScalar {
   SC_DOF = 4,       // effective dof (clamped)
   SC_BPE = 5,       // sum |A^H y|^2
   SC_YE = 6,        // ||y||^2
+  SC_GSUM = 7,      // sum gamma after the sweep (diffuse tail loading only)
   SC_COUNT = 8
 };
 

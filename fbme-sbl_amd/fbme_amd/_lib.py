@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 # return codes (include/fbme_hip.h)
 ERR_ARG = 1
 ERR_HIP = 2
@@ -51,6 +51,8 @@ class FbmePlanOptions(Structure):
         ("noise_shape", c_float), ("noise_rate", c_float),
         ("polish_damping", c_float),
         ("fft_kernel", c_int),
+        ("truncation_diagnostic", c_int), ("prune_tail", c_int),
+        ("reinitialize", c_int), ("evidence_gate", c_int),
     ]
 
 
@@ -60,7 +62,7 @@ class FbmeIO(Structure):
         ("precision", c_void_p), ("support_mask", c_void_p),
         ("noise", c_void_p), ("dof", c_void_p), ("nle", c_void_p),
         ("trace", c_void_p), ("exchanges", c_void_p), ("diag", c_void_p),
-        ("status", c_void_p), ("store_uamp", c_int),
+        ("status", c_void_p), ("store_uamp", c_int), ("truncation", c_void_p),
     ]
 
 

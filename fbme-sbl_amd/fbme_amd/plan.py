@@ -32,6 +32,7 @@ class PlanOutputs:
     exchanges: torch.Tensor            # [nscenes] int64
     diag: torch.Tensor                 # [nscenes, 4] float32
     status: torch.Tensor               # [nscenes] int32
+    truncation: torch.Tensor           # [nscenes] float32 (truncation_diagnostic)
 
 
 def options_from_fields(stable: bool, **fields) -> _lib.FbmePlanOptions:
@@ -141,6 +142,7 @@ class NativePlan:
                 exchanges=torch.zeros(b, dtype=torch.int64, device=device),
                 diag=torch.zeros((b, 4), dtype=torch.float32, device=device),
                 status=torch.zeros(b, dtype=torch.int32, device=device),
+                truncation=torch.zeros(b, dtype=torch.float32, device=device),
             )
         self.store_uamp = False
 
@@ -155,7 +157,8 @@ class NativePlan:
             noise=o.noise.data_ptr(), dof=o.dof.data_ptr(),
             nle=o.nle.data_ptr(), trace=o.trace.data_ptr(),
             exchanges=o.exchanges.data_ptr(), diag=o.diag.data_ptr(),
-            status=o.status.data_ptr(), store_uamp=1 if self.store_uamp else 0)
+            status=o.status.data_ptr(), store_uamp=1 if self.store_uamp else 0,
+            truncation=o.truncation.data_ptr())
 
     def run_stage(self, stage: int, y: torch.Tensor) -> None:
         """Enqueue one pass on the current plan state (kernel timing)."""

@@ -32,13 +32,11 @@ def _unsupported(cfg: SBLConfig) -> list[str]:
     bad = []
     checks = {
         "uamp_polish_size=0 (no active-set polish)": cfg.uamp_polish_size == 0,
-        "uamp_polish_prune_tail=False": not cfg.uamp_polish_prune_tail,
-        "uamp_polish_reinitialize=False": not cfg.uamp_polish_reinitialize,
-        "uamp_polish_evidence_gate=False": not cfg.uamp_polish_evidence_gate,
-        "uamp_polish_compute_truncation_diagnostic=True":
-            cfg.uamp_polish_compute_truncation_diagnostic,
-        "uamp_polish_max_truncation_delta": (
-            cfg.uamp_polish_max_truncation_delta is not None),
+        "uamp_polish_residual_exchange_size>0 without prune_tail and evidence_gate": (
+            cfg.uamp_polish_residual_exchange_size > 0
+            and not (cfg.uamp_polish_prune_tail and cfg.uamp_polish_evidence_gate)),
+        "uamp_polish_fixed_budget_diagnostics without prune_tail": (
+            cfg.uamp_polish_fixed_budget_diagnostics and not cfg.uamp_polish_prune_tail),
         "uamp_polish_global_certificate": cfg.uamp_polish_global_certificate,
         "uamp_polish_support_repairs": cfg.uamp_polish_support_repairs != 0,
         "uamp_polish_support_growth_steps": (
@@ -111,6 +109,13 @@ def plan_options(cfg: SBLConfig, grid_shape, sample_shape, observed: int | None 
     opt.noise_shape = float(cfg.noise_shape)
     opt.noise_rate = float(cfg.noise_rate)
     opt.polish_damping = float(cfg.uamp_polish_damping)
+    opt.prune_tail = int(cfg.uamp_polish_prune_tail)
+    opt.reinitialize = int(cfg.uamp_polish_reinitialize)
+    opt.evidence_gate = int(cfg.uamp_polish_evidence_gate)
+    # solver.py:4034-4040
+    opt.truncation_diagnostic = int(cfg.uamp_polish_prune_tail and (
+        cfg.uamp_polish_compute_truncation_diagnostic
+        or cfg.uamp_polish_max_truncation_delta is not None))
     return opt
 
 
@@ -168,6 +173,10 @@ class ResultViews:
         self.nle = out.nle.view(bshape)
         self.trace = out.trace
         self.hard = torch.zeros(b, dtype=torch.bool, device=dev).view(bshape)
+        # solver.py:4041-4062: the certificate, and hard pruning accepted
+        # when it is within uamp_polish_max_truncation_delta
+        self.truncation = (out.truncation.view(bshape)
+                           if plan.options.truncation_diagnostic else None)
         self.selected = torch.ones(b, dtype=torch.bool, device=dev).view(bshape)
         self.penalty = torch.zeros(b, dtype=torch.float32, device=dev).view(bshape)
         self.exchanges = (out.exchanges.view(bshape)
@@ -193,6 +202,9 @@ class ResultViews:
 
     def refresh(self) -> None:
         """Device-side derived fields (enqueued on the current stream)."""
+        delta = self.cfg.uamp_polish_max_truncation_delta
+        if self.truncation is not None and delta is not None:
+            torch.le(self.truncation, float(delta), out=self.hard)
         if self.monotone is not None:
             self.monotone.copy_(self.plan.out.diag[:, 3].view(self.batch_shape) > 0.5)
 
@@ -221,6 +233,7 @@ class ResultViews:
             active_set_size=self.k,
             negative_log_evidence=self.nle,
             evidence_trace=self.trace,
+            truncation_delta_bound=self.truncation,
             hard_pruning_accepted=self.hard,
             truncation_certified=self.hard,
             support_repairs_accepted=self.exchanges,

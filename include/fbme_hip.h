@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 5
+#define FBME_ABI_VERSION 6
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -58,6 +58,19 @@ typedef struct FbmePlanOptions {
   float polish_damping;    /* 0 damping of the polish EM updates           */
   int fft_kernel;          /* 0 auto, 1 LDS Stockham passes, 2 register
                               16 x 16 passes (requires N1 or N2 = 256)     */
+  int truncation_diagnostic; /* 0: off; 1: FbmeIO.truncation gets the
+                              hard-pruning certificate of
+                              _active_truncation_delta_bound
+                              (solver.py:3927-3958, 4034-4050); needs one
+                              grid and sample shape for every scene        */
+  int prune_tail;          /* 1 zero-tail reduced model (locked plans); 0
+                              diffuse tail loading nu = noise + sum of the
+                              tail variances (solver.py:4087-4093,
+                              3627-3633, 4342-4352)                        */
+  int reinitialize;        /* 1 gamma_S from the UAMP second moments
+                              (solver.py:4094-4106); 0 keep the UAMP gamma */
+  int evidence_gate;       /* 1 keep the best evidence (solver.py:4168-4298);
+                              0 take every evaluation                      */
 } FbmePlanOptions;
 
 /* Outputs of one plan replay; per-scene arrays are in the caller's scene
@@ -78,6 +91,8 @@ typedef struct FbmeIO {
                                    objective decrease, monotone flag, optional */
   int* status;                  /* [nscenes] non-zero on Cholesky breakdown     */
   int store_uamp;               /* keep the final UAMP E-step in the workspace  */
+  float* truncation;            /* [nscenes] truncation_delta_bound, required
+                                   when options.truncation_diagnostic is set   */
 } FbmeIO;
 
 typedef struct fbme_plan fbme_plan;

@@ -185,6 +185,38 @@ class PlanConfig:
     fixed_budget_diagnostics: bool = False
     false_alarm_budget: float | None = 0.5
     jitter_multiplier: float = 8.0
+    prune_tail: bool = True
+    reinitialize: bool = True
+    evidence_gate: bool = True
+    truncation_diagnostic: bool = False
+
+    @staticmethod
+    def _adaptive(damping, n, m, max_damping=0.8):
+        """uamp_adaptive_damping (solver.py:1180-1186)."""
+        return max(damping, min(max_damping, 0.02 * n / m))
+
+    @classmethod
+    def fast_hybrid(cls, n: int, m: int, **over) -> "PlanConfig":
+        """SBLConfig.fast_hybrid_uamp (solver.py:198-227) over the SBLConfig
+        defaults (solver.py:34-104): the default of make_cuda_graph_sbl_plan."""
+        kw = dict(uamp_damping=cls._adaptive(0.1, n, m), hyper_shape=1e-8, hyper_rate=1e-8,
+                  noise_shape=1e-8, noise_rate=1e-8, min_iter=1, uamp_min_iter=3,
+                  uamp_iteration_offset=1.0, uamp_iteration_scale=1.0, polish_iterations=2,
+                  margin_factor=0.0, false_alarm_budget=None, truncation_diagnostic=True)
+        kw.update(over)
+        return cls(**kw)
+
+    @classmethod
+    def fast_uamp(cls, n: int, m: int, **over) -> "PlanConfig":
+        """SBLConfig.fast_uamp (solver.py:178-195): one reduced evaluation
+        with diffuse tail loading, no pruning, re-initialisation or gate."""
+        kw = dict(uamp_damping=cls._adaptive(0.1, n, m), hyper_shape=1e-8, hyper_rate=1e-8,
+                  noise_shape=1e-8, noise_rate=1e-8, min_iter=3, uamp_min_iter=10,
+                  uamp_iteration_offset=10.0, uamp_iteration_scale=1.1, polish_iterations=1,
+                  margin_factor=0.0, false_alarm_budget=None, prune_tail=False,
+                  reinitialize=False, evidence_gate=False)
+        kw.update(over)
+        return cls(**kw)
 
     @classmethod
     def fast(cls, polish_size: int = 32) -> "PlanConfig":
@@ -397,17 +429,31 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
     score = np.abs(sweep["mean"]) ** 2 + sweep["variance"]
     support = _topk_desc(score, k)
     rows = np.arange(b)[:, None]
-    working = np.broadcast_to(floor[:, None], (b, n)).copy()
-    working[rows, support] = np.maximum(score[rows, support], floor[:, None])
+    # solver.py:4087-4106: zero-tail pruning to the floor, re-initialisation
+    # of gamma_S from the UAMP second moments
+    if cfg.prune_tail:
+        working = np.broadcast_to(floor[:, None], (b, n)).copy()
+        working[rows, support] = sweep["gamma"][rows, support]
+    else:
+        working = sweep["gamma"].copy()
+    if cfg.reinitialize:
+        working[rows, support] = np.maximum(score[rows, support], floor[:, None])
     bp = sweep["backprojection"]
     data_energy = np.sum(np.abs(y) ** 2, axis=1)
     noise = sweep["noise"].copy()
+
+    def tail_of(prior, sup):
+        """Diffuse tail loading sum(gamma) - sum(gamma_S) (solver.py:3627-3633,
+        4342-4350); zero for the pruned model."""
+        if cfg.prune_tail:
+            return np.zeros(b)
+        return np.maximum(prior.sum(1) - prior[rows, sup].sum(1), 0.0)
 
     def evaluate(prior, nu, sup):
         gram = gram_fn(sup, grid, samples)
         cdat = bp[rows, sup]          # C^H y = (A^H y)[S]
         mu_s, sig_s, dd_s, nle = reduced_posterior(
-            gram, cdat, data_energy, prior[rows, sup], nu, m, eps=eps,
+            gram, cdat, data_energy, prior[rows, sup], nu + tail_of(prior, sup), m, eps=eps,
             tiny=tiny, jitter_multiplier=cfg.jitter_multiplier)
         mean = np.zeros((b, n), np.complex128)
         mean[rows, sup] = mu_s
@@ -425,7 +471,7 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
                          else None)
     for pit in range(cfg.polish_iterations):
         cand = evaluate(working, noise, support)
-        if best is None:
+        if best is None or not cfg.evidence_gate:
             best = cand
         else:
             margin = cfg.margin_factor * eps * (
@@ -465,12 +511,14 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
                   + np.real(np.einsum("bk,bkl,bl->b", np.conj(mu_s), gram, mu_s)))
         energy = np.maximum(energy, 0.0)
         dof = np.clip(1.0 - sig_s / np.maximum(gamma_s, tiny), 0.0, 1.0).sum(1)
+        # EM on nu = noise + tail, then the tail removed (solver.py:4342-4383)
+        inactive = tail_of(base_prior, support)
         if cfg.noise_update == "em":
-            upd_noise = (energy + base_noise * dof + cfg.noise_rate) / (
-                m + cfg.noise_shape)
+            upd_noise = (energy + (base_noise + inactive) * dof + cfg.noise_rate) / (
+                m + cfg.noise_shape) - inactive
         else:
             upd_noise = (energy + cfg.noise_rate) / np.maximum(
-                m - dof + cfg.noise_shape, eps)
+                m - dof + cfg.noise_shape, eps) - inactive
         upd_noise = np.maximum(upd_noise, floor)
         noise = keep * base_noise + (1.0 - keep) * upd_noise
         working = base_prior.copy()
@@ -515,7 +563,44 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
         exchanges_accepted=exchange_accepted, initial_support=_topk_desc(
             score, k),
     )
+    if cfg.truncation_diagnostic and cfg.prune_tail:
+        out["truncation_delta_bound"] = truncation_delta_bound(
+            sweep["gamma"], sweep["noise"], out["initial_support"], grid, samples, tiny)
     return out
+
+
+def smooth_length(minimum: int) -> int:
+    """First 2/3/5/7-smooth length >= minimum (operators.py:10-20)."""
+    c = int(minimum)
+    while True:
+        r = c
+        for f in (2, 3, 5, 7):
+            while r % f == 0:
+                r //= f
+        if r == 1:
+            return c
+        c += 1
+
+
+def truncation_delta_bound(gamma, noise, support, grid, samples, tiny=F32_TINY):
+    """Hard-pruning certificate (solver.py:3927-3958): the largest eigenvalue
+    magnitude of the circulant embedding of the tail covariance
+    A Gamma_tail A^H (gamma with the support zeroed), over the noise.  The
+    embedding is E1 x E2 = smooth(2M - 1) per axis with the lag d at
+    position d mod E (operators.py:100-130, 299-326)."""
+    b = gamma.shape[0]
+    n1, n2 = grid
+    m1, m2 = samples
+    tail = np.array(gamma, np.float64).copy()
+    tail[np.arange(b)[:, None], support] = 0.0
+    lags = np.fft.fft2(tail.reshape(b, n1, n2), axes=(1, 2))
+    e1, e2 = smooth_length(2 * m1 - 1), smooth_length(2 * m2 - 1)
+    d1 = np.arange(-(m1 - 1), m1)
+    d2 = np.arange(-(m2 - 1), m2)
+    kern = np.zeros((b, e1, e2), np.complex128)
+    kern[:, (d1 % e1)[:, None], (d2 % e2)[None, :]] = lags[:, (d1 % n1)[:, None], (d2 % n2)[None, :]]
+    spec = np.fft.fft2(kern, axes=(1, 2))
+    return np.abs(spec).reshape(b, -1).max(1) / np.maximum(noise, tiny)
 
 
 def _gram_apply(mu_s, support, grid, samples, mask=None):
@@ -585,7 +670,7 @@ def project_relevance_dof(gamma, data_diag, m, eps):
 
 def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
               polish_size: int = 32, *, eps=F32_EPS, tiny=F32_TINY,
-              mask=None) -> dict:
+              mask=None, config: PlanConfig | None = None) -> dict:
     """End-to-end Fast/Stable plan on [B, M1, M2] (or [M1, M2]) data,
     optionally on the samples an observation mask [M1, M2] keeps.
 
@@ -605,8 +690,8 @@ def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
             raise ValueError("observation_mask must match the samples and keep one")
     y = restrict(data.reshape(b, -1).astype(np.complex128), mask)
     m = y.shape[1]
-    cfg = (PlanConfig.fast if profile == "fast" else PlanConfig.stable)(
-        polish_size)
+    cfg = config if config is not None else (
+        PlanConfig.fast if profile == "fast" else PlanConfig.stable)(polish_size)
     sweep = uamp_sweep(y, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
     res = polish(y, sweep, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
     out = dict(
@@ -623,6 +708,8 @@ def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
         floor=sweep["floor"],
         support_exchanges_accepted=res["exchanges_accepted"],
     )
+    if "truncation_delta_bound" in res:
+        out["truncation_delta_bound"] = res["truncation_delta_bound"]
     if cfg.fixed_budget_diagnostics:
         out.update({f"fixed_budget_{k}": v for k, v in fixed_budget_residuals(
             y, res, sweep["floor"], grid, samples, cfg, eps=eps,

@@ -377,11 +377,70 @@ def headline(reference_root: str) -> None:
         print("wrote", name, flush=True)
 
 
+GENERIC_CASES = [
+    # name, generator, seed, grid, samples, k, snr_db, preset, overrides
+    ("generic_hybrid_64x64_40x40", "uniform", 1060, (64, 64), (40, 40), 20, 15.0, "fast_hybrid_uamp", {}),
+    ("generic_hybrid_96x80_60x48", "ellipse", 1061, (96, 80), (60, 48), 24, 10.0, "fast_hybrid_uamp",
+     dict(uamp_polish_max_truncation_delta=50.0)),
+    ("generic_hybrid_256x256_160x160", "ellipse", 1000, (256, 256), (160, 160), 300, 10.0, "fast_hybrid_uamp",
+     dict(uamp_polish_max_truncation_delta=1e9)),
+    ("generic_uamp_64x64_40x40", "uniform", 1060, (64, 64), (40, 40), 20, 15.0, "fast_uamp", {}),
+    ("generic_uamp_96x80_60x48", "ellipse", 1061, (96, 80), (60, 48), 24, 10.0, "fast_uamp", {}),
+    ("generic_uamp_256x256_160x160", "ellipse", 1000, (256, 256), (160, 160), 300, 10.0, "fast_uamp", {}),
+]
+
+
+def generic(reference_root: str) -> None:
+    """``fast_sbl`` under the public presets the generic graph plan takes:
+    ``SBLConfig.fast_hybrid_uamp`` (the default of make_cuda_graph_sbl_plan,
+    with the truncation diagnostic on) and ``SBLConfig.fast_uamp`` (one
+    reduced evaluation with diffuse tail loading: no pruning,
+    re-initialisation or evidence gate).  The inputs are regenerated from the
+    stored generator arguments (SHA-256 of y kept); x is exactly zero off the
+    polished atoms, so it is stored sparsely, and the posterior variance /
+    precision are stored on the polished atoms plus their grid sums."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, fast_sbl
+
+    torch.set_num_threads(8)
+    for name, gen, seed, grid, samples, k, snr, preset, over in GENERIC_CASES:
+        fn = dict(uniform=scenes.uniform_scene, ellipse=scenes.ellipse_scene)[gen]
+        scene = fn(seed, grid, samples, k=k, snr_db=snr)
+        cfg = getattr(SBLConfig, preset)(device="cpu", uamp_backend="torch", **over)
+        res = fast_sbl(torch.from_numpy(scene.y), grid, config=cfg)
+        x = _np(res.x).reshape(-1)
+        nz = np.flatnonzero(x)
+        var = _np(res.posterior_variance).reshape(-1)
+        prec = _np(res.precision).reshape(-1)
+        arrays = dict(generator=np.array(gen), seed=np.array(seed), grid=np.array(grid),
+                      samples=np.array(samples), k=np.array(k), snr_db=np.array(snr),
+                      preset=np.array(preset), y_sha256=np.array(y_digest(scene.y)),
+                      x_index=nz.astype(np.int64), x_value=x[nz], variance_on_support=var[nz],
+                      variance_sum=np.array(float(np.sum(var, dtype=np.float64))),
+                      precision_on_support=prec[nz],
+                      precision_sum=np.array(float(np.sum(prec, dtype=np.float64))),
+                      iterations=np.array(res.iterations), active_set_size=np.array(res.active_set_size))
+        for key, value in over.items():
+            arrays["override_" + key] = np.array(value)
+        for field in ("noise_variance", "effective_dof", "negative_log_evidence", "evidence_trace",
+                      "truncation_delta_bound", "hard_pruning_accepted", "truncation_certified"):
+            value = getattr(res, field, None)
+            if value is not None:
+                arrays[field] = _np(value)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, {f: arrays.get(f) for f in ("truncation_delta_bound", "hard_pruning_accepted")},
+              "support", nz.size, flush=True)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "headline":
+    if which == "generic":
+        generic(root)
+    elif which == "headline":
         headline(root)
+        generic(root)
     elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
@@ -399,3 +458,4 @@ if __name__ == "__main__":
         joint_cg(root)
         offgrid(root)
         headline(root)
+        generic(root)

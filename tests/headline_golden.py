@@ -87,3 +87,57 @@ def compare(g, profile, out, *, trace_rtol=1e-5, trace_atol=1e-2):
         assert abs(got - want) < max(1e-4, 1e-3 * abs(want))
         assert bool(out["fixed_budget_monotone"]) == bool(g[p + "fixed_budget_monotone"])
     return rel
+
+
+# Generic-preset fixtures (tests/golden/make_golden.py, ``generic``):
+# SBLConfig.fast_hybrid_uamp (make_cuda_graph_sbl_plan's default, with the
+# truncation diagnostic) and SBLConfig.fast_uamp.
+GENERIC_HYBRID = ["generic_hybrid_64x64_40x40", "generic_hybrid_96x80_60x48",
+                  "generic_hybrid_256x256_160x160"]
+GENERIC_UAMP = ["generic_uamp_64x64_40x40", "generic_uamp_96x80_60x48",
+                "generic_uamp_256x256_160x160"]
+
+
+def load_generic(name):
+    g = np.load(os.path.join(HERE, "golden", name + ".npz"))
+    grid = tuple(int(v) for v in g["grid"])
+    samples = tuple(int(v) for v in g["samples"])
+    fn = dict(uniform=scenes.uniform_scene, ellipse=scenes.ellipse_scene)[str(g["generator"])]
+    scene = fn(int(g["seed"]), grid, samples, k=int(g["k"]), snr_db=float(g["snr_db"]))
+    digest = hashlib.sha256(np.ascontiguousarray(scene.y, np.complex64).tobytes()).hexdigest()
+    if digest != str(g["y_sha256"]):
+        raise AssertionError(f"{name}: regenerated scene differs from the fixture's input")
+    overrides = {k[len("override_"):]: g[k].item() for k in g.files if k.startswith("override_")}
+    return g, scene, overrides
+
+
+def compare_generic(g, out):
+    """x to 1e-4 relative L2 with the identical polished support; posterior
+    variance and precision on the support and summed over the grid; noise,
+    dof, evidence; the truncation certificate (a maximum over the embedding
+    spectrum, float32 in the reference) to 1e-4 relative."""
+    n = int(np.prod(g["grid"]))
+    x = np.asarray(out["x"]).reshape(-1)
+    xr = np.zeros(n, np.complex128)
+    xr[g["x_index"]] = g["x_value"]
+    rel = float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+    assert rel < 1e-4, f"relative L2 {rel:.3e}"
+    assert np.array_equal(np.flatnonzero(x), np.sort(g["x_index"]))
+    idx = g["x_index"]
+    for field in ("variance", "precision"):
+        v = np.asarray(out["posterior_variance" if field == "variance" else "precision"]).reshape(-1)
+        ref = g[field + "_on_support"]
+        assert float(np.linalg.norm(v[idx] - ref) / np.linalg.norm(ref)) < 1e-4, field
+        assert abs(float(np.sum(v, dtype=np.float64)) / float(g[field + "_sum"]) - 1) < 1e-4, field
+    assert abs(float(out["noise_variance"]) / float(g["noise_variance"]) - 1) < 1e-4
+    assert abs(float(out["effective_dof"]) - float(g["effective_dof"])) < 1e-2
+    np.testing.assert_allclose(np.asarray(out["evidence_trace"]).reshape(-1),
+                               g["evidence_trace"].reshape(-1), rtol=2e-5, atol=0.1)
+    assert int(out["iterations"]) == int(g["iterations"])
+    if "truncation_delta_bound" in g.files:
+        got = float(out["truncation_delta_bound"])
+        want = float(g["truncation_delta_bound"])
+        assert abs(got / want - 1) < 1e-4, (got, want)
+        if "hard_pruning_accepted" in out:
+            assert bool(out["hard_pruning_accepted"]) == bool(g["hard_pruning_accepted"])
+    return rel

@@ -105,3 +105,49 @@ def test_oracle_matches_reference_at_headline_sizes(case, profile):
     out = fo.fbme_plan(scene.y, scene.grid, profile)
     rel = headline_golden.compare(g, profile, out, trace_rtol=2e-4, trace_atol=0.1)
     print(case, profile, "oracle vs reference", rel)
+
+
+def _generic_config(g, scene, overrides):
+    n = int(np.prod(scene.grid))
+    m = int(np.prod(scene.samples))
+    preset = str(g["preset"])
+    return (fo.PlanConfig.fast_hybrid if preset == "fast_hybrid_uamp" else fo.PlanConfig.fast_uamp)(n, m)
+
+
+@pytest.mark.parametrize("case", headline_golden.GENERIC_HYBRID + headline_golden.GENERIC_UAMP)
+def test_oracle_matches_reference_generic_presets(case):
+    """SBLConfig.fast_hybrid_uamp (make_cuda_graph_sbl_plan's default: four
+    UAMP steps at 256x256 / 160x160, evidence margin 0, hyperpriors 1e-8,
+    truncation diagnostic) and SBLConfig.fast_uamp (13 UAMP steps at
+    256x256, one reduced evaluation with diffuse tail loading and no
+    pruning / re-initialisation / gate) against the reference."""
+    g, scene, overrides = headline_golden.load_generic(case)
+    cfg = _generic_config(g, scene, overrides)
+    out = fo.fbme_plan(scene.y, scene.grid, config=cfg)
+    delta = overrides.get("uamp_polish_max_truncation_delta")
+    if delta is not None:
+        out["hard_pruning_accepted"] = out["truncation_delta_bound"] <= delta
+    rel = headline_golden.compare_generic(g, out)
+    print(case, "oracle vs reference", rel)
+
+
+def test_truncation_bound_is_the_dense_tail_covariance_norm():
+    """The certificate's numerator equals the largest eigenvalue magnitude
+    of the circulant embedding, which bounds the spectral norm of the dense
+    tail covariance A Gamma_tail A^H (a principal submatrix of it): checked
+    against the dense matrix on a small grid, and the embedding against its
+    definition (a zero tail gives a zero bound)."""
+    grid, samples = (12, 10), (7, 6)
+    rng = np.random.default_rng(3)
+    gamma = rng.exponential(size=(1, 120))
+    sup = np.array([[4, 50, 77]])
+    noise = np.array([0.5])
+    bound = fo.truncation_delta_bound(gamma, noise, sup, grid, samples)[0]
+    a = fo.dense_prefix_matrix(grid, samples)
+    tail = gamma[0].copy()
+    tail[sup[0]] = 0.0
+    cov = (a * tail) @ a.conj().T
+    norm = np.linalg.norm(cov, 2)
+    assert norm <= bound * noise[0] * (1 + 1e-12)
+    assert bound * noise[0] <= 2.0 * norm          # the embedding is not loose here
+    assert fo.truncation_delta_bound(np.zeros((1, 120)), noise, sup, grid, samples)[0] == 0.0
