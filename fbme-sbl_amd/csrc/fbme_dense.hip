@@ -623,6 +623,167 @@ DenseLayout dense_layout(int batch, int n1, int n2, int m1, int m2, int m) {
   return l;
 }
 
+// ------------------------------------ reduced posterior for K up to 256
+// The general solver's active-set posterior (_active_posterior with the
+// diffuse tail loading nu = noise + max(sum gamma - sum gamma_S, 0),
+// solver.py:3591-3779, 3897-3924) for supports larger than one wavefront's
+// 32 atoms: S = diag(1 / gamma_S) + G / nu (Gram G = C_S^H C_S from the
+// Dirichlet tables or an observation mask's spectrum), equilibrated by
+// D = sqrt(diag S), loaded with jitter * eps_f32 * mean|diag| (the GPU
+// branch of _stable_cholesky, solver.py:2898-2920), factored and inverted on
+// the FP64 32 x 32 tile kernels above (padded to a multiple of 32 with an
+// identity block).
+constexpr int RP_KMAX = 256;
+constexpr double F32_EPS_DN = 1.1920928955078125e-07;
+constexpr double F32_TINY_DN = 1.17549435082228750797e-38;
+
+__device__ __forceinline__ double2 rp_lag(const double2* d1, const double2* d2, const double2* mh, int n1, int n2,
+                                          int a, int b) {
+  int l1 = a / n2 - b / n2, l2 = a % n2 - b % n2;
+  l1 += (l1 < 0) ? n1 : 0;
+  l2 += (l2 < 0) ? n2 : 0;
+  if (mh) return mh[(long long)l1 * n2 + l2];
+  return zm(d1[l1], d2[l2]);
+}
+
+// Per scene: nu, the equilibration D, the equilibrated right-hand side
+// (C^H y / nu) / D with C^H y = (A^H y)[S], and the loading.
+__global__ void __launch_bounds__(RP_KMAX)
+k_rp_prep(int k, int n1, int n2, const int* __restrict__ support, const float* __restrict__ gamma,
+          const double* __restrict__ gamma_sum, const float* __restrict__ noise, const float2* __restrict__ bp,
+          const double2* __restrict__ d1, const double2* __restrict__ d2, const double2* __restrict__ mh,
+          double jitter_multiplier, double* __restrict__ dsc, double2* __restrict__ rhs, double* __restrict__ rsc) {
+  __shared__ double red[RP_KMAX];
+  const int b = blockIdx.x, r = threadIdx.x;
+  const long long n = (long long)n1 * n2;
+  const int id = r < k ? support[(long long)b * k + r] : 0;
+  const double g = r < k ? (double)gamma[b * n + id] : 0.0;
+  red[r] = g;
+  __syncthreads();
+  for (int w = RP_KMAX / 2; w > 0; w >>= 1) {
+    if (r < w) red[r] += red[r + w];
+    __syncthreads();
+  }
+  const double nu = (double)noise[b] + fmax(gamma_sum[b] - red[0], 0.0);
+  __syncthreads();
+  double diag = 0.0;
+  if (r < k) {
+    const double srr = rp_lag(d1, d2, mh, n1, n2, id, id).x / nu + 1.0 / g;
+    const double d = sqrt(fmax(srr, F32_TINY_DN));
+    dsc[(long long)b * RP_KMAX + r] = d;
+    diag = fabs(srr / (d * d));
+    const float2 v = bp[b * n + id];
+    rhs[(long long)b * RP_KMAX + r] = make_double2((double)v.x / nu / d, (double)v.y / nu / d);
+  }
+  red[r] = diag;
+  __syncthreads();
+  for (int w = RP_KMAX / 2; w > 0; w >>= 1) {
+    if (r < w) red[r] += red[r + w];
+    __syncthreads();
+  }
+  if (r == 0) {
+    rsc[2 * b] = nu;
+    rsc[2 * b + 1] = jitter_multiplier * F32_EPS_DN * fmax(red[0] / k, F32_TINY_DN);
+  }
+}
+
+// Equilibrated, Hermitian-symmetrised and loaded system, padded to mp.
+__global__ void k_rp_build(int k, int mp, int n1, int n2, const int* __restrict__ support,
+                           const float* __restrict__ gamma, const double2* __restrict__ d1,
+                           const double2* __restrict__ d2, const double2* __restrict__ mh,
+                           const double* __restrict__ dsc, const double* __restrict__ rsc, double2* __restrict__ qs) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long long)mp * mp) return;
+  const int r = (int)(e / mp), q = (int)(e % mp);
+  double2 v = make_double2(r == q ? 1.0 : 0.0, 0.0);
+  if (r < k && q < k) {
+    const long long n = (long long)n1 * n2;
+    const int ir = support[(long long)b * k + r], iq = support[(long long)b * k + q];
+    const double inu = 1.0 / rsc[2 * b];
+    const double2 grq = rp_lag(d1, d2, mh, n1, n2, ir, iq), gqr = rp_lag(d1, d2, mh, n1, n2, iq, ir);
+    // 0.5 (S + S^H), S = G / nu + diag(1 / gamma_S)
+    v = make_double2(0.5 * (grq.x + gqr.x) * inu, 0.5 * (grq.y - gqr.y) * inu);
+    const double sc = 1.0 / (dsc[(long long)b * RP_KMAX + r] * dsc[(long long)b * RP_KMAX + q]);
+    if (r == q) v = make_double2(v.x + 1.0 / (double)gamma[b * n + ir], 0.0);
+    v = make_double2(v.x * sc, v.y * sc);
+    if (r == q) v.x += rsc[2 * b + 1];
+  }
+  qs[(long long)b * mp * mp + e] = v;
+}
+
+// mu = W^H (W rhs) / D and Sigma_jj = ||W[:, j]||^2 / D_j^2 from W = L^-1,
+// data diagonal (gamma - Sigma) / gamma^2, scattered to the support.
+__global__ void __launch_bounds__(RP_KMAX)
+k_rp_finish(int k, int mp, long long n, const int* __restrict__ support, const float* __restrict__ gamma,
+            const double* __restrict__ dsc, const double2* __restrict__ rhs, const double2* __restrict__ ws,
+            float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd) {
+  __shared__ double2 z[RP_KMAX];
+  const int b = blockIdx.x, r = threadIdx.x;
+  const double2* w = ws + (long long)b * mp * mp;
+  const double2* rh = rhs + (long long)b * RP_KMAX;
+  if (r < k) {
+    double sx = 0.0, sy = 0.0;
+    for (int p = 0; p <= r; ++p) {
+      const double2 t = zm(w[(long long)r * mp + p], rh[p]);
+      sx += t.x;
+      sy += t.y;
+    }
+    z[r] = make_double2(sx, sy);
+  }
+  __syncthreads();
+  if (r < k) {
+    double mx = 0.0, my = 0.0, ev = 0.0;
+    for (int i = r; i < k; ++i) {
+      const double2 a = w[(long long)i * mp + r];
+      const double2 zi = z[i];
+      mx += a.x * zi.x + a.y * zi.y;   // conj(W[i][r]) z[i]
+      my += a.x * zi.y - a.y * zi.x;
+      ev += a.x * a.x + a.y * a.y;
+    }
+    const double d = dsc[(long long)b * RP_KMAX + r];
+    const int id = support[(long long)b * k + r];
+    const double g = (double)gamma[b * n + id];
+    const double sg = fmax(ev / (d * d), 0.0);
+    mean[b * n + id] = make_float2((float)(mx / d), (float)(my / d));
+    var[b * n + id] = (float)sg;
+    dd[b * n + id] = (float)fmax((g - sg) / (g * g), 0.0);
+  }
+}
+
+__global__ void k_rp_defaults(const float* __restrict__ gamma, float2* __restrict__ mean, float* __restrict__ var,
+                              float* __restrict__ dd, long long total) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  mean[i] = make_float2(0.f, 0.f);
+  var[i] = gamma[i];
+  dd[i] = 0.f;
+}
+
+struct RpLayout {
+  size_t qs, ws, linv, spart, dsc, rhs, rsc, total;
+};
+
+RpLayout rp_layout(int batch, int k) {
+  const long long mp = (k + TS - 1) / TS * TS, nb = mp / TS;
+  RpLayout l{};
+  size_t off = 0;
+  auto reg = [&](size_t& at, size_t bytes) {
+    off = (off + 255) / 256 * 256;
+    at = off;
+    off += bytes;
+  };
+  reg(l.qs, 16 * (size_t)(batch * mp * mp));
+  reg(l.ws, 16 * (size_t)(batch * mp * mp));
+  reg(l.linv, 16 * (size_t)(batch * nb * TS * TS));
+  reg(l.spart, 16 * (size_t)(batch * nb * DN_KS * TS * TS));
+  reg(l.dsc, 8 * (size_t)(batch * RP_KMAX));
+  reg(l.rhs, 16 * (size_t)(batch * RP_KMAX));
+  reg(l.rsc, 8 * 2 * (size_t)batch);
+  l.total = (off + 255) / 256 * 256;
+  return l;
+}
+
 }  // namespace
 
 extern "C" {
@@ -704,6 +865,64 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
   hipLaunchKernelGGL(k_dense_moments, dim3(blocks, batch), dim3(DN_THREADS), 0, st, (const double2*)t1,
                      (const double*)ddpart, m, m1, n1, n2, nrch, gamma, (float2*)mean, var, dd, part);
   hipLaunchKernelGGL(k_dense_seg, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum);
+  return (int)hipGetLastError();
+}
+
+
+size_t fbme_reduced_posterior_large_bytes(int batch, int k) {
+  if (batch < 1 || batch > 65535 || k < 1 || k > RP_KMAX) return 0;
+  return rp_layout(batch, k).total;
+}
+
+int fbme_reduced_posterior_large(void* stream, int batch, int k, int n1, int n2, const int* support,
+                                 const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                                 const void* dir1, const void* dir2, const void* mask_spectrum,
+                                 float jitter_multiplier, void* work, size_t work_bytes, void* mean, float* var,
+                                 float* dd, int* status) {
+  if (batch < 1 || batch > 65535 || k < 1 || k > RP_KMAX || n1 < 1 || n2 < 1 || (long long)n1 * n2 < k)
+    return FBME_ERR_ARG;
+  if (!support || !gamma || !gamma_sum || !noise || !bp || !work || !mean || !var || !dd || !status)
+    return FBME_ERR_ARG;
+  if (!mask_spectrum && (!dir1 || !dir2)) return FBME_ERR_ARG;
+  const RpLayout l = rp_layout(batch, k);
+  if (work_bytes < l.total) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  char* w = (char*)work;
+  double2* qs = (double2*)(w + l.qs);
+  double2* ws = (double2*)(w + l.ws);
+  double2* linv = (double2*)(w + l.linv);
+  double2* spart = (double2*)(w + l.spart);
+  double* dsc = (double*)(w + l.dsc);
+  double2* rhs = (double2*)(w + l.rhs);
+  double* rsc = (double*)(w + l.rsc);
+  const int mp = (k + TS - 1) / TS * TS, nb = mp / TS;
+  const long long n = (long long)n1 * n2, total = (long long)batch * n;
+  const double2* d1 = (const double2*)dir1;
+  const double2* d2 = (const double2*)dir2;
+  const double2* mh = (const double2*)mask_spectrum;
+  hipLaunchKernelGGL(k_rp_defaults, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gamma, (float2*)mean,
+                     var, dd, total);
+  hipLaunchKernelGGL(k_rp_prep, dim3(batch), dim3(RP_KMAX), 0, st, k, n1, n2, support, gamma, gamma_sum, noise,
+                     (const float2*)bp, d1, d2, mh, (double)jitter_multiplier, dsc, rhs, rsc);
+  hipLaunchKernelGGL(k_rp_build, dim3((unsigned)(((long long)mp * mp + 255) / 256), batch), dim3(256), 0, st, k, mp,
+                     n1, n2, support, gamma, d1, d2, mh, (const double*)dsc, (const double*)rsc, qs);
+  for (int jb = 0; jb < nb; ++jb) {
+    hipLaunchKernelGGL(k_tile_diag, dim3(batch), dim3(DN_THREADS), 0, st, qs, mp, jb, F32_TINY_DN, linv, status);
+    const int rest = nb - jb - 1;
+    if (rest > 0) {
+      hipLaunchKernelGGL(k_tile_panel, dim3(rest, batch), dim3(DN_THREADS), 0, st, qs, mp, jb, (const double2*)linv);
+      hipLaunchKernelGGL(k_tile_update, dim3(rest * (rest + 1) / 2, batch), dim3(DN_THREADS), 0, st, qs, mp, jb);
+    }
+  }
+  for (int ib = 0; ib < nb; ++ib) {
+    if (ib > 0)
+      hipLaunchKernelGGL(k_tile_inverse_part, dim3(ib, DN_KS, batch), dim3(DN_THREADS), 0, st, (const double2*)qs,
+                         mp, ib, (const double2*)ws, spart);
+    hipLaunchKernelGGL(k_tile_inverse_row, dim3(ib + 1, batch), dim3(DN_THREADS), 0, st, mp, ib,
+                       (const double2*)linv, (const double2*)spart, ws);
+  }
+  hipLaunchKernelGGL(k_rp_finish, dim3(batch), dim3(RP_KMAX), 0, st, k, mp, n, support, gamma, (const double*)dsc,
+                     (const double2*)rhs, (const double2*)ws, (float2*)mean, var, dd);
   return (int)hipGetLastError();
 }
 

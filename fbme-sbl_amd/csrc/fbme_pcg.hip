@@ -159,6 +159,55 @@ k_topk_merge_keys(const unsigned long long* __restrict__ cand_in, int count, int
   wave_topk<32>(v, RANK, cand_out + ((long long)b * groups + g) * RANK);
 }
 
+// Top k <= 256 (the active-set selection, solver.py:3184-3216): every
+// 1024-value tile keeps its top k, then groups of 2048 candidates keep
+// theirs, level by level, until one wavefront holds them all.  Keys pack
+// (value, position), ties go to the lower position.
+__global__ void __launch_bounds__(64)
+k_topkk_tile(const float* __restrict__ x, long long n, int tiles, int k, unsigned long long* __restrict__ cand) {
+  const int t = blockIdx.x, b = blockIdx.y;
+  unsigned long long v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const long long e = (long long)t * 1024 + threadIdx.x + 64 * i;
+    v[i] = (e < n) ? pack_key(score_key(x[(long long)b * n + e]), (int)e) : 0ull;
+  }
+  wave_topk<16>(v, k, cand + ((long long)b * tiles + t) * k);
+}
+
+__global__ void __launch_bounds__(64)
+k_topkk_merge(const unsigned long long* __restrict__ cand_in, int count, int groups, int k,
+              unsigned long long* __restrict__ cand_out) {
+  const int g = blockIdx.x, b = blockIdx.y;
+  unsigned long long v[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int slot = g * 2048 + threadIdx.x + 64 * i;
+    v[i] = (slot < count) ? cand_in[(long long)b * count + slot] : 0ull;
+  }
+  wave_topk<32>(v, k, cand_out + ((long long)b * groups + g) * k);
+}
+
+__global__ void __launch_bounds__(64)
+k_topkk_final(const unsigned long long* __restrict__ cand, int count, int k, const float* __restrict__ x, long long n,
+              int* __restrict__ idx_out, float* __restrict__ val_out) {
+  __shared__ unsigned long long top[256];
+  const int b = blockIdx.x;
+  unsigned long long v[32];
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int slot = threadIdx.x + 64 * i;
+    v[i] = (slot < count) ? cand[(long long)b * count + slot] : 0ull;
+  }
+  wave_topk<32>(v, k, top);
+  __syncthreads();
+  for (int r = threadIdx.x; r < k; r += 64) {
+    const int e = packed_pos(top[r]);
+    idx_out[(long long)b * k + r] = e;
+    val_out[(long long)b * k + r] = x[(long long)b * n + e];
+  }
+}
+
 // Merge the tile candidates of each scene into the sorted top RANK
 // (values and indices), one wavefront per scene (tiles * RANK <= 2048).
 __global__ void __launch_bounds__(64)
@@ -1241,6 +1290,35 @@ int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand
     count = groups * RANK;
   }
   hipLaunchKernelGGL(k_topk_merge, dim3(batch), dim3(64), 0, st, (const unsigned long long*)cur, count / RANK, x, n,
+                     idx, val);
+  return (int)hipGetLastError();
+}
+
+size_t fbme_topk_scratch_bytes(int batch, long long n, int k) {
+  if (batch < 1 || batch > 65535 || k < 1 || k > 256 || n < k || n > (1ll << 26)) return 0;
+  long long count = (n + 1023) / 1024 * k, total = count;
+  while (count > 2048) {
+    count = (count + 2047) / 2048 * k;
+    total += count;
+  }
+  return (size_t)batch * (size_t)total * sizeof(unsigned long long);
+}
+
+int fbme_topk(void* stream, int batch, long long n, int k, const float* x, void* cand, int* idx, float* val) {
+  if (fbme_topk_scratch_bytes(batch, n, k) == 0 || !cand || !x || !idx || !val) return FBME_ERR_ARG;
+  const int tiles = (int)((n + 1023) / 1024);
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* cur = (unsigned long long*)cand;
+  hipLaunchKernelGGL(k_topkk_tile, dim3(tiles, batch), dim3(64), 0, st, x, n, tiles, k, cur);
+  int count = tiles * k;
+  while (count > 2048) {
+    const int groups = (count + 2047) / 2048;
+    unsigned long long* next = cur + (long long)batch * count;
+    hipLaunchKernelGGL(k_topkk_merge, dim3(groups, batch), dim3(64), 0, st, cur, count, groups, k, next);
+    cur = next;
+    count = groups * k;
+  }
+  hipLaunchKernelGGL(k_topkk_final, dim3(batch), dim3(64), 0, st, (const unsigned long long*)cur, count, k, x, n,
                      idx, val);
   return (int)hipGetLastError();
 }

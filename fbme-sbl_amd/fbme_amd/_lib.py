@@ -153,6 +153,12 @@ _SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32_scratch_bytes": (c_size_t, [c_int, c_longlong]),
+    "fbme_topk_scratch_bytes": (c_size_t, [c_int, c_longlong, c_int]),
+    "fbme_topk": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_reduced_posterior_large_bytes": (c_size_t, [c_int, c_int]),
+    "fbme_reduced_posterior_large": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
+                                             c_size_t, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32_noalloc": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p, c_void_p]),

@@ -24,9 +24,8 @@ freedom and noise update.  This is the reference's restricted operator
 (operators.py:57-75, 473-496) without the gather.
 
 Differences from the reference, all documented:
-  * an active-set switch is taken when the 32 strongest variances explain
-    95% of the total (the reference then also selects 32 atoms); cases where
-    the reference would switch to 33-256 atoms keep the PCG posterior;
+  * the active set holds at most 256 atoms (the reference's default
+    active_set_max_size); larger maxima raise NotImplementedError;
   * the low-rank preconditioner's top-32 selection takes N <= 2^26 cells;
   * 'circulant', 'nystrom' and 'none' preconditioners, Hadamard probes,
     adaptive probe counts and spatial coupling raise NotImplementedError.
@@ -144,6 +143,8 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append("spatial coupling")
     if cfg.fixed_noise_variance is not None:
         bad.append("fixed_noise_variance")
+    if cfg.active_set and cfg.active_set_max_size > 256:
+        bad.append("active_set_max_size > 256")
     if bad:
         raise NotImplementedError(
             "mode='cg' on MI355X does not implement: " + ", ".join(bad))
@@ -343,6 +344,7 @@ class _Solver:
         self.noise_part = torch.empty(b * ((m + 255) // 256), dtype=f64, device=dev)
         self.status = torch.zeros(b, dtype=torch.int32, device=dev)
         self.support = None
+        self.act_idx = self.act_cand = self.act_val = self.rp_work = None
         self.mask = None
         self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
 
@@ -621,11 +623,20 @@ class _Solver:
         if self.support is not None:
             t = self.t
             gsum = self.sums[:, 0].contiguous()
-            self._chk(self.lib.fbme_reduced_posterior_masked(
-                self.stream, self.b, self.support.shape[1], *self.grid, self.m_obs, _ptr(self.support),
-                _ptr(self.gamma), _ptr(gsum), _ptr(self.noise), _ptr(self.bp), _ptr(self.ye), _ptr(t.dir1),
-                _ptr(t.dir2), _ptr(self.mask_spec), float(self.cfg.cuda_cholesky_jitter_multiplier),
-                _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.status)), "reduced_posterior")
+            k = self.support.shape[1]
+            jitter = float(self.cfg.cuda_cholesky_jitter_multiplier)
+            if k <= RANK:
+                self._chk(self.lib.fbme_reduced_posterior_masked(
+                    self.stream, self.b, k, *self.grid, self.m_obs, _ptr(self.support),
+                    _ptr(self.gamma), _ptr(gsum), _ptr(self.noise), _ptr(self.bp), _ptr(self.ye), _ptr(t.dir1),
+                    _ptr(t.dir2), _ptr(self.mask_spec), jitter,
+                    _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.status)), "reduced_posterior")
+            else:
+                self._chk(self.lib.fbme_reduced_posterior_large(
+                    self.stream, self.b, k, *self.grid, _ptr(self.support), _ptr(self.gamma), _ptr(gsum),
+                    _ptr(self.noise), _ptr(self.bp), _ptr(t.dir1), _ptr(t.dir2), _ptr(self.mask_spec), jitter,
+                    _ptr(self.rp_work), self.rp_work.numel(), _ptr(self.mean), _ptr(self.var), _ptr(self.dd),
+                    _ptr(self.status)), "reduced_posterior_large")
             self._chk(self.lib.fbme_relevance_sum(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.dd),
                                                   _ptr(self.grid_part), _ptr(self.rel_sum)), "relevance")
             return 0
@@ -681,28 +692,32 @@ class _Solver:
                                       _ptr(self.dof_f)), "cg_em")
 
     def try_active_set(self):
-        """Switch to the exact reduced posterior when the 32 strongest
-        variances explain the tail fraction (solver.py:3184-3216)."""
+        """Switch to the exact reduced posterior when at most
+        active_set_max_size of the strongest variances explain all but the
+        tail fraction of their sum (solver.py:3184-3216)."""
         cfg = self.cfg
-        self._chk(self.lib.fbme_topk32(self.stream, self.b, self.n, _ptr(self.gamma), _ptr(self.cand),
-                                       _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
         total = float(self.sums[0, 0])
         if not np.isfinite(total):
             return
-        # solver.py:3200-3216 on the top min(max_size, N) variances; the
-        # kernel ranks the top RANK, so a switch that would need more atoms
-        # keeps the PCG posterior.
         cap = min(cfg.active_set_max_size, self.n)
-        top = self.top_val[0].double().cpu().numpy()[:min(cap, RANK)]
+        if self.act_idx is None or self.act_idx.shape[1] != cap:
+            nbytes = int(self.lib.fbme_topk_scratch_bytes(self.b, self.n, cap))
+            self.act_cand = torch.empty(nbytes // 8, dtype=torch.int64, device=self.dev)
+            self.act_idx = torch.empty((self.b, cap), dtype=torch.int32, device=self.dev)
+            self.act_val = torch.empty((self.b, cap), dtype=torch.float32, device=self.dev)
+        self._chk(self.lib.fbme_topk(self.stream, self.b, self.n, cap, _ptr(self.gamma), _ptr(self.act_cand),
+                                     _ptr(self.act_idx), _ptr(self.act_val)), "topk")
+        top = self.act_val[0].double().cpu().numpy()
         target = (1.0 - cfg.active_set_tail_fraction) * total
         if top.sum() < target:
             return
         cum = np.cumsum(top)
         required = min(int(np.searchsorted(cum, target)), cap - 1) + 1
         size = min(cap, max(cfg.active_set_min_size, required))
+        self.support = self.act_idx[:, :size].contiguous()
         if size > RANK:
-            return
-        self.support = self.top_idx[:, :size].contiguous()
+            nbytes = int(self.lib.fbme_reduced_posterior_large_bytes(self.b, size))
+            self.rp_work = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
         self.mask = torch.zeros((self.b, self.n), dtype=torch.uint8, device=self.dev)
         self.mask[0, self.support[0].long()] = 1
         self.have_solution = False

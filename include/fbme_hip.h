@@ -280,6 +280,12 @@ int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const doub
  * level of merged candidates per 64x reduction); 32 <= n <= 2^26. */
 size_t fbme_topk32_scratch_bytes(int batch, long long n);
 int fbme_topk32(void* stream, int batch, long long n, const float* x, void* cand, int* idx, float* val);
+/* k <= 256 largest values of each x[b, :n] sorted descending, ties to the
+ * lower index: the active-set selection of the general solver
+ * (torch.topk in _select_active_indices, solver.py:3184-3216).  cand:
+ * fbme_topk_scratch_bytes(batch, n, k) bytes; k <= n <= 2^26. */
+size_t fbme_topk_scratch_bytes(int batch, long long n, int k);
+int fbme_topk(void* stream, int batch, long long n, int k, const float* x, void* cand, int* idx, float* val);
 /* fbme_topk32 without a workspace argument and without any allocation: no
  * counterpart in the reference.  For callers that cannot allocate device
  * memory (code under hipGraph capture, tight serving loops, harnesses that
@@ -425,6 +431,19 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
                                   const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
                                   const double* ye, const void* dir1, const void* dir2, const void* mask_spectrum,
                                   float jitter_multiplier, void* mean, float* var, float* dd, int* status);
+/* fbme_reduced_posterior_masked for supports of 33 to 256 atoms (the
+ * general solver's active set up to active_set_max_size = 256,
+ * solver.py:3184-3216, 1316-1341, 3591-3779): the FP64 system padded to a
+ * multiple of 32 and factored / inverted on 32 x 32 tiles.  mask_spectrum
+ * (or NULL with the Dirichlet tables dir1, dir2) gives the Gram; gamma_sum:
+ * [batch] FP64 sum of gamma; work: fbme_reduced_posterior_large_bytes(batch,
+ * k) bytes; status[b] = 1 on a non-positive pivot. */
+size_t fbme_reduced_posterior_large_bytes(int batch, int k);
+int fbme_reduced_posterior_large(void* stream, int batch, int k, int n1, int n2, const int* support,
+                                 const float* gamma, const double* gamma_sum, const float* noise, const void* bp,
+                                 const void* dir1, const void* dir2, const void* mask_spectrum,
+                                 float jitter_multiplier, void* work, size_t work_bytes, void* mean, float* var,
+                                 float* dd, int* status);
 /* ------------------------------------------------------------------------
  * Exact dense posterior of the general solver, mode="dense": _posterior's
  * dense branch (torch_sbl/solver.py:2962-2990) with Q = noise I + A Gamma

@@ -30,13 +30,16 @@ def golden_path(name):
     return os.path.join(GOLDEN, name)
 
 
-def assert_history_matches(res, g, *, upto=None, rtol=1e-4):
+def assert_history_matches(res, g, *, upto=None, rtol=1e-4, pcg_slack=0):
     """Per-EM-iteration agreement of a CG solve with a reference golden run:
     identical PCG iteration counts, noise variance (batch mean) within
     ``rtol``, and relative changes ||mu_t - mu_t-1|| / ||mu_t|| within 2e-5
     absolute (a difference of two close iterates: a 1e-6 relative difference
     in mu moves it by ~1e-6 absolute, 1e-5 seen at changes of 1e-3), for
-    iterations 1..upto (all when None).  Returns the largest relative noise
+    iterations 1..upto (all when None).  ``pcg_slack`` EM iterations may
+    stop their PCG one convergence check (8 iterations) apart from the
+    reference: a relative residual that lands on the tolerance at a check
+    is decided by float32 rounding.  Returns the largest relative noise
     deviation seen."""
     import numpy as np
     hn = np.asarray(res.history["noise_variance"], np.float64)
@@ -47,7 +50,9 @@ def assert_history_matches(res, g, *, upto=None, rtol=1e-4):
     rr = np.asarray(g["cg_history_relative_change"], np.float64)
     n = min(len(hn), len(rn)) if upto is None else upto
     assert len(hn) >= n and len(rn) >= n
-    np.testing.assert_array_equal(hc[:n], rc[:n])
+    off = np.flatnonzero(hc[:n] != rc[:n])
+    assert off.size <= pcg_slack, ("PCG counts differ at EM iterations", (off + 1).tolist())
+    assert np.all(np.abs(hc[off] - rc[off]) == 8), (hc[off], rc[off])
     dev = np.abs(hn[:n] / rn[:n] - 1.0)
     assert dev.max() < rtol, (int(dev.argmax()) + 1, float(dev.max()))
     np.testing.assert_allclose(hr[1:n], rr[1:n], rtol=1e-3, atol=2e-5)

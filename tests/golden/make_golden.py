@@ -433,14 +433,53 @@ def generic(reference_root: str) -> None:
               "support", nz.size, flush=True)
 
 
+ACTIVE_CASES = [
+    # name, seed, grid, samples, k, snr_db, SBLConfig overrides: the reference
+    # switches to 58, 161 and 37 atoms; then max_size 16 and min_size 48
+    ("active_64x64_40x40_k60", 1080, (64, 64), (40, 40), 60, 20.0, dict(max_iter=60)),
+    ("active_48x48_32x32_k50", 1082, (48, 48), (32, 32), 50, 20.0, dict(max_iter=60)),
+    ("active_64x64_40x40_k45", 1083, (64, 64), (40, 40), 45, 30.0, dict(max_iter=60)),
+    ("active_64x64_40x40_k60_max16", 1080, (64, 64), (40, 40), 60, 20.0,
+     dict(max_iter=40, active_set_max_size=16, active_set_min_size=8)),
+    ("active_64x64_40x40_k45_min48", 1083, (64, 64), (40, 40), 45, 30.0,
+     dict(max_iter=40, active_set_min_size=48)),
+]
+
+
+def active_set(reference_root: str) -> None:
+    """``sbl_2d`` in mode 'cg' on single scenes where the reference's
+    active-set switch (solver.py:3184-3216, 1316-1341) takes more than 32
+    atoms, and with non-default active_set_max_size / active_set_min_size."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    for name, seed, grid, samples, k, snr, over in ACTIVE_CASES:
+        scene = scenes.uniform_scene(seed, grid, samples, k=k, snr_db=snr)
+        res = sbl_2d(torch.from_numpy(scene.y), grid, config=SBLConfig(device="cpu", mode="cg", **over))
+        arrays = dict(y=scene.y, x_true=scene.x, grid=np.array(grid), samples=np.array(samples))
+        for key, value in over.items():
+            arrays["override_" + key] = np.array(value)
+        arrays.update(_result_arrays(res, "cg_"))
+        for key in ("relative_change", "cg_iterations", "noise_variance", "active_set_size"):
+            arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "active set", res.active_set_size, "iterations", res.iterations, flush=True)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "generic":
+    if which == "active":
+        active_set(root)
+    elif which == "generic":
         generic(root)
+        active_set(root)
     elif which == "headline":
         headline(root)
         generic(root)
+        active_set(root)
     elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
@@ -459,3 +498,4 @@ if __name__ == "__main__":
         offgrid(root)
         headline(root)
         generic(root)
+        active_set(root)

@@ -225,3 +225,29 @@ def test_joint_sbl_cg_matches_reference_golden_run(name, grid, kw):
     np.testing.assert_allclose(out["noise_variance"], g["cg_noise_variance"], rtol=1e-3)
     np.testing.assert_allclose(out["effective_dof"], g["cg_effective_dof"], rtol=1e-3)
     np.testing.assert_allclose(out["cg_iterations"], g["cg_history_cg_iterations"], atol=8)
+
+
+ACTIVE = ["active_64x64_40x40_k60", "active_48x48_32x32_k50", "active_64x64_40x40_k45",
+          "active_64x64_40x40_k60_max16", "active_64x64_40x40_k45_min48"]
+
+
+@pytest.mark.parametrize("case", ACTIVE)
+def test_oracle_active_set_sizes_match_reference(case):
+    """The reference's active-set switch (solver.py:3184-3216) at 58, 161 and
+    37 atoms, with active_set_max_size=16 (never switches) and
+    active_set_min_size=48: the oracle takes the same switch at the same
+    iteration and matches x."""
+    g = np.load(golden_path(case + ".npz"))
+    over = {k[len("override_"):]: g[k].item() for k in g.files if k.startswith("override_")}
+    cfg = co.CGConfig(**over)
+    grid = tuple(int(v) for v in g["grid"])
+    trace = []
+    out = co.sbl_cg(g["y"], grid, cfg, trace=trace)
+    hist = [t["active"] for t in trace]
+    np.testing.assert_array_equal(hist, g["cg_history_active_set_size"][:len(hist)])
+    assert out["active_set_size"] == int(g["cg_active_set_size"])
+    assert out["iterations"] == int(g["cg_iterations"])
+    x, xr = out["x"].reshape(-1), g["cg_x"].reshape(-1)
+    rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    print(case, "active", out["active_set_size"], "rel", rel)
+    assert rel < 1e-4    # achieved 2.8e-6 .. 1.1e-5
