@@ -118,6 +118,49 @@ k_rowpass(const float2* __restrict__ in, float2* __restrict__ out, int n, int le
   }
 }
 
+// ------------------------------------------- Strang circulant preconditioner
+// (operators.py:373-416): the circulant column of Q on the M1 x M2 sample
+// grid takes lag (k <= M / 2 ? k : k - M) on each axis from fft2(gamma);
+// its eigenvalues fft2(column) + noise are floored at max(eps_f32 max|eig|,
+// noise), and M^-1 v = ifft2(fft2(v) / eig).
+__global__ void k_circ_gather(const float2* __restrict__ lags, int n1, int n2, int m1, int m2,
+                              float2* __restrict__ col) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m1 * m2) return;
+  const int k1 = i / m2, k2 = i - k1 * m2;
+  const int l1 = k1 <= m1 / 2 ? k1 : n1 - (m1 - k1);
+  const int l2 = k2 <= m2 / 2 ? k2 : n2 - (m2 - k2);
+  col[(long long)b * m1 * m2 + i] = lags[((long long)b * n1 + l1) * n2 + l2];
+}
+
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_circ_eig(const float2* __restrict__ spec, const float* __restrict__ noise, int m, float* __restrict__ eig) {
+  __shared__ float red[FBME_BLOCK / 64];
+  const int b = blockIdx.x;
+  const float nz = noise[b];
+  const float2* sp = spec + (long long)b * m;
+  float mx = 0.f;
+  for (int i = threadIdx.x; i < m; i += FBME_BLOCK) mx = fmaxf(mx, fabsf(sp[i].x + nz));
+  for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  float amax = 0.f;
+  for (int w = 0; w < FBME_BLOCK / 64; ++w) amax = fmaxf(amax, red[w]);
+  const float fl = fmaxf(1.1920928955078125e-07f * amax, nz);
+  for (int i = threadIdx.x; i < m; i += FBME_BLOCK) eig[(long long)b * m + i] = fmaxf(sp[i].x + nz, fl);
+}
+
+// data[lane] /= eig[scene] * M (the unnormalised inverse transform follows).
+__global__ void k_circ_scale(float2* __restrict__ data, const float* __restrict__ eig, int rhs, int m) {
+  const int l = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const float s = 1.0f / (eig[(long long)(l / rhs) * m + i] * (float)m);
+  float2* d = data + (long long)l * m + i;
+  *d = make_float2(d->x * s, d->y * s);
+}
+
 // -------------------------------------------------------------- tile top-k
 // Top RANK entries of each 1024-value tile of x[b, :] (by value, then lower
 // index), as packed keys.
@@ -1219,6 +1262,47 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
   (void)tw_n1;
   (void)tw_e1;
   if (rc) return rc;
+  return (int)hipGetLastError();
+}
+
+int fbme_circulant_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2, unsigned code_n2,
+                            unsigned code_m2, const float* gamma, const float* noise, void* work_a, void* work_b,
+                            float* eig, const void* tw_n2, const void* tw_m2) {
+  // work_a, work_b: batch * max(N1 N2, M1 M2) complex64 each; eig: batch * M1 M2.
+  if (batch < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2 || n1 > 1024 || n2 > 1024) return FBME_ERR_ARG;
+  if (code_n2 == 0xFFFFFFFFu || code_m2 == 0xFFFFFFFFu || !gamma || !noise || !eig) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)n1 * n2, m = (long long)m1 * m2;
+  float2* a = (float2*)work_a;
+  float2* b = (float2*)work_b;
+  hipLaunchKernelGGL(k_real_to_complex, dim3(nblk(batch * n)), dim3(FBME_BLOCK), 0, st, gamma, a, batch * n);
+  launch_rowpass<false>(a, b, n2, n2, n2, (long long)batch * n1, code_n2, (const float2*)tw_n2, st);
+  int rc = fbme_column_fft(stream, batch, n1, n1, n1, n2, b, a, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_circ_gather, dim3(nblk(m), batch), dim3(FBME_BLOCK), 0, st, a, n1, n2, m1, m2, b);
+  launch_rowpass<false>(b, a, m2, m2, m2, (long long)batch * m1, code_m2, (const float2*)tw_m2, st);
+  rc = fbme_column_fft(stream, batch, m1, m1, m1, m2, a, b, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_circ_eig, dim3(batch), dim3(FBME_BLOCK), 0, st, b, noise, (int)m, eig);
+  return (int)hipGetLastError();
+}
+
+int fbme_circulant_apply(void* stream, int lanes, int rhs, int m1, int m2, unsigned code_m2, const float* eig,
+                         const void* v, void* out, void* work, const void* tw_m2) {
+  // work: lanes * M1 M2 complex64.
+  if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 1024 || m2 > 1024) return FBME_ERR_ARG;
+  if (code_m2 == 0xFFFFFFFFu || !eig || !v || !out || !work || v == out) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int m = m1 * m2;
+  launch_rowpass<false>((const float2*)v, (float2*)work, m2, m2, m2, (long long)lanes * m1, code_m2,
+                        (const float2*)tw_m2, st);
+  int rc = fbme_column_fft(stream, lanes, m1, m1, m1, m2, work, out, 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_circ_scale, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, st, (float2*)out, eig, rhs, m);
+  rc = fbme_column_fft(stream, lanes, m1, m1, m1, m2, out, work, 1);
+  if (rc) return rc;
+  launch_rowpass<true>((const float2*)work, (float2*)out, m2, m2, m2, (long long)lanes * m1, code_m2,
+                       (const float2*)tw_m2, st);
   return (int)hipGetLastError();
 }
 

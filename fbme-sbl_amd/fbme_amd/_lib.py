@@ -153,6 +153,10 @@ _SIGNATURES = {
                                            c_void_p, c_void_p, c_void_p, c_void_p,
                                            c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_topk32_scratch_bytes": (c_size_t, [c_int, c_longlong]),
+    "fbme_circulant_spectrum": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_uint, c_uint, c_void_p,
+                                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_circulant_apply": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
     "fbme_topk_scratch_bytes": (c_size_t, [c_int, c_longlong, c_int]),
     "fbme_topk": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_reduced_posterior_large_bytes": (c_size_t, [c_int, c_int]),

@@ -27,8 +27,8 @@ Differences from the reference, all documented:
   * the active set holds at most 256 atoms (the reference's default
     active_set_max_size); larger maxima raise NotImplementedError;
   * the low-rank preconditioner's top-32 selection takes N <= 2^26 cells;
-  * 'circulant', 'nystrom' and 'none' preconditioners, Hadamard probes,
-    adaptive probe counts and spatial coupling raise NotImplementedError.
+  * the 'nystrom' preconditioner, Hadamard probes, adaptive probe counts
+    and spatial coupling raise NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -82,9 +82,12 @@ class _Tables:
         self.codes = {L: lib.fbme_radix_code(L) for L in (n1, n2, self.e1, self.e2)}
         if any(c == 0xFFFFFFFF for c in self.codes.values()):
             raise NotImplementedError("grid lengths must factor into 2, 3 and 5")
+        # sample-grid transforms of the circulant preconditioner (M1, M2 may
+        # not factor into 2, 3, 5: then only that preconditioner is refused)
+        self.sample_codes = {L: lib.fbme_radix_code(L) for L in (m1, m2)}
         stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
         self.tw = {}
-        for L in (n1, n2, self.e1, self.e2):
+        for L in {n1, n2, self.e1, self.e2} | {v for v, c in self.sample_codes.items() if c != 0xFFFFFFFF}:
             t = torch.empty(L, dtype=torch.complex64, device=device)
             _lib.check(lib.fbme_twiddle_table(stream, L, _ptr(t)), "twiddle")
             self.tw[L] = t
@@ -129,9 +132,9 @@ def _schedule(it: int, cfg: SBLConfig) -> float:
 
 def _check_supported(cfg: SBLConfig) -> None:
     bad = []
-    if cfg.cg_preconditioner != "low_rank":
+    if cfg.cg_preconditioner not in ("low_rank", "circulant", "none"):
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
-    if cfg.cg_preconditioner_rank != RANK:
+    if cfg.cg_preconditioner == "low_rank" and cfg.cg_preconditioner_rank != RANK:
         bad.append("cg_preconditioner_rank != 32")
     if cfg.cg_matvec != "tbt":
         bad.append("cg_matvec='fourier'")
@@ -229,7 +232,12 @@ class _Solver:
         # N1 = 256 grids project / synthesise the preconditioner's atoms
         # through the 256-point column transform
         self.grid_op = self.grid == (256, 256)
-        self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256
+        # preconditioner (solver.py:5726-5783): low-rank Woodbury, Strang
+        # circulant on the sample grid, or none
+        self.precond = "low_rank" if self.dense else cfg.cg_preconditioner
+        if self.precond == "circulant" and 0xFFFFFFFF in self.t.sample_codes.values():
+            raise NotImplementedError("the circulant preconditioner needs M1, M2 that factor into 2, 3 and 5")
+        self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256 and self.precond == "low_rank"
         # both on the column-transform passes: the PCG update / direction sum
         # the inner-product partials of those passes themselves
         self.fused = self.grid_op and self.lr_fft and self.obs is None
@@ -321,6 +329,10 @@ class _Solver:
         self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
         self.lr_map = torch.empty((ns, 386), dtype=torch.int32, device=dev)
         self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
+        self.circ_eig = self.circ_work = None
+        if self.precond == "circulant":
+            self.circ_eig = torch.empty((ns, m), dtype=f32, device=dev)
+            self.circ_work = torch.empty(self.lanes * m, dtype=c64, device=dev)
         self.mask_spec = None
         if self.obs is not None:
             self.mask_spec = torch.empty(n, dtype=torch.complex128, device=dev)
@@ -412,6 +424,18 @@ class _Solver:
         self._precondition(v, out, dot)
 
     def _precondition(self, v, out, dot=None):
+        if self.precond != "low_rank":
+            if self.precond == "circulant":
+                t = self.t
+                m2 = self.samples[1]
+                self._chk(self.lib.fbme_circulant_apply(
+                    self.stream, self.lanes, self.r, *self.samples, t.sample_codes[m2], _ptr(self.circ_eig),
+                    _ptr(v), _ptr(out), _ptr(self.circ_work), _ptr(t.tw[m2])), "circulant_apply")
+            else:
+                out.copy_(v)
+            if dot is not None:
+                self.lane_dot(v, out, dot)
+            return
         if self.lr_fft:
             self._chk(self.lib.fbme_lowrank_apply_fft_dot(
                 self.stream, self.lanes, self.r, *self.samples, _ptr(self.factors), _ptr(self.sinv),
@@ -461,6 +485,13 @@ class _Solver:
         m1, m2 = self.samples
         if not self.grid_op:
             self.spectrum_setup()
+        if self.precond == "circulant":
+            self._chk(self.lib.fbme_circulant_spectrum(
+                self.stream, self.nsys, n1, n2, m1, m2, t.codes[n2], t.sample_codes[m2], _ptr(self.gamma),
+                _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b), _ptr(self.circ_eig), _ptr(t.tw[n2]),
+                _ptr(t.tw[m2])), "circulant_spectrum")
+        if self.precond != "low_rank":
+            return
         self._chk(self.lib.fbme_topk32(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
         gsum = self.sums[:, 0].contiguous()

@@ -232,6 +232,19 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
                         unsigned code2, unsigned codee1, unsigned codee2, const float* gamma, const float* noise,
                         void* work_a, void* work_b, void* spectrum, const void* tw_n1, const void* tw_n2,
                         const void* tw_e1, const void* tw_e2);
+/* Strang circulant (BCCB) preconditioner of Q on the M1 x M2 sample grid
+ * (circulant_preconditioner_spectrum / apply_circulant_preconditioner,
+ * operators.py:373-416; cg_preconditioner="circulant",
+ * solver.py:5738-5744, 5779-5780).  eig [batch, M1 M2] float: the column of
+ * signed lags (k <= M / 2 ? k : k - M) of fft2(gamma), transformed, plus
+ * noise, floored at max(eps_f32 max|eig|, noise).  work_a, work_b: batch *
+ * max(N1 N2, M1 M2) complex64 each.  apply: out = ifft2(fft2(v) / eig) for
+ * `lanes` lanes, `rhs` per scene; work: lanes * M1 M2 complex64; v != out. */
+int fbme_circulant_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2, unsigned code_n2,
+                            unsigned code_m2, const float* gamma, const float* noise, void* work_a, void* work_b,
+                            float* eig, const void* tw_n2, const void* tw_m2);
+int fbme_circulant_apply(void* stream, int lanes, int rhs, int m1, int m2, unsigned code_m2, const float* eig,
+                         const void* v, void* out, void* work, const void* tw_m2);
 /* out = Q d for `lanes` lanes, `rhs` lanes per scene (operators.py:328-355).
  * work_a, work_b: lanes * M1 * E2 complex64 each. */
 int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,

@@ -63,6 +63,7 @@ class CGConfig:
     cg_max_iter: int = 150
     cg_check_interval: int = 8
     cg_fixed_iterations: int = 0
+    cg_preconditioner: str = "low_rank"
     cg_preconditioner_rank: int = 32
     cg_damping: float = 0.9
     hutchinson_probes: int = 32
@@ -193,7 +194,45 @@ def low_rank_preconditioner(gamma, noise, op: Operator, rank, eps, tiny):
     return bg, factors, inv
 
 
+def circulant_preconditioner(gamma, noise, op: Operator, eps):
+    """Strang circulant (BCCB) approximation of Q on the M1 x M2 sample grid
+    (operators.py:373-398): the circulant column takes the signed lag
+    (k <= M / 2 ? k : k - M) on each axis from fft2(gamma); eigenvalues
+    Re fft2(column) + noise, floored at max(eps max|eig|, noise)."""
+    b = gamma.shape[0]
+    (n1, n2), (m1, m2) = op.grid, op.samples
+    lags = np.fft.fft2(gamma.reshape(b, n1, n2), axes=(1, 2))
+    k1, k2 = np.arange(m1), np.arange(m2)
+    i1 = np.where(k1 <= m1 // 2, k1, n1 - (m1 - k1))
+    i2 = np.where(k2 <= m2 // 2, k2, n2 - (m2 - k2))
+    eig = np.real(np.fft.fft2(lags[:, i1[:, None], i2[None, :]], axes=(1, 2))) + noise[:, None, None]
+    floor = np.maximum(eps * np.abs(eig).reshape(b, -1).max(1), noise)
+    return ("circulant", np.maximum(eig, floor[:, None, None]), op)
+
+
+def make_preconditioner(kind, gamma, noise, op: Operator, rank, eps, tiny):
+    """_make_pcg_preconditioner (solver.py:5726-5762) for 'low_rank',
+    'circulant' and 'none'."""
+    if kind == "none":
+        return ("none",)
+    if kind == "circulant":
+        return circulant_preconditioner(gamma, noise, op, eps)
+    return low_rank_preconditioner(gamma, noise, op, rank, eps, tiny)
+
+
 def apply_preconditioner(v, pre):
+    """_apply_pcg_preconditioner (solver.py:5765-5783); the circulant form
+    is operators.py:399-416 (expand, transform, divide, invert, restrict)."""
+    kind = pre[0] if isinstance(pre[0], str) else "low_rank"
+    if kind == "none":
+        return v.copy()
+    if kind == "circulant":
+        _, eig, op = pre
+        b, _, r = v.shape
+        (m1, m2) = op.samples
+        field = op.expand(v).reshape(b, m1, m2, r)
+        solved = np.fft.ifft2(np.fft.fft2(field, axes=(1, 2)) / eig[..., None], axes=(1, 2))
+        return op.restrict(solved.reshape(b, m1 * m2, r))
     bg, factors, inv = pre
     base = v / bg[:, None, None]
     proj = np.conj(np.swapaxes(factors, 1, 2)) @ v
@@ -202,7 +241,7 @@ def apply_preconditioner(v, pre):
 
 
 def pcg(rhs, gamma, noise, op: Operator, *, x0, tol, max_iter, check, rank,
-        fixed=0, eps=F32_EPS, tiny=F32_TINY):
+        fixed=0, eps=F32_EPS, tiny=F32_TINY, precond="low_rank"):
     """Batched complex PCG with lane masks and true-residual restarts
     (solver.py:5582-5723)."""
     if x0 is None:
@@ -211,7 +250,7 @@ def pcg(rhs, gamma, noise, op: Operator, *, x0, tol, max_iter, check, rank,
     else:
         x = x0.copy()
         r = rhs - op.q(x, gamma, noise)
-    pre = low_rank_preconditioner(gamma, noise, op, rank, eps, tiny)
+    pre = make_preconditioner(precond, gamma, noise, op, rank, eps, tiny)
     z = apply_preconditioner(r, pre)
     d = z.copy()
     rho = np.real(np.sum(np.conj(r) * z, axis=1))
@@ -316,7 +355,8 @@ def _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg, tol, eps, 
     rhs = np.concatenate([y.T[None], probes], axis=2)           # [1, M, B + P]
     sol, nit = pcg(rhs, gamma[:1], noise[:1], op, x0=sol if cfg.cg_warm_start else None,
                    tol=tol, max_iter=cfg.cg_max_iter, check=cfg.cg_check_interval,
-                   rank=cfg.cg_preconditioner_rank, fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
+                   rank=cfg.cg_preconditioner_rank, fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny,
+                   precond=cfg.cg_preconditioner)
     lifted = op.adj(sol)                                        # [1, N, B + P]
     dd1 = np.maximum(np.mean(np.real(lifted[:, :, b:] * np.conj(lprobes)), 2), 0.0)   # [1, N]
     mean = gamma[:1] * lifted[0, :, :b].T                       # [B, N]
@@ -399,7 +439,7 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             sol, nit = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
                            tol=tol, max_iter=cfg.cg_max_iter,
                            check=cfg.cg_check_interval,
-                           rank=cfg.cg_preconditioner_rank,
+                           rank=cfg.cg_preconditioner_rank, precond=cfg.cg_preconditioner,
                            fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
             cg_iters.append(nit)
             lifted = op.adj(sol)
@@ -454,7 +494,7 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
         rhs = np.concatenate([y[:, :, None], probes], axis=2)
         sol, _ = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
                      tol=final_tol, max_iter=cfg.cg_max_iter,
-                     check=cfg.cg_check_interval, rank=cfg.cg_preconditioner_rank,
+                     check=cfg.cg_check_interval, rank=cfg.cg_preconditioner_rank, precond=cfg.cg_preconditioner,
                      fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
         lifted = op.adj(sol)
         dd = np.maximum(np.mean(np.real(lifted[:, :, 1:] * np.conj(lprobes)), 2), 0.0)

@@ -468,18 +468,64 @@ def active_set(reference_root: str) -> None:
         print("wrote", name, "active set", res.active_set_size, "iterations", res.iterations, flush=True)
 
 
+PRECOND_CASES = [
+    # name, seed, grid, samples, k, snr_db, mask keep fraction, SBLConfig overrides
+    ("precond_circulant_32x32_24x24", 1000, (32, 32), (24, 24), 20, 20.0, None,
+     dict(cg_preconditioner="circulant", max_iter=40)),
+    # the reference's circulant PCG reaches its 150-iteration cap after a
+    # few EM iterations on 64 x 64 grids: five iterations, 0 dB
+    ("precond_circulant_64x64_40x40", 1100, (64, 64), (40, 40), 30, 0.0, None,
+     dict(cg_preconditioner="circulant", hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4,
+          max_iter=5)),
+    ("precond_circulant_masked_64x64_40x40", 1101, (64, 64), (40, 40), 30, 0.0, 0.7,
+     dict(cg_preconditioner="circulant", hutchinson_probes=16, auto_probe_scaling=False, max_iter=5)),
+    ("precond_none_32x32_24x24", 1000, (32, 32), (24, 24), 20, 20.0, None,
+     dict(cg_preconditioner="none", max_iter=40)),
+]
+
+
+def preconditioners(reference_root: str) -> None:
+    """``sbl_2d`` in mode 'cg' with cg_preconditioner='circulant' (the Strang
+    BCCB preconditioner, operators.py:373-416) and 'none'."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    for name, seed, grid, samples, k, snr, keep, over in PRECOND_CASES:
+        scene = scenes.uniform_scene(seed, grid, samples, k=k, snr_db=snr)
+        mask = observation_mask(seed, samples, keep) if keep else None
+        y = scene.y if mask is None else np.where(mask, scene.y, 0).astype(np.complex64)
+        res = sbl_2d(torch.from_numpy(y), grid, config=SBLConfig(device="cpu", mode="cg", **over),
+                     observation_mask=None if mask is None else torch.from_numpy(mask))
+        arrays = dict(y=y, x_true=scene.x, grid=np.array(grid), samples=np.array(samples))
+        if mask is not None:
+            arrays["mask"] = mask
+        for key, value in over.items():
+            arrays["override_" + key] = np.array(value)
+        arrays.update(_result_arrays(res, "cg_"))
+        for key in ("relative_change", "cg_iterations", "noise_variance", "active_set_size"):
+            arrays["cg_history_" + key] = np.array(res.history.get(key, []))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "iterations", res.iterations, "pcg", res.history["cg_iterations"][:6], flush=True)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "active":
+    if which == "precond":
+        preconditioners(root)
+    elif which == "active":
         active_set(root)
     elif which == "generic":
         generic(root)
         active_set(root)
+        preconditioners(root)
     elif which == "headline":
         headline(root)
         generic(root)
         active_set(root)
+        preconditioners(root)
     elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
@@ -499,3 +545,4 @@ if __name__ == "__main__":
         headline(root)
         generic(root)
         active_set(root)
+        preconditioners(root)

@@ -323,3 +323,70 @@ def test_sbl_cg_512_nonsquare_vs_oracle(device):
     assert abs(scenes.nmse_db(x, truth) - scenes.nmse_db(g["x"], truth)) < 0.01
     assert float(res.noise_variance) == pytest.approx(float(g["noise_variance"][0]), rel=1e-4)
     assert _rel(res.posterior_variance.cpu().numpy().reshape(-1), g["posterior_variance"].astype(np.float64)) < 2e-3
+
+
+PRECOND = ["precond_circulant_32x32_24x24", "precond_circulant_64x64_40x40",
+           "precond_circulant_masked_64x64_40x40", "precond_none_32x32_24x24"]
+
+
+@pytest.mark.parametrize("case,pcg_slack,noise_rtol,change_atol", [
+    # 40 EM iterations, PCG at its 150-iteration cap from iteration 5: the
+    # unconverged solves let the noise drift to 1.3e-3 and the relative
+    # change by 2.3e-3 (x: 1.8e-5)
+    ("precond_circulant_32x32_24x24", 1, 2e-3, 3e-3),
+    ("precond_circulant_64x64_40x40", 0, 1e-4, 2e-5),           # x: 4.1e-6
+    ("precond_circulant_masked_64x64_40x40", 0, 1e-4, 2e-5),    # x: 6.7e-5
+    # no preconditioner: three of 40 EM iterations stop their PCG one check
+    # apart (x: 2.3e-6)
+    ("precond_none_32x32_24x24", 3, 1e-4, 2e-5),
+])
+def test_other_preconditioners_match_reference(device, case, pcg_slack, noise_rtol, change_atol):
+    """cg_preconditioner='circulant' (Strang BCCB on the sample grid,
+    operators.py:373-416) and 'none' against the reference's runs: same
+    iteration count, per-EM-iteration PCG counts and noise as stated per
+    case, relative L2 1e-4 on x."""
+    from torch_sbl import SBLConfig, sbl_2d
+    print("case", case, flush=True)
+    g = np.load(golden_path(case + ".npz"))
+    over = {k[len("override_"):]: g[k].item() for k in g.files if k.startswith("override_")}
+    grid = tuple(int(v) for v in g["grid"])
+    mask = torch.from_numpy(g["mask"]) if "mask" in g.files else None
+    res = sbl_2d(torch.from_numpy(g["y"]).to(device), grid, config=SBLConfig(mode="cg", device=str(device), **over),
+                 observation_mask=mask)
+    x = res.x.cpu().numpy().reshape(-1)
+    rel = _rel(x, g["cg_x"].reshape(-1))
+    print(case, "pcg", res.history["cg_iterations"][:8], list(g["cg_history_cg_iterations"][:8]), "rel", rel,
+          flush=True)
+    assert res.iterations == int(g["cg_iterations"])
+    dev = assert_history_matches(res, g, pcg_slack=pcg_slack, rtol=noise_rtol, change_atol=change_atol)
+    print("noise history", dev, flush=True)
+    assert rel < 1e-4
+
+
+@pytest.mark.parametrize("grid,samples", [((64, 48), (45, 40)), ((256, 256), (160, 160))])
+def test_circulant_apply_vs_oracle(device, grid, samples):
+    from torch_sbl import SBLConfig, pcg
+    print("shape", grid, samples, flush=True)
+    from fbme_amd import scenes
+    ys = [scenes.uniform_scene(70 + i, grid, samples, k=10, snr_db=20.0).y for i in range(2)]
+    y = torch.from_numpy(np.stack(ys)).to(device)
+    s = pcg._Solver(y, grid, SBLConfig(mode="cg", device=str(device), cg_preconditioner="circulant"))
+    s.init()
+    s.setup_operators()
+    rng = np.random.default_rng(4)
+    v = (rng.standard_normal((s.b, s.r, s.m)) + 1j * rng.standard_normal((s.b, s.r, s.m))).astype(np.complex64)
+    vt = torch.from_numpy(v).to(device)
+    out = torch.empty_like(vt)
+    dot = torch.empty(s.lanes, dtype=torch.float64, device=device)
+    s.precondition(vt, out, dot=dot)
+    torch.cuda.synchronize()
+    gamma = s.gamma.cpu().numpy().astype(np.float64)
+    noise = s.noise.cpu().numpy().astype(np.float64)
+    op = co.Operator(grid, samples)
+    pre = co.circulant_preconditioner(gamma, noise, op, F32_EPS)
+    np.testing.assert_allclose(s.circ_eig.cpu().numpy().reshape(s.b, *samples), pre[1], rtol=2e-5,
+                               atol=2e-6 * np.abs(pre[1]).max())
+    ref = np.moveaxis(co.apply_preconditioner(np.moveaxis(v.astype(np.complex128), 1, 2), pre), 2, 1)
+    got = out.cpu().numpy()
+    assert _rel(got, ref) < 1e-5
+    np.testing.assert_allclose(dot.cpu().numpy(), np.sum(np.real(np.conj(v) * got), -1).reshape(-1), rtol=1e-6)

@@ -251,3 +251,57 @@ def test_oracle_active_set_sizes_match_reference(case):
     rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
     print(case, "active", out["active_set_size"], "rel", rel)
     assert rel < 1e-4    # achieved 2.8e-6 .. 1.1e-5
+
+
+PRECOND = ["precond_circulant_32x32_24x24", "precond_circulant_64x64_40x40",
+           "precond_circulant_masked_64x64_40x40", "precond_none_32x32_24x24"]
+
+
+@pytest.mark.parametrize("case", PRECOND)
+def test_oracle_preconditioners_match_reference(case):
+    """cg_preconditioner='circulant' (the Strang BCCB approximation,
+    operators.py:373-416) and 'none' against the reference's runs."""
+    g = np.load(golden_path(case + ".npz"))
+    over = {k[len("override_"):]: g[k].item() for k in g.files if k.startswith("override_")}
+    cfg = co.CGConfig(**over)
+    grid = tuple(int(v) for v in g["grid"])
+    out = co.sbl_cg(g["y"], grid, cfg, mask=g["mask"] if "mask" in g.files else None)
+    x, xr = out["x"].reshape(-1), g["cg_x"].reshape(-1)
+    rel = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    print(case, "pcg", out["cg_iterations"][:8], list(g["cg_history_cg_iterations"][:8]), "rel", rel)
+    assert out["iterations"] == int(g["cg_iterations"])
+    assert rel < 1e-3
+
+
+def test_circulant_preconditioner_is_the_strang_approximation():
+    """The eigenvalues are those of the circulant matrix whose first column
+    holds Q's lags (k <= M/2 ? k : k - M): built densely on a small grid and
+    diagonalised, against the FFT form; the apply inverts it."""
+    grid, samples = (12, 10), (7, 6)
+    rng = np.random.default_rng(9)
+    gamma = rng.exponential(size=(1, 120))
+    noise = np.array([0.3])
+    op = co.Operator(grid, samples)
+    _, eig, _ = co.circulant_preconditioner(gamma, noise, op, F32_EPS)
+    a = dense_prefix_matrix(grid, samples)
+    q = (a * gamma[0]) @ a.conj().T + noise[0] * np.eye(42)
+    m1, m2 = samples
+    # Strang: wrap Q's first-column lags onto the 7 x 6 torus
+    col = np.zeros((m1, m2), complex)
+    for k1 in range(m1):
+        for k2 in range(m2):
+            d1 = k1 if k1 <= m1 // 2 else k1 - m1
+            d2 = k2 if k2 <= m2 // 2 else k2 - m2
+            if d1 >= 0 and d2 >= 0:
+                col[k1, k2] = q[d1 * m2 + d2, 0]
+            else:
+                # lag d of Q = A Gamma A^H: q[(m + d), m] for a row m where both are inside
+                r1, r2 = max(-d1, 0), max(-d2, 0)
+                col[k1, k2] = q[(r1 + d1) * m2 + (r2 + d2), r1 * m2 + r2]
+    want = np.real(np.fft.fft2(col))
+    want = np.maximum(want, max(F32_EPS * np.abs(want).max(), noise[0]))
+    np.testing.assert_allclose(eig[0], want, rtol=1e-10, atol=1e-10 * np.abs(want).max())
+    v = rng.standard_normal((1, 42, 2)) + 1j * rng.standard_normal((1, 42, 2))
+    z = co.apply_preconditioner(v, ("circulant", eig, op))
+    back = np.fft.ifft2(np.fft.fft2(z.reshape(1, m1, m2, 2), axes=(1, 2)) * eig[..., None], axes=(1, 2))
+    np.testing.assert_allclose(back.reshape(1, 42, 2), v, atol=1e-10)
