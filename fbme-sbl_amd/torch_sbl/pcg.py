@@ -219,8 +219,9 @@ class _Solver:
         self.shared = self.joint and bool(cfg.joint_noise_variance)
         if self.joint and self.b < 2:
             raise ValueError("joint_sparsity=True requires at least two leading batch items.")
-        if self.joint and self.dense:
-            raise NotImplementedError("mode='dense' with joint_sparsity is not implemented on MI355X.")
+        # mode="dense" with joint sparsity: the shared covariance factored per
+        # snapshot is the reference's one factorisation with many right-hand
+        # sides (_shared_mmv_posterior, solver.py:3039-3080)
         self.nsys = 1 if self.shared else self.b
         self.t = _tables(self.grid, self.samples, self.dev)
         self.p = 0 if self.dense else _probe_count(self.n, self.m_obs, cfg)

@@ -484,6 +484,30 @@ PRECOND_CASES = [
 ]
 
 
+def dense_joint(reference_root: str) -> None:
+    """``sbl_2d(mode='dense', joint_sparsity=True)``: three 32x32 / 24x24
+    snapshots sharing a 12-atom support, shared and per-snapshot noise,
+    complex64 (30 iterations)."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    for name, seed, shared in (("dense_joint_32x32_24x24_b3", 1040, True),
+                               ("dense_joint_32x32_24x24_b3_noise", 1041, False)):
+        batch = scenes.joint_scenes(seed, 3, (32, 32), (24, 24), k=12, snr_db=20.0)
+        y = np.stack([s.y for s in batch]).astype(np.complex64)
+        cfg = SBLConfig(device="cpu", mode="dense", joint_sparsity=True, joint_noise_variance=shared, max_iter=30)
+        res = sbl_2d(torch.from_numpy(y), (32, 32), config=cfg)
+        arrays = dict(y=y, x_true=np.stack([s.x for s in batch]), grid=np.array((32, 32)),
+                      samples=np.array((24, 24)), joint_noise_variance=np.array(shared))
+        arrays.update(_result_arrays(res, "dense_"))
+        for key in ("relative_change", "noise_variance"):
+            arrays["dense_history_" + key] = np.array(res.history.get(key, []))
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "iterations", res.iterations, flush=True)
+
+
 def preconditioners(reference_root: str) -> None:
     """``sbl_2d`` in mode 'cg' with cg_preconditioner='circulant' (the Strang
     BCCB preconditioner, operators.py:373-416) and 'none'."""
@@ -513,7 +537,9 @@ def preconditioners(reference_root: str) -> None:
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "precond":
+    if which == "dense_joint":
+        dense_joint(root)
+    elif which == "precond":
         preconditioners(root)
     elif which == "active":
         active_set(root)
@@ -521,11 +547,13 @@ if __name__ == "__main__":
         generic(root)
         active_set(root)
         preconditioners(root)
+        dense_joint(root)
     elif which == "headline":
         headline(root)
         generic(root)
         active_set(root)
         preconditioners(root)
+        dense_joint(root)
     elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
@@ -546,3 +574,4 @@ if __name__ == "__main__":
         generic(root)
         active_set(root)
         preconditioners(root)
+        dense_joint(root)

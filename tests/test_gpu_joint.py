@@ -30,7 +30,8 @@ def _rel(a, b):
 
 def _cfg(device, **kw):
     from torch_sbl import SBLConfig
-    return SBLConfig(mode="cg", device=str(device), joint_sparsity=True, **kw)
+    kw.setdefault("mode", "cg")
+    return SBLConfig(device=str(device), joint_sparsity=True, **kw)
 
 
 def _compare(x, ref, truth, grid):
@@ -106,3 +107,33 @@ def test_joint_requires_two_snapshots(device):
     g = np.load(golden_path("general_32x32_24x24.npz"))
     with pytest.raises(ValueError):
         sbl_2d(torch.from_numpy(g["y"]).to(device), (32, 32), config=_cfg(device, max_iter=2))
+
+
+@pytest.mark.parametrize("name", ["dense_joint_32x32_24x24_b3", "dense_joint_32x32_24x24_b3_noise"])
+def test_dense_joint_matches_reference(device, name):
+    """mode='dense' with joint_sparsity (shared and per-snapshot noise): the
+    exact covariance factored per snapshot, which the reference notes is the
+    same as its one factorisation with many right-hand sides
+    (solver.py:3039-3080), and the M-SBL update, against the reference's
+    30-iteration runs.  Achieved: x within 2.1e-5 .. 3.4e-5, noise history
+    within 5.2e-5 (shared) and 1.1e-4 (per snapshot); the reference factors
+    Q in complex64, this build in FP64, so the noise is held to 2e-4."""
+    from fbme_amd import scenes
+    from torch_sbl import sbl_2d
+    print("golden", name, flush=True)
+    g = np.load(golden_path(name + ".npz"))
+    shared = bool(g["joint_noise_variance"])
+    res = sbl_2d(torch.from_numpy(g["y"]).to(device), (32, 32),
+                 config=_cfg(device, mode="dense", joint_noise_variance=shared, max_iter=30))
+    assert res.mode == "dense" and res.iterations == int(g["dense_iterations"])
+    x = res.x.cpu().numpy().reshape(3, -1)
+    ref, truth = g["dense_x"].reshape(3, -1), g["x_true"].reshape(3, -1)
+    hn = np.asarray(res.history["noise_variance"])
+    rn = g["dense_history_noise_variance"]
+    print(name, "rel", [_rel(x[i], ref[i]) for i in range(3)], "noise history",
+          float(np.max(np.abs(hn / rn - 1))), flush=True)
+    for i in range(3):
+        assert _rel(x[i], ref[i]) < 1e-4
+        assert abs(scenes.nmse_db(x[i], truth[i]) - scenes.nmse_db(ref[i], truth[i])) < 0.01
+    np.testing.assert_allclose(hn, rn, rtol=2e-4)
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), g["dense_noise_variance"], rtol=2e-4)
