@@ -29,6 +29,46 @@ sys.path.insert(0, os.path.join(ROOT, "fbme-sbl_amd"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+# sources of the row pass whose PMC traffic profiles/row_pass_traffic.json holds
+ROW_PASS_SOURCES = ("fbme-sbl_amd/csrc/fbme_uamp16.hip", "fbme-sbl_amd/csrc/fbme_fft16.h",
+                    "fbme-sbl_amd/csrc/fbme_device.h", "fbme-sbl_amd/csrc/fbme_types.h")
+
+
+def sources_sha256(paths) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in paths:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def archive_latency(fn, start, stop, samples=20):
+    """The archive protocol (zip:examples/rtx4090_typical/PROTOCOL.md:22-33):
+    `samples` event-timed calls, P50 = mean of the sorted 10th and 11th,
+    P95 = the maximum (ms)."""
+    import torch
+    times = []
+    for _ in range(samples):
+        start.record()
+        fn()
+        stop.record()
+        torch.cuda.synchronize()
+        times.append(start.elapsed_time(stop))
+    times.sort()
+    mid = len(times) // 2
+    return {"p50_ms": 0.5 * (times[mid - 1] + times[mid]), "p95_ms": times[-1], "samples": len(times)}
 
 
 def parse():
@@ -538,6 +578,7 @@ def main():
     x_batched = plan.result.x.detach().cpu().numpy().reshape(b, -1)
     nmse_batched = float(np.mean([scenes.nmse_db(x_batched[i], truth[i].reshape(-1))
                                   for i in range(b)]))
+    batched_latency = archive_latency(lambda: plan.replay(data), start, stop)
     print(f"[bench] batched: {ms_per_step:.3f} ms/step, {value:.1f} scenes/s",
           file=sys.stderr, flush=True)
 
@@ -563,7 +604,8 @@ def main():
                                         for i in range(min(b, 16))]))
         plain = dict(ms_per_scene=plain_ms, scenes_per_s=1e3 / plain_ms,
                      nmse_db=float(np.mean(nmse_single)),
-                     batched_nmse_db_same_scenes=nmse_batched16)
+                     batched_nmse_db_same_scenes=nmse_batched16,
+                     latency=archive_latency(lambda: single.replay(data[0]), start, stop))
         print(f"[bench] plain single-scene: {plain_ms:.3f} ms/scene",
               file=sys.stderr, flush=True)
 
@@ -593,8 +635,10 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             t = json.load(fh)
+        # only a figure measured on these exact row-pass sources counts
         if (t.get("batch") == b and t.get("workload") == "256x256/160x160"
-                and t.get("kernel") == kname):
+                and t.get("kernel") == kname
+                and t.get("source_sha256") == sources_sha256(ROW_PASS_SOURCES)):
             traffic = t.get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": traffic,
@@ -616,34 +660,59 @@ def main():
         torch.cuda.synchronize()
         return nbytes / (start.elapsed_time(stop) / reps / 1e3) / 1e9
 
+    import ctypes
+    lib = _lib.load()
+    cur = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
     src = torch.empty(1 << 28, dtype=torch.float32, device=device).uniform_()
     dst = torch.empty_like(src)
     gib = src.numel() * src.element_size()
-    hbm_reference = {"copy_GB_per_s": stream_gbs(lambda: dst.copy_(src), 2 * gib),
-                     "write_GB_per_s": stream_gbs(lambda: dst.fill_(1.0), gib),
-                     "read_GB_per_s": stream_gbs(lambda: src.sum(), gib),
+    sp, dp = ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr())
+    own = {}
+    for unroll in (1, 2, 4, 8):
+        own[unroll] = (stream_gbs(lambda: _lib.check(lib.fbme_stream_copy(cur, sp, dp, gib, unroll), "copy"),
+                                  2 * gib),
+                       stream_gbs(lambda: _lib.check(lib.fbme_stream_fill(cur, dp, gib, 1.0, unroll), "fill"),
+                                  gib))
+    best = max(own, key=lambda k: own[k][0])
+    hbm_reference = {"copy_GB_per_s": own[best][0], "write_GB_per_s": max(v[1] for v in own.values()),
+                     "kernel": "k_stream_copy / k_stream_fill (libfbme_hip, one pass, 16-byte accesses; "
+                               f"best of 1/2/4/8 in flight per thread: copy at {best})",
+                     "copy_GB_per_s_by_unroll": {str(k): v[0] for k, v in own.items()},
+                     "torch_copy_GB_per_s": stream_gbs(lambda: dst.copy_(src), 2 * gib),
                      "buffer_bytes": gib}
     roofline["frac_of_copy"] = achieved / 1e9 / hbm_reference["copy_GB_per_s"]
     del src, dst
 
     if args.calibrate_copy:
+        # a known-size copy for the PMC calibration: 256 MiB read, 256 MiB written
         src = torch.empty(64 << 20, dtype=torch.float32, device=device)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        _lib.check(lib.fbme_stream_copy(cur, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                        src.numel() * 4, 1), "calibration copy")
         torch.cuda.synchronize()
 
     # ----------------------------------------------------------- CPU baseline
     cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import fbme_oracle as fo
+    if not args.no_cpu_baseline and args.profile == "fast":
+        from oracle import torch_cpu_plan as tcp
+        threads = torch.get_num_threads()
+        host = torch.from_numpy(np.stack([s.y for s in batch]))
+        tcp.fast_plan(host[:8], grid)                      # warm-up
         count, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < args.cpu_seconds and count < b:
-            fo.fbme_plan(batch[count].y, grid, args.profile)
-            count += 1
+        while time.perf_counter() - t0 < args.cpu_seconds:
+            first = count % max(b - 7, 1)
+            tcp.fast_plan(host[first:first + 8], grid)
+            count += 8
         el = time.perf_counter() - t0
-        cpu = {"value": count / el, "unit": "scenes/s", "cores": 1, "kind": "port",
-               "sample": f"{count} scenes of the same 256x256/160x160 workload, "
-                         "float64 NumPy oracle (single-threaded pocketfft)"}
+        cpu = {"value": count / el, "unit": "scenes/s", "cores": threads, "kind": "port",
+               "os_cpu_count": os.cpu_count(), "torch_threads": threads, "cpu_model": cpu_model(),
+               "sample": f"{count} scenes ({el:.1f} s) of the same 256x256/160x160 workload (8 per call), FBME-Fast "
+                         "restated on torch's CPU kernels (complex64 sweep, FP64 polish; "
+                         "oracle/torch_cpu_plan.py), every host thread"}
+        ref_path = os.path.join(ROOT, "tests", "golden", "reference_cpu_timings.json")
+        if os.path.exists(ref_path):
+            with open(ref_path) as fh:
+                cpu["reference_in_build_container"] = json.load(fh)
 
     uneven = None
     if not args.skip_uneven:
@@ -696,6 +765,7 @@ def main():
         "roofline": roofline,
         "hbm_reference": hbm_reference,
         "cpu_baseline": cpu,
+        "batched_latency": batched_latency,
         "plain_single_scene": plain,
         "speedup_vs_plain": (value / plain["scenes_per_s"]) if plain else None,
         "nmse_db_batched": nmse_batched,

@@ -256,9 +256,78 @@ k_joint_noise_finish(const double* __restrict__ partial, int nblocks, int batch,
 
 inline int nblk(int n) { return (n + FBME_BLOCK - 1) / FBME_BLOCK; }
 
+// Plain HBM streams (the measured ceiling the kernels are compared with):
+// one pass over the buffer, U independent 16-byte accesses in flight per
+// thread, coalesced across the workgroup.
+template <int U>
+__global__ void __launch_bounds__(256) k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                     long long n) {
+  const long long base = (long long)blockIdx.x * 256 * U + threadIdx.x;
+  float4 v[U];
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const long long i = base + 256 * k;
+    if (i < n) v[k] = src[i];
+  }
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const long long i = base + 256 * k;
+    if (i < n) dst[i] = v[k];
+  }
+}
+
+template <int U>
+__global__ void __launch_bounds__(256) k_stream_fill(float4* __restrict__ dst, long long n, float value) {
+  const long long base = (long long)blockIdx.x * 256 * U + threadIdx.x;
+  const float4 f = make_float4(value, value, value, value);
+#pragma unroll
+  for (int k = 0; k < U; ++k) {
+    const long long i = base + 256 * k;
+    if (i < n) dst[i] = f;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int fbme_stream_copy(void* stream, const void* src, void* dst, long long bytes, int unroll) {
+  // bytes: a multiple of 16; src, dst 16-byte aligned and disjoint; unroll 1, 2, 4 or 8.
+  if (!src || !dst || bytes <= 0 || (bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15))
+    return FBME_ERR_ARG;
+  const long long n = bytes / 16;
+  hipStream_t st = (hipStream_t)stream;
+#define FBME_COPY(U)                                                                                         \
+  hipLaunchKernelGGL(k_stream_copy<U>, dim3((unsigned)((n + 256 * U - 1) / (256 * U))), dim3(256), 0, st,   \
+                     (const float4*)src, (float4*)dst, n)
+  switch (unroll) {
+    case 1: FBME_COPY(1); break;
+    case 2: FBME_COPY(2); break;
+    case 4: FBME_COPY(4); break;
+    case 8: FBME_COPY(8); break;
+    default: return FBME_ERR_ARG;
+  }
+#undef FBME_COPY
+  return (int)hipGetLastError();
+}
+
+int fbme_stream_fill(void* stream, void* dst, long long bytes, float value, int unroll) {
+  if (!dst || bytes <= 0 || (bytes & 15) || ((uintptr_t)dst & 15)) return FBME_ERR_ARG;
+  const long long n = bytes / 16;
+  hipStream_t st = (hipStream_t)stream;
+#define FBME_FILL(U)                                                                                         \
+  hipLaunchKernelGGL(k_stream_fill<U>, dim3((unsigned)((n + 256 * U - 1) / (256 * U))), dim3(256), 0, st,   \
+                     (float4*)dst, n, value)
+  switch (unroll) {
+    case 1: FBME_FILL(1); break;
+    case 2: FBME_FILL(2); break;
+    case 4: FBME_FILL(4); break;
+    case 8: FBME_FILL(8); break;
+    default: return FBME_ERR_ARG;
+  }
+#undef FBME_FILL
+  return (int)hipGetLastError();
+}
 
 unsigned fbme_radix_code(int len) {
   if (len < 1 || len > 1024) return 0xFFFFFFFFu;

@@ -72,6 +72,8 @@ _LOAD_ERROR: Exception | None = None
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "fbme_abi_version": (c_int, []),
+    "fbme_stream_copy": (c_int, [c_void_p, c_void_p, c_void_p, c_longlong, c_int]),
+    "fbme_stream_fill": (c_int, [c_void_p, c_void_p, c_longlong, c_float, c_int]),
     "fbme_default_options": (None, [POINTER(FbmePlanOptions), c_int]),
     "fbme_plan_create": (c_void_p, [c_int, POINTER(c_int),
                                     POINTER(FbmePlanOptions), c_char_p, c_int]),

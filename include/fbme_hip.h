@@ -111,6 +111,14 @@ enum {
 
 int fbme_abi_version(void);
 
+/* Plain HBM streams, one pass with `unroll` (1, 2, 4 or 8) 16-byte
+ * accesses in flight per thread: the ceiling the kernels' bandwidth is
+ * compared with, and the known-size kernels that calibrate hardware byte
+ * counters.  No counterpart in the reference (measurement only).  bytes: a
+ * multiple of 16; pointers 16-byte aligned; copy buffers disjoint. */
+int fbme_stream_copy(void* stream, const void* src, void* dst, long long bytes, int unroll);
+int fbme_stream_fill(void* stream, void* dst, long long bytes, float value, int unroll);
+
 /* Defaults of the Fast (stable = 0) or Stable (stable = 1) profile. */
 void fbme_default_options(FbmePlanOptions* options, int stable);
 

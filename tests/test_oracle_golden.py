@@ -151,3 +151,25 @@ def test_truncation_bound_is_the_dense_tail_covariance_norm():
     assert norm <= bound * noise[0] * (1 + 1e-12)
     assert bound * noise[0] <= 2.0 * norm          # the embedding is not loose here
     assert fo.truncation_delta_bound(np.zeros((1, 120)), noise, sup, grid, samples)[0] == 0.0
+
+
+@pytest.mark.parametrize("case", ["plan_64x64_40x40", "plan_96x80_60x48", "headline_256x256_160x160_s1001"])
+def test_torch_cpu_baseline_port_matches_oracle(case):
+    """The benchmark's CPU baseline (oracle/torch_cpu_plan.py: the Fast plan
+    on torch's CPU kernels, complex64 sweep) against the float64 oracle."""
+    import torch
+    from oracle import torch_cpu_plan as tcp
+    if case.startswith("headline"):
+        _, scene = headline_golden.load(case)
+        y, grid = scene.y, scene.grid
+    else:
+        g = np.load(golden_path(case + ".npz"))
+        y, grid = g["y"], tuple(int(v) for v in g["grid"])
+    out = tcp.fast_plan(torch.from_numpy(np.stack([y, y]))[:1], grid)
+    ref = fo.fbme_plan(y, grid, "fast")
+    x = out["x"].numpy().reshape(-1)
+    rel = _rel(x, ref["x"])
+    print(case, "torch CPU port vs oracle", rel)
+    assert rel < 1e-4
+    assert set(np.flatnonzero(x)) == set(np.flatnonzero(ref["x"]))
+    assert abs(float(out["noise"][0]) / float(ref["noise_variance"]) - 1) < 1e-4
