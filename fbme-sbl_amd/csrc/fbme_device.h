@@ -216,6 +216,34 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Sum over the 64 lanes, broadcast to every lane (same value as wave_sum).
 __device__ __forceinline__ double wave_sum_all(double v) { return __shfl(wave_sum(v), 0, 64); }
 
+// DPP move of a double between the lanes of a row of 16.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes, in every lane, by row-local DPP steps (quad
+// permutes [1,0,3,2] and [2,3,0,1], half-row mirror, row mirror: each step
+// adds two equal-shaped partial sums, so all lanes of a row hold the same
+// bits) and the four row sums added in lane order.  Deterministic; call with
+// all 64 lanes active.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x141>(v);
+  v += dpp_f64<0x140>(v);
+  auto lane_f64 = [&](int l) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)u, l);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  };
+  return ((lane_f64(0) + lane_f64(16)) + lane_f64(32)) + lane_f64(48);
+}
+
 __device__ __forceinline__ double block_sum(double v, double* scratch /* >= 4 */) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -270,13 +298,32 @@ __device__ __forceinline__ unsigned long long pack_key(uint32_t key, int pos) {
 }
 __device__ __forceinline__ int packed_pos(unsigned long long p) { return (int)(0xFFFFFFFFu - (uint32_t)p); }
 
+// DPP move of a 64-bit value between the lanes of a row of 16 (the same
+// dpp_ctrl on both halves).
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int lane) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Maximum over the 64 lanes, in every lane.  Four row-local DPP steps (quad
+// permutes [1,0,3,2] and [2,3,0,1], half-row mirror, row mirror) leave every
+// row of 16 lanes holding its maximum; the row maxima are read from lanes 0,
+// 16, 32 and 48.  Call with all 64 lanes active.
 __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(v, off, 64);
-    v = o > v ? o : v;
-  }
-  return v;
+  auto mx = [](unsigned long long a, unsigned long long b) { return a > b ? a : b; };
+  v = mx(v, dpp_u64<0xB1>(v));
+  v = mx(v, dpp_u64<0x4E>(v));
+  v = mx(v, dpp_u64<0x141>(v));
+  v = mx(v, dpp_u64<0x140>(v));
+  return mx(mx(readlane_u64(v, 0), readlane_u64(v, 16)), mx(readlane_u64(v, 32), readlane_u64(v, 48)));
 }
 
 // Top-k of the NPL packed keys held by each lane of one wavefront, by k

@@ -14,10 +14,14 @@
 //    is the product of two 1-D Dirichlet kernels, gathered from per-axis
 //    tables; C^H y is the backprojection at S; ||y - C mu||^2 follows from
 //    ||y||^2, mu^H C^H y and mu^H G mu.  No M-sized work remains.
-//  * the 32x32 reduced system is factorised in FP64 in LDS; evidence terms
-//    and the noise/energy scalars are FP64.
+//  * the 32x32 reduced system is inverted in FP64 registers by one
+//    wavefront, each lane owning a 4x4 block (Gauss-Jordan on the Hermitian
+//    form, rows exchanged through LDS without barriers); evidence terms and
+//    the noise/energy scalars are FP64.
 //  * Stable's |A^H r|^2 over the full grid is a separate multi-workgroup
 //    kernel: (A^H C mu)[n] = sum_k mu_k D1[n1-k1] D2[n2-k2].
+#include <type_traits>
+
 #include "fbme_device.h"
 
 using namespace fbme;
@@ -29,7 +33,8 @@ constexpr double F32_TINY_D = 1.17549435082228750797e-38;
 constexpr int KM = FBME_KMAX;
 constexpr int KP = KM + 1;   // padded row length in LDS
 constexpr int TE_MAX = 8;
-constexpr int DIR_LDS = 1024;   // double2 slots for the two Dirichlet tables
+constexpr int DIR_LDS = 2048;   // double2 slots for the two Dirichlet tables (n1, n2 <= 1024)
+constexpr int PT = 64;          // threads of a polish workgroup (one wavefront)
 
 struct PolishArgs {
   float margin_factor, jitter_multiplier, max_precision, polish_damping;
@@ -72,6 +77,12 @@ __device__ __forceinline__ double2 zmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 zconjmul(double2 a, double2 b) {  // conj(a) * b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
+// Maximum over the 64 lanes, in every lane (NaN entries are ignored).
+__device__ __forceinline__ double wave_max_all(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 __device__ __forceinline__ float key_score(uint32_t key) {
   if (key == 0u) return -INFINITY;
   return (key & 0x80000000u) ? __uint_as_float(key & 0x7FFFFFFFu) : __uint_as_float(~key);
@@ -90,6 +101,16 @@ struct SceneCtx {
   const float2* bp;
 };
 
+// Gram entry of lag (d1, d2), |d| < n, from the mask spectrum (MH) or the
+// Dirichlet product; no branch, so the gathers of a loop issue together.
+template <bool MH>
+__device__ __forceinline__ double2 lag_entry(const SceneCtx& c, int d1, int d2) {
+  d1 += (d1 < 0) ? c.n1 : 0;
+  d2 += (d2 < 0) ? c.n2 : 0;
+  if (MH) return c.mh[(long long)d1 * c.n2 + d2];
+  return zmul(c.d1[d1], c.d2[d2]);
+}
+
 __device__ __forceinline__ double2 lag_gram(const SceneCtx& c, int d1, int d2) {
   d1 += (d1 < 0) ? c.n1 : 0;
   d2 += (d2 < 0) ? c.n2 : 0;
@@ -97,23 +118,19 @@ __device__ __forceinline__ double2 lag_gram(const SceneCtx& c, int d1, int d2) {
   return zmul(c.d1[d1], c.d2[d2]);
 }
 
-__device__ __forceinline__ double2 gram_entry(const SceneCtx& c, int a, int b) {
-  return lag_gram(c, (a / c.n2) - (b / c.n2), (a % c.n2) - (b % c.n2));
-}
-
 // LDS workspace of one polish workgroup.
 struct PolishLds {
-  double2 a[KM][KP];
-  double2 x[KM][KP];
-  double2 g[KM][KP];
-  double2 rhs[KM];
-  double2 z[KM];
+  double2 row[2][8][KM];  // Gauss-Jordan: row j of step j from every column block (block
+                          // j >> 2 holds it), double-buffered by step parity
+  double2 part[8][KM];    // per-column-block partial sums
+  double2 rhs[KM];        // C^H y / nu on the support
+  double2 sv[KM];         // D^-1 rhs (zero beyond K)
   double2 mu[KM];
+  double dgl[KM];         // diagonal of the inverse
   double dsc[KM];
-  double idsc[KM];    // 1 / dsc
-  double ipiv[KM];    // 1 / diag(L)
-  double iwg[KM];     // 1 / gamma_S
-  int wk1[KM], wk2[KM];  // grid coordinates of the support
+  double idsc[KM];        // 1 / dsc
+  double iwg[KM];         // 1 / gamma_S
+  int wk1[KM], wk2[KM];   // grid coordinates of the support
   double sig[KM];
   double dd[KM];
   double red[16];
@@ -131,238 +148,306 @@ __device__ __forceinline__ void support_coords(const SceneCtx& c, const int* w, 
   }
 }
 
-// G[r][q] from the Dirichlet tables and the support coordinates.
-__device__ __forceinline__ double2 gram_rq(const SceneCtx& c, const PolishLds& L, int r, int q) {
-  return lag_gram(c, L.wk1[r] - L.wk1[q], L.wk2[r] - L.wk2[q]);
+// 1/x from the hardware reciprocal estimate refined by two Newton-Raphson
+// steps (e = 1 - x y, y += y e): within an ulp or two of the rounded
+// quotient for normal x > 0, which is all the elimination divides by.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
 }
 
 // Exact reduced posterior and negative log evidence of the zero-tail model
-// on support w with prior variances wg and noise nu (solver.py:3591-3779).
-// Leaves G(w) in L.g.  Results in L.mu / L.sig / L.dd; returns the nle.
-// Called by one wavefront (blockDim.x == 64); K <= 32.
-__device__ double evaluate(const SceneCtx& c, const int* w, const double* wg, double nu, PolishLds& L,
-                           float jitter_multiplier) {
-  const int K = c.k, tid = threadIdx.x;
-  const int lane = tid & 63;
+// on support w with prior variances wg and noise nu (solver.py:3591-3779):
+// S = diag(1/gamma) + G / nu, its Hermitian part, diagonal equilibration
+// D = sqrt(diag S), the deterministic jitter of _stable_cholesky_raw
+// (solver.py:2907-2920), mu = S^-1 rhs, Sigma_kk = (S^-1)_kk and
+// log det = M log nu + sum log gamma + log det S; wb is C^H y = bp at w.
+// Results in L.mu / L.sig / L.dd; returns the nle.  Called by one wavefront
+// (PT == 64), K <= 32.
+//
+// Element-parallel: lane l owns the 4x4 block of rows 4 (l >> 3) + i and
+// columns 4 (l & 7) + k of the equilibrated system (padded to 32 with the
+// identity), 16 complex doubles in registers.  The system is inverted in
+// place by Gauss-Jordan elimination without pivoting, which is stable for a
+// Hermitian positive definite matrix; its pivots are the squared diagonal of
+// the Cholesky factor, so log det is the sum of their logs and a pivot that
+// is not positive is the factorisation failure.  The registers hold H = S A,
+// with A the partly eliminated matrix and S = diag(s), s = -1 on eliminated
+// indices: H stays Hermitian, so row j is conj(column j), which the owners of
+// column j hold, and after all K steps H = -A^-1.  Step j replaces column j
+// by e_j and subtracts f w^T, with w = conj(H[:, j]) (w_j = 1) and
+// f = H[:, j] / p (f_j = 1 + 1/p).  Rows go through LDS, entry j carrying
+// (p, 1/p); every column block stores into a slot of its own.  The step is
+// software-pipelined: the owners of column j + 1 update it first and store
+// its row, the lane loads that row at once and the remaining updates cover
+// the round trip.  LDS operations of one wavefront complete in issue order,
+// so the exchange needs no barrier.
+__device__ __forceinline__ double evaluate(const SceneCtx& c, const int* w, const float2* wb, const double* wg,
+                                           double nu, PolishLds& L, float jitter_multiplier) {
+  const int K = c.k, lane = threadIdx.x;
+  const int br = lane >> 3, bc = lane & 7, r0 = br << 2, c0 = bc << 2;
   const double inu = 1.0 / nu;
+  const double2 g0 = lag_gram(c, 0, 0);
   support_coords(c, w, L);
-  if (tid < K) L.iwg[tid] = 1.0 / wg[tid];
-  __syncthreads();
-  // S = diag(1/gamma) + G / nu, two rows per pass (lanes 0-31, 32-63).
-  const int q = lane & 31, half = lane >> 5;
-#pragma unroll
-  for (int r0 = 0; r0 < KM; r0 += 2) {
-    const int r = r0 + half;
-    if (r < K && q < K) {
-      const double2 gv = gram_rq(c, L, r, q);
-      L.g[r][q] = gv;
-      double2 v = make_double2(gv.x * inu, gv.y * inu);
-      if (r == q) v.x += L.iwg[r];
-      L.a[r][q] = v;
+  if (lane < KM) {
+    double2 rh = make_double2(0.0, 0.0), sv = rh;
+    if (lane < K) {
+      const double iw = 1.0 / wg[lane];
+      L.iwg[lane] = iw;
+      const double d = sqrt(fmax(g0.x * inu + iw, F32_TINY_D));
+      const double id = 1.0 / d;
+      L.dsc[lane] = d;
+      L.idsc[lane] = id;
+      const float2 b = wb[lane];
+      rh = make_double2((double)b.x * inu, (double)b.y * inu);
+      sv = make_double2(rh.x * id, rh.y * id);
     }
+    L.rhs[lane] = rh;
+    L.sv[lane] = sv;
   }
   __syncthreads();
-  // Hermitian part.
+  // mean |diag| of the equilibrated system, for the jitter
+  double md = 0.0;
+  if (lane < K) {
+    const double id = L.idsc[lane];
+    md = fabs((g0.x * inu + L.iwg[lane]) * (id * id));
+  }
+  md = wave_sum_dpp(md);
+  const double jit = (double)jitter_multiplier * F32_EPS_D * fmax(md / K, F32_TINY_D);
+  double2 a[4][4];
+  // S entries of the lane's block (padded rows and columns read lag 0 and
+  // become the identity).
+  auto build = [&](auto mh) {
+    constexpr bool MH = decltype(mh)::value;
+    int q1[4], q2[4];
+    double iq[4];
 #pragma unroll
-  for (int r0 = 0; r0 < KM; r0 += 2) {
-    const int r = r0 + half;
-    if (r < K && q < K) {
-      if (r < q) {
-        const double2 u = L.a[r][q], l = L.a[q][r];
-        const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
-        L.a[r][q] = h;
-        L.a[q][r] = make_double2(h.x, -h.y);
-      } else if (r == q) {
-        L.a[r][r].y = 0.0;
+    for (int k = 0; k < 4; ++k) {   // reads in bounds for every q < 32; selects after
+      const int q = c0 + k;
+      const bool qv = q < K;
+      const int a1 = L.wk1[q], a2 = L.wk2[q];
+      const double ai = L.idsc[q];
+      q1[k] = qv ? a1 : 0;
+      q2[k] = qv ? a2 : 0;
+      iq[k] = qv ? ai : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = r0 + i;
+      const bool rv = r < K;
+      const int b1 = L.wk1[r], b2 = L.wk2[r];
+      const double bi = L.idsc[r], bw = L.iwg[r];
+      const int r1 = rv ? b1 : 0, r2 = rv ? b2 : 0;
+      const double ir = rv ? bi : 0.0;
+      const double iwr = rv ? bw : 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int q = c0 + k;
+        // G[r][q] above the diagonal, conj(G[q][r]) below: exactly Hermitian
+        const bool up = r < q;
+        double2 g = lag_entry<MH>(c, up ? r1 - q1[k] : q1[k] - r1, up ? r2 - q2[k] : q2[k] - r2);
+        g.y = up ? g.y : -g.y;
+        const double sc = ir * iq[k];
+        double2 v = make_double2(g.x * inu * sc, g.y * inu * sc);
+        if (r == q) v = make_double2((g0.x * inu + iwr) * sc + jit, 0.0);
+        if (!(rv && q < K)) v = make_double2(r == q ? 1.0 : 0.0, 0.0);
+        a[i][k] = v;
       }
     }
-  }
-  __syncthreads();
-  // Diagonal equilibration D = sqrt(diag).
-  if (tid < K) {
-    const double d = sqrt(fmax(L.a[tid][tid].x, F32_TINY_D));
-    L.dsc[tid] = d;
-    L.idsc[tid] = 1.0 / d;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r0 = 0; r0 < KM; r0 += 2) {
-    const int r = r0 + half;
-    if (r < K && q < K) {
-      const double sc = L.idsc[r] * L.idsc[q];
-      L.a[r][q] = make_double2(L.a[r][q].x * sc, L.a[r][q].y * sc);
-    }
-  }
-  __syncthreads();
-  // Deterministic loading of the GPU branch of _stable_cholesky_raw
-  // (solver.py:2907-2920): jitter_multiplier * eps_f32 * mean|diag|.
-  {
-    double md = (tid < K) ? fabs(L.a[tid][tid].x) : 0.0;
-    md = wave_sum_all(md);
-    const double mean_diag = fmax(md / K, F32_TINY_D);
-    const double jit = (double)jitter_multiplier * F32_EPS_D * mean_diag;
-    __syncthreads();
-    if (tid < K) L.a[tid][tid].x += jit;
-  }
-  __syncthreads();
-  // Cholesky in registers: lane r owns row r of the (padded to 32 with
-  // identity) equilibrated system; rows are exchanged by shuffles.
+  };
+  if (c.mh) build(std::true_type{});
+  else build(std::false_type{});
   int fail = 0;
-  {
-    double2 a[32];
+  double pr = 1.0;   // pivot of row `lane` (lane < K)
+  // Row j for step j, entry j carrying the pivot and its reciprocal.
+  auto put_row = [&](const int j) {
+    const int kk = j & 3;
+    const bool dg = bc == (j >> 2) && br == (j >> 2);   // a[kk][kk] is H[j][j]
+    const double ip = rcp_nr(a[kk][kk].x);
+    double2* slot = L.row[j & 1][bc];
 #pragma unroll
-    for (int qq = 0; qq < 32; ++qq) {
-      double2 v = make_double2(lane == qq ? 1.0 : 0.0, 0.0);
-      if (lane < K && qq < K) v = L.a[lane][qq];
-      a[qq] = v;
+    for (int i = 0; i < 4; ++i) {
+      double2 wv = make_double2(a[i][kk].x, -a[i][kk].y);
+      if (i == kk && dg) wv = make_double2(a[kk][kk].x, ip);
+      slot[r0 + i] = wv;
     }
-    __syncthreads();
+  };
+  double2 wr[4], wq[4], pp;
+  auto get_row = [&](const int j) {
+    const double2* rowb = L.row[j & 1][j >> 2];
 #pragma unroll
-    for (int j = 0; j < 32; ++j) {
-      const double d = __shfl(a[j].x, j, 64);
-      fail |= !(d > 0.0);
-      const double piv = sqrt(fmax(d, 0.0));
-      const double ip = 1.0 / piv;
-      if (lane > j) {
-        a[j].x *= ip;
-        a[j].y *= ip;
-      } else if (lane == j) {
-        a[j] = make_double2(piv, 0.0);
-        L.ipiv[j] = ip;
+    for (int i = 0; i < 4; ++i) wr[i] = rowb[r0 + i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wq[k] = rowb[c0 + k];
+    pp = rowb[j];
+  };
+  if (K > 0) {
+    put_row(0);
+    get_row(0);
+  }
+#pragma unroll
+  for (int j = 0; j < KM; ++j) {
+    if (j < K) {
+      const double p = pp.x, piv = pp.y;
+      fail |= !(p > 0.0);
+      if (lane == j) pr = p;
+      const int kj = j & 3;
+      // Owners of column j (k == kj) replace it by e_j and take w_j = 1:
+      // keep = 0 there and 1 elsewhere; on row j, f_j = 1 + 1/p.
+      const bool own = bc == (j >> 2);
+      const double keep = own ? 0.0 : 1.0, one = 1.0 - keep;
+      const double djj = (own && br == (j >> 2)) ? 1.0 : 0.0;   // this lane holds H[j][j]
+      double2 f[4], w[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) f[i] = make_double2(wr[i].x * piv, -wr[i].y * piv);
+      if (br == (j >> 2)) f[kj] = make_double2(1.0 + piv, 0.0);   // r == j
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = wq[k];
+      w[kj] = make_double2(fma(wq[kj].x, keep, one), wq[kj].y * keep);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i][kj].x = fma(a[i][kj].x, keep, i == kj ? djj : 0.0);
+        a[i][kj].y *= keep;
+      }
+      auto upd = [&](const int i, const int k) {   // a -= f w
+        a[i][k].x = fma(-f[i].x, w[k].x, fma(f[i].y, w[k].y, a[i][k].x));
+        a[i][k].y = fma(-f[i].x, w[k].y, fma(-f[i].y, w[k].x, a[i][k].y));
+      };
+      const int kn = (j + 1) & 3;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) upd(i, kn);
+      if (j + 1 < KM) {   // also past K (in bounds, never used): no branch
+        put_row(j + 1);
+        get_row(j + 1);   // overwrites wr / wq / pp: f and w hold this step's values
       }
 #pragma unroll
-      for (int qq = j + 1; qq < 32; ++qq) {
-        const double lx = __shfl(a[j].x, qq, 64), ly = __shfl(a[j].y, qq, 64);
-        if (lane >= qq) {  // a[r][q] -= l[r][j] * conj(l[q][j])
-          a[qq].x -= a[j].x * lx + a[j].y * ly;
-          a[qq].y -= a[j].y * lx - a[j].x * ly;
+      for (int i = 0; i < 4; ++i) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          if (k != kn) upd(i, k);
         }
       }
     }
-    if (lane < 32) {
-#pragma unroll
-      for (int qq = 0; qq < 32; ++qq) L.a[lane][qq] = a[qq];
-    }
   }
-  if (tid == 0) L.flag[0] = 0;
-  __syncthreads();
-  if (fail && lane < K) L.flag[0] = 1;
-  // X = L^-1, lane c owns column c (X[p][c] = 0 for p < c needs no guard).
-  double2 x[32];
+  // a holds -B, B = A^-1 (rows and columns beyond K: the identity):
+  // mu_eq = B (D^-1 rhs) from the per-block partial sums, Sigma_eq = diag B
+  // from the diagonal blocks.
+  {
+    double2 sv[4];
 #pragma unroll
-  for (int r = 0; r < 32; ++r) {
-    double2 acc = make_double2(r == lane ? 1.0 : 0.0, 0.0);
+    for (int k = 0; k < 4; ++k) sv[k] = L.sv[c0 + k];
 #pragma unroll
-    for (int p = 0; p < r; ++p) {
-      const double2 l = L.a[r][p];
-      acc.x -= l.x * x[p].x - l.y * x[p].y;
-      acc.y -= l.x * x[p].y + l.y * x[p].x;
-    }
-    const double ig = L.ipiv[r];
-    x[r] = make_double2(acc.x * ig, acc.y * ig);
-  }
-  double ev = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      double2 acc = make_double2(0.0, 0.0);
 #pragma unroll
-  for (int r = 0; r < 32; ++r) ev += x[r].x * x[r].x + x[r].y * x[r].y;
-  if (lane < 32) {
-#pragma unroll
-    for (int r = 0; r < 32; ++r) L.x[r][lane] = x[r];
-  }
-  if (tid < K) {
-    const float2 b = c.bp[w[tid]];
-    L.rhs[tid] = make_double2((double)b.x * inu, (double)b.y * inu);
-  }
-  __syncthreads();
-  // z = X (rhs / d), lane r owns row r.
-  if (tid < K) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int p = 0; p <= tid; ++p) {
-      const double2 sv = make_double2(L.rhs[p].x * L.idsc[p], L.rhs[p].y * L.idsc[p]);
-      const double2 t = zmul(L.x[tid][p], sv);
-      acc.x += t.x;
-      acc.y += t.y;
-    }
-    L.z[tid] = acc;
-  }
-  __syncthreads();
-  // mu_eq = X^H z and Sigma_eq = ||X[:, c]||^2, lane c owns column c.
-  double terms = 0.0, proj = 0.0;
-  if (tid < K) {
-    double2 acc = make_double2(0.0, 0.0);
-#pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      if (r < K) {
-        const double2 zr = L.z[r];
-        acc.x += x[r].x * zr.x + x[r].y * zr.y;
-        acc.y += x[r].x * zr.y - x[r].y * zr.x;
+      for (int k = 0; k < 4; ++k) {
+        acc.x -= a[i][k].x * sv[k].x - a[i][k].y * sv[k].y;
+        acc.y -= a[i][k].x * sv[k].y + a[i][k].y * sv[k].x;
       }
+      L.part[bc][r0 + i] = acc;
+      if (br == bc) L.dgl[r0 + i] = -a[i][i].x;
     }
-    const double id = L.idsc[tid];
-    const double2 mu = make_double2(acc.x * id, acc.y * id);
-    L.mu[tid] = mu;
-    const double sg = fmax(ev * id * id, 0.0);
-    L.sig[tid] = sg;
-    const double igk = L.iwg[tid];
-    L.dd[tid] = fmax((wg[tid] - sg) * igk * igk, 0.0);
-    // per-atom terms of log det and of the data projection
-    terms = log(wg[tid]) + 2.0 * (log(fmax(L.a[tid][tid].x, F32_TINY_D)) + log(L.dsc[tid]));
-    proj = L.rhs[tid].x * mu.x + L.rhs[tid].y * mu.y;
-  }
-  terms = wave_sum(terms);
-  proj = wave_sum(proj);
-  double nle = 0.0;
-  if (tid == 0) {
-    const double logdet = (double)c.m_total * log(nu) + terms;
-    const double quad = fmax(c.ye * inu - proj, 0.0);
-    nle = L.flag[0] ? NAN : logdet + quad;
-    L.red[0] = nle;
   }
   __syncthreads();
-  nle = L.red[0];
+  double terms = 0.0, proj = 0.0;
+  if (lane < K) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      acc.x += L.part[b][lane].x;
+      acc.y += L.part[b][lane].y;
+    }
+    const double id = L.idsc[lane];
+    const double2 mu = make_double2(acc.x * id, acc.y * id);
+    L.mu[lane] = mu;
+    const double sg = fmax(L.dgl[lane] * id * id, 0.0);
+    L.sig[lane] = sg;
+    const double igk = L.iwg[lane];
+    L.dd[lane] = fmax((wg[lane] - sg) * igk * igk, 0.0);
+    // per-atom terms of log det and of the data projection
+    terms = log(wg[lane]) + log(fmax(pr, F32_TINY_D)) + 2.0 * log(L.dsc[lane]);
+    const double2 rh = L.rhs[lane];
+    proj = rh.x * mu.x + rh.y * mu.y;
+  }
+  terms = wave_sum_dpp(terms);
+  proj = wave_sum_dpp(proj);
+  if (lane == 0) L.flag[0] = fail;
+  const double logdet = (double)c.m_total * log(nu) + terms;
+  const double quad = fmax(c.ye * inu - proj, 0.0);
+  const double nle = fail ? NAN : logdet + quad;
   __syncthreads();
   return nle;
 }
 
-// ||y - C mu||^2 from the Gram matrix in L.g and C^H y = bp[S].
-__device__ double residual_energy(const SceneCtx& c, const int* s, const double2* mu, PolishLds& L) {
-  const int K = c.k, tid = threadIdx.x;
-  if (tid < K) {
-    double2 gm = make_double2(0.0, 0.0);
-    for (int q = 0; q < K; ++q) {
-      const double2 t = zmul(L.g[tid][q], mu[q]);
+// ||y - C mu||^2 = ||y||^2 - 2 Re(mu^H C^H y) + mu^H G mu on support s, with
+// C^H y = sb (bp at s) and the Gram entries from the lag tables (lane l covers
+// row l & 31, columns 16 (l >> 5) .. 16 (l >> 5) + 15).  One wavefront.
+__device__ __forceinline__ double residual_energy(const SceneCtx& c, const int* s, const float2* sb,
+                                                  const double2* mu, PolishLds& L) {
+  const int K = c.k, lane = threadIdx.x;
+  const int r = lane & 31, q0 = (lane >> 5) << 4;
+  support_coords(c, s, L);
+  __syncthreads();
+  double2 gm = make_double2(0.0, 0.0);
+  auto row_sum = [&](auto mh) {   // padded entries read lag 0 and weigh 0
+    constexpr bool MH = decltype(mh)::value;
+    const bool rv = r < K;
+    const int b1 = L.wk1[r], b2 = L.wk2[r];   // reads in bounds for every index < 32
+    const int r1 = rv ? b1 : 0, r2 = rv ? b2 : 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int q = q0 + i;
+      const bool v = rv && q < K;
+      const int a1 = L.wk1[q], a2 = L.wk2[q];
+      const double2 mq = mu[q];
+      const double2 g = lag_entry<MH>(c, v ? r1 - a1 : 0, v ? r2 - a2 : 0);
+      const double2 m = v ? mq : make_double2(0.0, 0.0);
+      const double2 t = zmul(g, m);
       gm.x += t.x;
       gm.y += t.y;
     }
-    const float2 b = c.bp[s[tid]];
-    const double2 bb = make_double2(b.x, b.y);
-    const double quad = mu[tid].x * gm.x + mu[tid].y * gm.y;   // Re(conj(mu) G mu)
-    const double lin = mu[tid].x * bb.x + mu[tid].y * bb.y;    // Re(conj(mu) b)
-    L.sig[tid] = quad - 2.0 * lin;   // scratch, reused below
+  };
+  if (c.mh) row_sum(std::true_type{});
+  else row_sum(std::false_type{});
+  gm.x += __shfl_xor(gm.x, 32, 64);
+  gm.y += __shfl_xor(gm.y, 32, 64);
+  double v = 0.0;
+  if (lane < K) {
+    const float2 b = sb[lane];
+    const double quad = mu[lane].x * gm.x + mu[lane].y * gm.y;    // Re(conj(mu) G mu)
+    const double lin = mu[lane].x * (double)b.x + mu[lane].y * (double)b.y;   // Re(conj(mu) b)
+    v = quad - 2.0 * lin;
   }
+  v = wave_sum_dpp(v);
   __syncthreads();
-  double e = 0.0;
-  if (tid == 0) {
-    double acc = c.ye;
-    for (int r = 0; r < K; ++r) acc += L.sig[r];
-    L.red[1] = fmax(acc, 0.0);
-  }
-  __syncthreads();
-  e = L.red[1];
-  __syncthreads();
-  return e;
+  return fmax(c.ye + v, 0.0);
 }
 
-__device__ void load_gram(const SceneCtx& c, const int* s, PolishLds& L) {
-  const int K = c.k;
-  support_coords(c, s, L);
-  __syncthreads();
-  const int q = threadIdx.x & 31, half = threadIdx.x >> 5;
-#pragma unroll
-  for (int r0 = 0; r0 < KM; r0 += 2) {
-    const int r = r0 + half;
-    if (r < K && q < K) L.g[r][q] = gram_rq(c, L, r, q);
+// Copy n candidate (key, index) pairs into LDS, 16-byte loads when both
+// arrays are 16-byte aligned and n is a multiple of 4 (unrolled so that the
+// loads of a thread are in flight together).
+__device__ __forceinline__ void load_candidates(const uint32_t* __restrict__ key, const int* __restrict__ idx, int n,
+                                                uint32_t* lkey, int* lidx) {
+  if ((n & 3) == 0 && (((uintptr_t)key | (uintptr_t)idx) & 15) == 0) {
+    const uint4* k4 = reinterpret_cast<const uint4*>(key);
+    const int4* i4 = reinterpret_cast<const int4*>(idx);
+    uint4* lk4 = reinterpret_cast<uint4*>(lkey);
+    int4* li4 = reinterpret_cast<int4*>(lidx);
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n / 4; i += PT) {
+      lk4[i] = k4[i];
+      li4[i] = i4[i];
+    }
+  } else {
+#pragma unroll 8
+    for (int i = threadIdx.x; i < n; i += PT) {
+      lkey[i] = key[i];
+      lidx[i] = idx[i];
+    }
   }
-  __syncthreads();
 }
 
 // Sort k (key, idx) pairs descending by key, ascending idx on ties, by rank
@@ -383,7 +468,7 @@ __device__ void rank_sort_desc(const uint32_t* key, const int* idx, int k, uint3
 // from the mask spectrum at mh + mh_off (the unmasked instantiation keeps
 // the Dirichlet-product Gram with no branch).
 template <bool MASKED>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(PT)
 k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const float2* __restrict__ bp_all,
          const double* __restrict__ sc, const double2* __restrict__ dir, const double2* __restrict__ mh,
          const uint32_t* __restrict__ cand_key,
@@ -410,9 +495,9 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   __shared__ uint32_t sel_key[KM], srt_key[KM];
   __shared__ int sel_idx[KM], srt_idx[KM];
   __shared__ int work_s[KM], best_s[KM];
+  __shared__ float2 work_b[KM], best_b[KM];   // C^H y = bp at work_s / best_s
   __shared__ double work_gam[KM], best_gam[KM], best_sig[KM], best_dd[KM];
   __shared__ double2 best_mu[KM];
-  __shared__ double sh[16];
   __shared__ double2 dir_lds[DIR_LDS];
   __shared__ unsigned long long packed[KM];
 
@@ -430,15 +515,13 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
   c.ye = scs[SC_YE];
   c.mh = (MASKED && s.mh_off >= 0) ? mh + s.mh_off : nullptr;
   // Dirichlet tables staged in LDS (the Gram gathers hit them K^2 times per
-  // evaluation); grids wider than the staging area read them from global.
-  if (s.n1 + s.n2 <= DIR_LDS) {
-    for (int i = tid; i < s.n1; i += blockDim.x) dir_lds[i] = dir[s.dir1_off + i];
-    for (int i = tid; i < s.n2; i += blockDim.x) dir_lds[s.n1 + i] = dir[s.dir2_off + i];
+  // evaluation; plan grids have n1, n2 <= 1024, so both always fit).
+  {
+    const int nd = s.n1 + s.n2;
+#pragma unroll 8
+    for (int i = tid; i < nd; i += PT) dir_lds[i] = (i < s.n1) ? dir[s.dir1_off + i] : dir[s.dir2_off + (i - s.n1)];
     c.d1 = dir_lds;
     c.d2 = dir_lds + s.n1;
-  } else {
-    c.d1 = dir + s.dir1_off;
-    c.d2 = dir + s.dir2_off;
   }
   c.bp = bp_all + s.grid_off;
   __syncthreads();
@@ -463,39 +546,37 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     if (s.row_items <= 64) {
       // Every row tile left its K candidates sorted by (score desc, grid
       // index asc): merge the sorted lists, lane l owning list l, by K
-      // rounds of wave arg-max over the list heads.
-      for (int i = tid; i < ncand; i += blockDim.x) {
-        keys[i] = cand_key[s.cand_off + i];
-        kidx[i] = cand_idx[s.cand_off + i];
-      }
+      // rounds of wave arg-max over the list heads.  A lane loads its next
+      // entry when it wins, for its next win; lane r keeps the r-th winner.
+      load_candidates(cand_key + s.cand_off, cand_idx + s.cand_off, ncand, keys, kidx);
       __syncthreads();
-      auto entry = [&](int p) -> unsigned long long {
-        if (tid >= s.row_items || p >= K) return 0ull;
-        const uint32_t kk = keys[tid * K + p];
-        return kk ? pack_key(kk, kidx[tid * K + p]) : 0ull;
-      };
-      int ptr = 0;
-      unsigned long long head = entry(0), next = entry(1);
-      for (int r = 0; r < K; ++r) {
-        const unsigned long long w = wave_max_u64(head);
-        if (tid == 0) packed[r] = w;
-        if (w != 0ull && head == w) {
-          head = next;
-          ++ptr;
-          next = entry(ptr + 1);
+      {
+        const bool has = tid < s.row_items;
+        const int base = has ? tid * K : 0;
+        unsigned long long head = 0ull;
+        if (has && keys[base]) head = pack_key(keys[base], kidx[base]);
+        int ptr = 1;   // position of the loaded next entry
+        uint32_t nk = keys[base + min(1, K - 1)];
+        int ni = kidx[base + min(1, K - 1)];
+        unsigned long long mine = 0ull;
+        for (int r = 0; r < K; ++r) {
+          const unsigned long long w = wave_max_u64(head);
+          if (tid == r) mine = w;
+          if (w != 0ull && head == w) {   // a lane with a list (heads of the others are 0)
+            head = (ptr < K && nk) ? pack_key(nk, ni) : 0ull;
+            ++ptr;
+            nk = keys[base + min(ptr, K - 1)];
+            ni = kidx[base + min(ptr, K - 1)];
+          }
+        }
+        if (tid < K) {
+          srt_key[tid] = (uint32_t)(mine >> 32);
+          srt_idx[tid] = mine ? packed_pos(mine) : -1;
         }
       }
       __syncthreads();
-      if (tid < K) {
-        srt_key[tid] = (uint32_t)(packed[tid] >> 32);
-        srt_idx[tid] = packed[tid] ? packed_pos(packed[tid]) : -1;
-      }
-      __syncthreads();
     } else {
-      for (int i = tid; i < ncand; i += blockDim.x) {
-        keys[i] = cand_key[s.cand_off + i];
-        kidx[i] = cand_idx[s.cand_off + i];
-      }
+      load_candidates(cand_key + s.cand_off, cand_idx + s.cand_off, ncand, keys, kidx);
       __syncthreads();
       block_select_topk(keys, ncand, K, pos, hist, misc);
       if (tid < K) {
@@ -510,13 +591,15 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     // gamma_S without re-initialisation.
     double gs = 0.0;
     if (tid < K) {
-      work_s[tid] = srt_idx[tid];
       const int id = srt_idx[tid];
-      gs = (id >= 0 && id < n_grid) ? (double)gamma_all[s.grid_off + id] : 0.0;
+      const bool ok = id >= 0 && id < n_grid;
+      gs = ok ? (double)gamma_all[s.grid_off + id] : 0.0;
+      work_s[tid] = id;
+      work_b[tid] = ok ? c.bp[id] : make_float2(0.f, 0.f);
       work_gam[tid] = a.reinitialize ? fmax((double)key_score(srt_key[tid]), c.floor_v) : gs;
-      st->init_s[tid] = srt_idx[tid];
+      st->init_s[tid] = id;
     }
-    gs = wave_sum_all(gs);   // every lane builds rows of S with nu = noise + tail
+    gs = wave_sum_dpp(gs);
     if (!a.prune_tail) tail = fmax(scs[SC_GSUM] - gs, 0.0);
     work_noise = scs[SC_NOISE];
     __syncthreads();
@@ -544,14 +627,16 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
     const int nx = s.xtile_items * exch;
     if (nx <= 64 * 32) {
-      unsigned long long v[32];
+      {
+        unsigned long long v[32];
 #pragma unroll
-      for (int i = 0; i < 32; ++i) {
-        const int slot = tid + 64 * i;
-        v[i] = (slot < nx && xcand_key[s.xcand_off + slot] != 0u)
-                   ? pack_key(xcand_key[s.xcand_off + slot], slot) : 0ull;
+        for (int i = 0; i < 32; ++i) {
+          const int slot = tid + 64 * i;
+          v[i] = (slot < nx && xcand_key[s.xcand_off + slot] != 0u)
+                     ? pack_key(xcand_key[s.xcand_off + slot], slot) : 0ull;
+        }
+        wave_topk<32>(v, exch, packed);
       }
-      wave_topk<32>(v, exch, packed);
       __syncthreads();
       if (tid < exch) {
         const int slot = packed_pos(packed[tid]);
@@ -560,10 +645,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
       }
       __syncthreads();
     } else {
-      for (int i = tid; i < nx; i += blockDim.x) {
-        keys[i] = xcand_key[s.xcand_off + i];
-        kidx[i] = xcand_idx[s.xcand_off + i];
-      }
+      load_candidates(xcand_key + s.xcand_off, xcand_idx + s.xcand_off, nx, keys, kidx);
       __syncthreads();
       block_select_topk(keys, nx, exch, pos, hist, misc);
       if (tid < exch) {
@@ -611,11 +693,15 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     }
     __syncthreads();
     pending = misc[0] != 0;
+    if (tid < K) {
+      best_b[tid] = c.bp[best_s[tid]];
+      work_b[tid] = c.bp[work_s[tid]];
+    }
     __syncthreads();
   }
 
   for (;;) {
-    const double nle = evaluate(c, work_s, work_gam, work_noise + tail, L, a.jitter_multiplier);
+    const double nle = evaluate(c, work_s, work_b, work_gam, work_noise + tail, L, a.jitter_multiplier);
     bool accept = true;
     if (have_best && a.evidence_gate) {
       const double margin = (double)a.margin_factor * F32_EPS_D * (1.0 + fabs(nle) + fabs(best_nle));
@@ -626,6 +712,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     if (accept) {
       if (tid < K) {
         best_s[tid] = work_s[tid];
+        best_b[tid] = work_b[tid];
         best_gam[tid] = work_gam[tid];
         best_mu[tid] = L.mu[tid];
         best_sig[tid] = L.sig[tid];
@@ -639,17 +726,9 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     trace[pit] = best_nle;
     if (pit + 1 == a.evaluations) break;
     // Support-fixed EM update of gamma_S and sigma^2 (solver.py:4320-4391).
-    load_gram(c, best_s, L);
-    const double energy = residual_energy(c, best_s, best_mu, L);
-    double dof = 0.0;
-    if (tid == 0) {
-      for (int r = 0; r < K; ++r) {
-        dof += fmin(fmax(1.0 - best_sig[r] / fmax(best_gam[r], tiny), 0.0), 1.0);
-      }
-      sh[0] = dof;
-    }
-    __syncthreads();
-    dof = sh[0];
+    const double energy = residual_energy(c, best_s, best_b, best_mu, L);
+    double dof = (tid < K) ? fmin(fmax(1.0 - best_sig[tid] / fmax(best_gam[tid], tiny), 0.0), 1.0) : 0.0;
+    dof = wave_sum_dpp(dof);
     const double keep = a.polish_damping;
     if (tid < K) {
       const double m2 = best_mu[tid].x * best_mu[tid].x + best_mu[tid].y * best_mu[tid].y;
@@ -657,6 +736,7 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
       ng = fmin(fmax(ng, c.floor_v), (double)a.max_precision);
       work_gam[tid] = keep * best_gam[tid] + (1.0 - keep) * ng;
       work_s[tid] = best_s[tid];
+      work_b[tid] = best_b[tid];
     }
     // EM on nu = noise + tail, then the tail removed (solver.py:4342-4383)
     double upd;
@@ -693,36 +773,38 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
 
   // ------------------------------------------------------------- finalize
   // Projected relevance dof (solver.py:2081-2092, 3156-3181).
-  if (tid == 0) {
-    double raw = 0.0;
-    for (int r = 0; r < K; ++r) raw += fmin(fmax(best_gam[r] * best_dd[r], 0.0), 1.0);
+  {
+    const double rel = (tid < K) ? fmin(fmax(best_gam[tid] * best_dd[tid], 0.0), 1.0) : 0.0;
+    const double raw = wave_sum_dpp(rel);
     const double cap = (double)c.m_total * (1.0 - 10.0 * F32_EPS_D);
     const double scale = fmin(1.0, cap / fmax(raw, F32_EPS_D));
-    double dof = 0.0;
-    for (int r = 0; r < K; ++r) dof += fmin(fmax(best_gam[r] * best_dd[r], 0.0), 1.0) * scale;
-    const int oi = s.orig;
-    out_dof[oi] = (float)dof;
-    out_noise[oi] = (float)best_noise;
-    out_nle[oi] = (float)best_nle;
-    for (int i = 0; i < a.evaluations; ++i) out_trace[(long long)oi * a.evaluations + i] = (float)trace[i];
-    if (out_exchanges) out_exchanges[oi] = exchanges;
-    if (status) status[oi] = L.flag[0];
+    const double dof = wave_sum_dpp(rel * scale);
+    if (tid == 0) {
+      const int oi = s.orig;
+      out_dof[oi] = (float)dof;
+      out_noise[oi] = (float)best_noise;
+      out_nle[oi] = (float)best_nle;
+      for (int i = 0; i < a.evaluations; ++i) out_trace[(long long)oi * a.evaluations + i] = (float)trace[i];
+      if (out_exchanges) out_exchanges[oi] = exchanges;
+      if (status) status[oi] = L.flag[0];
+    }
   }
   if (a.diagnostics) {
     // Fixed-budget EM residuals (solver.py:3408-3496) and trace flags.
-    load_gram(c, best_s, L);
-    const double energy = residual_energy(c, best_s, best_mu, L);
+    const double energy = residual_energy(c, best_s, best_b, best_mu, L);
+    double gr = 0.0, dr = 0.0;
+    if (tid < K) {
+      const double g = fmax(best_gam[tid], tiny);
+      const double m2 = best_mu[tid].x * best_mu[tid].x + best_mu[tid].y * best_mu[tid].y;
+      double t = (m2 + best_sig[tid] + a.hyper_rate) / (1.0 + a.hyper_shape);
+      t = fmin(fmax(t, c.floor_v), (double)a.max_precision);
+      const double sc_ = fmax(fmax(fabs(g), fabs(t)), tiny);
+      gr = fabs(g - t) / sc_;
+      dr = fmin(fmax(1.0 - best_sig[tid] / g, 0.0), 1.0);
+    }
+    const double gres = wave_max_all(gr);
+    const double dof = wave_sum_dpp(dr);
     if (tid == 0) {
-      double gres = 0.0, dof = 0.0;
-      for (int r = 0; r < K; ++r) {
-        const double g = fmax(best_gam[r], tiny);
-        const double m2 = best_mu[r].x * best_mu[r].x + best_mu[r].y * best_mu[r].y;
-        double t = (m2 + best_sig[r] + a.hyper_rate) / (1.0 + a.hyper_shape);
-        t = fmin(fmax(t, c.floor_v), (double)a.max_precision);
-        const double sc_ = fmax(fmax(fabs(g), fabs(t)), tiny);
-        gres = fmax(gres, fabs(g - t) / sc_);
-        dof += fmin(fmax(1.0 - best_sig[r] / g, 0.0), 1.0);
-      }
       double nt;
       if (a.evidence_noise) {
         nt = (energy + a.noise_rate) / fmax((double)c.m_total - dof + a.noise_shape, F32_EPS_D);
@@ -954,7 +1036,7 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
 // the general solver (solver.py:3591-3779 with tail_loading=None,
 // 3897-3924).  One wavefront per scene; outputs over the full grid:
 // mean (0 off S), variance (gamma off S) and data diagonal (0 off S).
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(PT)
 k_reduced_posterior(int k, int n1, int n2, int observed, const int* __restrict__ support,
                     const float* __restrict__ gamma, const double* __restrict__ gamma_sum,
                     const float* __restrict__ noise, const float2* __restrict__ bp, const double* __restrict__ ye,
@@ -964,6 +1046,7 @@ k_reduced_posterior(int k, int n1, int n2, int observed, const int* __restrict__
                     int* __restrict__ status) {
   __shared__ PolishLds L;
   __shared__ int w[KM];
+  __shared__ float2 wb[KM];
   __shared__ double wg[KM];
   __shared__ double red[2];
   const int b = blockIdx.x;
@@ -973,6 +1056,7 @@ k_reduced_posterior(int k, int n1, int n2, int observed, const int* __restrict__
   if (tid < k) {
     w[tid] = support[b * k + tid];
     wg[tid] = (double)gamma[b * n + w[tid]];
+    wb[tid] = bp[b * n + w[tid]];
     gs = wg[tid];
   }
   gs = wave_sum(gs);
@@ -990,7 +1074,7 @@ k_reduced_posterior(int k, int n1, int n2, int observed, const int* __restrict__
   c.d2 = d2;
   c.mh = mh;
   c.bp = bp + b * n;
-  const double nle = evaluate(c, w, wg, nu, L, jitter_multiplier);
+  const double nle = evaluate(c, w, wb, wg, nu, L, jitter_multiplier);
   if (tid < k) {
     const long long g = b * n + w[tid];
     mean[g] = make_float2((float)L.mu[tid].x, (float)L.mu[tid].y);
@@ -1124,7 +1208,7 @@ int fbme_reduced_posterior_masked(void* stream, int batch, int k, int n1, int n2
   const long long total = (long long)batch * n1 * n2;
   hipLaunchKernelGGL(k_reduced_defaults, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, gamma,
                      (float2*)mean, var, dd, total);
-  hipLaunchKernelGGL(k_reduced_posterior, dim3(batch), dim3(64), 0, st, k, n1, n2, observed, support, gamma,
+  hipLaunchKernelGGL(k_reduced_posterior, dim3(batch), dim3(PT), 0, st, k, n1, n2, observed, support, gamma,
                      gamma_sum, noise, (const float2*)bp, ye, (const double2*)dir1, (const double2*)dir2,
                      (const double2*)mask_spectrum, jitter_multiplier, (float2*)mean, var, dd, status);
   return (int)hipGetLastError();
@@ -1154,15 +1238,15 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
   a.prune_tail = o->prune_tail;
   a.reinitialize = o->reinitialize;
   a.evidence_gate = o->evidence_gate;
-  // One wavefront per scene: the K x K work is latency bound, and with a
-  // single wave every barrier is cheap.
+  // One wavefront per scene: the K x K elimination is latency bound, and
+  // within one wavefront its row exchange needs no barrier.
   if (mh)
-    hipLaunchKernelGGL(k_polish<true>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
+    hipLaunchKernelGGL(k_polish<true>, dim3(nscenes), dim3(PT), 0, stream, scenes, nscenes, phase,
                        (const float2*)bp, sc, (const double2*)dir, (const double2*)mh, cand_key, cand_idx,
                        xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
                        out_nle, out_trace, out_exchanges, out_diag, status, gamma, a);
   else
-    hipLaunchKernelGGL(k_polish<false>, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, phase,
+    hipLaunchKernelGGL(k_polish<false>, dim3(nscenes), dim3(PT), 0, stream, scenes, nscenes, phase,
                        (const float2*)bp, sc, (const double2*)dir, (const double2*)nullptr, cand_key, cand_idx,
                        xcand_key, xcand_idx, (PolishState*)states, (PolishResult*)results, out_noise, out_dof,
                        out_nle, out_trace, out_exchanges, out_diag, status, gamma, a);
