@@ -150,28 +150,41 @@ def pcg_kernels(device, batch=8, reps=20):
             s.stream, L, s.r, *samples, pcg._ptr(s.factors), pcg._ptr(s.sinv), pcg._ptr(s.bg),
             pcg._ptr(s.lr_map), pcg._ptr(s.res), pcg._ptr(s.z), pcg._ptr(s.lr_buf), pcg._ptr(s.lr_part),
             pcg._ptr(s.corr), pcg._ptr(tw), pcg._ptr(s.pc_part), None)
-        up = lambda: lib.fbme_pcg_update_parts(  # noqa: E731
+        up = lambda: lib.fbme_pcg_update_parts_rr(  # noqa: E731
             s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.d), pcg._ptr(s.q), pcg._ptr(s.rho),
-            pcg._ptr(s.mv_part), np_, pcg._ptr(s.curv), pcg._ptr(s.improved))
+            pcg._ptr(s.mv_part), np_, pcg._ptr(s.curv), pcg._ptr(s.improved), pcg._ptr(s.rr_part))
+        # the solver's second half: projection, <r, z> without z, and the
+        # synthesis pass writing d = z + beta d (every lane does the same
+        # work whether or not its update is valid)
+        pcd = lambda: lib.fbme_lowrank_apply_fft_direction(  # noqa: E731
+            s.stream, L, s.r, *samples, pcg._ptr(s.factors), pcg._ptr(s.sinv), pcg._ptr(s.bg),
+            pcg._ptr(s.lr_map), pcg._ptr(s.res), pcg._ptr(s.best_x), pcg._ptr(s.lr_part), pcg._ptr(s.corr),
+            pcg._ptr(tw), pcg._ptr(s.rr_part), s.rr_blocks, pcg._ptr(s.rho), pcg._ptr(s.curv),
+            pcg._ptr(s.nrho), pcg._ptr(s.improved))
         di = lambda: lib.fbme_pcg_direction_parts(  # noqa: E731
             s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.nrho), pcg._ptr(s.curv),
             pcg._ptr(s.pc_part), np_, pcg._ptr(s.lr_part), pcg._ptr(s.improved))
+    # name: (call, algorithmic bytes, part of the solver iteration)
     calls = {
         "matvec": (mv if s.fused else (lambda: s.matvec(s.d, s.q, dot=s.curv)),
                    (24 * L * M + 32 * L * 256 * samples[1] + 4 * batch * 65536) if s.grid_op
-                   else (16 * L * M + 32 * L * m1 * e2 + 8 * batch * e1 * e2)),
+                   else (16 * L * M + 32 * L * m1 * e2 + 8 * batch * e1 * e2), True),
+        # the FFT form reads r twice (projection, synthesis) and writes z
         "precondition": (pc if s.fused else (lambda: s.precondition(s.res, s.z, dot=s.nrho)),
-                         (24 * L * M + 16 * L * 32 * samples[1]) if s.lr_fft else 16 * L * M),
-        "lane_dot": (lambda: s.lane_dot(s.res, s.res, s.rr), 8 * L * M),
+                         24 * L * M if s.lr_fft else 16 * L * M, not s.fused),
+        "lane_dot": (lambda: s.lane_dot(s.res, s.res, s.rr), 8 * L * M, False),
         "update": (up if s.fused else
                    (lambda: lib.fbme_pcg_update(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z), pcg._ptr(s.d),
                                                 pcg._ptr(s.q), pcg._ptr(s.rho), pcg._ptr(s.curv),
-                                                pcg._ptr(s.improved))), 48 * L * M),
+                                                pcg._ptr(s.improved))), 48 * L * M, True),
         "direction": (di if s.fused else
                       (lambda: lib.fbme_pcg_direction(s.stream, L, M, pcg._ptr(s.best_x), pcg._ptr(s.z),
                                                       pcg._ptr(s.nrho), pcg._ptr(s.nrho), pcg._ptr(s.improved),
-                                                      1)), 24 * L * M),
+                                                      1)), 24 * L * M, not s.fused),
     }
+    if s.fused:
+        # projection reads r; the synthesis reads r and d and writes d
+        calls["precondition_direction"] = (pcd, 32 * L * M, True)
     s.rho.fill_(1.0)
     s.curv.fill_(2.0)
     s.nrho.fill_(1.0)
@@ -179,7 +192,7 @@ def pcg_kernels(device, batch=8, reps=20):
     out = {"workload": f"{batch} scenes x 17 lanes (data + 16 probes), 256x256/160x160",
            "lanes": L, "matvec_path": "grid 256x256 (register FFT)" if s.grid_op else f"BTTB {e1}x{e2}"}
     total_ms = total_bytes = 0.0
-    for name, (fn, nbytes) in calls.items():
+    for name, (fn, nbytes, in_iter) in calls.items():
         for _ in range(3):
             fn()
         torch.cuda.synchronize()
@@ -192,9 +205,11 @@ def pcg_kernels(device, batch=8, reps=20):
         ms = start.elapsed_time(stop) / reps
         out[name] = {"ms": ms, "bytes": nbytes, "GB/s": nbytes / (ms / 1e3) / 1e9,
                      "frac": nbytes / (ms / 1e3) / HBM_PEAK}
-        weight = 1   # <r, r> at check iterations only; <d, Qd> and <r, z> ride in matvec / precondition
-        total_ms += weight * ms
-        total_bytes += weight * nbytes
+        # <r, r> and the separate preconditioner / direction run at check
+        # iterations only; <d, Qd> and <r, z> ride in the passes
+        if in_iter:
+            total_ms += ms
+            total_bytes += nbytes
     out["pcg_iteration"] = {"ms": total_ms, "bytes": total_bytes, "GB/s": total_bytes / (total_ms / 1e3) / 1e9,
                             "frac": total_bytes / (total_ms / 1e3) / HBM_PEAK,
                             "us_per_scene": total_ms * 1e3 / batch}
@@ -213,10 +228,11 @@ def pcg_kernels(device, batch=8, reps=20):
     stop.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / reps
-    nb = total_bytes - calls["lane_dot"][1]
+    nb = total_bytes
     out["solver_iteration"] = {"ms": ms, "bytes": nb, "GB/s": nb / (ms / 1e3) / 1e9,
                                "frac": nb / (ms / 1e3) / HBM_PEAK, "us_per_scene": ms * 1e3 / batch,
-                               "kernels": "grid matvec + update + preconditioner + direction (no check)"}
+                               "kernels": ("grid matvec + update + preconditioner/direction pass (no check)" if s.fused
+                                           else "matvec + update + preconditioner + direction (no check)")}
     return out
 
 

@@ -99,19 +99,23 @@ k_gop_col_forward(const float2* __restrict__ mid, const float2* __restrict__ d, 
   const bool act = col < m2;
   load_tw256(tw, twtab);
   const float2* src = mid + (long long)lane * 256 * m2 + col;
-  float2 v[16];
+  float2 v[16], dl[16];
+  const long long base = (long long)lane * m1 * m2 + col;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = act ? src[(long long)(t + 16 * j) * m2] : make_float2(0.f, 0.f);
+  for (int j = 0; j < 16; ++j) {
+    const int k = t + 16 * j;
+    v[j] = act ? src[(long long)k * m2] : make_float2(0.f, 0.f);
+    dl[j] = (act && k < m1) ? d[base + (long long)k * m2] : make_float2(0.f, 0.f);   // loaded before the transform
+  }
   __syncthreads();
   fft256<false, GC>(v, tw, xs + c, t);
   const float nv = noise[lane / rhs];
-  const long long base = (long long)lane * m1 * m2 + col;
   double acc = 0.0;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const int k = t + 16 * n;
     if (act && k < m1) {
-      const float2 dv = d[base + (long long)k * m2];
+      const float2 dv = dl[n];
       const float2 o = make_float2(v[n].x + nv * dv.x, v[n].y + nv * dv.y);
       out[base + (long long)k * m2] = o;
       if (DOT) acc += (double)dv.x * o.x + (double)dv.y * o.y;
@@ -193,19 +197,28 @@ k_lrf_map(const int* __restrict__ idx, int n2, int* __restrict__ map) {
   if (k == 0) m[64] = ns;
 }
 
-// W[lane, s, c] for the slot rows of the column IFFT of v[lane, :, c].
+// Projection onto the atoms, one workgroup per (16 columns, lane): the
+// column IFFT of v[lane, :, c] gives W[s, c] on the slot rows (kept in LDS),
+// then part[lane, block, k] = sum_{c in block} conj(V[c, k]) W[slot(k), c]
+// in FP64 over the block's columns in order (the blocks are summed by
+// k_lr_corr).
 __global__ void __launch_bounds__(16 * GC)
-k_lrf_col_project(const float2* __restrict__ v, const int* __restrict__ map, float2* __restrict__ w, int m1, int m2,
-                  int rhs, const float2* __restrict__ twtab) {
+k_lrf_col_project(const float2* __restrict__ v, const float2* __restrict__ tab, const int* __restrict__ map, int m1,
+                  int m2, int rhs, const float2* __restrict__ twtab, double2* __restrict__ part) {
+  // xs is the transform's transpose buffer; before and after the transform
+  // it holds the slot rows ws [RANK][GC] and the group sums red [8][RANK]
   __shared__ float2 xs[16 * 17 * GC];
   __shared__ float2 tw[256];
   __shared__ int rowslot[256];
+  float2(*ws)[GC] = reinterpret_cast<float2(*)[GC]>(xs);
+  double2(*red)[RANK] = reinterpret_cast<double2(*)[RANK]>(xs + RANK * GC);
   const int c = threadIdx.x % GC, t = threadIdx.x / GC;
   const int lane = blockIdx.y;
+  const int b = lane / rhs;
   const int col = blockIdx.x * GC + c;
   const bool act = col < m2;
-  rowslot[threadIdx.x] = map[(long long)(lane / rhs) * MAP_INTS + 65 + threadIdx.x];
-  load_tw256(tw, twtab);
+  const int* mb = map + (long long)b * MAP_INTS;
+  const float2* V = tab + ((long long)b * (m1 + m2) + m1) * RANK;
   const float2* src = v + (long long)lane * m1 * m2;
   float2 x[16];
 #pragma unroll
@@ -213,49 +226,44 @@ k_lrf_col_project(const float2* __restrict__ v, const int* __restrict__ map, flo
     const int i = t + 16 * j;
     x[j] = (act && i < m1) ? src[(long long)i * m2 + col] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
-  fft256<true, GC>(x, tw, xs + c, t);
-  if (act) {
-    float2* dst = w + (long long)lane * RANK * m2 + col;
-#pragma unroll
-    for (int n = 0; n < 16; ++n) {
-      const int sl = rowslot[t + 16 * n];
-      if (sl >= 0) dst[(long long)sl * m2] = x[n];
-    }
-  }
-}
-
-// Partial projections over 16 samples per workgroup: part[lane, chunk, k] =
-// sum_{m2 in chunk} conj(V[m2, k]) W[lane, slot(k), m2]; thread (g, k) takes
-// samples 2g, 2g + 1 of the chunk, the eight groups are added in FP64 (the
-// chunks are summed by k_lr_corr).
-constexpr int LRF_CHUNK = 16;
-
-__global__ void __launch_bounds__(256)
-k_lrf_row_project(const float2* __restrict__ w, const float2* __restrict__ tab, const int* __restrict__ map,
-                  int m1, int m2, int rhs, double2* __restrict__ part) {
-  __shared__ float2 red[8][RANK];
-  const int lane = blockIdx.y;
-  const int b = lane / rhs;
-  const int k = threadIdx.x & (RANK - 1), g = threadIdx.x >> 5;
-  const float2* V = tab + ((long long)b * (m1 + m2) + m1) * RANK;
-  const int sl = map[(long long)b * MAP_INTS + k];
-  const float2* wr = w + ((long long)lane * RANK + sl) * m2;
-  float ax = 0.f, ay = 0.f;
+  // thread (g, k) reduces atom k over columns 2g, 2g + 1 of the block
+  const int rk = threadIdx.x & (RANK - 1), rg = threadIdx.x >> 5;
+  const int slot_k = mb[rk];
+  float2 vk[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) {
-    const int c = blockIdx.x * LRF_CHUNK + 2 * g + q;
-    if (c < m2) {
-      const float2 vv = V[(long long)c * RANK + k], ww = wr[c];
-      ax += vv.x * ww.x + vv.y * ww.y;
-      ay += vv.x * ww.y - vv.y * ww.x;
-    }
+    const int c2 = blockIdx.x * GC + 2 * rg + q;
+    vk[q] = (c2 < m2) ? V[(long long)c2 * RANK + rk] : make_float2(0.f, 0.f);
   }
-  red[g][k] = make_float2(ax, ay);
+  rowslot[threadIdx.x] = mb[65 + threadIdx.x];
+  load_tw256(tw, twtab);
+  __syncthreads();
+  fft256<true, GC>(x, tw, xs + c, t);
+  __syncthreads();   // the transpose buffer is free again
+  for (int e = threadIdx.x; e < RANK * GC; e += 16 * GC) ws[e / GC][e % GC] = make_float2(0.f, 0.f);
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int sl = rowslot[t + 16 * n];
+    if (sl >= 0) ws[sl][c] = x[n];
+  }
+  __syncthreads();
+  {
+    double ax = 0.0, ay = 0.0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const float2 vv = vk[q], ww = ws[slot_k][2 * rg + q];
+      ax += (double)(vv.x * ww.x + vv.y * ww.y);
+      ay += (double)(vv.x * ww.y - vv.y * ww.x);
+    }
+    red[rg][rk] = make_double2(ax, ay);
+  }
   __syncthreads();
   if (threadIdx.x < RANK) {
-    double2 sum = make_double2(0.0, 0.0);
-    for (int q = 0; q < 8; ++q) {
+    const int k = threadIdx.x;
+    double2 sum = red[0][k];
+#pragma unroll
+    for (int q = 1; q < 8; ++q) {
       sum.x += red[q][k].x;
       sum.y += red[q][k].y;
     }
@@ -267,10 +275,19 @@ k_lrf_row_project(const float2* __restrict__ w, const float2* __restrict__ tab, 
 // T[s, c] = sum_{k in s} V[c, k] corr[lane, k] formed in LDS for the
 // workgroup's columns (each atom feeds one slot; fixed summation order),
 // then the column FFT of the slot rows; part[lane, block] = sum Re(conj(v) out).
+// DIR: z is not stored; the pass writes the PCG direction d = z + beta d
+// (0 for a lane whose update is not valid), beta = nrho / rho with nrho
+// already formed from the projection (k_lr_corr), the lane logic of
+// k_pcg_direction_parts.
+template <bool DIR>
 __global__ void __launch_bounds__(16 * GC)
 k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab, const int* __restrict__ map,
                 const float2* __restrict__ v, const float* __restrict__ bg, float2* __restrict__ out, int m1,
-                int m2, int rhs, const float2* __restrict__ twtab, double* __restrict__ part) {
+                int m2, int rhs, const float2* __restrict__ twtab, double* __restrict__ part,
+                const double* __restrict__ rho, const double* __restrict__ curv, const double* __restrict__ nrho,
+                const int* __restrict__ active) {
+  // xs is the transform's transpose buffer; before the transform it holds
+  // the phase-table slice vsm [GC][RANK] and the slot rows ts [RANK][GC]
   __shared__ float2 xs[16 * 17 * GC];
   __shared__ float2 tw[256];
   __shared__ double red[8];
@@ -278,8 +295,8 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
   __shared__ int perm[RANK];
   __shared__ int start[RANK + 1];
   __shared__ float2 cs[RANK];
-  __shared__ float2 vsm[GC][RANK];
-  __shared__ float2 ts[RANK][GC];
+  float2(*vsm)[RANK] = reinterpret_cast<float2(*)[RANK]>(xs);
+  float2(*ts)[GC] = reinterpret_cast<float2(*)[GC]>(xs + GC * RANK);
   const int c = threadIdx.x % GC, t = threadIdx.x / GC;
   const int lane = blockIdx.y;
   const int b = lane / rhs;
@@ -293,6 +310,16 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
     cs[threadIdx.x] = corr[(long long)lane * RANK + threadIdx.x];
   }
   if (threadIdx.x <= RANK) start[threadIdx.x] = mb[MAP_START + threadIdx.x];
+  // the input column (and the direction) are loaded first: their latency
+  // overlaps the slot-row build
+  const long long base = (long long)lane * m1 * m2 + col;
+  float2 vin[16], dl[16];
+#pragma unroll
+  for (int n = 0; n < 16; ++n) {
+    const int k = t + 16 * n;
+    vin[n] = (act && k < m1) ? v[base + (long long)k * m2] : make_float2(0.f, 0.f);
+    if (DIR) dl[n] = (act && k < m1) ? out[base + (long long)k * m2] : make_float2(0.f, 0.f);
+  }
   for (int e = threadIdx.x; e < GC * RANK; e += 16 * GC) {
     const int cc = e / RANK, k = e - cc * RANK;
     const int c2 = blockIdx.x * GC + cc;
@@ -324,19 +351,31 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
   __syncthreads();
   fft256<false, GC>(x, tw, xs + c, t);
   const float ib = 1.0f / bg[b];
-  const long long base = (long long)lane * m1 * m2 + col;
+  float beta = 0.f;
+  bool vu = false;
+  if (DIR) {
+    const double nr = nrho[lane], rh = rho[lane], cv = curv[lane];
+    const double tiny = 1.17549435082228750797e-38;
+    vu = active[lane] != 0 && isfinite(cv) && isfinite(rh) && cv > tiny && rh > tiny && isfinite(nr) && nr > tiny;
+    beta = vu ? (float)(nr / fmax(rh, tiny)) : 0.f;
+  }
   double acc = 0.0;
 #pragma unroll
   for (int n = 0; n < 16; ++n) {
     const int k = t + 16 * n;
     if (act && k < m1) {
-      const float2 vv = v[base + (long long)k * m2];
+      const float2 vv = vin[n];
       const float2 o = make_float2(vv.x * ib - x[n].x, vv.y * ib - x[n].y);
-      out[base + (long long)k * m2] = o;
-      if (part) acc += (double)vv.x * o.x + (double)vv.y * o.y;
+      if (DIR) {
+        out[base + (long long)k * m2] =
+            vu ? make_float2(o.x + beta * dl[n].x, o.y + beta * dl[n].y) : make_float2(0.f, 0.f);
+      } else {
+        out[base + (long long)k * m2] = o;
+        if (part) acc += (double)vv.x * o.x + (double)vv.y * o.y;
+      }
     }
   }
-  if (part) {
+  if (!DIR && part) {
     acc = block_sum(acc, red);
     if (threadIdx.x == 0) part[(long long)lane * gridDim.x + blockIdx.x] = acc;
   }
@@ -346,7 +385,9 @@ k_lrf_col_apply(const float2* __restrict__ corr, const float2* __restrict__ tab,
 
 // k_lr_corr of fbme_pcg.hip: corr = S^-1 (proj / bg) / bg from chunk partials.
 int fbme_lr_corr_launch(int lanes, int rhs, int chunks, const void* part, const void* sinv, const float* bg,
-                        void* corr, hipStream_t st);
+                        void* corr, hipStream_t st, const double* rr_part, int rr_blocks, double* nrho);
+int fbme_pcg_commit_launch(int lanes, double* rho, const double* curv, const double* nrho, int* active,
+                           hipStream_t st);
 
 extern "C" {
 
@@ -365,18 +406,39 @@ int fbme_lowrank_apply_fft_dot(void* stream, int lanes, int rhs, int m1, int m2,
   hipStream_t st = (hipStream_t)stream;
   const dim3 cg((m2 + GC - 1) / GC, lanes);
   const float2* tab = (const float2*)factors;
-  hipLaunchKernelGGL(k_lrf_col_project, cg, dim3(16 * GC), 0, st, (const float2*)v, map, (float2*)wbuf, m1, m2, rhs,
-                     (const float2*)tw256);
-  const int chunks = (m2 + LRF_CHUNK - 1) / LRF_CHUNK;
-  hipLaunchKernelGGL(k_lrf_row_project, dim3(chunks, lanes), dim3(256), 0, st, (const float2*)wbuf, tab, map, m1,
-                     m2, rhs, (double2*)part);
-  int rc = fbme_lr_corr_launch(lanes, rhs, chunks, part, sinv, bg, corr, st);
+  (void)wbuf;
+  hipLaunchKernelGGL(k_lrf_col_project, cg, dim3(16 * GC), 0, st, (const float2*)v, tab, map, m1, m2, rhs,
+                     (const float2*)tw256, (double2*)part);
+  int rc = fbme_lr_corr_launch(lanes, rhs, (int)cg.x, part, sinv, bg, corr, st, nullptr, 0, nullptr);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_lrf_col_apply, cg, dim3(16 * GC), 0, st, (const float2*)corr, tab, map, (const float2*)v,
-                     bg, (float2*)out, m1, m2, rhs, (const float2*)tw256, dot_part);
+  hipLaunchKernelGGL(k_lrf_col_apply<false>, cg, dim3(16 * GC), 0, st, (const float2*)corr, tab, map,
+                     (const float2*)v, bg, (float2*)out, m1, m2, rhs, (const float2*)tw256, dot_part,
+                     (const double*)nullptr, (const double*)nullptr, (const double*)nullptr, (const int*)nullptr);
   if (dot)
     hipLaunchKernelGGL(k_gop_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)cg.x, lanes, dot);
   return (int)hipGetLastError();
+}
+
+int fbme_lowrank_apply_fft_direction(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                                     const void* sinv, const float* bg, const int* map, const void* r, void* d,
+                                     void* part, void* corr, const void* tw256, const double* rr_part,
+                                     int rr_blocks, double* rho, const double* curv, double* nrho, int* active) {
+  if (lanes < 1 || rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || m1 > 256 || r == d || rr_blocks < 1 ||
+      !rr_part || !rho || !curv || !nrho || !active)
+    return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 cg((m2 + GC - 1) / GC, lanes);
+  const float2* tab = (const float2*)factors;
+  hipLaunchKernelGGL(k_lrf_col_project, cg, dim3(16 * GC), 0, st, (const float2*)r, tab, map, m1, m2, rhs,
+                     (const float2*)tw256, (double2*)part);
+  int rc = fbme_lr_corr_launch(lanes, rhs, (int)cg.x, part, sinv, bg, corr, st, rr_part, rr_blocks, nrho);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_lrf_col_apply<true>, cg, dim3(16 * GC), 0, st, (const float2*)corr, tab, map,
+                     (const float2*)r, bg, (float2*)d, m1, m2, rhs, (const float2*)tw256, (double*)nullptr,
+                     (const double*)rho, curv, (const double*)nrho, (const int*)active);
+  rc = (int)hipGetLastError();
+  if (rc) return rc;
+  return fbme_pcg_commit_launch(lanes, rho, curv, nrho, active, st);
 }
 
 int fbme_grid_matvec_dot(void* stream, int lanes, int rhs, int m1, int m2, const float* gamma, const float* noise,

@@ -505,11 +505,16 @@ k_lr_project(const float2* __restrict__ tab, const float2* __restrict__ v, int m
 // thread (q, k) = (tid / 32, tid % 32) sums chunks q, q + 8, ... of atom k,
 // the eight partial sums are added in a fixed order, then thread k < 32
 // forms row k of S^-1 proj.
+// With rr_part: also nrho[lane] = <r, z> of z = r / bg - F corr without z,
+// ||r||^2 / bg - Re(proj^H corr) (||r||^2 from the update pass's block
+// partials), both sums in FP64 and fixed order.
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict__ sinv,
-          const float* __restrict__ bg, int rhs, float2* __restrict__ corr) {
+          const float* __restrict__ bg, int rhs, float2* __restrict__ corr, const double* __restrict__ rr_part,
+          int rr_blocks, double* __restrict__ nrho) {
   __shared__ double2 acc8[8][RANK];
   __shared__ double2 proj[RANK];
+  __shared__ double pcs[RANK];
   const int lanev = blockIdx.x;
   const int b = lanev / rhs;
   const int k = threadIdx.x & (RANK - 1), q = threadIdx.x >> 5;
@@ -538,7 +543,22 @@ k_lr_corr(const double2* __restrict__ part, int chunks, const float2* __restrict
       a.x += (double)w.x * proj[jj].x - (double)w.y * proj[jj].y;
       a.y += (double)w.x * proj[jj].y + (double)w.y * proj[jj].x;
     }
-    corr[(long long)lanev * RANK + k] = make_float2((float)(a.x * ib * ib), (float)(a.y * ib * ib));
+    const double2 c = make_double2(a.x * ib * ib, a.y * ib * ib);
+    const float2 cf = make_float2((float)c.x, (float)c.y);
+    corr[(long long)lanev * RANK + k] = cf;
+    // the synthesis uses the rounded corr, so the identity does too
+    pcs[k] = proj[k].x * (double)cf.x + proj[k].y * (double)cf.y;
+  }
+  if (rr_part) {   // uniform: a kernel argument
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double rr = 0.0;
+      for (int i = threadIdx.x; i < rr_blocks; i += 64) rr += rr_part[(long long)lanev * rr_blocks + i];
+      rr = wave_sum(rr);
+      double pc = threadIdx.x < RANK ? pcs[threadIdx.x] : 0.0;
+      pc = wave_sum(pc);
+      if (threadIdx.x == 0) nrho[lanev] = rr / (double)bg[b] - pc;
+    }
   }
 }
 
@@ -789,11 +809,13 @@ __device__ __forceinline__ bool pcg_valid(double cv, double rh) {
 }
 
 // x += alpha d, r -= alpha q with curv = sum of the matvec's partials; block
-// (0, lane) stores curv[lane].
+// (0, lane) stores curv[lane].  RR: also rr_part[lane, block] = ||r||^2 over
+// the block's samples after the update (FP64, fixed tree order).
+template <bool RR>
 __global__ void k_pcg_update_parts(float2* __restrict__ x, float2* __restrict__ r, const float2* __restrict__ d,
                                    const float2* __restrict__ q, const double* __restrict__ rho,
                                    const double* __restrict__ part, int nparts, double* __restrict__ curv,
-                                   const int* __restrict__ active, int m) {
+                                   const int* __restrict__ active, int m, double* __restrict__ rr_part) {
   const int lanev = blockIdx.y;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const long long g = (long long)lanev * m + i;
@@ -818,6 +840,11 @@ __global__ void k_pcg_update_parts(float2* __restrict__ x, float2* __restrict__ 
     rv.y -= alpha * qv.y;
     x[g] = xv;
     r[g] = rv;
+  }
+  if (RR) {
+    __shared__ double red[4];
+    const double rr = block_sum((double)rv.x * rv.x + (double)rv.y * rv.y, red);
+    if (threadIdx.x == 0) rr_part[(long long)lanev * gridDim.x + blockIdx.x] = rr;
   }
 }
 
@@ -1228,9 +1255,16 @@ int scratch_address(void** out) {
 
 // Launcher of k_lr_corr shared with the FFT-based preconditioner passes.
 int fbme_lr_corr_launch(int lanes, int rhs, int chunks, const void* part, const void* sinv, const float* bg,
-                        void* corr, hipStream_t st) {
+                        void* corr, hipStream_t st, const double* rr_part, int rr_blocks, double* nrho) {
   hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,
-                     (const float2*)sinv, bg, rhs, (float2*)corr);
+                     (const float2*)sinv, bg, rhs, (float2*)corr, rr_part, rr_blocks, nrho);
+  return (int)hipGetLastError();
+}
+
+// Launcher of k_pcg_commit_parts shared with the fused preconditioner pass.
+int fbme_pcg_commit_launch(int lanes, double* rho, const double* curv, const double* nrho, int* active,
+                           hipStream_t st) {
+  hipLaunchKernelGGL(k_pcg_commit_parts, dim3(nblk(lanes, 64)), dim3(64), 0, st, rho, curv, nrho, active, lanes);
   return (int)hipGetLastError();
 }
 
@@ -1458,8 +1492,8 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
   hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, m1, m2, rhs, rb,   \
                      (double2*)part);                                                                           \
   hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,             \
-                     (const float2*)sinv,                                                                       \
-                     bg, rhs, (float2*)corr);                                                                   \
+                     (const float2*)sinv, bg, rhs, (float2*)corr, (const double*)nullptr, 0,                    \
+                     (double*)nullptr);                                                                         \
   if (dot)                                                                                                      \
     hipLaunchKernelGGL((k_lr_apply<LG, true>), ag, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v,              \
                        (const float2*)corr, bg, m1, m2, rhs, (float2*)out, dot_part);                           \
@@ -1534,8 +1568,19 @@ int fbme_pcg_update(void* stream, int lanes, int m, void* x, void* r, const void
 int fbme_pcg_update_parts(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
                           const double* rho, const double* part, int nparts, double* curv, const int* active) {
   if (lanes < 1 || m < 1 || nparts < 1) return FBME_ERR_ARG;
-  hipLaunchKernelGGL(k_pcg_update_parts, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (float2*)x,
-                     (float2*)r, (const float2*)d, (const float2*)q, rho, part, nparts, curv, active, m);
+  hipLaunchKernelGGL(k_pcg_update_parts<false>, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, (hipStream_t)stream,
+                     (float2*)x, (float2*)r, (const float2*)d, (const float2*)q, rho, part, nparts, curv, active, m,
+                     (double*)nullptr);
+  return (int)hipGetLastError();
+}
+
+int fbme_pcg_update_parts_rr(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                             const double* rho, const double* part, int nparts, double* curv, const int* active,
+                             double* rr_part) {
+  if (lanes < 1 || m < 1 || nparts < 1 || !rr_part) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_pcg_update_parts<true>, dim3(nblk(m), lanes), dim3(FBME_BLOCK), 0, (hipStream_t)stream,
+                     (float2*)x, (float2*)r, (const float2*)d, (const float2*)q, rho, part, nparts, curv, active, m,
+                     rr_part);
   return (int)hipGetLastError();
 }
 

@@ -149,6 +149,12 @@ _SIGNATURES = {
     "fbme_pcg_direction_parts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_int,
                                          c_void_p, c_void_p]),
+    "fbme_pcg_update_parts_rr": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "fbme_lowrank_apply_fft_direction": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                                                 c_void_p]),
     "fbme_pcg_commit_valid": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "fbme_lowrank_apply_fft_dot": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                            c_void_p, c_void_p, c_void_p, c_void_p,

@@ -329,7 +329,11 @@ class _Solver:
         self.mv_part = torch.empty(self.lanes * ((m2 + 15) // 16), dtype=f64, device=dev)
         self.pc_part = torch.empty(self.lanes * max((m + 255) // 256, (m2 + 15) // 16), dtype=f64, device=dev)
         self.lr_map = torch.empty((ns, 386), dtype=torch.int32, device=dev)
-        self.lr_buf = torch.empty(self.lanes * RANK * m2, dtype=c64, device=dev) if self.lr_fft else None
+        self.lr_buf = None   # the FFT preconditioner reduces its projection in the same pass
+        # ||r||^2 block partials of the update pass (the fused second half
+        # forms <r, z> from them without storing z)
+        self.rr_blocks = (m + 255) // 256
+        self.rr_part = torch.empty(self.lanes * self.rr_blocks, dtype=f64, device=dev)
         self.circ_eig = self.circ_work = None
         if self.precond == "circulant":
             self.circ_eig = torch.empty((ns, m), dtype=f32, device=dev)
@@ -587,9 +591,10 @@ class _Solver:
             self._chk(lib.fbme_grid_matvec_dot(
                 self.stream, self.lanes, self.r, m1, m2, _ptr(self.gamma), _ptr(self.noise), _ptr(self.d),
                 _ptr(self.q), _ptr(self.gmid), _ptr(t.tw[256]), _ptr(self.mv_part), None), "grid_matvec")
-            self._chk(lib.fbme_pcg_update_parts(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
-                                                _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.mv_part),
-                                                self.nparts, _ptr(self.curv), _ptr(self.active)), "update")
+            self._chk(lib.fbme_pcg_update_parts_rr(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
+                                                   _ptr(self.d), _ptr(self.q), _ptr(self.rho), _ptr(self.mv_part),
+                                                   self.nparts, _ptr(self.curv), _ptr(self.active),
+                                                   _ptr(self.rr_part)), "update")
             return
         self.matvec(self.d, self.q, dot=self.curv)
         self._chk(lib.fbme_pcg_update(self.stream, self.lanes, self.m, _ptr(self.x), _ptr(self.res),
@@ -603,18 +608,18 @@ class _Solver:
                                                      _ptr(self.active)), "commit_valid")
 
     def _iter_second(self):
-        """z = M^-1 r with <r, z>; d = z + beta d (no restart)."""
+        """z = M^-1 r with <r, z>; d = z + beta d (no restart).  On the fused
+        path z is never stored: <r, z> = ||r||^2 / bg - Re(proj^H corr) from
+        the update pass's block partials and the projection, and the
+        synthesis pass writes d itself."""
         lib, t = self.lib, self.t
         if self.fused:
             m1, m2 = self.samples
-            self._chk(lib.fbme_lowrank_apply_fft_dot(
+            self._chk(lib.fbme_lowrank_apply_fft_direction(
                 self.stream, self.lanes, self.r, m1, m2, _ptr(self.factors), _ptr(self.sinv), _ptr(self.bg),
-                _ptr(self.lr_map), _ptr(self.res), _ptr(self.z), _ptr(self.lr_buf), _ptr(self.lr_part),
-                _ptr(self.corr), _ptr(t.tw[256]), _ptr(self.pc_part), None), "lowrank_apply_fft")
-            self._chk(lib.fbme_pcg_direction_parts(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),
-                                                   _ptr(self.rho), _ptr(self.curv), _ptr(self.pc_part),
-                                                   self.nparts, _ptr(self.nrho), _ptr(self.active)),
-                      "direction")
+                _ptr(self.lr_map), _ptr(self.res), _ptr(self.d), _ptr(self.lr_part), _ptr(self.corr),
+                _ptr(t.tw[256]), _ptr(self.rr_part), self.rr_blocks, _ptr(self.rho), _ptr(self.curv),
+                _ptr(self.nrho), _ptr(self.active)), "lowrank_apply_fft_direction")
             return
         self.precondition(self.res, self.z, dot=self.nrho)
         self._chk(lib.fbme_pcg_direction(self.stream, self.lanes, self.m, _ptr(self.d), _ptr(self.z),

@@ -294,6 +294,12 @@ int fbme_pcg_update_parts(void* stream, int lanes, int m, void* x, void* r, cons
                           const double* rho, const double* part, int nparts, double* curv, const int* active);
 int fbme_pcg_direction_parts(void* stream, int lanes, int m, void* d, const void* z, double* rho,
                              const double* curv, const double* part, int nparts, double* nrho, int* active);
+/* fbme_pcg_update_parts that also leaves rr_part[lane, block] = ||r||^2 over
+ * each 256-sample block after the update (FP64), the input of
+ * fbme_lowrank_apply_fft_direction.  rr_part: lanes * ceil(m / 256) doubles. */
+int fbme_pcg_update_parts_rr(void* stream, int lanes, int m, void* x, void* r, const void* d, const void* q,
+                             const double* rho, const double* part, int nparts, double* curv, const int* active,
+                             double* rr_part);
 int fbme_pcg_commit_valid(void* stream, int lanes, const double* rho, const double* curv, int* active);
 /* 32 largest values of each x[b, :n] sorted descending, ties to the lower
  * index (the stable argsort of solver.py:5797-5800).  cand: scratch of
@@ -344,19 +350,32 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
                            double* dot);
 /* The same preconditioner for N1 = 256 grids through the 256-point column
  * transform: the projection is the column IFFT of v restricted to the
- * atoms' distinct rows, then 32 row sums; the correction is a row synthesis
- * on those rows, then one column FFT (same operator as fbme_lowrank_apply,
- * solver.py:5836-5848, with about a quarter of the flops).
- * fbme_lowrank_map builds map [batch, 386] ints from the atoms of
- * fbme_topk32 (slot of each atom, row of each slot, slot count, slot of each
- * grid row, atoms ordered by slot, first position of each slot).  wbuf: lanes * 32 * M2 complex64; part: lanes * ceil(M2 / 16)
- * * 32 double2;
+ * atoms' distinct rows, reduced onto the 32 atoms in the same pass; the
+ * correction is a row synthesis on those rows, then one column FFT (same
+ * operator as fbme_lowrank_apply, solver.py:5836-5848, with about a quarter
+ * of the flops).  fbme_lowrank_map builds map [batch, 386] ints from the
+ * atoms of fbme_topk32 (slot of each atom, row of each slot, slot count,
+ * slot of each grid row, atoms ordered by slot, first position of each
+ * slot).  wbuf: unused (may be NULL); part: lanes * ceil(M2 / 16) * 32
+ * double2;
  * dot_part: lanes * ceil(M2 / 16) doubles; tw256: fbme_twiddle_table(256). */
 int fbme_lowrank_map(void* stream, int batch, int n1, int n2, const int* idx, int* map);
 int fbme_lowrank_apply_fft_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
                                const void* sinv, const float* bg, const int* map, const void* v, void* out,
                                void* wbuf, void* part, void* corr, const void* tw256, double* dot_part,
                                double* dot);
+/* The second half of a PCG iteration on N1 = 256 grids without storing z
+ * (replaces fbme_lowrank_apply_fft_dot + fbme_pcg_direction_parts,
+ * solver.py:5708-5720): nrho = <r, z> = ||r||^2 / bg - Re(proj^H corr) from
+ * rr_part (fbme_pcg_update_parts_rr, rr_blocks per lane) and the projection,
+ * then the synthesis pass writes d = z + (nrho / rho) d in place (0 for a
+ * lane whose update is not valid), then rho / active are committed as
+ * fbme_pcg_direction_parts does.  r and d distinct; part and corr as in
+ * fbme_lowrank_apply_fft_dot. */
+int fbme_lowrank_apply_fft_direction(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                                     const void* sinv, const float* bg, const int* map, const void* r, void* d,
+                                     void* part, void* corr, const void* tw256, const double* rr_part,
+                                     int rr_blocks, double* rho, const double* curv, double* nrho, int* active);
 /* out[l] = sum Re(conj(a_l) b_l) in FP64 (the PCG inner products of
  * _pcg, solver.py:5582-5723).  part: lanes * 64 doubles. */
 int fbme_lane_dot(void* stream, int lanes, int m, const void* a, const void* b, double* part, double* out);
