@@ -87,7 +87,7 @@ def parse():
                     help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--skip-uneven", action="store_true")
     ap.add_argument("--skip-pcg", action="store_true")
-    ap.add_argument("--pcg-batch", type=int, default=8)
+    ap.add_argument("--pcg-batch", type=int, default=32)
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--skip-masked", action="store_true")
     ap.add_argument("--skip-joint", action="store_true")
