@@ -17,6 +17,9 @@
 #include "fbme_types.h"
 
 #define FBME_BLOCK 256
+// Longest line of the LDS Stockham passes (twiddle tables up to this length;
+// the BTTB embedding of a 1024-sample axis is 2048 long).
+#define FBME_FFT_MAX 2048
 
 namespace fbme {
 

@@ -16,14 +16,15 @@ namespace {
 constexpr int OP_TILE = FBME_TILE_MAX;
 
 // Column pass on a [batch, rows_in, cols] array (row-major) producing
-// [batch, rows_out, cols]: FFT of length n along the rows axis, reading the
-// first `rows_in` entries (zero-padded) and keeping the first `rows_out`.
-template <bool INV>
+// [batch, rows_out, cols]: FFT of length n <= TWN along the rows axis,
+// reading the first `rows_in` entries (zero-padded) and keeping the first
+// `rows_out`.
+template <bool INV, int TWN>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_op_col(const float2* __restrict__ in, float2* __restrict__ out, int n, int rows_in, int rows_out, int cols,
          int tile_cols, unsigned code) {
   __shared__ float2 buf[FBME_LDS_SLOTS(OP_TILE)];
-  __shared__ float2 tw[1024];
+  __shared__ float2 tw[TWN];
   const int b = blockIdx.y;
   const int c0 = blockIdx.x * tile_cols;
   const int cc = min(tile_cols, cols - c0);
@@ -43,13 +44,14 @@ k_op_col(const float2* __restrict__ in, float2* __restrict__ out, int n, int row
   }
 }
 
-// Row pass on [batch, rows, len_in] -> [batch, rows, len_out], FFT length n.
-template <bool INV>
+// Row pass on [batch, rows, len_in] -> [batch, rows, len_out], FFT length
+// n <= TWN.
+template <bool INV, int TWN>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_op_row(const float2* __restrict__ in, float2* __restrict__ out, int n, int len_in, int len_out, int rows,
          int tile_rows, unsigned code) {
   __shared__ float2 buf[FBME_LDS_SLOTS(OP_TILE)];
-  __shared__ float2 tw[1024];
+  __shared__ float2 tw[TWN];
   const int b = blockIdx.y;
   const int r0 = blockIdx.x * tile_rows;
   const int rr = min(tile_rows, rows - r0);
@@ -330,23 +332,28 @@ int fbme_stream_fill(void* stream, void* dst, long long bytes, float value, int 
 }
 
 unsigned fbme_radix_code(int len) {
-  if (len < 1 || len > 1024) return 0xFFFFFFFFu;
+  if (len < 1 || len > FBME_FFT_MAX) return 0xFFFFFFFFu;
   return radix_code_host(len);
 }
 
 int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,
                     int inverse) {
-  if (batch < 1 || n < 1 || n > 1024 || rows_in > n || rows_out > n || cols < 1) return FBME_ERR_ARG;
+  if (batch < 1 || n < 1 || n > FBME_FFT_MAX || rows_in > n || rows_out > n || cols < 1) return FBME_ERR_ARG;
   const unsigned code = radix_code_host(n);
   if (code == 0xFFFFFFFFu) return FBME_ERR_ARG;
   const int tc = std::max(1, std::min(cols, std::min(16, OP_TILE / n)));
   dim3 grid((cols + tc - 1) / tc, batch);
-  if (inverse)
-    hipLaunchKernelGGL(k_op_col<true>, grid, dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)in,
-                       (float2*)out, n, rows_in, rows_out, cols, tc, code);
-  else
-    hipLaunchKernelGGL(k_op_col<false>, grid, dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)in,
-                       (float2*)out, n, rows_in, rows_out, cols, tc, code);
+#define FBME_OPCOL(INV, TWN)                                                                                   \
+  hipLaunchKernelGGL((k_op_col<INV, TWN>), grid, dim3(FBME_BLOCK), 0, (hipStream_t)stream, (const float2*)in, \
+                     (float2*)out, n, rows_in, rows_out, cols, tc, code)
+  if (n > 1024) {
+    if (inverse) FBME_OPCOL(true, FBME_FFT_MAX);
+    else FBME_OPCOL(false, FBME_FFT_MAX);
+  } else {
+    if (inverse) FBME_OPCOL(true, 1024);
+    else FBME_OPCOL(false, 1024);
+  }
+#undef FBME_OPCOL
   return (int)hipGetLastError();
 }
 
@@ -357,10 +364,10 @@ int fbme_prefix_forward(void* stream, int batch, int n1, int n2, int m1, int m2,
   if (c1 == 0xFFFFFFFFu || c2 == 0xFFFFFFFFu) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int tr = std::max(1, std::min(n1, std::min(8, OP_TILE / n2)));
-  hipLaunchKernelGGL(k_op_row<false>, dim3((n1 + tr - 1) / tr, batch), dim3(FBME_BLOCK), 0, st,
+  hipLaunchKernelGGL((k_op_row<false, 1024>), dim3((n1 + tr - 1) / tr, batch), dim3(FBME_BLOCK), 0, st,
                      (const float2*)x, (float2*)mid, n2, n2, m2, n1, tr, c2);
   const int tc = std::max(1, std::min(m2, std::min(16, OP_TILE / n1)));
-  hipLaunchKernelGGL(k_op_col<false>, dim3((m2 + tc - 1) / tc, batch), dim3(FBME_BLOCK), 0, st,
+  hipLaunchKernelGGL((k_op_col<false, 1024>), dim3((m2 + tc - 1) / tc, batch), dim3(FBME_BLOCK), 0, st,
                      (const float2*)mid, (float2*)y, n1, n1, m1, m2, tc, c1);
   return (int)hipGetLastError();
 }
@@ -372,10 +379,10 @@ int fbme_prefix_adjoint(void* stream, int batch, int n1, int n2, int m1, int m2,
   if (c1 == 0xFFFFFFFFu || c2 == 0xFFFFFFFFu) return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const int tc = std::max(1, std::min(m2, std::min(16, OP_TILE / n1)));
-  hipLaunchKernelGGL(k_op_col<true>, dim3((m2 + tc - 1) / tc, batch), dim3(FBME_BLOCK), 0, st,
+  hipLaunchKernelGGL((k_op_col<true, 1024>), dim3((m2 + tc - 1) / tc, batch), dim3(FBME_BLOCK), 0, st,
                      (const float2*)v, (float2*)mid, n1, m1, n1, m2, tc, c1);
   const int tr = std::max(1, std::min(n1, std::min(8, OP_TILE / n2)));
-  hipLaunchKernelGGL(k_op_row<true>, dim3((n1 + tr - 1) / tr, batch), dim3(FBME_BLOCK), 0, st,
+  hipLaunchKernelGGL((k_op_row<true, 1024>), dim3((n1 + tr - 1) / tr, batch), dim3(FBME_BLOCK), 0, st,
                      (const float2*)mid, (float2*)x, n2, m2, n2, n1, tr, c2);
   return (int)hipGetLastError();
 }

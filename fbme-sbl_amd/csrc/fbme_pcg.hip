@@ -57,12 +57,13 @@ __global__ void k_embed(const float2* __restrict__ lags, const float* __restrict
 // Fused column pass of the Toeplitz matvec on [BR, M1, E2]: for each column,
 // zero-pad M1 -> E1, forward FFT, multiply by spectrum[b][:, c] / (E1 E2),
 // inverse FFT, keep the first M1 rows.
-template <int EPT>
+// TWN bounds E1 (the twiddle table's LDS length).
+template <int EPT, int TWN>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_toeplitz_col(const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ spec,
                int rhs, int m1, int e1, int e2, int tile_cols, unsigned code, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
-  __shared__ float2 tw[1024];
+  __shared__ float2 tw[TWN];
   const int lanev = blockIdx.y;          // b * rhs + r
   const int b = lanev / rhs;
   const int c0 = blockIdx.x * tile_cols;
@@ -92,14 +93,14 @@ k_toeplitz_col(const float2* __restrict__ in, float2* __restrict__ out, const fl
   }
 }
 
-// Row pass on [rows, len_in] -> [rows, len_out] with an FFT of length n,
+// Row pass on [rows, len_in] -> [rows, len_out] with an FFT of length n <= TWN,
 // twiddles from a table (plan-owned).
-template <int EPT, bool INV>
+template <int EPT, bool INV, int TWN>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_rowpass(const float2* __restrict__ in, float2* __restrict__ out, int n, int len_in, int len_out,
           long long rows, int tile_rows, unsigned code, const float2* __restrict__ twtab) {
   __shared__ float2 buf[FBME_LDS_SLOTS(EPT * FBME_BLOCK)];
-  __shared__ float2 tw[1024];
+  __shared__ float2 tw[TWN];
   const long long r0 = (long long)blockIdx.x * tile_rows;
   const int rr = (int)min((long long)tile_rows, rows - r0);
   const int stride = n + 1;
@@ -1208,15 +1209,18 @@ void launch_rowpass(const float2* in, float2* out, int n, int len_in, int len_ou
   const int tr = want_rows > 0 ? std::min(want_rows, rowpass_cap(n)) : def;
   const int grid = (int)((rows + tr - 1) / tr);
   const int need = tr * (n + 1);
-  if (need <= 4 * FBME_BLOCK)
-    hipLaunchKernelGGL((k_rowpass<4, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
-                       tr, code, tw);
+#define FBME_ROWPASS(EPT, TWN)                                                                                \
+  hipLaunchKernelGGL((k_rowpass<EPT, INV, TWN>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, \
+                     rows, tr, code, tw)
+  if (n > 1024)
+    FBME_ROWPASS(16, FBME_FFT_MAX);
+  else if (need <= 4 * FBME_BLOCK)
+    FBME_ROWPASS(4, 1024);
   else if (need <= 8 * FBME_BLOCK)
-    hipLaunchKernelGGL((k_rowpass<8, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
-                       tr, code, tw);
+    FBME_ROWPASS(8, 1024);
   else
-    hipLaunchKernelGGL((k_rowpass<16, INV>), dim3(grid), dim3(FBME_BLOCK), 0, st, in, out, n, len_in, len_out, rows,
-                       tr, code, tw);
+    FBME_ROWPASS(16, 1024);
+#undef FBME_ROWPASS
 }
 
 // Columns per workgroup of the fused Toeplitz column pass: at most 16 and at
@@ -1278,6 +1282,9 @@ int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2,
                         void* work_a, void* work_b, void* spectrum, const void* tw_n1, const void* tw_n2,
                         const void* tw_e1, const void* tw_e2) {
   // work_a, work_b: batch * max(N, E1 E2) complex each.
+  if (batch < 1 || n1 < 1 || n2 < 1 || n1 > 1024 || n2 > 1024 || m1 > n1 || m2 > n2 || e1 < 2 * m1 - 1 ||
+      e2 < 2 * m2 - 1 || e1 > FBME_FFT_MAX || e2 > FBME_FFT_MAX)
+    return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const long long n = (long long)n1 * n2;
   float2* a = (float2*)work_a;
@@ -1344,7 +1351,8 @@ int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2,
                                unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
                                void* work_b, const void* tw_e1, const void* tw_e2, int row_tile, int col_tile) {
   // work_a, work_b: lanes * M1 * E2 complex each.
-  if (row_tile < 0 || col_tile < 0) return FBME_ERR_ARG;
+  if (row_tile < 0 || col_tile < 0 || e1 < 1 || e2 < 1 || e1 > FBME_FFT_MAX || e2 > FBME_FFT_MAX)
+    return FBME_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   launch_rowpass<false>((const float2*)d, (float2*)work_a, e2, m2, e2, (long long)lanes * m1, code_e2,
                         (const float2*)tw_e2, st, row_tile);
@@ -1352,15 +1360,21 @@ int fbme_toeplitz_matvec_tiled(void* stream, int lanes, int rhs, int m1, int m2,
   const int tc = col_tile > 0 ? std::min(col_tile, cap) : cap;
   const int need = tc * (e1 + 1);
   dim3 grid((e2 + tc - 1) / tc, lanes);
-#define FBME_TCOL(EPT)                                                                                          \
-  hipLaunchKernelGGL((k_toeplitz_col<EPT>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a,              \
+#define FBME_TCOL(EPT, TWN)                                                                                     \
+  hipLaunchKernelGGL((k_toeplitz_col<EPT, TWN>), grid, dim3(FBME_BLOCK), 0, st, (const float2*)work_a,         \
                      (float2*)work_b, (const float2*)spectrum, rhs, m1, e1, e2, tc, code_e1, (const float2*)tw_e1)
-  if (need <= 8 * FBME_BLOCK)
-    FBME_TCOL(8);
-  else if (need <= 16 * FBME_BLOCK)
-    FBME_TCOL(16);
-  else
-    FBME_TCOL(24);
+  if (e1 > 1024) {
+    if (need <= 16 * FBME_BLOCK)
+      FBME_TCOL(16, FBME_FFT_MAX);
+    else
+      FBME_TCOL(24, FBME_FFT_MAX);
+  } else if (need <= 8 * FBME_BLOCK) {
+    FBME_TCOL(8, 1024);
+  } else if (need <= 16 * FBME_BLOCK) {
+    FBME_TCOL(16, 1024);
+  } else {
+    FBME_TCOL(24, 1024);
+  }
 #undef FBME_TCOL
   launch_rowpass<true>((const float2*)work_b, (float2*)out, e2, e2, m2, (long long)lanes * m1, code_e2,
                        (const float2*)tw_e2, st, row_tile);
@@ -1375,7 +1389,7 @@ int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e
 }
 
 int fbme_toeplitz_tiles(int e1, int e2, int* row_tile_max, int* col_tile_max) {
-  if (e1 < 1 || e2 < 1 || e1 > 1024 || e2 > 1024) return FBME_ERR_ARG;
+  if (e1 < 1 || e2 < 1 || e1 > FBME_FFT_MAX || e2 > FBME_FFT_MAX) return FBME_ERR_ARG;
   *row_tile_max = rowpass_cap(e2);
   *col_tile_max = toeplitz_col_cap(e1, e2);
   return 0;
@@ -1711,7 +1725,7 @@ int fbme_masked_copy(void* stream, int lanes, int m, void* dst, const void* src,
 }
 
 int fbme_twiddle_table(void* stream, int len, void* out) {
-  if (len < 1 || len > 1024) return FBME_ERR_ARG;
+  if (len < 1 || len > FBME_FFT_MAX) return FBME_ERR_ARG;
   hipLaunchKernelGGL(k_twiddle, dim3(nblk(len)), dim3(FBME_BLOCK), 0, (hipStream_t)stream, (float2*)out, len);
   return (int)hipGetLastError();
 }

@@ -75,10 +75,13 @@ class _Tables:
         m1, m2 = samples
         self.e1 = _next_smooth_length(2 * m1 - 1)
         self.e2 = _next_smooth_length(2 * m2 - 1)
-        for length in (n1, n2, self.e1, self.e2):
-            if length > 1024:
+        if max(n1, n2) > 1024:
+            raise NotImplementedError(f"grid {grid} exceeds the kernels' 1024-point axes")
+        # embeddings of up to 1024 samples per axis: at most 2048 points
+        for length in (self.e1, self.e2):
+            if length > 2048:
                 raise NotImplementedError(
-                    f"FFT length {length} exceeds the kernels' 1024 limit")
+                    f"FFT length {length} exceeds the kernels' 2048 limit")
         self.codes = {L: lib.fbme_radix_code(L) for L in (n1, n2, self.e1, self.e2)}
         if any(c == 0xFFFFFFFF for c in self.codes.values()):
             raise NotImplementedError("grid lengths must factor into 2, 3 and 5")

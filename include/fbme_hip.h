@@ -221,21 +221,22 @@ int fbme_uamp_noise_update(void* stream, int batch, int m, const void* data, con
  * ------------------------------------------------------------------------ */
 
 /* Radix digit code of an FFT length (factors 2, 3, 4, 5, 8), or 0xFFFFFFFF
- * when the length has another prime factor or exceeds 1024.  Host only. */
+ * when the length has another prime factor or exceeds 2048.  Host only. */
 unsigned fbme_radix_code(int n);
 /* Column FFT of [batch, rows_in (zero padded to n), cols] -> first rows_out
- * rows of the length-n (inverse when `inverse`) transform. */
+ * rows of the length-n (inverse when `inverse`) transform, n <= 2048. */
 int fbme_column_fft(void* stream, int batch, int n, int rows_in, int rows_out, int cols, const void* in, void* out,
                     int inverse);
 /* Host table out[2 * n] (double re, im): D[d] = sum_{a < m} exp(2 pi i a d / n),
  * the Dirichlet kernel of the Gram closed form (operators.py:444-467 columns,
  * Gram of the prefix rows). */
 int fbme_dirichlet_table(int m, int n, double* out);
-/* exp(-2 pi i k / len), k < len <= 1024, complex64 (FFT twiddles). */
+/* exp(-2 pi i k / len), k < len <= 2048, complex64 (FFT twiddles). */
 int fbme_twiddle_table(void* stream, int len, void* out);
 /* BTTB covariance spectrum of Q = noise I + A Gamma A^H on the circulant
- * embedding E1 x E2 (operators.py:299-326).  work_a, work_b: batch *
- * max(N1 N2, E1 E2) complex64 each; spectrum: batch * E1 * E2. */
+ * embedding E1 x E2 (operators.py:299-326), N1, N2 <= 1024 and
+ * 2 M - 1 <= E <= 2048 per axis.  work_a, work_b: batch * max(N1 N2, E1 E2)
+ * complex64 each; spectrum: batch * E1 * E2. */
 int fbme_gamma_spectrum(void* stream, int batch, int n1, int n2, int m1, int m2, int e1, int e2, unsigned code1,
                         unsigned code2, unsigned codee1, unsigned codee2, const float* gamma, const float* noise,
                         void* work_a, void* work_b, void* spectrum, const void* tw_n1, const void* tw_n2,
@@ -253,8 +254,8 @@ int fbme_circulant_spectrum(void* stream, int batch, int n1, int n2, int m1, int
                             float* eig, const void* tw_n2, const void* tw_m2);
 int fbme_circulant_apply(void* stream, int lanes, int rhs, int m1, int m2, unsigned code_m2, const float* eig,
                          const void* v, void* out, void* work, const void* tw_m2);
-/* out = Q d for `lanes` lanes, `rhs` lanes per scene (operators.py:328-355).
- * work_a, work_b: lanes * M1 * E2 complex64 each. */
+/* out = Q d for `lanes` lanes, `rhs` lanes per scene (operators.py:328-355),
+ * E1, E2 <= 2048.  work_a, work_b: lanes * M1 * E2 complex64 each. */
 int fbme_toeplitz_matvec(void* stream, int lanes, int rhs, int m1, int m2, int e1, int e2, unsigned code_e1,
                          unsigned code_e2, const void* spectrum, const void* d, void* out, void* work_a,
                          void* work_b, const void* tw_e1, const void* tw_e2);

@@ -47,6 +47,40 @@ def test_dense_workspace_rejects_bad_shapes():
         assert lib.fbme_dense_workspace_bytes(*args) == 0, args
 
 
+def test_radix_codes_and_toeplitz_tiles():
+    """fbme_radix_code and fbme_toeplitz_tiles are host-only.  A code exists
+    exactly for the 5-smooth lengths up to 2048 and its nibbles (radices 2,
+    3, 4, 5, 8, first stage lowest) multiply back to the length; the BTTB
+    tile caps keep every LDS tile within the 16- (row) and 24-element
+    (column) per-thread instances."""
+    lib = _lib.load()
+
+    def smooth(n):
+        for p in (2, 3, 5):
+            while n % p == 0:
+                n //= p
+        return n == 1
+
+    for n in range(1, 2100):
+        code = lib.fbme_radix_code(n)
+        if n > 2048 or not smooth(n):
+            assert code == 0xFFFFFFFF, n
+            continue
+        prod, c = 1, code
+        while c:
+            assert c & 15 in (2, 3, 4, 5, 8), (n, hex(code))
+            prod *= c & 15
+            c >>= 4
+        assert prod == n, (n, hex(code))
+    rmax, cmax = ctypes.c_int(), ctypes.c_int()
+    for e1, e2 in [(8, 8), (1024, 1024), (1080, 12), (12, 2000), (2048, 3), (2048, 2048)]:
+        assert lib.fbme_toeplitz_tiles(e1, e2, ctypes.byref(rmax), ctypes.byref(cmax)) == 0
+        assert rmax.value >= 1 and rmax.value * (e2 + 1) <= 16 * 256
+        assert 1 <= cmax.value <= min(e2, 16) and cmax.value * (e1 + 1) <= 24 * 256
+    for e1, e2 in [(0, 8), (2049, 8), (8, 2049)]:
+        assert lib.fbme_toeplitz_tiles(e1, e2, ctypes.byref(rmax), ctypes.byref(cmax)) != 0
+
+
 def _plan(shapes, stable=False, **kw):
     lib = _lib.load()
     opt = _lib.default_options(stable)

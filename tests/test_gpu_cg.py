@@ -26,6 +26,8 @@ from oracle.fbme_oracle import F32_EPS, F32_TINY
 pytestmark = pytest.mark.gpu
 
 SHAPES = [((32, 32), (24, 24)), ((40, 30), (20, 15)), ((64, 48), (45, 40))]
+# circulant embeddings longer than 1024 points: E1 = 1080, E2 = 2000, E1 = 2048
+LONG = [((640, 8), (540, 6)), ((8, 1024), (6, 1000)), ((1024, 4), (1024, 2))]
 
 
 def _solver(device, grid, samples, batch=2, seed=0, probes=None):
@@ -45,7 +47,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("grid,samples", SHAPES)
+@pytest.mark.parametrize("grid,samples", SHAPES + LONG)
 def test_toeplitz_matvec_vs_dense_covariance(device, grid, samples):
     print("shape", grid, samples, flush=True)
     s, _ = _solver(device, grid, samples)
@@ -63,7 +65,7 @@ def test_toeplitz_matvec_vs_dense_covariance(device, grid, samples):
     assert _rel(got, ref) < 2e-5
 
 
-@pytest.mark.parametrize("grid,samples", SHAPES)
+@pytest.mark.parametrize("grid,samples", SHAPES + LONG[:1])
 def test_lowrank_preconditioner_vs_oracle(device, grid, samples):
     print("shape", grid, samples, flush=True)
     s, _ = _solver(device, grid, samples, seed=10)
@@ -154,6 +156,30 @@ def test_sbl_cg_batch_vs_oracle(device):
         assert _rel(x[i], xr[i]) < 1e-4
         truth = batch[i].x.reshape(-1)
         assert abs(scenes.nmse_db(x[i], truth) - scenes.nmse_db(xr[i], truth)) < 0.01
+    np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=1e-4)
+
+
+@pytest.mark.parametrize("grid,samples", LONG)
+def test_sbl_cg_long_embedding_vs_oracle(device, grid, samples):
+    """Whole CG solves whose BTTB embedding exceeds 1024 points on one axis
+    (the spectrum's column FFT, the matvec's column and row passes at 1080,
+    2000 and 2048 points) against the float64 oracle: 10 EM iterations of
+    8 PCG iterations each, the oracle running in a few seconds."""
+    from fbme_amd import scenes
+    from torch_sbl import SBLConfig, sbl_2d
+    print("shape", grid, samples, flush=True)
+    sc = scenes.uniform_scene(90, grid, samples, k=8, snr_db=20.0)
+    data = sc.y[None].astype(np.complex64)
+    cfg = SBLConfig(mode="cg", device=str(device), max_iter=10)
+    res = sbl_2d(torch.from_numpy(data).to(device), grid, config=cfg)
+    ref = co.sbl_cg(data, grid, co.CGConfig(max_iter=10))
+    x = res.x.cpu().numpy().reshape(-1)
+    xr = ref["x"].reshape(-1)
+    print("long embedding: its", res.iterations, ref["iterations"], "pcg", res.history["cg_iterations"],
+          ref["cg_iterations"], "rel", _rel(x, xr), flush=True)
+    assert res.iterations == ref["iterations"]
+    assert [int(v) for v in res.history["cg_iterations"]] == [int(v) for v in ref["cg_iterations"]]
+    assert _rel(x, xr) < 1e-4
     np.testing.assert_allclose(res.noise_variance.cpu().numpy(), ref["noise_variance"], rtol=1e-4)
 
 
