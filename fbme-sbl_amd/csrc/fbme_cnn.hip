@@ -250,6 +250,44 @@ __global__ void k_features(const float2* __restrict__ bp, const float* __restric
   for (int c = 2; c < 16; ++c) o[c] = 0;
 }
 
+// Inputs of the preconditioner network: channels 0 and 1 as k_features,
+// channel 2 = log(1 + N gamma / noise) / 8, the cell's loading of Q
+// relative to the noise floor (the quantity the grid-diagonal background
+// w = 1 / (N (noise + N gamma)) is built from).
+__global__ void k_features_precond(const float2* __restrict__ bp, const float* __restrict__ gamma,
+                                   const double* __restrict__ bpe, const double* __restrict__ gsum,
+                                   const float* __restrict__ noise, int n, uint16_t* __restrict__ x) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 v = bp[(size_t)b * n + i];
+  const float g = gamma[(size_t)b * n + i];
+  const float me = (float)(bpe[b] / n), mg = (float)(gsum[b] / n);
+  float f0 = logf((v.x * v.x + v.y * v.y) / fmaxf(me, 1e-30f) + 1e-6f);
+  float f1 = logf(fmaxf(g, 1e-37f) / fmaxf(mg, 1e-30f));
+  float f2 = log1pf((float)n * fmaxf(g, 0.f) / fmaxf(noise[b], 1e-37f)) * 0.125f;
+  f0 = fminf(fmaxf(f0, -30.f), 30.f);
+  f1 = fminf(fmaxf(f1, -30.f), 30.f);
+  f2 = fminf(f2, 30.f);
+  uint16_t* o = x + ((size_t)b * n + i) * 16;
+  o[0] = f2bf(f0);
+  o[1] = f2bf(f1);
+  o[2] = f2bf(f2);
+#pragma unroll
+  for (int c = 3; c < 16; ++c) o[c] = 0;
+}
+
+// dz4 channel 0 = g (a loss gradient at the output formed elsewhere), other
+// channels 0.
+__global__ void k_output_grad(const float* __restrict__ g, long long count, uint16_t* __restrict__ dz) {
+  const long long e = (long long)blockIdx.x * BLK + threadIdx.x;
+  if (e >= count) return;
+  uint16_t* o = dz + e * 16;
+  o[0] = f2bf(g[e]);
+#pragma unroll
+  for (int c = 1; c < 16; ++c) o[c] = 0;
+}
+
 // dz4 channel 0 = 2 (out - target) / count, other channels 0; FP64 loss partials.
 __global__ void __launch_bounds__(BLK)
 k_loss(const float* __restrict__ out, const float* __restrict__ target, long long count, uint16_t* __restrict__ dz,
@@ -342,6 +380,20 @@ int fbme_cnn_features(void* stream, int batch, int n, const void* bp, const floa
   if (batch < 1 || n < 1) return FBME_ERR_ARG;
   hipLaunchKernelGGL(k_features, dim3(nblk(n), batch), dim3(BLK), 0, (hipStream_t)stream, (const float2*)bp, gamma,
                      bpe, gsum, n, (uint16_t*)x);
+  return (int)hipGetLastError();
+}
+
+int fbme_cnn_features_precond(void* stream, int batch, int n, const void* bp, const float* gamma,
+                              const double* bpe, const double* gsum, const float* noise, void* x) {
+  if (batch < 1 || n < 1 || !noise) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_features_precond, dim3(nblk(n), batch), dim3(BLK), 0, (hipStream_t)stream,
+                     (const float2*)bp, gamma, bpe, gsum, noise, n, (uint16_t*)x);
+  return (int)hipGetLastError();
+}
+
+int fbme_cnn_output_grad(void* stream, long long count, const float* g, void* dz4) {
+  if (count < 1 || !g || !dz4) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_output_grad, dim3(nblk(count)), dim3(BLK), 0, (hipStream_t)stream, g, count, (uint16_t*)dz4);
   return (int)hipGetLastError();
 }
 

@@ -302,50 +302,13 @@ __global__ void k_lr_tables(const int* __restrict__ idx, const float* __restrict
   tab[(long long)b * (m1 + m2) * RANK + i] = make_float2((float)c * amp, (float)s * amp);
 }
 
-// One wavefront per scene: small = I + F^H F / bg with F^H F from the
-// Dirichlet Gram (or, under an observation mask, from the mask spectrum mh:
-// the Gram of the factor columns on the observed rows), GPU Cholesky loading
-// (10 eps mean|diag|, solver.py:5827-5833), inverse S^-1 = L^-H L^-1
-// (solver.py:5848).
-__global__ void __launch_bounds__(64)
-k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const double* __restrict__ sums,
-           const float* __restrict__ noise, const double2* __restrict__ d1, const double2* __restrict__ d2,
-           const double2* __restrict__ mh, int n1, int n2, float2* __restrict__ sinv, float* __restrict__ bg_out,
-           int* __restrict__ status) {
-  __shared__ double2 a[RANK][RANK + 1];
-  __shared__ double2 xs[RANK][RANK + 1];
-  __shared__ double sq[RANK];
-  __shared__ int gi[RANK];
-  const int b = blockIdx.x;
-  const int lane = threadIdx.x;
-  double vsum = 0.0;
-  if (lane < RANK) {
-    gi[lane] = idx[b * RANK + lane];
-    sq[lane] = sqrt(fmax((double)val[b * RANK + lane], 0.0));
-    vsum = (double)val[b * RANK + lane];
-  }
-  vsum = wave_sum_all(vsum);
-  const double total = sums[b];
-  const double omitted = fmax(total - vsum, 0.0);
-  const double bg = fmax((double)noise[b] + omitted, F32_EPS_D * total);
-  __syncthreads();
-  for (int e = lane; e < RANK * RANK; e += 64) {
-    const int r = e / RANK, q = e - r * RANK;
-    int dd1 = gi[r] / n2 - gi[q] / n2;
-    int dd2 = gi[r] % n2 - gi[q] % n2;
-    dd1 += dd1 < 0 ? n1 : 0;
-    dd2 += dd2 < 0 ? n2 : 0;
-    double2 gv;
-    if (mh) {
-      gv = mh[(long long)dd1 * n2 + dd2];
-    } else {
-      const double2 u = d1[dd1], w = d2[dd2];
-      gv = make_double2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x);
-    }
-    const double s = sq[r] * sq[q] / bg;
-    a[r][q] = make_double2((r == q ? 1.0 : 0.0) + gv.x * s, gv.y * s);
-  }
-  __syncthreads();
+// Inverse of the Hermitian RANK x RANK system held in a (one wavefront,
+// lane < RANK owns row `lane`): symmetrised, loaded with 10 eps mean|diag|
+// (solver.py:5827-5833), Cholesky-factored in registers, S^-1 = L^-H L^-1
+// (solver.py:5848) written to sinv (row-major complex64).  a is
+// overwritten by the factor; returns nonzero on a non-positive pivot.
+__device__ int lr_chol_inverse(double2 (*a)[RANK + 1], double2 (*xs)[RANK + 1], int lane,
+                               float2* __restrict__ sinv) {
   double2 row[RANK];
 #pragma unroll
   for (int q = 0; q < RANK; ++q) {
@@ -417,9 +380,57 @@ k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const dou
         acc.x += xi.x * x[r].x + xi.y * x[r].y;
         acc.y += xi.x * x[r].y - xi.y * x[r].x;
       }
-      sinv[((long long)b * RANK + i) * RANK + lane] = make_float2((float)acc.x, (float)acc.y);
+      sinv[(long long)i * RANK + lane] = make_float2((float)acc.x, (float)acc.y);
     }
   }
+  return fail;
+}
+
+// One wavefront per scene: small = I + F^H F / bg with F^H F from the
+// Dirichlet Gram (or, under an observation mask, from the mask spectrum mh:
+// the Gram of the factor columns on the observed rows), GPU Cholesky loading
+// (10 eps mean|diag|, solver.py:5827-5833), inverse S^-1 = L^-H L^-1
+// (solver.py:5848).
+__global__ void __launch_bounds__(64)
+k_lr_setup(const int* __restrict__ idx, const float* __restrict__ val, const double* __restrict__ sums,
+           const float* __restrict__ noise, const double2* __restrict__ d1, const double2* __restrict__ d2,
+           const double2* __restrict__ mh, int n1, int n2, float2* __restrict__ sinv, float* __restrict__ bg_out,
+           int* __restrict__ status) {
+  __shared__ double2 a[RANK][RANK + 1];
+  __shared__ double2 xs[RANK][RANK + 1];
+  __shared__ double sq[RANK];
+  __shared__ int gi[RANK];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double vsum = 0.0;
+  if (lane < RANK) {
+    gi[lane] = idx[b * RANK + lane];
+    sq[lane] = sqrt(fmax((double)val[b * RANK + lane], 0.0));
+    vsum = (double)val[b * RANK + lane];
+  }
+  vsum = wave_sum_all(vsum);
+  const double total = sums[b];
+  const double omitted = fmax(total - vsum, 0.0);
+  const double bg = fmax((double)noise[b] + omitted, F32_EPS_D * total);
+  __syncthreads();
+  for (int e = lane; e < RANK * RANK; e += 64) {
+    const int r = e / RANK, q = e - r * RANK;
+    int dd1 = gi[r] / n2 - gi[q] / n2;
+    int dd2 = gi[r] % n2 - gi[q] % n2;
+    dd1 += dd1 < 0 ? n1 : 0;
+    dd2 += dd2 < 0 ? n2 : 0;
+    double2 gv;
+    if (mh) {
+      gv = mh[(long long)dd1 * n2 + dd2];
+    } else {
+      const double2 u = d1[dd1], w = d2[dd2];
+      gv = make_double2(u.x * w.x - u.y * w.y, u.x * w.y + u.y * w.x);
+    }
+    const double s = sq[r] * sq[q] / bg;
+    a[r][q] = make_double2((r == q ? 1.0 : 0.0) + gv.x * s, gv.y * s);
+  }
+  __syncthreads();
+  const int fail = lr_chol_inverse(a, xs, lane, sinv + (long long)b * RANK * RANK);
   if (lane == 0) {
     bg_out[b] = (float)bg;
     if (status && fail) status[b] = 1;
@@ -616,6 +627,142 @@ k_lr_apply(const float2* __restrict__ tab, const float2* __restrict__ v, const f
       const float2 o = make_float2(vv.x * ib - ax, vv.y * ib - ay);
       out[(lane0 + j) * m + i] = o;
       if (DOT) dacc[j] = (double)vv.x * o.x + (double)vv.y * o.y;
+    }
+  }
+  if (DOT) {
+    const int wave = threadIdx.x >> 6, wl = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < LG; ++j) {
+      const double w = wave_sum(dacc[j]);
+      if (wl == 0) red[wave][j] = w;
+    }
+    __syncthreads();
+    if (threadIdx.x < nl) {
+      double sum = 0.0;
+      for (int w = 0; w < FBME_BLOCK / 64; ++w) sum += red[w][threadIdx.x];
+      part[(lane0 + threadIdx.x) * gridDim.x + blockIdx.x] = sum;
+    }
+  }
+}
+
+// ------------------------------------------- grid-diagonal background
+// Background of the low-rank preconditioner taken on the grid instead of as a
+// scalar: B^-1 = A diag(w) A^H with w > 0, so B^-1 is Hermitian positive
+// definite for any w (A A^H = N I).  For a uniform variance g,
+// Q = (noise + N g) I exactly, so w = 1 / (N (noise + N g)) makes B = Q; the
+// closed form w_n = 1 / (N (noise + N gamma_T,n)) takes gamma_T = gamma
+// without the RANK exact atoms of the Woodbury part.  A per-cell log-scale o
+// (the learned correction, clamped to +-cl) multiplies w by exp(o).
+__global__ void k_gdiag_weights(const float* __restrict__ gamma, const float* __restrict__ noise,
+                                const float* __restrict__ logscale, float cl, long long n,
+                                float* __restrict__ w) {
+  const int b = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long e = (long long)b * n + i;
+  const double nn = (double)n;
+  double v = 1.0 / (nn * ((double)noise[b] + nn * (double)fmaxf(gamma[e], 0.f)));
+  if (logscale) v *= exp((double)fminf(fmaxf(logscale[e], -cl), cl));
+  w[e] = (float)v;
+}
+
+// The RANK exact atoms carry no background variance: w = 1 / (N noise) there.
+__global__ void k_gdiag_atoms(const int* __restrict__ idx, const float* __restrict__ noise,
+                              const float* __restrict__ logscale, float cl, long long n, float* __restrict__ w) {
+  const int b = blockIdx.x, k = threadIdx.x;
+  if (k >= RANK) return;
+  const long long e = (long long)b * n + idx[b * RANK + k];
+  double v = 1.0 / ((double)n * (double)noise[b]);
+  if (logscale) v *= exp((double)fminf(fmaxf(logscale[e], -cl), cl));
+  w[e] = (float)v;
+}
+
+// Factor columns as lanes: out[(b, k), m] = U[m1, k] V[m2, k] (the phase
+// tables of k_lr_tables), so that B^-1 F can go through the matvec passes.
+__global__ void k_lr_columns(const float2* __restrict__ tab, int m1, int m2, float2* __restrict__ out) {
+  const int b = blockIdx.z, k = blockIdx.y;
+  const long long m = (long long)m1 * m2;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int a1 = (int)(i / m2), a2 = (int)(i - (long long)a1 * m2);
+  const float2* U = tab + (long long)b * (m1 + m2) * RANK;
+  const float2* V = U + (long long)m1 * RANK;
+  out[((long long)b * RANK + k) * m + i] = cmul(U[(long long)a1 * RANK + k], V[(long long)a2 * RANK + k]);
+}
+
+// One wavefront per scene: C = I + F^H B^-1 F from the projection partials of
+// the RANK lanes B^-1 F_l onto the atoms (part[(b RANK + l) chunks + c][k] =
+// partial of (F^H B^-1 F)[k][l], summed over the chunks in order), then
+// C^-1 by the Cholesky inverse of the low-rank setup.  C >= I, so the
+// factorisation cannot fail in exact arithmetic; a failure is reported in
+// status[b].
+__global__ void __launch_bounds__(64)
+k_gdiag_system(const double2* __restrict__ part, int chunks, float2* __restrict__ cinv, int* __restrict__ status) {
+  __shared__ double2 a[RANK][RANK + 1];
+  __shared__ double2 xs[RANK][RANK + 1];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  for (int e = lane; e < RANK * RANK; e += 64) {
+    const int l = e / RANK, k = e - l * RANK;
+    const double2* pp = part + ((long long)(b * RANK + l) * chunks) * RANK + k;
+    double2 s = make_double2(0.0, 0.0);
+    for (int c = 0; c < chunks; ++c) {
+      const double2 v = pp[(long long)c * RANK];
+      s.x += v.x;
+      s.y += v.y;
+    }
+    a[k][l] = make_double2((k == l ? 1.0 : 0.0) + s.x, s.y);
+  }
+  __syncthreads();
+  const int fail = lr_chol_inverse(a, xs, lane, cinv + (long long)b * RANK * RANK);
+  if (lane == 0 && status && fail) status[b] = 1;
+}
+
+// out = b - U corr with U = B^-1 F stored as RANK lanes per scene (the
+// grid-diagonal preconditioner (B + F F^H)^-1 v = b - U C^-1 U^H v,
+// b = B^-1 v, U^H v = F^H b): one sample per thread, the RANK column values
+// of the sample loaded once and applied to every lane of the group.  DOT:
+// part[lane, block] = sum Re(conj(vd) out) (FP64, fixed order), the PCG inner
+// product <r, M^-1 r> with vd = r.
+template <int LG, bool DOT>
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_lr_apply_stored(const float2* __restrict__ ucols, const float2* __restrict__ b, const float2* __restrict__ vd,
+                  const float2* __restrict__ corr, long long m, int rhs, float2* __restrict__ out,
+                  double* __restrict__ part) {
+  __shared__ double red[FBME_BLOCK / 64][LG];
+  __shared__ float2 cs[LG][RANK];
+  const int sc = blockIdx.z;
+  const int r0 = blockIdx.y * LG;
+  const int nl = min(LG, rhs - r0);
+  for (int e = threadIdx.x; e < nl * RANK; e += FBME_BLOCK)
+    cs[e / RANK][e % RANK] = corr[((long long)sc * rhs + r0) * RANK + e];
+  __syncthreads();
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
+  const bool live = i < m;
+  float2 f[RANK];
+  const float2* U = ucols + (long long)sc * RANK * m;
+#pragma unroll
+  for (int q = 0; q < RANK; ++q) f[q] = live ? U[(long long)q * m + i] : make_float2(0.f, 0.f);
+  const long long lane0 = (long long)sc * rhs + r0;
+  double dacc[LG];
+#pragma unroll
+  for (int j = 0; j < LG; ++j) {
+    dacc[j] = 0.0;
+    if (j < nl && live) {
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int q = 0; q < RANK; ++q) {
+        const float2 c = cs[j][q];
+        ax += f[q].x * c.x - f[q].y * c.y;
+        ay += f[q].x * c.y + f[q].y * c.x;
+      }
+      const float2 bb = b[(lane0 + j) * m + i];
+      const float2 o = make_float2(bb.x - ax, bb.y - ay);
+      out[(lane0 + j) * m + i] = o;
+      if (DOT) {
+        const float2 vv = vd[(lane0 + j) * m + i];
+        dacc[j] = (double)vv.x * o.x + (double)vv.y * o.y;
+      }
     }
   }
   if (DOT) {
@@ -1487,13 +1634,16 @@ int fbme_lowrank_setup_masked(void* stream, int batch, int n1, int n2, int m1, i
   return (int)hipGetLastError();
 }
 
-int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
-                           const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
-                           double* dot) {
+namespace {
+
+// out = vbase / bg - F S^-1 F^H vproj / bg^2 (vproj = vbase: the low-rank
+// preconditioner); with dot, dot[lane] = Re <vbase, out>.
+int lowrank_apply_general(hipStream_t st, int lanes, int rhs, int m1, int m2, const void* factors,
+                          const void* sinv, const float* bg, const void* vproj, const void* v, void* out, void* part,
+                          void* corr, double* dot_part, double* dot) {
   // part: lanes * 64 * 32 double2; corr: lanes * 32 complex; dot_part: lanes * ceil(M / 256) doubles.
   if (rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1) return FBME_ERR_ARG;
   if ((dot_part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
   const int batch = lanes / rhs;
   const int rb = (m1 + 63) / 64;
   const int chunks = (m1 + rb - 1) / rb;
@@ -1503,7 +1653,8 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
   const dim3 pg(chunks, groups, batch), ag((unsigned)nblk(m), groups, batch);
   const float2* tab = (const float2*)factors;
 #define FBME_LR_LAUNCH(LG)                                                                                      \
-  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)v, m1, m2, rhs, rb,   \
+  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)vproj, m1, m2, rhs,  \
+                     rb,                                                                                        \
                      (double2*)part);                                                                           \
   hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,             \
                      (const float2*)sinv, bg, rhs, (float2*)corr, (const double*)nullptr, 0,                    \
@@ -1527,6 +1678,99 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
   if (dot)
     hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)ag.x,
                        lanes, dot);
+  return (int)hipGetLastError();
+}
+
+}  // namespace
+
+int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
+                           const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
+                           double* dot) {
+  return lowrank_apply_general((hipStream_t)stream, lanes, rhs, m1, m2, factors, sinv, bg, v, v, out, part, corr,
+                               dot_part, dot);
+}
+
+int fbme_lowrank_apply_split(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                             const void* sinv, const float* bg, const void* vproj, const void* vbase, void* out,
+                             void* part, void* corr) {
+  if (!vproj || !vbase || vbase == out) return FBME_ERR_ARG;
+  return lowrank_apply_general((hipStream_t)stream, lanes, rhs, m1, m2, factors, sinv, bg, vproj, vbase, out, part,
+                               corr, nullptr, nullptr);
+}
+
+int fbme_gdiag_weights(void* stream, int batch, long long n, const float* gamma, const float* noise, const int* idx,
+                       const float* logscale, float clamp, float* w) {
+  if (batch < 1 || n < RANK || !gamma || !noise || !idx || !w || !(clamp >= 0.f)) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gdiag_weights, dim3((unsigned)nblk(n), batch), dim3(FBME_BLOCK), 0, st, gamma, noise,
+                     logscale, clamp, n, w);
+  hipLaunchKernelGGL(k_gdiag_atoms, dim3(batch), dim3(64), 0, st, idx, noise, logscale, clamp, n, w);
+  return (int)hipGetLastError();
+}
+
+int fbme_lowrank_columns(void* stream, int batch, int m1, int m2, const void* factors, void* out) {
+  if (batch < 1 || m1 < 1 || m2 < 1 || !factors || !out) return FBME_ERR_ARG;
+  const long long m = (long long)m1 * m2;
+  hipLaunchKernelGGL(k_lr_columns, dim3((unsigned)nblk(m), RANK, batch), dim3(FBME_BLOCK), 0, (hipStream_t)stream,
+                     (const float2*)factors, m1, m2, (float2*)out);
+  return (int)hipGetLastError();
+}
+
+int fbme_gdiag_system(void* stream, int batch, int m1, int m2, const void* factors, const void* bf, void* part,
+                      void* cinv, int* status) {
+  // part: batch * RANK * 64 * RANK double2 (the projection partials of the RANK lanes of bf)
+  if (batch < 1 || m1 < 1 || m2 < 1 || !factors || !bf || !part || !cinv) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int rb = (m1 + 63) / 64;
+  const int chunks = (m1 + rb - 1) / rb;
+  const dim3 pg(chunks, 2, batch);   // two groups of 16 lanes per scene
+  hipLaunchKernelGGL((k_lr_project<17>), pg, dim3(FBME_BLOCK), 0, st, (const float2*)factors, (const float2*)bf, m1,
+                     m2, RANK, rb, (double2*)part);
+  hipLaunchKernelGGL(k_gdiag_system, dim3(batch), dim3(64), 0, st, (const double2*)part, chunks, (float2*)cinv,
+                     status);
+  return (int)hipGetLastError();
+}
+
+int fbme_gdiag_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* cinv,
+                     const float* ones, const void* bf, const void* b, const void* v, void* out, void* part,
+                     void* corr, double* dot_part, double* dot) {
+  // part: lanes * 64 * 32 double2; corr: lanes * 32 complex; dot_part: lanes * ceil(M / 256) doubles.
+  if (rhs < 1 || lanes % rhs != 0 || m1 < 1 || m2 < 1 || !factors || !cinv || !ones || !bf || !b || !out)
+    return FBME_ERR_ARG;
+  if ((dot_part == nullptr) != (dot == nullptr) || (dot && !v) || b == out) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int batch = lanes / rhs;
+  const int rb = (m1 + 63) / 64;
+  const int chunks = (m1 + rb - 1) / rb;
+  const int groups = (rhs + 23) / 24;
+  const int per = (rhs + groups - 1) / groups;
+  const long long m = (long long)m1 * m2;
+  const dim3 pg(chunks, groups, batch), ag((unsigned)nblk(m), groups, batch);
+  const float2* tab = (const float2*)factors;
+#define FBME_GD_LAUNCH(LG)                                                                                       \
+  hipLaunchKernelGGL((k_lr_project<LG>), pg, dim3(FBME_BLOCK), 0, st, tab, (const float2*)b, m1, m2, rhs, rb,    \
+                     (double2*)part);                                                                            \
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,              \
+                     (const float2*)cinv, ones, rhs, (float2*)corr, (const double*)nullptr, 0, (double*)nullptr); \
+  if (dot)                                                                                                       \
+    hipLaunchKernelGGL((k_lr_apply_stored<LG, true>), ag, dim3(FBME_BLOCK), 0, st, (const float2*)bf,             \
+                       (const float2*)b, (const float2*)v, (const float2*)corr, m, rhs, (float2*)out, dot_part); \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_lr_apply_stored<LG, false>), ag, dim3(FBME_BLOCK), 0, st, (const float2*)bf,            \
+                       (const float2*)b, (const float2*)v, (const float2*)corr, m, rhs, (float2*)out,            \
+                       (double*)nullptr)
+  if (per <= 8) {
+    FBME_GD_LAUNCH(8);
+  } else if (per <= 12) {
+    FBME_GD_LAUNCH(12);
+  } else if (per <= 17) {
+    FBME_GD_LAUNCH(17);
+  } else {
+    FBME_GD_LAUNCH(24);
+  }
+#undef FBME_GD_LAUNCH
+  if (dot)
+    hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)ag.x, lanes, dot);
   return (int)hipGetLastError();
 }
 

@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 # return codes (include/fbme_hip.h)
 ERR_ARG = 1
 ERR_HIP = 2
@@ -143,6 +143,17 @@ _SIGNATURES = {
                                        c_void_p, c_void_p, c_void_p, c_void_p,
                                        c_void_p]),
     "fbme_lowrank_map": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "fbme_lowrank_apply_split": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                         c_void_p, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_gdiag_weights": (c_int, [c_void_p, c_int, c_longlong, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_float, c_void_p]),
+    "fbme_lowrank_columns": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p]),
+    "fbme_gdiag_apply": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p]),
+    "fbme_gdiag_system": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p]),
     "fbme_pcg_update_parts": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_void_p, c_void_p,
                                       c_int, c_void_p, c_void_p]),
@@ -215,6 +226,9 @@ _SIGNATURES = {
     "fbme_cnn_prepare": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_cnn_features": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_features_precond": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_cnn_output_grad": (c_int, [c_void_p, c_longlong, c_void_p, c_void_p]),
     "fbme_cnn_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p,
                                  c_void_p, c_void_p, c_void_p, c_void_p]),
     "fbme_cnn_grad_rows": (c_longlong, [c_int]),

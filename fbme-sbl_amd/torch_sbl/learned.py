@@ -36,17 +36,20 @@ from .config import SBLConfig, SBLResult
 CHANNELS = (16, 32, 32, 32, 16)
 CLAMP = 15.0
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "warm_start.npz")
+PRECOND_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "precond.npz")
+# bound of the preconditioner network's per-cell log-scale correction
+PRECOND_CLAMP = 4.0
 
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
-def init_params(seed: int = 0) -> np.ndarray:
+def init_params(seed: int = 0, inputs: int = 2) -> np.ndarray:
     """He-normal weights (zero on padded channels), zero biases; the output
     layer starts small so the untrained correction is close to zero."""
     rng = np.random.default_rng(seed)
-    real_in = (2, 32, 32, 32)
+    real_in = (inputs, 32, 32, 32)
     flat = []
     for l in range(4):
         ci, co = CHANNELS[l], CHANNELS[l + 1]
@@ -171,6 +174,50 @@ class WarmStartNet:
         return loss
 
 
+class PrecondNet(WarmStartNet):
+    """The same network (4 layers, 3x3, 32 channels, BF16 MFMA) emitting the
+    per-cell log-scale o of the grid-diagonal preconditioner background,
+    w = exp(clamp(o)) / (N (noise + N gamma_T)) (fbme_gdiag_weights).  Inputs
+    (fbme_cnn_features_precond): log|A^H y|^2 and log gamma relative to their
+    scene means, and log(1 + N gamma / noise) / 8.  Any output keeps the
+    preconditioner Hermitian positive definite; an output of zero is the
+    closed-form background."""
+
+    def __init__(self, device, params: np.ndarray | None = None, seed: int = 0) -> None:
+        super().__init__(device, init_params(seed, inputs=3) if params is None else params, seed)
+        self.clamp = PRECOND_CLAMP
+
+    @classmethod
+    def load(cls, device, path: str = PRECOND_WEIGHTS) -> "PrecondNet":
+        with np.load(path) as z:
+            return cls(device, z["params"])
+
+    def features(self, bp, gamma, bpe, gsum, noise, grid):
+        b = gamma.shape[0]
+        n = prod(grid)
+        x = torch.empty((b, grid[0], grid[1], 16), dtype=torch.int16, device=self.device)
+        _lib.check(self.lib.fbme_cnn_features_precond(self._stream(), b, n, _ptr(bp), _ptr(gamma), _ptr(bpe),
+                                                      _ptr(gsum), _ptr(noise), _ptr(x)), "cnn_features_precond")
+        return x
+
+    def backward_from(self, x, g):
+        """Parameter gradients (left in self.grads) for a loss gradient g
+        [B, H, W] at the output of the last forward on x."""
+        b, h, w, _ = x.shape
+        ws = self._workspace(b, h, w)
+        _lib.check(self.lib.fbme_cnn_output_grad(self._stream(), b * h * w, _ptr(g), _ptr(ws["dz4"])),
+                   "cnn_output_grad")
+        _lib.check(self.lib.fbme_cnn_backward(self._stream(), b, h, w, _ptr(x), _ptr(ws["act"]), _ptr(self.wb),
+                                              _ptr(ws["dz4"]), _ptr(ws["dz"]), _ptr(ws["part"]),
+                                              _ptr(self.grads)), "cnn_backward")
+
+    def adam(self, lr: float) -> None:
+        self.step += 1
+        _lib.check(self.lib.fbme_cnn_adam(self._stream(), self.params.numel(), _ptr(self.params), _ptr(self.grads),
+                                          _ptr(self.m), _ptr(self.v), float(lr), self.step), "cnn_adam")
+        self.prepare()
+
+
 def _stalled(solver, iters, cfg, sum_before) -> bool:
     sums = solver.sums.cpu().numpy()[:, 0]
     noise = solver.noise.cpu().numpy()
@@ -178,17 +225,24 @@ def _stalled(solver, iters, cfg, sum_before) -> bool:
     return bool(iters >= cfg.cg_max_iter or blown or not np.all(np.isfinite(noise)) or not np.all(np.isfinite(sums)))
 
 
-def learned_sbl_2d(data, grid_shape, net: WarmStartNet, config: SBLConfig | None = None) -> SBLResult:
-    """sbl_2d(mode='cg') started from the network's variances, with the plain
-    solver as fallback (see the module docstring).  Without a config the solve
-    runs net.warm_iter EM iterations (16 probes, PCG tolerance 1e-4)."""
-    cfg = config or SBLConfig(mode="cg", device=str(net.device), max_iter=net.warm_iter, min_iter=1, tol=0.0,
+def learned_sbl_2d(data, grid_shape, net: WarmStartNet | None, config: SBLConfig | None = None,
+                   precond_net: PrecondNet | None = None) -> SBLResult:
+    """sbl_2d(mode='cg') started from the network's variances (net) and/or
+    preconditioned by the grid-diagonal background with the preconditioner
+    network's correction (precond_net), with the plain solver as fallback
+    (see the module docstring).  Without a config the solve runs
+    net.warm_iter EM iterations (16 probes, PCG tolerance 1e-4)."""
+    if net is None and precond_net is None:
+        raise ValueError("learned_sbl_2d needs a warm-start or a preconditioner network")
+    lead = net if net is not None else precond_net
+    cfg = config or SBLConfig(mode="cg", device=str(lead.device), max_iter=lead.warm_iter, min_iter=1, tol=0.0,
                               hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
     y = torch.as_tensor(data)
     batch_shape = tuple(y.shape[:-2])
     b = prod(batch_shape) if batch_shape else 1
-    y = y.to(device=net.device, dtype=torch.complex64).reshape(b, *y.shape[-2:])
+    y = y.to(device=lead.device, dtype=torch.complex64).reshape(b, *y.shape[-2:])
     grid = tuple(int(v) for v in grid_shape)
+    extra = dict(precond="grid_diagonal", precond_net=precond_net) if precond_net is not None else {}
 
     def warm(solver):
         x = net.features(solver.bp, solver.gamma, solver.bpe, solver.sums[:, 0].contiguous(), grid)
@@ -198,7 +252,8 @@ def learned_sbl_2d(data, grid_shape, net: WarmStartNet, config: SBLConfig | None
         solver.grid_sums(with_mean=False)
 
     try:
-        res = pcg.sbl_cg(y, grid, cfg, batch_shape=batch_shape, warm_start=warm, stall_check=_stalled)
+        res = pcg.sbl_cg(y, grid, cfg, batch_shape=batch_shape, warm_start=warm if net is not None else None,
+                         stall_check=_stalled, **extra)
         res.history["learned_fallback"] = [0.0]
         return res
     except pcg.SolverStall:

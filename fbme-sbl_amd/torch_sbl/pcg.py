@@ -183,7 +183,8 @@ USE_GRAPHS = True
 class _Solver:
     """Device buffers and kernel calls of one (B, grid, samples, P) problem."""
 
-    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None, observation_mask=None, dense=False):
+    def __init__(self, y, grid, cfg: SBLConfig, probes_override=None, observation_mask=None, dense=False,
+                 precond=None, precond_net=None):
         self.lib = _lib.load()
         # mode="dense": exact posterior through the M x M covariance, no
         # probes and no Krylov solve (solver.py:2962-2990)
@@ -239,6 +240,19 @@ class _Solver:
         # preconditioner (solver.py:5726-5783): low-rank Woodbury, Strang
         # circulant on the sample grid, or none
         self.precond = "low_rank" if self.dense else cfg.cg_preconditioner
+        # 'grid_diagonal' (no reference counterpart; the learned solver's
+        # preconditioner): the low-rank Woodbury form with the scalar
+        # background replaced by B^-1 = A diag(w) A^H, w from the variances
+        # and, with precond_net, the network's per-cell correction
+        self.precond_net = precond_net
+        if precond is not None:
+            if precond != "grid_diagonal" or self.dense:
+                raise ValueError(f"unknown preconditioner override {precond!r}")
+            if observation_mask is not None and self.m_obs < self.m:
+                raise NotImplementedError("the grid-diagonal preconditioner takes unmasked samples")
+            self.precond = precond
+        elif precond_net is not None:
+            raise ValueError("precond_net needs precond='grid_diagonal'")
         if self.precond == "circulant" and 0xFFFFFFFF in self.t.sample_codes.values():
             raise NotImplementedError("the circulant preconditioner needs M1, M2 that factor into 2, 3 and 5")
         self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256 and self.precond == "low_rank"
@@ -337,6 +351,8 @@ class _Solver:
         # forms <r, z> from them without storing z)
         self.rr_blocks = (m + 255) // 256
         self.rr_part = torch.empty(self.lanes * self.rr_blocks, dtype=f64, device=dev)
+        if self.precond == "grid_diagonal":
+            self._gdiag_buffers()
         self.circ_eig = self.circ_work = None
         if self.precond == "circulant":
             self.circ_eig = torch.empty((ns, m), dtype=f32, device=dev)
@@ -367,6 +383,76 @@ class _Solver:
         self.act_idx = self.act_cand = self.act_val = self.rp_work = None
         self.mask = None
         self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
+
+    def _gdiag_buffers(self):
+        """Buffers of the grid-diagonal preconditioner: the weights w, the
+        factor columns F and B^-1 F as 32 lanes per system, C^-1, and the
+        lane temporary b = B^-1 v of the apply."""
+        dev, ns, n, m = self.dev, self.nsys, self.n, self.m
+        c64, f32 = torch.complex64, torch.float32
+        m1, m2 = self.samples
+        self.gd_w = torch.empty((ns, n), dtype=f32, device=dev)
+        self.gd_zero = torch.zeros(ns, dtype=f32, device=dev)
+        self.gd_ones = torch.ones(ns, dtype=f32, device=dev)
+        self.gd_cols = torch.empty((ns, RANK, m), dtype=c64, device=dev)
+        self.gd_bf = torch.empty((ns, RANK, m), dtype=c64, device=dev)
+        self.gd_cinv = torch.empty((ns, RANK, RANK), dtype=c64, device=dev)
+        self.gd_part = torch.empty(ns * RANK * 64 * RANK * 2, dtype=torch.float64, device=dev)
+        self.gd_b = torch.empty((self.lanes, m), dtype=c64, device=dev)
+        self.gd_logscale = None
+        big = max(self.lanes, ns * RANK)
+        if self.grid_op:
+            self.gd_gmid = torch.empty(big * 256 * m2, dtype=c64, device=dev)
+        else:
+            t = self.t
+            self.gd_spectrum = torch.empty((ns, t.e1, t.e2), dtype=c64, device=dev)
+            self.gd_work_a = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
+            self.gd_work_b = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
+
+    def _bmatvec(self, v, out, lanes, rhs):
+        """out = B^-1 v = A diag(w) A^H v: the matvec passes with
+        (gamma, noise) = (w, 0)."""
+        t = self.t
+        m1, m2 = self.samples
+        if self.grid_op:
+            self._chk(self.lib.fbme_grid_matvec_dot(
+                self.stream, lanes, rhs, m1, m2, _ptr(self.gd_w), _ptr(self.gd_zero), _ptr(v), _ptr(out),
+                _ptr(self.gd_gmid), _ptr(t.tw[256]), None, None), "gdiag_matvec")
+            return
+        self._chk(self.lib.fbme_toeplitz_matvec_tiled(
+            self.stream, lanes, rhs, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
+            _ptr(self.gd_spectrum), _ptr(v), _ptr(out), _ptr(self.gd_work_a), _ptr(self.gd_work_b),
+            _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2]), self.tiles[0], self.tiles[1]), "gdiag_toeplitz")
+
+    def _gdiag_setup(self):
+        """w (with the network's correction when one is attached), then
+        C^-1 = (I + F^H B^-1 F)^-1 of the Woodbury form of (B + F F^H)^-1."""
+        t = self.t
+        n1, n2 = self.grid
+        m1, m2 = self.samples
+        logscale = None
+        if self.precond_net is not None:
+            net = self.precond_net
+            x = net.features(self.bp[:self.nsys], self.gamma[:self.nsys], self.bpe[:self.nsys],
+                             self.sums[:self.nsys, 0].contiguous(), self.noise[:self.nsys], self.grid)
+            logscale = net.forward(x)
+        self.gd_logscale = logscale
+        clamp = float(self.precond_net.clamp) if self.precond_net is not None else 0.0
+        self._chk(self.lib.fbme_gdiag_weights(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.noise),
+                                              _ptr(self.top_idx), _ptr(logscale), clamp, _ptr(self.gd_w)),
+                  "gdiag_weights")
+        if not self.grid_op:
+            self._chk(self.lib.fbme_gamma_spectrum(
+                self.stream, self.nsys, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
+                t.codes[t.e2], _ptr(self.gd_w), _ptr(self.gd_zero), _ptr(self.spec_a), _ptr(self.spec_b),
+                _ptr(self.gd_spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
+                "gdiag_spectrum")
+        self._chk(self.lib.fbme_lowrank_columns(self.stream, self.nsys, m1, m2, _ptr(self.factors),
+                                                _ptr(self.gd_cols)), "lowrank_columns")
+        self._bmatvec(self.gd_cols, self.gd_bf, self.nsys * RANK, RANK)
+        self._chk(self.lib.fbme_gdiag_system(self.stream, self.nsys, m1, m2, _ptr(self.factors), _ptr(self.gd_bf),
+                                             _ptr(self.gd_part), _ptr(self.gd_cinv), _ptr(self.status)),
+                  "gdiag_system")
 
     # ------------------------------------------------------------ primitives
     def _chk(self, rc, what):
@@ -432,6 +518,15 @@ class _Solver:
         self._precondition(v, out, dot)
 
     def _precondition(self, v, out, dot=None):
+        if self.precond == "grid_diagonal":
+            # z = (B + F F^H)^-1 v = b - U C^-1 F^H b, b = B^-1 v, U = B^-1 F
+            m1, m2 = self.samples
+            self._bmatvec(v, self.gd_b, self.lanes, self.r)
+            self._chk(self.lib.fbme_gdiag_apply(
+                self.stream, self.lanes, self.r, m1, m2, _ptr(self.factors), _ptr(self.gd_cinv), _ptr(self.gd_ones),
+                _ptr(self.gd_bf), _ptr(self.gd_b), _ptr(v), _ptr(out), _ptr(self.lr_part), _ptr(self.corr),
+                _ptr(self.pc_part) if dot is not None else None, _ptr(dot)), "gdiag_apply")
+            return
         if self.precond != "low_rank":
             if self.precond == "circulant":
                 t = self.t
@@ -498,7 +593,7 @@ class _Solver:
                 self.stream, self.nsys, n1, n2, m1, m2, t.codes[n2], t.sample_codes[m2], _ptr(self.gamma),
                 _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b), _ptr(self.circ_eig), _ptr(t.tw[n2]),
                 _ptr(t.tw[m2])), "circulant_spectrum")
-        if self.precond != "low_rank":
+        if self.precond not in ("low_rank", "grid_diagonal"):
             return
         self._chk(self.lib.fbme_topk32(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
@@ -511,6 +606,8 @@ class _Solver:
             self._chk(self.lib.fbme_lowrank_map(self.stream, self.nsys, n1, n2, _ptr(self.top_idx),
                                                 _ptr(self.lr_map)),
                       "lowrank_map")
+        if self.precond == "grid_diagonal":
+            self._gdiag_setup()
 
     def spectrum_setup(self):
         t = self.t
@@ -769,21 +866,26 @@ class SolverStall(RuntimeError):
 
 def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
            probes_override=None, warm_start=None, stall_check=None,
-           observation_mask=None, dense=False) -> SBLResult:
+           observation_mask=None, dense=False, precond=None, precond_net=None) -> SBLResult:
     """Run the general solver in mode='cg' on y [B, M1, M2] (GPU, complex64),
     optionally on the samples of a boolean observation_mask [M1, M2].
 
     warm_start(solver) may replace the initial variances after the plain
     initialisation; stall_check(solver, pcg_iterations, cfg, sum_gamma_before)
-    is consulted after every EM iteration and raises SolverStall when true."""
+    is consulted after every EM iteration and raises SolverStall when true.
+
+    precond='grid_diagonal' replaces the low-rank preconditioner's scalar
+    background by the grid-diagonal one (fbme_gdiag_weights); precond_net,
+    a torch_sbl.learned.PrecondNet, adds its per-cell correction."""
     _check_supported(cfg)
     with torch.cuda.device(y.device):
-        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check, observation_mask, dense)
+        return _run(y, grid, cfg, batch_shape, probes_override, warm_start, stall_check, observation_mask, dense,
+                    precond, precond_net)
 
 
 def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None, observation_mask=None,
-         dense=False):
-    s = _Solver(y, grid, cfg, probes_override, observation_mask, dense)
+         dense=False, precond=None, precond_net=None):
+    s = _Solver(y, grid, cfg, probes_override, observation_mask, dense, precond, precond_net)
     s.init()
     if warm_start is not None:
         warm_start(s)

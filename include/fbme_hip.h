@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 6
+#define FBME_ABI_VERSION 7
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -349,6 +349,43 @@ int fbme_lowrank_apply(void* stream, int lanes, int rhs, int m1, int m2, const v
 int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* sinv,
                            const float* bg, const void* v, void* out, void* part, void* corr, double* dot_part,
                            double* dot);
+/* out = vbase / bg - F S^-1 F^H vproj / bg^2: the low-rank apply with the
+ * projection taken from a second vector (vproj = vbase is
+ * fbme_lowrank_apply).  The grid-diagonal preconditioner below uses it with
+ * bg = 1 and S^-1 = C^-1 to form r - F C^-1 F^H B^-1 r (Woodbury form of
+ * (B + F F^H)^-1, the structure of _make_low_rank_preconditioner,
+ * solver.py:5786-5848, with its isotropic background replaced).  part, corr
+ * as fbme_lowrank_apply. */
+int fbme_lowrank_apply_split(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
+                             const void* sinv, const float* bg, const void* vproj, const void* vbase, void* out,
+                             void* part, void* corr);
+/* Grid-diagonal background of the low-rank preconditioner (no counterpart in
+ * the reference, whose background is the scalar noise + omitted variance of
+ * solver.py:5813-5819): B^-1 = A diag(w) A^H, Hermitian positive definite for
+ * any w > 0, with w[b, n] = exp(clamp(o)) / (N (noise_b + N gamma_T[b, n])),
+ * gamma_T = gamma without the 32 exact atoms idx [batch, 32] (from
+ * fbme_topk32).  logscale o [batch, n] (NULL: 0) is the learned per-cell
+ * correction, clamped to +-clamp.  B^-1 v is a matvec with (gamma, noise) =
+ * (w, 0): fbme_grid_matvec on 256 x 256 grids, else fbme_toeplitz_matvec on
+ * the spectrum fbme_gamma_spectrum forms from w. */
+int fbme_gdiag_weights(void* stream, int batch, long long n, const float* gamma, const float* noise, const int* idx,
+                       const float* logscale, float clamp, float* w);
+/* The 32 factor columns of fbme_lowrank_setup as lanes: out [batch, 32, M1 M2]
+ * complex64, out[b, k, m] = exp(-2 pi i (m1 k1 / N1 + m2 k2 / N2)) sqrt(gamma_k). */
+int fbme_lowrank_columns(void* stream, int batch, int m1, int m2, const void* factors, void* out);
+/* C^-1 [batch, 32, 32] complex64 of C = I + F^H B^-1 F, bf = B^-1 F
+ * [batch, 32, M1 M2] (fbme_lowrank_columns through the B^-1 matvec); FP64
+ * sums and Cholesky inverse, status[b] = 1 on a non-positive pivot.
+ * part: batch * 32 * 64 * 32 double2. */
+int fbme_gdiag_system(void* stream, int batch, int m1, int m2, const void* factors, const void* bf, void* part,
+                      void* cinv, int* status);
+/* out = (B + F F^H)^-1 v = b - U C^-1 F^H b, given b = B^-1 v (the B^-1
+ * matvec), bf = U = B^-1 F and cinv from fbme_gdiag_system; ones: batch
+ * floats 1.0.  With dot, dot[lane] = Re <v, out> (FP64).  part, corr,
+ * dot_part as fbme_lowrank_apply_dot. */
+int fbme_gdiag_apply(void* stream, int lanes, int rhs, int m1, int m2, const void* factors, const void* cinv,
+                     const float* ones, const void* bf, const void* b, const void* v, void* out, void* part,
+                     void* corr, double* dot_part, double* dot);
 /* The same preconditioner for N1 = 256 grids through the 256-point column
  * transform: the projection is the column IFFT of v restricted to the
  * atoms' distinct rows, reduced onto the 32 atoms in the same pass; the
@@ -593,6 +630,13 @@ int fbme_cnn_prepare(void* stream, const float* params, void* wf, void* wb);
  * [B, n]; bpe = sum |bp|^2 and gsum = sum gamma per scene (FP64). */
 int fbme_cnn_features(void* stream, int batch, int n, const void* bp, const float* gamma, const double* bpe,
                       const double* gsum, void* x);
+/* Inputs of the preconditioner network: channels 0, 1 as fbme_cnn_features,
+ * channel 2 = log(1 + N gamma / noise) / 8 (noise [batch]). */
+int fbme_cnn_features_precond(void* stream, int batch, int n, const void* bp, const float* gamma,
+                              const double* bpe, const double* gsum, const float* noise, void* x);
+/* dz4 [count, 16] bf16 from a loss gradient g [count] at the network output
+ * (channel 0), for losses formed outside fbme_cnn_loss. */
+int fbme_cnn_output_grad(void* stream, long long count, const float* g, void* dz4);
 /* out [B, H, W] fp32; act: 3 * B * H * W * 32 bf16 hidden activations. */
 int fbme_cnn_forward(void* stream, int batch, int h, int w, const void* x, const void* wf, const float* params,
                      void* act, float* out);

@@ -210,11 +210,50 @@ def circulant_preconditioner(gamma, noise, op: Operator, eps):
     return ("circulant", np.maximum(eig, floor[:, None, None]), op)
 
 
+def grid_diagonal_weights(gamma, noise, order, logscale=None, clamp=4.0):
+    """w[b, n] = exp(clamp(o)) / (N (noise + N gamma_T)), gamma_T = gamma
+    without the atoms `order` [B, K] (no reference counterpart: the scalar
+    background noise + omitted of solver.py:5813-5819 taken per grid cell).
+    For M = N samples B^-1 = A diag(w) A^H is exactly Q^-1 of the variances
+    gamma_T (A = F, F F^H = N I)."""
+    b, n = gamma.shape
+    gt = gamma.astype(np.float64).copy()
+    gt[np.arange(b)[:, None], order] = 0.0
+    w = 1.0 / (n * (noise[:, None] + n * gt))
+    if logscale is not None:
+        w = w * np.exp(np.clip(np.asarray(logscale, np.float64).reshape(b, n), -clamp, clamp))
+    return w
+
+
+def grid_diagonal_preconditioner(gamma, noise, op: Operator, rank=32, logscale=None, clamp=4.0):
+    """(B + F F^H)^-1 with B^-1 = A diag(w) A^H (grid_diagonal_weights) and F
+    the `rank` strongest atoms scaled by sqrt(gamma): the low-rank
+    preconditioner of solver.py:5786-5848 with its isotropic background
+    replaced.  Applied in the Woodbury form B^-1 - U C^-1 U^H, U = B^-1 F,
+    C = I + F^H B^-1 F (Hermitian positive definite for any w > 0)."""
+    if op.idx is not None:
+        raise NotImplementedError("the grid-diagonal background takes unmasked samples")
+    order = np.argsort(-gamma, axis=1, kind="stable")[:, :rank]
+    vals = np.take_along_axis(gamma, order, 1)
+    w = grid_diagonal_weights(gamma, noise, order, logscale, clamp)
+    factors = op.columns(order) * np.sqrt(np.maximum(vals, 0.0))[:, None, :]      # [B, M, K]
+
+    def binv(v):                                                                    # [B, M, R]
+        return op.fwd(w[:, :, None] * op.adj(v))
+
+    u = binv(factors)
+    c = np.eye(rank) + np.einsum("bmk,bml->bkl", np.conj(factors), u)
+    ci = np.linalg.inv(0.5 * (c + np.conj(np.swapaxes(c, 1, 2))))
+    return ("grid_diagonal", binv, u, ci)
+
+
 def make_preconditioner(kind, gamma, noise, op: Operator, rank, eps, tiny):
     """_make_pcg_preconditioner (solver.py:5726-5762) for 'low_rank',
-    'circulant' and 'none'."""
+    'circulant' and 'none'; 'grid_diagonal' is grid_diagonal_preconditioner."""
     if kind == "none":
         return ("none",)
+    if kind == "grid_diagonal":
+        return grid_diagonal_preconditioner(gamma, noise, op, rank)
     if kind == "circulant":
         return circulant_preconditioner(gamma, noise, op, eps)
     return low_rank_preconditioner(gamma, noise, op, rank, eps, tiny)
@@ -226,6 +265,10 @@ def apply_preconditioner(v, pre):
     kind = pre[0] if isinstance(pre[0], str) else "low_rank"
     if kind == "none":
         return v.copy()
+    if kind == "grid_diagonal":
+        _, binv, u, ci = pre
+        proj = np.conj(np.swapaxes(u, 1, 2)) @ v
+        return binv(v) - u @ (ci @ proj)
     if kind == "circulant":
         _, eig, op = pre
         b, _, r = v.shape

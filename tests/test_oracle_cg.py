@@ -305,3 +305,82 @@ def test_circulant_preconditioner_is_the_strang_approximation():
     z = co.apply_preconditioner(v, ("circulant", eig, op))
     back = np.fft.ifft2(np.fft.fft2(z.reshape(1, m1, m2, 2), axes=(1, 2)) * eig[..., None], axes=(1, 2))
     np.testing.assert_allclose(back.reshape(1, 42, 2), v, atol=1e-10)
+
+
+# ---------------------------------------------------------------------------
+# grid-diagonal background of the learned solver's preconditioner
+# ---------------------------------------------------------------------------
+
+def test_grid_diagonal_background_is_exact_inverse_on_the_full_grid():
+    """M = N: A = F with F F^H = N I, so A diag(1 / (N (noise + N g))) A^H
+    is (noise I + A diag(g) A^H)^-1 for any variances g.  A dropped factor
+    N, or M in place of N, fails this."""
+    grid = samples = (6, 5)
+    rng = np.random.default_rng(11)
+    n = 30
+    gamma = rng.gamma(0.5, 1.0, (1, n))
+    noise = np.array([0.07])
+    a = dense_prefix_matrix(grid, samples)
+    none = np.zeros((1, 0), dtype=np.int64)
+    w = co.grid_diagonal_weights(gamma, noise, none)
+    binv = (a * w[0]) @ a.conj().T
+    q = noise[0] * np.eye(n) + (a * gamma[0]) @ a.conj().T
+    np.testing.assert_allclose(binv @ q, np.eye(n), atol=1e-10)
+
+
+def test_grid_diagonal_preconditioner_is_dense_inverse_of_b_plus_ffh():
+    rng, gamma, noise, a, op = _problem(3)
+    logscale = rng.uniform(-2.0, 2.0, gamma.shape)
+    kind, binv, u, ci = co.grid_diagonal_preconditioner(gamma, noise, op, rank=8, logscale=logscale)
+    v = rng.standard_normal((2, op.m, 3)) + 1j * rng.standard_normal((2, op.m, 3))
+    got = co.apply_preconditioner(v, (kind, binv, u, ci))
+    for b in range(2):
+        order = np.argsort(-gamma[b], kind="stable")[:8]
+        gt = gamma[b].copy()
+        gt[order] = 0.0
+        n = gamma.shape[1]
+        w = np.exp(logscale[b]) / (n * (noise[b] + n * gt))
+        bmat = np.linalg.inv((a * w) @ a.conj().T)
+        cols = a[:, order] * np.sqrt(gamma[b, order])
+        pmat = bmat + cols @ cols.conj().T
+        np.testing.assert_allclose(got[b], np.linalg.solve(pmat, v[b]), rtol=1e-8, atol=1e-10)
+        # Hermitian positive definite for any log-scale
+        dense = co.apply_preconditioner(np.eye(op.m)[None].repeat(2, 0), (kind, binv, u, ci))[b]
+        np.testing.assert_allclose(dense, dense.conj().T, atol=1e-10)
+        assert np.linalg.eigvalsh(0.5 * (dense + dense.conj().T)).min() > 0
+
+
+def test_grid_diagonal_preconditioner_is_q_for_at_most_32_atoms():
+    """With at most 32 nonzero variances gamma_T = 0, B^-1 = A A^H / (N noise)
+    = I / noise for any M < N, and B + F F^H = Q: PCG takes one step."""
+    grid, samples = (16, 12), (9, 7)
+    rng = np.random.default_rng(5)
+    n = 16 * 12
+    gamma = np.full((1, n), 0.0)
+    gamma[0, rng.choice(n, 20, replace=False)] = rng.gamma(1.0, 1.0, 20)
+    noise = np.array([0.03])
+    op = co.Operator(grid, samples)
+    pre = co.grid_diagonal_preconditioner(gamma, noise, op)
+    v = rng.standard_normal((1, op.m, 2)) + 1j * rng.standard_normal((1, op.m, 2))
+    np.testing.assert_allclose(co.apply_preconditioner(op.q(v, gamma, noise), pre), v, atol=1e-9)
+
+
+def test_grid_diagonal_pcg_converges_to_the_dense_solution_in_fewer_steps():
+    """PCG with the grid-diagonal background solves the same system as with
+    the low-rank one (dense solution), on a clustered 32 x 32 / 20 x 20
+    scene state, in no more iterations."""
+    from fbme_amd import scenes
+    sc = scenes.ellipse_scene(1000, (32, 32), (20, 20), k=60, snr_db=10.0)
+    y = sc.y.reshape(1, -1).astype(np.complex128)
+    op = co.Operator((32, 32), (20, 20))
+    gamma = (np.abs(sc.x.reshape(1, -1)) ** 2 + 1e-3)
+    noise = np.array([np.mean(np.abs(y) ** 2) * 0.02])
+    a = dense_prefix_matrix((32, 32), (20, 20))
+    exact = np.linalg.solve(noise[0] * np.eye(op.m) + (a * gamma[0]) @ a.conj().T, y[0])
+    its = {}
+    for kind in ("low_rank", "grid_diagonal"):
+        x, it = co.pcg(y[:, :, None], gamma, noise, op, x0=None, tol=1e-10, max_iter=500, check=1, rank=32,
+                       precond=kind, eps=np.finfo(np.float64).eps, tiny=1e-300)
+        np.testing.assert_allclose(x[0, :, 0], exact, rtol=1e-8, atol=1e-8 * np.abs(exact).max())
+        its[kind] = it
+    assert its["grid_diagonal"] <= its["low_rank"], its
