@@ -485,9 +485,12 @@ def dense_workload(device, cpu=True):
 
 
 def learned_comparison(device, count):
-    """Learned warm start against the plain general solver (mode='cg', 16
-    probes, PCG tolerance 1e-4), one 256x256/160x160 K=300 10 dB scene per
-    solve, held-out seeds."""
+    """Learned warm start and learned PCG preconditioner against the plain
+    general solver (mode='cg', 16 probes, PCG tolerance 1e-4), one
+    256x256/160x160 K=300 10 dB scene per solve, held-out seeds.  Kinds:
+    plain (low-rank preconditioner, plain start), learned (warm start),
+    learned_full (warm start + learned grid-diagonal preconditioner),
+    precond (learned preconditioner, plain start)."""
     import numpy as np
     import torch
 
@@ -496,26 +499,40 @@ def learned_comparison(device, count):
     if not os.path.exists(learned.WEIGHTS):
         return None
     net = learned.WarmStartNet.load(device)
+    pnet = learned.PrecondNet.load(device) if os.path.exists(learned.PRECOND_WEIGHTS) else None
     sc = [scenes.ellipse_scene(1300 + i, (256, 256), (160, 160), k=300, snr_db=10.0) for i in range(count)]
     w = net.warm_iter
     runs = [("plain_reference_budget", "plain", 100), ("plain_target_iter", "plain", net.target_iter),
             ("plain_warm_budget", "plain", w), ("learned", "learned", w),
             (f"plain_{w - 1}", "plain", w - 1), (f"learned_{w - 1}", "learned", w - 1)]
+    if pnet is not None:
+        runs += [("learned_full", "learned_full", w), (f"learned_full_{w - 1}", "learned_full", w - 1),
+                 ("precond_target_iter", "precond", net.target_iter), ("precond_warm_budget", "precond", w)]
     out = {"workload": f"{count} single-scene solves, 256x256/160x160, K=300, 10 dB",
            "target_iter": net.target_iter, "warm_iter": net.warm_iter}
+
+    def solve(kind, y, cfg, batch):
+        if kind == "learned":
+            return learned.learned_sbl_2d(y, (256, 256), net, cfg)
+        if kind == "learned_full":
+            return learned.learned_sbl_2d(y, (256, 256), net, cfg, precond_net=pnet)
+        if kind == "precond":
+            return learned.learned_sbl_2d(y, (256, 256), None, cfg, precond_net=pnet)
+        return pcg.sbl_cg(y, (256, 256), cfg, batch_shape=(batch,))
+
+    def config(iters):
+        return SBLConfig(mode="cg", device=str(device), max_iter=iters, min_iter=1, tol=0.0,
+                         hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4, return_history=True)
+
     for name, kind, iters in runs:
-        cfg = SBLConfig(mode="cg", device=str(device), max_iter=iters, min_iter=1, tol=0.0,
-                        hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
-        y0 = torch.from_numpy(sc[0].y[None]).to(device)
-        (learned.learned_sbl_2d(y0, (256, 256), net, cfg) if kind == "learned"
-         else pcg.sbl_cg(y0, (256, 256), cfg, batch_shape=(1,)))
+        cfg = config(iters)
+        solve(kind, torch.from_numpy(sc[0].y[None]).to(device), cfg, 1)
         torch.cuda.synchronize()
         secs, its, nm, fb = [], [], [], 0.0
         for s in sc:
             y = torch.from_numpy(s.y[None]).to(device)
             t0 = time.perf_counter()
-            r = (learned.learned_sbl_2d(y, (256, 256), net, cfg) if kind == "learned"
-                 else pcg.sbl_cg(y, (256, 256), cfg, batch_shape=(1,)))
+            r = solve(kind, y, cfg, 1)
             torch.cuda.synchronize()
             secs.append(time.perf_counter() - t0)
             its.append(float(np.sum(r.history["cg_iterations"])))
@@ -526,17 +543,15 @@ def learned_comparison(device, count):
                      "median_nmse_db": float(np.median(nm)), "fallbacks": fb}
         print(f"[bench] learned comparison {name}: {out[name]}", file=sys.stderr, flush=True)
     # batched: all scenes of the sample in one solve (probes in the lane
-    # dimension of every scene); a stalled warm start reruns the batch plain
+    # dimension of every scene); a stalled learned solve reruns the batch plain
     yb = torch.from_numpy(np.stack([s.y for s in sc])).to(device)
-    for name, kind, iters in [runs[1], runs[3], runs[5]]:
-        cfg = SBLConfig(mode="cg", device=str(device), max_iter=iters, min_iter=1, tol=0.0,
-                        hutchinson_probes=16, auto_probe_scaling=False, cg_tolerance=1e-4)
-        solve = ((lambda: learned.learned_sbl_2d(yb, (256, 256), net, cfg)) if kind == "learned"
-                 else (lambda: pcg.sbl_cg(yb, (256, 256), cfg, batch_shape=(count,))))
-        solve()
+    batched = [runs[1], runs[3]] + ([runs[6], runs[8]] if pnet is not None else []) + [runs[5]]
+    for name, kind, iters in batched:
+        cfg = config(iters)
+        solve(kind, yb, cfg, count)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = solve()
+        r = solve(kind, yb, cfg, count)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         x = r.x.reshape(count, -1).cpu().numpy()

@@ -1,11 +1,19 @@
-"""Learned warm start of the general solver (mode='cg').
+"""Learned warm start and learned PCG preconditioner of the general solver
+(mode='cg').
 
-A small convolutional network maps the backprojection magnitude |A^H y| and
-the solver's initial variances gamma_0 to a per-cell log-variance correction:
-gamma_0 <- gamma_0 * exp(net(log|A^H y|^2, log gamma_0)).  The corrected
-variances start the EM iterations and, through the low-rank preconditioner
-that is built from the current variances (isotropic background from the
-tail plus the strongest atoms), also precondition the first PCG solves.
+Warm start: a small convolutional network maps the backprojection magnitude
+|A^H y| and the solver's initial variances gamma_0 to a per-cell
+log-variance correction: gamma_0 <- gamma_0 * exp(net(log|A^H y|^2,
+log gamma_0)).  The corrected variances start the EM iterations.
+
+Preconditioner: a second network of the same shape (PrecondNet) maps
+|A^H y| and the current variances to the per-cell log-scale o of the
+grid-diagonal background of the PCG preconditioner, P = B + F F^H with
+B^-1 = A diag(w) A^H, w = exp(clamp(o)) / (N (noise + N gamma_T)) and F the
+32 strongest atoms (the low-rank preconditioner of solver.py:5786-5848 with
+its scalar background taken per grid cell).  P is Hermitian positive
+definite for any output.  It is trained by torch_sbl.train_precond on an
+unrolled-PCG loss.
 
 The network (4 layers, 3x3, 32 channels, BF16 weights on MFMA with FP32
 accumulation) runs and trains on the GPU through the fbme_cnn_* entry points
@@ -13,7 +21,7 @@ of libfbme_hip.so.  Training data are trajectories of the plain solver:
 inputs from its initial state, targets from its variances after target_iter
 EM iterations, so a warm-started solve skips that part of the trajectory.
 
-The plain solver is the fallback: if a warm-started solve stalls (a PCG solve
+The plain solver is the fallback: if a learned solve stalls (a PCG solve
 exhausts cg_max_iter, the variances blow up, or the noise becomes
 non-finite) the batch is solved again from the plain initialisation, and the
 result says so in history['learned_fallback'].
