@@ -9,6 +9,7 @@ overwritten by the next replay (torch_sbl/cuda_graph.py:12-20).
 
 from __future__ import annotations
 
+import ctypes
 from copy import deepcopy
 from math import prod
 from typing import Any, Sequence
@@ -45,13 +46,8 @@ class CUDAGraphSBLPlan:
                 "Capture the core SBL stage only; motion, off-grid, and "
                 "snapshot-covariance refinement are not graph-static.")
         config.validate()
-        if config.mode != "uamp":
-            raise NotImplementedError(
-                "Graph plans on MI355X capture the UAMP + active-set path "
-                "(mode='uamp').")
         if example_data.dtype != torch.complex64:
             raise NotImplementedError("Graph plans run complex64 data.")
-        _native.check_supported(config)
         self.grid_shape = tuple(int(v) for v in grid_shape)
         if len(self.grid_shape) != 2:
             raise ValueError("grid_shape must be (N1, N2).")
@@ -67,6 +63,17 @@ class CUDAGraphSBLPlan:
                 "Observed dimensions cannot exceed reconstruction dimensions.")
         batch_shape = self.shape[:-2]
         batch = prod(batch_shape) if batch_shape else 1
+        # mode resolution of fast_sbl (solver.py:1025-1036)
+        mode = self.config.mode
+        if mode == "auto":
+            mode = ("dense" if prod(samples) <= self.config.dense_threshold
+                    else "cg")
+        self._solver = None
+        if mode in ("cg", "dense"):
+            self._capture_general(example_data, samples, batch, batch_shape,
+                                  mode == "dense", warmup)
+            return
+        _native.check_supported(config)
         opt = _native.plan_options(self.config, self.grid_shape, samples)
         shapes = [(self.grid_shape[0], self.grid_shape[1], samples[0],
                    samples[1])] * batch
@@ -96,6 +103,43 @@ class CUDAGraphSBLPlan:
                     "Graph capture failed for this SBL configuration.") from error
         self._result = self._views.result()
 
+    def _capture_general(self, example_data, samples, batch, batch_shape,
+                         dense, warmup):
+        """mode='cg' with a fixed PCG budget, or mode='dense': the general
+        solver's EM loop captured once (torch_sbl.pcg.solve).  Solves whose
+        loop reads the device (PCG early exit, active-set switch,
+        convergence checks) cannot be captured, as in the reference, where
+        the capture itself fails."""
+        from . import pcg
+        why = pcg.capturable(self.config, batch, dense)
+        if why is not None:
+            raise RuntimeError(
+                "Graph capture failed for this SBL configuration: " + why + ".")
+        pcg._check_supported(self.config)
+        with torch.cuda.device(self.device):
+            self.input_buffer = torch.empty_like(example_data)
+            self.input_buffer.copy_(example_data)
+            flat = self.input_buffer.view(batch, *samples)
+            side = torch.cuda.Stream(device=self.device)
+            side.wait_stream(torch.cuda.current_stream(self.device))
+            with torch.cuda.stream(side):
+                self._solver = pcg._Solver(flat, self.grid_shape, self.config,
+                                           dense=dense)
+                for _ in range(warmup):
+                    pcg.solve(self._solver, batch_shape, capture=True)
+            torch.cuda.current_stream(self.device).wait_stream(side)
+            torch.cuda.synchronize(self.device)
+            self._graph = torch.cuda.CUDAGraph()
+            try:
+                with torch.cuda.graph(self._graph):
+                    self._solver.stream = ctypes.c_void_p(
+                        torch.cuda.current_stream(self.device).cuda_stream)
+                    self._result = pcg.solve(self._solver, batch_shape,
+                                             capture=True)
+            except Exception as error:
+                raise RuntimeError(
+                    "Graph capture failed for this SBL configuration.") from error
+
     @property
     def result(self) -> SBLResult:
         """Plan-owned output of the most recent replay."""
@@ -103,7 +147,17 @@ class CUDAGraphSBLPlan:
 
     @property
     def native_plan(self):
+        if self._solver is not None:
+            raise AttributeError("general-solver graph plans have no native FBME plan")
         return self._plan
+
+    def factorization_failed(self) -> bool:
+        """Dense mode: whether the last replay's FP64 Cholesky met a
+        non-positive pivot after its loading (the condition solver.py:2971
+        raises on; a replay cannot raise, so it is read here)."""
+        if self._solver is None or not self._solver.dense:
+            return False
+        return bool(self._solver.status.any())
 
     def replay(self, data: Tensor | None = None, *,
                synchronize: bool = False) -> SBLResult:

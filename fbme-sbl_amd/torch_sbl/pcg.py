@@ -384,6 +384,26 @@ class _Solver:
         self.mask = None
         self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
 
+    def reset(self):
+        """Start a new solve on the data in the input tensor the solver was
+        built on: data lanes, their norms and the iteration state are
+        reloaded on the device (fill and copy kernels only, so a captured
+        solve replays on new data)."""
+        if self.obs is not None:
+            raise NotImplementedError("a masked solver is not reloaded in place")
+        if self.shared:
+            self.rhs[0, :self.b] = self.y
+        else:
+            self.rhs[:, 0] = self.y
+        self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
+        self.x.zero_()
+        self.mean.zero_()
+        self.prev.zero_()
+        self.status.zero_()
+        self.have_solution = False
+        self.support = None
+        self.mask = None
+
     def _gdiag_buffers(self):
         """Buffers of the grid-diagonal preconditioner: the weights w, the
         factor columns F and B^-1 F as 32 lanes per system, C^-1, and the
@@ -886,6 +906,37 @@ def sbl_cg(y: torch.Tensor, grid, cfg: SBLConfig, *, batch_shape=(),
 def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_check=None, observation_mask=None,
          dense=False, precond=None, precond_net=None):
     s = _Solver(y, grid, cfg, probes_override, observation_mask, dense, precond, precond_net)
+    return solve(s, batch_shape, warm_start, stall_check)
+
+
+def capturable(cfg: SBLConfig, batch: int, dense: bool) -> str | None:
+    """Why a fixed-shape solve would synchronise with the host (None: it
+    does not, and `solve(..., capture=True)` can be captured into a graph).
+    These are the host reads of the reference's loop (solver.py:1316-1341,
+    5700-5708, 1800-1904) that make its own graph capture fail."""
+    if cfg.tol > 0.0 or cfg.return_history or cfg.verbose or cfg.eager_change_tracking:
+        return "convergence checks and history read the device"
+    if not dense and cfg.cg_fixed_iterations <= 0:
+        return "PCG's early exit reads the lane activity (set cg_fixed_iterations > 0)"
+    if (not dense and cfg.active_set and batch == 1 and not cfg.joint_sparsity
+            and cfg.max_iter >= cfg.active_set_start):
+        return "the active-set switch reads the variances"
+    return None
+
+
+def solve(s: "_Solver", batch_shape=(), warm_start=None, stall_check=None, capture: bool = False) -> SBLResult:
+    """The EM loop on a built solver (solver.py:1314-1904).  With capture
+    the loop issues device work only (the configuration must pass
+    `capturable`): the data lanes are reloaded from the input tensor, no
+    value is read back, and dense mode's factorisation status stays in
+    s.status for the caller to check."""
+    cfg = s.cfg
+    dense = s.dense
+    if capture:
+        why = capturable(cfg, s.b, dense)
+        if why is not None:
+            raise RuntimeError(f"this solve cannot be captured: {why}")
+        s.reset()
     s.init()
     if warm_start is not None:
         warm_start(s)
@@ -913,7 +964,7 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
             raise SolverStall(f"warm-started solve stalled at iteration {it}")
         check = (convergence and it >= min_iters
                  and ((it - min_iters) % cfg.convergence_check_interval == 0 or it == cfg.max_iter))
-        need = cfg.return_history or check or cfg.verbose or it == cfg.max_iter
+        need = not capture and (cfg.return_history or check or cfg.verbose or it == cfg.max_iter)
         if need:
             sums = s.sums.cpu().numpy()
             change = float(np.max(np.sqrt(sums[:, 1]) / np.maximum(np.sqrt(sums[:, 2]), F32_EPS)))
@@ -936,7 +987,7 @@ def _run(y, grid, cfg, batch_shape, probes_override, warm_start=None, stall_chec
     final_tol = max(cfg.cg_tolerance, tol_floor)
     s.posterior(final_tol)
     s.em(update=False)
-    if dense and bool(s.status.any()):
+    if dense and not capture and bool(s.status.any()):
         if bool((s.status == 2).any()):
             raise RuntimeError("an observed-sample index lies outside the sample grid.")
         raise RuntimeError("CUDA Cholesky failed after deterministic diagonal loading.")

@@ -92,3 +92,54 @@ def test_batched_default_plan_equals_single(device):
         one = make_cuda_graph_sbl_plan(torch.from_numpy(s.y).to(device), (64, 64), warmup=1).replay(synchronize=True)
         assert torch.equal(res.x[i], one.x)
         assert torch.equal(res.truncation_delta_bound[i], one.truncation_delta_bound)
+
+
+def _general_cfg(device, mode, **kw):
+    from torch_sbl import SBLConfig
+    base = dict(mode=mode, device=str(device), max_iter=6, min_iter=1, tol=0.0, return_history=False)
+    if mode == "cg":
+        base.update(cg_fixed_iterations=12, hutchinson_probes=8, auto_probe_scaling=False)
+    base.update(kw)
+    return SBLConfig(**base)
+
+
+@pytest.mark.parametrize("mode,grid,samples,batch", [("cg", (64, 64), (40, 40), 2),
+                                                      ("cg", (256, 256), (160, 160), 1),
+                                                      ("dense", (32, 32), (24, 24), 2)])
+def test_general_solver_graph_plan_equals_eager(device, mode, grid, samples, batch):
+    """mode='cg' with a fixed PCG budget and mode='dense' captured as one
+    graph: every replay, on the example and on new data, equals the eager
+    solve of the same data bitwise (same kernels in the same order)."""
+    from fbme_amd import scenes
+    from torch_sbl import CUDAGraphSBLPlan, sbl_2d
+    print("general graph plan", mode, grid, samples, batch, flush=True)
+    cfg = _general_cfg(device, mode)
+    sets = []
+    for seed in (1500, 1510):
+        sc = [scenes.uniform_scene(seed + i, grid, samples, k=max(6, grid[0] * grid[1] // 400), snr_db=15.0)
+              for i in range(batch)]
+        sets.append(torch.from_numpy(np.stack([s.y for s in sc]).astype(np.complex64)).to(device))
+    plan = CUDAGraphSBLPlan(sets[0], grid, config=cfg, warmup=1)
+    for data in (sets[1], sets[0], sets[1]):
+        res = plan.replay(data, synchronize=True)
+        ref = sbl_2d(data, grid, config=cfg)
+        assert res.mode == ref.mode == mode
+        assert torch.equal(res.x, ref.x)
+        assert torch.equal(res.posterior_variance, ref.posterior_variance)
+        assert torch.equal(res.noise_variance, ref.noise_variance)
+    assert not plan.factorization_failed()
+
+
+def test_general_solver_graph_plan_refuses_host_reads(device):
+    """The captures the reference cannot make either fail with RuntimeError:
+    PCG with early exit (cg_fixed_iterations=0), the single-scene
+    active-set switch within max_iter, convergence checks."""
+    from fbme_amd import scenes
+    from torch_sbl import CUDAGraphSBLPlan
+    sc = scenes.uniform_scene(1520, (64, 64), (40, 40), k=10, snr_db=15.0)
+    data = torch.from_numpy(sc.y[None].astype(np.complex64)).to(device)
+    for kw in (dict(cg_fixed_iterations=0), dict(max_iter=40)):
+        with pytest.raises(RuntimeError, match="Graph capture failed"):
+            CUDAGraphSBLPlan(data, (64, 64), config=_general_cfg(device, "cg", **kw), warmup=1)
+    with pytest.raises(ValueError):
+        CUDAGraphSBLPlan(data, (64, 64), config=_general_cfg(device, "cg", tol=1e-4), warmup=1)
