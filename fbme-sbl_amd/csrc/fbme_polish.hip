@@ -1031,6 +1031,40 @@ k_output(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work
 }
 
 
+// Output of a plan without a polish (uamp_polish_size = 0,
+// solver.py:1907-1925, 2081-2126): the final E-step's mean and variance,
+// precision 1 / gamma, the support mask |mu|^2 / v > log(N / budget), and per
+// scene the noise and the clamped dof of the final E-step (no evidence: nle
+// is NaN).
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_output_uamp(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
+              const double* __restrict__ sc, const float2* __restrict__ mean, const float* __restrict__ var,
+              const float* __restrict__ gamma, float2* __restrict__ x, float* __restrict__ pvar,
+              float* __restrict__ prec, unsigned char* __restrict__ mask, float threshold, int with_mask,
+              float* __restrict__ noise_out, float* __restrict__ dof_out, float* __restrict__ nle_out) {
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x) {
+    const FbmeWork wk = work[w];
+    const FbmeScene s = scenes[wk.scene];
+    const float thr = (float)log((double)s.n1 * (double)s.n2 / (double)threshold);
+    const long long g0 = s.grid_off + wk.first;
+    for (int e = threadIdx.x; e < wk.count; e += FBME_BLOCK) {
+      const long long gi = g0 + e;
+      const float2 mu = mean[gi];
+      const float v = var[gi];
+      x[gi] = mu;
+      pvar[gi] = v;
+      prec[gi] = 1.0f / gamma[gi];
+      if (with_mask) mask[gi] = ((mu.x * mu.x + mu.y * mu.y) / fmaxf(v, 1.17549435e-38f)) > thr ? 1 : 0;
+    }
+    if (wk.slot == 0 && threadIdx.x == 0) {
+      const double* scs = sc + (long long)wk.scene * SC_COUNT;
+      noise_out[s.orig] = (float)scs[SC_NOISE];
+      dof_out[s.orig] = (float)scs[SC_DOF];
+      if (nle_out) nle_out[s.orig] = NAN;
+    }
+  }
+}
+
 // Exact reduced posterior on a given support with diffuse-tail loading
 // nu = noise + max(sum gamma - sum gamma_S, 0): the active-set posterior of
 // the general solver (solver.py:3591-3779 with tail_loading=None,
@@ -1274,6 +1308,17 @@ int fbme_launch_output(const FbmeScene* scenes, const FbmeWork* work, int nwork,
   if (nwork <= 0) return 0;
   hipLaunchKernelGGL(k_output, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork, sc,
                      (const PolishResult*)results, (float2*)x, pvar, prec, mask, threshold, with_mask, tail_gamma);
+  return (int)hipGetLastError();
+}
+
+int fbme_launch_output_uamp(const FbmeScene* scenes, const FbmeWork* work, int nwork, int grid, const double* sc,
+                            const void* mean, const float* var, const float* gamma, void* x, float* pvar, float* prec,
+                            unsigned char* mask, float threshold, int with_mask, float* noise_out, float* dof_out,
+                            float* nle_out, hipStream_t stream) {
+  if (nwork <= 0) return 0;
+  hipLaunchKernelGGL(k_output_uamp, dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork, sc,
+                     (const float2*)mean, var, gamma, (float2*)x, pvar, prec, mask, threshold, with_mask, noise_out,
+                     dof_out, nle_out);
   return (int)hipGetLastError();
 }
 

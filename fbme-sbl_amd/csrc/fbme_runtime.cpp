@@ -48,6 +48,9 @@ int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, c
 int fbme_launch_gamma_sum(const FbmeScene*, int, const float*, double*, hipStream_t);
 int fbme_launch_exchange_score(const FbmeScene*, const FbmeWork*, int, int, const void*, const void*, const void*,
                                const void*, uint32_t*, int*, int, hipStream_t);
+int fbme_launch_output_uamp(const FbmeScene*, const FbmeWork*, int, int, const double*, const void*, const float*,
+                            const float*, void*, float*, float*, unsigned char*, float, int, float*, float*, float*,
+                            hipStream_t);
 int fbme_launch_output(const FbmeScene*, const FbmeWork*, int, int, const double*, const void*, void*, float*,
                        float*, unsigned char*, float, int, const float*, hipStream_t);
 int fbme_launch_truncation(const FbmeScene*, int, long long, long long, const float*, const void*, const double*,
@@ -218,8 +221,8 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
   };
   if (nscenes < 1) return fail("a plan needs at least one scene");
   if (!opt) return fail("options pointer is null");
-  if (opt->polish_size < 1 || opt->polish_size > FBME_KMAX)
-    return fail("polish_size must be in [1, " + std::to_string(FBME_KMAX) + "]");
+  if (opt->polish_size < 0 || opt->polish_size > FBME_KMAX)
+    return fail("polish_size must be in [0, " + std::to_string(FBME_KMAX) + "] (0: no polish)");
   if (opt->polish_iterations < 1 || opt->polish_iterations > 8)
     return fail("polish_iterations must be in [1, 8]");
   if (opt->exchange_size < 0 || opt->exchange_size > 32) return fail("exchange_size must be in [0, 32]");
@@ -615,15 +618,20 @@ int fbme_plan_set_mask(fbme_plan* p, const unsigned char* mask, void* stream) {
   return 0;
 }
 
-int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold, void* stream) {
-  (void)support_threshold;
+namespace {
+
+// One phase of a plan's stage sequence on stream st: 0 the initialisation,
+// 1 UAMP iteration t_step (track: also the relative change of the step into
+// SC_CHANGE), 2 the final E-step, polish (or, without a polish, the UAMP
+// posterior) and outputs.  fbme_plan_run runs 0, 1 for t = 1 .. max_iters
+// and 2.
+int plan_phase(const fbme_plan* p, const FbmeIO* io, hipStream_t st, int phase, int t_step, int track) {
   if (!p || !io || !p->bound) return FBME_ERR_ARG;
   if (p->masked && !p->mask) return FBME_ERR_ARG;
   if (!io->y || !io->x || !io->posterior_variance || !io->precision || !io->noise || !io->dof || !io->nle ||
       !io->trace)
     return FBME_ERR_ARG;
   if (p->opt.truncation_diagnostic && !io->truncation) return FBME_ERR_ARG;
-  hipStream_t st = (hipStream_t)stream;
   char* w = p->ws;
   const FbmeScene* scenes = (const FbmeScene*)(w + p->r_scenes.off);
   const FbmeWork* colw = (const FbmeWork*)(w + p->r_col.off);
@@ -690,24 +698,45 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
                                   part, ps, first, o.uamp_damping, tw, st);
     return r;
   };
-  // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
-  FBME_TRY(col_adjoint(p->col_split, ncol16, 0, 0));
-  // Register-FFT rows form A^H y inside their first UAMP step (every scene
-  // runs at least one: min_iters >= 1); the LDS kernels write it here.
-  FBME_TRY(row(p->row_split, 0, 0, nullptr, nullptr, 0));
-  FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction, o.blend,
-                               o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
-  for (int t = 1; t <= p->max_iters; ++t) {
+  if (phase == 0) {
+    // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
+    FBME_TRY(col_adjoint(p->col_split, ncol16, 0, 0));
+    // Register-FFT rows form A^H y inside their first UAMP step (every scene
+    // runs at least one: min_iters >= 1); the LDS kernels write it here.
+    FBME_TRY(row(p->row_split, 0, 0, nullptr, nullptr, 0));
+    FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction,
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+    return 0;
+  }
+  if (phase == 1) {
+    // UAMP iteration t over the scenes whose budget T_u reaches t
+    const int t = t_step;
+    if (t < 1 || t > p->max_iters) return FBME_ERR_ARG;
     const int nc = p->col_prefix[t - 1], nr = p->row_prefix[t - 1], ns = p->scene_prefix[t - 1];
     const int nc16 = p->col16_prefix[t - 1], nr16 = p->row16_prefix[t - 1];
     if (t > 1) FBME_TRY(col_adjoint(nc, nc16, 1, 1));
-    FBME_TRY(row(nr, nr16, t == 1 ? 1 : 2, nullptr, nullptr, 0));
+    FBME_TRY(row(nr, nr16, (t == 1 ? 1 : 2) + (track ? 3 : 0), nullptr, nullptr, 0));
     FBME_TRY(col_forward(nc, nc16, t == 1 ? 1 : 0));
-    FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, 1, nullptr, o.min_variance, o.noise_fraction, o.blend, o.damping,
-                                 o.noise_shape, o.noise_rate, o.evidence_noise, st));
+    FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, track ? 2 : 1, nullptr, o.min_variance, o.noise_fraction,
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+    return 0;
   }
-  // Final consistency E-step (no prediction) + per-tile top-k candidates.
+  if (phase != 2) return FBME_ERR_ARG;
+  // Final consistency E-step (no prediction).
   FBME_TRY(col_adjoint(p->col_split, ncol16, 1, 0));
+  if (o.polish_size == 0) {
+    // No polish: the UAMP posterior of the final hyperparameters is the
+    // result (solver.py:1907-1925, 2081-2126).
+    FBME_TRY(row(p->row_split, nrow16, 3, nullptr, nullptr, 1));
+    FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 3, nullptr, o.min_variance, o.noise_fraction,
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+    FBME_TRY(fbme_launch_output_uamp(scenes, tilew, ntile, ntile, sc, mean, var, gamma, io->x, io->posterior_variance,
+                                     io->precision, io->support_mask, o.false_alarm,
+                                     (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, io->noise, io->dof, io->nle,
+                                     st));
+    return 0;
+  }
+  // per-tile top-k candidates of the polish selection
   FBME_TRY(row(p->row_split, nrow16, 3, ckey, cidx, io->store_uamp ? 1 : 0));
   // Polish (the un-pruned model first sums the variances for its tail loading).
   if (!o.prune_tail) FBME_TRY(fbme_launch_gamma_sum(scenes, p->nscenes, gamma, sc, st));
@@ -740,6 +769,28 @@ int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold,
   }
 #undef FBME_TRY
   return 0;
+}
+
+}  // namespace
+
+int fbme_plan_run(const fbme_plan* p, const FbmeIO* io, float support_threshold, void* stream) {
+  (void)support_threshold;
+  hipStream_t st = (hipStream_t)stream;
+  int rc = plan_phase(p, io, st, 0, 0, 0);
+  for (int t = 1; !rc && t <= p->max_iters; ++t) rc = plan_phase(p, io, st, 1, t, 0);
+  return rc ? rc : plan_phase(p, io, st, 2, 0, 0);
+}
+
+int fbme_plan_begin(const fbme_plan* p, const FbmeIO* io, void* stream) {
+  return plan_phase(p, io, (hipStream_t)stream, 0, 0, 0);
+}
+
+int fbme_plan_iterate(const fbme_plan* p, int t, int track_change, const FbmeIO* io, void* stream) {
+  return plan_phase(p, io, (hipStream_t)stream, 1, t, track_change ? 1 : 0);
+}
+
+int fbme_plan_finish(const fbme_plan* p, const FbmeIO* io, void* stream) {
+  return plan_phase(p, io, (hipStream_t)stream, 2, 0, 0);
 }
 
 int fbme_plan_run_stage(const fbme_plan* p, int stage, const FbmeIO* io, void* stream) {

@@ -59,7 +59,8 @@ enum FbmeScalar {
   SC_BPE = 5,       // sum |A^H y|^2
   SC_YE = 6,        // ||y||^2
   SC_GSUM = 7,      // sum gamma after the sweep (diffuse tail loading only)
-  SC_COUNT = 8
+  SC_CHANGE = 8,    // ||mu_t - mu_t-1|| / max(||mu_t||, eps) of the last tracked step
+  SC_COUNT = 9
 };
 
 // Per-scene partial sums (doubles), `FBME_PART_STRIDE` slots per quantity.
@@ -68,7 +69,9 @@ enum FbmePart {
   PT_DOF = 1,  // sum relevance                  (row pass)
   PT_VS = 2,   // sum posterior variance         (row pass)
   PT_RES = 3,  // sum |y - A mu|^2               (column pass)
-  PT_COUNT = 4
+  PT_DMU = 4,  // sum |mu_t - mu_t-1|^2          (row pass, tracked steps)
+  PT_MU = 5,   // sum |mu_t|^2                   (row pass, tracked steps)
+  PT_COUNT = 6
 };
 
 

@@ -134,7 +134,10 @@ k_col_adjoint(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__
 // MODE 2: UAMP step: lifted = row pass of A^H of the column output.
 // MODE 3: final E-step: no ARD update, no prediction; writes the polish score
 //         |mu|^2 + v as the top-k candidates of this work item.
-template <int EPT, int MODE>
+// TRACK (modes 1, 2): also the partial sums of |mu_t - mu_t-1|^2 and |mu_t|^2.
+// MODE 3 without cand_key: the final E-step of a plan without a polish (no
+// candidate selection).
+template <int EPT, int MODE, bool TRACK>
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
       float2* __restrict__ mid, float2* __restrict__ bp, float* __restrict__ gamma,
@@ -213,7 +216,7 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
     const float first_scale = update / (dv + noise);
     const float bpe_clamped = fmaxf((float)scs[SC_BPE], F32_EPS);
     const float uniform = power / (float)n_grid;
-    double dof_acc = 0.0, vs_acc = 0.0;
+    double dof_acc = 0.0, vs_acc = 0.0, dmu_acc = 0.0, mu_acc = 0.0;
 #pragma unroll
     for (int q = 0; q < EPT; ++q) {
       const int e = threadIdx.x + q * FBME_BLOCK;
@@ -247,6 +250,11 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
         rel = fmaxf(0.0f, fminf(1.0f, rel));
         dof_acc += (double)rel;
         vs_acc += (double)v;
+        if (TRACK) {
+          const float dx = mu.x - mean_old.x, dy = mu.y - mean_old.y;
+          dmu_acc += (double)(dx * dx + dy * dy);
+          mu_acc += (double)(mu.x * mu.x + mu.y * mu.y);
+        }
         if (MODE == 3) {
           keys[e] = score_key(mu.x * mu.x + mu.y * mu.y + v);
           if (store_final) {
@@ -271,7 +279,18 @@ k_row(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, i
       part[(long long)PT_DOF * pstride + s.row_part + wk.slot] = dof_acc;
       part[(long long)PT_VS * pstride + s.row_part + wk.slot] = vs_acc;
     }
+    if (TRACK) {
+      block_sum2(dmu_acc, mu_acc, red);
+      if (threadIdx.x == 0) {
+        part[(long long)PT_DMU * pstride + s.row_part + wk.slot] = dmu_acc;
+        part[(long long)PT_MU * pstride + s.row_part + wk.slot] = mu_acc;
+      }
+    }
     if (MODE == 3) {
+      if (!cand_key) {   // uniform: a kernel argument
+        __syncthreads();
+        continue;
+      }
       // Top-k candidates of this tile for the polish selection (solver.py:1926-1933).
       __syncthreads();
       const int got = block_select_topk(keys, nel, s.k, pos, hist, misc);
@@ -373,6 +392,10 @@ struct ScalarArgs {
 // mode 1: end of a UAMP iteration: dof clamp, noise EM/evidence update with
 //         damping (triton_kernels.py:375-417, solver.py:1764-1798), data
 //         variance = sum v for the next message update.
+// mode 2: mode 1 and the relative change ||mu_t - mu_t-1|| / max(||mu_t||,
+//         eps) of the step (solver.py:1815-1825) from the tracked row pass.
+// mode 3: the final E-step's clamped dof only (the UAMP posterior without a
+//         polish, solver.py:1907-1925, 2081-2087).
 __global__ void __launch_bounds__(64)
 k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict__ sc,
           const double* __restrict__ part, int pstride, int mode, const unsigned char* __restrict__ active,
@@ -384,20 +407,32 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
   double* scs = sc + (long long)si * SC_COUNT;
   const int n_samp = s.m_obs;
   const double n_grid = (double)s.n1 * s.n2;
-  double q0 = 0.0, q1 = 0.0, q2 = 0.0;
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0;
   if (mode == 0) {
     for (int i = threadIdx.x; i < s.col_items; i += 64) q0 += part_get(part, pstride, PT_YE, s.col_part + i);
   } else {
     for (int i = threadIdx.x; i < s.row_items; i += 64) {
       q0 += part_get(part, pstride, PT_DOF, s.row_part + i);
       q1 += part_get(part, pstride, PT_VS, s.row_part + i);
+      if (mode == 2) {
+        q3 += part_get(part, pstride, PT_DMU, s.row_part + i);
+        q4 += part_get(part, pstride, PT_MU, s.row_part + i);
+      }
     }
-    for (int i = threadIdx.x; i < s.col_items; i += 64) q2 += part_get(part, pstride, PT_RES, s.col_part + i);
+    if (mode != 3)
+      for (int i = threadIdx.x; i < s.col_items; i += 64) q2 += part_get(part, pstride, PT_RES, s.col_part + i);
   }
   q0 = wave_sum(q0);
   q1 = wave_sum(q1);
   q2 = wave_sum(q2);
+  q3 = wave_sum(q3);
+  q4 = wave_sum(q4);
   if (threadIdx.x != 0) return;
+  if (mode == 3) {
+    scs[SC_DOF] = (float)fmin(q0, (double)n_samp * (1.0 - 10.0 * (double)F32_EPS));
+    return;
+  }
+  if (mode == 2) scs[SC_CHANGE] = sqrt(q3) / fmax(sqrt(q4), (double)F32_EPS);
   if (mode == 0) {
     const float power = fmaxf((float)(q0 / n_samp), F32_EPS);
     const float floor_v = fmaxf(power * a.min_variance, F32_TINY);
@@ -485,16 +520,19 @@ int fbme_launch_row(const FbmeScene* scenes, const FbmeWork* work, int nwork, in
                              prior_damping, max_precision, hyper_shape, hyper_rate, blend, cand_key, cand_idx,
                              store_final, score, twtab, stream);
   UampArgs a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
-#define FBME_ROW(M)                                                                                      \
-  hipLaunchKernelGGL((k_row<E, M>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,       \
+#define FBME_ROW(M, T)                                                                                   \
+  hipLaunchKernelGGL((k_row<E, M, T>), dim3(grid), dim3(FBME_BLOCK), 0, stream, scenes, work, nwork,    \
                      (float2*)mid, (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, cand_key, \
                      cand_idx, store_final, (const float2*)twtab)
+  // modes 4 and 5: modes 1 and 2 with the change partials
   FBME_EPT_DISPATCH(ept, {
     switch (mode) {
-      case 0: FBME_ROW(0); break;
-      case 1: FBME_ROW(1); break;
-      case 2: FBME_ROW(2); break;
-      default: FBME_ROW(3); break;
+      case 0: FBME_ROW(0, false); break;
+      case 1: FBME_ROW(1, false); break;
+      case 2: FBME_ROW(2, false); break;
+      case 4: FBME_ROW(1, true); break;
+      case 5: FBME_ROW(2, true); break;
+      default: FBME_ROW(3, false); break;
     }
   })
 #undef FBME_ROW

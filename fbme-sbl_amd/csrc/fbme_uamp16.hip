@@ -158,7 +158,9 @@ __device__ void wave_tile_topk(const uint32_t (&kv)[16], int g, int t, int k, in
 // column output itself and stores the backprojection A^H y (read by the
 // polish) on the way, so the plans launch no separate mode-0 pass for
 // register-FFT rows.
-template <int MODE>
+// TRACK (modes 1, 2): also the partial sums of |mu_t - mu_t-1|^2 and |mu_t|^2
+// for the relative change of the step (eager convergence checks).
+template <int MODE, bool TRACK>
 __global__ void __launch_bounds__(64)
 k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work, int nwork,
         float2* __restrict__ mid, float2* __restrict__ bp, float* __restrict__ gamma,
@@ -213,7 +215,7 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     const float uniform = power / (float)n_grid;
     const float struct_scale = power / bpe_clamped;
     const float ard_scale = 1.0f / (1.0f + a.hyper_shape);
-    double dof_acc = 0.0, vs_acc = 0.0;
+    double dof_acc = 0.0, vs_acc = 0.0, dmu_acc = 0.0, mu_acc = 0.0;
     // The per-element state is loaded after the inverse transform, in two
     // halves, so that little of it is live at once (fewer registers, more
     // resident waves to cover the HBM latency).
@@ -266,12 +268,18 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
         if (act) {
           dof_acc += (double)rel;
           vs_acc += (double)vn;
+          if (TRACK) {
+            const float dx = mu.x - mean_old.x, dy = mu.y - mean_old.y;
+            dmu_acc += (double)(dx * dx + dy * dy);
+            mu_acc += (double)(mu.x * mu.x + mu.y * mu.y);
+          }
         }
         if (MODE == 3) {
           // polish score |mu|^2 + v as an order-preserving key; the tile's
           // top-k runs in k_tile_topk16 (a latency-bound select would hold
-          // this bandwidth-bound pass's registers and waves)
-          if (act) score[k] = score_key(mu.x * mu.x + mu.y * mu.y + vn);
+          // this bandwidth-bound pass's registers and waves).  No score
+          // buffer: the plan has no polish.
+          if (score && act) score[k] = score_key(mu.x * mu.x + mu.y * mu.y + vn);
           if (store_final && act) {
             mean[k] = mu;
             var[k] = vn;
@@ -296,6 +304,13 @@ k_row16(const FbmeScene* __restrict__ scenes, const FbmeWork* __restrict__ work,
     if (threadIdx.x == 0) {
       part[(long long)PT_DOF * pstride + s.row_part + wk.slot] = dof_acc;
       part[(long long)PT_VS * pstride + s.row_part + wk.slot] = vs_acc;
+    }
+    if (TRACK) {
+      block_sum2(dmu_acc, mu_acc, red);
+      if (threadIdx.x == 0) {
+        part[(long long)PT_DMU * pstride + s.row_part + wk.slot] = dmu_acc;
+        part[(long long)PT_MU * pstride + s.row_part + wk.slot] = mu_acc;
+      }
     }
     if (MODE == 3) {
       __syncthreads();
@@ -494,20 +509,26 @@ int fbme_launch_row16(const FbmeScene* scenes, const FbmeWork* work, int nwork, 
                       float blend, uint32_t* cand_key, int* cand_idx, int store_final, uint32_t* score,
                       const void* twtab, hipStream_t stream) {
   if (nwork <= 0) return 0;
-  if (mode == 3 && !score) return FBME_ERR_ARG;
+  // mode 3 without a candidate buffer: the final E-step of a plan without a
+  // polish (no scores, no selection)
+  if (mode == 3 && cand_key && !score) return FBME_ERR_ARG;
+  if (mode == 3 && !cand_key) score = nullptr;
   UampArgs16 a{damping, prior_damping, max_precision, hyper_shape, hyper_rate, blend};
-#define FBME_ROW16(M)                                                                                        \
-  hipLaunchKernelGGL((k_row16<M>), dim3(grid), dim3(64), 0, stream, scenes, work, nwork, (float2*)mid,       \
+#define FBME_ROW16(M, T)                                                                                     \
+  hipLaunchKernelGGL((k_row16<M, T>), dim3(grid), dim3(64), 0, stream, scenes, work, nwork, (float2*)mid,    \
                      (float2*)bp, gamma, (float2*)mean, var, sc, part, pstride, a, store_final, score,       \
                      (const float2*)twtab)
+  // modes 4 and 5: modes 1 and 2 with the change partials
   switch (mode) {
-    case 0: FBME_ROW16(0); break;
-    case 1: FBME_ROW16(1); break;
-    case 2: FBME_ROW16(2); break;
-    default: FBME_ROW16(3); break;
+    case 0: FBME_ROW16(0, false); break;
+    case 1: FBME_ROW16(1, false); break;
+    case 2: FBME_ROW16(2, false); break;
+    case 4: FBME_ROW16(1, true); break;
+    case 5: FBME_ROW16(2, true); break;
+    default: FBME_ROW16(3, false); break;
   }
 #undef FBME_ROW16
-  if (mode == 3) {
+  if (mode == 3 && cand_key) {
     const int e = (int)hipGetLastError();
     if (e) return e;
     hipLaunchKernelGGL(k_tile_topk16, dim3(grid), dim3(64), 0, stream, scenes, work, nwork, score, cand_key,

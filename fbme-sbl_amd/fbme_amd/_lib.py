@@ -87,6 +87,9 @@ _SIGNATURES = {
     "fbme_plan_bind": (c_int, [c_void_p, c_void_p, c_void_p]),
     "fbme_plan_run": (c_int, [c_void_p, POINTER(FbmeIO), c_float, c_void_p]),
     "fbme_plan_debug_pointer": (c_int, [c_void_p, c_int, POINTER(c_void_p)]),
+    "fbme_plan_begin": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "fbme_plan_iterate": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "fbme_plan_finish": (c_int, [c_void_p, c_void_p, c_void_p]),
     "fbme_plan_run_stage": (c_int, [c_void_p, c_int, POINTER(FbmeIO), c_void_p]),
     "fbme_prefix_forward": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
                                     c_int, c_void_p, c_void_p, c_void_p]),

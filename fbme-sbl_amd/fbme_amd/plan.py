@@ -18,6 +18,12 @@ import torch
 
 from . import _lib
 
+# per-scene scalar slots of the plan (FbmeScalar, csrc/fbme_types.h)
+SC_NOISE = 2
+SC_DOF = 4
+SC_CHANGE = 8
+SC_COUNT = 9
+
 
 @dataclass
 class PlanOutputs:
@@ -169,22 +175,53 @@ class NativePlan:
                                                  ctypes.c_void_p(stream)),
                    "fbme_plan_run_stage")
 
-    def run(self, y: torch.Tensor) -> PlanOutputs:
-        """Enqueue one solve of packed complex64 observations on the current
-        stream (no synchronisation, no allocation)."""
+    def _check_input(self, y: torch.Tensor) -> None:
         if y.device != self.device or y.dtype != torch.complex64:
             raise ValueError("y must be complex64 on the plan's device")
         if y.numel() != self.total_samples or not y.is_contiguous():
             raise ValueError(
                 f"y must be contiguous with {self.total_samples} elements")
-        o = self.out
+
+    def begin(self, y: torch.Tensor) -> None:
+        """Initialisation phase of a solve driven iteration by iteration."""
+        self._check_input(y)
+        io = self._io(y)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self._lib.fbme_plan_begin(self._handle, ctypes.byref(io), ctypes.c_void_p(stream)),
+                   "fbme_plan_begin")
+
+    def iterate(self, y: torch.Tensor, t: int, track_change: bool) -> None:
+        """UAMP iteration t (1-based); with track_change the per-scene
+        relative change of the step lands in scalars()[:, SC_CHANGE]."""
+        io = self._io(y)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self._lib.fbme_plan_iterate(self._handle, int(t), 1 if track_change else 0, ctypes.byref(io),
+                                               ctypes.c_void_p(stream)), "fbme_plan_iterate")
+
+    def finish(self, y: torch.Tensor) -> None:
+        """Final E-step, polish (or the UAMP posterior) and outputs."""
+        io = self._io(y)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self._lib.fbme_plan_finish(self._handle, ctypes.byref(io), ctypes.c_void_p(stream)),
+                   "fbme_plan_finish")
+
+    def scalars(self) -> torch.Tensor:
+        """[nscenes, SC_COUNT] FP64 per-scene scalars in the plan's internal
+        scene order (scenes sorted by iteration budget): power, floor,
+        noise, data variance, dof, |A^H y|^2, |y|^2, sum gamma, change."""
+        return self.debug_tensor(4, torch.float64, self.nscenes * SC_COUNT).view(self.nscenes, SC_COUNT)
+
+    def run(self, y: torch.Tensor) -> PlanOutputs:
+        """Enqueue one solve of packed complex64 observations on the current
+        stream (no synchronisation, no allocation)."""
+        self._check_input(y)
         io = self._io(y)
         stream = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self._lib.fbme_plan_run(self._handle, ctypes.byref(io),
                                            ctypes.c_float(0.0),
                                            ctypes.c_void_p(stream)),
                    "fbme_plan_run")
-        return o
+        return self.out
 
     def debug_tensor(self, which: int, dtype: torch.dtype, count: int):
         """View of internal plan state (tests only): 0 gamma, 1 mean,

@@ -31,7 +31,6 @@ def clear_cache() -> None:
 def _unsupported(cfg: SBLConfig) -> list[str]:
     bad = []
     checks = {
-        "uamp_polish_size=0 (no active-set polish)": cfg.uamp_polish_size == 0,
         "uamp_polish_residual_exchange_size>0 without prune_tail and evidence_gate": (
             cfg.uamp_polish_residual_exchange_size > 0
             and not (cfg.uamp_polish_prune_tail and cfg.uamp_polish_evidence_gate)),
@@ -54,7 +53,6 @@ def _unsupported(cfg: SBLConfig) -> list[str]:
                              or cfg.learn_spatial_coupling),
         "initialization='uniform'": cfg.initialization != "backprojection",
         "uamp_auto_iterations=False": not cfg.uamp_auto_iterations,
-        "tol>0 (eager convergence checks)": cfg.tol > 0.0,
         "phase_error_correction": cfg.phase_error_correction,
         "range_migration_correction": cfg.range_migration_correction,
         "learn_snapshot_covariance": cfg.learn_snapshot_covariance,
@@ -199,6 +197,15 @@ class ResultViews:
         # T_u of the plan's first scene (N / observed samples, solver.py:1296-1309)
         self.iterations = int(plan.scene_info[0][2])
         self.k = int(plan.options.polish_size)
+        if self.k == 0:
+            # no polish (solver.py:1926): no evidence, certificate or
+            # exchange fields
+            self.nle = self.trace = self.truncation = self.hard = None
+            self.selected = self.penalty = self.exchanges = None
+            self.gres = self.nres = self.decrease = self.monotone = None
+        self.converged = False
+        self.change = float("nan")
+        self.history = {}
 
     def refresh(self) -> None:
         """Device-side derived fields (enqueued on the current stream)."""
@@ -225,8 +232,8 @@ class ResultViews:
             noise_variance=self.noise,
             effective_dof=self.dof,
             iterations=self.iterations,
-            converged=False,
-            relative_change=float("nan"),
+            converged=self.converged,
+            relative_change=self.change,
             mode="uamp",
             arithmetic_dtype="complex64",
             uamp_backend_used="hip",
@@ -248,5 +255,61 @@ class ResultViews:
             fixed_budget_monotone=self.monotone,
             support_mask=self.mask,
             support_score_threshold=self.threshold,
-            history={},
+            history=self.history,
         )
+
+
+def run_eager(plan: NativePlan, views: ResultViews, y: torch.Tensor, cfg: SBLConfig) -> None:
+    """One solve driven iteration by iteration from the host: the convergence
+    checks, history and change tracking of the reference's loop
+    (solver.py:1285-1312, 1800-1904) around the plan's phases.  Stopping at
+    iteration t finishes every scene there (the final E-step and the polish
+    or UAMP posterior of fbme_plan_finish)."""
+    from fbme_amd.plan import SC_CHANGE, SC_DOF, SC_NOISE
+    limit = plan.max_iters
+    minimum = max(cfg.min_iter, cfg.uamp_min_iter)
+    convergence = cfg.tol > 0.0 and minimum < limit
+    track_all = cfg.return_history or cfg.eager_change_tracking or cfg.verbose
+    history = {k: [] for k in ("relative_change", "noise_variance", "effective_dof", "cg_iterations",
+                               "hutchinson_probes", "cg_tolerance", "active_set_size",
+                               "spatial_vertical_coupling", "spatial_horizontal_coupling")}
+    scalars = plan.scalars()
+    change = float("inf")
+    streak = 0
+    converged = False
+    plan.begin(y)
+    it = 0
+    for it in range(1, limit + 1):
+        check = convergence and it >= minimum and ((it - minimum) % cfg.convergence_check_interval == 0
+                                                   or it == limit)
+        need = track_all or check or it == limit
+        plan.iterate(y, it, need)
+        if need:
+            sc = scalars.cpu()
+            change = float(sc[:, SC_CHANGE].max())
+            if cfg.return_history or cfg.verbose:
+                noise = float(sc[:, SC_NOISE].float().mean())
+                dof = float(sc[:, SC_DOF].float().mean())
+        if cfg.return_history:
+            history["relative_change"].append(change)
+            history["noise_variance"].append(noise)
+            history["effective_dof"].append(dof)
+            for key in ("cg_iterations", "hutchinson_probes", "cg_tolerance", "active_set_size"):
+                history[key].append(0.0)
+            history["spatial_vertical_coupling"].append(float(cfg.spatial_coupling))
+            history["spatial_horizontal_coupling"].append(float(cfg.spatial_coupling))
+        if cfg.verbose and (it == 1 or it % 10 == 0):
+            print(f"[SBL] iter={it:4d} change={change:.3e} noise={noise:.3e} dof={dof:.2f} "
+                  f"cg={0:3d} probes={0:2d} active=0")
+        if check:
+            streak = streak + 1 if change <= cfg.tol else 0
+            if streak >= cfg.convergence_patience:
+                converged = True
+                break
+    plan.finish(y)
+    views.refresh()
+    views.iterations = it
+    views.converged = converged
+    views.change = (change if (cfg.return_history or cfg.eager_change_tracking or cfg.tol > 0.0 or cfg.verbose)
+                    else float("nan"))
+    views.history = history if cfg.return_history else {}

@@ -8,7 +8,13 @@ independent scenes solved together.  Execution modes:
   kernels (torch_sbl.pcg);
 * ``mode="dense"`` (and ``"auto"`` resolving to it when M <= 2048) is the
   exact observation-domain EM (FP64 tiled Cholesky of the M x M covariance on
-  HIP kernels; M <= 2048, grid axes <= 1024, no joint sparsity).
+  HIP kernels; M <= 2048, grid axes <= 1024; with joint sparsity one FP64
+  posterior per snapshot).
+
+``mode="uamp"`` with ``tol > 0`` (or history / change tracking) drives the
+plan's UAMP iterations from the host and stops at the reference's
+convergence checks; ``uamp_polish_size=0`` returns the UAMP posterior
+without a polish.
 
 Only GPU tensors are accepted: there is no CPU execution path.
 """
@@ -161,8 +167,14 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
     plan = _native.cached_plan(shapes, opt, device, mask)
     views = _native.ResultViews(plan, cfg, batch_shape, grid)
     with torch.cuda.device(device):
-        plan.run(y.reshape(-1).contiguous())
-        views.refresh()
+        flat = y.reshape(-1).contiguous()
+        if (cfg.tol > 0.0 or cfg.return_history or cfg.eager_change_tracking
+                or cfg.verbose):
+            # convergence checks / history: the loop is driven from the host
+            _native.run_eager(plan, views, flat, cfg)
+        else:
+            plan.run(flat)
+            views.refresh()
         # Eager calls return fresh tensors; graph plans return plan storage.
         return _with_offgrid(views.result(clone=True), y, grid, cfg)
 

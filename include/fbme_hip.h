@@ -40,7 +40,7 @@ typedef struct FbmePlanOptions {
   float margin_factor;     /* 64 evidence-gate margin in float32 eps       */
   float jitter_multiplier; /* 8 Cholesky loading in float32 eps            */
   float false_alarm;       /* 0.5 support-mask budget (<= 0: no mask)      */
-  int polish_size;         /* 32 active-set capacity (<= 32)               */
+  int polish_size;         /* 32 active-set capacity (<= 32; 0: no polish) */
   int polish_iterations;   /* 2 Fast / 3 Stable evaluations                */
   int exchange_size;       /* 0 Fast / 12 Stable residual exchange         */
   int diagnostics;         /* Stable fixed-budget diagnostics              */
@@ -158,6 +158,18 @@ int fbme_plan_bind(fbme_plan* plan, void* workspace, void* stream);
  * `support_threshold` is unused (the mask threshold log(N / 0.5) is
  * evaluated per scene, solver.py:2108-2126). */
 int fbme_plan_run(const fbme_plan* plan, const FbmeIO* io, float support_threshold, void* stream);
+/* fbme_plan_run in three calls, for callers that decide between iterations
+ * (the eager convergence checks of solver.py:1800-1904): fbme_plan_begin
+ * (initialisation), fbme_plan_iterate for t = 1, 2, ... (UAMP iteration t
+ * over the scenes whose budget reaches t; with track_change the step also
+ * leaves ||mu_t - mu_t-1|| / max(||mu_t||, eps) per scene in the plan's
+ * scalars, slot 8 of 9 doubles per scene, fbme_plan_debug_pointer 4), and
+ * fbme_plan_finish (final E-step, polish or, with polish_size 0, the UAMP
+ * posterior, and outputs).  Stopping before the budget finishes every
+ * scene at that iteration. */
+int fbme_plan_begin(const fbme_plan* plan, const FbmeIO* io, void* stream);
+int fbme_plan_iterate(const fbme_plan* plan, int t, int track_change, const FbmeIO* io, void* stream);
+int fbme_plan_finish(const fbme_plan* plan, const FbmeIO* io, void* stream);
 /* Enqueue one pass of the plan on its current state, for per-kernel timing
  * (no reference counterpart; the reference's passes are separate Triton and
  * cuFFT launches inside one captured graph, torch_sbl/solver.py:2726-2870).

@@ -189,6 +189,9 @@ class PlanConfig:
     reinitialize: bool = True
     evidence_gate: bool = True
     truncation_diagnostic: bool = False
+    tol: float = 0.0
+    convergence_check_interval: int = 2
+    convergence_patience: int = 2
 
     @staticmethod
     def _adaptive(damping, n, m, max_damping=0.8):
@@ -215,6 +218,21 @@ class PlanConfig:
                   uamp_iteration_offset=10.0, uamp_iteration_scale=1.1, polish_iterations=1,
                   margin_factor=0.0, false_alarm_budget=None, prune_tail=False,
                   reinitialize=False, evidence_gate=False)
+        kw.update(over)
+        return cls(**kw)
+
+    @classmethod
+    def uamp_default(cls, n: int, m: int, **over) -> "PlanConfig":
+        """SBLConfig(mode='uamp') with every other field at its default
+        (solver.py:34-104): adaptive UAMP damping, T_u = ceil(10 + 1.1 N / M)
+        with at least 10 iterations, convergence checks at tol 1e-4 every 2
+        iterations with patience 2, and no polish (the UAMP posterior is the
+        result)."""
+        kw = dict(uamp_damping=cls._adaptive(0.1, n, m), hyper_shape=1e-8, hyper_rate=1e-8,
+                  noise_shape=1e-8, noise_rate=1e-8, min_iter=3, uamp_min_iter=10,
+                  uamp_iteration_offset=10.0, uamp_iteration_scale=1.1, polish_size=0,
+                  polish_iterations=1, margin_factor=0.0, false_alarm_budget=None, prune_tail=False,
+                  reinitialize=False, evidence_gate=False, tol=1e-4)
         kw.update(over)
         return cls(**kw)
 
@@ -325,11 +343,22 @@ def uamp_sweep(y, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
         y, grid, samples, cfg, eps, tiny, mask)
     t_u = uamp_iteration_budget(n, m, cfg)
     max_dof = m * (1.0 - 10.0 * eps)
+    # convergence checks (solver.py:1285-1312, 1800-1904): from the larger
+    # of min_iter and uamp_min_iter, every convergence_check_interval
+    # iterations and at the budget, on the batch's largest relative change
+    # of the UAMP mean; stop after `patience` checks in a row within tol
+    minimum = max(cfg.min_iter, cfg.uamp_min_iter)
+    convergence = cfg.tol > 0.0 and minimum < t_u
+    previous = np.zeros_like(bp)
+    streak, converged, change, done = 0, False, float("inf"), t_u
     for it in range(1, t_u + 1):
-        _, _, dof, next_prior, state = uamp_step(
+        mean_t, _, dof, next_prior, state = uamp_step(
             y, gamma, noise, state, grid, samples, damping=cfg.uamp_damping,
             tiny=tiny, backprojection=bp if it == 1 else None,
             prior_floor=floor, cfg=cfg, mask=mask)
+        change = float(np.max(np.linalg.norm(mean_t - previous, axis=1)
+                              / np.maximum(np.linalg.norm(mean_t, axis=1), eps)))
+        previous = mean_t
         dof = np.minimum(dof, max_dof)
         energy = np.sum(np.abs(y - state["prediction"]) ** 2, axis=1)
         if cfg.noise_update == "em":
@@ -346,11 +375,20 @@ def uamp_sweep(y, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
             trace.gamma.append(gamma.copy())
             trace.noise.append(noise.copy())
             trace.dof.append(dof.copy())
-    mean, var, _, _, state = uamp_step(
+        check = convergence and it >= minimum and (
+            (it - minimum) % cfg.convergence_check_interval == 0 or it == t_u)
+        if check:
+            streak = streak + 1 if change <= cfg.tol else 0
+            if streak >= cfg.convergence_patience:
+                converged, done = True, it
+                break
+    mean, var, dof, _, state = uamp_step(
         y, gamma, noise, state, grid, samples, damping=cfg.uamp_damping,
         tiny=tiny, compute_prediction=False, mask=mask)
     return dict(mean=mean, variance=var, gamma=gamma, noise=noise,
-                floor=floor, power=power, backprojection=bp, iterations=t_u)
+                floor=floor, power=power, backprojection=bp, iterations=done,
+                converged=converged, relative_change=change,
+                dof=np.minimum(dof, max_dof))
 
 
 # --------------------------------------------------------------------------
@@ -693,6 +731,18 @@ def fbme_plan(data: np.ndarray, grid, profile: str = "fast",
     cfg = config if config is not None else (
         PlanConfig.fast if profile == "fast" else PlanConfig.stable)(polish_size)
     sweep = uamp_sweep(y, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
+    if cfg.polish_size == 0:
+        # the UAMP posterior of the final hyperparameters (solver.py:1907-1925,
+        # 2081-2087)
+        out = dict(x=sweep["mean"], posterior_variance=sweep["variance"],
+                   precision=1.0 / sweep["gamma"], noise_variance=sweep["noise"],
+                   effective_dof=sweep["dof"], iterations=sweep["iterations"],
+                   converged=sweep["converged"], relative_change=sweep["relative_change"],
+                   gamma=sweep["gamma"], floor=sweep["floor"])
+        if single:
+            out = {k: (v[0] if isinstance(v, np.ndarray) and v.shape[:1] == (1,) else v)
+                   for k, v in out.items()}
+        return out
     res = polish(y, sweep, grid, samples, cfg, eps=eps, tiny=tiny, mask=mask)
     out = dict(
         x=res["mean"], posterior_variance=res["posterior_variance"],

@@ -20,6 +20,8 @@ What it pins (functions no closed-form check fixes end to end):
   * ``sbl_2d`` in mode "cg" with ``joint_sparsity=True`` (shared and
     per-snapshot noise).
   * ``refine_offgrid_fourier`` on scenes with off-grid point sources.
+  * ``sbl_2d`` in mode "uamp" with the default ``SBLConfig`` (convergence
+    checks, no polish) and with a polish (``uamp_default``).
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -534,10 +536,55 @@ def preconditioners(reference_root: str) -> None:
         print("wrote", name, "iterations", res.iterations, "pcg", res.history["cg_iterations"][:6], flush=True)
 
 
+UAMP_CASES = [
+    # name, generator, seeds (one scene per seed), grid, samples, k, snr_db,
+    # SBLConfig overrides of mode='uamp' (tol 1e-4, no polish by default)
+    ("uamp_64x64_40x40", "uniform", (1090,), (64, 64), (40, 40), 15, 20.0, {}),
+    ("uamp_128x96_48x40_b2", "ellipse", (1091, 1092), (128, 96), (48, 40), 30, 15.0, {}),
+    # converges before its budget of 42 iterations (stops at the check of iteration 25)
+    ("uamp_64x48_60x44_tol1e-3", "uniform", (1099,), (64, 48), (60, 44), 5, 30.0,
+     dict(max_iter=60, uamp_iteration_offset=40.0, tol=1e-3)),
+    ("uamp_64x64_40x40_polish", "uniform", (1094,), (64, 64), (40, 40), 12, 20.0, dict(uamp_polish_size=16)),
+]
+
+
+def uamp_default(reference_root: str) -> None:
+    """``sbl_2d`` in mode 'uamp' with the SBLConfig defaults (adaptive damping,
+    T_u = ceil(10 + 1.1 N / M), convergence checks at tol 1e-4 from iteration
+    10 every 2 iterations with patience 2, no polish), on CPU with the Torch
+    UAMP branch, with the per-iteration history; one case converges before
+    its iteration budget and one adds a 16-atom polish."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, sbl_2d
+
+    torch.set_num_threads(8)
+    for name, gen, seeds, grid, samples, k, snr, over in UAMP_CASES:
+        fn = dict(uniform=scenes.uniform_scene, ellipse=scenes.ellipse_scene)[gen]
+        batch = [fn(seed, grid, samples, k=k, snr_db=snr) for seed in seeds]
+        y = np.stack([sc.y for sc in batch]).astype(np.complex64)
+        cfg = SBLConfig(mode="uamp", device="cpu", uamp_backend="torch", return_history=True, **over)
+        res = sbl_2d(torch.from_numpy(y), grid, config=cfg)
+        arrays = dict(y=y, truth=np.stack([sc.x for sc in batch]), grid=np.array(grid),
+                      samples=np.array(samples), converged=np.array(bool(res.converged)),
+                      relative_change=np.array(float(res.relative_change)),
+                      history_relative_change=np.array(res.history["relative_change"], np.float64),
+                      history_noise_variance=np.array(res.history["noise_variance"], np.float64),
+                      history_effective_dof=np.array(res.history["effective_dof"], np.float64))
+        arrays.update(_result_arrays(res))
+        for key, value in over.items():
+            arrays["override_" + key] = np.array(value)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "iterations", res.iterations, "converged", res.converged,
+              "change", res.relative_change, flush=True)
+
+
 if __name__ == "__main__":
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
-    if which == "dense_joint":
+    if which == "uamp":
+        uamp_default(root)
+    elif which == "dense_joint":
         dense_joint(root)
     elif which == "precond":
         preconditioners(root)

@@ -173,3 +173,43 @@ def test_torch_cpu_baseline_port_matches_oracle(case):
     assert rel < 1e-4
     assert set(np.flatnonzero(x)) == set(np.flatnonzero(ref["x"]))
     assert abs(float(out["noise"][0]) / float(ref["noise_variance"]) - 1) < 1e-4
+
+
+UAMP_MODE_CASES = ["uamp_64x64_40x40", "uamp_128x96_48x40_b2", "uamp_64x48_60x44_tol1e-3",
+                   "uamp_64x64_40x40_polish"]
+_UAMP_FIELDS = dict(tol="tol", max_iter="max_iter", uamp_iteration_offset="uamp_iteration_offset",
+                    uamp_polish_size="polish_size")
+
+
+def uamp_mode_config(g):
+    """PlanConfig of SBLConfig(mode='uamp') with the golden's overrides."""
+    grid, samples = tuple(int(v) for v in g["grid"]), tuple(int(v) for v in g["samples"])
+    over = {_UAMP_FIELDS[k[len("override_"):]]: g[k].item() for k in g.files if k.startswith("override_")}
+    return grid, samples, fo.PlanConfig.uamp_default(int(np.prod(grid)), int(np.prod(samples)), **over)
+
+
+@pytest.mark.parametrize("case", UAMP_MODE_CASES)
+def test_oracle_matches_reference_uamp_mode(case):
+    """sbl_2d(mode='uamp') with the SBLConfig defaults (convergence checks at
+    tol, no polish) and with a 16-atom polish: same iteration count and
+    convergence decision, x and variances to 1e-4 relative (the polish case:
+    identical support), noise and dof to 1e-4."""
+    g = np.load(golden_path(case + ".npz"))
+    grid, samples, cfg = uamp_mode_config(g)
+    out = fo.fbme_plan(g["y"].astype(np.complex128), grid, config=cfg)
+    assert int(out["iterations"]) == int(g["iterations"])
+    b = g["y"].shape[0]
+    x = np.asarray(out["x"]).reshape(b, -1)
+    ref = g["x"].reshape(b, -1)
+    rel = np.linalg.norm(x - ref) / np.linalg.norm(ref)
+    print(case, "iterations", int(g["iterations"]), "x rel", rel)
+    assert rel < 1e-4
+    v = np.asarray(out["posterior_variance"]).reshape(b, -1)
+    assert np.linalg.norm(v - g["posterior_variance"].reshape(b, -1)) / np.linalg.norm(g["posterior_variance"]) < 1e-4
+    np.testing.assert_allclose(np.atleast_1d(out["noise_variance"]), g["noise_variance"].reshape(-1), rtol=1e-4)
+    np.testing.assert_allclose(np.atleast_1d(out["effective_dof"]), g["effective_dof"].reshape(-1), rtol=1e-4)
+    if cfg.polish_size == 0:
+        assert bool(out["converged"]) == bool(g["converged"])
+        assert abs(out["relative_change"] / float(g["relative_change"]) - 1) < 1e-3
+    else:
+        assert np.array_equal(np.flatnonzero(x), np.flatnonzero(ref))
