@@ -53,6 +53,7 @@ struct OgArgs {
   int max_components;
   float relative_threshold, minimum_separation;
   int max_iter;
+  int dims;              // 1: a 1-D grid run as 1 x N (N1 = M1 = 1), 2: a 2-D grid
   double tolerance, max_step, damping, ridge, eps, tiny;
 };
 
@@ -592,7 +593,8 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
         cum += L.z[i].x * L.z[i].x + L.z[i].y * L.z[i].y;
         const double resid = fmax(L.yy - cum, a.eps);
         const double cnt = (double)(i + 1);
-        const double bic = mm * log(resid / mm) + 4.0 * cnt * log(mm) + 2.0 * cnt * log((double)n);
+        // one complex amplitude and `dims` position coordinates per component
+        const double bic = mm * log(resid / mm) + (2.0 + a.dims) * cnt * log(mm) + 2.0 * cnt * log((double)n);
         if (bic < best) {
           best = bic;
           arg = i;
@@ -708,6 +710,16 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   }
   // ------------------------------------------- position standard deviation
   normal_eq(k, a, L);
+  if (a.dims == 1) {
+    // 1-D grids: the axis-0 coordinates (N1 = M1 = 1) have zero Jacobian
+    // columns; the Fisher matrix of the axis-1 coordinates is the block the
+    // identity on axis 0 leaves (its std is not reported)
+    for (int e = t; e < n2k * n2k; e += OG_THREADS) {
+      const int r = e / n2k, q = e % n2k;
+      if (!(r & 1) || !(q & 1)) L.u.gn.nm[r][q] = (r == q) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
   // cov = 0.5 noise Fisher^-1, noise = ||r||^2 / max(M - K, 1)
   const double half_noise = 0.5 * L.obj / (double)max((int)m - k, 1);
   const bool fisher_ok = chol_r<2 * OG_KMAX + 1>(L.u.gn.nm, n2k, &L.flag);
@@ -747,12 +759,15 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
 
 extern "C" {
 
-int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
-                        const void* coefficients, int complex128, int component_count, int max_components, float relative_threshold,
-                        float minimum_separation, int max_iter, double tolerance, double max_step, double damping,
-                        double ridge, double eps, double tiny, double* initial_positions, double* positions,
-                        void* amplitudes, double* position_std, double* norms, int* counts) {
+int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                             const void* coefficients, int complex128, int component_count, int max_components,
+                             float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                             double max_step, double damping, double ridge, double eps, double tiny,
+                             double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                             double* norms, int* counts, int dims) {
   if (batch < 1 || n1 < 1 || n2 < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2) return FBME_ERR_ARG;
+  if (dims != 1 && dims != 2) return FBME_ERR_ARG;
+  if (dims == 1 && (n1 != 1 || m1 != 1)) return FBME_ERR_ARG;
   if ((long long)m1 * m2 < 2 || (long long)n1 * n2 > (1ll << 28)) return FBME_ERR_ARG;
   if (!data || !coefficients || !initial_positions || !positions || !amplitudes || !position_std || !norms ||
       !counts)
@@ -769,6 +784,7 @@ int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2,
   a.relative_threshold = relative_threshold;
   a.minimum_separation = minimum_separation;
   a.max_iter = max_iter;
+  a.dims = dims;
   a.tolerance = tolerance;
   a.max_step = max_step;
   a.damping = damping;
@@ -784,6 +800,18 @@ int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2,
                        (const float2*)data, (const float2*)coefficients, a, initial_positions, positions,
                        (double2*)amplitudes, position_std, norms, counts);
   return (int)hipGetLastError();
+}
+
+int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                        const void* coefficients, int complex128, int component_count, int max_components,
+                        float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                        double max_step, double damping, double ridge, double eps, double tiny,
+                        double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                        double* norms, int* counts) {
+  return fbme_offgrid_refine_dims(stream, batch, n1, n2, m1, m2, data, coefficients, complex128, component_count,
+                                  max_components, relative_threshold, minimum_separation, max_iter, tolerance,
+                                  max_step, damping, ridge, eps, tiny, initial_positions, positions, amplitudes,
+                                  position_std, norms, counts, 2);
 }
 
 }  // extern "C"

@@ -265,6 +265,12 @@ _SIGNATURES = {
                                     c_double, c_double, c_double, c_double,
                                     c_double, c_void_p, c_void_p, c_void_p,
                                     c_void_p, c_void_p, c_void_p]),
+    "fbme_offgrid_refine_dims": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                         c_int, c_void_p, c_void_p, c_int, c_int,
+                                         c_int, c_float, c_float, c_int, c_double,
+                                         c_double, c_double, c_double, c_double,
+                                         c_double, c_void_p, c_void_p, c_void_p,
+                                         c_void_p, c_void_p, c_void_p, c_int]),
     # joint sparsity of the general solver
     "fbme_joint_init": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_float, c_int, c_void_p,

@@ -181,6 +181,29 @@ def offgrid_scene(seed: int, grid=(64, 64), samples=(40, 40), k=6, snr_db=25.0,
     return sc, pos, amp
 
 
+def offgrid_scene_1d(seed: int, n=256, m=160, k=6, snr_db=25.0,
+                     min_separation=4.0) -> tuple[Scene, np.ndarray, np.ndarray]:
+    """offgrid_scene on a 1-D grid of n points sampled at its first m: x [n],
+    y [m], positions [K, 1], amplitudes [K]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pos = []
+    while len(pos) < k:
+        p = rng.uniform(0, n)
+        if all(min(abs(p - q), n - abs(p - q)) >= min_separation for q in pos):
+            pos.append(p)
+    pos = np.array(pos)
+    amp = rng.standard_normal(k) + 1j * rng.standard_normal(k)
+    amp = amp / np.abs(amp) * rng.uniform(0.5, 1.5, k)
+    clean = np.exp(-2j * np.pi * np.outer(np.arange(m), pos) / n) @ amp
+    sigma2 = float(np.sum(np.abs(clean) ** 2) / (m * 10.0 ** (snr_db / 10.0)))
+    y = clean + np.sqrt(sigma2 / 2.0) * (rng.standard_normal(m) + 1j * rng.standard_normal(m))
+    x = 1e-3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    x[np.round(pos).astype(int) % n] = amp
+    sc = Scene(x=x, y=y.astype(np.complex64), noise_variance=sigma2, grid=(n,), samples=(m,),
+               snr_db=float(snr_db), seed=int(seed))
+    return sc, pos[:, None], amp
+
+
 def uneven_batch(seed: int, count=64, grids=(64, 128, 192, 256),
                  ratio=(0.3, 0.8), k_fraction=(0.005, 0.03),
                  snr_db=(5.0, 25.0)) -> list[Scene]:

@@ -6,8 +6,9 @@ torch_sbl/offgrid.py:11-226).
 order, projected Gauss--Newton and the position standard deviations all
 happen in the kernel, in FP64, without forming the M x K dictionary.  Same
 arguments, validation and result fields as the reference; results are
-returned in the data's real / complex precision.  2-D grids only; at most
-64 candidate and 32 refined components per item.
+returned in the data's real / complex precision.  1-D grids run as 1 x N
+(fbme_offgrid_refine_dims with one position coordinate per component); at
+most 64 candidate and 32 refined components per item.
 """
 
 from __future__ import annotations
@@ -99,8 +100,11 @@ def refine_offgrid_fourier(data: Tensor, coefficients: Tensor, grid_shape: Seque
     batch = prod(batch_shape) if batch_shape else 1
     if batch > 1 and component_count is None:
         raise ValueError("Batch off-grid refinement requires a fixed component_count.")
-    if len(grid_shape) != 2:
-        raise NotImplementedError("off-grid refinement on MI355X takes 2-D grids.")
+    dims = len(grid_shape)
+    # a 1-D grid runs as 1 x N sampled at 1 x M (the kernel's axis 0 then has
+    # one sample and carries no position information)
+    kgrid = grid_shape if dims == 2 else (1, grid_shape[0])
+    ksamples = sample_shape if dims == 2 else (1, sample_shape[0])
     if not data.is_cuda:
         raise RuntimeError("off-grid refinement runs on the GPU; pass GPU tensors.")
     double = data.dtype in (torch.complex128, torch.float64)
@@ -131,13 +135,13 @@ def refine_offgrid_fourier(data: Tensor, coefficients: Tensor, grid_shape: Seque
     lib = _lib.load()
     with torch.cuda.device(device):
         stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-        _lib.check(lib.fbme_offgrid_refine(
-            stream, batch, *grid_shape, *sample_shape, ctypes.c_void_p(y.data_ptr()),
+        _lib.check(lib.fbme_offgrid_refine_dims(
+            stream, batch, *kgrid, *ksamples, ctypes.c_void_p(y.data_ptr()),
             ctypes.c_void_p(x.data_ptr()), 1 if double else 0, count, maximum, float(relative_threshold),
             float(minimum_separation), int(max_iter), float(tolerance), float(max_step), float(damping),
             float(ridge), float(finfo.eps), float(finfo.tiny), ctypes.c_void_p(init.data_ptr()),
             ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(amp.data_ptr()), ctypes.c_void_p(std.data_ptr()),
-            ctypes.c_void_p(norms.data_ptr()), ctypes.c_void_p(counts.data_ptr())), "fbme_offgrid_refine")
+            ctypes.c_void_p(norms.data_ptr()), ctypes.c_void_p(counts.data_ptr()), dims), "fbme_offgrid_refine")
         cnt = counts.cpu()
     if bool((cnt[:, 1] < 0).any()):
         raise NotImplementedError(
@@ -146,12 +150,13 @@ def refine_offgrid_fourier(data: Tensor, coefficients: Tensor, grid_shape: Seque
     if bool((cnt[:, 0] != k).any()):
         raise RuntimeError("batch items refined to different component counts")
     out_batch = batch_shape if batch_shape else ()
-    pshape = (*out_batch, k, 2)
+    pshape = (*out_batch, k, dims)
+    axes = slice(0, 2) if dims == 2 else slice(1, 2)
     return OffGridResult(
-        initial_positions=init[:, :k].to(real).reshape(pshape),
-        positions=pos[:, :k].to(real).reshape(pshape),
+        initial_positions=init[:, :k, axes].to(real).reshape(pshape),
+        positions=pos[:, :k, axes].to(real).reshape(pshape),
         amplitudes=amp[:, :k].to(cplx).reshape(*out_batch, k),
-        position_std=std[:, :k].to(real).reshape(pshape),
+        position_std=std[:, :k, axes].to(real).reshape(pshape),
         initial_residual_norm=norms[:, 0].to(real).reshape(out_batch),
         residual_norm=norms[:, 1].to(real).reshape(out_batch),
         iterations=cnt[:, 1].to(device=device, dtype=torch.long).reshape(out_batch),

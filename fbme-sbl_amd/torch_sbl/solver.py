@@ -47,16 +47,17 @@ def sbl_1d(data: Any, grid_size: int, *, config: SBLConfig | None = None,
            observation_mask: Any | None = None) -> SBLResult:
     """1-D spectra: data [..., M] -> x [..., grid_size] (solver.py:847-867).
     Runs as a 1 x N grid."""
-    tensor = torch.as_tensor(data)
-    res = fast_sbl(tensor.unsqueeze(-2), (1, int(grid_size)), config=config,
-                   observation_mask=(None if observation_mask is None else
-                                     torch.as_tensor(observation_mask)[None]))
-    squeeze = lambda t: None if t is None else t.squeeze(-2)  # noqa: E731
-    res.x = squeeze(res.x)
-    res.posterior_variance = squeeze(res.posterior_variance)
-    res.precision = squeeze(res.precision)
-    if res.support_mask is not None:
-        res.support_mask = squeeze(res.support_mask)
+    return fast_sbl(data, (int(grid_size),), config=config,
+                    observation_mask=observation_mask)
+
+
+def _squeeze_1d(res: SBLResult) -> SBLResult:
+    """A 1-D grid ran as 1 x N: grid-shaped fields lose the unit axis, so
+    they are [..., N] as the reference returns them."""
+    for name in ("x", "posterior_variance", "precision", "support_mask"):
+        value = getattr(res, name, None)
+        if value is not None:
+            setattr(res, name, value.squeeze(-2))
     return res
 
 
@@ -106,9 +107,12 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
     grid = tuple(int(v) for v in grid_shape)
     if len(grid) not in (1, 2):
         raise ValueError("grid_shape must contain one or two dimensions.")
-    if len(grid) == 1:
+    one_d = len(grid) == 1
+    if one_d:
         grid = (1, grid[0])
         data = torch.as_tensor(data).unsqueeze(-2)
+        if observation_mask is not None:
+            observation_mask = torch.as_tensor(observation_mask).unsqueeze(0)
     if (cfg.uamp_support_false_alarm_budget is not None
             and cfg.uamp_support_false_alarm_budget >= prod(grid)):
         raise ValueError(
@@ -156,7 +160,7 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
         res = pcg.sbl_cg(y.reshape(batch, *samples), grid, cfg,
                          batch_shape=batch_shape, observation_mask=mask,
                          dense=mode == "dense")
-        return _with_offgrid(res, y, grid, cfg)
+        return _finish(res, y, grid, cfg, one_d)
     if cfg.uamp_backend == "torch":
         raise NotImplementedError(
             "uamp_backend='torch' selects the reference's PyTorch UAMP; this "
@@ -176,11 +180,16 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
             plan.run(flat)
             views.refresh()
         # Eager calls return fresh tensors; graph plans return plan storage.
-        return _with_offgrid(views.result(clone=True), y, grid, cfg)
+        return _finish(views.result(clone=True), y, grid, cfg, one_d)
 
 
-def _with_offgrid(res: SBLResult, y: torch.Tensor, grid, cfg: SBLConfig) -> SBLResult:
-    """Off-grid refinement of the reconstruction (solver.py:2139-2153)."""
+def _finish(res: SBLResult, y: torch.Tensor, grid, cfg: SBLConfig, one_d: bool) -> SBLResult:
+    """Result shapes of the caller's grid, then the off-grid refinement of
+    the reconstruction on that grid (solver.py:2139-2153)."""
+    if one_d:
+        res = _squeeze_1d(res)
+        y = y.squeeze(-2)
+        grid = (grid[1],)
     if not cfg.offgrid_refinement:
         return res
     from .offgrid import refine_offgrid_fourier

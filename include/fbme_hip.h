@@ -581,6 +581,18 @@ int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2,
                         double max_step, double damping, double ridge, double eps, double tiny,
                         double* initial_positions, double* positions, void* amplitudes, double* position_std,
                         double* norms, int* counts);
+/* fbme_offgrid_refine for 1-D or 2-D grids (refine_offgrid_fourier with
+ * len(grid_shape) = dims, offgrid.py:70-226): dims = 1 runs a grid of N points
+ * sampled at its first M as n1 = m1 = 1, n2 = N, m2 = M; the model order
+ * counts one position coordinate per component (offgrid.py:349-350) and the
+ * axis-0 outputs (positions 0, std NaN-free but meaningless) are dropped by
+ * the caller.  dims = 2 is fbme_offgrid_refine. */
+int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                             const void* coefficients, int complex128, int component_count, int max_components,
+                             float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                             double max_step, double damping, double ridge, double eps, double tiny,
+                             double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                             double* norms, int* counts, int dims);
 
 /* ------------------------------------------------------------------------
  * Joint sparsity of the general solver: SBLConfig(joint_sparsity=True)

@@ -19,7 +19,8 @@ What it pins (functions no closed-form check fixes end to end):
     under the Fast / Stable numerics with an ``observation_mask``.
   * ``sbl_2d`` in mode "cg" with ``joint_sparsity=True`` (shared and
     per-snapshot noise).
-  * ``refine_offgrid_fourier`` on scenes with off-grid point sources.
+  * ``refine_offgrid_fourier`` on scenes with off-grid point sources, on 2-D
+    and on 1-D grids.
   * ``sbl_2d`` in mode "uamp" with the default ``SBLConfig`` (convergence
     checks, no polish) and with a polish (``uamp_default``).
 
@@ -299,6 +300,40 @@ def offgrid(reference_root: str) -> None:
             arrays[field] = getattr(res, field).detach().cpu().numpy()   # leading batch dimension
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
         print("wrote", name)
+
+
+OFFGRID_1D_CASES = [
+    # name, seeds, N, M, sources, snr_db, dtype, component_count
+    ("offgrid_1d_256_160_c64", [1054], 256, 160, 6, 25.0, np.complex64, None),
+    ("offgrid_1d_512_200_c128", [1055], 512, 200, 9, 20.0, np.complex128, None),
+    ("offgrid_1d_128_96_b2_k4", [1056, 1057], 128, 96, 4, 20.0, np.complex64, 4),
+]
+
+
+def offgrid_1d(reference_root: str) -> None:
+    """``refine_offgrid_fourier`` on 1-D grids (one position coordinate per
+    component: the model order's parameter count and the Fisher matrix are
+    those of D = 1)."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl.offgrid import refine_offgrid_fourier
+
+    torch.set_num_threads(8)
+    for name, seeds, n, m, k, snr, dtype, count in OFFGRID_1D_CASES:
+        made = [scenes.offgrid_scene_1d(s, n, m, k=k, snr_db=snr) for s in seeds]
+        y = np.stack([mm[0].y for mm in made])
+        x = np.stack([mm[0].x for mm in made]).astype(dtype)
+        res = refine_offgrid_fourier(torch.from_numpy(y.astype(dtype)), torch.from_numpy(x), (n,),
+                                     component_count=count)
+        arrays = dict(y=y.astype(dtype), x=x, grid=np.array((n,)), samples=np.array((m,)),
+                      true_positions=np.stack([mm[1] for mm in made]),
+                      true_amplitudes=np.stack([mm[2] for mm in made]),
+                      component_count=np.array(-1 if count is None else count))
+        for field in ("initial_positions", "positions", "amplitudes", "position_std",
+                      "initial_residual_norm", "residual_norm", "iterations", "converged"):
+            arrays[field] = getattr(res, field).detach().cpu().numpy()
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "components", arrays["positions"].shape, "iterations", arrays["iterations"])
 
 
 HEADLINE_CASES = [
@@ -584,6 +619,8 @@ if __name__ == "__main__":
     which = sys.argv[2] if len(sys.argv) > 2 else "all"
     if which == "uamp":
         uamp_default(root)
+    elif which == "offgrid_1d":
+        offgrid_1d(root)
     elif which == "dense_joint":
         dense_joint(root)
     elif which == "precond":

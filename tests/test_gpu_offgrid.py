@@ -1,5 +1,6 @@
 """Off-grid refinement (refine_offgrid_fourier, torch_sbl/offgrid.py:70-226)
-on the HIP kernel against the float64 oracle and the reference's golden runs.
+on the HIP kernel against the float64 oracle and the reference's golden runs,
+on 2-D and 1-D grids.
 
 The kernel evaluates the same variable-projection objective in FP64 through
 closed-form atom inner products, so it matches the float64 oracle to
@@ -21,7 +22,8 @@ from oracle import offgrid_oracle as oo
 pytestmark = pytest.mark.gpu
 
 GOLDEN = ["offgrid_64x64_40x40_c128", "offgrid_64x64_40x40_c64", "offgrid_64x64_40x40_b2_k5",
-          "offgrid_128x96_80x60"]
+          "offgrid_128x96_80x60", "offgrid_1d_256_160_c64", "offgrid_1d_512_200_c128",
+          "offgrid_1d_128_96_b2_k4"]
 
 
 def _run(y, x, grid, **kw):
@@ -42,10 +44,11 @@ def test_offgrid_matches_reference_golden_and_oracle(device, name):
     ftype = np.float64 if exact else np.float32
     orc = oo.refine_offgrid(g["y"], g["x"], grid, component_count=count,
                             eps=float(np.finfo(ftype).eps), tiny=float(np.finfo(ftype).tiny))
-    pos = res.positions.cpu().numpy().reshape(len(orc), -1, 2)
+    d = len(grid)          # 1-D grids: one position coordinate per component
+    pos = res.positions.cpu().numpy().reshape(len(orc), -1, d)
     amp = res.amplitudes.cpu().numpy().reshape(len(orc), -1)
-    std = res.position_std.cpu().numpy().reshape(len(orc), -1, 2)
-    init = res.initial_positions.cpu().numpy().reshape(len(orc), -1, 2)
+    std = res.position_std.cpu().numpy().reshape(len(orc), -1, d)
+    init = res.initial_positions.cpu().numpy().reshape(len(orc), -1, d)
     norm = res.residual_norm.cpu().numpy().reshape(-1)
     for i, o in enumerate(orc):
         np.testing.assert_array_equal(init[i], g["initial_positions"][i])
