@@ -99,6 +99,18 @@ struct fbme_plan {
   size_t workspace_bytes = 0;
   char* ws = nullptr;
   int bound = 0;
+  // A pass whose scenes split between the LDS-Stockham list and the
+  // register 16 x 16 list runs the two as parallel branches: the register
+  // list on `side`, forked from and joined to the caller's stream by events
+  // (a graph capture records the fork and join as edges).  Both lists touch
+  // disjoint scenes and partial-sum slots.
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  ~fbme_plan() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
+  }
 };
 
 namespace {
@@ -596,6 +608,13 @@ int fbme_plan_bind(fbme_plan* p, void* workspace, void* stream) {
   if (e == hipSuccess) e = up(p->r_dir, p->dir.data());
   if (e == hipSuccess) e = up(p->r_tw, p->twid.data());
   if (e == hipSuccess) e = hipStreamSynchronize(st);
+  // side branch of passes split between both kernel lists (default priority)
+  const bool split_col = p->col_split > 0 && p->col_split < (int)p->col_work.size();
+  const bool split_row = p->row_split > 0 && p->row_split < (int)p->row_work.size();
+  if (e == hipSuccess && !p->side && (split_col || split_row))
+    e = hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking);
+  if (e == hipSuccess && p->side && !p->ev_fork) e = hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess && p->side && !p->ev_join) e = hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming);
   if (e != hipSuccess) return FBME_ERR_HIP;
   p->bound = 1;
   return 0;
@@ -671,32 +690,60 @@ int plan_phase(const fbme_plan* p, const FbmeIO* io, hipStream_t st, int phase, 
   const FbmeWork* colw16 = colw + p->col_split;
   const FbmeWork* roww16 = roww + p->row_split;
   const int ncol16 = ncol - p->col_split, nrow16 = nrow - p->row_split;
+  // Runs a pass's two kernel lists: in parallel branches when both have
+  // work (the register list on the side stream, between a fork and a join
+  // event), else in order on st.
+  auto split = [&](int ng, int n16, auto&& lds, auto&& r16) -> int {
+    if (ng > 0 && n16 > 0 && p->side) {
+      if (hipEventRecord(p->ev_fork, st) != hipSuccess || hipStreamWaitEvent(p->side, p->ev_fork, 0) != hipSuccess)
+        return FBME_ERR_HIP;
+      int r = lds(st);
+      if (!r) r = r16(p->side);
+      if (hipEventRecord(p->ev_join, p->side) != hipSuccess || hipStreamWaitEvent(st, p->ev_join, 0) != hipSuccess)
+        return FBME_ERR_HIP;
+      return r;
+    }
+    int r = lds(st);
+    return r ? r : r16(st);
+  };
   auto col_adjoint = [&](int ng, int n16, int mode, int store) {
-    int r = fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, mask, mbits, pred, msg, mid, sc, part,
-                                    ps, mode, store, o.uamp_damping, tw, st);
-    if (!r)
-      r = fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, mask, mbits, pred, msg, mid, sc,
-                                  part, ps, mode, store, o.uamp_damping, tw, st);
-    return r;
+    return split(
+        ng, n16,
+        [&](hipStream_t q) {
+          return fbme_launch_col_adjoint(scenes, colw, ng, ng, p->ept_col, io->y, mask, mbits, pred, msg, mid, sc,
+                                         part, ps, mode, store, o.uamp_damping, tw, q);
+        },
+        [&](hipStream_t q) {
+          return fbme_launch_col_adjoint(scenes, colw16, n16, n16, FBME_EPT_R16, io->y, mask, mbits, pred, msg, mid,
+                                         sc, part, ps, mode, store, o.uamp_damping, tw, q);
+        });
   };
   uint32_t* score = p->r_score.bytes ? (uint32_t*)(w + p->r_score.off) : nullptr;
   auto row = [&](int ng, int n16, int mode, uint32_t* key, int* idx, int store) {
-    int r = fbme_launch_row(scenes, roww, ng, ng, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, mode,
-                            o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, nullptr,
-                            tw, st);
-    if (!r)
-      r = fbme_launch_row(scenes, roww16, n16, n16, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps, mode,
-                          o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store, score, tw,
-                          st);
-    return r;
+    return split(
+        ng, n16,
+        [&](hipStream_t q) {
+          return fbme_launch_row(scenes, roww, ng, ng, p->ept_row, mid, bp, gamma, mean, var, sc, part, ps, mode,
+                                 o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store,
+                                 nullptr, tw, q);
+        },
+        [&](hipStream_t q) {
+          return fbme_launch_row(scenes, roww16, n16, n16, FBME_EPT_R16, mid, bp, gamma, mean, var, sc, part, ps,
+                                 mode, o.uamp_damping, o.damping, o.max_precision, hs, hr, o.blend, key, idx, store,
+                                 score, tw, q);
+        });
   };
   auto col_forward = [&](int ng, int n16, int first) {
-    int r = fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, mask, mbits, pred, msg, sc, part,
-                                    ps, first, o.uamp_damping, tw, st);
-    if (!r)
-      r = fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, mask, mbits, pred, msg, sc,
-                                  part, ps, first, o.uamp_damping, tw, st);
-    return r;
+    return split(
+        ng, n16,
+        [&](hipStream_t q) {
+          return fbme_launch_col_forward(scenes, colw, ng, ng, p->ept_col, mid, io->y, mask, mbits, pred, msg, sc,
+                                         part, ps, first, o.uamp_damping, tw, q);
+        },
+        [&](hipStream_t q) {
+          return fbme_launch_col_forward(scenes, colw16, n16, n16, FBME_EPT_R16, mid, io->y, mask, mbits, pred, msg,
+                                         sc, part, ps, first, o.uamp_damping, tw, q);
+        });
   };
   if (phase == 0) {
     // Initialisation: A^H y, power, floor, sigma^2_0, sum gamma_0.
