@@ -1,6 +1,6 @@
 // Off-grid refinement of on-grid SBL peaks (gfx950): refine_offgrid_fourier
-// of torch_sbl/offgrid.py:70-226 for 2-D grids, one workgroup per scene with
-// the whole Gauss-Newton loop inside the kernel.
+// of torch_sbl/offgrid.py:70-226 for 2-D grids (1-D grids run as 1 x N), one
+// workgroup per scene with the whole Gauss-Newton loop inside the kernel.
 //
 // Nothing of size M x K is formed.  For atoms a_k[m] = exp(-2 pi i (m1 p1/N1
 // + m2 p2/N2)) at continuous positions p_k:
@@ -11,8 +11,10 @@
 //     delta / N}, p = 0, 1, 2, over the position differences (closed-form
 //     Gram, D^H J and J^H J of the variable-projection Jacobian);
 //   * ||y - D a||^2 = ||y||^2 - 2 Re(a^H D^H y) + a^H G a.
-// Linear algebra (K <= 32 atoms, 2K <= 64 position coordinates) runs in FP64
-// in LDS.  Candidate peaks come from a block radix select of the top
+// Linear algebra (K <= 64 atoms, 2K <= 128 position coordinates) runs in
+// FP64: in LDS for K <= 32, in a caller's global workspace (388 KiB per
+// scene) for 32 < K <= 64; both run the same code in the same order, so a
+// scene refined either way gives the same bits.  Candidate peaks come from a block radix select of the top
 // max(64, 16 Kmax) magnitudes, sorted, then greedy periodic non-maximum
 // suppression (offgrid.py:229-295) and the extended BIC model order
 // (offgrid.py:318-361, orthogonalisation by the Cholesky factor of the Gram,
@@ -26,7 +28,8 @@ using namespace fbme;
 namespace {
 
 constexpr int OG_THREADS = 256;
-constexpr int OG_KMAX = 32;       // refined atoms
+constexpr int OG_KLDS = 32;       // refined atoms whose matrices sit in LDS
+constexpr int OG_KMAX = 64;       // refined atoms with a global workspace
 constexpr int OG_BICMAX = 64;     // candidates entering the model-order choice
 constexpr int OG_CAND = 1024;     // sorted candidates
 constexpr double OG_PI = 3.14159265358979323846;
@@ -69,10 +72,10 @@ struct OgLds {
       double2 g[OG_BICMAX][OG_BICMAX + 1];
     } bic;
     struct {
-      double2 g[OG_KMAX][OG_KMAX + 1];       // Gram of the atoms
-      double2 l[OG_KMAX][OG_KMAX + 1];       // Cholesky factor of the loaded Gram
-      double2 h[OG_KMAX][2 * OG_KMAX + 1];   // D^H J -> L^-1 D^H J -> loaded-Gram^-1 D^H J
-      double nm[2 * OG_KMAX][2 * OG_KMAX + 1];   // normal matrix / its factor
+      double2 g[OG_KLDS][OG_KLDS + 1];       // Gram of the atoms
+      double2 l[OG_KLDS][OG_KLDS + 1];       // Cholesky factor of the loaded Gram
+      double2 h[OG_KLDS][2 * OG_KLDS + 1];   // D^H J -> L^-1 D^H J -> loaded-Gram^-1 D^H J
+      double nm[2 * OG_KLDS][2 * OG_KLDS + 1];   // normal matrix / its factor
     } gn;
   } u;
   double2 s0[OG_BICMAX], s1[OG_KMAX], s2[OG_KMAX];   // D^H y and the weighted sums
@@ -88,6 +91,23 @@ struct OgLds {
   unsigned thresh;
   int need, taken;
 };
+
+// The refinement matrices of K <= KM atoms (layout of OgLds::u.gn): in LDS
+// for KM = 32, in the scene's slice of the workspace for KM = 64.
+template <int KM>
+struct GnMats {
+  double2 (*g)[KM + 1];
+  double2 (*l)[KM + 1];
+  double2 (*h)[2 * KM + 1];
+  double (*nm)[2 * KM + 1];
+};
+
+template <int KM>
+constexpr size_t gn_bytes() {
+  return sizeof(double2) * (2 * KM * (KM + 1) + KM * (2 * KM + 1)) + sizeof(double) * 2 * KM * (2 * KM + 1);
+}
+// workspace bytes per scene (a multiple of 256)
+constexpr size_t OG_WS_BYTES = (gn_bytes<OG_KMAX>() + 255) / 256 * 256;
 
 // ------------------------------------------------------------- block helpers
 __device__ double block_sum_d(double v, double* red) {
@@ -270,34 +290,36 @@ __device__ void gram(const double (*pos)[2], int k, const OgArgs& a, double2 (*g
 
 // Amplitudes of the ridge-loaded least squares at L.tpos (offgrid.py:479-499)
 // and the objective ||y - D a||^2: needs L.s0 at the same positions.
-// Leaves the Gram in L.u.gn.g and the loaded factor in L.u.gn.l.
-__device__ bool fit(const double (*pos)[2], int k, const OgArgs& a, OgLds& L, double2* amp, double* obj) {
+// Leaves the Gram in M.g and the loaded factor in M.l.
+template <int KM>
+__device__ bool fit(const double (*pos)[2], int k, const OgArgs& a, OgLds& L, const GnMats<KM>& M, double2* amp,
+                    double* obj) {
   const int t = threadIdx.x;
-  gram<OG_KMAX + 1>(pos, k, a, L.u.gn.g);
+  gram<KM + 1>(pos, k, a, M.g);
   if (t == 0) {
     double md = 0.0;
-    for (int i = 0; i < k; ++i) md += L.u.gn.g[i][i].x;
+    for (int i = 0; i < k; ++i) md += M.g[i][i].x;
     L.reg = fmax(a.ridge, a.eps) * fmax(md / k, a.tiny);
   }
   __syncthreads();
   for (int e = t; e < k * k; e += OG_THREADS) {
     const int r = e / k, q = e % k;
-    double2 v = L.u.gn.g[r][q];
+    double2 v = M.g[r][q];
     if (r == q) v.x += L.reg;
-    L.u.gn.l[r][q] = v;
+    M.l[r][q] = v;
   }
   __syncthreads();
-  const bool ok = chol_z<OG_KMAX + 1>(L.u.gn.l, k, &L.flag);
+  const bool ok = chol_z<KM + 1>(M.l, k, &L.flag);
   if (t == 0) {
     for (int i = 0; i < k; ++i) amp[i] = L.s0[i];
-    fwd_z<OG_KMAX + 1>(L.u.gn.l, amp, k);
-    bwd_z<OG_KMAX + 1>(L.u.gn.l, amp, k);
+    fwd_z<KM + 1>(M.l, amp, k);
+    bwd_z<KM + 1>(M.l, amp, k);
     // ||y||^2 - 2 Re(a^H D^H y) + a^H G a
     double cross = 0.0, quad = 0.0;
     for (int i = 0; i < k; ++i) {
       cross += zcmul(amp[i], L.s0[i]).x;
       double2 ga = make_double2(0.0, 0.0);
-      for (int j = 0; j < k; ++j) ga = zadd(ga, zmul(L.u.gn.g[i][j], amp[j]));
+      for (int j = 0; j < k; ++j) ga = zadd(ga, zmul(M.g[i][j], amp[j]));
       quad += zcmul(amp[i], ga).x;
     }
     *obj = fmax(L.yy - 2.0 * cross + quad, 0.0);
@@ -309,7 +331,8 @@ __device__ bool fit(const double (*pos)[2], int k, const OgArgs& a, OgLds& L, do
 // Normal matrix Re(Jp^H Jp) and rhs Re(Jp^H r) of the projected Jacobian at
 // L.pos with amplitudes L.amp (offgrid.py:502-535, 413-419).  Needs L.s0 /
 // s1 / s2, the Gram and the loaded factor of fit() at L.pos.
-__device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
+template <int KM>
+__device__ void normal_eq(int k, const OgArgs& a, OgLds& L, const GnMats<KM>& M) {
   const int t = threadIdx.x;
   const int n2k = 2 * k;
   const double tp = 2.0 * OG_PI;
@@ -322,13 +345,13 @@ __device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
     // J_{q,d}[m] = -2 pi i (m_d / N_d) a_q[m] amp_q
     const double2 mi = make_double2(0.0, -tp);
     const double2 aq = L.amp[q];
-    L.u.gn.h[r][2 * q] = zmul(zmul(mi, aq), zmul(e1[1], e2[0]));
-    L.u.gn.h[r][2 * q + 1] = zmul(zmul(mi, aq), zmul(e1[0], e2[1]));
+    M.h[r][2 * q] = zmul(zmul(mi, aq), zmul(e1[1], e2[0]));
+    M.h[r][2 * q + 1] = zmul(zmul(mi, aq), zmul(e1[0], e2[1]));
     const double2 ca = zscale(zcmul(L.amp[r], aq), tp * tp);
-    L.u.gn.nm[2 * r][2 * q] = zmul(ca, zmul(e1[2], e2[0])).x;
-    L.u.gn.nm[2 * r][2 * q + 1] = zmul(ca, zmul(e1[1], e2[1])).x;
-    L.u.gn.nm[2 * r + 1][2 * q] = zmul(ca, zmul(e1[1], e2[1])).x;
-    L.u.gn.nm[2 * r + 1][2 * q + 1] = zmul(ca, zmul(e1[0], e2[2])).x;
+    M.nm[2 * r][2 * q] = zmul(ca, zmul(e1[2], e2[0])).x;
+    M.nm[2 * r][2 * q + 1] = zmul(ca, zmul(e1[1], e2[1])).x;
+    M.nm[2 * r + 1][2 * q] = zmul(ca, zmul(e1[1], e2[1])).x;
+    M.nm[2 * r + 1][2 * q + 1] = zmul(ca, zmul(e1[0], e2[2])).x;
   }
   __syncthreads();
   // J^H r = J^H y - H^H amp  (r = y - D amp)
@@ -336,16 +359,16 @@ __device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
     const int q = t >> 1, d = t & 1;
     const double2 sy = d ? L.s2[q] : L.s1[q];
     double2 v = zmul(make_double2(0.0, tp), zcmul(L.amp[q], sy));   // 2 pi i conj(amp_q) S_d
-    for (int l = 0; l < k; ++l) v = zsub(v, zcmul(L.u.gn.h[l][t], L.amp[l]));
+    for (int l = 0; l < k; ++l) v = zsub(v, zcmul(M.h[l][t], L.amp[l]));
     L.rhs[t] = v.x;
   }
   __syncthreads();
   // X = L^-1 H (column per thread)
   if (t < n2k) {
     for (int i = 0; i < k; ++i) {
-      double2 s = L.u.gn.h[i][t];
-      for (int j = 0; j < i; ++j) s = zsub(s, zmul(L.u.gn.l[i][j], L.u.gn.h[j][t]));
-      L.u.gn.h[i][t] = zscale(s, 1.0 / L.u.gn.l[i][i].x);
+      double2 s = M.h[i][t];
+      for (int j = 0; j < i; ++j) s = zsub(s, zmul(M.l[i][j], M.h[j][t]));
+      M.h[i][t] = zscale(s, 1.0 / M.l[i][i].x);
     }
   }
   __syncthreads();
@@ -353,16 +376,16 @@ __device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
   for (int e = t; e < n2k * n2k; e += OG_THREADS) {
     const int r = e / n2k, q = e % n2k;
     double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += zcmul(L.u.gn.h[i][r], L.u.gn.h[i][q]).x;
-    L.u.gn.nm[r][q] -= acc;
+    for (int i = 0; i < k; ++i) acc += zcmul(M.h[i][r], M.h[i][q]).x;
+    M.nm[r][q] -= acc;
   }
   __syncthreads();
   // C = L^-H X
   if (t < n2k) {
     for (int i = k - 1; i >= 0; --i) {
-      double2 s = L.u.gn.h[i][t];
-      for (int j = i + 1; j < k; ++j) s = zsub(s, zcmul(L.u.gn.l[j][i], L.u.gn.h[j][t]));
-      L.u.gn.h[i][t] = zscale(s, 1.0 / L.u.gn.l[i][i].x);
+      double2 s = M.h[i][t];
+      for (int j = i + 1; j < k; ++j) s = zsub(s, zcmul(M.l[j][i], M.h[j][t]));
+      M.h[i][t] = zscale(s, 1.0 / M.l[i][i].x);
     }
   }
   __syncthreads();
@@ -370,18 +393,18 @@ __device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
   for (int e = t; e < n2k * n2k; e += OG_THREADS) {
     const int r = e / n2k, q = e % n2k;
     double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += zcmul(L.u.gn.h[i][r], L.u.gn.h[i][q]).x;
-    L.u.gn.nm[r][q] -= L.reg * acc;
+    for (int i = 0; i < k; ++i) acc += zcmul(M.h[i][r], M.h[i][q]).x;
+    M.nm[r][q] -= L.reg * acc;
   }
   if (t < k) {
     double2 v = L.s0[t];
-    for (int j = 0; j < k; ++j) v = zsub(v, zmul(L.u.gn.g[t][j], L.amp[j]));
+    for (int j = 0; j < k; ++j) v = zsub(v, zmul(M.g[t][j], L.amp[j]));
     L.z[t] = v;
   }
   __syncthreads();
   if (t < n2k) {
     double acc = 0.0;
-    for (int i = 0; i < k; ++i) acc += zcmul(L.u.gn.h[i][t], L.z[i]).x;
+    for (int i = 0; i < k; ++i) acc += zcmul(M.h[i][t], L.z[i]).x;
     L.rhs[t] -= acc;
   }
   __syncthreads();
@@ -389,23 +412,24 @@ __device__ void normal_eq(int k, const OgArgs& a, OgLds& L) {
 
 // Solve (nm + lambda diag(max(diag nm, eps))) step = rhs; nm is overwritten
 // by the factor.  Returns false on a non-positive pivot.
-__device__ bool solve_step(int n, const OgArgs& a, OgLds& L) {
+template <int KM>
+__device__ bool solve_step(int n, const OgArgs& a, OgLds& L, const GnMats<KM>& M) {
   const int t = threadIdx.x;
-  if (t < n) L.diag[t] = fmax(L.u.gn.nm[t][t], a.eps);
+  if (t < n) L.diag[t] = fmax(M.nm[t][t], a.eps);
   __syncthreads();
-  if (t < n) L.u.gn.nm[t][t] += L.lambda * L.diag[t];
+  if (t < n) M.nm[t][t] += L.lambda * L.diag[t];
   __syncthreads();
-  const bool ok = chol_r<2 * OG_KMAX + 1>(L.u.gn.nm, n, &L.flag);
+  const bool ok = chol_r<2 * KM + 1>(M.nm, n, &L.flag);
   if (t == 0) {
     for (int i = 0; i < n; ++i) {
       double s = L.rhs[i];
-      for (int j = 0; j < i; ++j) s -= L.u.gn.nm[i][j] * L.step[j];
-      L.step[i] = s / L.u.gn.nm[i][i];
+      for (int j = 0; j < i; ++j) s -= M.nm[i][j] * L.step[j];
+      L.step[i] = s / M.nm[i][i];
     }
     for (int i = n - 1; i >= 0; --i) {
       double s = L.step[i];
-      for (int j = i + 1; j < n; ++j) s -= L.u.gn.nm[j][i] * L.step[j];
-      L.step[i] = s / L.u.gn.nm[i][i];
+      for (int j = i + 1; j < n; ++j) s -= M.nm[j][i] * L.step[j];
+      L.step[i] = s / M.nm[i][i];
     }
   }
   __syncthreads();
@@ -419,12 +443,160 @@ __device__ __forceinline__ unsigned mag_key(double2 v) {
   return __float_as_uint((float)sqrt(v.x * v.x + v.y * v.y));
 }
 
+// Projected Gauss--Newton from the selected positions L.pos (k atoms, the
+// same in every thread), the position standard deviations and the outputs
+// (offgrid.py:364-476, 538-564), with the K x K and 2K x 2K matrices in M.
+template <int KM, typename C>
+__device__ void refine(const C* __restrict__ y, const OgArgs& a, OgLds& L, const GnMats<KM>& M, int k, int b,
+                       long long m, int kmax_out, double* __restrict__ out_init, double* __restrict__ out_pos,
+                       double2* __restrict__ out_amp,
+                       double* __restrict__ out_std, double* __restrict__ out_norm, int* __restrict__ out_count) {
+  const int t = threadIdx.x;
+  // ----------------------------------------------- projected Gauss--Newton
+  data_sums(y, L.pos, 0, k, a, true, L);
+  // The ridge loading keeps the Gram positive definite.  A failed pivot of
+  // the normal equations (no finite step exists) or a NaN step ends the
+  // iteration, as the non-finite test after the clamp does in the reference
+  // (offgrid.py:407-409).
+  fit<KM>(L.pos, k, a, L, M, L.amp, &L.obj);
+  if (t == 0) {
+    L.lambda = a.damping;
+    out_norm[2 * b] = sqrt(L.obj);
+  }
+  __syncthreads();
+  int iteration = 0, converged = 0;
+  const int n2k = 2 * k;
+  for (int it = 1; it <= a.max_iter; ++it) {
+    iteration = it;
+    normal_eq<KM>(k, a, L, M);
+    const bool solved = solve_step<KM>(n2k, a, L, M);
+    if (t == 0) {
+      int finite = solved ? 1 : 0;
+      for (int i = 0; i < n2k; ++i) {
+        // torch.clamp semantics: NaN stays NaN (and ends the iteration),
+        // +-Inf clamps to +-max_step (fmin/fmax alone would drop the NaN).
+        const double v = L.step[i];
+        L.step[i] = isnan(v) ? v : fmin(fmax(v, -a.max_step), a.max_step);
+        finite &= isfinite(L.step[i]) ? 1 : 0;
+      }
+      L.flag = finite;
+    }
+    __syncthreads();
+    if (!L.flag) break;
+    const double previous = L.obj;
+    double scale_taken = 0.0;
+    bool accepted = false;
+    for (int ls = 0; ls < 5 && !accepted; ++ls) {
+      const double scale = 1.0 / (double)(1 << ls);
+      if (t < k) {
+        for (int d = 0; d < 2; ++d) {
+          const double nd = d ? a.n2 : a.n1;
+          double v = fmod(L.pos[t][d] + scale * L.step[2 * t + d], nd);
+          if (v < 0.0) v += nd;
+          L.tpos[t][d] = v;
+        }
+      }
+      __syncthreads();
+      data_sums(y, L.tpos, 0, k, a, false, L);
+      fit<KM>(L.tpos, k, a, L, M, L.tamp, &L.tobj);
+      // one decision for the whole workgroup (L.obj changes below)
+      if (t == 0) L.flag = L.tobj < L.obj ? 1 : 0;
+      __syncthreads();
+      const bool better = L.flag != 0;
+      __syncthreads();
+      if (better) {
+        accepted = true;
+        scale_taken = scale;
+        if (t < k) {
+          L.pos[t][0] = L.tpos[t][0];
+          L.pos[t][1] = L.tpos[t][1];
+          L.amp[t] = L.tamp[t];
+        }
+        if (t == 0) {
+          L.obj = L.tobj;
+          L.lambda = fmax(L.lambda * 0.5, 1e-8);
+        }
+      }
+      __syncthreads();
+    }
+    if (!accepted) {
+      if (t == 0) L.lambda = fmin(L.lambda * 10.0, 1e8);
+      __syncthreads();
+      // restore the factor of the accepted positions for the next Jacobian
+      data_sums(y, L.pos, 0, k, a, true, L);
+      fit<KM>(L.pos, k, a, L, M, L.amp, &L.obj);
+      if (L.lambda >= 1e8) {
+        converged = 1;
+        break;
+      }
+      continue;
+    }
+    // weighted sums and factor at the accepted positions
+    data_sums(y, L.pos, 0, k, a, true, L);
+    fit<KM>(L.pos, k, a, L, M, L.amp, &L.obj);
+    double maxstep = 0.0;
+    for (int i = 0; i < n2k; ++i) maxstep = fmax(maxstep, fabs(scale_taken * L.step[i]));
+    const double rel = (previous - L.obj) / fmax(previous, a.eps);
+    if (maxstep <= a.tolerance || rel <= a.tolerance * a.tolerance) {
+      converged = 1;
+      break;
+    }
+  }
+  // ------------------------------------------- position standard deviation
+  normal_eq<KM>(k, a, L, M);
+  if (a.dims == 1) {
+    // 1-D grids: the axis-0 coordinates (N1 = M1 = 1) have zero Jacobian
+    // columns; the Fisher matrix of the axis-1 coordinates is the block the
+    // identity on axis 0 leaves (its std is not reported)
+    for (int e = t; e < n2k * n2k; e += OG_THREADS) {
+      const int r = e / n2k, q = e % n2k;
+      if (!(r & 1) || !(q & 1)) M.nm[r][q] = (r == q) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+  }
+  // cov = 0.5 noise Fisher^-1, noise = ||r||^2 / max(M - K, 1)
+  const double half_noise = 0.5 * L.obj / (double)max((int)m - k, 1);
+  const bool fisher_ok = chol_r<2 * KM + 1>(M.nm, n2k, &L.flag);
+  // diag of the inverse: || L^-1 e_i ||^2, one column per thread, the
+  // columns kept in the storage of h (free here; 2K x 2K doubles fit in it)
+  if (t < n2k) {
+    double* v = &M.h[0][0].x;
+    for (int i = 0; i < n2k; ++i) {
+      double s = (i == t) ? 1.0 : 0.0;
+      for (int j = t; j < i; ++j) s -= M.nm[i][j] * v[j * n2k + t];
+      v[i * n2k + t] = i < t ? 0.0 : s / M.nm[i][i];
+    }
+    double acc = 0.0;
+    for (int i = t; i < n2k; ++i) acc += v[i * n2k + t] * v[i * n2k + t];
+    L.diag[t] = fisher_ok ? sqrt(fmax(half_noise * acc, 0.0)) : NAN;
+  }
+  __syncthreads();
+  if (t < kmax_out) {
+    const long long o = (long long)b * kmax_out + t;
+    const bool live = t < k;
+    out_init[2 * o] = live ? L.init_pos[t][0] : NAN;
+    out_init[2 * o + 1] = live ? L.init_pos[t][1] : NAN;
+    out_pos[2 * o] = live ? L.pos[t][0] : NAN;
+    out_pos[2 * o + 1] = live ? L.pos[t][1] : NAN;
+    out_amp[o] = live ? L.amp[t] : make_double2(NAN, NAN);
+    out_std[2 * o] = live ? L.diag[2 * t] : NAN;
+    out_std[2 * o + 1] = live ? L.diag[2 * t + 1] : NAN;
+  }
+  if (t == 0) {
+    out_norm[2 * b + 1] = sqrt(L.obj);
+    out_count[3 * b] = k;
+    out_count[3 * b + 1] = iteration;
+    out_count[3 * b + 2] = converged;
+  }
+}
+
 // C: float2 (complex64 data) or double2 (complex128 data).
 template <typename C>
 __global__ void __launch_bounds__(OG_THREADS)
 k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
           double* __restrict__ out_init, double* __restrict__ out_pos, double2* __restrict__ out_amp,
-          double* __restrict__ out_std, double* __restrict__ out_norm, int* __restrict__ out_count) {
+          double* __restrict__ out_std, double* __restrict__ out_norm, int* __restrict__ out_count,
+          char* __restrict__ ws) {
   __shared__ OgLds L;
   const int b = blockIdx.x, t = threadIdx.x;
   const long long n = (long long)a.n1 * a.n2, m = (long long)a.m1 * a.m2;
@@ -580,7 +752,7 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
   // ------------------------------------------------------- BIC model order
   int k = L.nsel;
   if (a.component_count <= 0 && k > 1) {
-    for (int k0 = 0; k0 < k; k0 += OG_KMAX) data_sums(y, L.pos, k0, min(k, k0 + OG_KMAX), a, false, L);
+    for (int k0 = 0; k0 < k; k0 += OG_KLDS) data_sums(y, L.pos, k0, min(k, k0 + OG_KLDS), a, false, L);
     gram<OG_BICMAX + 1>(L.pos, k, a, L.u.bic.g);
     chol_z<OG_BICMAX + 1>(L.u.bic.g, k, &L.flag);
     if (t == 0) {
@@ -610,148 +782,23 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
     L.init_pos[t][1] = L.pos[t][1];
   }
   __syncthreads();
-  if (k > OG_KMAX) {
-    if (t == 0) {
-      out_count[3 * b] = k;
-      out_count[3 * b + 1] = -1;   // more atoms than the kernel refines
-      out_count[3 * b + 2] = 0;
-    }
-    return;
-  }
-  // ----------------------------------------------- projected Gauss--Newton
-  data_sums(y, L.pos, 0, k, a, true, L);
-  // The ridge loading keeps the Gram positive definite.  A failed pivot of
-  // the normal equations (no finite step exists) or a NaN step ends the
-  // iteration, as the non-finite test after the clamp does in the reference
-  // (offgrid.py:407-409).
-  fit(L.pos, k, a, L, L.amp, &L.obj);
-  if (t == 0) {
-    L.lambda = a.damping;
-    out_norm[2 * b] = sqrt(L.obj);
-  }
-  __syncthreads();
-  int iteration = 0, converged = 0;
-  const int n2k = 2 * k;
-  for (int it = 1; it <= a.max_iter; ++it) {
-    iteration = it;
-    normal_eq(k, a, L);
-    const bool solved = solve_step(n2k, a, L);
-    if (t == 0) {
-      int finite = solved ? 1 : 0;
-      for (int i = 0; i < n2k; ++i) {
-        // torch.clamp semantics: NaN stays NaN (and ends the iteration),
-        // +-Inf clamps to +-max_step (fmin/fmax alone would drop the NaN).
-        const double v = L.step[i];
-        L.step[i] = isnan(v) ? v : fmin(fmax(v, -a.max_step), a.max_step);
-        finite &= isfinite(L.step[i]) ? 1 : 0;
-      }
-      L.flag = finite;
-    }
-    __syncthreads();
-    if (!L.flag) break;
-    const double previous = L.obj;
-    double scale_taken = 0.0;
-    bool accepted = false;
-    for (int ls = 0; ls < 5 && !accepted; ++ls) {
-      const double scale = 1.0 / (double)(1 << ls);
-      if (t < k) {
-        for (int d = 0; d < 2; ++d) {
-          const double nd = d ? a.n2 : a.n1;
-          double v = fmod(L.pos[t][d] + scale * L.step[2 * t + d], nd);
-          if (v < 0.0) v += nd;
-          L.tpos[t][d] = v;
-        }
-      }
-      __syncthreads();
-      data_sums(y, L.tpos, 0, k, a, false, L);
-      fit(L.tpos, k, a, L, L.tamp, &L.tobj);
-      // one decision for the whole workgroup (L.obj changes below)
-      if (t == 0) L.flag = L.tobj < L.obj ? 1 : 0;
-      __syncthreads();
-      const bool better = L.flag != 0;
-      __syncthreads();
-      if (better) {
-        accepted = true;
-        scale_taken = scale;
-        if (t < k) {
-          L.pos[t][0] = L.tpos[t][0];
-          L.pos[t][1] = L.tpos[t][1];
-          L.amp[t] = L.tamp[t];
-        }
-        if (t == 0) {
-          L.obj = L.tobj;
-          L.lambda = fmax(L.lambda * 0.5, 1e-8);
-        }
-      }
-      __syncthreads();
-    }
-    if (!accepted) {
-      if (t == 0) L.lambda = fmin(L.lambda * 10.0, 1e8);
-      __syncthreads();
-      // restore the factor of the accepted positions for the next Jacobian
-      data_sums(y, L.pos, 0, k, a, true, L);
-      fit(L.pos, k, a, L, L.amp, &L.obj);
-      if (L.lambda >= 1e8) {
-        converged = 1;
-        break;
-      }
-      continue;
-    }
-    // weighted sums and factor at the accepted positions
-    data_sums(y, L.pos, 0, k, a, true, L);
-    fit(L.pos, k, a, L, L.amp, &L.obj);
-    double maxstep = 0.0;
-    for (int i = 0; i < n2k; ++i) maxstep = fmax(maxstep, fabs(scale_taken * L.step[i]));
-    const double rel = (previous - L.obj) / fmax(previous, a.eps);
-    if (maxstep <= a.tolerance || rel <= a.tolerance * a.tolerance) {
-      converged = 1;
-      break;
-    }
-  }
-  // ------------------------------------------- position standard deviation
-  normal_eq(k, a, L);
-  if (a.dims == 1) {
-    // 1-D grids: the axis-0 coordinates (N1 = M1 = 1) have zero Jacobian
-    // columns; the Fisher matrix of the axis-1 coordinates is the block the
-    // identity on axis 0 leaves (its std is not reported)
-    for (int e = t; e < n2k * n2k; e += OG_THREADS) {
-      const int r = e / n2k, q = e % n2k;
-      if (!(r & 1) || !(q & 1)) L.u.gn.nm[r][q] = (r == q) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-  }
-  // cov = 0.5 noise Fisher^-1, noise = ||r||^2 / max(M - K, 1)
-  const double half_noise = 0.5 * L.obj / (double)max((int)m - k, 1);
-  const bool fisher_ok = chol_r<2 * OG_KMAX + 1>(L.u.gn.nm, n2k, &L.flag);
-  // diag of the inverse: || L^-1 e_i ||^2, one column per thread
-  if (t < n2k) {
-    double v[2 * OG_KMAX];
-    for (int i = 0; i < n2k; ++i) {
-      double s = (i == t) ? 1.0 : 0.0;
-      for (int j = t; j < i; ++j) s -= L.u.gn.nm[i][j] * v[j];
-      v[i] = i < t ? 0.0 : s / L.u.gn.nm[i][i];
-    }
-    double acc = 0.0;
-    for (int i = t; i < n2k; ++i) acc += v[i] * v[i];
-    L.diag[t] = fisher_ok ? sqrt(fmax(half_noise * acc, 0.0)) : NAN;
-  }
-  __syncthreads();
-  if (t < kmax_out) {
-    const long long o = (long long)b * kmax_out + t;
-    const bool live = t < k;
-    out_init[2 * o] = live ? L.init_pos[t][0] : NAN;
-    out_init[2 * o + 1] = live ? L.init_pos[t][1] : NAN;
-    out_pos[2 * o] = live ? L.pos[t][0] : NAN;
-    out_pos[2 * o + 1] = live ? L.pos[t][1] : NAN;
-    out_amp[o] = live ? L.amp[t] : make_double2(NAN, NAN);
-    out_std[2 * o] = live ? L.diag[2 * t] : NAN;
-    out_std[2 * o + 1] = live ? L.diag[2 * t + 1] : NAN;
-  }
-  if (t == 0) {
-    out_norm[2 * b + 1] = sqrt(L.obj);
+  if (k <= OG_KLDS) {
+    const GnMats<OG_KLDS> M{L.u.gn.g, L.u.gn.l, L.u.gn.h, L.u.gn.nm};
+    refine<OG_KLDS>(y, a, L, M, k, b, m, kmax_out, out_init, out_pos, out_amp, out_std, out_norm,
+                     out_count);
+  } else if (ws != nullptr) {
+    char* base = ws + (size_t)b * OG_WS_BYTES;
+    GnMats<OG_KMAX> M;
+    M.g = reinterpret_cast<double2 (*)[OG_KMAX + 1]>(base);
+    M.l = M.g + OG_KMAX;
+    M.h = reinterpret_cast<double2 (*)[2 * OG_KMAX + 1]>(M.l + OG_KMAX);
+    M.nm = reinterpret_cast<double (*)[2 * OG_KMAX + 1]>(M.h + OG_KMAX);
+    refine<OG_KMAX>(y, a, L, M, k, b, m, kmax_out, out_init, out_pos, out_amp, out_std, out_norm,
+                     out_count);
+  } else if (t == 0) {
     out_count[3 * b] = k;
-    out_count[3 * b + 1] = iteration;
-    out_count[3 * b + 2] = converged;
+    out_count[3 * b + 1] = -1;   // more atoms than LDS holds and no workspace
+    out_count[3 * b + 2] = 0;
   }
 }
 
@@ -759,12 +806,14 @@ k_offgrid(const C* __restrict__ data, const C* __restrict__ coef, OgArgs a,
 
 extern "C" {
 
-int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
-                             const void* coefficients, int complex128, int component_count, int max_components,
-                             float relative_threshold, float minimum_separation, int max_iter, double tolerance,
-                             double max_step, double damping, double ridge, double eps, double tiny,
-                             double* initial_positions, double* positions, void* amplitudes, double* position_std,
-                             double* norms, int* counts, int dims) {
+size_t fbme_offgrid_workspace_bytes(int batch) { return batch < 1 ? 0 : (size_t)batch * OG_WS_BYTES; }
+
+int fbme_offgrid_refine_ws(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                           const void* coefficients, int complex128, int component_count, int max_components,
+                           float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                           double max_step, double damping, double ridge, double eps, double tiny,
+                           double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                           double* norms, int* counts, int dims, void* workspace, size_t workspace_bytes) {
   if (batch < 1 || n1 < 1 || n2 < 1 || m1 < 1 || m2 < 1 || m1 > n1 || m2 > n2) return FBME_ERR_ARG;
   if (dims != 1 && dims != 2) return FBME_ERR_ARG;
   if (dims == 1 && (n1 != 1 || m1 != 1)) return FBME_ERR_ARG;
@@ -772,7 +821,10 @@ int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, in
   if (!data || !coefficients || !initial_positions || !positions || !amplitudes || !position_std || !norms ||
       !counts)
     return FBME_ERR_ARG;
-  if (max_components < 1 || max_components > OG_BICMAX || component_count > OG_KMAX) return FBME_ERR_ARG;
+  if (workspace && (workspace_bytes < (size_t)batch * OG_WS_BYTES || (reinterpret_cast<size_t>(workspace) & 15u)))
+    return FBME_ERR_ARG;
+  if (max_components < 1 || max_components > OG_BICMAX || component_count > (workspace ? OG_KMAX : OG_KLDS))
+    return FBME_ERR_ARG;
   if (max_iter < 1 || !(tolerance > 0.0) || !(max_step > 0.0) || !(damping > 0.0) || ridge < 0.0) return FBME_ERR_ARG;
   OgArgs a;
   a.n1 = n1;
@@ -794,12 +846,24 @@ int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, in
   if (complex128)
     hipLaunchKernelGGL(k_offgrid<double2>, dim3(batch), dim3(OG_THREADS), 0, (hipStream_t)stream,
                        (const double2*)data, (const double2*)coefficients, a, initial_positions, positions,
-                       (double2*)amplitudes, position_std, norms, counts);
+                       (double2*)amplitudes, position_std, norms, counts, (char*)workspace);
   else
     hipLaunchKernelGGL(k_offgrid<float2>, dim3(batch), dim3(OG_THREADS), 0, (hipStream_t)stream,
                        (const float2*)data, (const float2*)coefficients, a, initial_positions, positions,
-                       (double2*)amplitudes, position_std, norms, counts);
+                       (double2*)amplitudes, position_std, norms, counts, (char*)workspace);
   return (int)hipGetLastError();
+}
+
+int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                             const void* coefficients, int complex128, int component_count, int max_components,
+                             float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                             double max_step, double damping, double ridge, double eps, double tiny,
+                             double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                             double* norms, int* counts, int dims) {
+  return fbme_offgrid_refine_ws(stream, batch, n1, n2, m1, m2, data, coefficients, complex128, component_count,
+                                max_components, relative_threshold, minimum_separation, max_iter, tolerance,
+                                max_step, damping, ridge, eps, tiny, initial_positions, positions, amplitudes,
+                                position_std, norms, counts, dims, nullptr, 0);
 }
 
 int fbme_offgrid_refine(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,

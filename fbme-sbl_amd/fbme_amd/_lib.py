@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 7
+ABI_VERSION = 8
 # return codes (include/fbme_hip.h)
 ERR_ARG = 1
 ERR_HIP = 2
@@ -271,6 +271,14 @@ _SIGNATURES = {
                                          c_double, c_double, c_double, c_double,
                                          c_double, c_void_p, c_void_p, c_void_p,
                                          c_void_p, c_void_p, c_void_p, c_int]),
+    "fbme_offgrid_workspace_bytes": (c_size_t, [c_int]),
+    "fbme_offgrid_refine_ws": (c_int, [c_void_p, c_int, c_int, c_int, c_int,
+                                       c_int, c_void_p, c_void_p, c_int, c_int,
+                                       c_int, c_float, c_float, c_int, c_double,
+                                       c_double, c_double, c_double, c_double,
+                                       c_double, c_void_p, c_void_p, c_void_p,
+                                       c_void_p, c_void_p, c_void_p, c_int,
+                                       c_void_p, c_size_t]),
     # joint sparsity of the general solver
     "fbme_joint_init": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p,
                                 c_void_p, c_void_p, c_float, c_int, c_void_p,

@@ -7,8 +7,10 @@ order, projected Gauss--Newton and the position standard deviations all
 happen in the kernel, in FP64, without forming the M x K dictionary.  Same
 arguments, validation and result fields as the reference; results are
 returned in the data's real / complex precision.  1-D grids run as 1 x N
-(fbme_offgrid_refine_dims with one position coordinate per component); at
-most 64 candidate and 32 refined components per item.
+(fbme_offgrid_refine_ws with one position coordinate per component); at
+most 64 candidate and refined components per item (the default
+max_components).  Items with more than 32 refined components keep their
+refinement matrices in a device workspace of 388 KiB per item.
 """
 
 from __future__ import annotations
@@ -23,7 +25,8 @@ from torch import Tensor
 
 from fbme_amd import _lib
 
-_KMAX = 32
+_KMAX = 64        # refined components (more than _KLDS: with a workspace)
+_KLDS = 32
 _CANDIDATES = 64
 
 
@@ -133,19 +136,23 @@ def refine_offgrid_fourier(data: Tensor, coefficients: Tensor, grid_shape: Seque
     counts = torch.empty((batch, 3), dtype=torch.int32, device=device)
     finfo = torch.finfo(real)
     lib = _lib.load()
+    ws, ws_bytes = None, 0
+    if rows > _KLDS:
+        ws_bytes = int(lib.fbme_offgrid_workspace_bytes(batch))
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
     with torch.cuda.device(device):
         stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
-        _lib.check(lib.fbme_offgrid_refine_dims(
+        _lib.check(lib.fbme_offgrid_refine_ws(
             stream, batch, *kgrid, *ksamples, ctypes.c_void_p(y.data_ptr()),
             ctypes.c_void_p(x.data_ptr()), 1 if double else 0, count, maximum, float(relative_threshold),
             float(minimum_separation), int(max_iter), float(tolerance), float(max_step), float(damping),
             float(ridge), float(finfo.eps), float(finfo.tiny), ctypes.c_void_p(init.data_ptr()),
             ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(amp.data_ptr()), ctypes.c_void_p(std.data_ptr()),
-            ctypes.c_void_p(norms.data_ptr()), ctypes.c_void_p(counts.data_ptr()), dims), "fbme_offgrid_refine")
+            ctypes.c_void_p(norms.data_ptr()), ctypes.c_void_p(counts.data_ptr()), dims,
+            ctypes.c_void_p(ws.data_ptr() if ws is not None else None), ws_bytes), "fbme_offgrid_refine")
         cnt = counts.cpu()
     if bool((cnt[:, 1] < 0).any()):
-        raise NotImplementedError(
-            f"the model order exceeds the {_KMAX} components the MI355X refinement kernel refines.")
+        raise RuntimeError("off-grid refinement: a model order above 32 components without a workspace")
     k = int(cnt[0, 0])
     if bool((cnt[:, 0] != k).any()):
         raise RuntimeError("batch items refined to different component counts")

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 7
+#define FBME_ABI_VERSION 8
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -559,7 +559,8 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
 
 /* ------------------------------------------------------------------------
  * Off-grid refinement: refine_offgrid_fourier (torch_sbl/offgrid.py:70-226)
- * for 2-D grids.  One workgroup per scene runs the whole refinement:
+ * for 2-D grids (1-D: fbme_offgrid_refine_dims).  One workgroup per scene
+ * runs the whole refinement:
  * top max(64, 16 max_components) magnitudes of the coefficients, periodic
  * non-maximum suppression (offgrid.py:229-295), the extended BIC model order
  * when component_count <= 0 (offgrid.py:318-361), projected Gauss-Newton
@@ -568,7 +569,8 @@ int fbme_dense_posterior(void* stream, int batch, int n1, int n2, int m1, int m2
  * data [B, M1*M2] and coefficients [B, N1*N2] are complex64 (complex128 when
  * `complex128` is nonzero).  Limits: M1 M2 >= 2, max_components <= 64, at
  * most 32 refined atoms (counts[b*3+1] = -1 when the model order exceeds
- * 32); every pointer is required (FBME_ERR_ARG otherwise).
+ * 32; fbme_offgrid_refine_ws refines up to 64); every pointer is required
+ * (FBME_ERR_ARG otherwise).
  * Outputs, with Kout = component_count > 0 ? component_count :
  * max_components rows per scene (rows past the scene's count are NaN):
  * initial_positions / positions / position_std [B, Kout, 2] double,
@@ -593,6 +595,20 @@ int fbme_offgrid_refine_dims(void* stream, int batch, int n1, int n2, int m1, in
                              double max_step, double damping, double ridge, double eps, double tiny,
                              double* initial_positions, double* positions, void* amplitudes, double* position_std,
                              double* norms, int* counts, int dims);
+/* fbme_offgrid_refine_dims with up to 64 refined atoms (the reference's
+ * default max_components = 64 with a model order above 32, or
+ * component_count <= 64).  workspace: fbme_offgrid_workspace_bytes(batch)
+ * bytes of device memory, 16-byte aligned, scratch of the K x K and 2K x 2K
+ * refinement matrices of scenes with K > 32 (scenes with K <= 32 keep them in
+ * LDS and do not touch it; both give the same bits for the same K).  A null
+ * workspace is fbme_offgrid_refine_dims. */
+size_t fbme_offgrid_workspace_bytes(int batch);
+int fbme_offgrid_refine_ws(void* stream, int batch, int n1, int n2, int m1, int m2, const void* data,
+                           const void* coefficients, int complex128, int component_count, int max_components,
+                           float relative_threshold, float minimum_separation, int max_iter, double tolerance,
+                           double max_step, double damping, double ridge, double eps, double tiny,
+                           double* initial_positions, double* positions, void* amplitudes, double* position_std,
+                           double* norms, int* counts, int dims, void* workspace, size_t workspace_bytes);
 
 /* ------------------------------------------------------------------------
  * Joint sparsity of the general solver: SBLConfig(joint_sparsity=True)

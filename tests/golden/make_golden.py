@@ -20,7 +20,7 @@ What it pins (functions no closed-form check fixes end to end):
   * ``sbl_2d`` in mode "cg" with ``joint_sparsity=True`` (shared and
     per-snapshot noise).
   * ``refine_offgrid_fourier`` on scenes with off-grid point sources, on 2-D
-    and on 1-D grids.
+    and on 1-D grids, with model orders up to 48 components.
   * ``sbl_2d`` in mode "uamp" with the default ``SBLConfig`` (convergence
     checks, no polish) and with a polish (``uamp_default``).
 
@@ -336,6 +336,44 @@ def offgrid_1d(reference_root: str) -> None:
         print("wrote", name, "components", arrays["positions"].shape, "iterations", arrays["iterations"])
 
 
+OFFGRID_WIDE_CASES = [
+    # more than 32 refined components: name, seeds, grid, samples, sources, snr_db, dtype, component_count
+    ("offgrid_128x128_80x80_k40", [1058], (128, 128), (80, 80), 40, 25.0, np.complex64, None),
+    ("offgrid_128x128_80x80_c128_k36", [1059], (128, 128), (80, 80), 36, 25.0, np.complex128, None),
+    ("offgrid_96x96_64x64_b2_c128_k48", [1060, 1061], (96, 96), (64, 64), 48, 20.0, np.complex128, 48),
+    ("offgrid_1d_1024_640_c128_k40", [1062], (1024,), (640,), 40, 25.0, np.complex128, None),
+]
+
+
+def offgrid_wide(reference_root: str) -> None:
+    """``refine_offgrid_fourier`` with model orders above 32 components
+    (the default ``max_components=64``), on 2-D and 1-D grids."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl.offgrid import refine_offgrid_fourier
+
+    torch.set_num_threads(8)
+    for name, seeds, grid, samples, k, snr, dtype, count in OFFGRID_WIDE_CASES:
+        if len(grid) == 2:
+            made = [scenes.offgrid_scene(s, grid, samples, k=k, snr_db=snr) for s in seeds]
+        else:
+            made = [scenes.offgrid_scene_1d(s, grid[0], samples[0], k=k, snr_db=snr) for s in seeds]
+        y = np.stack([mm[0].y for mm in made])
+        x = np.stack([mm[0].x for mm in made]).astype(dtype)
+        res = refine_offgrid_fourier(torch.from_numpy(y.astype(dtype)), torch.from_numpy(x), grid,
+                                     component_count=count)
+        arrays = dict(y=y.astype(dtype), x=x, grid=np.array(grid), samples=np.array(samples),
+                      true_positions=np.stack([mm[1] for mm in made]),
+                      true_amplitudes=np.stack([mm[2] for mm in made]),
+                      component_count=np.array(-1 if count is None else count))
+        for field in ("initial_positions", "positions", "amplitudes", "position_std",
+                      "initial_residual_norm", "residual_norm", "iterations", "converged"):
+            arrays[field] = getattr(res, field).detach().cpu().numpy()
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "components", arrays["positions"].shape, "iterations", arrays["iterations"],
+              flush=True)
+
+
 HEADLINE_CASES = [
     # name, generator, seed, grid, samples, k, snr_db, profiles
     ("headline_256x256_160x160_s1000", "ellipse", 1000, (256, 256), (160, 160), 300, 10.0, ("fast", "stable")),
@@ -621,6 +659,8 @@ if __name__ == "__main__":
         uamp_default(root)
     elif which == "offgrid_1d":
         offgrid_1d(root)
+    elif which == "offgrid_wide":
+        offgrid_wide(root)
     elif which == "dense_joint":
         dense_joint(root)
     elif which == "precond":
@@ -654,6 +694,8 @@ if __name__ == "__main__":
         masked_plans(root)
         joint_cg(root)
         offgrid(root)
+        offgrid_1d(root)
+        offgrid_wide(root)
         headline(root)
         generic(root)
         active_set(root)

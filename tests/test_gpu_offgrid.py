@@ -4,7 +4,9 @@ on 2-D and 1-D grids.
 
 The kernel evaluates the same variable-projection objective in FP64 through
 closed-form atom inner products, so it matches the float64 oracle to
-rounding (1e-7 bins, 1e-6 relative amplitude).  Against the reference's own
+rounding (1e-7 bins, 1e-6 relative amplitude).  Model orders above 32
+components keep the refinement matrices in a device workspace instead of
+LDS; a scene refined either way gives the same bits.  Against the reference's own
 runs, which compute in float32 for complex64 data, the bar is that of the
 oracle: identical candidate peaks and model order, positions within 5e-5
 bins, amplitudes within 5e-5 relative, residual norms within 1e-5, position
@@ -23,7 +25,10 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = ["offgrid_64x64_40x40_c128", "offgrid_64x64_40x40_c64", "offgrid_64x64_40x40_b2_k5",
           "offgrid_128x96_80x60", "offgrid_1d_256_160_c64", "offgrid_1d_512_200_c128",
-          "offgrid_1d_128_96_b2_k4"]
+          "offgrid_1d_128_96_b2_k4",
+          # model orders above 32 (the refinement matrices in the workspace)
+          "offgrid_128x128_80x80_k40", "offgrid_128x128_80x80_c128_k36", "offgrid_96x96_64x64_b2_c128_k48",
+          "offgrid_1d_1024_640_c128_k40"]
 
 
 def _run(y, x, grid, **kw):
@@ -108,6 +113,53 @@ def test_fast_sbl_offgrid_refinement(device):
         d = np.abs(pos - p)
         d = np.minimum(d, 64 - d)
         assert np.min(np.hypot(d[:, 0], d[:, 1])) < 0.05
+
+
+def test_workspace_path_equals_lds_path(device):
+    """fbme_offgrid_refine_ws with a workspace and fbme_offgrid_refine_dims
+    (LDS only) on scenes with K <= 32: bitwise equal outputs."""
+    import ctypes
+
+    from fbme_amd import _lib, scenes
+    lib = _lib.load()
+    made = [scenes.offgrid_scene(1810 + i, (64, 64), (40, 40), k=7, snr_db=20.0) for i in range(3)]
+    y = torch.from_numpy(np.stack([m[0].y for m in made])).to(device)
+    x = torch.from_numpy(np.stack([m[0].x for m in made]).astype(np.complex64)).to(device)
+    batch, rows = 3, 7
+    eps, tiny = float(np.finfo(np.float32).eps), float(np.finfo(np.float32).tiny)
+
+    def run(ws):
+        f64 = dict(dtype=torch.float64, device=device)
+        out = [torch.full((batch, rows, 2), -1.0, **f64), torch.full((batch, rows, 2), -1.0, **f64),
+               torch.zeros((batch, rows), dtype=torch.complex128, device=device),
+               torch.full((batch, rows, 2), -1.0, **f64), torch.zeros((batch, 2), **f64),
+               torch.zeros((batch, 3), dtype=torch.int32, device=device)]
+        stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+        args = [stream, batch, 64, 64, 40, 40, ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(x.data_ptr()), 0,
+                rows, rows, 0.1, 1.5, 15, 1e-4, 0.35, 1e-3, 1e-7, eps, tiny,
+                *[ctypes.c_void_p(t.data_ptr()) for t in out], 2]
+        if ws is None:
+            _lib.check(lib.fbme_offgrid_refine_dims(*args), "fbme_offgrid_refine_dims")
+        else:
+            _lib.check(lib.fbme_offgrid_refine_ws(*args, ctypes.c_void_p(ws.data_ptr()), ws.numel()),
+                       "fbme_offgrid_refine_ws")
+        torch.cuda.synchronize()
+        return out
+
+    nbytes = int(lib.fbme_offgrid_workspace_bytes(batch))
+    assert nbytes == batch * 397312
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    a, b = run(None), run(ws)
+    assert int(a[5][:, 0].min()) == rows and int(a[5][:, 1].min()) > 0
+    for u, v in zip(a, b):
+        assert torch.equal(u, v)
+    # a workspace smaller than batch x 388 KiB is refused
+    stream = ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+    rc = lib.fbme_offgrid_refine_ws(stream, batch, 64, 64, 40, 40, ctypes.c_void_p(y.data_ptr()),
+                                    ctypes.c_void_p(x.data_ptr()), 0, 40, 64, 0.1, 1.5, 15, 1e-4, 0.35, 1e-3,
+                                    1e-7, eps, tiny, *[ctypes.c_void_p(t.data_ptr()) for t in a], 2,
+                                    ctypes.c_void_p(ws.data_ptr()), nbytes - 256)
+    assert rc == _lib.ERR_ARG
 
 
 def test_validation(device):

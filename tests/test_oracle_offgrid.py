@@ -73,7 +73,9 @@ def test_refinement_recovers_planted_positions():
 @pytest.mark.parametrize("name", ["offgrid_64x64_40x40_c128", "offgrid_64x64_40x40_c64",
                                   "offgrid_64x64_40x40_b2_k5", "offgrid_128x96_80x60",
                                   "offgrid_1d_256_160_c64", "offgrid_1d_512_200_c128",
-                                  "offgrid_1d_128_96_b2_k4"])
+                                  "offgrid_1d_128_96_b2_k4", "offgrid_128x128_80x80_k40",
+                                  "offgrid_128x128_80x80_c128_k36", "offgrid_96x96_64x64_b2_c128_k48",
+                                  "offgrid_1d_1024_640_c128_k40"])
 def test_offgrid_matches_reference_golden(name):
     g = np.load(golden_path(name + ".npz"))
     exact = g["y"].dtype == np.complex128
