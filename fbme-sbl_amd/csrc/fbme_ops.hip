@@ -7,6 +7,8 @@
 // fbme_uamp.hip instead.
 #include <algorithm>
 
+#include <cfloat>
+
 #include "fbme_device.h"
 
 using namespace fbme;
@@ -256,6 +258,23 @@ k_joint_noise_finish(const double* __restrict__ partial, int nblocks, int batch,
     for (int b = 0; b < batch; ++b) out[b] = damping * old_noise[b] + (1.0f - damping) * shared;
 }
 
+// A fixed noise variance: the value itself, or max(value, floor) in place
+// of an update (shared: the batch's largest floor; the mean of the clamped
+// values, clamped again at the largest floor, is that).
+__global__ void k_fixed_noise(int batch, float value, const float* __restrict__ floor_, float* __restrict__ noise,
+                              int shared) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  float f = 0.f;
+  if (floor_) {
+    if (shared)
+      for (int i = 0; i < batch; ++i) f = fmaxf(f, floor_[i]);
+    else
+      f = floor_[b];
+  }
+  noise[b] = floor_ ? fmaxf(value, f) : value;
+}
+
 inline int nblk(int n) { return (n + FBME_BLOCK - 1) / FBME_BLOCK; }
 
 // Plain HBM streams (the measured ceiling the kernels are compared with):
@@ -445,6 +464,13 @@ int fbme_uamp_noise_update(void* stream, int batch, int m, const void* data, con
   return fbme_uamp_noise_update_masked(stream, batch, m, nullptr, m, data, prediction, old_noise, effective_dof,
                                        variance_floor, output, noise_shape, noise_rate, damping, evidence_update,
                                        epsilon, partial);
+}
+
+int fbme_fixed_noise(void* stream, int batch, float value, const float* variance_floor, float* noise, int shared) {
+  if (batch < 1 || !noise || !(value > 0.f) || !(value <= FLT_MAX)) return FBME_ERR_ARG;
+  hipLaunchKernelGGL(k_fixed_noise, dim3((batch + 63) / 64), dim3(64), 0, (hipStream_t)stream, batch, value,
+                     variance_floor, noise, shared);
+  return (int)hipGetLastError();
 }
 
 int fbme_joint_noise_update(void* stream, int batch, int m, const unsigned char* mask, int observed,

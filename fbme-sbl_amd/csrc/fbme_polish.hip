@@ -40,6 +40,7 @@ struct PolishArgs {
   float margin_factor, jitter_multiplier, max_precision, polish_damping;
   float hyper_shape, hyper_rate, noise_shape, noise_rate;
   float false_alarm_threshold;
+  float fixed_noise;   // > 0: fixed_noise_variance
   int evaluations, exchange_size, diagnostics, evidence_noise;
   int prune_tail, reinitialize, evidence_gate;
 };
@@ -747,6 +748,8 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
     }
     upd = fmax(upd, c.floor_v);
     work_noise = keep * best_noise + (1.0 - keep) * upd;
+    // a fixed noise variance: max(fixed, floor), undamped (solver.py:4358-4387)
+    if (a.fixed_noise > 0.f) work_noise = fmax((double)a.fixed_noise, c.floor_v);
     __syncthreads();
     const int exch = min(a.exchange_size, min(max(K - 1, 0), (int)(n_grid - K)));
     if (exch > 0 && pit == 0 && phase == 0) {
@@ -813,7 +816,8 @@ k_polish(const FbmeScene* __restrict__ scenes, int nscenes, int phase, const flo
       }
       nt = fmin(fmax(nt, c.floor_v), (double)a.max_precision);
       const double ns = fmax(fmax(fabs(best_noise), fabs(nt)), tiny);
-      const double nres = fabs(best_noise - nt) / ns;
+      // no noise residual with a fixed noise variance (solver.py:3457-3458)
+      const double nres = a.fixed_noise > 0.f ? 0.0 : fabs(best_noise - nt) / ns;
       bool finite = true, nonincreasing = true;
       for (int i = 0; i < a.evaluations; ++i) {
         const float t = (float)trace[i];
@@ -1265,6 +1269,7 @@ int fbme_launch_polish(const FbmeScene* scenes, int nscenes, int phase, const vo
   a.noise_shape = o->noise_shape;
   a.noise_rate = o->noise_rate;
   a.false_alarm_threshold = 0.0f;
+  a.fixed_noise = o->fixed_noise;
   a.evaluations = o->polish_iterations;
   a.exchange_size = o->exchange_size;
   a.diagnostics = o->diagnostics;

@@ -38,7 +38,7 @@ int fbme_launch_col_forward(const FbmeScene*, const FbmeWork*, int, int, int, co
                             const void*, hipStream_t);
 int fbme_launch_mask_bits16(const unsigned char*, int, int, void*, hipStream_t);
 int fbme_launch_scalars(const FbmeScene*, int, double*, const double*, int, int, const unsigned char*, float,
-                        float, float, float, float, float, int, hipStream_t);
+                        float, float, float, float, float, int, float, hipStream_t);
 size_t fbme_polish_state_bytes(void);
 size_t fbme_polish_result_bytes(void);
 int fbme_launch_polish(const FbmeScene*, int, int, const void*, const double*, const void*, const void*,
@@ -219,6 +219,7 @@ void fbme_default_options(FbmePlanOptions* o, int stable) {
   o->prune_tail = 1;
   o->reinitialize = 1;
   o->evidence_gate = 1;
+  o->fixed_noise = 0.0f;
 }
 
 fbme_plan* fbme_plan_create(int nscenes, const int* shapes, const FbmePlanOptions* opt, char* err, int errlen) {
@@ -247,6 +248,8 @@ fbme_plan* fbme_plan_create_masked(int nscenes, const int* shapes, const int* ob
     return fail("the residual exchange runs on the pruned, evidence-gated polish only");
   if (opt->diagnostics && !opt->prune_tail)
     return fail("the fixed-budget diagnostics run on the pruned polish only");
+  if (!(opt->fixed_noise >= 0.0f) || !std::isfinite(opt->fixed_noise))
+    return fail("fixed_noise must be 0 (estimated) or a positive noise variance");
   fbme_plan* p = new fbme_plan();
   p->opt = *opt;
   p->nscenes = nscenes;
@@ -752,7 +755,8 @@ int plan_phase(const fbme_plan* p, const FbmeIO* io, hipStream_t st, int phase, 
     // runs at least one: min_iters >= 1); the LDS kernels write it here.
     FBME_TRY(row(p->row_split, 0, 0, nullptr, nullptr, 0));
     FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 0, nullptr, o.min_variance, o.noise_fraction,
-                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, o.fixed_noise,
+                                 st));
     return 0;
   }
   if (phase == 1) {
@@ -765,7 +769,8 @@ int plan_phase(const fbme_plan* p, const FbmeIO* io, hipStream_t st, int phase, 
     FBME_TRY(row(nr, nr16, (t == 1 ? 1 : 2) + (track ? 3 : 0), nullptr, nullptr, 0));
     FBME_TRY(col_forward(nc, nc16, t == 1 ? 1 : 0));
     FBME_TRY(fbme_launch_scalars(scenes, ns, sc, part, ps, track ? 2 : 1, nullptr, o.min_variance, o.noise_fraction,
-                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, o.fixed_noise,
+                                 st));
     return 0;
   }
   if (phase != 2) return FBME_ERR_ARG;
@@ -776,7 +781,8 @@ int plan_phase(const fbme_plan* p, const FbmeIO* io, hipStream_t st, int phase, 
     // result (solver.py:1907-1925, 2081-2126).
     FBME_TRY(row(p->row_split, nrow16, 3, nullptr, nullptr, 1));
     FBME_TRY(fbme_launch_scalars(scenes, p->nscenes, sc, part, ps, 3, nullptr, o.min_variance, o.noise_fraction,
-                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, st));
+                                 o.blend, o.damping, o.noise_shape, o.noise_rate, o.evidence_noise, o.fixed_noise,
+                                 st));
     FBME_TRY(fbme_launch_output_uamp(scenes, tilew, ntile, ntile, sc, mean, var, gamma, io->x, io->posterior_variance,
                                      io->precision, io->support_mask, o.false_alarm,
                                      (io->support_mask && o.false_alarm > 0.f) ? 1 : 0, io->noise, io->dof, io->nle,

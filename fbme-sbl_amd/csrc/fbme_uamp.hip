@@ -381,6 +381,7 @@ struct ScalarArgs {
   float min_variance, noise_fraction, blend, damping;
   float noise_shape, noise_rate;
   int evidence_noise;
+  float fixed_noise;   // > 0: fixed_noise_variance
 };
 
 // mode 0: initialisation (solver.py:1063-1189): power, floor, sigma^2_0,
@@ -391,7 +392,9 @@ struct ScalarArgs {
 //         samples are zero); no pass over the backprojection is needed.
 // mode 1: end of a UAMP iteration: dof clamp, noise EM/evidence update with
 //         damping (triton_kernels.py:375-417, solver.py:1764-1798), data
-//         variance = sum v for the next message update.
+//         variance = sum v for the next message update.  A fixed noise
+//         variance starts unclamped and is max(fixed, floor) after every
+//         update, undamped (solver.py:1151-1156, 1747-1750, 1788-1798).
 // mode 2: mode 1 and the relative change ||mu_t - mu_t-1|| / max(||mu_t||,
 //         eps) of the step (solver.py:1815-1825) from the tracked row pass.
 // mode 3: the final E-step's clamped dof only (the UAMP posterior without a
@@ -443,7 +446,7 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
     scs[SC_POWER] = power;
     scs[SC_FLOOR] = floor_v;
     scs[SC_BPE] = bpe;
-    scs[SC_NOISE] = fmaxf(a.noise_fraction * power, floor_v);
+    scs[SC_NOISE] = a.fixed_noise > 0.f ? a.fixed_noise : fmaxf(a.noise_fraction * power, floor_v);
     scs[SC_DV] = dv0;
     scs[SC_DOF] = 0.0;
   } else {
@@ -460,7 +463,7 @@ k_scalars(const FbmeScene* __restrict__ scenes, int nscenes, double* __restrict_
     }
     cand = fmax(cand, floor_v);
     const double updated = fmax(a.damping * noise + (1.0 - a.damping) * cand, floor_v);
-    scs[SC_NOISE] = (float)updated;
+    scs[SC_NOISE] = a.fixed_noise > 0.f ? fmaxf(a.fixed_noise, (float)floor_v) : (float)updated;
     scs[SC_DOF] = (float)dof;
     scs[SC_DV] = (float)q1;
   }
@@ -558,9 +561,9 @@ int fbme_launch_col_forward(const FbmeScene* scenes, const FbmeWork* work, int n
 int fbme_launch_scalars(const FbmeScene* scenes, int nscenes, double* sc, const double* part, int pstride,
                         int mode, const unsigned char* active, float min_variance, float noise_fraction,
                         float blend, float damping, float noise_shape, float noise_rate, int evidence_noise,
-                        hipStream_t stream) {
+                        float fixed_noise, hipStream_t stream) {
   if (nscenes <= 0) return 0;
-  ScalarArgs a{min_variance, noise_fraction, blend, damping, noise_shape, noise_rate, evidence_noise};
+  ScalarArgs a{min_variance, noise_fraction, blend, damping, noise_shape, noise_rate, evidence_noise, fixed_noise};
   hipLaunchKernelGGL(k_scalars, dim3(nscenes), dim3(64), 0, stream, scenes, nscenes, sc, part, pstride, mode,
                      active, a);
   return (int)hipGetLastError();

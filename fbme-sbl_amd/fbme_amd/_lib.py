@@ -14,7 +14,7 @@ from ctypes import (POINTER, Structure, c_char_p, c_double, c_float, c_int,
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
                         "libfbme_hip.so")
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 # return codes (include/fbme_hip.h)
 ERR_ARG = 1
 ERR_HIP = 2
@@ -53,6 +53,7 @@ class FbmePlanOptions(Structure):
         ("fft_kernel", c_int),
         ("truncation_diagnostic", c_int), ("prune_tail", c_int),
         ("reinitialize", c_int), ("evidence_gate", c_int),
+        ("fixed_noise", c_float),
     ]
 
 
@@ -319,6 +320,7 @@ _SIGNATURES = {
                                               c_void_p, c_void_p, c_float,
                                               c_float, c_float, c_int,
                                               c_float, c_void_p]),
+    "fbme_fixed_noise": (c_int, [c_void_p, c_int, c_float, c_void_p, c_void_p, c_int]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

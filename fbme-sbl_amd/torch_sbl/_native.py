@@ -47,12 +47,9 @@ def _unsupported(cfg: SBLConfig) -> list[str]:
         "uamp_polish_coordinate_steps": cfg.uamp_polish_coordinate_steps != 0,
         "uamp_polish_noise_coordinate_steps": (
             cfg.uamp_polish_noise_coordinate_steps != 0),
-        "fixed_noise_variance": cfg.fixed_noise_variance is not None,
         "joint_sparsity": cfg.joint_sparsity,
         "spatial coupling": (cfg.spatial_coupling > 0.0
                              or cfg.learn_spatial_coupling),
-        "initialization='uniform'": cfg.initialization != "backprojection",
-        "uamp_auto_iterations=False": not cfg.uamp_auto_iterations,
         "phase_error_correction": cfg.phase_error_correction,
         "range_migration_correction": cfg.range_migration_correction,
         "learn_snapshot_covariance": cfg.learn_snapshot_covariance,
@@ -88,7 +85,10 @@ def plan_options(cfg: SBLConfig, grid_shape, sample_shape, observed: int | None 
     opt.damping = float(cfg.damping)
     opt.min_variance = float(cfg.min_variance)
     opt.max_precision = float(cfg.max_precision)
-    opt.blend = float(cfg.initialization_blend)
+    # 'uniform' is the blend 0 (solver.py:1070-1089); its first lifted
+    # message is then A^H of the first message, the backprojection scaled by
+    # the same factor (solver.py:2823-2830)
+    opt.blend = float(cfg.initialization_blend) if cfg.initialization == "backprojection" else 0.0
     opt.noise_fraction = float(cfg.initial_noise_fraction)
     opt.margin_factor = float(cfg.uamp_polish_evidence_margin_factor)
     opt.jitter_multiplier = float(cfg.cuda_cholesky_jitter_multiplier)
@@ -100,6 +100,10 @@ def plan_options(cfg: SBLConfig, grid_shape, sample_shape, observed: int | None 
     opt.evidence_noise = 1 if cfg.noise_update == "evidence" else 0
     opt.iteration_offset = float(cfg.uamp_iteration_offset)
     opt.iteration_scale = float(cfg.uamp_iteration_scale)
+    if not cfg.uamp_auto_iterations:
+        # T_u = max_iter (solver.py:1295-1309): ceil(max_iter + 0 N / M)
+        opt.iteration_offset = float(cfg.max_iter)
+        opt.iteration_scale = 0.0
     opt.min_iters = int(max(cfg.min_iter, cfg.uamp_min_iter))
     opt.max_iters = int(cfg.max_iter)
     opt.hyper_shape = float(cfg.hyper_shape)
@@ -110,6 +114,7 @@ def plan_options(cfg: SBLConfig, grid_shape, sample_shape, observed: int | None 
     opt.prune_tail = int(cfg.uamp_polish_prune_tail)
     opt.reinitialize = int(cfg.uamp_polish_reinitialize)
     opt.evidence_gate = int(cfg.uamp_polish_evidence_gate)
+    opt.fixed_noise = float(cfg.fixed_noise_variance or 0.0)
     # solver.py:4034-4040
     opt.truncation_diagnostic = int(cfg.uamp_polish_prune_tail and (
         cfg.uamp_polish_compute_truncation_diagnostic

@@ -1,7 +1,8 @@
 """General solver, mode="cg": covariance-free PCG + Hutchinson on HIP kernels.
 
 Control flow follows torch_sbl/solver.py:1194-1904 and 2023-2080 (probe
-count and Rademacher probes, adaptive CG tolerance schedule, warm start,
+count, Rademacher or randomised Hadamard probes, adaptive CG tolerance
+schedule, a fixed noise variance, warm start,
 active-set switch for a single scene, convergence checks); the arithmetic is
 in libfbme_hip.so (fbme_pcg.hip): BTTB spectrum and Toeplitz matvec with the
 1 + P right-hand sides batched into the FFT leading dimension, low-rank
@@ -27,8 +28,10 @@ Differences from the reference, all documented:
   * the active set holds at most 256 atoms (the reference's default
     active_set_max_size); larger maxima raise NotImplementedError;
   * the low-rank preconditioner's top-32 selection takes N <= 2^26 cells;
-  * the 'nystrom' preconditioner, Hadamard probes, adaptive probe counts
-    and spatial coupling raise NotImplementedError.
+  * cg_matvec='fourier' runs the same operator as 'tbt' (the BTTB products,
+    or the grid transforms on 256-point grids): results agree to rounding;
+  * the 'nystrom' preconditioner, a low-rank rank other than 32, adaptive
+    probe counts and spatial coupling raise NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -139,16 +142,10 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
     if cfg.cg_preconditioner == "low_rank" and cfg.cg_preconditioner_rank != RANK:
         bad.append("cg_preconditioner_rank != 32")
-    if cfg.cg_matvec != "tbt":
-        bad.append("cg_matvec='fourier'")
-    if cfg.probe_type != "rademacher":
-        bad.append("probe_type='hadamard'")
     if cfg.adaptive_probes:
         bad.append("adaptive_probes")
     if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
         bad.append("spatial coupling")
-    if cfg.fixed_noise_variance is not None:
-        bad.append("fixed_noise_variance")
     if cfg.active_set and cfg.active_set_max_size > 256:
         bad.append("active_set_max_size > 256")
     if bad:
@@ -159,15 +156,51 @@ def _check_supported(cfg: SBLConfig) -> None:
 _PROBES = {}
 
 
-def _probes(seed: int, b: int, m: int, p: int, dev) -> torch.Tensor:
-    """Rademacher probes [b, p, m] (complex64, device) drawn as the reference
-    draws them (torch CPU generator, solver.py:5990-6000), cached."""
-    key = (int(seed), b, m, p, str(dev))
+def _hadamard_signs(g: torch.Generator, b: int, shape: tuple, p: int) -> torch.Tensor:
+    """Randomised hierarchical Hadamard probes [b, m, p] (solver.py:6002-6083):
+    D H with one Rademacher diagonal per block of columns; H[i, j] =
+    (-1)^popcount(code_i & gray_j), code_i the Morton (bit-interleaved) code
+    of sample i's coordinates in `shape` and gray_j the Gray code of j within
+    blocks of min(8, 2^floor(log2 p)) columns (one block when p fills the
+    power-of-two embedding of `shape`)."""
+    bits = [max(0, (length - 1).bit_length()) for length in shape]
+    embed = 1 << sum(bits)
+    if p > embed:
+        raise ValueError("Hadamard probe count cannot exceed its embedding.")
+    block = p if p == embed else min(8, 1 << max(0, p.bit_length() - 1))
+    nblocks = -(-p // block)
+    m = int(np.prod(shape))
+    signs = 2 * torch.randint(0, 2, (b, m, nblocks), generator=g) - 1
+    coords = np.indices(shape).reshape(len(shape), m).astype(np.int64)
+    code = np.zeros(m, np.int64)
+    out = 0
+    for ib in range(max(bits, default=0)):
+        for d, nb in enumerate(bits):
+            if ib < nb:
+                code |= ((coords[d] >> ib) & 1) << out
+                out += 1
+    j = np.arange(p, dtype=np.int64)
+    gray = (j % block) ^ ((j % block) >> 1)
+    had = 1 - 2 * (np.bitwise_count(code[:, None] & gray[None, :]) & 1).astype(np.int64)
+    return signs[:, :, torch.from_numpy(j // block)] * torch.from_numpy(had)[None]
+
+
+def _probes(seed: int, b: int, m: int, p: int, dev, kind: str = "rademacher",
+            shape: tuple | None = None) -> torch.Tensor:
+    """Hutchinson probes [b, p, m] (complex64, device) drawn as the reference
+    draws them (torch CPU generator, solver.py:5979-6083): Rademacher signs,
+    or randomised Hadamard columns over the sample grid `shape` (the
+    observed samples flat under a mask), cached."""
+    shape = tuple(shape) if shape is not None else (m,)
+    key = (int(seed), b, m, p, str(dev), kind, shape)
     t = _PROBES.get(key)
     if t is None:
         g = torch.Generator(device="cpu")
         g.manual_seed(seed)
-        t = (2 * torch.randint(0, 2, (b, m, p), generator=g) - 1).to(torch.complex64)
+        if kind == "hadamard":
+            t = _hadamard_signs(g, b, shape, p).to(torch.complex64)
+        else:
+            t = (2 * torch.randint(0, 2, (b, m, p), generator=g) - 1).to(torch.complex64)
         t = t.permute(0, 2, 1).contiguous().to(dev)
         if len(_PROBES) >= 16:
             _PROBES.pop(next(iter(_PROBES)))
@@ -280,7 +313,8 @@ class _Solver:
         if probes_override is not None:
             probes = probes_override.permute(0, 2, 1).to(dev)
         else:
-            probes = _probes(cfg.seed, ns, self.m_obs, self.p, dev)
+            probes = _probes(cfg.seed, ns, self.m_obs, self.p, dev, cfg.probe_type,
+                             self.samples if self.obs is None else None)
         if self.obs is None:
             self.rhs[:, nd:] = probes
         else:
@@ -589,6 +623,10 @@ class _Solver:
                                                _ptr(self.power), _ptr(self.floor),
                                                self._blend(), 1 if self.shared else 0,
                                                _ptr(self.gamma), _ptr(self.noise)), "joint_init")
+        if cfg.fixed_noise_variance is not None:
+            # solver.py:1151-1156: the fixed value, unclamped
+            self._chk(self.lib.fbme_fixed_noise(self.stream, self.b, float(cfg.fixed_noise_variance), None,
+                                                _ptr(self.noise), 0), "fixed_noise")
         self.grid_sums(with_mean=False)
 
     def _blend(self):
@@ -830,6 +868,12 @@ class _Solver:
         else:
             self._cg_em(keep, update)
         if not update:
+            return
+        if cfg.fixed_noise_variance is not None:
+            # max(fixed, floor), undamped (solver.py:1747-1750, 1788-1798)
+            self._chk(self.lib.fbme_fixed_noise(self.stream, self.b, float(cfg.fixed_noise_variance),
+                                                _ptr(self.floor), _ptr(self.noise), 1 if self.shared else 0),
+                      "fixed_noise")
             return
         self._forward(self.mean, self.pred)
         noise = self.lib.fbme_joint_noise_update if self.shared else self.lib.fbme_uamp_noise_update_masked

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define FBME_ABI_VERSION 8
+#define FBME_ABI_VERSION 9
 
 #define FBME_ERR_ARG 1
 #define FBME_ERR_HIP 2
@@ -71,6 +71,10 @@ typedef struct FbmePlanOptions {
                               (solver.py:4094-4106); 0 keep the UAMP gamma */
   int evidence_gate;       /* 1 keep the best evidence (solver.py:4168-4298);
                               0 take every evaluation                      */
+  float fixed_noise;       /* 0: noise estimated; > 0: fixed_noise_variance
+                              (solver.py:1151-1156, 1747-1750, 4358-4361,
+                              3457-3458): sigma^2_0 = fixed, every update
+                              max(fixed, floor), no noise residual         */
 } FbmePlanOptions;
 
 /* Outputs of one plan replay; per-scene arrays are in the caller's scene
@@ -651,6 +655,13 @@ int fbme_uamp_noise_update_masked(void* stream, int batch, int m, const unsigned
                                   const float* effective_dof, const float* variance_floor, float* output,
                                   float noise_shape, float noise_rate, float damping, int evidence_update,
                                   float epsilon, double* partial);
+/* SBLConfig.fixed_noise_variance of the general solver (solver.py:1151-1156,
+ * 1747-1750, 1765-1770, 1788-1798): noise[b] = value at the start
+ * (variance_floor NULL) and max(value, variance_floor[b]) in place of every
+ * noise update; with `shared` (joint sparsity with a shared noise) the
+ * largest floor of the batch, max(value, max_b variance_floor[b]), for every
+ * snapshot.  value > 0. */
+int fbme_fixed_noise(void* stream, int batch, float value, const float* variance_floor, float* noise, int shared);
 
 /* ------------------------------------------------------------------------
  * Learned warm start of the general solver (no counterpart in the

@@ -53,6 +53,8 @@ class CGConfig:
     noise_shape: float = 1e-8
     noise_rate: float = 1e-8
     initial_noise_fraction: float = 1e-2
+    fixed_noise_variance: float | None = None
+    probe_type: str = "rademacher"
     initialization_blend: float = 0.1
     noise_update: str = "em"
     damping: float = 0.15
@@ -102,6 +104,52 @@ def rademacher_probes(batch: int, m: int, count: int, seed: int) -> np.ndarray:
     g.manual_seed(seed)
     signs = 2 * torch.randint(0, 2, (batch, m, count), generator=g) - 1
     return signs.numpy().astype(np.float64)
+
+
+def hadamard_probes(batch: int, shape, count: int, seed: int) -> np.ndarray:
+    """Randomised hierarchical Hadamard probes (solver.py:6002-6083): one
+    Rademacher sign per sample and block of columns, times columns of the
+    Sylvester Hadamard matrix of the smallest power-of-two embedding of
+    `shape`, its rows in Morton (bit-interleaved) order of the sample
+    coordinates and its columns in Gray-code order within blocks of
+    min(8, 2^floor(log2 count)) (one block when count fills the embedding).
+    The Hadamard matrix is built by the Sylvester doubling H -> [[H, H],
+    [H, -H]], and the Morton codes by explicit loops."""
+    import torch
+    shape = tuple(int(v) for v in shape)
+    m = int(np.prod(shape))
+    bits = [max(0, (length - 1).bit_length()) for length in shape]
+    embed = 1 << sum(bits)
+    if count > embed:
+        raise ValueError("Hadamard probe count cannot exceed its embedding.")
+    block = count if count == embed else min(8, 1 << max(0, count.bit_length() - 1))
+    nblocks = (count + block - 1) // block
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    signs = (2 * torch.randint(0, 2, (batch, m, nblocks), generator=g) - 1).numpy().astype(np.float64)
+    codes = np.zeros(m, np.int64)
+    for flat in range(m):
+        coord, rem = [], flat
+        for length in reversed(shape):
+            coord.append(rem % length)
+            rem //= length
+        coord = coord[::-1]
+        code, out = 0, 0
+        for ib in range(max(bits, default=0)):
+            for d, nb in enumerate(bits):
+                if ib < nb:
+                    code |= ((coord[d] >> ib) & 1) << out
+                    out += 1
+        codes[flat] = code
+    cols = [(j % block) ^ ((j % block) >> 1) for j in range(count)]
+    if embed <= 4096:
+        h = np.ones((1, 1))
+        while h.shape[0] < embed:
+            h = np.block([[h, h], [h, -h]])
+        had = h[np.ix_(codes, cols)]
+    else:   # H[i, j] = (-1)^popcount(i & j), the entries of the same matrix
+        had = 1.0 - 2.0 * (np.bitwise_count(codes[:, None] & np.array(cols)[None, :]) & 1)
+    return signs[:, :, [j // block for j in range(count)]] * had[None]
 
 
 def cg_schedule(it: int, cfg: CGConfig) -> float:
@@ -446,10 +494,18 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
         prior_floor = floor
     gamma = np.maximum(gamma, prior_floor[:, None])
     noise = np.maximum(cfg.initial_noise_fraction * power, floor)
+    if cfg.fixed_noise_variance is not None:
+        noise = np.full(b, cfg.fixed_noise_variance)      # solver.py:1151-1156
     if shared:
         noise = np.full(b, noise.mean())
     pcount = probe_count(n, m, cfg)
-    probes = rademacher_probes(1 if shared else b, m, pcount, cfg.seed).astype(np.complex128)
+    if cfg.probe_type == "hadamard":
+        # probe_shape: the sample grid, or the observed samples flat under a
+        # mask (solver.py:1209-1220)
+        pshape = op.samples if op.idx is None else (m,)
+        probes = hadamard_probes(1 if shared else b, pshape, pcount, cfg.seed).astype(np.complex128)
+    else:
+        probes = rademacher_probes(1 if shared else b, m, pcount, cfg.seed).astype(np.complex128)
     lprobes = op.adj(probes)
     prev = np.zeros((b, n), np.complex128)
     keep = max(cfg.damping, cfg.cg_damping)
@@ -509,11 +565,16 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             new_noise = (e + noise * dof + cfg.noise_rate) / (m + cfg.noise_shape)
         else:
             new_noise = (e + cfg.noise_rate) / np.maximum(m - dof + cfg.noise_shape, eps)
+        if cfg.fixed_noise_variance is not None:
+            new_noise = np.full(b, cfg.fixed_noise_variance)   # solver.py:1747-1750
         new_noise = np.maximum(new_noise, floor)
         if shared:
             new_noise = np.full(b, max(new_noise.mean(), floor.max()))
         gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, prior_floor[:, None])
-        noise = keep * noise + (1 - keep) * new_noise
+        if cfg.fixed_noise_variance is not None:
+            noise = new_noise                                  # undamped (solver.py:1788-1798)
+        else:
+            noise = keep * noise + (1 - keep) * new_noise
         change = np.max(np.linalg.norm(mean - prev, axis=1)
                         / np.maximum(np.linalg.norm(mean, axis=1), eps))
         prev = mean

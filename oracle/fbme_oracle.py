@@ -190,6 +190,8 @@ class PlanConfig:
     evidence_gate: bool = True
     truncation_diagnostic: bool = False
     tol: float = 0.0
+    uamp_auto_iterations: bool = True
+    fixed_noise_variance: float | None = None
     convergence_check_interval: int = 2
     convergence_patience: int = 2
 
@@ -249,7 +251,10 @@ class PlanConfig:
 
 def uamp_iteration_budget(n: int, m: int, cfg: PlanConfig) -> int:
     """T_u = min(max_iter, max(max(min_iter, uamp_min_iter),
-    ceil(offset + scale N/M)))  (solver.py:1280-1309)."""
+    ceil(offset + scale N/M)))  (solver.py:1280-1309); max_iter without
+    uamp_auto_iterations (solver.py:1294-1295)."""
+    if not cfg.uamp_auto_iterations:
+        return cfg.max_iter
     minimum = max(cfg.min_iter, cfg.uamp_min_iter)
     budget = ceil(cfg.uamp_iteration_offset
                   + cfg.uamp_iteration_scale * n / m)
@@ -281,6 +286,8 @@ def uamp_initial_state(y, grid, samples, cfg: PlanConfig, eps, tiny, mask=None):
              + (1.0 - cfg.initialization_blend) * (power[:, None] / n))
     gamma = np.maximum(gamma, floor[:, None])
     noise = np.maximum(cfg.initial_noise_fraction * power, floor)
+    if cfg.fixed_noise_variance is not None:
+        noise = np.full_like(power, cfg.fixed_noise_variance)      # solver.py:1151-1156
     state = dict(
         mean=np.zeros_like(bp), variance=gamma.copy(),
         message=np.zeros((y.shape[0], m), np.complex128),
@@ -370,6 +377,8 @@ def uamp_sweep(y, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
         cand = np.maximum(cand, floor)
         noise = np.maximum(cfg.damping * noise + (1.0 - cfg.damping) * cand,
                            floor)
+        if cfg.fixed_noise_variance is not None:
+            noise = np.maximum(cfg.fixed_noise_variance, floor)      # solver.py:1747-1750, 1788-1798
         gamma = np.maximum(next_prior, floor[:, None])
         if trace is not None:
             trace.gamma.append(gamma.copy())
@@ -559,6 +568,8 @@ def polish(y, sweep, grid, samples, cfg: PlanConfig, *, eps=F32_EPS,
                 m - dof + cfg.noise_shape, eps) - inactive
         upd_noise = np.maximum(upd_noise, floor)
         noise = keep * base_noise + (1.0 - keep) * upd_noise
+        if cfg.fixed_noise_variance is not None:
+            noise = np.maximum(cfg.fixed_noise_variance, floor)      # solver.py:4358-4387
         working = base_prior.copy()
         working[rows, support] = upd_gamma
         exch = min(cfg.exchange_size, max(k - 1, 0), n - k)
@@ -690,6 +701,8 @@ def fixed_budget_residuals(y, result, floor, grid, samples, cfg, *, eps,
     ntarget = np.minimum(np.maximum(ntarget, floor), cfg.max_precision)
     nscale = np.maximum(np.maximum(np.abs(noise), np.abs(ntarget)), tiny)
     noise_res = np.abs(noise - ntarget) / nscale
+    if cfg.fixed_noise_variance is not None:
+        noise_res = np.zeros_like(noise_res)                          # solver.py:3457-3458
     tr = result["evidence_trace"]
     finite = np.all(np.isfinite(tr), axis=1)
     nonincreasing = np.all(tr[:, 1:] <= tr[:, :-1], axis=1)

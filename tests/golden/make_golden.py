@@ -23,6 +23,8 @@ What it pins (functions no closed-form check fixes end to end):
     and on 1-D grids, with model orders up to 48 components.
   * ``sbl_2d`` in mode "uamp" with the default ``SBLConfig`` (convergence
     checks, no polish) and with a polish (``uamp_default``).
+  * options the presets leave at their defaults: a fixed noise variance,
+    uniform initial variances, Hadamard probes (``options``).
 
 Each fixture stores the scene inputs and the reference outputs as plain
 arrays in a compressed ``.npz`` (no pickles).
@@ -374,6 +376,74 @@ def offgrid_wide(reference_root: str) -> None:
               flush=True)
 
 
+OPTION_CASES = [
+    # SBLConfig options beyond the presets' defaults.  name, generator, seeds,
+    # grid, samples, k, snr_db, entry ("fast" / "stable": fast_sbl under the
+    # plan numerics of plan_*; "cg": sbl_2d in mode 'cg'), overrides; a
+    # fixed_noise_variance of "true" is the scenes' mean true noise variance
+    ("options_stable_fixed_noise_64x64_40x40", "uniform", (1100,), (64, 64), (40, 40), 20, 15.0, "stable",
+     dict(fixed_noise_variance="true")),
+    ("options_fast_fixed_noise_96x80_60x48", "ellipse", (1101,), (96, 80), (60, 48), 24, 10.0, "fast",
+     dict(fixed_noise_variance="true")),
+    ("options_fast_uniform_init_96x80_60x48", "ellipse", (1102,), (96, 80), (60, 48), 24, 10.0, "fast",
+     dict(initialization="uniform")),
+    ("options_cg_fixed_noise_32x32_24x24", "uniform", (1103,), (32, 32), (24, 24), 8, 20.0, "cg",
+     dict(fixed_noise_variance="true", max_iter=30, return_history=True)),
+    ("options_cg_joint_fixed_noise_32x32_24x24_b3", "uniform", (1104, 1105, 1106), (32, 32), (24, 24), 8, 20.0,
+     "cg", dict(fixed_noise_variance="true", joint_sparsity=True, max_iter=20, return_history=True)),
+    # randomised hierarchical Hadamard probes (32 of a 1024 embedding; 40 x 30
+    # samples: 64 x 32 embedding, 16 probes in blocks of 8)
+    ("options_cg_hadamard_32x32_24x24", "uniform", (1107,), (32, 32), (24, 24), 8, 20.0, "cg",
+     dict(probe_type="hadamard", max_iter=30, return_history=True)),
+    ("options_cg_hadamard_48x40_40x30", "uniform", (1108,), (48, 40), (40, 30), 10, 15.0, "cg",
+     dict(probe_type="hadamard", hutchinson_probes=16, auto_probe_scaling=False, max_iter=25,
+          return_history=True)),
+    # Q d through full-grid FFTs instead of the BTTB embedding (the same operator)
+    ("options_cg_fourier_32x32_24x24", "uniform", (1109,), (32, 32), (24, 24), 8, 20.0, "cg",
+     dict(cg_matvec="fourier", max_iter=30, return_history=True)),
+]
+
+
+def options(reference_root: str) -> None:
+    """Reference runs with SBLConfig options the presets leave at their
+    defaults: a fixed noise variance (plan numerics and mode 'cg', also with
+    joint sparsity), uniform initial variances and Hadamard probes."""
+    sys.path.insert(0, reference_root)
+    import torch
+    from torch_sbl import SBLConfig, fast_sbl, sbl_2d
+
+    torch.set_num_threads(8)
+    plan_over = dict(
+        fast=dict(uamp_polish_size=32, uamp_polish_iterations=2, uamp_polish_residual_exchange_size=0,
+                  uamp_polish_fixed_budget_diagnostics=False, uamp_polish_compute_truncation_diagnostic=False),
+        stable=dict(uamp_polish_size=32, uamp_polish_iterations=3, uamp_polish_residual_exchange_size=12,
+                    uamp_polish_fixed_budget_diagnostics=True, uamp_polish_compute_truncation_diagnostic=False))
+    for name, gen, seeds, grid, samples, k, snr, entry, over in OPTION_CASES:
+        fn = dict(uniform=scenes.uniform_scene, ellipse=scenes.ellipse_scene)[gen]
+        batch = [fn(seed, grid, samples, k=k, snr_db=snr) for seed in seeds]
+        y = np.stack([sc.y for sc in batch]).astype(np.complex64)
+        over = dict(over)
+        if over.get("fixed_noise_variance") == "true":
+            over["fixed_noise_variance"] = float(np.float32(np.mean([sc.noise_variance for sc in batch])))
+        data = torch.from_numpy(y if len(seeds) > 1 else y[0])
+        if entry == "cg":
+            res = sbl_2d(data, grid, config=SBLConfig(mode="cg", device="cpu", **over))
+        else:
+            cfg = SBLConfig.paper_hybrid_uamp(device="cpu", uamp_backend="torch", **plan_over[entry], **over)
+            res = fast_sbl(data, grid, config=cfg)
+        arrays = dict(y=y, x_true=np.stack([sc.x for sc in batch]), grid=np.array(grid), samples=np.array(samples),
+                      entry=np.array(entry))
+        arrays.update(_result_arrays(res))
+        for key in ("relative_change", "cg_iterations", "noise_variance"):
+            if res.history.get(key):
+                arrays["history_" + key] = np.array(res.history[key], np.float64)
+        for key, value in over.items():
+            arrays["override_" + key] = np.array(value)
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print("wrote", name, "iterations", res.iterations, "noise", _np(res.noise_variance).reshape(-1)[:3],
+              "active", res.active_set_size, flush=True)
+
+
 HEADLINE_CASES = [
     # name, generator, seed, grid, samples, k, snr_db, profiles
     ("headline_256x256_160x160_s1000", "ellipse", 1000, (256, 256), (160, 160), 300, 10.0, ("fast", "stable")),
@@ -618,6 +688,10 @@ UAMP_CASES = [
     ("uamp_64x48_60x44_tol1e-3", "uniform", (1099,), (64, 48), (60, 44), 5, 30.0,
      dict(max_iter=60, uamp_iteration_offset=40.0, tol=1e-3)),
     ("uamp_64x64_40x40_polish", "uniform", (1094,), (64, 64), (40, 40), 12, 20.0, dict(uamp_polish_size=16)),
+    # uniform initial variances; a fixed budget T_u = max_iter
+    ("uamp_64x64_40x40_uniform", "uniform", (1095,), (64, 64), (40, 40), 15, 20.0, dict(initialization="uniform")),
+    ("uamp_96x64_48x40_fixed_budget", "ellipse", (1096,), (96, 64), (48, 40), 20, 15.0,
+     dict(uamp_auto_iterations=False, max_iter=24)),
 ]
 
 
@@ -661,6 +735,8 @@ if __name__ == "__main__":
         offgrid_1d(root)
     elif which == "offgrid_wide":
         offgrid_wide(root)
+    elif which == "options":
+        options(root)
     elif which == "dense_joint":
         dense_joint(root)
     elif which == "precond":
@@ -672,12 +748,16 @@ if __name__ == "__main__":
         active_set(root)
         preconditioners(root)
         dense_joint(root)
+        uamp_default(root)
+        options(root)
     elif which == "headline":
         headline(root)
         generic(root)
         active_set(root)
         preconditioners(root)
         dense_joint(root)
+        uamp_default(root)
+        options(root)
     elif which == "long_cg":
         long_cg(root)
     elif which == "masked":
@@ -701,3 +781,5 @@ if __name__ == "__main__":
         active_set(root)
         preconditioners(root)
         dense_joint(root)
+        uamp_default(root)
+        options(root)

@@ -176,15 +176,25 @@ def test_torch_cpu_baseline_port_matches_oracle(case):
 
 
 UAMP_MODE_CASES = ["uamp_64x64_40x40", "uamp_128x96_48x40_b2", "uamp_64x48_60x44_tol1e-3",
-                   "uamp_64x64_40x40_polish"]
+                   "uamp_64x64_40x40_polish", "uamp_64x64_40x40_uniform", "uamp_96x64_48x40_fixed_budget"]
 _UAMP_FIELDS = dict(tol="tol", max_iter="max_iter", uamp_iteration_offset="uamp_iteration_offset",
-                    uamp_polish_size="polish_size")
+                    uamp_polish_size="polish_size", uamp_auto_iterations="uamp_auto_iterations")
 
 
 def uamp_mode_config(g):
-    """PlanConfig of SBLConfig(mode='uamp') with the golden's overrides."""
+    """PlanConfig of SBLConfig(mode='uamp') with the golden's overrides
+    (initialization='uniform' is the backprojection blend 0,
+    solver.py:1070-1089)."""
     grid, samples = tuple(int(v) for v in g["grid"]), tuple(int(v) for v in g["samples"])
-    over = {_UAMP_FIELDS[k[len("override_"):]]: g[k].item() for k in g.files if k.startswith("override_")}
+    over = {}
+    for key in g.files:
+        if not key.startswith("override_"):
+            continue
+        name, value = key[len("override_"):], g[key].item()
+        if name == "initialization":
+            over["initialization_blend"] = 0.1 if value == "backprojection" else 0.0
+        else:
+            over[_UAMP_FIELDS[name]] = value
     return grid, samples, fo.PlanConfig.uamp_default(int(np.prod(grid)), int(np.prod(samples)), **over)
 
 
