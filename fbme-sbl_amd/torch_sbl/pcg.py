@@ -30,8 +30,10 @@ Differences from the reference, all documented:
   * the low-rank preconditioner's top-32 selection takes N <= 2^26 cells;
   * cg_matvec='fourier' runs the same operator as 'tbt' (the BTTB products,
     or the grid transforms on 256-point grids): results agree to rounding;
-  * the 'nystrom' preconditioner, a low-rank rank other than 32, adaptive
-    probe counts and spatial coupling raise NotImplementedError.
+  * the low-rank preconditioner keeps 32 selected atoms and weights those
+    past cg_preconditioner_rank with 0 (ranks above 32 raise);
+  * the 'nystrom' preconditioner, adaptive probe counts and spatial
+    coupling raise NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -140,8 +142,8 @@ def _check_supported(cfg: SBLConfig) -> None:
     bad = []
     if cfg.cg_preconditioner not in ("low_rank", "circulant", "none"):
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
-    if cfg.cg_preconditioner == "low_rank" and cfg.cg_preconditioner_rank != RANK:
-        bad.append("cg_preconditioner_rank != 32")
+    if cfg.cg_preconditioner == "low_rank" and cfg.cg_preconditioner_rank > RANK:
+        bad.append("cg_preconditioner_rank > 32")
     if cfg.adaptive_probes:
         bad.append("adaptive_probes")
     if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
@@ -289,6 +291,12 @@ class _Solver:
         if self.precond == "circulant" and 0xFFFFFFFF in self.t.sample_codes.values():
             raise NotImplementedError("the circulant preconditioner needs M1, M2 that factor into 2, 3 and 5")
         self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256 and self.precond == "low_rank"
+        # rank of the low-rank preconditioner (solver.py:5797-5800): the
+        # strongest min(rank, M, N) atoms; below 32 the other selected atoms
+        # get weight 0 (zero factor columns, identity rows of the small
+        # system, not subtracted from the background)
+        self.lr_rank = (min(int(cfg.cg_preconditioner_rank), self.m_obs, self.n)
+                        if self.precond == "low_rank" and not self.dense else RANK)
         # both on the column-transform passes: the PCG update / direction sum
         # the inner-product partials of those passes themselves
         self.fused = self.grid_op and self.lr_fft and self.obs is None
@@ -655,6 +663,8 @@ class _Solver:
             return
         self._chk(self.lib.fbme_topk32(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
+        if self.lr_rank < RANK:
+            self.top_val[:, self.lr_rank:].zero_()
         gsum = self.sums[:, 0].contiguous()
         self._chk(self.lib.fbme_lowrank_setup_masked(
             self.stream, self.nsys, n1, n2, m1, m2, _ptr(self.top_idx), _ptr(self.top_val), _ptr(gsum),

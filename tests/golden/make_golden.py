@@ -401,6 +401,11 @@ OPTION_CASES = [
     # Q d through full-grid FFTs instead of the BTTB embedding (the same operator)
     ("options_cg_fourier_32x32_24x24", "uniform", (1109,), (32, 32), (24, 24), 8, 20.0, "cg",
      dict(cg_matvec="fourier", max_iter=30, return_history=True)),
+    # low-rank preconditioners of rank 8 and 0 (the scalar background)
+    ("options_cg_rank8_48x40_40x30", "uniform", (1110,), (48, 40), (40, 30), 10, 15.0, "cg",
+     dict(cg_preconditioner_rank=8, max_iter=25, return_history=True)),
+    ("options_cg_rank0_32x32_24x24", "uniform", (1111,), (32, 32), (24, 24), 8, 20.0, "cg",
+     dict(cg_preconditioner_rank=0, max_iter=20, return_history=True)),
 ]
 
 

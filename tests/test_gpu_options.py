@@ -7,8 +7,12 @@ the reference's runs (tests/options_golden.py) and the float64 oracles:
   and as a graph plan (replays bitwise equal to the eager solve);
 * initialization='uniform' under the Fast numerics;
 * fixed_noise_variance in mode 'cg', also with joint sparsity and a shared
-  noise, and randomised Hadamard probes (one at 36 active-set atoms): the
-  same iteration, PCG and active-set counts as the reference, x to 1e-4.
+  noise, randomised Hadamard probes (one at 36 active-set atoms) and
+  cg_matvec='fourier': the same iteration, PCG and active-set counts as the
+  reference, x to 1e-4 (achieved 4e-7 to 6e-6);
+* low-rank preconditioners of rank 8 and 0, whose PCG counts differ from the
+  reference's by whole convergence checks at a few EM iterations (bars
+  below).
 """
 
 import numpy as np
@@ -64,27 +68,48 @@ def _cg(device, g, grid, over, joint=False):
     return sbl_2d(y, grid, config=cfg)
 
 
+# Weak preconditioners need many PCG steps: a relative residual that lands
+# on the tolerance at a convergence check is decided by rounding, so some EM
+# iterations stop their PCG a check or two (8 steps each) apart from the
+# reference's float32 run, in both directions, and the iterates drift by
+# about the PCG tolerance.  Per case: EM iterations allowed to differ in PCG
+# count, largest difference, bar on x, bar on the noise.  Achieved: rank 8,
+# one iteration 8 apart, x 2.3e-4, noise 1.2e-7; rank 0 (the scalar
+# background), 7 iterations up to 16 apart, x 9.8e-4, noise 5.9e-5.
+SLACK = {"options_cg_rank8_48x40_40x30": (1, 8, 5e-4, 1e-4),
+         "options_cg_rank0_32x32_24x24": (8, 16, 2e-3, 1e-4)}
+
+
 @pytest.mark.parametrize("case", og.CG + og.CG_JOINT)
-def test_cg_fixed_noise_vs_reference(device, case):
+def test_cg_options_vs_reference(device, case):
     print("case", case, flush=True)
     g, grid, over = og.load(case)
     joint = case in og.CG_JOINT
     res = _cg(device, g, grid, over, joint)
     x = res.x.cpu().numpy()
     rel = og.rel(x, g["x"])
-    print(case, "iterations", res.iterations, int(g["iterations"]), "x vs reference", rel, flush=True)
+    hc = np.asarray(res.history["cg_iterations"]).astype(int)
+    rc = g["history_cg_iterations"].astype(int)
+    noise = res.noise_variance.reshape(-1).cpu().numpy()
+    nrel = float(np.max(np.abs(noise / g["noise_variance"].reshape(-1) - 1)))
+    print(case, "iterations", res.iterations, int(g["iterations"]), "x vs reference", rel, "noise", nrel,
+          "\n  PCG counts", hc.tolist(), "\n  reference ", rc.tolist(), flush=True)
+    slack, apart, xbar, nbar = SLACK.get(case, (0, 0, 1e-4, 1e-4))
     assert res.iterations == int(g["iterations"])
     assert res.active_set_size == int(g["active_set_size"])
-    assert rel < 1e-4
-    np.testing.assert_array_equal(np.asarray(res.history["cg_iterations"]), g["history_cg_iterations"])
-    noise = res.noise_variance.reshape(-1).cpu().numpy()
+    off = np.flatnonzero(hc != rc)
+    assert off.size <= slack and np.all(np.abs(hc[off] - rc[off]) <= apart), off
+    assert np.all((hc - rc) % 8 == 0)
+    assert rel < xbar
     if "fixed_noise_variance" in over:
         np.testing.assert_allclose(noise, g["noise_variance"].reshape(-1), rtol=1e-6)
         assert np.all(noise == np.float32(over["fixed_noise_variance"]))
         np.testing.assert_allclose(res.history["noise_variance"], g["history_noise_variance"], rtol=1e-6)
     else:
-        np.testing.assert_allclose(noise, g["noise_variance"].reshape(-1), rtol=1e-4)
-        np.testing.assert_allclose(res.history["noise_variance"], g["history_noise_variance"], rtol=1e-4)
+        assert nrel < nbar
+        if slack == 0:
+            np.testing.assert_allclose(res.history["noise_variance"], g["history_noise_variance"], rtol=1e-4)
     if not joint:
         orc = co.sbl_cg(g["y"][0], grid, og.cg_config(over))
-        assert og.rel(x, orc["x"]) < 1e-3
+        print("  x vs oracle", og.rel(x, orc["x"]), flush=True)
+        assert og.rel(x, orc["x"]) < max(1e-3, xbar)
