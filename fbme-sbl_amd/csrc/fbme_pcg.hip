@@ -513,6 +513,71 @@ k_lr_project(const float2* __restrict__ tab, const float2* __restrict__ v, int m
   }
 }
 
+// Projection partials proj[(b, r), k] = sum_m conj(U[b, k, m]) v[(b, r), m]
+// onto RANK stored columns (the Nystrom eigenvectors) over the samples
+// [chunk * ch, (chunk + 1) * ch) of scene b, for the LG lanes r0 .. r0 + LG - 1
+// of one lane group.  Sample tiles of U and of the lanes are staged in LDS;
+// thread (g, k) = (tid / 32, tid % 32) takes the samples g (mod 8) of a
+// tile.  The eight partial sums are added in FP64 in a fixed order; the
+// layout is k_lr_project's, for k_lr_corr.
+constexpr int SP_TILE = 64;
+constexpr int SP_CHUNK = 1024;
+
+template <int LG>
+__global__ void __launch_bounds__(FBME_BLOCK)
+k_stored_project(const float2* __restrict__ ucols, const float2* __restrict__ v, long long m, int rhs,
+                 double2* __restrict__ part) {
+  __shared__ float2 us[RANK][SP_TILE + 1];
+  __shared__ float2 vs[LG][SP_TILE];
+  __shared__ float2 buf[8][LG][RANK];
+  const int b = blockIdx.z;
+  const int r0 = blockIdx.y * LG;
+  const int nl = min(LG, rhs - r0);
+  const int k = threadIdx.x & (RANK - 1), g = threadIdx.x >> 5;
+  const float2* U = ucols + (long long)b * RANK * m;
+  const float2* vb = v + ((long long)b * rhs + r0) * m;
+  const long long s0 = (long long)blockIdx.x * SP_CHUNK, s1 = min(m, s0 + SP_CHUNK);
+  float2 acc[LG];
+#pragma unroll
+  for (int j = 0; j < LG; ++j) acc[j] = make_float2(0.f, 0.f);
+  for (long long base = s0; base < s1; base += SP_TILE) {
+    const int tl = (int)min((long long)SP_TILE, s1 - base);
+    __syncthreads();
+    for (int e = threadIdx.x; e < RANK * SP_TILE; e += FBME_BLOCK) {
+      const int q = e / SP_TILE, c = e - q * SP_TILE;
+      us[q][c] = c < tl ? U[(long long)q * m + base + c] : make_float2(0.f, 0.f);
+    }
+    for (int e = threadIdx.x; e < LG * SP_TILE; e += FBME_BLOCK) {
+      const int j = e / SP_TILE, c = e - j * SP_TILE;
+      vs[j][c] = (j < nl && c < tl) ? vb[(long long)j * m + base + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int c = g; c < tl; c += 8) {
+      const float2 u = us[k][c];
+      const float2 f = make_float2(u.x, -u.y);
+#pragma unroll
+      for (int j = 0; j < LG; ++j) {
+        const float2 w = vs[j][c];
+        acc[j].x += f.x * w.x - f.y * w.y;
+        acc[j].y += f.x * w.y + f.y * w.x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < LG; ++j) buf[g][j][k] = acc[j];
+  __syncthreads();
+  for (int e = threadIdx.x; e < nl * RANK; e += FBME_BLOCK) {
+    const int j = e / RANK, kk = e - j * RANK;
+    double2 sum = make_double2(0.0, 0.0);
+    for (int q = 0; q < 8; ++q) {
+      sum.x += buf[q][j][kk].x;
+      sum.y += buf[q][j][kk].y;
+    }
+    part[(((long long)b * rhs + r0 + j) * gridDim.x + blockIdx.x) * RANK + kk] = sum;
+  }
+}
+
 // corr[(b, r), :] = S^-1 (proj / bg) / bg, from the row-chunk partials:
 // thread (q, k) = (tid / 32, tid % 32) sums chunks q, q + 8, ... of atom k,
 // the eight partial sums are added in a fixed order, then thread k < 32
@@ -1728,6 +1793,46 @@ int fbme_gdiag_system(void* stream, int batch, int m1, int m2, const void* facto
                      m2, RANK, rb, (double2*)part);
   hipLaunchKernelGGL(k_gdiag_system, dim3(batch), dim3(64), 0, st, (const double2*)part, chunks, (float2*)cinv,
                      status);
+  return (int)hipGetLastError();
+}
+
+int fbme_stored_apply(void* stream, int lanes, int rhs, long long m, const void* ucols, const void* cinv,
+                      const float* ones, const void* v, void* out, void* part, void* corr, double* dot_part,
+                      double* dot) {
+  // part: lanes * ceil(m / 1024) * 32 double2; corr: lanes * 32 complex; dot_part: lanes * ceil(m / 256) doubles.
+  if (rhs < 1 || lanes % rhs != 0 || m < 1 || !ucols || !cinv || !ones || !v || !out || !part || !corr || v == out)
+    return FBME_ERR_ARG;
+  if ((dot_part == nullptr) != (dot == nullptr)) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int batch = lanes / rhs;
+  const int chunks = (int)((m + SP_CHUNK - 1) / SP_CHUNK);
+  const int groups = (rhs + 23) / 24;
+  const int per = (rhs + groups - 1) / groups;
+  const dim3 pg(chunks, groups, batch), ag((unsigned)nblk(m), groups, batch);
+  const float2* uc = (const float2*)ucols;
+#define FBME_SA_LAUNCH(LG)                                                                                       \
+  hipLaunchKernelGGL((k_stored_project<LG>), pg, dim3(FBME_BLOCK), 0, st, uc, (const float2*)v, m, rhs,          \
+                     (double2*)part);                                                                            \
+  hipLaunchKernelGGL(k_lr_corr, dim3(lanes), dim3(FBME_BLOCK), 0, st, (const double2*)part, chunks,              \
+                     (const float2*)cinv, ones, rhs, (float2*)corr, (const double*)nullptr, 0, (double*)nullptr); \
+  if (dot)                                                                                                       \
+    hipLaunchKernelGGL((k_lr_apply_stored<LG, true>), ag, dim3(FBME_BLOCK), 0, st, uc, (const float2*)v,          \
+                       (const float2*)v, (const float2*)corr, m, rhs, (float2*)out, dot_part);                    \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_lr_apply_stored<LG, false>), ag, dim3(FBME_BLOCK), 0, st, uc, (const float2*)v,         \
+                       (const float2*)v, (const float2*)corr, m, rhs, (float2*)out, (double*)nullptr)
+  if (per <= 8) {
+    FBME_SA_LAUNCH(8);
+  } else if (per <= 12) {
+    FBME_SA_LAUNCH(12);
+  } else if (per <= 17) {
+    FBME_SA_LAUNCH(17);
+  } else {
+    FBME_SA_LAUNCH(24);
+  }
+#undef FBME_SA_LAUNCH
+  if (dot)
+    hipLaunchKernelGGL(k_lane_finish, dim3(lanes), dim3(64), 0, st, (const double*)dot_part, (int)ag.x, lanes, dot);
   return (int)hipGetLastError();
 }
 

@@ -321,6 +321,15 @@ _SIGNATURES = {
                                               c_float, c_float, c_int,
                                               c_float, c_void_p]),
     "fbme_fixed_noise": (c_int, [c_void_p, c_int, c_float, c_void_p, c_void_p, c_int]),
+    # Nystrom preconditioner and the stored-column apply
+    "fbme_stored_apply": (c_int, [c_void_p, c_int, c_int, c_longlong, c_void_p, c_void_p, c_void_p, c_void_p,
+                                  c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_nystrom_part_bytes": (c_size_t, [c_int]),
+    "fbme_nystrom_omega": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                   c_void_p]),
+    "fbme_nystrom_setup": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_nystrom_vectors": (c_int, [c_void_p, c_int, c_longlong, c_int, c_void_p, c_void_p, c_void_p]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -6,7 +6,8 @@ schedule, a fixed noise variance, warm start,
 active-set switch for a single scene, convergence checks); the arithmetic is
 in libfbme_hip.so (fbme_pcg.hip): BTTB spectrum and Toeplitz matvec with the
 1 + P right-hand sides batched into the FFT leading dimension, low-rank
-Woodbury preconditioner, masked PCG lanes with FP64 inner products,
+Woodbury preconditioner (or the Nystrom / Strang circulant ones), masked
+PCG lanes with FP64 inner products,
 Hutchinson diagonal, relevance projection, MacKay and noise updates.
 
 Like the reference, this path synchronises with the host at PCG
@@ -32,8 +33,12 @@ Differences from the reference, all documented:
     or the grid transforms on 256-point grids): results agree to rounding;
   * the low-rank preconditioner keeps 32 selected atoms and weights those
     past cg_preconditioner_rank with 0 (ranks above 32 raise);
-  * the 'nystrom' preconditioner, adaptive probe counts and spatial
-    coupling raise NotImplementedError.
+  * the 'nystrom' preconditioner forms its eigenpairs from r x r Grams of
+    the test matrix and Y = A Gamma A^H Omega (FP64, complex Jacobi) instead
+    of a QR and an SVD of M x r matrices: the same eigenpairs (the Nystrom
+    approximation depends on the span of the test matrix only); unmasked
+    samples only;
+  * adaptive probe counts and spatial coupling raise NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -140,9 +145,9 @@ def _schedule(it: int, cfg: SBLConfig) -> float:
 
 def _check_supported(cfg: SBLConfig) -> None:
     bad = []
-    if cfg.cg_preconditioner not in ("low_rank", "circulant", "none"):
+    if cfg.cg_preconditioner not in ("low_rank", "nystrom", "circulant", "none"):
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
-    if cfg.cg_preconditioner == "low_rank" and cfg.cg_preconditioner_rank > RANK:
+    if cfg.cg_preconditioner in ("low_rank", "nystrom") and cfg.cg_preconditioner_rank > RANK:
         bad.append("cg_preconditioner_rank > 32")
     if cfg.adaptive_probes:
         bad.append("adaptive_probes")
@@ -185,6 +190,28 @@ def _hadamard_signs(g: torch.Generator, b: int, shape: tuple, p: int) -> torch.T
     gray = (j % block) ^ ((j % block) >> 1)
     had = 1 - 2 * (np.bitwise_count(code[:, None] & gray[None, :]) & 1).astype(np.int64)
     return signs[:, :, torch.from_numpy(j // block)] * torch.from_numpy(had)[None]
+
+
+_NYSTROM = {}
+
+
+def _nystrom_vectors(seed: int, b: int, m: int, count: int, dev) -> torch.Tensor:
+    """The Gaussian test vectors of the Nystrom preconditioner [b, count, m]
+    (complex64, device): torch.randn real parts, then imaginary parts, of
+    shape [b, m, count] from torch's CPU generator seeded with `seed`, as
+    the reference draws them on CPU (solver.py:5880-5902); cached."""
+    key = (int(seed), b, m, count, str(dev))
+    t = _NYSTROM.get(key)
+    if t is None:
+        g = torch.Generator(device="cpu")
+        g.manual_seed(seed)
+        re = torch.randn(b, m, count, generator=g, dtype=torch.float32)
+        im = torch.randn(b, m, count, generator=g, dtype=torch.float32)
+        t = torch.complex(re, im).permute(0, 2, 1).contiguous().to(dev)
+        if len(_NYSTROM) >= 8:
+            _NYSTROM.pop(next(iter(_NYSTROM)))
+        _NYSTROM[key] = t
+    return t
 
 
 def _probes(seed: int, b: int, m: int, p: int, dev, kind: str = "rademacher",
@@ -288,6 +315,8 @@ class _Solver:
             self.precond = precond
         elif precond_net is not None:
             raise ValueError("precond_net needs precond='grid_diagonal'")
+        if self.precond == "nystrom" and observation_mask is not None and self.m_obs < self.m:
+            raise NotImplementedError("the Nystrom preconditioner takes unmasked samples")
         if self.precond == "circulant" and 0xFFFFFFFF in self.t.sample_codes.values():
             raise NotImplementedError("the circulant preconditioner needs M1, M2 that factor into 2, 3 and 5")
         self.lr_fft = self.grid[0] == 256 and self.samples[0] <= 256 and self.precond == "low_rank"
@@ -395,6 +424,8 @@ class _Solver:
         self.rr_part = torch.empty(self.lanes * self.rr_blocks, dtype=f64, device=dev)
         if self.precond == "grid_diagonal":
             self._gdiag_buffers()
+        if self.precond == "nystrom":
+            self._nystrom_buffers()
         self.circ_eig = self.circ_work = None
         if self.precond == "circulant":
             self.circ_eig = torch.empty((ns, m), dtype=f32, device=dev)
@@ -470,6 +501,72 @@ class _Solver:
             self.gd_spectrum = torch.empty((ns, t.e1, t.e2), dtype=c64, device=dev)
             self.gd_work_a = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
             self.gd_work_b = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
+
+    def _nystrom_buffers(self):
+        """Buffers of the Nystrom preconditioner (solver.py:5851-5976): the
+        test matrix, Y = A Gamma A^H Omega and the eigenvectors U as 32 lanes
+        per system, the r x r outputs, the zero-noise covariance matvec's
+        spectrum / work and the projection partials of the apply."""
+        dev, ns, m = self.dev, self.nsys, self.m
+        c64, f32 = torch.complex64, torch.float32
+        m1, m2 = self.samples
+        self.ny_rank = min(int(self.cfg.cg_preconditioner_rank), self.m_obs)
+        self.ny_ri = min(self.ny_rank, max(1, self.ny_rank // 2))
+        self.ny_rand = None
+        if self.ny_rank > self.ny_ri:
+            self.ny_rand = _nystrom_vectors(self.cfg.seed, ns, m, self.ny_rank - self.ny_ri, dev)
+        self.ny_omega = torch.empty((ns, RANK, m), dtype=c64, device=dev)
+        self.ny_y = torch.empty((ns, RANK, m), dtype=c64, device=dev)
+        self.ny_u = torch.empty((ns, RANK, m), dtype=c64, device=dev)
+        self.ny_part = torch.empty(int(self.lib.fbme_nystrom_part_bytes(ns)), dtype=torch.uint8, device=dev)
+        self.ny_z = torch.empty((ns, RANK, RANK), dtype=c64, device=dev)
+        self.ny_cinv = torch.empty((ns, RANK, RANK), dtype=c64, device=dev)
+        self.ny_lam = torch.empty((ns, RANK), dtype=f32, device=dev)
+        self.ny_bg = torch.empty(ns, dtype=f32, device=dev)
+        self.ny_ones = torch.ones(ns, dtype=f32, device=dev)
+        self.ny_zero = torch.zeros(ns, dtype=f32, device=dev)
+        self.ny_proj = torch.empty(self.lanes * ((m + 1023) // 1024) * RANK * 2, dtype=torch.float64, device=dev)
+        big = max(self.lanes, ns * RANK)
+        if self.grid_op:
+            self.ny_gmid = torch.empty(big * 256 * m2, dtype=c64, device=dev)
+        else:
+            t = self.t
+            self.ny_spectrum = torch.empty((ns, t.e1, t.e2), dtype=c64, device=dev)
+            self.ny_work_a = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
+            self.ny_work_b = torch.empty(big * m1 * t.e2, dtype=c64, device=dev)
+
+    def _nystrom_setup(self):
+        """Test matrix, Y = A Gamma A^H Omega (the matvec passes with a zero
+        noise), then the eigenpairs and U = Y Z (solver.py:5851-5956)."""
+        t = self.t
+        n1, n2 = self.grid
+        m1, m2 = self.samples
+        r = self.ny_rank
+        self._chk(self.lib.fbme_nystrom_omega(self.stream, self.nsys, n1, n2, m1, m2, r, _ptr(self.top_idx),
+                                              _ptr(self.ny_rand), _ptr(self.ny_omega)), "nystrom_omega")
+        lanes = self.nsys * RANK
+        if self.grid_op:
+            self._chk(self.lib.fbme_grid_matvec_dot(
+                self.stream, lanes, RANK, m1, m2, _ptr(self.gamma), _ptr(self.ny_zero), _ptr(self.ny_omega),
+                _ptr(self.ny_y), _ptr(self.ny_gmid), _ptr(t.tw[256]), None, None), "nystrom_matvec")
+        else:
+            self._chk(self.lib.fbme_gamma_spectrum(
+                self.stream, self.nsys, n1, n2, m1, m2, t.e1, t.e2, t.codes[n1], t.codes[n2], t.codes[t.e1],
+                t.codes[t.e2], _ptr(self.gamma), _ptr(self.ny_zero), _ptr(self.spec_a), _ptr(self.spec_b),
+                _ptr(self.ny_spectrum), _ptr(t.tw[n1]), _ptr(t.tw[n2]), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2])),
+                "nystrom_spectrum")
+            self._chk(self.lib.fbme_toeplitz_matvec_tiled(
+                self.stream, lanes, RANK, m1, m2, t.e1, t.e2, t.codes[t.e1], t.codes[t.e2],
+                _ptr(self.ny_spectrum), _ptr(self.ny_omega), _ptr(self.ny_y), _ptr(self.ny_work_a),
+                _ptr(self.ny_work_b), _ptr(t.tw[t.e1]), _ptr(t.tw[t.e2]), self.tiles[0], self.tiles[1]),
+                "nystrom_toeplitz")
+        gsum = self.sums[:self.nsys, 0].contiguous()
+        self._chk(self.lib.fbme_nystrom_setup(
+            self.stream, self.nsys, self.m, r, _ptr(self.ny_omega), _ptr(self.ny_y), _ptr(gsum), _ptr(self.noise),
+            _ptr(self.ny_part), _ptr(self.ny_z), _ptr(self.ny_cinv), _ptr(self.ny_lam), _ptr(self.ny_bg)),
+            "nystrom_setup")
+        self._chk(self.lib.fbme_nystrom_vectors(self.stream, self.nsys, self.m, r, _ptr(self.ny_y), _ptr(self.ny_z),
+                                                _ptr(self.ny_u)), "nystrom_vectors")
 
     def _bmatvec(self, v, out, lanes, rhs):
         """out = B^-1 v = A diag(w) A^H v: the matvec passes with
@@ -580,6 +677,13 @@ class _Solver:
         self._precondition(v, out, dot)
 
     def _precondition(self, v, out, dot=None):
+        if self.precond == "nystrom" and self.ny_rank > 0:
+            # v + U ((w - 1) o U^H v) (solver.py:5959-5976)
+            self._chk(self.lib.fbme_stored_apply(
+                self.stream, self.lanes, self.r, self.m, _ptr(self.ny_u), _ptr(self.ny_cinv), _ptr(self.ny_ones),
+                _ptr(v), _ptr(out), _ptr(self.ny_proj), _ptr(self.corr),
+                _ptr(self.pc_part) if dot is not None else None, _ptr(dot)), "nystrom_apply")
+            return
         if self.precond == "grid_diagonal":
             # z = (B + F F^H)^-1 v = b - U C^-1 F^H b, b = B^-1 v, U = B^-1 F
             m1, m2 = self.samples
@@ -659,10 +763,15 @@ class _Solver:
                 self.stream, self.nsys, n1, n2, m1, m2, t.codes[n2], t.sample_codes[m2], _ptr(self.gamma),
                 _ptr(self.noise), _ptr(self.spec_a), _ptr(self.spec_b), _ptr(self.circ_eig), _ptr(t.tw[n2]),
                 _ptr(t.tw[m2])), "circulant_spectrum")
-        if self.precond not in ("low_rank", "grid_diagonal"):
+        if self.precond not in ("low_rank", "grid_diagonal", "nystrom"):
+            return
+        if self.precond == "nystrom" and self.ny_rank == 0:
             return
         self._chk(self.lib.fbme_topk32(self.stream, self.nsys, self.n, _ptr(self.gamma), _ptr(self.cand),
                                        _ptr(self.top_idx), _ptr(self.top_val)), "topk32")
+        if self.precond == "nystrom":
+            self._nystrom_setup()
+            return
         if self.lr_rank < RANK:
             self.top_val[:, self.lr_rank:].zero_()
         gsum = self.sums[:, 0].contiguous()

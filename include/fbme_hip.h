@@ -375,6 +375,40 @@ int fbme_lowrank_apply_dot(void* stream, int lanes, int rhs, int m1, int m2, con
 int fbme_lowrank_apply_split(void* stream, int lanes, int rhs, int m1, int m2, const void* factors,
                              const void* sinv, const float* bg, const void* vproj, const void* vbase, void* out,
                              void* part, void* corr);
+/* out = v - U S^-1 U^H v with U [batch, 32, M] complex64 stored columns
+ * (lanes of `rhs` per scene share them), S^-1 [batch, 32, 32] complex64 and
+ * ones: batch floats 1.0 (the Woodbury apply of fbme_lowrank_apply_split
+ * with stored columns).  With dot, dot[lane] = Re <v, out> (FP64).  part:
+ * lanes * ceil(M / 1024) * 32 double2; corr: lanes * 32 complex64; dot_part:
+ * lanes * ceil(M / 256) doubles.  v != out. */
+int fbme_stored_apply(void* stream, int lanes, int rhs, long long m, const void* ucols, const void* cinv,
+                      const float* ones, const void* v, void* out, void* part, void* corr, double* dot_part,
+                      double* dot);
+/* Randomised Nystrom preconditioner (cg_preconditioner="nystrom",
+ * _make_nystrom_preconditioner / _apply_nystrom_preconditioner,
+ * solver.py:5851-5976), rank r <= 32, ri = max(1, r / 2) informed atoms.
+ * fbme_nystrom_omega: the test matrix Omega [batch, 32, M1 M2] complex64:
+ * columns k < ri the atoms idx[b, k] (fbme_topk32 order), ri <= k < r the
+ * Gaussian vectors rnd [batch, r - ri, M] (NULL when r = ri), the rest 0.
+ * The caller then forms Y = A Gamma A^H Omega (the matvec with a zero
+ * noise).  fbme_nystrom_setup: from the r x r Grams of Omega and Y (FP64;
+ * part: fbme_nystrom_part_bytes(batch)) the eigenpairs of the Nystrom
+ * approximation Y (Omega^H Y)^-1 Y^H (orthonormalised test matrix,
+ * 10 eps_f32 mean|diag| Cholesky loading, complex Jacobi): z [batch, 32, 32]
+ * complex64 (U = Y z), cinv = diag(1 - w) [batch, 32, 32] complex64 with
+ * w = (lambda_min + bg) / (bg + lambda), lambda [batch, 32] (descending,
+ * zero past r), bg [batch] = max(noise + max(gamma_sum - sum lambda / M, 0),
+ * tiny).  A failed factorisation leaves z = cinv = 0 (the apply is then the
+ * identity, the reference's scalar fallback up to a scale PCG ignores).
+ * fbme_nystrom_vectors: U = Y z [batch, 32, M] for fbme_stored_apply with
+ * cinv, whose out = v + U ((w - 1) o U^H v) is the reference's apply. */
+size_t fbme_nystrom_part_bytes(int batch);
+int fbme_nystrom_omega(void* stream, int batch, int n1, int n2, int m1, int m2, int rank, const int* idx,
+                       const void* rnd, void* omega);
+int fbme_nystrom_setup(void* stream, int batch, long long m, int rank, const void* omega, const void* y,
+                       const double* gamma_sum, const float* noise, void* part, void* z, void* cinv, float* lambda,
+                       float* bg);
+int fbme_nystrom_vectors(void* stream, int batch, long long m, int rank, const void* y, const void* z, void* u);
 /* Grid-diagonal background of the low-rank preconditioner (no counterpart in
  * the reference, whose background is the scalar noise + omitted variance of
  * solver.py:5813-5819): B^-1 = A diag(w) A^H, Hermitian positive definite for

@@ -295,11 +295,66 @@ def grid_diagonal_preconditioner(gamma, noise, op: Operator, rank=32, logscale=N
     return ("grid_diagonal", binv, u, ci)
 
 
-def make_preconditioner(kind, gamma, noise, op: Operator, rank, eps, tiny):
+def nystrom_random(batch: int, m: int, count: int, seed: int) -> np.ndarray:
+    """The Gaussian test vectors of the Nystrom preconditioner: torch.randn
+    real parts, then imaginary parts, [batch, m, count] from torch's CPU
+    generator seeded with `seed` (solver.py:5880-5902)."""
+    import torch
+    g = torch.Generator(device="cpu")
+    g.manual_seed(seed)
+    re = torch.randn(batch, m, count, generator=g, dtype=torch.float32).numpy()
+    im = torch.randn(batch, m, count, generator=g, dtype=torch.float32).numpy()
+    return re.astype(np.float64) + 1j * im.astype(np.float64)
+
+
+def nystrom_preconditioner(gamma, noise, op: Operator, rank, seed, eps, tiny):
+    """Randomised Nystrom approximation of A Gamma A^H (solver.py:5851-5956):
+    test matrix = the rank // 2 strongest atoms' columns and rank - rank // 2
+    Gaussian vectors, orthonormalised (QR); Y = A Gamma A^H Q, core = Q^H Y
+    (symmetrised, jitter-loaded Cholesky L), Y L^-H = U S V^H (SVD),
+    eigenvalues S^2; the background is the noise plus the trace the
+    approximation misses per sample.  Returns ("nystrom", bg, U, lambda)."""
+    b = gamma.shape[0]
+    rank = min(rank, op.m)
+    total = gamma.sum(1)
+    empty = ("nystrom", noise + total, np.zeros((b, op.m, 0), np.complex128), np.zeros((b, 0)))
+    if rank == 0:
+        return empty
+    informed = min(rank, max(1, rank // 2))
+    order = np.argsort(-gamma, axis=1, kind="stable")[:, :informed]
+    omega = op.columns(order)
+    if rank > informed:
+        omega = np.concatenate([omega, nystrom_random(b, op.m, rank - informed, seed)], axis=2)
+    q = np.linalg.qr(omega)[0]
+    y = op.fwd(gamma[:, :, None] * op.adj(q))
+    core = np.conj(np.swapaxes(q, 1, 2)) @ y
+    core = 0.5 * (core + np.conj(np.swapaxes(core, 1, 2)))
+    scale = np.maximum(np.mean(np.abs(np.real(np.diagonal(core, axis1=1, axis2=2))), 1), tiny)
+    chol = None
+    for mult in (10.0, 1e2, 1e3, 1e4, 1e5):
+        try:
+            chol = np.linalg.cholesky(core + (mult * eps * scale)[:, None, None] * np.eye(rank))
+            break
+        except np.linalg.LinAlgError:
+            chol = None
+    if chol is None:
+        return empty
+    factor = np.conj(np.swapaxes(np.linalg.solve(chol, np.conj(np.swapaxes(y, 1, 2))), 1, 2))
+    u, s, _ = np.linalg.svd(factor, full_matrices=False)
+    lam = s ** 2
+    resid = np.maximum(total - lam.sum(1) / op.m, 0.0)
+    bg = np.maximum(noise + resid, tiny)
+    return ("nystrom", bg, u, lam)
+
+
+def make_preconditioner(kind, gamma, noise, op: Operator, rank, eps, tiny, seed=0):
     """_make_pcg_preconditioner (solver.py:5726-5762) for 'low_rank',
-    'circulant' and 'none'; 'grid_diagonal' is grid_diagonal_preconditioner."""
+    'nystrom', 'circulant' and 'none'; 'grid_diagonal' is
+    grid_diagonal_preconditioner."""
     if kind == "none":
         return ("none",)
+    if kind == "nystrom":
+        return nystrom_preconditioner(gamma, noise, op, rank, seed, eps, tiny)
     if kind == "grid_diagonal":
         return grid_diagonal_preconditioner(gamma, noise, op, rank)
     if kind == "circulant":
@@ -313,6 +368,14 @@ def apply_preconditioner(v, pre):
     kind = pre[0] if isinstance(pre[0], str) else "low_rank"
     if kind == "none":
         return v.copy()
+    if kind == "nystrom":
+        # Frangella-Tropp-Udell flattening (solver.py:5959-5976)
+        _, bg, u, lam = pre
+        if u.shape[2] == 0:
+            return v / bg[:, None, None]
+        proj = np.conj(np.swapaxes(u, 1, 2)) @ v
+        w = (lam[:, -1:] + bg[:, None]) / (bg[:, None] + lam)
+        return v + u @ ((w - 1.0)[:, :, None] * proj)
     if kind == "grid_diagonal":
         _, binv, u, ci = pre
         proj = np.conj(np.swapaxes(u, 1, 2)) @ v
@@ -332,7 +395,7 @@ def apply_preconditioner(v, pre):
 
 
 def pcg(rhs, gamma, noise, op: Operator, *, x0, tol, max_iter, check, rank,
-        fixed=0, eps=F32_EPS, tiny=F32_TINY, precond="low_rank"):
+        fixed=0, eps=F32_EPS, tiny=F32_TINY, precond="low_rank", seed=0):
     """Batched complex PCG with lane masks and true-residual restarts
     (solver.py:5582-5723)."""
     if x0 is None:
@@ -341,7 +404,7 @@ def pcg(rhs, gamma, noise, op: Operator, *, x0, tol, max_iter, check, rank,
     else:
         x = x0.copy()
         r = rhs - op.q(x, gamma, noise)
-    pre = make_preconditioner(precond, gamma, noise, op, rank, eps, tiny)
+    pre = make_preconditioner(precond, gamma, noise, op, rank, eps, tiny, seed)
     z = apply_preconditioner(r, pre)
     d = z.copy()
     rho = np.real(np.sum(np.conj(r) * z, axis=1))
@@ -447,7 +510,7 @@ def _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg, tol, eps, 
     sol, nit = pcg(rhs, gamma[:1], noise[:1], op, x0=sol if cfg.cg_warm_start else None,
                    tol=tol, max_iter=cfg.cg_max_iter, check=cfg.cg_check_interval,
                    rank=cfg.cg_preconditioner_rank, fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny,
-                   precond=cfg.cg_preconditioner)
+                   precond=cfg.cg_preconditioner, seed=cfg.seed)
     lifted = op.adj(sol)                                        # [1, N, B + P]
     dd1 = np.maximum(np.mean(np.real(lifted[:, :, b:] * np.conj(lprobes)), 2), 0.0)   # [1, N]
     mean = gamma[:1] * lifted[0, :, :b].T                       # [B, N]

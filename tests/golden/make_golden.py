@@ -406,6 +406,13 @@ OPTION_CASES = [
      dict(cg_preconditioner_rank=8, max_iter=25, return_history=True)),
     ("options_cg_rank0_32x32_24x24", "uniform", (1111,), (32, 32), (24, 24), 8, 20.0, "cg",
      dict(cg_preconditioner_rank=0, max_iter=20, return_history=True)),
+    # the randomised Nystrom preconditioner (rank 32 and 12; shared joint noise)
+    ("options_cg_nystrom_32x32_24x24", "uniform", (1112,), (32, 32), (24, 24), 8, 20.0, "cg",
+     dict(cg_preconditioner="nystrom", max_iter=30, return_history=True)),
+    ("options_cg_nystrom_rank12_64x64_40x40", "uniform", (1117,), (64, 64), (40, 40), 20, 15.0, "cg",
+     dict(cg_preconditioner="nystrom", cg_preconditioner_rank=12, max_iter=20, return_history=True)),
+    ("options_cg_joint_nystrom_32x32_24x24_b3", "uniform", (1114, 1115, 1116), (32, 32), (24, 24), 8, 20.0,
+     "cg", dict(cg_preconditioner="nystrom", joint_sparsity=True, max_iter=15, return_history=True)),
 ]
 
 

@@ -197,8 +197,10 @@ def test_sbl_cg_is_deterministic(device):
 def test_sbl_cg_unsupported_options_raise(device):
     from torch_sbl import SBLConfig, sbl_2d
     y = torch.zeros((24, 24), dtype=torch.complex64, device=device)
-    with pytest.raises(NotImplementedError):
-        sbl_2d(y, (32, 32), config=SBLConfig(mode="cg", device=str(device), cg_preconditioner="nystrom"))
+    for over in (dict(adaptive_probes=True), dict(cg_preconditioner_rank=64),
+                 dict(cg_preconditioner="nystrom", cg_preconditioner_rank=48)):
+        with pytest.raises(NotImplementedError):
+            sbl_2d(y, (32, 32), config=SBLConfig(mode="cg", device=str(device), **over))
 
 
 @pytest.mark.parametrize("grid,samples", [((64, 48), (45, 40)), ((256, 256), (160, 160))])

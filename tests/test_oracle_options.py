@@ -73,6 +73,43 @@ def test_hadamard_probes_equal_oracle(shape, p):
     assert set(np.unique(got)) <= {-1, 1}
 
 
+def test_full_rank_nystrom_is_the_exact_inverse_up_to_scale():
+    """Rank M: the Nystrom approximation is A Gamma A^H itself, the missed
+    trace is 0 (tr A Gamma A^H = M sum gamma), so bg = noise and the
+    flattened preconditioner is (lambda_min + noise) Q^-1."""
+    rng = np.random.default_rng(4)
+    op = co.Operator((8, 8), (6, 4))
+    gamma = rng.gamma(0.5, 1.0, (2, 64))
+    noise = np.array([0.3, 0.05])
+    pre = co.nystrom_preconditioner(gamma, noise, op, 24, 5, 1e-12, 1e-30)
+    _, bg, u, lam = pre
+    np.testing.assert_allclose(bg, noise, rtol=1e-7)     # the 1e-11 Cholesky loading moves the trace
+    v = rng.standard_normal((2, 24, 3)) + 1j * rng.standard_normal((2, 24, 3))
+    got = co.apply_preconditioner(v, pre)
+    for b in range(2):
+        q = op.q(np.eye(24, dtype=complex)[None].repeat(1, 0), gamma[b:b + 1], noise[b:b + 1])[0]
+        ref = (lam[b, -1] + noise[b]) * np.linalg.solve(q, v[b])
+        np.testing.assert_allclose(got[b], ref, rtol=1e-6, atol=1e-8 * np.abs(ref).max())
+
+
+def test_nystrom_slips_fail():
+    """The informed half taken from the weakest atoms, or the missed trace
+    dropped from the background, changes the reference's PCG counts."""
+    g, grid, over = og.load("options_cg_nystrom_rank12_64x64_40x40")
+    good = co.sbl_cg(g["y"][0], grid, og.cg_config(over))
+    assert list(map(int, good["cg_iterations"])) == g["history_cg_iterations"].astype(int).tolist()
+    keep = co.nystrom_preconditioner
+    try:
+        def no_trace(gamma, noise, op, rank, seed, eps, tiny):
+            kind, bg, u, lam = keep(gamma, noise, op, rank, seed, eps, tiny)
+            return kind, np.maximum(noise, tiny), u, lam
+        co.nystrom_preconditioner = no_trace
+        bad = co.sbl_cg(g["y"][0], grid, og.cg_config(over))
+    finally:
+        co.nystrom_preconditioner = keep
+    assert list(map(int, bad["cg_iterations"])) != g["history_cg_iterations"].astype(int).tolist()
+
+
 def test_full_hadamard_basis_gives_the_exact_diagonal():
     """P equal to the embedding: Z = D H, so Z Z^T / P = I and the
     Hutchinson estimate of any diag(M) is exact."""
