@@ -86,6 +86,9 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device copy (counter calibration)")
     ap.add_argument("--skip-uneven", action="store_true")
+    ap.add_argument("--uneven-only", action="store_true",
+                    help="time only the 64-scene uneven workload (tuned shapes) and print its JSON: the "
+                         "command its kernel profile is taken with")
     ap.add_argument("--skip-pcg", action="store_true")
     ap.add_argument("--pcg-batch", type=int, default=32)
     ap.add_argument("--skip-learned", action="store_true")
@@ -236,7 +239,7 @@ def pcg_kernels(device, batch=8, reps=20):
     return out
 
 
-def uneven_workload(device, reps=20):
+def uneven_workload(device, reps=20, variants=("tuned", "default")):
     """64 scenes, grids from {64, 128, 192, 256}^2, per-axis sampling ratios
     in [0.3, 0.8], K in [0.5 %, 3 %] of the cells, SNR in [5, 25] dB: one
     graph-captured plan, pass shapes from the autotuner's table against the
@@ -252,6 +255,8 @@ def uneven_workload(device, reps=20):
     ys = [torch.from_numpy(s.y).to(device) for s in batch]
     out = {"workload": "64 uneven scenes (grids 64^2..256^2, ratios 0.3-0.8), FBME-Fast, one plan + hipGraph"}
     for name, opt in (("tuned", None), ("default", _lib.default_options(False))):
+        if name not in variants:
+            continue
         plan = UnevenPlan(shapes, device=device, options=opt)
         plan.replay(ys)
         for _ in range(3):
@@ -580,6 +585,9 @@ def main():
     from torch_sbl_fastest_stable import make_fastest_plan, make_stable_plan
 
     device = torch.device("cuda", 0)
+    if args.uneven_only:
+        print(json.dumps({"uneven_workload": uneven_workload(device, args.steps, ("tuned",))}), flush=True)
+        return
     grid, samples = (256, 256), (160, 160)
     n, m = grid[0] * grid[1], samples[0] * samples[1]
     b = args.batch
