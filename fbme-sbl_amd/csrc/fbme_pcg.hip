@@ -1126,7 +1126,7 @@ __global__ void k_pcg_check(const double* __restrict__ rr, const double* __restr
 // v = max(gamma - gamma^2 dd, 0); partial sums of clamp(gamma dd, 0, 1).
 __global__ void __launch_bounds__(FBME_BLOCK)
 k_hutch(const float2* __restrict__ lifted, const float2* __restrict__ lprobes, const float* __restrict__ gamma,
-        int n, int probes, float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd_out,
+        int n, int probes, int used, float2* __restrict__ mean, float* __restrict__ var, float* __restrict__ dd_out,
         double* __restrict__ part) {
   __shared__ double red[8];
   const int b = blockIdx.y;
@@ -1135,12 +1135,12 @@ k_hutch(const float2* __restrict__ lifted, const float2* __restrict__ lprobes, c
   if (i < n) {
     const int rhs = probes + 1;
     double acc = 0.0;
-    for (int p = 0; p < probes; ++p) {
+    for (int p = 0; p < used; ++p) {
       const float2 u = lifted[((long long)b * rhs + 1 + p) * n + i];
       const float2 w = lprobes[((long long)b * probes + p) * n + i];
       acc += (double)u.x * w.x + (double)u.y * w.y;
     }
-    const float dd = fmaxf((float)(acc / probes), 0.f);
+    const float dd = fmaxf((float)(acc / used), 0.f);
     const float g = gamma[(long long)b * n + i];
     const float2 l0 = lifted[((long long)b * rhs) * n + i];
     mean[(long long)b * n + i] = make_float2(g * l0.x, g * l0.y);
@@ -1985,12 +1985,25 @@ int fbme_pcg_check(void* stream, int lanes, const double* rr, const double* init
   return (int)hipGetLastError();
 }
 
+int fbme_hutchinson_used(void* stream, int batch, int n, int probes, int used, const void* lifted,
+                         const void* lprobes, const float* gamma, void* mean, float* var, float* dd, double* part,
+                         double* rel_sum) {
+  if (batch < 1 || n < 1 || used < 1 || used > probes) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const int blocks = nblk(n);
+  hipLaunchKernelGGL(k_hutch, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, (const float2*)lifted,
+                     (const float2*)lprobes, gamma, n, probes, used, (float2*)mean, var, dd, part);
+  hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum,
+                     (float*)nullptr);
+  return (int)hipGetLastError();
+}
+
 int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
                     const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum) {
   hipStream_t st = (hipStream_t)stream;
   const int blocks = nblk(n);
   hipLaunchKernelGGL(k_hutch, dim3(blocks, batch), dim3(FBME_BLOCK), 0, st, (const float2*)lifted,
-                     (const float2*)lprobes, gamma, n, probes, (float2*)mean, var, dd, part);
+                     (const float2*)lprobes, gamma, n, probes, probes, (float2*)mean, var, dd, part);
   hipLaunchKernelGGL(k_seg_sum, dim3(batch), dim3(64), 0, st, (const double*)part, blocks, rel_sum,
                      (float*)nullptr);
   return (int)hipGetLastError();

@@ -2,7 +2,7 @@
 
 Control flow follows torch_sbl/solver.py:1194-1904 and 2023-2080 (probe
 count, Rademacher or randomised Hadamard probes, adaptive CG tolerance
-schedule, a fixed noise variance, warm start,
+and probe-count schedule, a fixed noise variance, warm start,
 active-set switch for a single scene, convergence checks); the arithmetic is
 in libfbme_hip.so (fbme_pcg.hip): BTTB spectrum and Toeplitz matvec with the
 1 + P right-hand sides batched into the FFT leading dimension, low-rank
@@ -38,7 +38,10 @@ Differences from the reference, all documented:
     of a QR and an SVD of M x r matrices: the same eigenpairs (the Nystrom
     approximation depends on the span of the test matrix only); unmasked
     samples only;
-  * adaptive probe counts and spatial coupling raise NotImplementedError.
+  * adaptive probe counts keep every drawn probe lane and give the unused
+    ones a zero right-hand side (PCG leaves them inactive; the passes still
+    carry them);
+  * spatial coupling raises NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -132,7 +135,7 @@ def _probe_count(n, m, cfg: SBLConfig) -> int:
 
 
 def _schedule(it: int, cfg: SBLConfig) -> float:
-    """solver.py:6086-6118 (non-adaptive probe count)."""
+    """The PCG tolerance of EM iteration `it` (solver.py:6086-6118)."""
     if not cfg.adaptive_cg:
         return cfg.cg_tolerance
     if it <= 5:
@@ -143,14 +146,26 @@ def _schedule(it: int, cfg: SBLConfig) -> float:
     return cfg.cg_tolerance
 
 
+def _probe_schedule(it: int, cfg: SBLConfig, maximum: int) -> int:
+    """Probes used at EM iteration `it` (solver.py:6086-6118): with
+    adaptive_probes, initial_hutchinson_probes for iterations 1-5, at least
+    min(24, maximum) for 6-15, then all `maximum` drawn."""
+    if not cfg.adaptive_cg or not cfg.adaptive_probes:
+        return maximum
+    initial = min(cfg.initial_hutchinson_probes, maximum)
+    if it <= 5:
+        return initial
+    if it <= 15:
+        return min(maximum, max(initial, min(24, maximum)))
+    return maximum
+
+
 def _check_supported(cfg: SBLConfig) -> None:
     bad = []
     if cfg.cg_preconditioner not in ("low_rank", "nystrom", "circulant", "none"):
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
     if cfg.cg_preconditioner in ("low_rank", "nystrom") and cfg.cg_preconditioner_rank > RANK:
         bad.append("cg_preconditioner_rank > 32")
-    if cfg.adaptive_probes:
-        bad.append("adaptive_probes")
     if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
         bad.append("spatial coupling")
     if cfg.active_set and cfg.active_set_max_size > 256:
@@ -352,6 +367,8 @@ class _Solver:
         else:
             probes = _probes(cfg.seed, ns, self.m_obs, self.p, dev, cfg.probe_type,
                              self.samples if self.obs is None else None)
+        self.probe_src = probes
+        self.p_used = self.p
         if self.obs is None:
             self.rhs[:, nd:] = probes
         else:
@@ -968,11 +985,37 @@ class _Solver:
         self.setup_operators()
         iters = self.pcg(tol)
         self._adjoint(self.x.view(self.lanes, self.m), self.lifted.view(self.lanes, self.n))
-        hutch = self.lib.fbme_hutchinson_shared if self.shared else self.lib.fbme_hutchinson
-        self._chk(hutch(self.stream, self.b, self.n, self.p, _ptr(self.lifted), _ptr(self.lprobes), _ptr(self.gamma),
-                        _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.grid_part), _ptr(self.rel_sum)),
-                  "hutchinson")
+        if self.shared:
+            # probe lanes follow the B data lanes: the first `used` are a prefix
+            self._chk(self.lib.fbme_hutchinson_shared(
+                self.stream, self.b, self.n, self.p_used, _ptr(self.lifted), _ptr(self.lprobes), _ptr(self.gamma),
+                _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.grid_part), _ptr(self.rel_sum)),
+                "hutchinson")
+        else:
+            self._chk(self.lib.fbme_hutchinson_used(
+                self.stream, self.b, self.n, self.p, self.p_used, _ptr(self.lifted), _ptr(self.lprobes),
+                _ptr(self.gamma), _ptr(self.mean), _ptr(self.var), _ptr(self.dd), _ptr(self.grid_part),
+                _ptr(self.rel_sum)), "hutchinson")
         return iters
+
+    def use_probes(self, count: int):
+        """Solve with the first `count` probes of the drawn set (the adaptive
+        probe counts of _cg_schedule, solver.py:6086-6118): the other probe
+        lanes get a zero right-hand side (so PCG leaves them inactive) and a
+        zero start, as _resize_warm_start (solver.py:6121-6138) gives lanes
+        that are new to the solve."""
+        count = min(int(count), self.p)
+        if count == self.p_used or self.p == 0:
+            return
+        nd = self.r - self.p
+        if self.obs is None:
+            self.rhs[:, nd:nd + count] = self.probe_src[:, :count]
+        else:
+            self.rhs[:, nd:nd + count, self.obs_idx] = self.probe_src[:, :count]
+        self.rhs[:, nd + count:] = 0
+        self.x[:, nd + count:] = 0
+        self.lane_dot(self.rhs, self.rhs, self.rhs_rr)
+        self.p_used = count
 
     def em(self, update=True):
         cfg = self.cfg
@@ -1119,6 +1162,7 @@ def solve(s: "_Solver", batch_shape=(), warm_start=None, stall_check=None, captu
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             s.try_active_set()
         tol = max(_schedule(it, cfg), tol_floor)
+        s.use_probes(_probe_schedule(it, cfg, s.p))
         sum_before = s.sums[:, 0].cpu().numpy() if stall_check is not None else None
         iters = s.posterior(tol)
         s.em(update=True)
@@ -1136,7 +1180,7 @@ def solve(s: "_Solver", batch_shape=(), warm_start=None, stall_check=None, captu
             history["noise_variance"].append(float(s.noise.mean()))
             history["effective_dof"].append(float(s.dof.mean()))
             history["cg_iterations"].append(float(iters))
-            history["hutchinson_probes"].append(float(s.p))
+            history["hutchinson_probes"].append(float(s.p_used if s.support is None else 0))
             history["cg_tolerance"].append(float(tol))
             history["active_set_size"].append(float(0 if s.support is None else s.support.shape[1]))
             history["spatial_vertical_coupling"].append(float(cfg.spatial_coupling))

@@ -490,6 +490,12 @@ int fbme_masked_copy(void* stream, int lanes, int m, void* dst, const void* src,
  * part: batch * ceil(N / 256) doubles. */
 int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lifted, const void* lprobes,
                     const float* gamma, void* mean, float* var, float* dd, double* part, double* rel_sum);
+/* fbme_hutchinson averaging over the first `used` of the `probes` probe
+ * lanes (lifted keeps 1 + probes lanes per scene): the adaptive probe
+ * counts of _cg_schedule (solver.py:6086-6118).  1 <= used <= probes. */
+int fbme_hutchinson_used(void* stream, int batch, int n, int probes, int used, const void* lifted,
+                         const void* lprobes, const float* gamma, void* mean, float* var, float* dd, double* part,
+                         double* rel_sum);
 /* Relevance projection (_project_relevance, solver.py:3156-3181), MacKay
  * update (1669-1677) and damping with floor (1776-1798); mask (may be NULL)
  * keeps gamma outside an active set.  dof[b] = projected relevance sum. */

@@ -55,6 +55,8 @@ class CGConfig:
     initial_noise_fraction: float = 1e-2
     fixed_noise_variance: float | None = None
     probe_type: str = "rademacher"
+    adaptive_probes: bool = False
+    initial_hutchinson_probes: int = 16
     initialization_blend: float = 0.1
     noise_update: str = "em"
     damping: float = 0.15
@@ -150,6 +152,31 @@ def hadamard_probes(batch: int, shape, count: int, seed: int) -> np.ndarray:
     else:   # H[i, j] = (-1)^popcount(i & j), the entries of the same matrix
         had = 1.0 - 2.0 * (np.bitwise_count(codes[:, None] & np.array(cols)[None, :]) & 1)
     return signs[:, :, [j // block for j in range(count)]] * had[None]
+
+
+def probe_schedule(it: int, cfg: CGConfig, maximum: int) -> int:
+    """Probes used at EM iteration `it` (the probe half of _cg_schedule,
+    solver.py:6086-6118): with adaptive_probes, initial_hutchinson_probes
+    for iterations 1-5, at least min(24, maximum) for 6-15, then all."""
+    if not cfg.adaptive_cg or not cfg.adaptive_probes:
+        return maximum
+    initial = min(cfg.initial_hutchinson_probes, maximum)
+    if it <= 5:
+        return initial
+    if it <= 15:
+        return min(maximum, max(initial, min(24, maximum)))
+    return maximum
+
+
+def resize_warm_start(sol, lanes):
+    """_resize_warm_start (solver.py:6121-6138): keep the shared leading
+    lanes, zeros for new ones."""
+    if sol is None or sol.shape[2] == lanes:
+        return sol
+    out = np.zeros(sol.shape[:2] + (lanes,), sol.dtype)
+    k = min(lanes, sol.shape[2])
+    out[:, :, :k] = sol[:, :, :k]
+    return out
 
 
 def cg_schedule(it: int, cfg: CGConfig) -> float:
@@ -507,6 +534,7 @@ def _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg, tol, eps, 
     (_shared_mmv_posterior, solver.py:3039-3153)."""
     b = y.shape[0]
     rhs = np.concatenate([y.T[None], probes], axis=2)           # [1, M, B + P]
+    sol = resize_warm_start(sol, rhs.shape[2])
     sol, nit = pcg(rhs, gamma[:1], noise[:1], op, x0=sol if cfg.cg_warm_start else None,
                    tol=tol, max_iter=cfg.cg_max_iter, check=cfg.cg_check_interval,
                    rank=cfg.cg_preconditioner_rank, fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny,
@@ -588,24 +616,26 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             if sup is not None:
                 sol = None
         tol = max(cg_schedule(it, cfg), tol_floor)
+        used = probe_schedule(it, cfg, pcount)
         if sup is not None:
             mean, var, dd = active_posterior(y, gamma, noise, sup, op, eps, tiny)
             cg_iters.append(0)
             sol = None
         elif shared:
-            mean, var, dd, sol, nit = _posterior_shared(y, gamma, noise, probes, lprobes, op, sol, cfg,
-                                                        tol, eps, tiny)
+            mean, var, dd, sol, nit = _posterior_shared(y, gamma, noise, probes[:, :, :used],
+                                                        lprobes[:, :, :used], op, sol, cfg, tol, eps, tiny)
             cg_iters.append(nit)
         else:
-            rhs = np.concatenate([y[:, :, None], probes], axis=2)
+            rhs = np.concatenate([y[:, :, None], probes[:, :, :used]], axis=2)
+            sol = resize_warm_start(sol, rhs.shape[2])
             sol, nit = pcg(rhs, gamma, noise, op, x0=sol if cfg.cg_warm_start else None,
                            tol=tol, max_iter=cfg.cg_max_iter,
                            check=cfg.cg_check_interval,
                            rank=cfg.cg_preconditioner_rank, precond=cfg.cg_preconditioner,
-                           fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny)
+                           fixed=cfg.cg_fixed_iterations, eps=eps, tiny=tiny, seed=cfg.seed)
             cg_iters.append(nit)
             lifted = op.adj(sol)
-            dd = np.maximum(np.mean(np.real(lifted[:, :, 1:] * np.conj(lprobes)), 2), 0.0)
+            dd = np.maximum(np.mean(np.real(lifted[:, :, 1:] * np.conj(lprobes[:, :, :used])), 2), 0.0)
             mean = gamma * lifted[:, :, 0]
             var = np.maximum(gamma - gamma ** 2 * dd, 0.0)
         rel = project_relevance(gamma, dd, m, eps)

@@ -413,6 +413,12 @@ OPTION_CASES = [
      dict(cg_preconditioner="nystrom", cg_preconditioner_rank=12, max_iter=20, return_history=True)),
     ("options_cg_joint_nystrom_32x32_24x24_b3", "uniform", (1114, 1115, 1116), (32, 32), (24, 24), 8, 20.0,
      "cg", dict(cg_preconditioner="nystrom", joint_sparsity=True, max_iter=15, return_history=True)),
+    # adaptive probe counts: 16, then 24 (iterations 6-15), then all 48 / 32
+    ("options_cg_adaptive_probes_64x64_40x40", "uniform", (1118,), (64, 64), (40, 40), 20, 15.0, "cg",
+     dict(adaptive_probes=True, hutchinson_probes=48, auto_probe_scaling=False, max_iter=25,
+          return_history=True)),
+    ("options_cg_joint_adaptive_probes_32x32_24x24_b2", "uniform", (1119, 1120), (32, 32), (24, 24), 8, 20.0,
+     "cg", dict(adaptive_probes=True, joint_sparsity=True, max_iter=20, return_history=True)),
 ]
 
 
@@ -446,7 +452,7 @@ def options(reference_root: str) -> None:
         arrays = dict(y=y, x_true=np.stack([sc.x for sc in batch]), grid=np.array(grid), samples=np.array(samples),
                       entry=np.array(entry))
         arrays.update(_result_arrays(res))
-        for key in ("relative_change", "cg_iterations", "noise_variance"):
+        for key in ("relative_change", "cg_iterations", "noise_variance", "hutchinson_probes"):
             if res.history.get(key):
                 arrays["history_" + key] = np.array(res.history[key], np.float64)
         for key, value in over.items():

@@ -13,6 +13,8 @@ the reference's runs (tests/options_golden.py) and the float64 oracles:
 * low-rank preconditioners of rank 8 and 0, whose PCG counts differ from the
   reference's by whole convergence checks at a few EM iterations (bars
   below);
+* adaptive probe counts (16, 24, then all; the unused probe lanes carry a
+  zero right-hand side), the per-iteration counts as the reference's;
 * the randomised Nystrom preconditioner at ranks 32 and 12 (also with joint
   sparsity) against the reference's runs (x 4e-7 to 6e-6), and its apply
   against the float64 oracle at fixed variances on the BTTB and grid
@@ -101,6 +103,8 @@ def test_cg_options_vs_reference(device, case):
     nrel = float(np.max(np.abs(noise / g["noise_variance"].reshape(-1) - 1)))
     print(case, "iterations", res.iterations, int(g["iterations"]), "x vs reference", rel, "noise", nrel,
           "\n  PCG counts", hc.tolist(), "\n  reference ", rc.tolist(), flush=True)
+    if "history_hutchinson_probes" in g.files:
+        np.testing.assert_array_equal(res.history["hutchinson_probes"], g["history_hutchinson_probes"])
     slack, apart, xbar, nbar = SLACK.get(case, (0, 0, 1e-4, 1e-4))
     assert res.iterations == int(g["iterations"])
     assert res.active_set_size == int(g["active_set_size"])
