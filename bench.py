@@ -10,8 +10,11 @@ NMSE of both on the same scenes, the roofline of the dominant kernel (the
 fused row pass) timed with events on the stream it runs on, beside the
 bandwidth plain streaming kernels reach on the same GPU, the float64 CPU
 oracle timed on the host, a 64-scene uneven workload in one graph-captured
-plan (autotuned pass shapes against the defaults), and the learned warm start
-against the plain general solver (PCG iterations, scenes/s, NMSE).
+plan (autotuned pass shapes against the defaults), the Fast / Stable replay
+latency at the shape of the reference's published RTX 4090 example (40x32
+grid, 20x16 prefix) beside its published P50 / P95, and the learned warm
+start and preconditioner against the plain general solver (PCG iterations,
+scenes/s, NMSE).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
 """
@@ -236,6 +239,45 @@ def pcg_kernels(device, batch=8, reps=20):
                                "frac": nb / (ms / 1e3) / HBM_PEAK, "us_per_scene": ms * 1e3 / batch,
                                "kernels": ("grid matvec + update + preconditioner/direction pass (no check)" if s.fused
                                            else "matvec + update + preconditioner + direction (no check)")}
+    return out
+
+
+# The reference's only published timings: one 40x32 grid / 20x16 prefix
+# scene, 21 targets, 10 dB, on an RTX 4090 (archived summary.csv:8-9).
+PUBLISHED_SCENE = {"grid": (40, 32), "prefix": (20, 16), "k": 21, "snr_db": 10.0,
+                   "reference_rtx4090": {"fast": {"p50_ms": 0.6415, "p95_ms": 0.9513},
+                                         "stable": {"p50_ms": 1.0988, "p95_ms": 1.5503}}}
+
+
+def published_scene_latency(device):
+    """Single-scene replay latency of the Fast and Stable plans at the shape
+    of the reference's published example (synthetic scene of that shape and
+    target count: the archived scene's generator and arrays are not in the
+    reference), under the same protocol (resident input copied into the plan
+    by replay, 20 event-timed replays, P50 / P95)."""
+    import numpy as np
+    import torch
+
+    from fbme_amd import scenes
+    from torch_sbl_fastest_stable import make_fastest_plan, make_stable_plan
+    cfg = PUBLISHED_SCENE
+    sc = scenes.uniform_scene(200, cfg["grid"], cfg["prefix"], k=cfg["k"], snr_db=cfg["snr_db"])
+    data = torch.from_numpy(sc.y).to(device)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    out = {"workload": "one scene, grid 40x32, prefix 20x16, 21 uniform targets, 10 dB (the shape of the "
+                       "reference's published RTX 4090 example; synthetic data)",
+           "reference_rtx4090": cfg["reference_rtx4090"]}
+    for name, make in (("fast", make_fastest_plan), ("stable", make_stable_plan)):
+        plan = make(data, cfg["grid"], polish_size=32, warmup=2)
+        for _ in range(5):
+            plan.replay(data)
+        torch.cuda.synchronize()
+        lat = archive_latency(lambda: plan.replay(data), start, stop)
+        res = plan.replay(data, synchronize=True)
+        lat["nmse_db"] = scenes.nmse_db(res.x.cpu().numpy().reshape(-1), sc.x.reshape(-1))
+        lat["p50_speedup_vs_reference_rtx4090"] = cfg["reference_rtx4090"][name]["p50_ms"] / lat["p50_ms"]
+        out[name] = lat
+        del plan
     return out
 
 
@@ -753,6 +795,8 @@ def main():
             with open(ref_path) as fh:
                 cpu["reference_in_build_container"] = json.load(fh)
 
+    published = published_scene_latency(device)
+    print(f"[bench] published-example scene: {published}", file=sys.stderr, flush=True)
     uneven = None
     if not args.skip_uneven:
         print("[bench] uneven workload", file=sys.stderr, flush=True)
@@ -806,6 +850,7 @@ def main():
         "cpu_baseline": cpu,
         "batched_latency": batched_latency,
         "plain_single_scene": plain,
+        "published_example_scene": published,
         "speedup_vs_plain": (value / plain["scenes_per_s"]) if plain else None,
         "nmse_db_batched": nmse_batched,
         "per_kernel": per_kernel,
