@@ -245,8 +245,12 @@ class Operator:
 
 
 def low_rank_preconditioner(gamma, noise, op: Operator, rank, eps, tiny):
-    """Isotropic background plus the strongest atoms (solver.py:5786-5848)."""
+    """Isotropic background plus the strongest atoms (solver.py:5786-5848);
+    rank 0 is the scalar background noise + sum gamma (solver.py:5800-5810)."""
     rank = min(rank, op.m, op.n)
+    if rank == 0:
+        b = gamma.shape[0]
+        return noise + gamma.sum(1), np.zeros((b, op.m, 0), np.complex128), np.zeros((b, 0, 0), np.complex128)
     order = np.argsort(-gamma, axis=1, kind="stable")[:, :rank]
     vals = np.take_along_axis(gamma, order, 1)
     cols = op.columns(order)
