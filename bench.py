@@ -291,7 +291,7 @@ def uneven_workload(device, reps=20, variants=("tuned", "default")):
 
     from fbme_amd import _lib, scenes
     from fbme_amd.batched import UnevenPlan
-    from fbme_amd.bytes_model import survey_scene_bytes
+    from fbme_amd.bytes_model import stagewise_scene_bytes
     batch = scenes.uneven_batch(2024, count=64)
     shapes = [(s.grid[0], s.grid[1], s.samples[0], s.samples[1]) for s in batch]
     ys = [torch.from_numpy(s.y).to(device) for s in batch]
@@ -314,7 +314,7 @@ def uneven_workload(device, reps=20, variants=("tuned", "default")):
         nm = [scenes.nmse_db(plan.scene(i)["x"].reshape(-1).cpu().numpy(), s.x.reshape(-1))
               for i, s in enumerate(batch)]
         t_us = [plan.plan.scene_info[i][2] for i in range(len(batch))]
-        model = sum(survey_scene_bytes(a * b, c * d, t) for (a, b, c, d), t in zip(shapes, t_us))
+        model = sum(stagewise_scene_bytes(a * b, c * d, t) for (a, b, c, d), t in zip(shapes, t_us))
         out[name] = {"ms_per_replay": ms, "scenes_per_s": len(batch) / (ms / 1e3),
                      "nmse_db": float(np.mean(nm)), "model_GB_per_s": model / (ms / 1e3) / 1e9,
                      "options": {"fft_kernel": plan.options.fft_kernel, "row_lines": plan.options.row_lines,
@@ -623,7 +623,7 @@ def main():
     import torch
 
     from fbme_amd import _lib, scenes
-    from fbme_amd.bytes_model import kernel_bytes, survey_scene_bytes
+    from fbme_amd.bytes_model import kernel_bytes, stagewise_scene_bytes
     from torch_sbl_fastest_stable import make_fastest_plan, make_stable_plan
 
     device = torch.device("cuda", 0)
@@ -861,7 +861,7 @@ def main():
         "joint_sparsity": joint,
         "offgrid_refinement": offgrid,
         "dense_mode": dense,
-        "survey_model_bytes_per_scene": survey_scene_bytes(n, m, t_u,
+        "stagewise_model_bytes_per_scene": stagewise_scene_bytes(n, m, t_u,
                                                            args.profile == "stable"),
         "wall_s": wall,
     }

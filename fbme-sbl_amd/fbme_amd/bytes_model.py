@@ -2,7 +2,7 @@
 
 Two views are kept:
 
-* ``survey_scene_bytes`` -- the stage-wise model of the whole solve used to
+* ``stagewise_scene_bytes`` -- the stage-wise model of the whole solve used to
   compare against the reference: every stage reads its inputs once and
   writes its outputs once, FFT intermediates are free.  Per UAMP iteration
   64 M + 56 N bytes; Fast = (56 + 64 T_u) M + (93 + 56 T_u) N, Stable adds
@@ -17,7 +17,7 @@ Two views are kept:
 from __future__ import annotations
 
 
-def survey_scene_bytes(n: int, m: int, t_u: int, stable: bool = False) -> int:
+def stagewise_scene_bytes(n: int, m: int, t_u: int, stable: bool = False) -> int:
     fast = (56 + 64 * t_u) * m + (93 + 56 * t_u) * n
     return fast + (8 * m + 12 * n if stable else 0)
 

@@ -5,7 +5,7 @@ Public names follow torch_sbl/__init__.py:1-22 of the reference for the FBME
 hot path, including off-grid refinement (``refine_offgrid_fourier``, one HIP
 launch per batch: 1-D and 2-D grids, at most 64 candidates and 64 refined
 components).  The reference's autofocus and correlated-MMV research
-extensions are not part of this build.
+extensions are not implemented here.
 """
 
 import os as _os

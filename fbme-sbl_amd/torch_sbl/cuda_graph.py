@@ -239,7 +239,7 @@ class RealtimeSBLPlan(CUDAGraphSBLPlan):
         if config.uamp_backend != "triton":
             raise ValueError(
                 "RealtimeSBLPlan requires the fused-kernel UAMP backend "
-                "(uamp_backend='triton' selects it in this build too).")
+                "(uamp_backend='triton' selects it here too).")
 
 
 def make_cuda_graph_sbl_plan(example_data: Any, grid_shape: Sequence[int], *,

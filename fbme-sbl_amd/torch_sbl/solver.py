@@ -73,7 +73,7 @@ def sbl_2d(data: Any, grid_shape: Sequence[int], *,
                                or config.learn_snapshot_covariance):
         raise NotImplementedError(
             "autofocus and correlated-MMV reconstruction are outside the "
-            "FBME hot path of this build.")
+            "FBME hot path on MI355X.")
     return fast_sbl(data, shape, config=config,
                     observation_mask=observation_mask)
 
@@ -129,7 +129,7 @@ def fast_sbl(data: Any, grid_shape: Sequence[int], *,
     device = _resolve_device(cfg.device, tensor)
     if device.type != "cuda":
         raise RuntimeError(
-            "This build runs FBME on MI355X GPUs only; pass a GPU tensor or "
+            "FBME runs on MI355X GPUs only here; pass a GPU tensor or "
             "device (the CPU oracle lives in the test suite).")
     mask = None
     if observation_mask is not None:

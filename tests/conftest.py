@@ -22,7 +22,7 @@ def pytest_configure(config):
 def device():
     import torch
     if not torch.cuda.is_available():
-        pytest.fail("GPU tests need a GPU (run them with -m gpu on the GPU box)")
+        pytest.fail("GPU tests need a GPU (run them with -m gpu on an MI355X machine)")
     return torch.device("cuda", 0)
 
 

@@ -1,6 +1,6 @@
 """Write golden input/output vectors from the reference's CPU Torch backend.
 
-Run in the build container only (the reference never travels to the GPU box):
+Run where the reference is available only (it never runs on the GPU machine):
 
     python tests/golden/make_golden.py /root/reference
 

@@ -12,8 +12,9 @@ timings as a fixture (tests/golden/reference_cpu_timings.json):
     one scene per call and 8 scenes per call.
 
 torch.set_num_threads(os.cpu_count()), one warm-up, the median of five
-perf_counter runs.  The reference never travels to the GPU box; the
-benchmark reports these figures beside its own on-box CPU baseline.
+perf_counter runs.  The reference itself never runs on the GPU machine;
+the benchmark reports these figures beside its own CPU baseline, timed on
+the GPU machine's host.
 """
 
 from __future__ import annotations

@@ -117,7 +117,7 @@ def test_dense_joint_matches_reference(device, name):
     (solver.py:3039-3080), and the M-SBL update, against the reference's
     30-iteration runs.  Achieved: x within 2.1e-5 .. 3.4e-5, noise history
     within 5.2e-5 (shared) and 1.1e-4 (per snapshot); the reference factors
-    Q in complex64, this build in FP64, so the noise is held to 2e-4."""
+    Q in complex64, the HIP path in FP64, so the noise is held to 2e-4."""
     from fbme_amd import scenes
     from torch_sbl import sbl_2d
     print("golden", name, flush=True)
