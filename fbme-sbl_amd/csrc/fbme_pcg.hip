@@ -1183,6 +1183,86 @@ k_cg_em(float* __restrict__ gamma, const float2* __restrict__ mean, const float*
   if (threadIdx.x == 0) part[(long long)b * gridDim.x + blockIdx.x] = dof;
 }
 
+// ------------------------------------------------- pattern-coupled prior
+// Fixed coupling beta (solver.py:1100-1150, 1489-1590, 2396-2720): the
+// precision of cell i is eta_i = alpha_i + beta sum_{j in N(i)} alpha_j over
+// the four axial neighbours (circular: wrapped; else zero outside), added in
+// the order (i-1, j), (i+1, j), (i, j-1), (i, j+1).
+__device__ __forceinline__ float nsum4(const float* __restrict__ f, int n1, int n2, int i, int j, int circular) {
+  float acc = 0.f;
+  const int di[4] = {-1, 1, 0, 0}, dj[4] = {0, 0, -1, 1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    int a = i + di[q], c = j + dj[q];
+    if (circular) {
+      a = (a + n1) % n1;
+      c = (c + n2) % n2;
+    } else if (a < 0 || a >= n1 || c < 0 || c >= n2) {
+      continue;
+    }
+    acc += f[(long long)a * n2 + c];
+  }
+  return acc;
+}
+
+// pattern = (1 / gamma) / (1 + beta degree) (solver.py:2674-2687)
+__global__ void k_spatial_init(const float* __restrict__ gamma, int n1, int n2, float beta, int circular,
+                               float* __restrict__ pattern) {
+  const int b = blockIdx.y;
+  const long long n = (long long)n1 * n2;
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int r = (int)(i / n2), c = (int)(i - (long long)r * n2);
+  int deg = 0;
+  if (circular) {
+    deg = 4;
+  } else {
+    deg = (r > 0) + (r < n1 - 1) + (c > 0) + (c < n2 - 1);
+  }
+  pattern[b * n + i] = (1.0f / gamma[b * n + i]) / (1.0f + beta * (float)deg);
+}
+
+// gamma = max(1 / clamp(pattern + beta N(pattern)), floor) (solver.py:2662-2672, 1590-1610)
+__global__ void k_spatial_gamma(const float* __restrict__ pattern, int n1, int n2, float beta, int circular,
+                                float max_prec, const float* __restrict__ floor_, float* __restrict__ gamma) {
+  const int b = blockIdx.y;
+  const long long n = (long long)n1 * n2;
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int r = (int)(i / n2), c = (int)(i - (long long)r * n2);
+  const float* pb = pattern + b * n;
+  const float eta = pb[i] + beta * nsum4(pb, n1, n2, r, c, circular);
+  const float prec = fminf(fmaxf(eta, 1.0f / max_prec), max_prec);
+  gamma[b * n + i] = fmaxf(1.0f / prec, floor_[b]);
+}
+
+// second = |mu|^2 + v
+__global__ void k_spatial_second(const float2* __restrict__ mean, const float* __restrict__ var, long long n,
+                                 float* __restrict__ second) {
+  const int b = blockIdx.y;
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const float2 mu = mean[b * n + i];
+  second[b * n + i] = mu.x * mu.x + mu.y * mu.y + var[b * n + i];
+}
+
+// pattern <- keep pattern + (1 - keep) clamp((1 + a) / (second + beta N(second) + b))
+// (_pattern_precision_update, solver.py:2690-2720, damped at 1575-1580)
+__global__ void k_spatial_pattern(const float* __restrict__ second, int n1, int n2, float beta, int circular,
+                                  float hyper_shape, float hyper_rate, float max_prec, float keep,
+                                  float* __restrict__ pattern) {
+  const int b = blockIdx.y;
+  const long long n = (long long)n1 * n2;
+  const long long i = (long long)blockIdx.x * FBME_BLOCK + threadIdx.x;
+  if (i >= n) return;
+  const int r = (int)(i / n2), c = (int)(i - (long long)r * n2);
+  const float* sb = second + b * n;
+  const float coupled = sb[i] + beta * nsum4(sb, n1, n2, r, c, circular);
+  float base = (1.0f + hyper_shape) / (coupled + hyper_rate);
+  base = fminf(fmaxf(base, 1.0f / max_prec), max_prec);
+  pattern[b * n + i] = keep * pattern[b * n + i] + (1.0f - keep) * base;
+}
+
 // ------------------------------------------------------------- joint sparsity
 // Snapshots b = 0..B-1 share one variance per cell (M-SBL).  gamma is kept
 // as B identical rows so the per-snapshot kernels read it unchanged.
@@ -1982,6 +2062,36 @@ int fbme_pcg_check(void* stream, int lanes, const double* rr, const double* init
                    int* improved, float tol, int fixed) {
   hipLaunchKernelGGL(k_pcg_check, dim3(nblk(lanes, 64)), dim3(64), 0, (hipStream_t)stream, rr, init, active,
                      best_rel, improved, tol, fixed, lanes);
+  return (int)hipGetLastError();
+}
+
+int fbme_spatial_init(void* stream, int batch, int n1, int n2, float coupling, int circular, float max_prec,
+                      const float* floor_, float* gamma, float* pattern) {
+  if (batch < 1 || n1 < 1 || n2 < 1 || !(coupling >= 0.f) || !floor_ || !gamma || !pattern) return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)n1 * n2;
+  const dim3 g((unsigned)nblk(n), batch);
+  hipLaunchKernelGGL(k_spatial_init, g, dim3(FBME_BLOCK), 0, st, (const float*)gamma, n1, n2, coupling, circular,
+                     pattern);
+  hipLaunchKernelGGL(k_spatial_gamma, g, dim3(FBME_BLOCK), 0, st, (const float*)pattern, n1, n2, coupling, circular,
+                     max_prec, floor_, gamma);
+  return (int)hipGetLastError();
+}
+
+int fbme_spatial_update(void* stream, int batch, int n1, int n2, float coupling, int circular, float hyper_shape,
+                        float hyper_rate, float max_prec, float keep, const void* mean, const float* var,
+                        const float* floor_, float* second, float* pattern, float* gamma) {
+  if (batch < 1 || n1 < 1 || n2 < 1 || !(coupling >= 0.f) || !mean || !var || !floor_ || !second || !pattern ||
+      !gamma)
+    return FBME_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const long long n = (long long)n1 * n2;
+  const dim3 g((unsigned)nblk(n), batch);
+  hipLaunchKernelGGL(k_spatial_second, g, dim3(FBME_BLOCK), 0, st, (const float2*)mean, var, n, second);
+  hipLaunchKernelGGL(k_spatial_pattern, g, dim3(FBME_BLOCK), 0, st, (const float*)second, n1, n2, coupling, circular,
+                     hyper_shape, hyper_rate, max_prec, keep, pattern);
+  hipLaunchKernelGGL(k_spatial_gamma, g, dim3(FBME_BLOCK), 0, st, (const float*)pattern, n1, n2, coupling, circular,
+                     max_prec, floor_, gamma);
   return (int)hipGetLastError();
 }
 

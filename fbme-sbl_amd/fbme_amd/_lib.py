@@ -323,6 +323,10 @@ _SIGNATURES = {
     "fbme_fixed_noise": (c_int, [c_void_p, c_int, c_float, c_void_p, c_void_p, c_int]),
     "fbme_hutchinson_used": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "fbme_spatial_init": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_float, c_void_p, c_void_p,
+                                  c_void_p]),
+    "fbme_spatial_update": (c_int, [c_void_p, c_int, c_int, c_int, c_float, c_int, c_float, c_float, c_float,
+                                    c_float, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     # Nystrom preconditioner and the stored-column apply
     "fbme_stored_apply": (c_int, [c_void_p, c_int, c_int, c_longlong, c_void_p, c_void_p, c_void_p, c_void_p,
                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

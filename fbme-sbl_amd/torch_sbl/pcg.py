@@ -41,7 +41,9 @@ Differences from the reference, all documented:
   * adaptive probe counts keep every drawn probe lane and give the unused
     ones a zero right-hand side (PCG leaves them inactive; the passes still
     carry them);
-  * spatial coupling raises NotImplementedError.
+  * a fixed spatial coupling runs (pattern-coupled update, the active set
+    off as in the reference); a learned coupling, or one with joint
+    sparsity, raises NotImplementedError.
 
 Joint sparsity (solver.py:1090-1099, 1157-1159, 1653-1666, 1769-1774,
 3039-3153) keeps one variance per cell for all snapshots (B identical rows)
@@ -166,8 +168,8 @@ def _check_supported(cfg: SBLConfig) -> None:
         bad.append(f"cg_preconditioner={cfg.cg_preconditioner!r}")
     if cfg.cg_preconditioner in ("low_rank", "nystrom") and cfg.cg_preconditioner_rank > RANK:
         bad.append("cg_preconditioner_rank > 32")
-    if cfg.spatial_coupling > 0.0 or cfg.learn_spatial_coupling:
-        bad.append("spatial coupling")
+    if cfg.learn_spatial_coupling:
+        bad.append("learn_spatial_coupling")
     if cfg.active_set and cfg.active_set_max_size > 256:
         bad.append("active_set_max_size > 256")
     if bad:
@@ -300,6 +302,12 @@ class _Solver:
         self.shared = self.joint and bool(cfg.joint_noise_variance)
         if self.joint and self.b < 2:
             raise ValueError("joint_sparsity=True requires at least two leading batch items.")
+        # pattern-coupled prior with a fixed coupling (solver.py:1100-1150,
+        # 1489-1590): the EM update acts on a base precision field whose
+        # neighbours load each cell's precision
+        self.spatial = cfg.spatial_coupling > 0.0
+        if self.spatial and self.joint:
+            raise NotImplementedError("spatial coupling with joint sparsity is not implemented on MI355X")
         # mode="dense" with joint sparsity: the shared covariance factored per
         # snapshot is the reference's one factorisation with many right-hand
         # sides (_shared_mmv_posterior, solver.py:3039-3080)
@@ -443,6 +451,10 @@ class _Solver:
             self._gdiag_buffers()
         if self.precond == "nystrom":
             self._nystrom_buffers()
+        self.pattern = self.second = None
+        if self.spatial:
+            self.pattern = torch.empty((b, n), dtype=f32, device=dev)
+            self.second = torch.empty((b, n), dtype=f32, device=dev)
         self.circ_eig = self.circ_work = None
         if self.precond == "circulant":
             self.circ_eig = torch.empty((ns, m), dtype=f32, device=dev)
@@ -752,6 +764,11 @@ class _Solver:
                                                _ptr(self.power), _ptr(self.floor),
                                                self._blend(), 1 if self.shared else 0,
                                                _ptr(self.gamma), _ptr(self.noise)), "joint_init")
+        if self.spatial:
+            self._chk(self.lib.fbme_spatial_init(self.stream, self.b, *self.grid, float(cfg.spatial_coupling),
+                                                 1 if cfg.spatial_boundary == "circular" else 0,
+                                                 float(cfg.max_precision), _ptr(self.floor), _ptr(self.gamma),
+                                                 _ptr(self.pattern)), "spatial_init")
         if cfg.fixed_noise_variance is not None:
             # solver.py:1151-1156: the fixed value, unclamped
             self._chk(self.lib.fbme_fixed_noise(self.stream, self.b, float(cfg.fixed_noise_variance), None,
@@ -1027,6 +1044,15 @@ class _Solver:
                 _ptr(self.dd), _ptr(self.rel_sum), _ptr(self.floor), float(cfg.hyper_shape), float(cfg.hyper_rate),
                 float(cfg.max_precision), float(keep), 1 if update else 0, _ptr(self.grid_part), _ptr(self.dof),
                 _ptr(self.dof_f)), "cg_em_joint")
+        elif self.spatial:
+            # relevance and dof as usual, then the pattern-coupled update
+            self._cg_em(keep, False)
+            if update:
+                self._chk(self.lib.fbme_spatial_update(
+                    self.stream, self.b, *self.grid, float(cfg.spatial_coupling),
+                    1 if cfg.spatial_boundary == "circular" else 0, float(cfg.hyper_shape), float(cfg.hyper_rate),
+                    float(cfg.max_precision), float(keep), _ptr(self.mean), _ptr(self.var), _ptr(self.floor),
+                    _ptr(self.second), _ptr(self.pattern), _ptr(self.gamma)), "spatial_update")
         else:
             self._cg_em(keep, update)
         if not update:
@@ -1158,7 +1184,8 @@ def solve(s: "_Solver", batch_shape=(), warm_start=None, stall_check=None, captu
     change = float("inf")
     it = 0
     for it in range(1, cfg.max_iter + 1):
-        if (cfg.active_set and not dense and s.b == 1 and s.support is None and it >= cfg.active_set_start
+        if (cfg.active_set and not dense and not s.spatial and s.b == 1 and s.support is None
+                and it >= cfg.active_set_start
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             s.try_active_set()
         tol = max(_schedule(it, cfg), tol_floor)

@@ -496,6 +496,21 @@ int fbme_hutchinson(void* stream, int batch, int n, int probes, const void* lift
 int fbme_hutchinson_used(void* stream, int batch, int n, int probes, int used, const void* lifted,
                          const void* lprobes, const float* gamma, void* mean, float* var, float* dd, double* part,
                          double* rel_sum);
+/* Pattern-coupled prior with a fixed coupling (SBLConfig.spatial_coupling,
+ * solver.py:1100-1150, 1489-1590, 2396-2720) on [batch, N1 x N2] fields:
+ * precision eta = pattern + coupling * (sum of the four axial neighbours of
+ * pattern; circular != 0 wraps, else zero outside), gamma = max(1 /
+ * clamp(eta, 1 / max_prec, max_prec), floor[b]).  fbme_spatial_init forms
+ * pattern = (1 / gamma) / (1 + coupling * degree) from the initial gamma,
+ * then gamma.  fbme_spatial_update replaces the MacKay update: second =
+ * |mean|^2 + var (scratch [batch, N]), pattern <- keep pattern + (1 - keep)
+ * clamp((1 + hyper_shape) / (second + coupling N(second) + hyper_rate)),
+ * then gamma. */
+int fbme_spatial_init(void* stream, int batch, int n1, int n2, float coupling, int circular, float max_prec,
+                      const float* floor_, float* gamma, float* pattern);
+int fbme_spatial_update(void* stream, int batch, int n1, int n2, float coupling, int circular, float hyper_shape,
+                        float hyper_rate, float max_prec, float keep, const void* mean, const float* var,
+                        const float* floor_, float* second, float* pattern, float* gamma);
 /* Relevance projection (_project_relevance, solver.py:3156-3181), MacKay
  * update (1669-1677) and damping with floor (1776-1798); mask (may be NULL)
  * keeps gamma outside an active set.  dof[b] = projected relevance sum. */

@@ -57,6 +57,8 @@ class CGConfig:
     probe_type: str = "rademacher"
     adaptive_probes: bool = False
     initial_hutchinson_probes: int = 16
+    spatial_coupling: float = 0.0
+    spatial_boundary: str = "zero"
     initialization_blend: float = 0.1
     noise_update: str = "em"
     damping: float = 0.15
@@ -166,6 +168,31 @@ def probe_schedule(it: int, cfg: CGConfig, maximum: int) -> int:
     if it <= 15:
         return min(maximum, max(initial, min(24, maximum)))
     return maximum
+
+
+def neighbor_sum(values, grid, boundary):
+    """Sum of the four axial neighbours on the grid (_neighbor_sum_2d,
+    solver.py:2396-2425), by explicit loops: 'circular' wraps, 'zero' pads."""
+    n1, n2 = grid
+    f = values.reshape(values.shape[0], n1, n2)
+    out = np.zeros_like(f)
+    for i in range(n1):
+        for j in range(n2):
+            acc = np.zeros(f.shape[0])
+            for di, dj in ((-1, 0), (1, 0), (0, -1), (0, 1)):
+                a, c = i + di, j + dj
+                if boundary == "circular":
+                    acc = acc + f[:, a % n1, c % n2]
+                elif 0 <= a < n1 and 0 <= c < n2:
+                    acc = acc + f[:, a, c]
+            out[:, i, j] = acc
+    return out.reshape(values.shape)
+
+
+def spatial_variance(pattern, grid, cfg):
+    """gamma = 1 / clamp(pattern + beta N(pattern)) (solver.py:2662-2672)."""
+    eta = pattern + cfg.spatial_coupling * neighbor_sum(pattern, grid, cfg.spatial_boundary)
+    return 1.0 / np.clip(eta, 1.0 / cfg.max_precision, cfg.max_precision)
 
 
 def resize_warm_start(sol, lanes):
@@ -588,6 +615,12 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     else:
         prior_floor = floor
     gamma = np.maximum(gamma, prior_floor[:, None])
+    pattern = None
+    if cfg.spatial_coupling > 0.0:
+        # scale-matched base precision of the coupled prior (solver.py:1121-1150, 2674-2687)
+        degree = neighbor_sum(np.ones((1, n)), grid, cfg.spatial_boundary)
+        pattern = (1.0 / gamma) / (1.0 + cfg.spatial_coupling * degree)
+        gamma = np.maximum(spatial_variance(pattern, grid, cfg), prior_floor[:, None])
     noise = np.maximum(cfg.initial_noise_fraction * power, floor)
     if cfg.fixed_noise_variance is not None:
         noise = np.full(b, cfg.fixed_noise_variance)      # solver.py:1151-1156
@@ -613,7 +646,7 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
     it = 0
     cg_iters = []
     for it in range(1, cfg.max_iter + 1):
-        if (cfg.active_set and b == 1 and not joint and sup is None
+        if (cfg.active_set and b == 1 and not joint and sup is None and pattern is None
                 and it >= cfg.active_set_start
                 and (it - cfg.active_set_start) % cfg.active_set_check_interval == 0):
             sup = select_active(gamma, cfg)
@@ -644,7 +677,15 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
             var = np.maximum(gamma - gamma ** 2 * dd, 0.0)
         rel = project_relevance(gamma, dd, m, eps)
         dof = rel.sum(1)
-        if joint:
+        if pattern is not None:
+            # pattern-coupled update (solver.py:1489-1590, 2690-2720)
+            second = np.abs(mean) ** 2 + var
+            coupled = second + cfg.spatial_coupling * neighbor_sum(second, grid, cfg.spatial_boundary)
+            base = np.clip((1.0 + cfg.hyper_shape) / (coupled + cfg.hyper_rate),
+                           1.0 / cfg.max_precision, cfg.max_precision)
+            pattern = keep * pattern + (1 - keep) * base
+            new_gamma = spatial_variance(pattern, grid, cfg)
+        elif joint:
             second = np.mean(np.abs(mean) ** 2 + var, 0, keepdims=True)
             new_gamma = np.repeat(np.clip((second + cfg.hyper_rate) / (1.0 + cfg.hyper_shape),
                                           prior_floor.max(), cfg.max_precision), b, 0)
@@ -667,7 +708,10 @@ def sbl_cg(data, grid, cfg: CGConfig | None = None, *, eps=F32_EPS,
         new_noise = np.maximum(new_noise, floor)
         if shared:
             new_noise = np.full(b, max(new_noise.mean(), floor.max()))
-        gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, prior_floor[:, None])
+        if pattern is not None:
+            gamma = np.maximum(new_gamma, prior_floor[:, None])   # damped in the pattern
+        else:
+            gamma = np.maximum(keep * gamma + (1 - keep) * new_gamma, prior_floor[:, None])
         if cfg.fixed_noise_variance is not None:
             noise = new_noise                                  # undamped (solver.py:1788-1798)
         else:

@@ -419,6 +419,11 @@ OPTION_CASES = [
           return_history=True)),
     ("options_cg_joint_adaptive_probes_32x32_24x24_b2", "uniform", (1119, 1120), (32, 32), (24, 24), 8, 20.0,
      "cg", dict(adaptive_probes=True, joint_sparsity=True, max_iter=20, return_history=True)),
+    # pattern-coupled prior with a fixed coupling, zero and circular boundaries
+    ("options_cg_spatial_zero_32x32_24x24", "ellipse", (1121,), (32, 32), (24, 24), 30, 15.0, "cg",
+     dict(spatial_coupling=0.3, max_iter=25, return_history=True)),
+    ("options_cg_spatial_circular_48x40_40x30", "ellipse", (1122,), (48, 40), (40, 30), 40, 15.0, "cg",
+     dict(spatial_coupling=0.2, spatial_boundary="circular", max_iter=20, return_history=True)),
 ]
 
 

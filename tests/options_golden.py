@@ -8,8 +8,9 @@ a fixed noise variance (solver.py:1151-1156, 1747-1750, 1788-1798,
 operators.py:299-371: the same operator as the BTTB products the solver
 runs), low-rank preconditioners of rank 8 and 0 (solver.py:5786-5848) and
 the randomised Nystrom preconditioner (solver.py:5851-5976) at ranks 32 and
-12, also with joint sparsity and a shared noise, and adaptive probe counts
-(16, 24, then all; solver.py:6086-6138)."""
+12, also with joint sparsity and a shared noise, adaptive probe counts
+(16, 24, then all; solver.py:6086-6138) and the pattern-coupled prior with
+a fixed coupling (solver.py:1100-1150, 1489-1590, 2396-2720)."""
 
 import numpy as np
 
@@ -22,7 +23,8 @@ PLAN = ["options_stable_fixed_noise_64x64_40x40", "options_fast_fixed_noise_96x8
 CG = ["options_cg_fixed_noise_32x32_24x24", "options_cg_hadamard_32x32_24x24", "options_cg_hadamard_48x40_40x30",
       "options_cg_fourier_32x32_24x24", "options_cg_rank8_48x40_40x30", "options_cg_rank0_32x32_24x24",
       "options_cg_nystrom_32x32_24x24", "options_cg_nystrom_rank12_64x64_40x40",
-      "options_cg_adaptive_probes_64x64_40x40"]
+      "options_cg_adaptive_probes_64x64_40x40", "options_cg_spatial_zero_32x32_24x24",
+      "options_cg_spatial_circular_48x40_40x30"]
 CG_JOINT = ["options_cg_joint_fixed_noise_32x32_24x24_b3", "options_cg_joint_nystrom_32x32_24x24_b3",
             "options_cg_joint_adaptive_probes_32x32_24x24_b2"]
 
@@ -47,7 +49,7 @@ def cg_config(over):
     kw = {k: v for k, v in over.items() if k in ("fixed_noise_variance", "max_iter", "probe_type",
                                                   "hutchinson_probes", "auto_probe_scaling",
                                                   "cg_preconditioner_rank", "cg_preconditioner",
-                                                  "adaptive_probes")}
+                                                  "adaptive_probes", "spatial_coupling", "spatial_boundary")}
     return co.CGConfig(**kw)
 
 

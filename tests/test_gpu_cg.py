@@ -197,7 +197,7 @@ def test_sbl_cg_is_deterministic(device):
 def test_sbl_cg_unsupported_options_raise(device):
     from torch_sbl import SBLConfig, sbl_2d
     y = torch.zeros((24, 24), dtype=torch.complex64, device=device)
-    for over in (dict(spatial_coupling=0.1), dict(cg_preconditioner_rank=64),
+    for over in (dict(learn_spatial_coupling=True), dict(cg_preconditioner_rank=64),
                  dict(cg_preconditioner="nystrom", cg_preconditioner_rank=48)):
         with pytest.raises(NotImplementedError):
             sbl_2d(y, (32, 32), config=SBLConfig(mode="cg", device=str(device), **over))

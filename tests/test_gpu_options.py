@@ -15,6 +15,8 @@ the reference's runs (tests/options_golden.py) and the float64 oracles:
   below);
 * adaptive probe counts (16, 24, then all; the unused probe lanes carry a
   zero right-hand side), the per-iteration counts as the reference's;
+* the pattern-coupled prior with a fixed coupling (zero and circular
+  boundaries);
 * the randomised Nystrom preconditioner at ranks 32 and 12 (also with joint
   sparsity) against the reference's runs (x 4e-7 to 6e-6), and its apply
   against the float64 oracle at fixed variances on the BTTB and grid
